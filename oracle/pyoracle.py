@@ -1,0 +1,226 @@
+"""ctypes wrapper of oracle/liboracle.so (TEST INFRASTRUCTURE ONLY).
+
+The oracle is a CPU restatement of IMSRG++'s scalar IMSRG(2) commutator,
+BCH and Magnus path (see oracle/oracle.c for the file:line citations).  Only
+tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use it, as
+the checker; the product (imsrg_amd/) never imports it.
+
+Operators are plain dicts {"zero": float, "one": ndarray(norb, norb) in
+Fortran order, "two": ndarray(tbsize), "herm": +1/-1/0} in the packed layout
+described in oracle/oracle.h.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+TERMS = {
+    "comm110ss": 1 << 0,
+    "comm220ss": 1 << 1,
+    "comm111ss": 1 << 2,
+    "comm121ss": 1 << 3,
+    "comm221ss": 1 << 4,
+    "comm122ss": 1 << 5,
+    "comm222_pp_hhss": 1 << 6,
+    "comm222_phss": 1 << 7,
+}
+ALL_TERMS = 0xFF
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class _Op(C.Structure):
+    _fields_ = [("zero", _dp), ("one", _dp), ("two", _dp), ("herm", C.c_int)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
+        L = C.CDLL(path)
+        L.orc_ms_new_ho.restype = C.c_void_p
+        L.orc_ms_new_ho.argtypes = [C.c_int, C.c_double, C.c_double]
+        L.orc_ms_new.restype = C.c_void_p
+        L.orc_ms_new.argtypes = [C.c_int, _ip, _ip, _ip, _ip, _dp, _ip, C.c_int]
+        L.orc_ms_free.argtypes = [C.c_void_p]
+        for f in ("orc_ms_norb", "orc_ms_nkets", "orc_ms_nch"):
+            getattr(L, f).restype = C.c_int
+            getattr(L, f).argtypes = [C.c_void_p]
+        L.orc_ms_tbsize.restype = C.c_long
+        L.orc_ms_tbsize.argtypes = [C.c_void_p]
+        L.orc_ms_orbit.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _ip, _ip, _dp, _ip]
+        L.orc_ms_channel.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _ip, _ip, C.POINTER(C.c_long)]
+        L.orc_ms_channel_ket.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, _ip]
+        L.orc_ms_cc_channel.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _ip, _ip, _ip]
+        L.orc_ms_cc_channel_ket.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, _ip]
+        L.orc_sixj_exact.restype = C.c_double
+        L.orc_sixj_exact.argtypes = [C.c_int] * 6
+        L.orc_sixj_cached.restype = C.c_double
+        L.orc_sixj_cached.argtypes = [C.c_void_p] + [C.c_int] * 6
+        L.orc_tbme_J.restype = C.c_double
+        L.orc_tbme_J.argtypes = [C.c_void_p, _dp] + [C.c_int] * 5
+        L.orc_tbme_monopole.restype = C.c_double
+        L.orc_tbme_monopole.argtypes = [C.c_void_p, _dp] + [C.c_int] * 4
+        L.orc_commutator.restype = C.c_int
+        L.orc_commutator.argtypes = [C.c_void_p, C.POINTER(_Op), C.POINTER(_Op), C.POINTER(_Op), C.c_uint]
+        for f in ("orc_norm", "orc_onebody_norm", "orc_twobody_norm"):
+            getattr(L, f).restype = C.c_double
+            getattr(L, f).argtypes = [C.c_void_p, C.POINTER(_Op)]
+        L.orc_bch_transform.restype = C.c_int
+        L.orc_bch_transform.argtypes = [C.c_void_p, C.POINTER(_Op), C.POINTER(_Op), C.POINTER(_Op), C.c_int,
+                                        C.c_double]
+        L.orc_bch_product.restype = C.c_int
+        L.orc_bch_product.argtypes = [C.c_void_p, C.POINTER(_Op), C.POINTER(_Op), C.POINTER(_Op), C.c_double]
+        L.orc_white_generator.argtypes = [C.c_void_p, C.POINTER(_Op), C.POINTER(_Op)]
+        L.orc_magnus_flow.restype = C.c_int
+        L.orc_magnus_flow.argtypes = [C.c_void_p, C.POINTER(_Op), C.POINTER(_Op), C.c_double, C.c_double,
+                                      C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, _dp, _dp, _ip]
+        L.orc_set_fault.argtypes = [C.c_int]
+        L.orc_num_threads.restype = C.c_int
+        L.orc_set_num_threads.argtypes = [C.c_int]
+        _LIB = L
+    return _LIB
+
+
+def set_fault(f):
+    """Plant a known mistake in the oracle (checker tests only); 0 turns it off."""
+    lib().orc_set_fault(int(f))
+
+
+def sixj_exact(ja2, jb2, jc2, jd2, je2, jf2):
+    """Exact (double-double Racah) 6j of doubled arguments."""
+    return lib().orc_sixj_exact(ja2, jb2, jc2, jd2, je2, jf2)
+
+
+class _Holder:
+    """Keeps the numpy buffers and the ctypes struct of one operator alive."""
+
+    def __init__(self, op):
+        self.zero = np.array([op["zero"]], dtype=np.float64)
+        self.one = np.asfortranarray(op["one"], dtype=np.float64)
+        self.two = np.ascontiguousarray(op["two"], dtype=np.float64)
+        self.s = _Op(self.zero.ctypes.data_as(_dp), self.one.ctypes.data_as(_dp), self.two.ctypes.data_as(_dp),
+                     int(op["herm"]))
+
+    def op(self):
+        return {"zero": float(self.zero[0]), "one": self.one, "two": self.two, "herm": int(self.s.herm)}
+
+
+class OracleModelSpace:
+    """Basis + channel tables of the oracle (restates src/ModelSpace.cc)."""
+
+    def __init__(self, emax, A=16, Z=8, orbits=None):
+        L = lib()
+        if orbits is None:
+            self.h = L.orc_ms_new_ho(emax, float(A), float(Z))
+        else:
+            n, l, j2, tz2, occ, cvq = (np.ascontiguousarray(orbits[k], dtype=(np.float64 if k == "occ" else np.int32))
+                                       for k in ("n", "l", "j2", "tz2", "occ", "cvq"))
+            self._keep = (n, l, j2, tz2, occ, cvq)
+            self.h = L.orc_ms_new(len(n), n.ctypes.data_as(_ip), l.ctypes.data_as(_ip), j2.ctypes.data_as(_ip),
+                                  tz2.ctypes.data_as(_ip), occ.ctypes.data_as(_dp), cvq.ctypes.data_as(_ip), emax)
+        self.emax = emax
+        self.norb = L.orc_ms_norb(self.h)
+        self.nkets = L.orc_ms_nkets(self.h)
+        self.nch = L.orc_ms_nch(self.h)
+        self.tbsize = L.orc_ms_tbsize(self.h)
+        self.orbits = []
+        for i in range(self.norb):
+            v = [C.c_int() for _ in range(4)]
+            o = C.c_double()
+            cq = C.c_int()
+            L.orc_ms_orbit(self.h, i, *[C.byref(x) for x in v], C.byref(o), C.byref(cq))
+            self.orbits.append(dict(n=v[0].value, l=v[1].value, j2=v[2].value, tz2=v[3].value, occ=o.value,
+                                    cvq=cq.value))
+        self.channels = []
+        for ch in range(self.nch):
+            J, p, t, nk = (C.c_int() for _ in range(4))
+            off = C.c_long()
+            L.orc_ms_channel(self.h, ch, C.byref(J), C.byref(p), C.byref(t), C.byref(nk), C.byref(off))
+            kets = []
+            for i in range(nk.value):
+                a, b = C.c_int(), C.c_int()
+                L.orc_ms_channel_ket(self.h, ch, i, C.byref(a), C.byref(b))
+                kets.append((a.value, b.value))
+            self.channels.append(dict(J=J.value, par=p.value, tz=t.value, n=nk.value, off=off.value, kets=kets))
+        self.cc_channels = []
+        for ch in range(self.nch):
+            J, p, t, nk, nph = (C.c_int() for _ in range(5))
+            L.orc_ms_cc_channel(self.h, ch, C.byref(J), C.byref(p), C.byref(t), C.byref(nk), C.byref(nph))
+            self.cc_channels.append(dict(J=J.value, par=p.value, tz=t.value, n=nk.value, nph=nph.value))
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().orc_ms_free(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    # ---- operators -------------------------------------------------
+    def zero_op(self, herm=1):
+        return {"zero": 0.0, "one": np.zeros((self.norb, self.norb), order="F"), "two": np.zeros(self.tbsize),
+                "herm": herm}
+
+    def block(self, op, ch):
+        c = self.channels[ch]
+        n = c["n"]
+        return op["two"][c["off"]:c["off"] + n * n].reshape((n, n), order="F")
+
+    def sixj(self, jj1, jj2, J3, jj4, jj5, J6):
+        return lib().orc_sixj_cached(self.h, jj1, jj2, J3, jj4, jj5, J6)
+
+    def tbme_J(self, op, J, a, b, c, d):
+        two = np.ascontiguousarray(op["two"])
+        return lib().orc_tbme_J(self.h, two.ctypes.data_as(_dp), J, a, b, c, d)
+
+    def monopole(self, op, a, b, c, d):
+        two = np.ascontiguousarray(op["two"])
+        return lib().orc_tbme_monopole(self.h, two.ctypes.data_as(_dp), a, b, c, d)
+
+    def commutator(self, X, Y, mask=ALL_TERMS):
+        hx, hy = _Holder(X), _Holder(Y)
+        hz = _Holder(self.zero_op(0))
+        lib().orc_commutator(self.h, C.byref(hx.s), C.byref(hy.s), C.byref(hz.s), mask)
+        return hz.op()
+
+    def norm(self, op):
+        return lib().orc_norm(self.h, C.byref(_Holder(op).s))
+
+    def onebody_norm(self, op):
+        return lib().orc_onebody_norm(self.h, C.byref(_Holder(op).s))
+
+    def twobody_norm(self, op):
+        return lib().orc_twobody_norm(self.h, C.byref(_Holder(op).s))
+
+    def bch_transform(self, Op, Omega, mode=0, thr=1e-9):
+        a, b, o = _Holder(Op), _Holder(Omega), _Holder(self.zero_op(Op["herm"]))
+        n = lib().orc_bch_transform(self.h, C.byref(a.s), C.byref(b.s), C.byref(o.s), mode, thr)
+        return o.op(), n
+
+    def bch_product(self, X, Y, thr=1e-4):
+        a, b, o = _Holder(X), _Holder(Y), _Holder(self.zero_op(X["herm"]))
+        n = lib().orc_bch_product(self.h, C.byref(a.s), C.byref(b.s), C.byref(o.s), thr)
+        return o.op(), n
+
+    def white_generator(self, H):
+        a, o = _Holder(H), _Holder(self.zero_op(-1))
+        lib().orc_white_generator(self.h, C.byref(a.s), C.byref(o.s))
+        return o.op()
+
+    def magnus_flow(self, H0, ds=0.5, smax=50.0, eta_crit=1e-6, omega_norm_max=2.0, bch_mode=0, bch_thr=1e-9,
+                    max_steps=200):
+        a, o = _Holder(H0), _Holder(self.zero_op(1))
+        e0 = np.zeros(max_steps + 1)
+        en = np.zeros(max_steps + 1)
+        nc = np.zeros(max_steps + 1, dtype=np.int32)
+        n = lib().orc_magnus_flow(self.h, C.byref(a.s), C.byref(o.s), ds, smax, eta_crit, omega_norm_max,
+                                  bch_mode, bch_thr, max_steps, e0.ctypes.data_as(_dp), en.ctypes.data_as(_dp),
+                                  nc.ctypes.data_as(_ip))
+        return o.op(), e0[:n].copy(), en[:n + 1].copy(), nc[:n].copy()
