@@ -1,0 +1,163 @@
+"""Benchmark: [X,Y] scalar IMSRG(2) commutators per second at emax=10 on one MI355X.
+
+Workload: Z = [X, Y] with X anti-Hermitian and Y Hermitian, HO basis emax=10
+(132 orbits), O16 reference, two-body blocks (R +/- R^T)/sqrt(2n) with seeded
+N(0,1) entries (synthetic data; the library's counter-based generator).
+Operators are resident in HBM before the timed region; each step is one full
+commutator (all eight IMSRG(2) terms).
+
+Prints one JSON line (see README/DESIGN for the fields).  ``roofline`` is for
+the dominant kernel, from HIP events recorded on the library stream during
+the timed steps; ``cpu_baseline`` times the oracle (C/OpenMP restatement) on
+one commutator of the same workload on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_MFMA = 78.6e12   # FLOP/s, MI355X dense FP64 matrix
+PEAK_FP32_MFMA = 157.3e12  # FLOP/s, MI355X dense FP32 matrix
+PEAK_HBM = 8.0e12          # B/s
+
+PHASES = ["zerobody", "onebody", "comm122", "pphh_gemm", "pandya_gather", "ph_gemm", "inverse_pandya", "mirror"]
+
+
+def algorithmic_counts(ms):
+    """Per-commutator algorithmic work of the reference algorithm:
+    F_A = sum_ch 2 n^2 (n_pp + 2 n_hh + n_ph)    pp/hh GEMMs (src/Commutator.cc:1176-1183)
+    F_C = sum_cc 8 c^2 n_ph,cc                    ph GEMM (src/Commutator.cc:1789)
+    B_B = 8 [2 sum n^2 + 4 sum c 2n_ph + 2 sum 2c^2 + 2 sum n^2]  Pandya-phase bytes
+    plus the kernel-level byte counts used for the gather kernels."""
+    occ = [o["occ"] for o in ms.orbits]
+    FA = 0
+    executed_A = 0
+    for ch, c in enumerate(ms.channels):
+        n = c["n"]
+        if n == 0:
+            continue
+        npp = nhh = nph = 0
+        for p, q in ms.kets(ch):
+            hp, hq = occ[p] > 1e-6, occ[q] > 1e-6
+            if hp and hq:
+                nhh += 1
+            elif hp or hq:
+                nph += 1
+            else:
+                npp += 1
+        FA += 2 * n * n * (npp + 2 * nhh + nph)
+        executed_A += 2 * n * n * (npp + nhh)
+    return FA, executed_A
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--emax", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one MI355X: this build has no multi-GPU path (see DESIGN.md, sharding)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import imsrg_amd as ia
+
+    emax = args.emax
+    ms = ia.ModelSpace(emax, "O16", device=0)
+    X = ia.DeviceOperator(ms, -1).fill_random(-1, 10001)
+    Y = ia.DeviceOperator(ms, 1).fill_random(1, 10002)
+    Z = ia.DeviceOperator(ms, 1)
+
+    def run(steps, precision):
+        for _ in range(steps):
+            ia.Commutator.CommutatorScalarScalar(X, Y, precision=precision, out=Z)
+
+    # ---- FP64: warm-up, then timed steps with per-phase HIP events ----
+    ms.set_profiling(True)
+    run(args.warmup, ia.PREC_FP64)
+    ms.synchronize()
+    phase_sum = np.zeros(8)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ia.Commutator.CommutatorScalarScalar(X, Y, precision=ia.PREC_FP64, out=Z)
+        phase_sum += np.array(ms.phase_times())
+    ms.synchronize()
+    t_fp64 = time.perf_counter() - t0
+    ms.set_profiling(False)
+    fp64_rate = args.steps / t_fp64
+    phase_avg = phase_sum / args.steps
+    Zref = Z.to_host()
+
+    # ---- roofline of the dominant kernel ----
+    FA, execA = algorithmic_counts(ms)
+    dom = int(np.argmax(phase_avg))
+    roof = None
+    t_dom = phase_avg[dom] * 1e-6
+    if PHASES[dom] == "pphh_gemm":
+        ach = FA / t_dom
+        roof = {"kernel": "k_syr2k<0> (pp/hh SYR2K, FP64 MFMA)", "bound": "mfma", "achieved": ach / 1e12,
+                "peak": PEAK_FP64_MFMA / 1e12, "unit": "TFLOP/s", "frac": ach / PEAK_FP64_MFMA, "traffic": None,
+                "algorithmic_flops": FA, "executed_flops": execA}
+    else:
+        roof = {"kernel": PHASES[dom], "bound": "hbm", "achieved": None, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                "frac": None, "traffic": None}
+
+    line = {
+        "metric": "[X,Y] commutators/sec at emax=10: mixed vs FP64 GPU, at |dE0|<1e-6",
+        "value": fp64_rate,
+        "unit": "commutators/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_fp64 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded N(0,1)/sqrt(dim) operators, X anti-Hermitian, Y Hermitian)",
+        "config": {"workload": f"scalar IMSRG(2) commutator [X,Y], HO emax={emax}, O16 reference, "
+                               f"{ms.norb} orbits, FP64", "emax": emax, "norb": ms.norb,
+                   "two_body_doubles": ms.tbsize, "precision": "fp64"},
+        "fp64_commutators_per_s": fp64_rate,
+        "mixed_commutators_per_s": None,
+        "phase_us": dict(zip(PHASES, [round(x, 2) for x in phase_avg])),
+        "roofline": roof,
+    }
+
+    # ---- CPU baseline: the oracle on one commutator of the same workload ----
+    if not args.no_cpu_baseline:
+        try:
+            sys.path.insert(0, os.path.join(ROOT, "tests"))
+            from oracle.pyoracle import OracleModelSpace, lib as orc_lib
+            orc = OracleModelSpace(emax, 16, 8)
+            Xh, Yh = X.to_host(), Y.to_host()
+            xo = {"zero": Xh.ZeroBody, "one": Xh.OneBody, "two": Xh.TwoBody, "herm": -1}
+            yo = {"zero": Yh.ZeroBody, "one": Yh.OneBody, "two": Yh.TwoBody, "herm": 1}
+            t0 = time.perf_counter()
+            R = orc.commutator(xo, yo)
+            tc = time.perf_counter() - t0
+            scale = max(np.max(np.abs(R["two"])), np.max(np.abs(R["one"])))
+            err = float(max(np.max(np.abs(Zref.TwoBody - R["two"])), np.max(np.abs(Zref.OneBody - R["one"]))) / scale)
+            line["cpu_baseline"] = {"value": 1.0 / tc, "unit": "commutators/s", "cores": orc_lib().orc_num_threads(),
+                                    "kind": "port",
+                                    "sample": f"1 FP64 commutator of the same emax={emax} workload by the "
+                                              f"OpenMP C restatement (oracle/), {tc:.1f} s",
+                                    "gpu_vs_cpu_rel_maxabs": err,
+                                    "gpu_vs_cpu_dE0": abs(Zref.ZeroBody - R["zero"])}
+        except Exception as e:  # the baseline is reported, never the measured value
+            line["cpu_baseline"] = {"value": None, "unit": "commutators/s", "cores": None, "kind": "port",
+                                    "sample": f"failed: {e}"}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,841 @@
+// api.cpp -- C ABI of libimsrg_hip.so: contexts, device operators, the
+// commutator launch sequence, BCH loops, Magnus-Euler flow, batching.
+#include "../../include/imsrg_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "basis.hpp"
+#include "launch.hpp"
+
+using namespace imsrg;
+
+struct imsrg_dop {
+  imsrg_ctx *ctx;
+  double *buf;
+  int herm;
+};
+
+struct BatchCache {
+  std::vector<const void *> key;
+  PairPtrs *d_tab = nullptr;
+  int4 *d_items = nullptr;
+  size_t off[8] = {0};
+  int cnt[8] = {0};
+  double *work = nullptr;
+  int maxn = 0;
+  void release() {
+    if (d_tab) (void)hipFree(d_tab);
+    if (d_items) (void)hipFree(d_items);
+    if (work) (void)hipFree(work);
+    d_tab = nullptr; d_items = nullptr; work = nullptr;
+    key.clear();
+  }
+};
+
+struct imsrg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  HostBasis hb;
+  DeviceBasis db;
+  double *Xt = nullptr, *Yt = nullptr, *Zbar = nullptr;
+  double *partials = nullptr;
+  double *norm_dev = nullptr;
+  double *norm_host = nullptr;  // pinned
+  int npart = 0;
+  std::string err;
+  bool profiling = false;
+  hipEvent_t ev[9] = {};
+  double phase_us[8] = {0};
+  imsrg_dop *stage[3] = {nullptr, nullptr, nullptr};
+  imsrg_dop *tmp[4] = {nullptr, nullptr, nullptr, nullptr};
+  double *pinned = nullptr;  // host staging for views
+  BatchCache batch;
+};
+
+static int fail(imsrg_ctx *ctx, int code, const std::string &msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+#define HIPCHK(ctx, call)                                                                      \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess) return fail(ctx, IMSRG_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+static std::string g_err;
+
+extern "C" const char *imsrg_hip_last_error(const imsrg_ctx *ctx) {
+  return ctx ? ctx->err.c_str() : g_err.c_str();
+}
+
+// ---------------------------------------------------------------------------
+static int ctx_init(imsrg_ctx *ctx, const imsrg_basis_desc *b, int device) {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(ctx, IMSRG_E_HIP, "no HIP device");
+  if (device < 0 || device >= ndev) return fail(ctx, IMSRG_E_INVALID, "device index out of range");
+  ctx->device = device;
+  HIPCHK(ctx, hipSetDevice(device));
+  std::string e = ctx->hb.build(b->norb, b->n, b->l, b->j2, b->tz2, b->occ, b->cvq, b->emax);
+  if (!e.empty()) return fail(ctx, IMSRG_E_INVALID, e);
+  e = ctx->db.upload(ctx->hb, b->emax);
+  if (!e.empty()) return fail(ctx, IMSRG_E_HIP, e);
+  HIPCHK(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  const BasisDev &B = ctx->db.h;
+  HIPCHK(ctx, hipMalloc(&ctx->Xt, sizeof(double) * std::max<long>(B.xsize, 1)));
+  HIPCHK(ctx, hipMalloc(&ctx->Yt, sizeof(double) * std::max<long>(B.xsize, 1)));
+  HIPCHK(ctx, hipMalloc(&ctx->Zbar, sizeof(double) * std::max<long>(B.zsize, 1)));
+  ctx->npart = 257;
+  HIPCHK(ctx, hipMalloc(&ctx->partials, sizeof(double) * ctx->npart));
+  HIPCHK(ctx, hipMalloc(&ctx->norm_dev, sizeof(double) * 2));
+  HIPCHK(ctx, hipHostMalloc(&ctx->norm_host, sizeof(double) * 2, hipHostMallocDefault));
+  for (auto &ev : ctx->ev) HIPCHK(ctx, hipEventCreate(&ev));
+  return IMSRG_OK;
+}
+
+extern "C" int imsrg_hip_ctx_create(const imsrg_basis_desc *b, int device, imsrg_ctx **out) {
+  if (!b || !out) {
+    g_err = "null argument";
+    return IMSRG_E_INVALID;
+  }
+  *out = nullptr;
+  if (b->norb <= 0 || !b->n || !b->l || !b->j2 || !b->tz2 || !b->occ || !b->cvq || b->emax < 0) {
+    g_err = "invalid basis description";
+    return IMSRG_E_INVALID;
+  }
+  imsrg_ctx *ctx = new imsrg_ctx();
+  int rc = ctx_init(ctx, b, device);
+  if (rc != IMSRG_OK) {
+    g_err = ctx->err;
+    imsrg_hip_ctx_destroy(ctx);
+    return rc;
+  }
+  *out = ctx;
+  return IMSRG_OK;
+}
+
+extern "C" int imsrg_hip_ctx_create_ho(int emax, double A, double Z, int device, imsrg_ctx **out) {
+  if (emax < 0 || emax > 20 || A < 0 || Z < 0 || Z > A) {
+    g_err = "invalid emax/A/Z";
+    return IMSRG_E_INVALID;
+  }
+  std::vector<int> n, l, j2, tz2, cvq;
+  std::vector<double> occ;
+  HostBasis::ho_orbits(emax, A, Z, n, l, j2, tz2, occ, cvq);
+  imsrg_basis_desc d{(int)n.size(), n.data(), l.data(), j2.data(), tz2.data(), occ.data(), cvq.data(), emax};
+  return imsrg_hip_ctx_create(&d, device, out);
+}
+
+extern "C" int imsrg_hip_ctx_destroy(imsrg_ctx *ctx) {
+  if (!ctx) return IMSRG_OK;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto *&o : ctx->stage) if (o) { imsrg_hip_op_destroy(o); o = nullptr; }
+  for (auto *&o : ctx->tmp) if (o) { imsrg_hip_op_destroy(o); o = nullptr; }
+  ctx->batch.release();
+  if (ctx->Xt) (void)hipFree(ctx->Xt);
+  if (ctx->Yt) (void)hipFree(ctx->Yt);
+  if (ctx->Zbar) (void)hipFree(ctx->Zbar);
+  if (ctx->partials) (void)hipFree(ctx->partials);
+  if (ctx->norm_dev) (void)hipFree(ctx->norm_dev);
+  if (ctx->norm_host) (void)hipHostFree(ctx->norm_host);
+  if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+  for (auto &ev : ctx->ev) if (ev) (void)hipEventDestroy(ev);
+  ctx->db.release();
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return IMSRG_OK;
+}
+
+extern "C" int imsrg_hip_ctx_info(const imsrg_ctx *ctx, int *norb, int *nch, long *tbsize) {
+  if (!ctx) return IMSRG_E_INVALID;
+  if (norb) *norb = ctx->hb.norb;
+  if (nch) *nch = ctx->hb.nch;
+  if (tbsize) *tbsize = ctx->hb.tbsize;
+  return IMSRG_OK;
+}
+extern "C" int imsrg_hip_ctx_orbit(const imsrg_ctx *ctx, int i, int *n, int *l, int *j2, int *tz2, double *occ,
+                                   int *cvq) {
+  if (!ctx || i < 0 || i >= ctx->hb.norb) return IMSRG_E_INVALID;
+  *n = ctx->hb.n[i]; *l = ctx->hb.l[i]; *j2 = ctx->hb.j2[i]; *tz2 = ctx->hb.tz2[i];
+  *occ = ctx->hb.occ[i]; *cvq = ctx->hb.cvq[i];
+  return IMSRG_OK;
+}
+extern "C" int imsrg_hip_ctx_channel(const imsrg_ctx *ctx, int ch, int *J, int *par, int *tz, int *nk, long *off) {
+  if (!ctx || ch < 0 || ch >= ctx->hb.nch) return IMSRG_E_INVALID;
+  const HostChannel &c = ctx->hb.ch[ch];
+  *J = c.J; *par = c.par; *tz = c.tz; *nk = (int)c.kets.size(); *off = ctx->hb.ch_off[ch];
+  return IMSRG_OK;
+}
+extern "C" int imsrg_hip_ctx_channel_ket(const imsrg_ctx *ctx, int ch, int i, int *p, int *q) {
+  if (!ctx || ch < 0 || ch >= ctx->hb.nch) return IMSRG_E_INVALID;
+  const HostChannel &c = ctx->hb.ch[ch];
+  if (i < 0 || i >= (int)c.kets.size()) return IMSRG_E_INVALID;
+  *p = ctx->hb.kp[c.kets[i]]; *q = ctx->hb.kq[c.kets[i]];
+  return IMSRG_OK;
+}
+extern "C" int imsrg_hip_ctx_sixj(imsrg_ctx *ctx, int jj1, int jj2, int J3, int jj4, int jj5, int J6, double *v) {
+  if (!ctx || !v) return IMSRG_E_INVALID;
+  const BasisDev &B = ctx->db.h;
+  int jjm = 2 * B.d1 - 1;
+  for (int jj : {jj1, jj2, jj4, jj5})
+    if (jj < 1 || jj > jjm || (jj & 1) == 0) return fail(ctx, IMSRG_E_INVALID, "6j: half-integer argument out of range");
+  if (J3 < 0 || J3 >= B.d2 || J6 < 0 || J6 >= B.d2) return fail(ctx, IMSRG_E_INVALID, "6j: integer argument out of range");
+  long idx = sixj_index(B.d1, B.d2, jj1, jj2, J3, jj4, jj5, J6);
+  HIPCHK(ctx, hipMemcpy(v, B.sixj + idx, sizeof(double), hipMemcpyDeviceToHost));
+  return IMSRG_OK;
+}
+extern "C" int imsrg_hip_synchronize(imsrg_ctx *ctx) {
+  if (!ctx) return IMSRG_E_INVALID;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return IMSRG_OK;
+}
+
+// ---------------------------------------------------------------------------
+extern "C" int imsrg_hip_op_create(imsrg_ctx *ctx, int herm, imsrg_dop **out) {
+  if (!ctx || !out) return IMSRG_E_INVALID;
+  *out = nullptr;
+  if (herm != 1 && herm != -1 && herm != 0) return fail(ctx, IMSRG_E_INVALID, "herm must be +1, -1 or 0");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  imsrg_dop *o = new imsrg_dop{ctx, nullptr, herm};
+  hipError_t e = hipMalloc(&o->buf, sizeof(double) * ctx->db.h.opsize);
+  if (e != hipSuccess) {
+    delete o;
+    return fail(ctx, IMSRG_E_NOMEM, "hipMalloc of an operator failed");
+  }
+  e = hipMemsetAsync(o->buf, 0, sizeof(double) * ctx->db.h.opsize, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    (void)hipFree(o->buf);
+    delete o;
+    return fail(ctx, IMSRG_E_HIP, "operator clear failed");
+  }
+  *out = o;
+  return IMSRG_OK;
+}
+extern "C" int imsrg_hip_op_destroy(imsrg_dop *op) {
+  if (!op) return IMSRG_OK;
+  if (op->buf) (void)hipFree(op->buf);
+  delete op;
+  return IMSRG_OK;
+}
+extern "C" int imsrg_hip_op_get_herm(const imsrg_dop *op, int *h) {
+  if (!op || !h) return IMSRG_E_INVALID;
+  *h = op->herm;
+  return IMSRG_OK;
+}
+extern "C" int imsrg_hip_op_set_herm(imsrg_dop *op, int h) {
+  if (!op || (h != 1 && h != -1 && h != 0)) return IMSRG_E_INVALID;
+  op->herm = h;
+  return IMSRG_OK;
+}
+
+static int ensure_pinned(imsrg_ctx *ctx) {
+  if (ctx->pinned) return IMSRG_OK;
+  HIPCHK(ctx, hipHostMalloc(&ctx->pinned, sizeof(double) * ctx->db.h.opsize, hipHostMallocDefault));
+  return IMSRG_OK;
+}
+
+static int view_ok(imsrg_ctx *ctx, const imsrg_op_view *v, bool need_two) {
+  if (!v) return fail(ctx, IMSRG_E_INVALID, "null operator view");
+  if (v->rank_J != 0 || v->rank_T != 0 || v->parity != 0 || v->reduced)
+    return fail(ctx, IMSRG_E_UNSUPPORTED, "only scalar, parity-conserving, non-reduced operators are supported");
+  if (v->herm != 1 && v->herm != -1)
+    return fail(ctx, IMSRG_E_UNSUPPORTED, "only Hermitian or anti-Hermitian operators are supported");
+  if (v->particle_rank < 1 || v->particle_rank > 2)
+    return fail(ctx, IMSRG_E_UNSUPPORTED, "particle rank must be 1 or 2");
+  if (!v->one_body) return fail(ctx, IMSRG_E_INVALID, "null one-body pointer");
+  if (need_two && v->particle_rank == 2 && !v->two_body) return fail(ctx, IMSRG_E_INVALID, "null two-body table");
+  return IMSRG_OK;
+}
+
+extern "C" int imsrg_hip_op_upload(imsrg_dop *op, const imsrg_op_view *v) {
+  if (!op) return IMSRG_E_INVALID;
+  imsrg_ctx *ctx = op->ctx;
+  int rc = view_ok(ctx, v, true);
+  if (rc) return rc;
+  rc = ensure_pinned(ctx);
+  if (rc) return rc;
+  const BasisDev &B = ctx->db.h;
+  double *h = ctx->pinned;
+  std::memset(h, 0, sizeof(double) * B.opsize);
+  h[0] = v->zero_body;
+  std::memcpy(h + B.one_off, v->one_body, sizeof(double) * B.norb * B.norb);
+  if (v->particle_rank == 2)
+    for (int c = 0; c < ctx->hb.nch; ++c) {
+      size_t n = ctx->hb.ch[c].kets.size();
+      if (n == 0) continue;
+      if (!v->two_body[c]) return fail(ctx, IMSRG_E_INVALID, "null two-body block for a non-empty channel");
+      std::memcpy(h + B.two_off + ctx->hb.ch_off[c], v->two_body[c], sizeof(double) * n * n);
+    }
+  HIPCHK(ctx, hipMemcpyAsync(op->buf, h, sizeof(double) * B.opsize, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  op->herm = v->herm;
+  return IMSRG_OK;
+}
+
+extern "C" int imsrg_hip_op_download(imsrg_dop *op, imsrg_op_view *v) {
+  if (!op || !v || !v->one_body) return IMSRG_E_INVALID;
+  imsrg_ctx *ctx = op->ctx;
+  int rc = ensure_pinned(ctx);
+  if (rc) return rc;
+  const BasisDev &B = ctx->db.h;
+  double *h = ctx->pinned;
+  HIPCHK(ctx, hipMemcpyAsync(h, op->buf, sizeof(double) * B.opsize, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  v->zero_body = h[0];
+  std::memcpy(v->one_body, h + B.one_off, sizeof(double) * B.norb * B.norb);
+  if (v->two_body)
+    for (int c = 0; c < ctx->hb.nch; ++c) {
+      size_t n = ctx->hb.ch[c].kets.size();
+      if (n == 0 || !v->two_body[c]) continue;
+      std::memcpy(v->two_body[c], h + B.two_off + ctx->hb.ch_off[c], sizeof(double) * n * n);
+    }
+  v->herm = op->herm;
+  return IMSRG_OK;
+}
+
+extern "C" int imsrg_hip_op_upload_packed(imsrg_dop *op, double zero, const double *one, const double *two, int herm) {
+  if (!op || !one || !two) return IMSRG_E_INVALID;
+  imsrg_ctx *ctx = op->ctx;
+  if (herm != 1 && herm != -1 && herm != 0) return fail(ctx, IMSRG_E_INVALID, "herm must be +1, -1 or 0");
+  const BasisDev &B = ctx->db.h;
+  HIPCHK(ctx, hipMemcpyAsync(op->buf, &zero, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(op->buf + B.one_off, one, sizeof(double) * B.norb * B.norb, hipMemcpyHostToDevice,
+                             ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(op->buf + B.two_off, two, sizeof(double) * B.tbsize, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  op->herm = herm;
+  return IMSRG_OK;
+}
+
+extern "C" int imsrg_hip_op_download_packed(imsrg_dop *op, double *zero, double *one, double *two, int *herm) {
+  if (!op) return IMSRG_E_INVALID;
+  imsrg_ctx *ctx = op->ctx;
+  const BasisDev &B = ctx->db.h;
+  if (zero) HIPCHK(ctx, hipMemcpyAsync(zero, op->buf, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  if (one)
+    HIPCHK(ctx, hipMemcpyAsync(one, op->buf + B.one_off, sizeof(double) * B.norb * B.norb, hipMemcpyDeviceToHost,
+                               ctx->stream));
+  if (two)
+    HIPCHK(ctx, hipMemcpyAsync(two, op->buf + B.two_off, sizeof(double) * B.tbsize, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (herm) *herm = op->herm;
+  return IMSRG_OK;
+}
+
+extern "C" int imsrg_hip_op_fill_random(imsrg_dop *op, int herm, unsigned long long seed) {
+  if (!op || (herm != 1 && herm != -1)) return IMSRG_E_INVALID;
+  imsrg_ctx *ctx = op->ctx;
+  HIPCHK(ctx, launch_fill_random(ctx->db.d, ctx->db.h, op->buf, herm, seed, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  op->herm = herm;
+  return IMSRG_OK;
+}
+
+static int same_ctx(const imsrg_dop *a, const imsrg_dop *b) { return a && b && a->ctx == b->ctx; }
+
+extern "C" int imsrg_hip_op_copy(imsrg_dop *dst, const imsrg_dop *src) {
+  if (!same_ctx(dst, src)) return IMSRG_E_INVALID;
+  imsrg_ctx *ctx = dst->ctx;
+  if (dst != src)
+    HIPCHK(ctx, hipMemcpyAsync(dst->buf, src->buf, sizeof(double) * ctx->db.h.opsize, hipMemcpyDeviceToDevice,
+                               ctx->stream));
+  dst->herm = src->herm;
+  return IMSRG_OK;
+}
+extern "C" int imsrg_hip_op_scale(imsrg_dop *op, double f) {
+  if (!op) return IMSRG_E_INVALID;
+  HIPCHK(op->ctx, launch_scale(op->buf, f, op->ctx->db.h.opsize, op->ctx->stream));
+  return IMSRG_OK;
+}
+extern "C" int imsrg_hip_op_axpy(imsrg_dop *y, double a, const imsrg_dop *x) {
+  if (!same_ctx(y, x)) return IMSRG_E_INVALID;
+  HIPCHK(y->ctx, launch_axpy(y->buf, a, x->buf, y->ctx->db.h.opsize, y->ctx->stream));
+  return IMSRG_OK;
+}
+extern "C" int imsrg_hip_op_zero(imsrg_dop *op) {
+  if (!op) return IMSRG_E_INVALID;
+  HIPCHK(op->ctx, launch_scale(op->buf, 0.0, op->ctx->db.h.opsize, op->ctx->stream));
+  return IMSRG_OK;
+}
+
+static int op_norm(imsrg_dop *op, double *n1, double *n2, double *n) {
+  imsrg_ctx *ctx = op->ctx;
+  HIPCHK(ctx, launch_norm(ctx->db.d, ctx->db.h, op->buf, ctx->partials, ctx->npart, ctx->norm_dev, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->norm_host, ctx->norm_dev, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  double a = std::sqrt(ctx->norm_host[0]), b = std::sqrt(ctx->norm_host[1]);
+  if (n1) *n1 = a;
+  if (n2) *n2 = b;
+  if (n) *n = std::sqrt(a * a + b * b);
+  if (!std::isfinite(ctx->norm_host[0]) || !std::isfinite(ctx->norm_host[1]))
+    return fail(ctx, IMSRG_E_NONFINITE, "operator holds non-finite values");
+  return IMSRG_OK;
+}
+extern "C" int imsrg_hip_op_norm(imsrg_dop *op, double *n1, double *n2, double *n) {
+  if (!op) return IMSRG_E_INVALID;
+  return op_norm(op, n1, n2, n);
+}
+
+// ---------------------------------------------------------------------------
+static int comm_opts_ok(imsrg_ctx *ctx, const imsrg_comm_opts *o, unsigned *mask, int *prec) {
+  *mask = IMSRG_ALL_TERMS;
+  *prec = IMSRG_PREC_FP64;
+  if (o) {
+    if (o->use_imsrg3) return fail(ctx, IMSRG_E_UNSUPPORTED, "IMSRG(3) terms are not provided");
+    if (o->term_mask) *mask = o->term_mask & IMSRG_ALL_TERMS;
+    if (o->precision != IMSRG_PREC_FP64 && o->precision != IMSRG_PREC_MIXED)
+      return fail(ctx, IMSRG_E_INVALID, "unknown precision mode");
+    *prec = o->precision;
+  }
+  return IMSRG_OK;
+}
+
+static int herm_rule(int hx, int hy) { return (hx == hy) ? -1 : 1; }  // src/Commutator.cc:292-297
+
+// Launch sequence of one commutator (or a batch when pa.tab is set).
+static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const int4 *const *items, const int *cnt,
+                          unsigned mask, int prec, int maxn) {
+  hipStream_t s = ctx->stream;
+  bool prof = ctx->profiling;
+  auto mark = [&](int i) -> hipError_t { return prof ? hipEventRecord(ctx->ev[i], s) : hipSuccess; };
+  HIPCHK(ctx, mark(0));
+  HIPCHK(ctx, launch_zerobody(pa, npairs, s));
+  HIPCHK(ctx, mark(1));
+  HIPCHK(ctx, launch_onebody(pa, items[5], cnt[5], s));
+  HIPCHK(ctx, mark(2));
+  if (mask & MPPHH)
+    HIPCHK(ctx, launch_pphh(pa, items[0], cnt[0], prec, s));
+  else
+    HIPCHK(ctx, launch_zero_two(pa, items[6], cnt[6], s));
+  HIPCHK(ctx, mark(3));
+  if (mask & M122) HIPCHK(ctx, launch_c122(pa, items[2], cnt[2], maxn, s));
+  HIPCHK(ctx, mark(4));
+  if (mask & MPH) HIPCHK(ctx, launch_pandya(pa, items[3], cnt[3], s));
+  HIPCHK(ctx, mark(5));
+  if (mask & MPH) HIPCHK(ctx, launch_ph_gemm(pa, items[1], cnt[1], prec, s));
+  HIPCHK(ctx, mark(6));
+  if (mask & MPH) HIPCHK(ctx, launch_invpandya(pa, items[4], cnt[4], s));
+  HIPCHK(ctx, mark(7));
+  HIPCHK(ctx, launch_mirror(pa, items[6], cnt[6], s));
+  HIPCHK(ctx, mark(8));
+  if (prof) {
+    HIPCHK(ctx, hipEventSynchronize(ctx->ev[8]));
+    // order of phases in the sequence: zero-body, one-body, pp/hh, 122, Pandya, ph GEMM, inverse, mirror
+    const int map[8] = {0, 1, 3, 2, 4, 5, 6, 7};
+    for (int i = 0; i < 8; ++i) {
+      float ms = 0.f;
+      HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[i], ctx->ev[i + 1]));
+      ctx->phase_us[map[i]] = ms * 1000.0;
+    }
+  }
+  return IMSRG_OK;
+}
+
+static int commutator_dev(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y, imsrg_dop *Z, unsigned mask,
+                          int prec) {
+  if (X->herm != 1 && X->herm != -1) return fail(ctx, IMSRG_E_UNSUPPORTED, "X must be Hermitian or anti-Hermitian");
+  if (Y->herm != 1 && Y->herm != -1) return fail(ctx, IMSRG_E_UNSUPPORTED, "Y must be Hermitian or anti-Hermitian");
+  if (Z == X || Z == Y) return fail(ctx, IMSRG_E_INVALID, "Z must not alias X or Y");
+  const DeviceBasis &db = ctx->db;
+  PairArg pa;
+  pa.tab = nullptr;
+  pa.p0.basis = db.d;
+  pa.p0.X = X->buf;
+  pa.p0.Y = Y->buf;
+  pa.p0.Z = Z->buf;
+  pa.p0.Xt = ctx->Xt;
+  pa.p0.Yt = ctx->Yt;
+  pa.p0.Zbar = ctx->Zbar;
+  pa.p0.hX = X->herm;
+  pa.p0.hY = Y->herm;
+  pa.p0.hZ = herm_rule(X->herm, Y->herm);
+  pa.p0.mask = mask;
+  Z->herm = pa.p0.hZ;
+  const int4 *items[7] = {db.d_pphh, db.d_ph, db.d_c122, db.d_pandya, db.d_invp, db.d_one, db.d_mirror};
+  int cnt[7] = {(int)db.wl.pphh.size(), (int)db.wl.ph.size(), (int)db.wl.c122.size(), (int)db.wl.pandya.size(),
+                (int)db.wl.invp.size(), (int)db.wl.one.size(), (int)db.wl.mirror.size()};
+  return run_commutator(ctx, pa, 1, items, cnt, mask, prec, db.h.maxn);
+}
+
+extern "C" int imsrg_hip_commutator_ss_dev(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y, imsrg_dop *Z,
+                                           const imsrg_comm_opts *opts) {
+  if (!ctx || !X || !Y || !Z) return IMSRG_E_INVALID;
+  if (X->ctx != ctx || Y->ctx != ctx || Z->ctx != ctx) return fail(ctx, IMSRG_E_INVALID, "operator from another context");
+  unsigned mask;
+  int prec;
+  int rc = comm_opts_ok(ctx, opts, &mask, &prec);
+  if (rc) return rc;
+  return commutator_dev(ctx, X, Y, Z, mask, prec);
+}
+
+static int ensure_ops(imsrg_ctx *ctx, imsrg_dop **arr, int n) {
+  for (int i = 0; i < n; ++i)
+    if (!arr[i]) {
+      int rc = imsrg_hip_op_create(ctx, 1, &arr[i]);
+      if (rc) return rc;
+    }
+  return IMSRG_OK;
+}
+
+extern "C" int imsrg_hip_commutator_ss(imsrg_ctx *ctx, const imsrg_op_view *X, const imsrg_op_view *Y,
+                                       imsrg_op_view *Z, const imsrg_comm_opts *opts) {
+  if (!ctx || !X || !Y || !Z) return IMSRG_E_INVALID;
+  unsigned mask;
+  int prec;
+  int rc = comm_opts_ok(ctx, opts, &mask, &prec);
+  if (rc) return rc;
+  if ((rc = view_ok(ctx, X, true)) || (rc = view_ok(ctx, Y, true))) return rc;
+  if (!Z->one_body) return fail(ctx, IMSRG_E_INVALID, "null Z one-body pointer");
+  if ((rc = ensure_ops(ctx, ctx->stage, 3))) return rc;
+  if ((rc = imsrg_hip_op_upload(ctx->stage[0], X))) return rc;
+  if ((rc = imsrg_hip_op_upload(ctx->stage[1], Y))) return rc;
+  if ((rc = commutator_dev(ctx, ctx->stage[0], ctx->stage[1], ctx->stage[2], mask, prec))) return rc;
+  return imsrg_hip_op_download(ctx->stage[2], Z);
+}
+
+// ---------------------------------------------------------------------------
+// BCH on device operators
+static void bch_defaults(const imsrg_bch_opts *o, imsrg_bch_opts *d) {
+  d->mode = 0;
+  d->transform_thr = 1e-9;
+  d->product_thr = 1e-4;
+  d->max_terms = 40;
+  d->precision = IMSRG_PREC_FP64;
+  if (o) {
+    d->mode = o->mode;
+    if (o->transform_thr > 0) d->transform_thr = o->transform_thr;
+    if (o->product_thr > 0) d->product_thr = o->product_thr;
+    if (o->max_terms > 0) d->max_terms = o->max_terms;
+    d->precision = o->precision;
+  }
+}
+
+// Standard_BCH_Transform, src/BCH.cc:87-201
+static int bch_transform(imsrg_ctx *ctx, const imsrg_dop *Op, const imsrg_dop *Omega, imsrg_dop *Out,
+                         const imsrg_bch_opts &o, int *ncomm) {
+  int rc;
+  if ((rc = ensure_ops(ctx, ctx->tmp, 2))) return rc;
+  imsrg_dop *A = ctx->tmp[0], *Bn = ctx->tmp[1];
+  double nx, ny;
+  if ((rc = op_norm(const_cast<imsrg_dop *>(Op), nullptr, nullptr, &nx))) return rc;
+  if ((rc = op_norm(const_cast<imsrg_dop *>(Omega), nullptr, nullptr, &ny))) return rc;
+  if ((rc = imsrg_hip_op_copy(Out, Op))) return rc;
+  int nc = 0;
+  bool converged = true;
+  if (nx > o.transform_thr) {
+    converged = false;
+    if ((rc = imsrg_hip_op_copy(A, Op))) return rc;
+    imsrg_dop *nested = A, *next = Bn;
+    double fact = 1.0;
+    double eps = nx * std::exp(-2 * ny) * o.transform_thr / (2 * ny);
+    for (int i = 1; i <= o.max_terms; ++i) {
+      if ((rc = commutator_dev(ctx, Omega, nested, next, IMSRG_ALL_TERMS, o.precision))) return rc;
+      nc++;
+      std::swap(nested, next);
+      fact /= i;
+      if ((rc = imsrg_hip_op_axpy(Out, fact, nested))) return rc;
+      eps *= i + 1;
+      double nn;
+      if ((rc = op_norm(nested, nullptr, nullptr, &nn))) return rc;
+      if ((o.mode == 0 && nn < eps) || (o.mode == 1 && nn < o.transform_thr)) {
+        converged = true;
+        break;
+      }
+    }
+  }
+  if (ncomm) *ncomm = nc;
+  Out->herm = Op->herm;
+  if (!converged) return fail(ctx, IMSRG_E_NOT_CONVERGED, "BCH_Transform did not converge within max_terms");
+  return IMSRG_OK;
+}
+
+// BCH_Product, src/BCH.cc:267-350
+static int bch_product(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y, imsrg_dop *Out,
+                       const imsrg_bch_opts &o, int *ncomm) {
+  static const double bern[9] = {1.0, -0.5, 1. / 6, 0.0, -1. / 30, 0.0, 1. / 42, 0, -1. / 30};
+  static const double fact[9] = {1.0, 1.0, 2.0, 6.0, 24., 120., 720., 5040., 40320.};
+  int rc;
+  if ((rc = ensure_ops(ctx, ctx->tmp, 3))) return rc;
+  if (Out == X || Out == Y) return fail(ctx, IMSRG_E_INVALID, "Out must not alias X or Y");
+  double nx, ny;
+  if ((rc = op_norm(const_cast<imsrg_dop *>(X), nullptr, nullptr, &nx))) return rc;
+  if ((rc = op_norm(const_cast<imsrg_dop *>(Y), nullptr, nullptr, &ny))) return rc;
+  if ((rc = imsrg_hip_op_copy(Out, X))) return rc;
+  if ((rc = imsrg_hip_op_axpy(Out, 1.0, Y))) return rc;
+  Out->herm = X->herm;
+  imsrg_dop *nested = ctx->tmp[0], *next = ctx->tmp[1];
+  int nc = 0;
+  if ((rc = commutator_dev(ctx, Y, X, nested, IMSRG_ALL_TERMS, o.precision))) return rc;
+  nc++;
+  double nxy;
+  if ((rc = op_norm(nested, nullptr, nullptr, &nxy))) return rc;
+  if (nxy * nx > o.product_thr) {
+    if ((rc = commutator_dev(ctx, nested, X, next, IMSRG_ALL_TERMS, o.precision))) return rc;
+    nc++;
+    if ((rc = imsrg_hip_op_axpy(Out, 1. / 12, next))) return rc;
+  }
+  int k = 1;
+  while (nxy > o.product_thr) {
+    if (k < 2 || k % 2 == 0)
+      if ((rc = imsrg_hip_op_axpy(Out, bern[k] / fact[k], nested))) return rc;
+    k++;
+    if (k >= 9) break;
+    if (2 * ny * nxy < o.product_thr) break;
+    if ((rc = commutator_dev(ctx, Y, nested, next, IMSRG_ALL_TERMS, o.precision))) return rc;
+    nc++;
+    std::swap(nested, next);
+    if ((rc = op_norm(nested, nullptr, nullptr, &nxy))) return rc;
+  }
+  if (ncomm) *ncomm = nc;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return IMSRG_OK;
+}
+
+extern "C" int imsrg_hip_bch_transform_dev(imsrg_ctx *ctx, const imsrg_dop *Op, const imsrg_dop *Omega,
+                                           imsrg_dop *Out, const imsrg_bch_opts *opts, int *ncomm) {
+  if (!ctx || !Op || !Omega || !Out) return IMSRG_E_INVALID;
+  if (Op->ctx != ctx || Omega->ctx != ctx || Out->ctx != ctx) return fail(ctx, IMSRG_E_INVALID, "foreign operator");
+  if (Out == Omega) return fail(ctx, IMSRG_E_INVALID, "Out must not alias Omega");
+  imsrg_bch_opts o;
+  bch_defaults(opts, &o);
+  return bch_transform(ctx, Op, Omega, Out, o, ncomm);
+}
+extern "C" int imsrg_hip_bch_product_dev(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y, imsrg_dop *Out,
+                                         const imsrg_bch_opts *opts, int *ncomm) {
+  if (!ctx || !X || !Y || !Out) return IMSRG_E_INVALID;
+  if (X->ctx != ctx || Y->ctx != ctx || Out->ctx != ctx) return fail(ctx, IMSRG_E_INVALID, "foreign operator");
+  imsrg_bch_opts o;
+  bch_defaults(opts, &o);
+  return bch_product(ctx, X, Y, Out, o, ncomm);
+}
+
+extern "C" int imsrg_hip_bch_transform(imsrg_ctx *ctx, const imsrg_op_view *Op, const imsrg_op_view *Omega,
+                                       imsrg_op_view *Out, const imsrg_bch_opts *opts, int *ncomm) {
+  if (!ctx || !Op || !Omega || !Out) return IMSRG_E_INVALID;
+  int rc;
+  if ((rc = view_ok(ctx, Op, true)) || (rc = view_ok(ctx, Omega, true))) return rc;
+  if ((rc = ensure_ops(ctx, ctx->stage, 3))) return rc;
+  if ((rc = imsrg_hip_op_upload(ctx->stage[0], Op))) return rc;
+  if ((rc = imsrg_hip_op_upload(ctx->stage[1], Omega))) return rc;
+  imsrg_bch_opts o;
+  bch_defaults(opts, &o);
+  rc = bch_transform(ctx, ctx->stage[0], ctx->stage[1], ctx->stage[2], o, ncomm);
+  if (rc && rc != IMSRG_E_NOT_CONVERGED) return rc;
+  int rc2 = imsrg_hip_op_download(ctx->stage[2], Out);
+  return rc2 ? rc2 : rc;
+}
+extern "C" int imsrg_hip_bch_product(imsrg_ctx *ctx, const imsrg_op_view *X, const imsrg_op_view *Y,
+                                     imsrg_op_view *Out, const imsrg_bch_opts *opts, int *ncomm) {
+  if (!ctx || !X || !Y || !Out) return IMSRG_E_INVALID;
+  int rc;
+  if ((rc = view_ok(ctx, X, true)) || (rc = view_ok(ctx, Y, true))) return rc;
+  if ((rc = ensure_ops(ctx, ctx->stage, 3))) return rc;
+  if ((rc = imsrg_hip_op_upload(ctx->stage[0], X))) return rc;
+  if ((rc = imsrg_hip_op_upload(ctx->stage[1], Y))) return rc;
+  imsrg_bch_opts o;
+  bch_defaults(opts, &o);
+  if ((rc = bch_product(ctx, ctx->stage[0], ctx->stage[1], ctx->stage[2], o, ncomm))) return rc;
+  return imsrg_hip_op_download(ctx->stage[2], Out);
+}
+
+// ---------------------------------------------------------------------------
+extern "C" int imsrg_hip_white_generator_dev(imsrg_ctx *ctx, const imsrg_dop *H, imsrg_dop *Eta) {
+  if (!ctx || !H || !Eta || H->ctx != ctx || Eta->ctx != ctx || H == Eta) return IMSRG_E_INVALID;
+  HIPCHK(ctx, launch_scale(Eta->buf, 0.0, ctx->db.h.opsize, ctx->stream));
+  HIPCHK(ctx, launch_white(ctx->db.d, ctx->db.h, H->buf, Eta->buf, ctx->db.d_gen, (int)ctx->db.wl.gen.size(),
+                           ctx->stream));
+  Eta->herm = -1;
+  return IMSRG_OK;
+}
+
+extern "C" int imsrg_hip_magnus_flow(imsrg_ctx *ctx, const imsrg_dop *H0, imsrg_dop *Hs, double ds, double smax,
+                                     double eta_crit, double omega_norm_max, const imsrg_bch_opts *opts,
+                                     int max_steps, double *e0, double *eta_norm, int *ncomm, int *nsteps) {
+  if (!ctx || !H0 || !Hs || H0 == Hs || H0->ctx != ctx || Hs->ctx != ctx || ds <= 0 || max_steps < 0)
+    return IMSRG_E_INVALID;
+  imsrg_bch_opts o;
+  bch_defaults(opts, &o);
+  imsrg_dop *Eta = nullptr, *Om = nullptr, *Om2 = nullptr, *Hsaved = nullptr;
+  int rc = IMSRG_OK;
+  auto cleanup = [&]() {
+    imsrg_hip_op_destroy(Eta); imsrg_hip_op_destroy(Om); imsrg_hip_op_destroy(Om2); imsrg_hip_op_destroy(Hsaved);
+  };
+  if ((rc = imsrg_hip_op_create(ctx, -1, &Eta)) || (rc = imsrg_hip_op_create(ctx, -1, &Om)) ||
+      (rc = imsrg_hip_op_create(ctx, -1, &Om2)) || (rc = imsrg_hip_op_create(ctx, 1, &Hsaved))) {
+    cleanup();
+    return rc;
+  }
+  rc = imsrg_hip_op_copy(Hs, H0);
+  if (!rc) rc = imsrg_hip_op_copy(Hsaved, H0);
+  if (!rc) rc = imsrg_hip_white_generator_dev(ctx, Hs, Eta);
+  int nsplit = 0, step = 0;
+  double s = 0.0, Elast = 0.0;
+  if (!rc) {
+    HIPCHK(ctx, hipMemcpy(&Elast, H0->buf, sizeof(double), hipMemcpyDeviceToHost));
+  }
+  while (!rc && s < smax && step < max_steps) {
+    double ne;
+    if ((rc = op_norm(Eta, nullptr, nullptr, &ne))) break;
+    if (eta_norm) eta_norm[step] = ne;
+    if (ne < eta_crit) break;
+    if (ne > 1e12 || std::fabs(Elast) > 1e9) {
+      rc = fail(ctx, IMSRG_E_NONFINITE, "flow diverged (||eta|| > 1e12 or |E0| > 1e9)");
+      break;
+    }
+    double no;
+    if ((rc = op_norm(Om, nullptr, nullptr, &no))) break;
+    if (no > omega_norm_max) {  // NewOmega, src/IMSRGSolver.cc:44-84
+      if ((rc = imsrg_hip_op_copy(Hsaved, Hs))) break;
+      if ((rc = imsrg_hip_op_zero(Om))) break;
+      nsplit++;
+    }
+    double dss = std::min(ds, smax - s);
+    s += dss;
+    if ((rc = imsrg_hip_op_scale(Eta, dss))) break;
+    int nc1 = 0, nc2 = 0;
+    if ((rc = bch_product(ctx, Eta, Om, Om2, o, &nc1))) break;
+    std::swap(Om, Om2);
+    Om->herm = -1;
+    rc = bch_transform(ctx, nsplit == 0 ? H0 : Hsaved, Om, Hs, o, &nc2);
+    if (rc) break;
+    if ((rc = imsrg_hip_white_generator_dev(ctx, Hs, Eta))) break;
+    double ez;
+    HIPCHK(ctx, hipMemcpyAsync(&ez, Hs->buf, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (e0) e0[step] = ez;
+    if (ncomm) ncomm[step] = nc1 + nc2;
+    Elast = ez;
+    step++;
+  }
+  if (nsteps) *nsteps = step;
+  cleanup();
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Batched commutators: one launch per phase over the concatenated work lists.
+extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctxs, const imsrg_dop *const *X,
+                                               const imsrg_dop *const *Y, imsrg_dop *const *Z,
+                                               const imsrg_comm_opts *opts) {
+  if (npairs <= 0 || !ctxs || !X || !Y || !Z) return IMSRG_E_INVALID;
+  imsrg_ctx *c0 = ctxs[0];
+  if (!c0) return IMSRG_E_INVALID;
+  unsigned mask;
+  int prec;
+  int rc = comm_opts_ok(c0, opts, &mask, &prec);
+  if (rc) return rc;
+  for (int i = 0; i < npairs; ++i) {
+    if (!ctxs[i] || !X[i] || !Y[i] || !Z[i]) return fail(c0, IMSRG_E_INVALID, "null entry in batch");
+    if (ctxs[i]->device != c0->device) return fail(c0, IMSRG_E_INVALID, "batch spans devices");
+    if (X[i]->ctx != ctxs[i] || Y[i]->ctx != ctxs[i] || Z[i]->ctx != ctxs[i])
+      return fail(c0, IMSRG_E_INVALID, "operator from another context");
+    if (X[i]->herm == 0 || Y[i]->herm == 0) return fail(c0, IMSRG_E_UNSUPPORTED, "non-Hermitian operator in batch");
+    if (Z[i] == X[i] || Z[i] == Y[i]) return fail(c0, IMSRG_E_INVALID, "Z aliases X or Y");
+  }
+  BatchCache &bc = c0->batch;
+  std::vector<const void *> key;
+  for (int i = 0; i < npairs; ++i) {
+    key.push_back(ctxs[i]); key.push_back(X[i]); key.push_back(Y[i]); key.push_back(Z[i]);
+    key.push_back((const void *)(intptr_t)(X[i]->herm * 4 + Y[i]->herm + 8 * (int)mask));
+  }
+  if (key != bc.key) {
+    bc.release();
+    // per-pair workspaces (a context used by several pairs needs separate ones)
+    long wtot = 0;
+    std::vector<long> woff(npairs);
+    for (int i = 0; i < npairs; ++i) {
+      woff[i] = wtot;
+      wtot += 2 * ctxs[i]->db.h.xsize + ctxs[i]->db.h.zsize + 64;
+    }
+    HIPCHK(c0, hipMalloc(&bc.work, sizeof(double) * std::max<long>(wtot, 1)));
+    std::vector<PairPtrs> tab(npairs);
+    for (int i = 0; i < npairs; ++i) {
+      const BasisDev &B = ctxs[i]->db.h;
+      PairPtrs &p = tab[i];
+      p.basis = ctxs[i]->db.d;
+      p.X = X[i]->buf; p.Y = Y[i]->buf; p.Z = Z[i]->buf;
+      p.Xt = bc.work + woff[i];
+      p.Yt = p.Xt + B.xsize;
+      p.Zbar = p.Yt + B.xsize;
+      p.hX = X[i]->herm; p.hY = Y[i]->herm; p.hZ = herm_rule(X[i]->herm, Y[i]->herm);
+      p.mask = mask;
+    }
+    HIPCHK(c0, hipMalloc(&bc.d_tab, sizeof(PairPtrs) * npairs));
+    HIPCHK(c0, hipMemcpy(bc.d_tab, tab.data(), sizeof(PairPtrs) * npairs, hipMemcpyHostToDevice));
+    // concatenated lists: pphh tiles are merged across pairs and sorted by cost
+    std::vector<int4> all[7];
+    struct T { int4 w; long cost; };
+    std::vector<T> pp;
+    for (int i = 0; i < npairs; ++i) {
+      const WorkLists &wl = ctxs[i]->db.wl;
+      const std::vector<int4> *src[7] = {&wl.pphh, &wl.ph, &wl.c122, &wl.pandya, &wl.invp, &wl.one, &wl.mirror};
+      for (int t = 0; t < 7; ++t)
+        for (int4 w : *src[t]) {
+          w.x = i;
+          if (t == 0) {
+            const HostBasis &hb = ctxs[i]->hb;
+            long n = hb.ch[w.y].kets.size();
+            pp.push_back({w, n});
+          } else {
+            all[t].push_back(w);
+          }
+        }
+      bc.maxn = std::max(bc.maxn, ctxs[i]->db.h.maxn);
+    }
+    std::stable_sort(pp.begin(), pp.end(), [](const T &a, const T &b) { return a.cost > b.cost; });
+    for (auto &t : pp) all[0].push_back(t.w);
+    size_t tot = 0;
+    for (int t = 0; t < 7; ++t) {
+      bc.off[t] = tot;
+      bc.cnt[t] = (int)all[t].size();
+      tot += all[t].size();
+    }
+    HIPCHK(c0, hipMalloc(&bc.d_items, sizeof(int4) * std::max<size_t>(tot, 1)));
+    for (int t = 0; t < 7; ++t)
+      if (!all[t].empty())
+        HIPCHK(c0, hipMemcpy(bc.d_items + bc.off[t], all[t].data(), sizeof(int4) * all[t].size(),
+                             hipMemcpyHostToDevice));
+    bc.key = key;
+  }
+  for (int i = 0; i < npairs; ++i) Z[i]->herm = herm_rule(X[i]->herm, Y[i]->herm);
+  PairArg pa;
+  std::memset(&pa.p0, 0, sizeof(pa.p0));
+  pa.tab = bc.d_tab;
+  const int4 *items[7];
+  for (int t = 0; t < 7; ++t) items[t] = bc.d_items + bc.off[t];
+  return run_commutator(c0, pa, npairs, items, bc.cnt, mask, prec, bc.maxn);
+}
+
+// ---------------------------------------------------------------------------
+extern "C" int imsrg_hip_set_profiling(imsrg_ctx *ctx, int on) {
+  if (!ctx) return IMSRG_E_INVALID;
+  ctx->profiling = on != 0;
+  return IMSRG_OK;
+}
+extern "C" int imsrg_hip_phase_times(imsrg_ctx *ctx, double *us, int n) {
+  if (!ctx || !us) return IMSRG_E_INVALID;
+  for (int i = 0; i < n && i < 8; ++i) us[i] = ctx->phase_us[i];
+  return IMSRG_OK;
+}
+
+// Lane layout self-test of v_mfma_f64_16x16x4f64 (not part of the reference interface).
+extern "C" int imsrg_hip_selftest_mfma(imsrg_ctx *ctx, double *out512) {
+  if (!ctx || !out512) return IMSRG_E_INVALID;
+  double *d = nullptr;
+  HIPCHK(ctx, hipMalloc(&d, 512 * sizeof(double)));
+  hipError_t e = launch_mfma_probe(d, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out512, d, 512 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(ctx, IMSRG_E_HIP, hipGetErrorString(e));
+  return IMSRG_OK;
+}

@@ -1,0 +1,371 @@
+// basis.cpp -- model-space construction and device tables.
+#include "basis.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "launch.hpp"
+
+namespace imsrg {
+
+static const double kOccCut = 1e-6;  // ModelSpace::OCC_CUT, src/ModelSpace.cc:14
+
+void HostBasis::ho_orbits(int emax, double A, double Z, std::vector<int> &n, std::vector<int> &l,
+                          std::vector<int> &j2, std::vector<int> &tz2, std::vector<double> &occ,
+                          std::vector<int> &cvq) {
+  // orbit order of src/ModelSpace.cc:376-399
+  n.clear(); l.clear(); j2.clear(); tz2.clear(); occ.clear(); cvq.clear();
+  for (int N = 0; N <= emax; ++N)
+    for (int ll = N; ll >= 0; ll -= 2) {
+      int nn = (N - ll) / 2;
+      for (int jj = 2 * ll + 1; jj >= 2 * ll - 1 && jj > 0; jj -= 2)
+        for (int t : {-1, 1}) {
+          n.push_back(nn); l.push_back(ll); j2.push_back(jj); tz2.push_back(t);
+          occ.push_back(0.0); cvq.push_back(2);
+        }
+    }
+  // filling of GetOrbitsAZ, src/ModelSpace.cc:611-645; core = reference
+  double zz = 0, nn = 0;
+  bool done = false;
+  for (int N = 0; N <= emax && !done; ++N)
+    for (int g = 2 * N + 1; g >= -2 * N && !done; g -= 4) {
+      int jj = std::abs(g);
+      int ll = g < 0 ? (jj + 1) / 2 : (jj - 1) / 2;
+      int nr = (N - ll) / 2;
+      for (size_t i = 0; i < n.size(); ++i)
+        if (n[i] == nr && l[i] == ll && j2[i] == jj && tz2[i] == -1 && zz < Z) {
+          double dz = std::min(Z - zz, jj + 1.0);
+          occ[i] = dz / (jj + 1.0);
+          cvq[i] = 0;
+          zz += dz;
+        }
+      for (size_t i = 0; i < n.size(); ++i)
+        if (n[i] == nr && l[i] == ll && j2[i] == jj && tz2[i] == 1 && nn < A - Z) {
+          double dn = std::min(A - Z - nn, jj + 1.0);
+          occ[i] = dn / (jj + 1.0);
+          cvq[i] = 0;
+          nn += dn;
+        }
+      if (zz == Z && nn == A - Z) done = true;
+    }
+}
+
+std::string HostBasis::build(int norb_, const int *n_, const int *l_, const int *j2_, const int *tz2_,
+                             const double *occ_, const int *cvq_, int emax_) {
+  if (norb_ <= 0) return "norb must be positive";
+  emax = emax_;
+  norb = norb_;
+  n.assign(n_, n_ + norb); l.assign(l_, l_ + norb); j2.assign(j2_, j2_ + norb); tz2.assign(tz2_, tz2_ + norb);
+  occ.assign(occ_, occ_ + norb); cvq.assign(cvq_, cvq_ + norb);
+  jmax = 0;
+  for (int i = 0; i < norb; ++i) {
+    if (j2[i] <= 0 || (j2[i] & 1) == 0) return "j2 must be odd and positive";
+    if (tz2[i] != 1 && tz2[i] != -1) return "tz2 must be +-1";
+    if (l[i] < 0 || std::abs(2 * l[i] - j2[i]) != 1) return "inconsistent l and j2";
+    if (j2[i] > 2 * emax + 1) return "j2 exceeds 2*emax+1 (6j table range)";
+    jmax = std::max(jmax, j2[i]);
+  }
+  nch = 6 * (jmax + 1);
+  nkets = norb * (norb + 1) / 2;
+  kp.assign(nkets, 0); kq.assign(nkets, 0);
+  for (int p = 0; p < norb; ++p)
+    for (int q = p; q < norb; ++q) {
+      int k = ket_index(norb, p, q);
+      kp[k] = p; kq[k] = q;
+    }
+  ob.assign(norb, {});
+  holes.clear();
+  for (int i = 0; i < norb; ++i) {
+    for (int j = 0; j < norb; ++j)
+      if (l[j] == l[i] && j2[j] == j2[i] && tz2[j] == tz2[i]) ob[i].push_back(j);
+    if (occ[i] > kOccCut) holes.push_back(i);
+  }
+  ch.assign(nch, {});
+  cc.assign(nch, {});
+  kmap.assign((size_t)nkets * (jmax + 1), -1);
+  cmap.assign((size_t)nkets * (jmax + 1), -1);
+  ch_off.assign(nch, 0);
+  long off = 0;
+  for (int c = 0; c < nch; ++c) {
+    int J = c / 6, tz = (c % 6) / 2 - 1, par = c % 2;
+    ch[c].J = cc[c].J = J; ch[c].tz = cc[c].tz = tz; ch[c].par = cc[c].par = par;
+    for (int k = 0; k < nkets; ++k) {
+      int p = kp[k], q = kq[k];
+      bool tri = (j2[p] + j2[q] >= 2 * J) && (std::abs(j2[p] - j2[q]) <= 2 * J);
+      bool parok = ((l[p] + l[q]) % 2) == par;
+      // TwoBodyChannel::CheckChannel_ket, src/TwoBodyChannel.cc:142-151
+      if (tri && parok && !(p == q && (J % 2) != 0) && tz2[p] + tz2[q] == 2 * tz) {
+        kmap[(size_t)k * (jmax + 1) + J] = (int)ch[c].kets.size();
+        ch[c].kets.push_back(k);
+      }
+      // TwoBodyChannel_CC::CheckChannel_ket, src/TwoBodyChannel.cc:180-188
+      if (tri && parok && std::abs(tz2[p] - tz2[q]) == 2 * tz) {
+        cmap[(size_t)k * (jmax + 1) + J] = (int)cc[c].kets.size();
+        cc[c].kets.push_back(k);
+      }
+    }
+    ch_off[c] = off;
+    off += (long)ch[c].kets.size() * ch[c].kets.size();
+  }
+  tbsize = off;
+  return "";
+}
+
+// ---------------------------------------------------------------------------
+template <typename T>
+static T *dupload(DeviceBasis &db, const std::vector<T> &v, std::string &err) {
+  size_t bytes = std::max<size_t>(v.size(), 1) * sizeof(T);
+  void *p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    err = "hipMalloc failed for basis table";
+    return nullptr;
+  }
+  db.allocs.push_back(p);
+  if (!v.empty() && hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+    err = "hipMemcpy failed for basis table";
+  return (T *)p;
+}
+
+std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
+  std::string err;
+  BasisDev &B = h;
+  std::memset(&B, 0, sizeof(B));
+  B.norb = hb.norb; B.nkets = hb.nkets; B.nch = hb.nch; B.jmax = hb.jmax;
+  int jjmax = 2 * emax6j + 1;
+  B.d1 = (jjmax - 1) / 2 + 1;
+  B.d2 = jjmax + 1;
+  B.nholes = (int)hb.holes.size();
+  B.tbsize = hb.tbsize;
+  B.one_off = 8;
+  long n2 = (long)hb.norb * hb.norb;
+  B.two_off = 8 + ((n2 + 7) / 8) * 8;
+  B.opsize = B.two_off + ((hb.tbsize + 7) / 8) * 8;
+  const int norb = hb.norb, nch = hb.nch;
+
+  B.j2 = dupload(*this, hb.j2, err); B.l = dupload(*this, hb.l, err); B.tz2 = dupload(*this, hb.tz2, err);
+  B.cvq = dupload(*this, hb.cvq, err); B.occ = dupload(*this, hb.occ, err);
+  B.holes = dupload(*this, hb.holes, err);
+  std::vector<int> obstart(norb + 1, 0), oblist;
+  for (int i = 0; i < norb; ++i) {
+    obstart[i] = (int)oblist.size();
+    oblist.insert(oblist.end(), hb.ob[i].begin(), hb.ob[i].end());
+  }
+  obstart[norb] = (int)oblist.size();
+  B.obstart = dupload(*this, obstart, err); B.oblist = dupload(*this, oblist, err);
+  B.kp = dupload(*this, hb.kp, err); B.kq = dupload(*this, hb.kq, err);
+  B.kmap = dupload(*this, hb.kmap, err); B.cmap = dupload(*this, hb.cmap, err);
+
+  // ---- standard channels ----
+  std::vector<int> ch_n(nch), ch_kstart(nch + 1), ch_kets;
+  std::vector<double> w220;
+  std::vector<int> hh_start(nch + 1), hh_loc, k_start(nch + 1), k_loc, kpp_start(nch + 1), kpp_loc,
+      khh_start(nch + 1), khh_loc, gcore_start(nch + 1), gcore_loc, gnc_start(nch + 1), gnc_loc;
+  std::vector<double> hh_occ, k_w, kpp_w, khh_w;
+  std::vector<int> p122_start;
+  std::vector<int4> p122_e;
+  std::vector<double> p122_f;
+  int maxn = 0;
+  const double sqrt2 = std::sqrt(2.0);
+  for (int c = 0; c < nch; ++c) {
+    const HostChannel &C = hb.ch[c];
+    int nk = (int)C.kets.size();
+    maxn = std::max(maxn, nk);
+    ch_n[c] = nk;
+    ch_kstart[c] = (int)ch_kets.size();
+    hh_start[c] = (int)hh_loc.size(); k_start[c] = (int)k_loc.size();
+    kpp_start[c] = (int)kpp_loc.size(); khh_start[c] = (int)khh_loc.size();
+    gcore_start[c] = (int)gcore_loc.size(); gnc_start[c] = (int)gnc_loc.size();
+    for (int i = 0; i < nk; ++i) {
+      int k = C.kets[i];
+      int p = hb.kp[k], q = hb.kq[k];
+      ch_kets.push_back(k);
+      double op = hb.occ[p], oq = hb.occ[q];
+      bool hp = op > kOccCut, hq = oq > kOccCut;
+      bool is_pp = (op < kOccCut && oq < kOccCut), is_hh = hp && hq, is_ph = (hp != hq);
+      // comm220ss weights, src/Commutator.cc:615-627
+      w220.push_back(is_pp ? 1.0 : ((is_hh || is_ph) ? (1 - op) * (1 - oq) : 0.0));
+      if (is_hh) { hh_loc.push_back(i); hh_occ.push_back(op * oq); }
+      // ConstructScalarMpp_Mhh weights, src/Commutator.cc:1171-1183
+      double wpp = is_pp ? 1.0 : ((is_hh || is_ph) ? (1 - op) * (1 - oq) : 0.0);
+      double whh = is_hh ? op * oq : 0.0;
+      if (wpp != 0.0) { kpp_loc.push_back(i); kpp_w.push_back(wpp); }
+      if (whh != 0.0) { khh_loc.push_back(i); khh_w.push_back(whh); }
+      if (wpp - whh != 0.0) { k_loc.push_back(i); k_w.push_back(wpp - whh); }
+      // generator lists (src/Generator.cc:263-271): cc kets and (qq, vv, qv) kets
+      if (hb.cvq[p] == 0 && hb.cvq[q] == 0) gcore_loc.push_back(i);
+      if (hb.cvq[p] != 0 && hb.cvq[q] != 0) gnc_loc.push_back(i);
+      // comm122ss partner list, src/Commutator.cc:857-896
+      p122_start.push_back((int)p122_e.size());
+      int ii = p, jj = q;
+      int flip = -sgn_parity((hb.j2[ii] + hb.j2[jj]) / 2 - C.J);
+      for (int a : hb.ob[ii]) {
+        int lo = std::min(a, jj), hi = std::max(a, jj);
+        int loc = hb.kmap[(size_t)ket_index(norb, lo, hi) * (hb.jmax + 1) + C.J];
+        if (loc < 0) continue;
+        double f = (a > jj) ? (double)flip : (a == jj ? sqrt2 : 1.0);
+        if (ii == jj) f /= sqrt2;
+        p122_e.push_back(make_int4(loc, a, ii, 0));
+        p122_f.push_back(f);
+      }
+      if (ii != jj)
+        for (int a : hb.ob[jj]) {
+          int lo = std::min(a, ii), hi = std::max(a, ii);
+          int loc = hb.kmap[(size_t)ket_index(norb, lo, hi) * (hb.jmax + 1) + C.J];
+          if (loc < 0) continue;
+          double f = (ii > a) ? (double)flip : (ii == a ? sqrt2 : 1.0);
+          p122_e.push_back(make_int4(loc, a, jj, 0));
+          p122_f.push_back(f);
+        }
+      if ((int)p122_e.size() - p122_start.back() > kMax122) return "comm122 partner list too long";
+    }
+  }
+  ch_kstart[nch] = (int)ch_kets.size();
+  hh_start[nch] = (int)hh_loc.size(); k_start[nch] = (int)k_loc.size();
+  kpp_start[nch] = (int)kpp_loc.size(); khh_start[nch] = (int)khh_loc.size();
+  gcore_start[nch] = (int)gcore_loc.size(); gnc_start[nch] = (int)gnc_loc.size();
+  p122_start.push_back((int)p122_e.size());
+  B.maxn = maxn;
+  B.ch_off = dupload(*this, hb.ch_off, err);
+  B.ch_n = dupload(*this, ch_n, err); B.ch_kstart = dupload(*this, ch_kstart, err);
+  B.ch_kets = dupload(*this, ch_kets, err); B.w220 = dupload(*this, w220, err);
+  B.hh_start = dupload(*this, hh_start, err); B.hh_loc = dupload(*this, hh_loc, err);
+  B.hh_occ = dupload(*this, hh_occ, err);
+  B.k_start = dupload(*this, k_start, err); B.k_loc = dupload(*this, k_loc, err); B.k_w = dupload(*this, k_w, err);
+  B.kpp_start = dupload(*this, kpp_start, err); B.kpp_loc = dupload(*this, kpp_loc, err);
+  B.kpp_w = dupload(*this, kpp_w, err);
+  B.khh_start = dupload(*this, khh_start, err); B.khh_loc = dupload(*this, khh_loc, err);
+  B.khh_w = dupload(*this, khh_w, err);
+  B.p122_start = dupload(*this, p122_start, err); B.p122_e = dupload(*this, p122_e, err);
+  B.p122_f = dupload(*this, p122_f, err);
+  B.gcore_start = dupload(*this, gcore_start, err); B.gcore_loc = dupload(*this, gcore_loc, err);
+  B.gnc_start = dupload(*this, gnc_start, err); B.gnc_loc = dupload(*this, gnc_loc, err);
+
+  // ---- cross-coupled channels ----
+  std::vector<int> cc_n(nch), cc_nph(nch), cc_kstart(nch + 1), cc_kets, cc_phstart(nch + 1), cc_ph;
+  std::vector<signed char> cc_phi;
+  std::vector<long> cc_xoff(nch + 1), cc_zoff(nch + 1);
+  long xo = 0, zo = 0;
+  int maxcc = 0;
+  for (int c = 0; c < nch; ++c) {
+    const HostChannel &C = hb.cc[c];
+    int nk = (int)C.kets.size();
+    maxcc = std::max(maxcc, nk);
+    cc_n[c] = nk;
+    cc_kstart[c] = (int)cc_kets.size();
+    cc_phstart[c] = (int)cc_ph.size();
+    std::vector<int> hh, ph;
+    for (int i = 0; i < nk; ++i) {
+      int k = C.kets[i];
+      int p = hb.kp[k], q = hb.kq[k];
+      cc_kets.push_back(k);
+      cc_phi.push_back((signed char)sgn_parity((hb.j2[p] + hb.j2[q]) / 2));
+      bool hp = hb.occ[p] > kOccCut, hq = hb.occ[q] > kOccCut;
+      if (hp && hq) hh.push_back(i);
+      if (hp != hq) ph.push_back(i);
+    }
+    // kets_ph = join_cols(hh, ph), src/Commutator.cc:1398
+    cc_ph.insert(cc_ph.end(), hh.begin(), hh.end());
+    cc_ph.insert(cc_ph.end(), ph.begin(), ph.end());
+    cc_nph[c] = (int)(hh.size() + ph.size());
+    cc_xoff[c] = xo;
+    cc_zoff[c] = zo;
+    xo += (long)nk * 2 * cc_nph[c];
+    zo += (long)nk * 2 * nk;
+  }
+  cc_kstart[nch] = (int)cc_kets.size();
+  cc_phstart[nch] = (int)cc_ph.size();
+  cc_xoff[nch] = xo;
+  cc_zoff[nch] = zo;
+  B.maxcc = maxcc;
+  B.xsize = xo;
+  B.zsize = zo;
+  B.cc_n = dupload(*this, cc_n, err); B.cc_nph = dupload(*this, cc_nph, err);
+  B.cc_kstart = dupload(*this, cc_kstart, err); B.cc_kets = dupload(*this, cc_kets, err);
+  B.cc_phstart = dupload(*this, cc_phstart, err); B.cc_ph = dupload(*this, cc_ph, err);
+  B.cc_phi = dupload(*this, cc_phi, err);
+  B.cc_xoff = dupload(*this, cc_xoff, err); B.cc_zoff = dupload(*this, cc_zoff, err);
+
+  // ---- one-body pairs and norm weights ----
+  std::vector<int2> pairs1;
+  std::vector<double> w1(n2, 0.0);
+  for (int i = 0; i < norb; ++i)
+    for (int j : hb.ob[i]) {
+      if (j >= i) pairs1.push_back(make_int2(i, j));
+      w1[(long)j * norb + i] = hb.j2[i] + 1;  // Operator::OneBodyNorm degeneracy (scalar)
+    }
+  B.n1pairs = (int)pairs1.size();
+  B.pairs1 = dupload(*this, pairs1, err);
+  B.w1norm = dupload(*this, w1, err);
+  if (!err.empty()) return err;
+
+  // ---- 6j table on the device ----
+  {
+    long tot = (long)B.d1 * B.d1 * B.d2 * B.d1 * B.d1 * B.d2;
+    void *p = nullptr;
+    if (hipMalloc(&p, tot * sizeof(double)) != hipSuccess) return "hipMalloc failed for the 6j table";
+    allocs.push_back(p);
+    if (launch_sixj_table((double *)p, B.d1, B.d2, 0) != hipSuccess) return "6j table kernel failed";
+    if (hipDeviceSynchronize() != hipSuccess) return "6j table kernel failed";
+    B.sixj = (const double *)p;
+  }
+
+  // ---- work lists ----
+  wl = WorkLists();
+  {
+    struct T { int4 w; long cost; };
+    std::vector<T> tiles;
+    for (int c = 0; c < nch; ++c) {
+      int nk = ch_n[c];
+      if (nk == 0) continue;
+      int nt = (nk + kTile - 1) / kTile;
+      long kk = k_start[c + 1] - k_start[c];
+      for (int rb = 0; rb < nt; ++rb)
+        for (int sb = rb; sb < nt; ++sb) tiles.push_back({make_int4(0, c, rb, sb), kk});
+    }
+    std::stable_sort(tiles.begin(), tiles.end(), [](const T &a, const T &b) { return a.cost > b.cost; });
+    for (auto &t : tiles) wl.pphh.push_back(t.w);
+    for (int c = 0; c < nch; ++c) {
+      int nk = ch_n[c];
+      if (nk == 0) continue;
+      int nt = (nk + kTile - 1) / kTile;
+      for (int rb = 0; rb < nt; ++rb)
+        for (int sb = rb; sb < nt; ++sb) wl.mirror.push_back(make_int4(0, c, rb, sb));
+      for (int s = 0; s < nk; ++s) wl.c122.push_back(make_int4(0, c, s, 0));
+      for (int ib = 0; ib < nk; ++ib)
+        for (int ks = ib; ks < nk; ks += kChunk) wl.invp.push_back(make_int4(0, c, ib, ks));
+      if (gcore_start[c + 1] > gcore_start[c] && gnc_start[c + 1] > gnc_start[c])
+        wl.gen.push_back(make_int4(0, c, 0, 0));
+    }
+    for (int c = 0; c < nch; ++c) {
+      int nk = cc_n[c], nph = cc_nph[c];
+      if (nk == 0 || nph == 0) continue;
+      int nt = (nk + kTile - 1) / kTile;
+      for (int rb = 0; rb < nt; ++rb)
+        for (int sb = rb; sb < nt; ++sb) wl.ph.push_back(make_int4(0, c, rb, sb));
+      long tot = (long)nk * 2 * nph;
+      for (long s = 0; s < tot; s += kChunk) wl.pandya.push_back(make_int4(0, c, (int)s, 0));
+    }
+    for (auto &pr : pairs1) wl.one.push_back(make_int4(0, pr.x, pr.y, 0));
+  }
+  d_pphh = dupload(*this, wl.pphh, err); d_ph = dupload(*this, wl.ph, err);
+  d_c122 = dupload(*this, wl.c122, err); d_pandya = dupload(*this, wl.pandya, err);
+  d_invp = dupload(*this, wl.invp, err); d_one = dupload(*this, wl.one, err);
+  d_mirror = dupload(*this, wl.mirror, err); d_gen = dupload(*this, wl.gen, err);
+  if (!err.empty()) return err;
+
+  void *p = nullptr;
+  if (hipMalloc(&p, sizeof(BasisDev)) != hipSuccess) return "hipMalloc failed for BasisDev";
+  allocs.push_back(p);
+  d = (BasisDev *)p;
+  if (hipMemcpy(d, &B, sizeof(BasisDev), hipMemcpyHostToDevice) != hipSuccess) return "hipMemcpy BasisDev failed";
+  return "";
+}
+
+void DeviceBasis::release() {
+  for (void *p : allocs) (void)hipFree(p);
+  allocs.clear();
+  d = nullptr;
+}
+
+}  // namespace imsrg

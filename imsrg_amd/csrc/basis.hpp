@@ -1,0 +1,61 @@
+// basis.hpp -- host-side model space (orbits, kets, channels) and its device
+// tables.  Restates the construction of ModelSpace::Init / SetupKets
+// (src/ModelSpace.cc:353-417, 1120-1201) and TwoBodyChannel::Initialize /
+// CheckChannel_ket (src/TwoBodyChannel.cc:19-188).
+#pragma once
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace imsrg {
+
+struct HostChannel {
+  int J = 0, par = 0, tz = 0;
+  std::vector<int> kets;  // global ket indices, ascending
+};
+
+struct HostBasis {
+  int emax = 0, norb = 0, nkets = 0, nch = 0, jmax = 0;
+  std::vector<int> n, l, j2, tz2, cvq;
+  std::vector<double> occ;
+  std::vector<int> kp, kq;
+  std::vector<HostChannel> ch, cc;
+  std::vector<long> ch_off;
+  long tbsize = 0;
+  std::vector<int> kmap, cmap;
+  std::vector<int> holes;
+  std::vector<std::vector<int>> ob;  // one-body block of each orbit
+
+  // returns empty string on success, else an error message
+  std::string build(int norb, const int *n, const int *l, const int *j2, const int *tz2, const double *occ,
+                    const int *cvq, int emax);
+  static void ho_orbits(int emax, double A, double Z, std::vector<int> &n, std::vector<int> &l, std::vector<int> &j2,
+                        std::vector<int> &tz2, std::vector<double> &occ, std::vector<int> &cvq);
+};
+
+// Work-item lists (int4: pair, a, b, c) for every kernel of one commutator.
+struct WorkLists {
+  std::vector<int4> pphh;    // (pair, ch, rb, sb), rb <= sb
+  std::vector<int4> ph;      // (pair, cc, rb, sb), rb <= sb
+  std::vector<int4> c122;    // (pair, ch, col, 0)
+  std::vector<int4> pandya;  // (pair, cc, start, 0)
+  std::vector<int4> invp;    // (pair, ch, ibra, kstart)
+  std::vector<int4> one;     // (pair, i, j, 0)
+  std::vector<int4> mirror;  // (pair, ch, rb, sb), rb < sb  and diagonal tiles
+  std::vector<int4> gen;     // (pair, ch, 0, 0) channels with work for the generator
+};
+
+// Device copy of the tables; owns the allocations.
+struct DeviceBasis {
+  BasisDev h{};             // host copy holding device pointers
+  BasisDev *d = nullptr;    // device copy of h
+  std::vector<void *> allocs;
+  WorkLists wl;             // host lists with pair = 0
+  int4 *d_pphh = nullptr, *d_ph = nullptr, *d_c122 = nullptr, *d_pandya = nullptr, *d_invp = nullptr,
+       *d_one = nullptr, *d_mirror = nullptr, *d_gen = nullptr;
+  std::string upload(const HostBasis &hb, int emax6j);  // builds tables, lists, 6j table
+  void release();
+};
+
+}  // namespace imsrg

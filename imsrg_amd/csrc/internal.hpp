@@ -1,0 +1,91 @@
+// internal.hpp -- device-side tables, operator addressing and work items shared
+// by the host driver (api.cpp, basis.cpp) and the HIP kernels (*.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace imsrg {
+
+constexpr int kTile = 64;         // SYR2K output tile edge
+constexpr int kTileK = 16;        // k-depth of one LDS stage
+constexpr int kChunk = 256;       // elements per work item of the gather kernels
+constexpr int kMax122 = 32;       // max one-body partners of a ket in comm122
+
+// Term bits (same values as IMSRG_COMM* in include/imsrg_hip.h)
+constexpr unsigned M110 = 1u << 0, M220 = 1u << 1, M111 = 1u << 2, M121 = 1u << 3, M221 = 1u << 4,
+                   M122 = 1u << 5, MPPHH = 1u << 6, MPH = 1u << 7;
+
+// All basis tables of one model space, device pointers.  Index conventions:
+//  * global ket k of orbits p<=q: k = p(2 norb - 1 - p)/2 + q   (ModelSpace::Index2)
+//  * standard channel ch = 6J + 2(Tz+1) + parity, J = 0..jmax  (GetTwoBodyChannelIndex)
+//  * cross-coupled channel uses the same formula with Tz = |tz_p - tz_q|/2
+//  * kmap[k*(jmax+1)+J]: local index of ket k in the standard channel of
+//    angular momentum J with the ket's parity and Tz, or -1
+//  * cmap[k*(jmax+1)+J]: same for the cross-coupled channel
+struct BasisDev {
+  int norb, nkets, nch, jmax, d1, d2;
+  int nholes, n1pairs;
+  long tbsize;   // doubles in the two-body arena
+  long one_off;  // offset of the one-body block inside an operator buffer
+  long two_off;  // offset of the two-body arena inside an operator buffer
+  long opsize;   // doubles per operator buffer
+  int maxn, maxcc;
+  const int *j2, *l, *tz2, *cvq;
+  const double *occ;
+  const int *holes;
+  const int *obstart, *oblist;  // one-body block of each orbit
+  const int *kp, *kq;
+  const int *kmap, *cmap;
+  // standard channels
+  const long *ch_off;
+  const int *ch_n, *ch_kstart, *ch_kets;
+  const double *w220;  // per channel ket: (1-n_p)(1-n_q) for hh/ph kets, 1 for pp kets
+  const int *hh_start, *hh_loc;
+  const double *hh_occ;
+  const int *k_start, *k_loc;  // pp/hh product list with combined weight (w_pp - w_hh)
+  const double *k_w;
+  const int *kpp_start, *kpp_loc;  // M_pp list (weights w_pp)
+  const double *kpp_w;
+  const int *khh_start, *khh_loc;  // M_hh list (weights n_p n_q)
+  const double *khh_w;
+  const int *p122_start;  // per (channel, ket) (index ch_kstart[ch]+i): entries [start, start+1)
+  const int4 *p122_e;     // (ind2, a, o, diag) ; coefficient X1(a, o) * f
+  const double *p122_f;
+  // cross-coupled channels
+  const int *cc_n, *cc_nph, *cc_kstart, *cc_kets, *cc_phstart, *cc_ph;
+  const signed char *cc_phi;  // (-1)^{(j_p+j_q)} per CC ket
+  const long *cc_xoff, *cc_zoff;
+  long xsize, zsize;
+  const int2 *pairs1;   // one-body pairs (i<=j, same (l,j2,tz2) block)
+  const double *w1norm; // norb*norb one-body norm weights
+  const double *sixj;   // dense 6j table, layout of SixJCache_112112
+  // generator lists: per channel, core-core kets and (non-core)-(non-core) kets
+  const int *gcore_start, *gcore_loc, *gnc_start, *gnc_loc;
+  const long *twoch;    // per arena element? (unused)
+};
+
+// Everything one commutator needs: operand buffers, workspaces, flags.
+struct PairPtrs {
+  const BasisDev *basis;  // device pointer
+  const double *X, *Y;    // operator buffers (zero | one | two)
+  double *Z;
+  double *Xt, *Yt, *Zbar; // Pandya workspaces
+  int hX, hY, hZ;
+  unsigned mask;
+};
+
+struct PairArg {
+  PairPtrs p0;           // used when tab == nullptr
+  const PairPtrs *tab;   // batched: device table
+  __device__ __forceinline__ PairPtrs get(int i) const { return tab ? tab[i] : p0; }
+};
+
+__host__ __device__ inline int ket_index(int norb, int p, int q) { return p * (2 * norb - 1 - p) / 2 + q; }
+__host__ __device__ inline int sgn_parity(int x) { return (x & 1) ? -1 : 1; }
+__host__ __device__ inline long sixj_index(int d1, int d2, int jj1, int jj2, int J3, int jj4, int jj5, int J6) {
+  return (((((long)((jj1 - 1) >> 1) * d1 + ((jj2 - 1) >> 1)) * d2 + J3) * d1 + ((jj4 - 1) >> 1)) * d1 +
+          ((jj5 - 1) >> 1)) * d2 + J6;
+}
+
+}  // namespace imsrg

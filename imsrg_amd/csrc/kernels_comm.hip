@@ -1,0 +1,625 @@
+// kernels_comm.hip -- HIP kernels of the scalar IMSRG(2) commutator Z = [X, Y]
+// on CDNA4 (gfx950).  Every kernel reads work items (int4: pair, a, b, c)
+// so that one launch can cover all channels of one or many operator pairs.
+//
+// Two-body accumulation scheme: for Hermitian / anti-Hermitian Z every term
+// adds only to the upper triangle (row <= col) of each channel block; the
+// mirror kernel writes the lower triangle as hZ * upper at the end.  This is
+// exact because each term of the reference produces a contribution whose
+// (s,r) element is hZ times its (r,s) element (src/Commutator.cc:929, 1185-1194,
+// 1611-1612).
+#include <hip/hip_runtime.h>
+
+#include "internal.hpp"
+#include "launch.hpp"
+
+namespace imsrg {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+#define SQRT2_D 1.4142135623730951
+
+__device__ __forceinline__ double X1at(const double *one, int norb, int p, int q) { return one[(long)q * norb + p]; }
+
+// Normalized scalar TBME <ab|V|cd>_J with the swapped-ket phases of
+// TwoBodyME::GetTBME_norm (src/TwoBodyME.cc:128-147).
+__device__ __forceinline__ double tbme_norm(const BasisDev &B, const double *two, int J, int a, int b, int c, int d) {
+  int pab = (B.l[a] + B.l[b]) & 1, pcd = (B.l[c] + B.l[d]) & 1;
+  if (pab != pcd) return 0.0;
+  int tab = (B.tz2[a] + B.tz2[b]) / 2, tcd = (B.tz2[c] + B.tz2[d]) / 2;
+  if (tab != tcd || J > B.jmax) return 0.0;
+  int k1 = ket_index(B.norb, min(a, b), max(a, b));
+  int k2 = ket_index(B.norb, min(c, d), max(c, d));
+  int i = B.kmap[(long)k1 * (B.jmax + 1) + J];
+  int k = B.kmap[(long)k2 * (B.jmax + 1) + J];
+  if (i < 0 || k < 0) return 0.0;
+  int ch = 6 * J + 2 * (tab + 1) + pab;
+  double ph = 1.0;
+  if (a > b) ph *= sgn_parity((B.j2[a] + B.j2[b]) / 2 + 1 + J);
+  if (c > d) ph *= sgn_parity((B.j2[c] + B.j2[d]) / 2 + 1 + J);
+  return ph * two[B.ch_off[ch] + (long)k * B.ch_n[ch] + i];
+}
+
+// TwoBodyME::GetTBMEmonopole for a scalar operator (src/TwoBodyME.cc:615-682)
+__device__ double tbme_monopole(const BasisDev &B, const double *two, int a, int b, int c, int d) {
+  double nrm = (a == b ? SQRT2_D : 1.0) * (c == d ? SQRT2_D : 1.0);
+  int jmin = abs(B.j2[a] - B.j2[b]) / 2, jmax = (B.j2[a] + B.j2[b]) / 2;
+  double mon = 0.0;
+  for (int J = jmin; J <= jmax; ++J) mon += (2 * J + 1) * tbme_norm(B, two, J, a, b, c, d);
+  mon /= (B.j2[a] + 1) * (B.j2[b] + 1);
+  return nrm * mon;
+}
+
+template <int NV>
+__device__ __forceinline__ void block_reduce(double (&v)[NV]) {
+  __shared__ double red[NV][4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double x = v[q];
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+    if (lane == 0) red[q][w] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] = (red[q][0] + red[q][1]) + (red[q][2] + red[q][3]);
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// comm110ss (src/Commutator.cc:564-581) and comm220ss (601-631) -> Z0; also
+// clears Z's one-body block.  One block per pair.
+__global__ __launch_bounds__(256) void k_zerobody(PairArg pa) {
+  PairPtrs P = pa.get(blockIdx.x);
+  const BasisDev &B = *P.basis;
+  const int norb = B.norb;
+  const double *X1 = P.X + B.one_off, *Y1 = P.Y + B.one_off;
+  double *Z1 = P.Z + B.one_off;
+  for (int e = threadIdx.x; e < norb * norb; e += blockDim.x) Z1[e] = 0.0;
+  double acc[2] = {0.0, 0.0};
+  const bool skip = (P.hX == P.hY);
+  if (!skip && (P.mask & M110)) {
+    for (int t = threadIdx.x; t < B.nholes * norb; t += blockDim.x) {
+      int a = B.holes[t / norb], k = t % norb;
+      double xy = X1at(X1, norb, a, k) * X1at(Y1, norb, k, a);
+      double yx = X1at(Y1, norb, a, k) * X1at(X1, norb, k, a);
+      acc[0] += (B.j2[a] + 1) * B.occ[a] * (xy - yx);
+    }
+  }
+  if (!skip && (P.mask & M220)) {
+    const double *X2 = P.X + B.two_off, *Y2 = P.Y + B.two_off;
+    for (int ch = 0; ch < B.nch; ++ch) {
+      int h0 = B.hh_start[ch], nhh = B.hh_start[ch + 1] - h0;
+      if (nhh == 0) continue;
+      int n = B.ch_n[ch];
+      long off = B.ch_off[ch];
+      const double *w = B.w220 + B.ch_kstart[ch];
+      double f = 2.0 * (2 * (ch / 6) + 1);
+      for (int t = threadIdx.x; t < nhh * n; t += blockDim.x) {
+        int ih = t / n, m = t % n;
+        int h = B.hh_loc[h0 + ih];
+        acc[1] += f * B.hh_occ[h0 + ih] * (X2[off + (long)m * n + h] * (Y2[off + (long)h * n + m] * w[m]));
+      }
+    }
+  }
+  block_reduce<2>(acc);
+  if (threadIdx.x == 0) P.Z[0] = (0.0 + acc[0]) + acc[1];
+}
+
+// ---------------------------------------------------------------------------
+// One-body terms for one (i, j) pair of the same (l, j2, tz2) block, i <= j:
+// comm111ss (src/Commutator.cc:644-649), comm121ss (670-728) and the one-body
+// part of comm222_pp_hh_221ss (1246-1294).  M_pp / M_hh elements are formed
+// directly as folded dot products, M(r,s) + hZ M(s,r) with M = X W Y.
+__global__ __launch_bounds__(256) void k_onebody(PairArg pa, const int4 *items) {
+  const int4 it = items[blockIdx.x];
+  PairPtrs P = pa.get(it.x);
+  const BasisDev &B = *P.basis;
+  const int norb = B.norb, i = it.y, j = it.z;
+  const double *X1 = P.X + B.one_off, *Y1 = P.Y + B.one_off;
+  const double *X2 = P.X + B.two_off, *Y2 = P.Y + B.two_off;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  if (P.mask & M111) {
+    for (int k = threadIdx.x; k < norb; k += blockDim.x) {
+      acc[0] += X1at(X1, norb, i, k) * X1at(Y1, norb, k, j);
+      acc[1] += X1at(Y1, norb, i, k) * X1at(X1, norb, k, j);
+    }
+  }
+  if (P.mask & M121) {
+    for (int ia = threadIdx.x; ia < B.nholes; ia += blockDim.x) {
+      int a = B.holes[ia];
+      for (int e = B.obstart[a]; e < B.obstart[a + 1]; ++e) {
+        int b = B.oblist[e];
+        double nanb = B.occ[a] * (1.0 - B.occ[b]);
+        if (fabs(nanb) < 1e-6) continue;
+        double ybiaj = tbme_monopole(B, Y2, b, i, a, j);
+        double yaibj = tbme_monopole(B, Y2, a, i, b, j);
+        double xbiaj = tbme_monopole(B, X2, b, i, a, j);
+        double xaibj = tbme_monopole(B, X2, a, i, b, j);
+        acc[2] += (B.j2[b] + 1) * nanb * X1at(X1, norb, a, b) * ybiaj;
+        acc[2] -= (B.j2[a] + 1) * nanb * X1at(X1, norb, b, a) * yaibj;
+        acc[2] -= (B.j2[b] + 1) * nanb * X1at(Y1, norb, a, b) * xbiaj;
+        acc[2] += (B.j2[a] + 1) * nanb * X1at(Y1, norb, b, a) * xaibj;
+      }
+    }
+  }
+  if (P.mask & M221) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int nJ = B.jmax + 1;
+    const double hX = P.hX, hZ = P.hZ;
+    for (int t = w; t < norb * nJ; t += 4) {
+      int c = t / nJ, J = t % nJ;
+      int Jmin = max(abs(B.j2[c] - B.j2[i]), abs(B.j2[c] - B.j2[j])) / 2;
+      int Jmax = (B.j2[c] + min(B.j2[i], B.j2[j])) / 2;
+      if (J < Jmin || J > Jmax) continue;
+      double nc = B.occ[c], nbc = 1.0 - nc;
+      int kci = ket_index(norb, min(c, i), max(c, i)), kcj = ket_index(norb, min(c, j), max(c, j));
+      int r = B.kmap[(long)kci * nJ + J], s = B.kmap[(long)kcj * nJ + J];
+      if (r < 0 || s < 0) continue;
+      int par = (B.l[c] + B.l[i]) & 1, tz = (B.tz2[c] + B.tz2[i]) / 2;
+      int ch = 6 * J + 2 * (tz + 1) + par;
+      int n = B.ch_n[ch];
+      long off = B.ch_off[ch];
+      double fac = (c == i ? SQRT2_D : 1.0) * (c == j ? SQRT2_D : 1.0);
+      if (c > i) fac *= sgn_parity((B.j2[c] + B.j2[i]) / 2 + 1 + J);
+      if (c > j) fac *= sgn_parity((B.j2[c] + B.j2[j]) / 2 + 1 + J);
+      const double *xr = X2 + off + (long)r * n, *xs = X2 + off + (long)s * n;
+      const double *yr = Y2 + off + (long)r * n, *ys = Y2 + off + (long)s * n;
+      if (fabs(nc) > 1e-9) {
+        int k0 = B.kpp_start[ch], nk = B.kpp_start[ch + 1] - k0;
+        double m = 0.0;
+        for (int q = lane; q < nk; q += 64) {
+          int k = B.kpp_loc[k0 + q];
+          m += B.kpp_w[k0 + q] * (xr[k] * ys[k] + hZ * (xs[k] * yr[k]));
+        }
+        acc[3] += (2 * J + 1) * nc * fac * hX * m;
+      }
+      if (fabs(nbc) > 1e-9) {
+        int k0 = B.khh_start[ch], nk = B.khh_start[ch + 1] - k0;
+        double m = 0.0;
+        for (int q = lane; q < nk; q += 64) {
+          int k = B.khh_loc[k0 + q];
+          m += B.khh_w[k0 + q] * (xr[k] * ys[k] + hZ * (xs[k] * yr[k]));
+        }
+        acc[3] += (2 * J + 1) * nbc * fac * hX * m;
+      }
+    }
+  }
+  block_reduce<4>(acc);
+  if (threadIdx.x == 0) {
+    double z = 0.0;
+    if (P.mask & M111) z += acc[0] - acc[1];
+    z += acc[2];
+    z += acc[3] / (B.j2[i] + 1.0);
+    double *Z1 = P.Z + B.one_off;
+    Z1[(long)j * norb + i] = z;
+    if (i != j) Z1[(long)i * norb + j] = P.hZ * z;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// SYR2K tile on MFMA f64 16x16x4:
+//   D(r,s) = alpha * cph(s) * sum_k w_k [ A(r,ka_k) B(s,kb_k) + h B(r,kb_k) A(s,ka_k) ],  r <= s
+// with A, B column-major n x * matrices (A(r,k) = A[k*n + r]).
+// MODE 0: pp/hh term of comm222_pp_hh_221ss (src/Commutator.cc:1171-1194, 1237-1238):
+//         A = X2, B = Y2 of one channel, k over pp/hh kets with weight w_pp - w_hh,
+//         alpha = hY, h = hZ.  (X W Y + hZ (X W Y)^T written with Y^T = hY Y.)
+// MODE 1: head of Zbar in comm222_phss (src/Commutator.cc:1789-1799): A = Xbar,
+//         B = Ybar^T, h = +1 (Z Hermitian) or -1.
+// MODE 2: tail of Zbar (src/Commutator.cc:1788-1789, 1805): B columns permuted
+//         by ab <-> ba with weights hY*phi(ph ket), cph(s) = phi(s), h = +1.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_syr2k(PairArg pa, const int4 *items, int accumulate) {
+  constexpr int LD = kTile + 2;
+  __shared__ double As[kTileK][LD];
+  __shared__ double Bs[kTileK][LD];
+  const int4 it = items[blockIdx.x];
+  PairPtrs P = pa.get(it.x);
+  const BasisDev &B = *P.basis;
+  const double *Am, *Bm;
+  int n, nk, nph = 0;
+  const int *kl = nullptr;
+  const double *kw = nullptr;
+  const int *phl = nullptr;
+  const signed char *phi = nullptr;
+  double alpha, h;
+  double *D;
+  if (MODE == 0) {
+    const int ch = it.y;
+    n = B.ch_n[ch];
+    const long off = B.ch_off[ch];
+    Am = P.X + B.two_off + off;
+    Bm = P.Y + B.two_off + off;
+    const int k0 = B.k_start[ch];
+    nk = B.k_start[ch + 1] - k0;
+    kl = B.k_loc + k0;
+    kw = B.k_w + k0;
+    alpha = (double)P.hY;
+    h = (double)P.hZ;
+    D = P.Z + B.two_off + off;
+  } else {
+    const int cc = it.y;
+    n = B.cc_n[cc];
+    nph = B.cc_nph[cc];
+    nk = 2 * nph;
+    const long xo = B.cc_xoff[cc];
+    Am = P.Xt + xo;
+    Bm = P.Yt + xo;
+    D = P.Zbar + B.cc_zoff[cc] + (MODE == 2 ? (long)n * n : 0);
+    alpha = 1.0;
+    h = (MODE == 1) ? (P.hZ == 1 ? 1.0 : -1.0) : 1.0;
+    phi = B.cc_phi + B.cc_kstart[cc];
+    phl = B.cc_ph + B.cc_phstart[cc];
+  }
+  const int R0 = it.z * kTile, S0 = it.w * kTile;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int wr = (wv & 1) * 32, wc = (wv >> 1) * 32;
+  dbl4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
+
+  const int tk = threadIdx.x >> 4, tr = (threadIdx.x & 15) * 4;
+  const int K2 = 2 * nk;
+  for (int k0 = 0; k0 < K2; k0 += kTileK) {
+    double av[4] = {0.0, 0.0, 0.0, 0.0}, bv[4] = {0.0, 0.0, 0.0, 0.0};
+    const int kk = k0 + tk;
+    if (kk < K2) {
+      const int k = kk < nk ? kk : kk - nk;
+      int ca, cb;
+      double wk;
+      if (MODE == 0) {
+        ca = cb = kl[k];
+        wk = kw[k];
+      } else if (MODE == 1) {
+        ca = cb = k;
+        wk = 1.0;
+      } else {
+        ca = k;
+        cb = k < nph ? k + nph : k - nph;
+        wk = (double)P.hY * (double)phi[phl[k < nph ? k : k - nph]];
+      }
+      const double *sa, *sb;
+      double fa;
+      if (kk < nk) {
+        sa = Am + (long)ca * n;
+        sb = Bm + (long)cb * n;
+        fa = wk;
+      } else {
+        sa = Bm + (long)cb * n;
+        sb = Am + (long)ca * n;
+        fa = wk * h;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = R0 + tr + q, c = S0 + tr + q;
+        av[q] = (r < n) ? sa[r] * fa : 0.0;
+        bv[q] = (c < n) ? sb[c] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      As[tk][tr + q] = av[q];
+      Bs[tk][tr + q] = bv[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < kTileK / 4; ++ks) {
+      const int kr = ks * 4 + (lane >> 4);
+      double a0 = As[kr][wr + (lane & 15)], a1 = As[kr][wr + 16 + (lane & 15)];
+      double b0 = Bs[kr][wc + (lane & 15)], b1 = Bs[kr][wc + 16 + (lane & 15)];
+      // D^T blocks; f64 16x16x4 result layout: lane l, reg q holds Dm[4q + l/16][l%16],
+      // i.e. D row (lane&15), column 4q + (lane>>4)
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a0, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a1, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a1, acc[1][1], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = R0 + wr + mi * 16 + (lane & 15);
+        const int col = S0 + wc + ni * 16 + 4 * q + (lane >> 4);
+        if (row < n && col < n && row <= col) {
+          double v = alpha * acc[mi][ni][q];
+          if (MODE == 2) v *= (double)phi[col];
+          double *dp = D + (long)col * n + row;
+          *dp = accumulate ? *dp + v : v;
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------
+// comm122ss (src/Commutator.cc:837-932): column s of one channel.
+//   Z(r,s) -= W(r,s) + hZ W(s,r),  r <= s,
+//   W(:,q) = sum_m [Y2(:,ind_m) x_m - X2(:,ind_m) y_m]  (x2 if ket q has p == q),
+// with W(s,r) formed from column s of X2 / Y2 (staged in LDS) via Y2(s,i) = hY Y2(i,s).
+__global__ __launch_bounds__(256) void k_c122(PairArg pa, const int4 *items) {
+  extern __shared__ double lds[];
+  __shared__ int sind[kMax122];
+  __shared__ double scx[kMax122], scy[kMax122];
+  const int4 it = items[blockIdx.x];
+  PairPtrs P = pa.get(it.x);
+  const BasisDev &B = *P.basis;
+  const int ch = it.y, s = it.z;
+  const int n = B.ch_n[ch], norb = B.norb;
+  const long off = B.ch_off[ch];
+  const double *X2 = P.X + B.two_off + off, *Y2 = P.Y + B.two_off + off;
+  const double *X1 = P.X + B.one_off, *Y1 = P.Y + B.one_off;
+  double *Z2 = P.Z + B.two_off + off;
+  double *colX = lds, *colY = lds + n;
+  for (int q = threadIdx.x; q < n; q += blockDim.x) {
+    colX[q] = X2[(long)s * n + q];
+    colY[q] = Y2[(long)s * n + q];
+  }
+  const int ks = B.ch_kstart[ch];
+  const int e0 = B.p122_start[ks + s], ne = B.p122_start[ks + s + 1] - e0;
+  if (threadIdx.x < ne) {
+    int4 e = B.p122_e[e0 + threadIdx.x];
+    double f = B.p122_f[e0 + threadIdx.x];
+    sind[threadIdx.x] = e.x;
+    scx[threadIdx.x] = X1at(X1, norb, e.y, e.z) * f;
+    scy[threadIdx.x] = X1at(Y1, norb, e.y, e.z) * f;
+  }
+  __syncthreads();
+  const int kets = B.ch_kets[ks + s];
+  const double ds = (B.kp[kets] == B.kq[kets]) ? 2.0 : 1.0;
+  const double hX = P.hX, hY = P.hY, hZ = P.hZ;
+  for (int r = threadIdx.x; r <= s; r += blockDim.x) {
+    double wrs = 0.0;
+    for (int m = 0; m < ne; ++m) {
+      const long c = (long)sind[m] * n + r;
+      wrs += Y2[c] * scx[m] - X2[c] * scy[m];
+    }
+    wrs *= ds;
+    double wsr = 0.0;
+    const int f0 = B.p122_start[ks + r], f1 = B.p122_start[ks + r + 1];
+    for (int e = f0; e < f1; ++e) {
+      int4 pe = B.p122_e[e];
+      double f = B.p122_f[e];
+      double cx = X1at(X1, norb, pe.y, pe.z) * f, cy = X1at(Y1, norb, pe.y, pe.z) * f;
+      wsr += hY * colY[pe.x] * cx - hX * colX[pe.x] * cy;
+    }
+    const int ketr = B.ch_kets[ks + r];
+    if (B.kp[ketr] == B.kq[ketr]) wsr *= 2.0;
+    Z2[(long)s * n + r] -= wrs + hZ * wsr;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// DoPandyaTransformation_SingleChannel_XandY (src/Commutator.cc:1392-1485):
+// Xbar (nk x 2nph) and Ybar^T (nk x 2nph), one element each per thread.
+__global__ __launch_bounds__(256) void k_pandya(PairArg pa, const int4 *items) {
+  const int4 it = items[blockIdx.x];
+  PairPtrs P = pa.get(it.x);
+  const BasisDev &B = *P.basis;
+  const int cc = it.y;
+  const int nk = B.cc_n[cc], nph = B.cc_nph[cc];
+  const long tot = (long)nk * 2 * nph;
+  const long e = (long)it.z + threadIdx.x;
+  if (e >= tot) return;
+  const int mp = (int)(e / nk), ik = (int)(e % nk);
+  const int abc = mp / nph, ibra = mp - abc * nph;
+  const int locb = B.cc_ph[B.cc_phstart[cc] + ibra];
+  const int kb = B.cc_kets[B.cc_kstart[cc] + locb];
+  const int p = B.kp[kb], q = B.kq[kb];
+  const int a = abc ? q : p, b = abc ? p : q;
+  const int kk = B.cc_kets[B.cc_kstart[cc] + ik];
+  const int c = B.kp[kk], d = B.kq[kk];
+  const int Jcc = cc / 6;
+  const double *X2 = P.X + B.two_off, *Y2 = P.Y + B.two_off;
+  double xbar = 0.0, ybar = 0.0, normf = 1.0;
+  const double nanb = B.occ[a] - B.occ[b];
+  if (B.tz2[a] + B.tz2[d] == B.tz2[b] + B.tz2[c]) {
+    if (a == d) normf *= SQRT2_D;
+    if (c == b) normf *= SQRT2_D;
+    const int ja = B.j2[a], jb = B.j2[b], jc = B.j2[c], jd = B.j2[d];
+    int jmin = max(abs(ja - jd), abs(jc - jb)) / 2;
+    const int jmax = min(ja + jd, jc + jb) / 2;
+    int dJ = 1;
+    if (a == d || c == b) {
+      dJ = 2;
+      jmin += jmin & 1;
+    }
+    // index work common to X and Y (GetTBME_J_norm_twoOps, src/TwoBodyME.cc:368-409)
+    const int pcb = (B.l[c] + B.l[b]) & 1, pad = (B.l[a] + B.l[d]) & 1;
+    const int tcb = (B.tz2[c] + B.tz2[b]) / 2;
+    const int k1 = ket_index(B.norb, min(c, b), max(c, b)), k2 = ket_index(B.norb, min(a, d), max(a, d));
+    const double hY = P.hY;
+    if (pcb == pad) {
+      for (int J = jmin; J <= jmax; J += dJ) {
+        const double sj = B.sixj[sixj_index(B.d1, B.d2, ja, jb, Jcc, jc, jd, J)];
+        if (fabs(sj) < 1e-8) continue;
+        const int i1 = B.kmap[(long)k1 * (B.jmax + 1) + J], i2 = B.kmap[(long)k2 * (B.jmax + 1) + J];
+        if (i1 < 0 || i2 < 0) continue;
+        const int ch = 6 * J + 2 * (tcb + 1) + pcb;
+        double ph = 1.0;
+        if (c > b) ph *= sgn_parity((jc + jb) / 2 + 1 + J);
+        if (a > d) ph *= sgn_parity((ja + jd) / 2 + 1 + J);
+        const long idx = B.ch_off[ch] + (long)i2 * B.ch_n[ch] + i1;
+        const double w = (2 * J + 1) * sj;
+        xbar -= w * (ph * X2[idx]);
+        ybar -= w * (ph * Y2[idx]) * hY;
+      }
+    }
+  }
+  const long o = B.cc_xoff[cc] + (long)mp * nk + ik;
+  P.Xt[o] = xbar * normf * nanb;
+  P.Yt[o] = ybar * normf;
+}
+
+// Zbar element (row, col) of CC channel cc; only the upper triangles of the
+// head (Hermitian-folded) and tail (phase-folded) blocks are stored.
+__device__ __forceinline__ double zbar_get(const PairPtrs &P, const BasisDev &B, int cc, int row, int col) {
+  const int nk = B.cc_n[cc];
+  const double *Zb = P.Zbar + B.cc_zoff[cc];
+  if (col < nk) {
+    if (row <= col) return Zb[(long)col * nk + row];
+    return (P.hZ == 1 ? 1.0 : -1.0) * Zb[(long)row * nk + col];
+  }
+  const int s = col - nk;
+  const double *T = Zb + (long)nk * nk;
+  if (row <= s) return T[(long)s * nk + row];
+  const signed char *phi = B.cc_phi + B.cc_kstart[cc];
+  return (double)(phi[row] * phi[s]) * T[(long)row * nk + s];
+}
+
+// AddInversePandyaTransformation (src/Commutator.cc:1501-1616), upper triangle.
+__global__ __launch_bounds__(256) void k_invpandya(PairArg pa, const int4 *items) {
+  const int4 it = items[blockIdx.x];
+  PairPtrs P = pa.get(it.x);
+  const BasisDev &B = *P.basis;
+  const int ch = it.y, ibra = it.z;
+  const int n = B.ch_n[ch], J = ch / 6;
+  const int iket = it.w + threadIdx.x;
+  const int ketmin = (P.hZ == 1) ? ibra : ibra + 1;
+  if (iket < ketmin || iket >= n) return;
+  const int ks = B.ch_kstart[ch];
+  const int kb = B.ch_kets[ks + ibra], kk = B.ch_kets[ks + iket];
+  const int i = B.kp[kb], j = B.kq[kb], k = B.kp[kk], l = B.kq[kk];
+  const int jji = B.j2[i], jjj = B.j2[j], jjk = B.j2[k], jjl = B.j2[l];
+  const int nJ = B.jmax + 1;
+  double commij = 0.0, commji = 0.0;
+  {
+    const int pcc = (B.l[i] + B.l[l]) & 1, tcc = abs(B.tz2[i] - B.tz2[l]) / 2;
+    const int Jpmin = max(abs(jji - jjl), abs(jjk - jjj)) / 2, Jpmax = min(jji + jjl, jjk + jjj) / 2;
+    const int kil = ket_index(B.norb, min(i, l), max(i, l)), kjk = ket_index(B.norb, min(j, k), max(j, k));
+    for (int Jp = Jpmin; Jp <= Jpmax; ++Jp) {
+      const double sj = B.sixj[sixj_index(B.d1, B.d2, jji, jjj, J, jjk, jjl, Jp)];
+      if (fabs(sj) < 1e-8) continue;
+      const int cc = 6 * Jp + 2 * (tcc + 1) + pcc;
+      const int nkc = B.cc_n[cc];
+      const int il = B.cmap[(long)kil * nJ + Jp] + (i > l ? nkc : 0);
+      const int kj = B.cmap[(long)kjk * nJ + Jp] + (k > j ? nkc : 0);
+      commij -= (2 * Jp + 1) * sj * zbar_get(P, B, cc, il, kj);
+    }
+  }
+  if (k == l) {
+    commji = commij;
+  } else if (i == j) {
+    commji = sgn_parity((jji + jjj + jjk + jjl) / 2) * commij;
+  } else {
+    const int pcc = (B.l[i] + B.l[k]) & 1, tcc = abs(B.tz2[i] - B.tz2[k]) / 2;
+    const int Jpmin = max(abs(jjj - jjl), abs(jjk - jji)) / 2, Jpmax = min(jjj + jjl, jjk + jji) / 2;
+    const int kik = ket_index(B.norb, min(i, k), max(i, k)), klj = ket_index(B.norb, min(l, j), max(l, j));
+    for (int Jp = Jpmin; Jp <= Jpmax; ++Jp) {
+      const double sj = B.sixj[sixj_index(B.d1, B.d2, jjj, jji, J, jjk, jjl, Jp)];
+      if (fabs(sj) < 1e-8) continue;
+      const int cc = 6 * Jp + 2 * (tcc + 1) + pcc;
+      const int nkc = B.cc_n[cc];
+      const int ik = B.cmap[(long)kik * nJ + Jp] + (i > k ? nkc : 0);
+      const int lj = B.cmap[(long)klj * nJ + Jp] + (l > j ? nkc : 0);
+      commji -= (2 * Jp + 1) * sj * zbar_get(P, B, cc, ik, lj);
+    }
+  }
+  const int dbra = (i == j), dket = (k == l);
+  const double nrm = (dbra == dket) ? 1.0 + dbra : SQRT2_D;
+  const double z = -(commij - sgn_parity((jjk + jjl) / 2 - J) * commji) / nrm;
+  double *Z2 = P.Z + B.two_off + B.ch_off[ch];
+  Z2[(long)iket * n + ibra] += z;
+}
+
+// Lower triangle <- hZ * upper triangle; anti-Hermitian diagonal <- 0.
+__global__ __launch_bounds__(256) void k_mirror(PairArg pa, const int4 *items) {
+  __shared__ double t[kTile][kTile + 1];
+  const int4 it = items[blockIdx.x];
+  PairPtrs P = pa.get(it.x);
+  const BasisDev &B = *P.basis;
+  const int ch = it.y, n = B.ch_n[ch];
+  const int R0 = it.z * kTile, S0 = it.w * kTile;
+  double *Z2 = P.Z + B.two_off + B.ch_off[ch];
+  for (int e = threadIdx.x; e < kTile * kTile; e += blockDim.x) {
+    const int rr = e & (kTile - 1), cc = e / kTile;
+    const int r = R0 + rr, c = S0 + cc;
+    if (r < n && c < n) t[cc][rr] = Z2[(long)c * n + r];
+  }
+  __syncthreads();
+  const double hZ = P.hZ;
+  for (int e = threadIdx.x; e < kTile * kTile; e += blockDim.x) {
+    const int rr = e & (kTile - 1), cc = e / kTile;
+    const int row = S0 + rr, col = R0 + cc;
+    if (row >= n || col >= n) continue;
+    if (row > col)
+      Z2[(long)col * n + row] = hZ * t[rr][cc];
+    else if (row == col && P.hZ == -1)
+      Z2[(long)col * n + row] = 0.0;
+  }
+}
+
+// Z2 <- 0 over whole tiles (used when the pp/hh term is switched off)
+__global__ __launch_bounds__(256) void k_zero_two(PairArg pa, const int4 *items) {
+  const int4 it = items[blockIdx.x];
+  PairPtrs P = pa.get(it.x);
+  const BasisDev &B = *P.basis;
+  const int ch = it.y, n = B.ch_n[ch];
+  const int R0 = it.z * kTile, S0 = it.w * kTile;
+  double *Z2 = P.Z + B.two_off + B.ch_off[ch];
+  for (int e = threadIdx.x; e < kTile * kTile; e += blockDim.x) {
+    const int r = R0 + (e & (kTile - 1)), c = S0 + e / kTile;
+    if (r < n && c < n) {
+      Z2[(long)c * n + r] = 0.0;
+      Z2[(long)r * n + c] = 0.0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+hipError_t launch_zerobody(const PairArg &pa, int npairs, hipStream_t s) {
+  if (npairs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zerobody, dim3(npairs), dim3(256), 0, s, pa);
+  return hipGetLastError();
+}
+hipError_t launch_onebody(const PairArg &pa, const int4 *items, int nitems, hipStream_t s) {
+  if (nitems <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_onebody, dim3(nitems), dim3(256), 0, s, pa, items);
+  return hipGetLastError();
+}
+hipError_t launch_pphh(const PairArg &pa, const int4 *items, int nitems, int precision, hipStream_t s) {
+  (void)precision;
+  if (nitems <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_syr2k<0>, dim3(nitems), dim3(256), 0, s, pa, items, 0);
+  return hipGetLastError();
+}
+hipError_t launch_ph_gemm(const PairArg &pa, const int4 *items, int nitems, int precision, hipStream_t s) {
+  (void)precision;
+  if (nitems <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_syr2k<1>, dim3(nitems), dim3(256), 0, s, pa, items, 0);
+  hipLaunchKernelGGL(k_syr2k<2>, dim3(nitems), dim3(256), 0, s, pa, items, 0);
+  return hipGetLastError();
+}
+hipError_t launch_c122(const PairArg &pa, const int4 *items, int nitems, int maxn, hipStream_t s) {
+  if (nitems <= 0) return hipSuccess;
+  size_t lds = (size_t)2 * maxn * sizeof(double);
+  hipLaunchKernelGGL(k_c122, dim3(nitems), dim3(256), lds, s, pa, items);
+  return hipGetLastError();
+}
+hipError_t launch_pandya(const PairArg &pa, const int4 *items, int nitems, hipStream_t s) {
+  if (nitems <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pandya, dim3(nitems), dim3(kChunk), 0, s, pa, items);
+  return hipGetLastError();
+}
+hipError_t launch_invpandya(const PairArg &pa, const int4 *items, int nitems, hipStream_t s) {
+  if (nitems <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_invpandya, dim3(nitems), dim3(kChunk), 0, s, pa, items);
+  return hipGetLastError();
+}
+hipError_t launch_mirror(const PairArg &pa, const int4 *items, int nitems, hipStream_t s) {
+  if (nitems <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mirror, dim3(nitems), dim3(256), 0, s, pa, items);
+  return hipGetLastError();
+}
+hipError_t launch_zero_two(const PairArg &pa, const int4 *items, int nitems, hipStream_t s) {
+  if (nitems <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero_two, dim3(nitems), dim3(256), 0, s, pa, items);
+  return hipGetLastError();
+}
+
+}  // namespace imsrg

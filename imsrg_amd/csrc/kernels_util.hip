@@ -1,0 +1,383 @@
+// kernels_util.hip -- 6j table, operator arithmetic and norms, synthetic
+// operators, White generator, MFMA layout probe.
+#include <hip/hip_runtime.h>
+
+#include "internal.hpp"
+#include "launch.hpp"
+
+namespace imsrg {
+
+// ---------------------------------------------------------------------------
+// double-double arithmetic (Dekker / Knuth two-sum, fma two-product)
+struct dd {
+  double hi, lo;
+};
+__device__ __forceinline__ dd dd_quick(double a, double b) {
+  double s = a + b;
+  return {s, b - (s - a)};
+}
+__device__ __forceinline__ dd dd_two_sum(double a, double b) {
+  double s = a + b;
+  double bb = s - a;
+  return {s, (a - (s - bb)) + (b - bb)};
+}
+__device__ __forceinline__ dd dd_add(dd x, dd y) {
+  dd s = dd_two_sum(x.hi, y.hi);
+  dd t = dd_two_sum(x.lo, y.lo);
+  s.lo += t.hi;
+  s = dd_quick(s.hi, s.lo);
+  s.lo += t.lo;
+  return dd_quick(s.hi, s.lo);
+}
+__device__ __forceinline__ dd dd_neg(dd x) { return {-x.hi, -x.lo}; }
+__device__ __forceinline__ dd dd_mul(dd x, dd y) {
+  double p = x.hi * y.hi;
+  double e = __fma_rn(x.hi, y.hi, -p);
+  e += x.hi * y.lo + x.lo * y.hi;
+  return dd_quick(p, e);
+}
+__device__ __forceinline__ dd dd_mul_d(dd x, double y) {
+  double p = x.hi * y;
+  double e = __fma_rn(x.hi, y, -p);
+  e += x.lo * y;
+  return dd_quick(p, e);
+}
+__device__ __forceinline__ dd dd_div(dd x, dd y) {
+  double q1 = x.hi / y.hi;
+  dd r = dd_add(x, dd_neg(dd_mul_d(y, q1)));
+  double q2 = r.hi / y.hi;
+  r = dd_add(r, dd_neg(dd_mul_d(y, q2)));
+  double q3 = r.hi / y.hi;
+  return dd_add(dd_quick(q1, q2), dd{q3, 0.0});
+}
+__device__ __forceinline__ dd dd_sqrt(dd a) {
+  if (a.hi <= 0.0) return {0.0, 0.0};
+  double x = sqrt(a.hi);
+  dd xx = dd_mul(dd{x, 0.0}, dd{x, 0.0});
+  dd r = dd_add(a, dd_neg(xx));
+  return dd_two_sum(x, r.hi / (2.0 * x));
+}
+
+constexpr int kNFact = 256;
+
+__global__ void k_factorials(dd *f) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    dd v = {1.0, 0.0};
+    f[0] = v;
+    for (int i = 1; i < kNFact; ++i) {
+      v = dd_mul_d(v, (double)i);
+      f[i] = v;
+    }
+  }
+}
+
+__device__ __forceinline__ bool tri_ok(int a, int b, int c) {
+  return ((a + b + c) & 1) == 0 && c >= abs(a - b) && c <= a + b;
+}
+
+// Wigner 6j {a b c; d e f} (doubled arguments), Racah single-sum formula
+// (Edmonds eq. 6.3.7) in double-double; replaces gsl_sf_coupling_6j
+// (src/AngMom.cc:94-97).
+__device__ double sixj_dd(const dd *F, int a, int b, int c, int d, int e, int f) {
+  if (!tri_ok(a, b, c) || !tri_ok(a, e, f) || !tri_ok(d, b, f) || !tri_ok(d, e, c)) return 0.0;
+  const int tri[4][3] = {{a, b, c}, {a, e, f}, {d, b, f}, {d, e, c}};
+  dd num = {1.0, 0.0}, den = {1.0, 0.0};
+  for (int k = 0; k < 4; ++k) {
+    int x = tri[k][0], y = tri[k][1], z = tri[k][2];
+    num = dd_mul(num, F[(x + y - z) / 2]);
+    num = dd_mul(num, F[(x - y + z) / 2]);
+    num = dd_mul(num, F[(-x + y + z) / 2]);
+    den = dd_mul(den, F[(x + y + z) / 2 + 1]);
+  }
+  dd pref = dd_sqrt(dd_div(num, den));
+  const int al[4] = {(a + b + c) / 2, (a + e + f) / 2, (d + b + f) / 2, (d + e + c) / 2};
+  const int be[3] = {(a + b + d + e) / 2, (b + c + e + f) / 2, (a + c + d + f) / 2};
+  int tmin = max(max(al[0], al[1]), max(al[2], al[3]));
+  int tmax = min(min(be[0], be[1]), be[2]);
+  dd sum = {0.0, 0.0};
+  for (int t = tmin; t <= tmax; ++t) {
+    dd dn = F[t - al[0]];
+    for (int k = 1; k < 4; ++k) dn = dd_mul(dn, F[t - al[k]]);
+    for (int k = 0; k < 3; ++k) dn = dd_mul(dn, F[be[k] - t]);
+    dd term = dd_div(F[t + 1], dn);
+    if (t & 1) term = dd_neg(term);
+    sum = dd_add(sum, term);
+  }
+  dd r = dd_mul(pref, sum);
+  return r.hi + r.lo;
+}
+
+__global__ __launch_bounds__(256) void k_sixj_table(const dd *F, double *table, int d1, int d2, long total) {
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    long r = idx;
+    const int J6 = r % d2; r /= d2;
+    const int i5 = r % d1; r /= d1;
+    const int i4 = r % d1; r /= d1;
+    const int J3 = r % d2; r /= d2;
+    const int i2 = r % d1; r /= d1;
+    const int i1 = (int)r;
+    table[idx] = sixj_dd(F, 2 * i1 + 1, 2 * i2 + 1, 2 * J3, 2 * i4 + 1, 2 * i5 + 1, 2 * J6);
+  }
+}
+
+hipError_t launch_sixj_table(double *table, int d1, int d2, hipStream_t s) {
+  if (4 * (d1 - 1) + 4 > kNFact - 4) return hipErrorInvalidValue;
+  dd *F = nullptr;
+  hipError_t e = hipMalloc(&F, sizeof(dd) * kNFact);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_factorials, dim3(1), dim3(64), 0, s, F);
+  long total = (long)d1 * d1 * d2 * d1 * d1 * d2;
+  int blocks = (int)std::min<long>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_sixj_table, dim3(blocks), dim3(256), 0, s, F, table, d1, d2, total);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  e = hipStreamSynchronize(s);
+  (void)hipFree(F);
+  return e;
+}
+
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_axpy(double *y, double a, const double *x, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double t = a * x[i];  // (a*x) formed first, then added (Operator operator* then +=)
+    y[i] += t;
+  }
+}
+__global__ __launch_bounds__(256) void k_scale(double *y, double a, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] *= a;
+}
+static int grid_for(long n) { return (int)std::max<long>(1, std::min<long>((n + 255) / 256, 8192)); }
+hipError_t launch_axpy(double *y, double a, const double *x, long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_axpy, dim3(grid_for(n)), dim3(256), 0, s, y, a, x, n);
+  return hipGetLastError();
+}
+hipError_t launch_scale(double *y, double a, long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_scale, dim3(grid_for(n)), dim3(256), 0, s, y, a, n);
+  return hipGetLastError();
+}
+
+// Norm partial sums.  Block b < npart-1 covers a slice of the two-body arena
+// (weights (2J+1)^2 of the element's channel); the last block does the
+// one-body part with weights (2 j_p + 1).
+__global__ __launch_bounds__(256) void k_norm_partial(const BasisDev *bd, const double *op, double *part, int nblk2) {
+  const BasisDev &B = *bd;
+  __shared__ double red[4];
+  double acc = 0.0;
+  if ((int)blockIdx.x == nblk2) {
+    const double *one = op + B.one_off;
+    for (long e = threadIdx.x; e < (long)B.norb * B.norb; e += blockDim.x) acc += one[e] * one[e] * B.w1norm[e];
+  } else {
+    const double *two = op + B.two_off;
+    // channels are visited in order; block b takes channel-aligned strided elements
+    for (int ch = 0; ch < B.nch; ++ch) {
+      const long nn = (long)B.ch_n[ch] * B.ch_n[ch];
+      if (nn == 0) continue;
+      const double w = (double)((2 * (ch / 6) + 1) * (2 * (ch / 6) + 1));
+      const double *m = two + B.ch_off[ch];
+      double s = 0.0;
+      for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < nn; e += (long)nblk2 * blockDim.x)
+        s += m[e] * m[e];
+      acc += w * s;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__global__ void k_norm_final(const double *part, int nblk2, double *out) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nblk2; i += blockDim.x) acc += part[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = part[nblk2];
+    out[1] = red[0];
+  }
+}
+
+hipError_t launch_norm(const BasisDev *bd, const BasisDev &bh, const double *op, double *partials, int npart,
+                       double *out, hipStream_t s) {
+  (void)bh;
+  int nblk2 = npart - 1;
+  hipLaunchKernelGGL(k_norm_partial, dim3(npart), dim3(256), 0, s, bd, op, partials, nblk2);
+  hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(256), 0, s, partials, nblk2, out);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Counter-based N(0,1): SplitMix64 of (seed, index) -> two uniforms -> Box-Muller.
+__device__ __forceinline__ unsigned long long splitmix64(unsigned long long x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+__device__ __forceinline__ double normal_at(unsigned long long seed, unsigned long long idx) {
+  unsigned long long a = splitmix64(seed ^ splitmix64(2 * idx + 1));
+  unsigned long long b = splitmix64(a ^ 0x5851F42D4C957F2Dull);
+  double u1 = ((a >> 11) + 1.0) * (1.0 / 9007199254740993.0);  // (0, 1]
+  double u2 = (b >> 11) * (1.0 / 9007199254740992.0);          // [0, 1)
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+__global__ __launch_bounds__(256) void k_fill_random(const BasisDev *bd, double *op, int herm, unsigned long long seed) {
+  const BasisDev &B = *bd;
+  const double inv_sqrt2 = 0.7071067811865476;
+  const long n2 = (long)B.norb * B.norb;
+  const long total = n2 + B.tbsize;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    if (e < n2) {
+      const int p = e % B.norb, q = e / B.norb;
+      double v = 0.0;
+      if (B.l[p] == B.l[q] && B.j2[p] == B.j2[q] && B.tz2[p] == B.tz2[q]) {
+        // the value of the pair is formed once from the canonical (lo, hi) order,
+        // so that (p,q) and (q,p) are exactly herm-related
+        const int lo = min(p, q), hi = max(p, q);
+        const int m = B.obstart[p + 1] - B.obstart[p];
+        double a = normal_at(seed, (unsigned long long)((long)hi * B.norb + lo));
+        double b = normal_at(seed, (unsigned long long)((long)lo * B.norb + hi));
+        double vc = (lo == hi) ? (herm == 1 ? 2.0 * a : 0.0) : (a + herm * b);
+        vc = vc * inv_sqrt2 / sqrt((double)m);
+        v = (p <= q) ? vc : herm * vc;
+      }
+      op[B.one_off + e] = v;
+    } else {
+      const long t = e - n2;
+      int clo = 0, chi = B.nch - 1;  // last channel starting at or before t (non-empty)
+      while (clo < chi) {
+        int mid = (clo + chi + 1) / 2;
+        if (B.ch_off[mid] <= t) clo = mid; else chi = mid - 1;
+      }
+      const int ch = clo;
+      const int n = B.ch_n[ch];
+      const long loc = t - B.ch_off[ch];
+      const int r = loc % n, c = loc / n;
+      const int lo = min(r, c), hi = max(r, c);
+      double a = normal_at(seed, (unsigned long long)(n2 + B.ch_off[ch] + (long)hi * n + lo));
+      double b = normal_at(seed, (unsigned long long)(n2 + B.ch_off[ch] + (long)lo * n + hi));
+      double vc = (lo == hi) ? (herm == 1 ? 2.0 * a : 0.0) : (a + herm * b);
+      vc = vc * inv_sqrt2 / sqrt((double)n);
+      op[B.two_off + t] = (r <= c) ? vc : herm * vc;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) op[0] = 0.0;
+}
+
+hipError_t launch_fill_random(const BasisDev *bd, const BasisDev &bh, double *op, int herm, unsigned long long seed,
+                              hipStream_t s) {
+  long total = (long)bh.norb * bh.norb + bh.tbsize;
+  hipLaunchKernelGGL(k_fill_random, dim3(grid_for(total)), dim3(256), 0, s, bd, op, herm, seed);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// White generator, Epstein-Nesbet denominators (src/Generator.cc:110-188, 236-291).
+__device__ double tbme_norm_u(const BasisDev &B, const double *two, int J, int a, int b, int c, int d) {
+  int pab = (B.l[a] + B.l[b]) & 1, pcd = (B.l[c] + B.l[d]) & 1;
+  if (pab != pcd) return 0.0;
+  int tab = (B.tz2[a] + B.tz2[b]) / 2, tcd = (B.tz2[c] + B.tz2[d]) / 2;
+  if (tab != tcd || J > B.jmax) return 0.0;
+  int k1 = ket_index(B.norb, min(a, b), max(a, b)), k2 = ket_index(B.norb, min(c, d), max(c, d));
+  int i = B.kmap[(long)k1 * (B.jmax + 1) + J], k = B.kmap[(long)k2 * (B.jmax + 1) + J];
+  if (i < 0 || k < 0) return 0.0;
+  int ch = 6 * J + 2 * (tab + 1) + pab;
+  double ph = 1.0;
+  if (a > b) ph *= sgn_parity((B.j2[a] + B.j2[b]) / 2 + 1 + J);
+  if (c > d) ph *= sgn_parity((B.j2[c] + B.j2[d]) / 2 + 1 + J);
+  return ph * two[B.ch_off[ch] + (long)k * B.ch_n[ch] + i];
+}
+__device__ double monopole_u(const BasisDev &B, const double *two, int a, int b, int c, int d) {
+  double nrm = (a == b ? 1.4142135623730951 : 1.0) * (c == d ? 1.4142135623730951 : 1.0);
+  int jmin = abs(B.j2[a] - B.j2[b]) / 2, jmax = (B.j2[a] + B.j2[b]) / 2;
+  double mon = 0.0;
+  for (int J = jmin; J <= jmax; ++J) mon += (2 * J + 1) * tbme_norm_u(B, two, J, a, b, c, d);
+  mon /= (B.j2[a] + 1) * (B.j2[b] + 1);
+  return nrm * mon;
+}
+
+// one-body part: one thread per (a core, i non-core)
+__global__ __launch_bounds__(256) void k_white1(const BasisDev *bd, const double *H, double *Eta) {
+  const BasisDev &B = *bd;
+  const int norb = B.norb;
+  const double *H1 = H + B.one_off, *H2 = H + B.two_off;
+  double *E1 = Eta + B.one_off;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < (long)norb * norb;
+       e += (long)gridDim.x * blockDim.x) {
+    const int i = e % norb, a = e / norb;
+    if (B.cvq[a] != 0 || B.cvq[i] == 0) continue;
+    double d = H1[(long)i * norb + i] - H1[(long)a * norb + a];
+    d += (B.occ[i] - B.occ[a]) * monopole_u(B, H2, i, a, i, a);
+    if (fabs(d) < 1e-6) d = 1e-6;
+    const double v = H1[(long)a * norb + i] / d;
+    E1[(long)a * norb + i] = v;
+    E1[(long)i * norb + a] = -v;
+  }
+}
+
+// two-body part: block per channel, threads over (core ket, non-core ket)
+__global__ __launch_bounds__(256) void k_white2(const BasisDev *bd, const double *H, double *Eta,
+                                                const int4 *items) {
+  const BasisDev &B = *bd;
+  const int ch = items[blockIdx.x].y;
+  const int n = B.ch_n[ch];
+  const long off = B.ch_off[ch];
+  const int norb = B.norb;
+  const double *H1 = H + B.one_off, *H2 = H + B.two_off;
+  double *E2 = Eta + B.two_off + off;
+  const int c0 = B.gcore_start[ch], nc = B.gcore_start[ch + 1] - c0;
+  const int q0 = B.gnc_start[ch], nq = B.gnc_start[ch + 1] - q0;
+  const int ks = B.ch_kstart[ch];
+  for (int t = threadIdx.x; t < nc * nq; t += blockDim.x) {
+    const int iket = B.gcore_loc[c0 + t / nq], ibra = B.gnc_loc[q0 + t % nq];
+    const int kb = B.ch_kets[ks + ibra], kk = B.ch_kets[ks + iket];
+    const int i = B.kp[kb], j = B.kq[kb], k = B.kp[kk], l = B.kq[kk];
+    const double ni = B.occ[i], nj = B.occ[j], nk = B.occ[k], nl = B.occ[l];
+    double d = H1[(long)i * norb + i] + H1[(long)j * norb + j] - H1[(long)k * norb + k] - H1[(long)l * norb + l];
+    d += (1 - ni - nj) * monopole_u(B, H2, i, j, i, j);
+    d -= (1 - nk - nl) * monopole_u(B, H2, k, l, k, l);
+    d += (ni - nk) * monopole_u(B, H2, i, k, i, k);
+    d += (ni - nl) * monopole_u(B, H2, i, l, i, l);
+    d += (nj - nk) * monopole_u(B, H2, j, k, j, k);
+    d += (nj - nl) * monopole_u(B, H2, j, l, j, l);
+    if (fabs(d) < 1e-6) d = 1e-6;
+    const double v = H2[off + (long)iket * n + ibra] / d;
+    E2[(long)iket * n + ibra] = v;
+    E2[(long)ibra * n + iket] = -v;
+  }
+}
+
+hipError_t launch_white(const BasisDev *bd, const BasisDev &bh, const double *H, double *Eta, const int4 *gen_items,
+                        int ngen, hipStream_t s) {
+  long n2 = (long)bh.norb * bh.norb;
+  hipLaunchKernelGGL(k_white1, dim3(grid_for(n2)), dim3(256), 0, s, bd, H, Eta);
+  if (ngen > 0) hipLaunchKernelGGL(k_white2, dim3(ngen), dim3(256), 0, s, bd, H, Eta, gen_items);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+typedef double dbl4u __attribute__((ext_vector_type(4)));
+// One wave: out[0..255] = D with A-operand = lane id, B-operand = 1;
+//           out[256..511] = D with A-operand = 1, B-operand = lane id.
+__global__ void k_mfma_probe(double *out) {
+  const int lane = threadIdx.x;
+  dbl4u z = {0.0, 0.0, 0.0, 0.0};
+  dbl4u r1 = __builtin_amdgcn_mfma_f64_16x16x4f64((double)lane, 1.0, z, 0, 0, 0);
+  dbl4u r2 = __builtin_amdgcn_mfma_f64_16x16x4f64(1.0, (double)lane, z, 0, 0, 0);
+  for (int q = 0; q < 4; ++q) {
+    out[lane * 4 + q] = r1[q];
+    out[256 + lane * 4 + q] = r2[q];
+  }
+}
+hipError_t launch_mfma_probe(double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, s, out);
+  return hipGetLastError();
+}
+
+}  // namespace imsrg

@@ -1,0 +1,37 @@
+// launch.hpp -- host-callable launchers of the HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "internal.hpp"
+
+namespace imsrg {
+
+// dense 6j table {jj1 jj2 J3; jj4 jj5 J6} (double-double Racah), layout of SixJCache_112112
+hipError_t launch_sixj_table(double *table, int d1, int d2, hipStream_t s);
+
+// commutator phases; items are device arrays of int4 work items
+hipError_t launch_zerobody(const PairArg &pa, int npairs, hipStream_t s);
+hipError_t launch_onebody(const PairArg &pa, const int4 *items, int nitems, hipStream_t s);
+hipError_t launch_pphh(const PairArg &pa, const int4 *items, int nitems, int precision, hipStream_t s);
+hipError_t launch_c122(const PairArg &pa, const int4 *items, int nitems, int maxn, hipStream_t s);
+hipError_t launch_pandya(const PairArg &pa, const int4 *items, int nitems, hipStream_t s);
+hipError_t launch_ph_gemm(const PairArg &pa, const int4 *items, int nitems, int precision, hipStream_t s);
+hipError_t launch_invpandya(const PairArg &pa, const int4 *items, int nitems, hipStream_t s);
+hipError_t launch_mirror(const PairArg &pa, const int4 *items, int nitems, hipStream_t s);
+hipError_t launch_zero_two(const PairArg &pa, const int4 *items, int nitems, hipStream_t s);
+
+// operator arithmetic
+hipError_t launch_axpy(double *y, double a, const double *x, long n, hipStream_t s);
+hipError_t launch_scale(double *y, double a, long n, hipStream_t s);
+// partial sums of squares; out[0] = one-body, out[1] = two-body (weighted)
+hipError_t launch_norm(const BasisDev *bd, const BasisDev &bh, const double *op, double *partials, int npart,
+                       double *out, hipStream_t s);
+hipError_t launch_fill_random(const BasisDev *bd, const BasisDev &bh, double *op, int herm,
+                              unsigned long long seed, hipStream_t s);
+hipError_t launch_white(const BasisDev *bd, const BasisDev &bh, const double *H, double *Eta, const int4 *gen_items,
+                        int ngen, hipStream_t s);
+
+// self-test of the MFMA f64 lane layout: out[256] receives D of a 16x16x4 product
+hipError_t launch_mfma_probe(double *out, hipStream_t s);
+
+}  // namespace imsrg

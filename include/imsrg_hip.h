@@ -1,0 +1,184 @@
+/*
+ * imsrg_hip.h -- C ABI of the MI355X-native scalar IMSRG(2) commutator and
+ * Magnus/BCH path (libimsrg_hip.so).
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, never
+ * throws or exits, and returns an int status (IMSRG_OK or a negative code;
+ * imsrg_hip_last_error() gives the message).  Each declaration names the
+ * reference interface it replaces (paths relative to the IMSRG++ source).
+ *
+ * Operator layout (identical to the reference's Operator storage):
+ *   zero_body  double                        Operator::ZeroBody   (src/Operator.hh:49)
+ *   one_body   norb*norb, column-major       Operator::OneBody    (arma::mat)
+ *   two_body   one n_ch x n_ch column-major  TwoBodyME::MatEl[{ch,ch}] (src/TwoBodyME.hh:55)
+ *              block per channel ch = 6J + 2(Tz+1) + parity, J = 0..max j2,
+ *              kets p<=q in ascending ket index, normalized TBMEs.
+ * The "packed" form concatenates the channel blocks in ascending ch.
+ *
+ * Supported operators: scalar (rank_J = rank_T = 0), parity-conserving,
+ * not reduced, particle rank <= 2, Hermitian (+1) or anti-Hermitian (-1)
+ * and exactly (anti)symmetric as declared.  Anything else returns
+ * IMSRG_E_UNSUPPORTED so that the caller keeps the reference path
+ * (src/Commutator.cc:182-276 dispatch).
+ */
+#ifndef IMSRG_HIP_H
+#define IMSRG_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define IMSRG_OK 0
+#define IMSRG_E_INVALID (-1)
+#define IMSRG_E_UNSUPPORTED (-2)
+#define IMSRG_E_HIP (-3)
+#define IMSRG_E_NOMEM (-4)
+#define IMSRG_E_NONFINITE (-5)
+#define IMSRG_E_NOT_CONVERGED (-6)
+
+/* term switches, src/Commutator.cc:36-46 (comm_term_on) */
+#define IMSRG_COMM110 (1u << 0)
+#define IMSRG_COMM220 (1u << 1)
+#define IMSRG_COMM111 (1u << 2)
+#define IMSRG_COMM121 (1u << 3)
+#define IMSRG_COMM221 (1u << 4)
+#define IMSRG_COMM122 (1u << 5)
+#define IMSRG_COMM222_PPHH (1u << 6)
+#define IMSRG_COMM222_PH (1u << 7)
+#define IMSRG_ALL_TERMS 0xFFu
+
+/* arithmetic of the channel GEMMs */
+#define IMSRG_PREC_FP64 0  /* every product and sum in FP64 */
+#define IMSRG_PREC_MIXED 1 /* pp/hh and ph GEMMs in FP32 MFMA (split operands), rest FP64 */
+
+typedef struct imsrg_ctx imsrg_ctx; /* basis tables, 6j table, stream, workspaces */
+typedef struct imsrg_dop imsrg_dop; /* device-resident operator */
+
+/* Single-particle basis in the reference's orbit order (src/ModelSpace.cc:376-399). */
+typedef struct {
+  int norb;
+  const int *n, *l, *j2, *tz2; /* j2, tz2 doubled */
+  const double *occ;           /* occupation in [0,1] */
+  const int *cvq;              /* 0 core, 1 valence, 2 q-space */
+  int emax;                    /* 6j table covers j <= emax + 1/2 (src/ModelSpace.cc:176) */
+} imsrg_basis_desc;
+
+/* Zero-copy view of a host Operator (src/Operator.hh:49-71). */
+typedef struct {
+  double zero_body;
+  double *one_body;         /* norb*norb column-major */
+  double *const *two_body;  /* [nch] block pointers; entries may be NULL where n_ch = 0 */
+  int herm;                 /* +1 Hermitian, -1 anti-Hermitian, 0 non-Hermitian */
+  int particle_rank;        /* 1 or 2 */
+  int rank_J, rank_T, parity, reduced;
+} imsrg_op_view;
+
+typedef struct {
+  unsigned term_mask; /* IMSRG_COMM* bits; 0 means all */
+  int precision;      /* IMSRG_PREC_* */
+  int use_imsrg3;     /* must be 0 (IMSRG(3) terms, src/Commutator.cc:337-420, are not provided) */
+} imsrg_comm_opts;
+
+typedef struct {
+  int mode;            /* 0: reference rule eps = ||Op|| e^{-2||Omega||} thr / (2||Omega||) growing by (i+1)
+                          (src/BCH.cc:132,189-191); 1: stop when ||nested|| < thr */
+  double transform_thr;/* bch_transform_threshold, default 1e-9 (src/BCH.cc:16) */
+  double product_thr;  /* bch_product_threshold, default 1e-4 (src/BCH.cc:17) */
+  int max_terms;       /* default 40 (src/BCH.cc:91) */
+  int precision;       /* IMSRG_PREC_* for the nested commutators */
+} imsrg_bch_opts;
+
+/* ---- context ---------------------------------------------------------- */
+/* Builds kets, (J,parity,Tz) and cross-coupled channels (ModelSpace::SetupKets,
+ * src/ModelSpace.cc:1120-1201; TwoBodyChannel.cc:19-188) and the FP64 6j table
+ * on the device (replaces SixJCache_112112, src/AngMomCache.cc:8-54). */
+int imsrg_hip_ctx_create(const imsrg_basis_desc *basis, int device, imsrg_ctx **out);
+/* Convenience: HO basis to emax with the single-reference filling of
+ * ModelSpace::GetOrbitsAZ (src/ModelSpace.cc:611-645), core = reference. */
+int imsrg_hip_ctx_create_ho(int emax, double A, double Z, int device, imsrg_ctx **out);
+int imsrg_hip_ctx_destroy(imsrg_ctx *ctx);
+const char *imsrg_hip_last_error(const imsrg_ctx *ctx);
+int imsrg_hip_ctx_info(const imsrg_ctx *ctx, int *norb, int *nch, long *two_body_size);
+int imsrg_hip_ctx_orbit(const imsrg_ctx *ctx, int i, int *n, int *l, int *j2, int *tz2, double *occ, int *cvq);
+int imsrg_hip_ctx_channel(const imsrg_ctx *ctx, int ch, int *J, int *parity, int *Tz, int *nkets, long *offset);
+int imsrg_hip_ctx_channel_ket(const imsrg_ctx *ctx, int ch, int i, int *p, int *q);
+int imsrg_hip_ctx_sixj(imsrg_ctx *ctx, int jj1, int jj2, int J3, int jj4, int jj5, int J6, double *value);
+int imsrg_hip_synchronize(imsrg_ctx *ctx);
+
+/* ---- host-buffer drop-ins (PCIe-inclusive) ----------------------------- */
+/* Operator Commutator::CommutatorScalarScalar(const Operator& X, const Operator& Y)
+ * (src/Commutator.hh:58, src/Commutator.cc:280-428).  Z->herm is written with the
+ * rule of src/Commutator.cc:292-297; Z's buffers are overwritten. */
+int imsrg_hip_commutator_ss(imsrg_ctx *ctx, const imsrg_op_view *X, const imsrg_op_view *Y, imsrg_op_view *Z,
+                            const imsrg_comm_opts *opts);
+/* Operator BCH::BCH_Transform(const Operator& Op, const Operator& Omega) (src/BCH.hh:23, src/BCH.cc:78-201) */
+int imsrg_hip_bch_transform(imsrg_ctx *ctx, const imsrg_op_view *Op, const imsrg_op_view *Omega, imsrg_op_view *Out,
+                            const imsrg_bch_opts *opts, int *n_commutators);
+/* Operator BCH::BCH_Product(Operator& X, Operator& Y) (src/BCH.hh:22, src/BCH.cc:267-350) */
+int imsrg_hip_bch_product(imsrg_ctx *ctx, const imsrg_op_view *X, const imsrg_op_view *Y, imsrg_op_view *Out,
+                          const imsrg_bch_opts *opts, int *n_commutators);
+
+/* ---- device-resident operators ----------------------------------------- */
+int imsrg_hip_op_create(imsrg_ctx *ctx, int herm, imsrg_dop **out);
+int imsrg_hip_op_destroy(imsrg_dop *op);
+int imsrg_hip_op_get_herm(const imsrg_dop *op, int *herm);
+int imsrg_hip_op_set_herm(imsrg_dop *op, int herm);
+int imsrg_hip_op_upload(imsrg_dop *op, const imsrg_op_view *view);
+int imsrg_hip_op_download(imsrg_dop *op, imsrg_op_view *view);
+/* packed layout: two = concatenated channel blocks, length two_body_size */
+int imsrg_hip_op_upload_packed(imsrg_dop *op, double zero, const double *one, const double *two, int herm);
+int imsrg_hip_op_download_packed(imsrg_dop *op, double *zero, double *one, double *two, int *herm);
+/* Seeded synthetic operator: one-body (R +/- R^T)/sqrt(2m) per (l,j,tz) block,
+ * two-body (R +/- R^T)/sqrt(2n) per channel, R ~ N(0,1) from SplitMix64 +
+ * Box-Muller; zero-body 0. */
+int imsrg_hip_op_fill_random(imsrg_dop *op, int herm, unsigned long long seed);
+/* Operator arithmetic used by BCH (src/Operator.cc:132-246) */
+int imsrg_hip_op_copy(imsrg_dop *dst, const imsrg_dop *src);
+int imsrg_hip_op_scale(imsrg_dop *op, double f);
+int imsrg_hip_op_axpy(imsrg_dop *y, double a, const imsrg_dop *x); /* y += a*x */
+int imsrg_hip_op_zero(imsrg_dop *op);
+/* Operator::Norm / OneBodyNorm / TwoBodyNorm (src/Operator.cc:1610-1646, src/TwoBodyME.cc:703-724) */
+int imsrg_hip_op_norm(imsrg_dop *op, double *norm1, double *norm2, double *norm);
+
+/* device-resident versions of the drop-ins (no PCIe traffic) */
+int imsrg_hip_commutator_ss_dev(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y, imsrg_dop *Z,
+                                const imsrg_comm_opts *opts);
+int imsrg_hip_bch_transform_dev(imsrg_ctx *ctx, const imsrg_dop *Op, const imsrg_dop *Omega, imsrg_dop *Out,
+                                const imsrg_bch_opts *opts, int *n_commutators);
+int imsrg_hip_bch_product_dev(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y, imsrg_dop *Out,
+                              const imsrg_bch_opts *opts, int *n_commutators);
+
+/* Many independent commutators Z_i = [X_i, Y_i] (contexts may differ) in one
+ * grouped launch sequence.  All contexts must live on the same device. */
+int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctxs, const imsrg_dop *const *X,
+                                    const imsrg_dop *const *Y, imsrg_dop *const *Z, const imsrg_comm_opts *opts);
+
+/* White generator with Epstein-Nesbet denominators, single reference
+ * (Generator::ConstructGenerator_SingleRef, src/Generator.cc:110-291). */
+int imsrg_hip_white_generator_dev(imsrg_ctx *ctx, const imsrg_dop *H, imsrg_dop *Eta);
+
+/* Magnus-Euler flow (IMSRGSolver::Solve_magnus_euler, src/IMSRGSolver.cc:236-321)
+ * with fixed ds, Omega splitting at omega_norm_max (NewOmega, :44-84).
+ * Hs receives H(s).  e0/eta_norm/ncomm (length max_steps, may be NULL) receive
+ * E0(s), ||eta|| and commutators per step; *nsteps the number of steps. */
+int imsrg_hip_magnus_flow(imsrg_ctx *ctx, const imsrg_dop *H0, imsrg_dop *Hs, double ds, double smax,
+                          double eta_criterion, double omega_norm_max, const imsrg_bch_opts *opts, int max_steps,
+                          double *e0, double *eta_norm, int *ncomm, int *nsteps);
+
+/* Per-phase timing of the last commutator on ctx (microseconds, HIP events):
+ * [0] zero-body, [1] one-body, [2] comm122, [3] pp/hh GEMM, [4] Pandya gather,
+ * [5] ph GEMM, [6] inverse Pandya, [7] mirror.  Enabled by imsrg_hip_set_profiling. */
+int imsrg_hip_set_profiling(imsrg_ctx *ctx, int on);
+int imsrg_hip_phase_times(imsrg_ctx *ctx, double *us, int n);
+
+/* Diagnostic with no counterpart in the reference: out512[0..255] = D of one
+ * v_mfma_f64_16x16x4f64 with A = lane id, B = 1; out512[256..511] with A = 1,
+ * B = lane id (lane l, register q at index 4l+q).  Used by the tests to pin the
+ * MFMA lane layout the GEMM kernels assume. */
+int imsrg_hip_selftest_mfma(imsrg_ctx *ctx, double *out512);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

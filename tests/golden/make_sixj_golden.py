@@ -37,6 +37,13 @@ def main():
         a, b, c, d, e, f = args
         if tri(a, b, c) and tri(a, e, f) and tri(d, b, f) and tri(d, e, c):
             cases.append(args)
+    # emax <= 6 table range: half-integers <= 13/2, integers <= 13
+    while len(cases) < 660:
+        a, b, d, e = (rng.randrange(1, 14, 2) for _ in range(4))
+        c = 2 * rng.randrange(0, 14)
+        f = 2 * rng.randrange(0, 14)
+        if tri(a, b, c) and tri(a, e, f) and tri(d, b, f) and tri(d, e, c):
+            cases.append((a, b, c, d, e, f))
     out = []
     for a, b, c, d, e, f in cases:
         v = wigner_6j(Rational(a, 2), Rational(b, 2), Rational(c, 2), Rational(d, 2), Rational(e, 2), Rational(f, 2))
