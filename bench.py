@@ -77,6 +77,12 @@ def main():
     Y = ia.DeviceOperator(ms, 1).fill_random(1, 10002)
     Z = ia.DeviceOperator(ms, 1)
 
+    # plain device-to-device copy of one operator buffer (known size), used to
+    # calibrate the FETCH_SIZE / WRITE_SIZE counters of profiling runs
+    Zc = ia.DeviceOperator(ms, 1)
+    Zc.copy_from(Y)
+    ms.synchronize()
+
     def run(steps, precision):
         for _ in range(steps):
             ia.Commutator.CommutatorScalarScalar(X, Y, precision=precision, out=Z)

@@ -31,7 +31,8 @@ struct BatchCache {
   size_t off[8] = {0};
   int cnt[8] = {0};
   double *work = nullptr;
-  int maxn = 0;
+  int maxn = 0, maxnch = 0;
+  long maxn122 = 0;
   void release() {
     if (d_tab) (void)hipFree(d_tab);
     if (d_items) (void)hipFree(d_items);
@@ -47,6 +48,8 @@ struct imsrg_ctx {
   HostBasis hb;
   DeviceBasis db;
   double *Xt = nullptr, *Yt = nullptr, *Zbar = nullptr;
+  double *scratch = nullptr;
+  double2 *c122 = nullptr;
   double *partials = nullptr;
   double *norm_dev = nullptr;
   double *norm_host = nullptr;  // pinned
@@ -93,6 +96,10 @@ static int ctx_init(imsrg_ctx *ctx, const imsrg_basis_desc *b, int device) {
   HIPCHK(ctx, hipMalloc(&ctx->Xt, sizeof(double) * std::max<long>(B.xsize, 1)));
   HIPCHK(ctx, hipMalloc(&ctx->Yt, sizeof(double) * std::max<long>(B.xsize, 1)));
   HIPCHK(ctx, hipMalloc(&ctx->Zbar, sizeof(double) * std::max<long>(B.zsize, 1)));
+  // cross-coupled channels without ph kets are never written and must read as 0
+  HIPCHK(ctx, hipMemset(ctx->Zbar, 0, sizeof(double) * std::max<long>(B.zsize, 1)));
+  HIPCHK(ctx, hipMalloc(&ctx->scratch, sizeof(double) * (B.nch + 8)));
+  HIPCHK(ctx, hipMalloc(&ctx->c122, sizeof(double2) * std::max<long>(B.n122, 1)));
   ctx->npart = 257;
   HIPCHK(ctx, hipMalloc(&ctx->partials, sizeof(double) * ctx->npart));
   HIPCHK(ctx, hipMalloc(&ctx->norm_dev, sizeof(double) * 2));
@@ -143,6 +150,8 @@ extern "C" int imsrg_hip_ctx_destroy(imsrg_ctx *ctx) {
   if (ctx->Xt) (void)hipFree(ctx->Xt);
   if (ctx->Yt) (void)hipFree(ctx->Yt);
   if (ctx->Zbar) (void)hipFree(ctx->Zbar);
+  if (ctx->scratch) (void)hipFree(ctx->scratch);
+  if (ctx->c122) (void)hipFree(ctx->c122);
   if (ctx->partials) (void)hipFree(ctx->partials);
   if (ctx->norm_dev) (void)hipFree(ctx->norm_dev);
   if (ctx->norm_host) (void)hipHostFree(ctx->norm_host);
@@ -399,16 +408,18 @@ static int comm_opts_ok(imsrg_ctx *ctx, const imsrg_comm_opts *o, unsigned *mask
   return IMSRG_OK;
 }
 
+static long r8(long n) { return (n + 7) / 8 * 8; }  // 64-byte aligned segment length in doubles
+
 static int herm_rule(int hx, int hy) { return (hx == hy) ? -1 : 1; }  // src/Commutator.cc:292-297
 
 // Launch sequence of one commutator (or a batch when pa.tab is set).
 static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const int4 *const *items, const int *cnt,
-                          unsigned mask, int prec, int maxn) {
+                          unsigned mask, int prec, int maxn, int maxnch, long maxn122) {
   hipStream_t s = ctx->stream;
   bool prof = ctx->profiling;
   auto mark = [&](int i) -> hipError_t { return prof ? hipEventRecord(ctx->ev[i], s) : hipSuccess; };
   HIPCHK(ctx, mark(0));
-  HIPCHK(ctx, launch_zerobody(pa, npairs, s));
+  HIPCHK(ctx, launch_zerobody(pa, npairs, maxnch, s));
   HIPCHK(ctx, mark(1));
   HIPCHK(ctx, launch_onebody(pa, items[5], cnt[5], s));
   HIPCHK(ctx, mark(2));
@@ -417,7 +428,7 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
   else
     HIPCHK(ctx, launch_zero_two(pa, items[6], cnt[6], s));
   HIPCHK(ctx, mark(3));
-  if (mask & M122) HIPCHK(ctx, launch_c122(pa, items[2], cnt[2], maxn, s));
+  if (mask & M122) HIPCHK(ctx, launch_c122(pa, items[2], cnt[2], maxn, npairs, maxn122, s));
   HIPCHK(ctx, mark(4));
   if (mask & MPH) HIPCHK(ctx, launch_pandya(pa, items[3], cnt[3], s));
   HIPCHK(ctx, mark(5));
@@ -455,6 +466,8 @@ static int commutator_dev(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y
   pa.p0.Xt = ctx->Xt;
   pa.p0.Yt = ctx->Yt;
   pa.p0.Zbar = ctx->Zbar;
+  pa.p0.scratch = ctx->scratch;
+  pa.p0.c122 = ctx->c122;
   pa.p0.hX = X->herm;
   pa.p0.hY = Y->herm;
   pa.p0.hZ = herm_rule(X->herm, Y->herm);
@@ -463,7 +476,7 @@ static int commutator_dev(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y
   const int4 *items[7] = {db.d_pphh, db.d_ph, db.d_c122, db.d_pandya, db.d_invp, db.d_one, db.d_mirror};
   int cnt[7] = {(int)db.wl.pphh.size(), (int)db.wl.ph.size(), (int)db.wl.c122.size(), (int)db.wl.pandya.size(),
                 (int)db.wl.invp.size(), (int)db.wl.one.size(), (int)db.wl.mirror.size()};
-  return run_commutator(ctx, pa, 1, items, cnt, mask, prec, db.h.maxn);
+  return run_commutator(ctx, pa, 1, items, cnt, mask, prec, db.h.maxn, db.h.nch, db.h.n122);
 }
 
 extern "C" int imsrg_hip_commutator_ss_dev(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y, imsrg_dop *Z,
@@ -754,9 +767,11 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
     std::vector<long> woff(npairs);
     for (int i = 0; i < npairs; ++i) {
       woff[i] = wtot;
-      wtot += 2 * ctxs[i]->db.h.xsize + ctxs[i]->db.h.zsize + 64;
+      const BasisDev &Bi = ctxs[i]->db.h;
+      wtot += 2 * r8(Bi.xsize) + r8(Bi.zsize) + r8(Bi.nch + 8) + 2 * r8(Bi.n122) + 64;
     }
     HIPCHK(c0, hipMalloc(&bc.work, sizeof(double) * std::max<long>(wtot, 1)));
+    HIPCHK(c0, hipMemset(bc.work, 0, sizeof(double) * std::max<long>(wtot, 1)));
     std::vector<PairPtrs> tab(npairs);
     for (int i = 0; i < npairs; ++i) {
       const BasisDev &B = ctxs[i]->db.h;
@@ -764,35 +779,41 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
       p.basis = ctxs[i]->db.d;
       p.X = X[i]->buf; p.Y = Y[i]->buf; p.Z = Z[i]->buf;
       p.Xt = bc.work + woff[i];
-      p.Yt = p.Xt + B.xsize;
-      p.Zbar = p.Yt + B.xsize;
+      p.Yt = p.Xt + r8(B.xsize);
+      p.Zbar = p.Yt + r8(B.xsize);
+      p.scratch = p.Zbar + r8(B.zsize);
+      p.c122 = (double2 *)(p.scratch + r8(B.nch + 8));
       p.hX = X[i]->herm; p.hY = Y[i]->herm; p.hZ = herm_rule(X[i]->herm, Y[i]->herm);
       p.mask = mask;
     }
     HIPCHK(c0, hipMalloc(&bc.d_tab, sizeof(PairPtrs) * npairs));
     HIPCHK(c0, hipMemcpy(bc.d_tab, tab.data(), sizeof(PairPtrs) * npairs, hipMemcpyHostToDevice));
-    // concatenated lists: pphh tiles are merged across pairs and sorted by cost
+    // concatenated lists; pp/hh tiles of all pairs go through the XCD-balanced order
     std::vector<int4> all[7];
-    struct T { int4 w; long cost; };
-    std::vector<T> pp;
+    std::vector<TileGroup> groups;
     for (int i = 0; i < npairs; ++i) {
       const WorkLists &wl = ctxs[i]->db.wl;
       const std::vector<int4> *src[7] = {&wl.pphh, &wl.ph, &wl.c122, &wl.pandya, &wl.invp, &wl.one, &wl.mirror};
-      for (int t = 0; t < 7; ++t)
+      for (int t = 1; t < 7; ++t)
         for (int4 w : *src[t]) {
           w.x = i;
-          if (t == 0) {
-            const HostBasis &hb = ctxs[i]->hb;
-            long n = hb.ch[w.y].kets.size();
-            pp.push_back({w, n});
-          } else {
-            all[t].push_back(w);
-          }
+          all[t].push_back(w);
         }
+      // regroup this pair's pp/hh tiles by (channel, row block)
+      std::map<std::pair<int, int>, TileGroup> gm;
+      for (int4 w : wl.pphh) {
+        if (w.y < 0) continue;
+        w.x = i;
+        TileGroup &g = gm[{w.y, w.z}];
+        g.tiles.push_back(w);
+        g.cost += (long)ctxs[i]->hb.ch[w.y].kets.size();
+      }
+      for (auto &kv : gm) groups.push_back(kv.second);
       bc.maxn = std::max(bc.maxn, ctxs[i]->db.h.maxn);
+      bc.maxnch = std::max(bc.maxnch, ctxs[i]->db.h.nch);
+      bc.maxn122 = std::max(bc.maxn122, ctxs[i]->db.h.n122);
     }
-    std::stable_sort(pp.begin(), pp.end(), [](const T &a, const T &b) { return a.cost > b.cost; });
-    for (auto &t : pp) all[0].push_back(t.w);
+    all[0] = xcd_order(groups, kXcd);
     size_t tot = 0;
     for (int t = 0; t < 7; ++t) {
       bc.off[t] = tot;
@@ -812,7 +833,7 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
   pa.tab = bc.d_tab;
   const int4 *items[7];
   for (int t = 0; t < 7; ++t) items[t] = bc.d_items + bc.off[t];
-  return run_commutator(c0, pa, npairs, items, bc.cnt, mask, prec, bc.maxn);
+  return run_commutator(c0, pa, npairs, items, bc.cnt, mask, prec, bc.maxn, bc.maxnch, bc.maxn122);
 }
 
 // ---------------------------------------------------------------------------

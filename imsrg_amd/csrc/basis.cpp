@@ -3,7 +3,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <map>
 
 #include "launch.hpp"
 
@@ -127,6 +129,27 @@ static T *dupload(DeviceBasis &db, const std::vector<T> &v, std::string &err) {
   return (T *)p;
 }
 
+std::vector<int4> xcd_order(std::vector<TileGroup> groups, int nxcd) {
+  // longest-processing-time assignment of tile groups to XCDs, then an
+  // interleaved order in which launch position p runs on XCD p % nxcd
+  std::stable_sort(groups.begin(), groups.end(),
+                   [](const TileGroup &a, const TileGroup &b) { return a.cost > b.cost; });
+  std::vector<long> load(nxcd, 0);
+  std::vector<std::vector<int4>> per(nxcd);
+  for (const auto &g : groups) {
+    int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    load[x] += g.cost;
+    per[x].insert(per[x].end(), g.tiles.begin(), g.tiles.end());
+  }
+  size_t len = 0;
+  for (auto &v : per) len = std::max(len, v.size());
+  std::vector<int4> out;
+  out.reserve(len * nxcd);
+  for (size_t t = 0; t < len; ++t)
+    for (int x = 0; x < nxcd; ++x) out.push_back(t < per[x].size() ? per[x][t] : make_int4(0, -1, 0, 0));
+  return out;
+}
+
 std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
   std::string err;
   BasisDev &B = h;
@@ -155,6 +178,12 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
   B.obstart = dupload(*this, obstart, err); B.oblist = dupload(*this, oblist, err);
   B.kp = dupload(*this, hb.kp, err); B.kq = dupload(*this, hb.kq, err);
   B.kmap = dupload(*this, hb.kmap, err); B.cmap = dupload(*this, hb.cmap, err);
+  {
+    std::vector<int> cT((size_t)hb.nkets * (hb.jmax + 1));
+    for (int k = 0; k < hb.nkets; ++k)
+      for (int J = 0; J <= hb.jmax; ++J) cT[(size_t)J * hb.nkets + k] = hb.cmap[(size_t)k * (hb.jmax + 1) + J];
+    B.cmapT = dupload(*this, cT, err);
+  }
 
   // ---- standard channels ----
   std::vector<int> ch_n(nch), ch_kstart(nch + 1), ch_kets;
@@ -162,9 +191,9 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
   std::vector<int> hh_start(nch + 1), hh_loc, k_start(nch + 1), k_loc, kpp_start(nch + 1), kpp_loc,
       khh_start(nch + 1), khh_loc, gcore_start(nch + 1), gcore_loc, gnc_start(nch + 1), gnc_loc;
   std::vector<double> hh_occ, k_w, kpp_w, khh_w;
-  std::vector<int> p122_start;
-  std::vector<int4> p122_e;
-  std::vector<double> p122_f;
+  // comm122 lists per ket, converted to ELL after the channel loop
+  std::vector<std::vector<int4>> p122_e(nch);   // (ind2, a, o, ket)
+  std::vector<std::vector<double>> p122_f(nch);
   int maxn = 0;
   const double sqrt2 = std::sqrt(2.0);
   for (int c = 0; c < nch; ++c) {
@@ -196,7 +225,7 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
       if (hb.cvq[p] == 0 && hb.cvq[q] == 0) gcore_loc.push_back(i);
       if (hb.cvq[p] != 0 && hb.cvq[q] != 0) gnc_loc.push_back(i);
       // comm122ss partner list, src/Commutator.cc:857-896
-      p122_start.push_back((int)p122_e.size());
+      const size_t e_begin = p122_e[c].size();
       int ii = p, jj = q;
       int flip = -sgn_parity((hb.j2[ii] + hb.j2[jj]) / 2 - C.J);
       for (int a : hb.ob[ii]) {
@@ -205,8 +234,8 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
         if (loc < 0) continue;
         double f = (a > jj) ? (double)flip : (a == jj ? sqrt2 : 1.0);
         if (ii == jj) f /= sqrt2;
-        p122_e.push_back(make_int4(loc, a, ii, 0));
-        p122_f.push_back(f);
+        p122_e[c].push_back(make_int4(loc, a, ii, i));
+        p122_f[c].push_back(f);
       }
       if (ii != jj)
         for (int a : hb.ob[jj]) {
@@ -214,18 +243,59 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
           int loc = hb.kmap[(size_t)ket_index(norb, lo, hi) * (hb.jmax + 1) + C.J];
           if (loc < 0) continue;
           double f = (ii > a) ? (double)flip : (ii == a ? sqrt2 : 1.0);
-          p122_e.push_back(make_int4(loc, a, jj, 0));
-          p122_f.push_back(f);
+          p122_e[c].push_back(make_int4(loc, a, jj, i));
+          p122_f[c].push_back(f);
         }
-      if ((int)p122_e.size() - p122_start.back() > kMax122) return "comm122 partner list too long";
+      if (p122_e[c].size() - e_begin > (size_t)kMax122) return "comm122 partner list too long";
     }
   }
   ch_kstart[nch] = (int)ch_kets.size();
   hh_start[nch] = (int)hh_loc.size(); k_start[nch] = (int)k_loc.size();
   kpp_start[nch] = (int)kpp_loc.size(); khh_start[nch] = (int)khh_loc.size();
   gcore_start[nch] = (int)gcore_loc.size(); gnc_start[nch] = (int)gnc_loc.size();
-  p122_start.push_back((int)p122_e.size());
+  // ELL layout of the comm122 lists
+  std::vector<long> ell_off(nch + 1, 0);
+  std::vector<int> ell_mx(nch, 0), ell_cnt(ch_kets.size(), 0), ell_ind;
+  std::vector<int2> ell_ao;
+  std::vector<double> ell_f;
+  {
+    long tot = 0;
+    for (int c = 0; c < nch; ++c) {
+      const int nk = ch_n[c];
+      std::vector<int> cnt(nk, 0);
+      for (auto &e : p122_e[c]) cnt[e.w]++;
+      int mx = 0;
+      for (int r = 0; r < nk; ++r) {
+        mx = std::max(mx, cnt[r]);
+        ell_cnt[ch_kstart[c] + r] = cnt[r];
+      }
+      ell_mx[c] = mx;
+      ell_off[c] = tot;
+      tot += (long)mx * nk;
+    }
+    ell_off[nch] = tot;
+    ell_ind.assign(tot, 0);
+    ell_ao.assign(tot, make_int2(-1, -1));
+    ell_f.assign(tot, 0.0);
+    for (int c = 0; c < nch; ++c) {
+      const int nk = ch_n[c];
+      std::vector<int> fill(nk, 0);
+      for (size_t t = 0; t < p122_e[c].size(); ++t) {
+        const int4 e = p122_e[c][t];
+        const long pos = ell_off[c] + (long)fill[e.w]++ * nk + e.w;
+        ell_ind[pos] = e.x;
+        ell_ao[pos] = make_int2(e.y, e.z);
+        ell_f[pos] = p122_f[c][t];
+      }
+    }
+    B.n122 = tot;
+  }
   B.maxn = maxn;
+  {
+    int mb = 0;
+    for (int i = 0; i < norb; ++i) mb = std::max(mb, (int)hb.ob[i].size());
+    B.maxblk = mb;
+  }
   B.ch_off = dupload(*this, hb.ch_off, err);
   B.ch_n = dupload(*this, ch_n, err); B.ch_kstart = dupload(*this, ch_kstart, err);
   B.ch_kets = dupload(*this, ch_kets, err); B.w220 = dupload(*this, w220, err);
@@ -236,8 +306,9 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
   B.kpp_w = dupload(*this, kpp_w, err);
   B.khh_start = dupload(*this, khh_start, err); B.khh_loc = dupload(*this, khh_loc, err);
   B.khh_w = dupload(*this, khh_w, err);
-  B.p122_start = dupload(*this, p122_start, err); B.p122_e = dupload(*this, p122_e, err);
-  B.p122_f = dupload(*this, p122_f, err);
+  B.ell_off = dupload(*this, ell_off, err); B.ell_mx = dupload(*this, ell_mx, err);
+  B.ell_cnt = dupload(*this, ell_cnt, err); B.ell_ind = dupload(*this, ell_ind, err);
+  B.ell_ao = dupload(*this, ell_ao, err); B.ell_f = dupload(*this, ell_f, err);
   B.gcore_start = dupload(*this, gcore_start, err); B.gcore_loc = dupload(*this, gcore_loc, err);
   B.gnc_start = dupload(*this, gnc_start, err); B.gnc_loc = dupload(*this, gnc_loc, err);
 
@@ -302,29 +373,47 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
   // ---- 6j table on the device ----
   {
     long tot = (long)B.d1 * B.d1 * B.d2 * B.d1 * B.d1 * B.d2;
-    void *p = nullptr;
+    void *p = nullptr, *pT = nullptr;
     if (hipMalloc(&p, tot * sizeof(double)) != hipSuccess) return "hipMalloc failed for the 6j table";
     allocs.push_back(p);
-    if (launch_sixj_table((double *)p, B.d1, B.d2, 0) != hipSuccess) return "6j table kernel failed";
+    if (hipMalloc(&pT, tot * sizeof(double)) != hipSuccess) return "hipMalloc failed for the 6j table";
+    allocs.push_back(pT);
+    if (launch_sixj_table((double *)p, (double *)pT, B.d1, B.d2, 0) != hipSuccess) return "6j table kernel failed";
     if (hipDeviceSynchronize() != hipSuccess) return "6j table kernel failed";
     B.sixj = (const double *)p;
+    B.sixjT = (const double *)pT;
   }
 
   // ---- work lists ----
   wl = WorkLists();
   {
-    struct T { int4 w; long cost; };
-    std::vector<T> tiles;
+    std::vector<TileGroup> groups;
     for (int c = 0; c < nch; ++c) {
       int nk = ch_n[c];
       if (nk == 0) continue;
-      int nt = (nk + kTile - 1) / kTile;
+      int nt = (nk + kTileB - 1) / kTileB;
       long kk = k_start[c + 1] - k_start[c];
-      for (int rb = 0; rb < nt; ++rb)
-        for (int sb = rb; sb < nt; ++sb) tiles.push_back({make_int4(0, c, rb, sb), kk});
+      for (int rb = 0; rb < nt; ++rb) {
+        TileGroup g;
+        for (int sb = rb; sb < nt; ++sb) g.tiles.push_back(make_int4(0, c, rb, sb));
+        g.cost = (kk + 1) * (long)g.tiles.size();
+        groups.push_back(g);
+      }
     }
-    std::stable_sort(tiles.begin(), tiles.end(), [](const T &a, const T &b) { return a.cost > b.cost; });
-    for (auto &t : tiles) wl.pphh.push_back(t.w);
+    // IMSRG_HIP_PPHH_ORDER: "cost" (tiles by decreasing k-length) or "xcd"
+    // (row-block groups balanced over the XCDs)
+    const char *po = std::getenv("IMSRG_HIP_PPHH_ORDER");
+    if (po && std::string(po) == "xcd") {
+      wl.pphh = xcd_order(groups, kXcd);
+    } else {
+      std::vector<std::pair<long, int4>> t;
+      for (auto &g : groups)
+        for (auto &w : g.tiles) t.push_back({g.cost / (long)g.tiles.size(), w});
+      std::stable_sort(t.begin(), t.end(), [](const std::pair<long, int4> &a, const std::pair<long, int4> &b) {
+        return a.first > b.first;
+      });
+      for (auto &x : t) wl.pphh.push_back(x.second);
+    }
     for (int c = 0; c < nch; ++c) {
       int nk = ch_n[c];
       if (nk == 0) continue;
@@ -336,6 +425,37 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
         for (int ks = ib; ks < nk; ks += kChunk) wl.invp.push_back(make_int4(0, c, ib, ks));
       if (gcore_start[c + 1] > gcore_start[c] && gnc_start[c + 1] > gnc_start[c])
         wl.gen.push_back(make_int4(0, c, 0, 0));
+    }
+    // IMSRG_HIP_INVP_ORDER: "channel" (channel-major), "bra" (all channels of one
+    // bra orbit pair (i,j) consecutively, so the Zbar blocks rows(i) x cols(j)
+    // they share are reused from cache) or "xcd" (bra-major, bras with the same
+    // first orbit i on one XCD)
+    {
+      const char *io = std::getenv("IMSRG_HIP_INVP_ORDER");
+      std::string mode = io ? io : "bra";
+      if (mode != "channel") {
+        auto bra_of = [&](const int4 &w) {
+          int k = hb.ch[w.y].kets[w.z];
+          return std::make_pair(hb.kp[k], hb.kq[k]);
+        };
+        std::stable_sort(wl.invp.begin(), wl.invp.end(), [&](const int4 &a, const int4 &b2) {
+          auto pa = bra_of(a), pb = bra_of(b2);
+          if (pa != pb) return pa < pb;
+          if (a.y != b2.y) return a.y < b2.y;
+          return a.w < b2.w;
+        });
+        if (mode == "xcd") {
+          std::map<int, TileGroup> gm;
+          for (auto &w : wl.invp) {
+            TileGroup &g = gm[bra_of(w).first];
+            g.tiles.push_back(w);
+            g.cost += std::min(kChunk, ch_n[w.y] - w.w);
+          }
+          std::vector<TileGroup> groups;
+          for (auto &kv : gm) groups.push_back(kv.second);
+          wl.invp = xcd_order(groups, kXcd);
+        }
+      }
     }
     for (int c = 0; c < nch; ++c) {
       int nk = cc_n[c], nph = cc_nph[c];

@@ -34,6 +34,15 @@ struct HostBasis {
                         std::vector<int> &tz2, std::vector<double> &occ, std::vector<int> &cvq);
 };
 
+constexpr int kXcd = 8;  // XCDs of an MI355X; workgroup i runs on XCD i % 8
+
+struct TileGroup {
+  std::vector<int4> tiles;  // tiles sharing one 128-row block of one channel
+  long cost = 0;
+};
+// Launch order of tile groups balanced over the XCDs (padding items have .y = -1).
+std::vector<int4> xcd_order(std::vector<TileGroup> groups, int nxcd);
+
 // Work-item lists (int4: pair, a, b, c) for every kernel of one commutator.
 struct WorkLists {
   std::vector<int4> pphh;    // (pair, ch, rb, sb), rb <= sb

@@ -8,6 +8,7 @@
 namespace imsrg {
 
 constexpr int kTile = 64;         // SYR2K output tile edge
+constexpr int kTileB = 128;       // output tile edge of the large pp/hh SYR2K
 constexpr int kTileK = 16;        // k-depth of one LDS stage
 constexpr int kChunk = 256;       // elements per work item of the gather kernels
 constexpr int kMax122 = 32;       // max one-body partners of a ket in comm122
@@ -30,13 +31,15 @@ struct BasisDev {
   long one_off;  // offset of the one-body block inside an operator buffer
   long two_off;  // offset of the two-body arena inside an operator buffer
   long opsize;   // doubles per operator buffer
-  int maxn, maxcc;
+  int maxn, maxcc, maxblk;
+  long n122;           // comm122 ELL entries (including padding)
   const int *j2, *l, *tz2, *cvq;
   const double *occ;
   const int *holes;
   const int *obstart, *oblist;  // one-body block of each orbit
   const int *kp, *kq;
   const int *kmap, *cmap;
+  const int *cmapT;     // cmapT[J*nkets + k] = cmap[k*(jmax+1) + J]
   // standard channels
   const long *ch_off;
   const int *ch_n, *ch_kstart, *ch_kets;
@@ -49,9 +52,14 @@ struct BasisDev {
   const double *kpp_w;
   const int *khh_start, *khh_loc;  // M_hh list (weights n_p n_q)
   const double *khh_w;
-  const int *p122_start;  // per (channel, ket) (index ch_kstart[ch]+i): entries [start, start+1)
-  const int4 *p122_e;     // (ind2, a, o, diag) ; coefficient X1(a, o) * f
-  const double *p122_f;
+  // comm122 partner lists in ELL layout: entry m of ket r of channel ch at
+  // ell_off[ch] + m * n_ch + r, m < ell_mx[ch]; padding has a = -1
+  const long *ell_off;
+  const int *ell_mx;
+  const int *ell_cnt;     // entries of ket (ch_kstart[ch] + r)
+  const int *ell_ind;     // local ket index ind2 of the partner
+  const int2 *ell_ao;     // (a, o): coefficient X1(a, o) * f
+  const double *ell_f;
   // cross-coupled channels
   const int *cc_n, *cc_nph, *cc_kstart, *cc_kets, *cc_phstart, *cc_ph;
   const signed char *cc_phi;  // (-1)^{(j_p+j_q)} per CC ket
@@ -60,9 +68,9 @@ struct BasisDev {
   const int2 *pairs1;   // one-body pairs (i<=j, same (l,j2,tz2) block)
   const double *w1norm; // norb*norb one-body norm weights
   const double *sixj;   // dense 6j table, layout of SixJCache_112112
+  const double *sixjT;  // same values, layout [jj1][jj2][J3][jj4][J6][jj5] (j5 innermost)
   // generator lists: per channel, core-core kets and (non-core)-(non-core) kets
   const int *gcore_start, *gcore_loc, *gnc_start, *gnc_loc;
-  const long *twoch;    // per arena element? (unused)
 };
 
 // Everything one commutator needs: operand buffers, workspaces, flags.
@@ -71,6 +79,8 @@ struct PairPtrs {
   const double *X, *Y;    // operator buffers (zero | one | two)
   double *Z;
   double *Xt, *Yt, *Zbar; // Pandya workspaces
+  double *scratch;        // >= nch + 8 doubles (zero-body partial sums)
+  double2 *c122;          // per-call comm122 coefficients (X1(a,o) f, Y1(a,o) f), n122 entries
   int hX, hY, hZ;
   unsigned mask;
 };
@@ -83,6 +93,10 @@ struct PairArg {
 
 __host__ __device__ inline int ket_index(int norb, int p, int q) { return p * (2 * norb - 1 - p) / 2 + q; }
 __host__ __device__ inline int sgn_parity(int x) { return (x & 1) ? -1 : 1; }
+__host__ __device__ inline long sixjT_index(int d1, int d2, int jj1, int jj2, int J3, int jj4, int jj5, int J6) {
+  return (((((long)((jj1 - 1) >> 1) * d1 + ((jj2 - 1) >> 1)) * d2 + J3) * d1 + ((jj4 - 1) >> 1)) * d2 + J6) * d1 +
+         ((jj5 - 1) >> 1);
+}
 __host__ __device__ inline long sixj_index(int d1, int d2, int jj1, int jj2, int J3, int jj4, int jj5, int J6) {
   return (((((long)((jj1 - 1) >> 1) * d1 + ((jj2 - 1) >> 1)) * d2 + J3) * d1 + ((jj4 - 1) >> 1)) * d1 +
           ((jj5 - 1) >> 1)) * d2 + J6;

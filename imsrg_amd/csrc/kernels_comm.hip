@@ -69,43 +69,73 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV]) {
 }
 
 // ---------------------------------------------------------------------------
-// comm110ss (src/Commutator.cc:564-581) and comm220ss (601-631) -> Z0; also
-// clears Z's one-body block.  One block per pair.
+// comm110ss (src/Commutator.cc:564-581) and comm220ss (601-631) -> Z0.
+// Grid (nch + 1, npairs): block 0 clears Z's one-body block and forms the
+// comm110 sum, block 1+ch the comm220 contribution of channel ch; partial
+// sums go to scratch and k_zerobody_final adds them in channel order.
 __global__ __launch_bounds__(256) void k_zerobody(PairArg pa) {
-  PairPtrs P = pa.get(blockIdx.x);
+  PairPtrs P = pa.get(blockIdx.y);
   const BasisDev &B = *P.basis;
   const int norb = B.norb;
-  const double *X1 = P.X + B.one_off, *Y1 = P.Y + B.one_off;
-  double *Z1 = P.Z + B.one_off;
-  for (int e = threadIdx.x; e < norb * norb; e += blockDim.x) Z1[e] = 0.0;
-  double acc[2] = {0.0, 0.0};
+  const int bid = blockIdx.x;
+  if (bid > B.nch) return;
+  double acc[1] = {0.0};
   const bool skip = (P.hX == P.hY);
-  if (!skip && (P.mask & M110)) {
-    for (int t = threadIdx.x; t < B.nholes * norb; t += blockDim.x) {
-      int a = B.holes[t / norb], k = t % norb;
-      double xy = X1at(X1, norb, a, k) * X1at(Y1, norb, k, a);
-      double yx = X1at(Y1, norb, a, k) * X1at(X1, norb, k, a);
-      acc[0] += (B.j2[a] + 1) * B.occ[a] * (xy - yx);
+  if (bid == 0) {
+    const double *X1 = P.X + B.one_off, *Y1 = P.Y + B.one_off;
+    double *Z1 = P.Z + B.one_off;
+    for (int e = threadIdx.x; e < norb * norb; e += blockDim.x) Z1[e] = 0.0;
+    if (!skip && (P.mask & M110)) {
+      for (int t = threadIdx.x; t < B.nholes * norb; t += blockDim.x) {
+        int a = B.holes[t / norb], k = t % norb;
+        double xy = X1at(X1, norb, a, k) * X1at(Y1, norb, k, a);
+        double yx = X1at(Y1, norb, a, k) * X1at(X1, norb, k, a);
+        acc[0] += (B.j2[a] + 1) * B.occ[a] * (xy - yx);
+      }
     }
-  }
-  if (!skip && (P.mask & M220)) {
-    const double *X2 = P.X + B.two_off, *Y2 = P.Y + B.two_off;
-    for (int ch = 0; ch < B.nch; ++ch) {
-      int h0 = B.hh_start[ch], nhh = B.hh_start[ch + 1] - h0;
-      if (nhh == 0) continue;
-      int n = B.ch_n[ch];
-      long off = B.ch_off[ch];
+  } else if (!skip && (P.mask & M220)) {
+    const int ch = bid - 1;
+    const int h0 = B.hh_start[ch], nhh = B.hh_start[ch + 1] - h0;
+    if (nhh > 0) {
+      const int n = B.ch_n[ch];
+      const long off = B.ch_off[ch];
+      const double *X2 = P.X + B.two_off + off, *Y2 = P.Y + B.two_off + off;
       const double *w = B.w220 + B.ch_kstart[ch];
-      double f = 2.0 * (2 * (ch / 6) + 1);
+      const double f = 2.0 * (2 * (ch / 6) + 1);
+      // X(h,m) = hX X(m,h): both operands read down column h
       for (int t = threadIdx.x; t < nhh * n; t += blockDim.x) {
         int ih = t / n, m = t % n;
         int h = B.hh_loc[h0 + ih];
-        acc[1] += f * B.hh_occ[h0 + ih] * (X2[off + (long)m * n + h] * (Y2[off + (long)h * n + m] * w[m]));
+        acc[0] += f * B.hh_occ[h0 + ih] * ((P.hX * X2[(long)h * n + m]) * (Y2[(long)h * n + m] * w[m]));
       }
     }
   }
-  block_reduce<2>(acc);
-  if (threadIdx.x == 0) P.Z[0] = (0.0 + acc[0]) + acc[1];
+  block_reduce<1>(acc);
+  if (threadIdx.x == 0) P.scratch[bid] = acc[0];
+}
+
+__global__ void k_zerobody_final(PairArg pa) {
+  PairPtrs P = pa.get(blockIdx.x);
+  if (threadIdx.x != 0) return;
+  const BasisDev &B = *P.basis;
+  double s220 = 0.0;
+  for (int c = 0; c < B.nch; ++c) s220 += P.scratch[1 + c];
+  P.Z[0] = (0.0 + P.scratch[0]) + s220;
+}
+
+// comm122 coefficients of this call: x = X1(a,o) f, y = Y1(a,o) f for every
+// partner entry (src/Commutator.cc:857-896).
+__global__ __launch_bounds__(256) void k_c122_coef(PairArg pa, int npairs) {
+  PairPtrs P = pa.get(blockIdx.y);
+  const BasisDev &B = *P.basis;
+  const int norb = B.norb;
+  const double *X1 = P.X + B.one_off, *Y1 = P.Y + B.one_off;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < B.n122; e += (long)gridDim.x * blockDim.x) {
+    const int2 ao = B.ell_ao[e];
+    const double f = B.ell_f[e];
+    P.c122[e] = ao.x < 0 ? make_double2(0.0, 0.0)
+                         : make_double2(X1at(X1, norb, ao.x, ao.y) * f, X1at(Y1, norb, ao.x, ao.y) * f);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -113,6 +143,17 @@ __global__ __launch_bounds__(256) void k_zerobody(PairArg pa) {
 // comm111ss (src/Commutator.cc:644-649), comm121ss (670-728) and the one-body
 // part of comm222_pp_hh_221ss (1246-1294).  M_pp / M_hh elements are formed
 // directly as folded dot products, M(r,s) + hZ M(s,r) with M = X W Y.
+__device__ __forceinline__ double fold_dot(const double *xr, const double *xs, const double *yr, const double *ys,
+                                           const int *loc, const double *w, int nk, int start, int stride,
+                                           double hZ) {
+  double m = 0.0;
+  for (int q = start; q < nk; q += stride) {
+    const int k = loc[q];
+    m += w[q] * (xr[k] * ys[k] + hZ * (xs[k] * yr[k]));
+  }
+  return m;
+}
+
 __global__ __launch_bounds__(256) void k_onebody(PairArg pa, const int4 *items) {
   const int4 it = items[blockIdx.x];
   PairPtrs P = pa.get(it.x);
@@ -128,62 +169,60 @@ __global__ __launch_bounds__(256) void k_onebody(PairArg pa, const int4 *items) 
     }
   }
   if (P.mask & M121) {
-    for (int ia = threadIdx.x; ia < B.nholes; ia += blockDim.x) {
-      int a = B.holes[ia];
-      for (int e = B.obstart[a]; e < B.obstart[a + 1]; ++e) {
-        int b = B.oblist[e];
-        double nanb = B.occ[a] * (1.0 - B.occ[b]);
-        if (fabs(nanb) < 1e-6) continue;
-        double ybiaj = tbme_monopole(B, Y2, b, i, a, j);
-        double yaibj = tbme_monopole(B, Y2, a, i, b, j);
-        double xbiaj = tbme_monopole(B, X2, b, i, a, j);
-        double xaibj = tbme_monopole(B, X2, a, i, b, j);
-        acc[2] += (B.j2[b] + 1) * nanb * X1at(X1, norb, a, b) * ybiaj;
-        acc[2] -= (B.j2[a] + 1) * nanb * X1at(X1, norb, b, a) * yaibj;
-        acc[2] -= (B.j2[b] + 1) * nanb * X1at(Y1, norb, a, b) * xbiaj;
-        acc[2] += (B.j2[a] + 1) * nanb * X1at(Y1, norb, b, a) * xaibj;
-      }
+    // one task per (hole a, partner b, monopole term)
+    const int ntask = B.nholes * B.maxblk * 4;
+    for (int t = threadIdx.x; t < ntask; t += blockDim.x) {
+      const int term = t & 3, ib = (t >> 2) % B.maxblk, ia = (t >> 2) / B.maxblk;
+      const int a = B.holes[ia];
+      if (ib >= B.obstart[a + 1] - B.obstart[a]) continue;
+      const int b = B.oblist[B.obstart[a] + ib];
+      const double nanb = B.occ[a] * (1.0 - B.occ[b]);
+      if (fabs(nanb) < 1e-6) continue;
+      double v;
+      if (term == 0) v = (B.j2[b] + 1) * nanb * X1at(X1, norb, a, b) * tbme_monopole(B, Y2, b, i, a, j);
+      else if (term == 1) v = -(B.j2[a] + 1) * nanb * X1at(X1, norb, b, a) * tbme_monopole(B, Y2, a, i, b, j);
+      else if (term == 2) v = -(B.j2[b] + 1) * nanb * X1at(Y1, norb, a, b) * tbme_monopole(B, X2, b, i, a, j);
+      else v = (B.j2[a] + 1) * nanb * X1at(Y1, norb, b, a) * tbme_monopole(B, X2, a, i, b, j);
+      acc[2] += v;
     }
   }
   if (P.mask & M221) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int nJ = B.jmax + 1;
     const double hX = P.hX, hZ = P.hZ;
-    for (int t = w; t < norb * nJ; t += 4) {
-      int c = t / nJ, J = t % nJ;
-      int Jmin = max(abs(B.j2[c] - B.j2[i]), abs(B.j2[c] - B.j2[j])) / 2;
-      int Jmax = (B.j2[c] + min(B.j2[i], B.j2[j])) / 2;
-      if (J < Jmin || J > Jmax) continue;
-      double nc = B.occ[c], nbc = 1.0 - nc;
-      int kci = ket_index(norb, min(c, i), max(c, i)), kcj = ket_index(norb, min(c, j), max(c, j));
-      int r = B.kmap[(long)kci * nJ + J], s = B.kmap[(long)kcj * nJ + J];
-      if (r < 0 || s < 0) continue;
-      int par = (B.l[c] + B.l[i]) & 1, tz = (B.tz2[c] + B.tz2[i]) / 2;
-      int ch = 6 * J + 2 * (tz + 1) + par;
-      int n = B.ch_n[ch];
-      long off = B.ch_off[ch];
-      double fac = (c == i ? SQRT2_D : 1.0) * (c == j ? SQRT2_D : 1.0);
-      if (c > i) fac *= sgn_parity((B.j2[c] + B.j2[i]) / 2 + 1 + J);
-      if (c > j) fac *= sgn_parity((B.j2[c] + B.j2[j]) / 2 + 1 + J);
-      const double *xr = X2 + off + (long)r * n, *xs = X2 + off + (long)s * n;
-      const double *yr = Y2 + off + (long)r * n, *ys = Y2 + off + (long)s * n;
-      if (fabs(nc) > 1e-9) {
-        int k0 = B.kpp_start[ch], nk = B.kpp_start[ch + 1] - k0;
-        double m = 0.0;
-        for (int q = lane; q < nk; q += 64) {
-          int k = B.kpp_loc[k0 + q];
-          m += B.kpp_w[k0 + q] * (xr[k] * ys[k] + hZ * (xs[k] * yr[k]));
+    // (c, J) combos; M_pp parts (long k lists) one wave each, M_hh parts (short
+    // lists) one lane each
+    for (int pass = 0; pass < 2; ++pass) {
+      const int ncombo = norb * nJ;
+      const int first = pass == 0 ? w : threadIdx.x;
+      const int step = pass == 0 ? 4 : (int)blockDim.x;
+      for (int t = first; t < ncombo; t += step) {
+        const int c = t / nJ, J = t % nJ;
+        const int Jmin = max(abs(B.j2[c] - B.j2[i]), abs(B.j2[c] - B.j2[j])) / 2;
+        const int Jmax = (B.j2[c] + min(B.j2[i], B.j2[j])) / 2;
+        if (J < Jmin || J > Jmax) continue;
+        const double nc = B.occ[c], nbc = 1.0 - nc;
+        if (pass == 0 ? !(fabs(nc) > 1e-9) : !(fabs(nbc) > 1e-9)) continue;
+        const int kci = ket_index(norb, min(c, i), max(c, i)), kcj = ket_index(norb, min(c, j), max(c, j));
+        const int r = B.kmap[(long)kci * nJ + J], s = B.kmap[(long)kcj * nJ + J];
+        if (r < 0 || s < 0) continue;
+        const int ch = 6 * J + 2 * ((B.tz2[c] + B.tz2[i]) / 2 + 1) + ((B.l[c] + B.l[i]) & 1);
+        const int n = B.ch_n[ch];
+        const long off = B.ch_off[ch];
+        double fac = (c == i ? SQRT2_D : 1.0) * (c == j ? SQRT2_D : 1.0);
+        if (c > i) fac *= sgn_parity((B.j2[c] + B.j2[i]) / 2 + 1 + J);
+        if (c > j) fac *= sgn_parity((B.j2[c] + B.j2[j]) / 2 + 1 + J);
+        const double *xr = X2 + off + (long)r * n, *xs = X2 + off + (long)s * n;
+        const double *yr = Y2 + off + (long)r * n, *ys = Y2 + off + (long)s * n;
+        if (pass == 0) {
+          const int k0 = B.kpp_start[ch];
+          const double m = fold_dot(xr, xs, yr, ys, B.kpp_loc + k0, B.kpp_w + k0, B.kpp_start[ch + 1] - k0, lane, 64, hZ);
+          acc[3] += (2 * J + 1) * nc * fac * hX * m;
+        } else {
+          const int k0 = B.khh_start[ch];
+          const double m = fold_dot(xr, xs, yr, ys, B.khh_loc + k0, B.khh_w + k0, B.khh_start[ch + 1] - k0, 0, 1, hZ);
+          acc[3] += (2 * J + 1) * nbc * fac * hX * m;
         }
-        acc[3] += (2 * J + 1) * nc * fac * hX * m;
-      }
-      if (fabs(nbc) > 1e-9) {
-        int k0 = B.khh_start[ch], nk = B.khh_start[ch + 1] - k0;
-        double m = 0.0;
-        for (int q = lane; q < nk; q += 64) {
-          int k = B.khh_loc[k0 + q];
-          m += B.khh_w[k0 + q] * (xr[k] * ys[k] + hZ * (xs[k] * yr[k]));
-        }
-        acc[3] += (2 * J + 1) * nbc * fac * hX * m;
       }
     }
   }
@@ -213,8 +252,8 @@ __global__ __launch_bounds__(256) void k_onebody(PairArg pa, const int4 *items) 
 template <int MODE>
 __global__ __launch_bounds__(256) void k_syr2k(PairArg pa, const int4 *items, int accumulate) {
   constexpr int LD = kTile + 2;
-  __shared__ double As[kTileK][LD];
-  __shared__ double Bs[kTileK][LD];
+  __shared__ double As[2][kTileK][LD];
+  __shared__ double Bs[2][kTileK][LD];
   const int4 it = items[blockIdx.x];
   PairPtrs P = pa.get(it.x);
   const BasisDev &B = *P.basis;
@@ -264,8 +303,10 @@ __global__ __launch_bounds__(256) void k_syr2k(PairArg pa, const int4 *items, in
 
   const int tk = threadIdx.x >> 4, tr = (threadIdx.x & 15) * 4;
   const int K2 = 2 * nk;
-  for (int k0 = 0; k0 < K2; k0 += kTileK) {
-    double av[4] = {0.0, 0.0, 0.0, 0.0}, bv[4] = {0.0, 0.0, 0.0, 0.0};
+  // global -> register loads of one k-stage (16 values of k' per stage)
+  auto load_stage = [&](int k0, double (&av)[4], double (&bv)[4]) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) av[q] = bv[q] = 0.0;
     const int kk = k0 + tk;
     if (kk < K2) {
       const int k = kk < nk ? kk : kk - nk;
@@ -300,17 +341,24 @@ __global__ __launch_bounds__(256) void k_syr2k(PairArg pa, const int4 *items, in
         bv[q] = (c < n) ? sb[c] : 0.0;
       }
     }
+  };
+  double av[4], bv[4];
+  load_stage(0, av, bv);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      As[tk][tr + q] = av[q];
-      Bs[tk][tr + q] = bv[q];
-    }
-    __syncthreads();
+  for (int q = 0; q < 4; ++q) {
+    As[0][tk][tr + q] = av[q];
+    Bs[0][tk][tr + q] = bv[q];
+  }
+  __syncthreads();
+  int cur = 0;
+  for (int k0 = 0; k0 < K2; k0 += kTileK) {
+    const bool more = k0 + kTileK < K2;
+    if (more) load_stage(k0 + kTileK, av, bv);  // in flight during the MFMAs below
 #pragma unroll
     for (int ks = 0; ks < kTileK / 4; ++ks) {
       const int kr = ks * 4 + (lane >> 4);
-      double a0 = As[kr][wr + (lane & 15)], a1 = As[kr][wr + 16 + (lane & 15)];
-      double b0 = Bs[kr][wc + (lane & 15)], b1 = Bs[kr][wc + 16 + (lane & 15)];
+      const double a0 = As[cur][kr][wr + (lane & 15)], a1 = As[cur][kr][wr + 16 + (lane & 15)];
+      const double b0 = Bs[cur][kr][wc + (lane & 15)], b1 = Bs[cur][kr][wc + 16 + (lane & 15)];
       // D^T blocks; f64 16x16x4 result layout: lane l, reg q holds Dm[4q + l/16][l%16],
       // i.e. D row (lane&15), column 4q + (lane>>4)
       acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc[0][0], 0, 0, 0);
@@ -318,22 +366,177 @@ __global__ __launch_bounds__(256) void k_syr2k(PairArg pa, const int4 *items, in
       acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a1, acc[1][0], 0, 0, 0);
       acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a1, acc[1][1], 0, 0, 0);
     }
+    if (more) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        As[cur ^ 1][tk][tr + q] = av[q];
+        Bs[cur ^ 1][tk][tr + q] = bv[q];
+      }
+    }
     __syncthreads();
+    cur ^= 1;
   }
+  if (MODE == 0) {
 #pragma unroll
-  for (int mi = 0; mi < 2; ++mi)
+    for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
-    for (int ni = 0; ni < 2; ++ni)
+      for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int row = R0 + wr + mi * 16 + (lane & 15);
+          const int col = S0 + wc + ni * 16 + 4 * q + (lane >> 4);
+          if (row < n && col < n && row <= col) {
+            const double v = alpha * acc[mi][ni][q];
+            double *dp = D + (long)col * n + row;
+            *dp = accumulate ? *dp + v : v;
+          }
+        }
+  } else {
+    // Zbar blocks are stored in full: the upper triangle as computed, the lower
+    // one as (head) hZ-sign or (tail) phi_r phi_s times the upper element, as
+    // the reference's folds make them (src/Commutator.cc:1792-1805).
+    __shared__ double Tt[kTile][kTile + 1];
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int rr = wr + mi * 16 + (lane & 15), cc = wc + ni * 16 + 4 * q + (lane >> 4);
+          double v = alpha * acc[mi][ni][q];
+          if (MODE == 2) {
+            const int col = S0 + cc;
+            v *= (col < n) ? (double)phi[col] : 0.0;
+          }
+          Tt[rr][cc] = v;
+        }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kTile * kTile; e += blockDim.x) {
+      const int rr = e & (kTile - 1), cc = e / kTile;
+      const int row = R0 + rr, col = S0 + cc;
+      if (row < n && col < n && row <= col) D[(long)col * n + row] = Tt[rr][cc];
+      // mirrored element (S0+rr, R0+cc) from the upper element (R0+cc, S0+rr)
+      const int row2 = S0 + rr, col2 = R0 + cc;
+      if (row2 < n && col2 < n && row2 > col2) {
+        const double sym = (MODE == 1) ? h : (double)(phi[row2] * phi[col2]);
+        D[(long)col2 * n + row2] = sym * Tt[cc][rr];
+      }
+    }
+  }
+}
+
+
+// ---------------------------------------------------------------------------
+// pp/hh SYR2K with 128x128 output tiles: each of the 4 waves owns a 64x64
+// sub-tile (4x4 MFMA blocks), so one LDS read feeds four MFMAs; k-stages of
+// 16 are double-buffered with the next stage's global loads in flight.
+//   Z(r,s) = hY sum_k w_k [X(r,k) Y(s,k) + hZ Y(r,k) X(s,k)],  r <= s
+// (comm222_pp_hh_221ss two-body part, src/Commutator.cc:1171-1194, 1237-1238).
+__global__ __launch_bounds__(256, 2) void k_pphh_big(PairArg pa, const int4 *items) {
+  constexpr int T = kTileB, LD = kTileB + 4;
+  __shared__ double As[2][kTileK][LD];
+  __shared__ double Bs[2][kTileK][LD];
+  __shared__ double Ws[2][kTileK];
+  const int4 it = items[blockIdx.x];
+  if (it.y < 0) return;  // padding item of the XCD-balanced order
+  PairPtrs P = pa.get(it.x);
+  const BasisDev &B = *P.basis;
+  const int ch = it.y;
+  const int n = B.ch_n[ch];
+  const long off = B.ch_off[ch];
+  const double *Am = P.X + B.two_off + off, *Bm = P.Y + B.two_off + off;
+  const int k0s = B.k_start[ch], nk = B.k_start[ch + 1] - k0s;
+  const int *kl = B.k_loc + k0s;
+  const double *kw = B.k_w + k0s;
+  const double h = (double)P.hZ;
+  double *D = P.Z + B.two_off + off;
+  const int R0 = it.z * T, S0 = it.w * T;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int wr = (wv & 1) * 64, wc = (wv >> 1) * 64;
+  dbl4 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
+  // loader: thread t owns row (t & 127) of the A and B row blocks for the eight
+  // k' values kh*8 .. kh*8+7 (kh = t >> 7); loads are plain copies from clamped
+  // addresses (padding rows are never written back, padding k' get weight 0),
+  // so no arithmetic waits on them before they are stored to LDS
+  const int trow = threadIdx.x & (T - 1), kh = threadIdx.x >> 7;
+  const int K2 = 2 * nk;
+  const int ra = min(R0 + trow, n - 1), rb = min(S0 + trow, n - 1);
+  auto load_stage = [&](int k0, double (&av)[8], double (&bv)[8]) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int kk = min(k0 + kh * 8 + q, K2 - 1);
+      const int k = kk < nk ? kk : kk - nk;
+      const long c = (long)kl[k] * n;
+      av[q] = (kk < nk ? Am : Bm)[c + ra];
+      bv[q] = (kk < nk ? Bm : Am)[c + rb];
+    }
+  };
+  auto load_w = [&](int k0, int buf) {
+    if (threadIdx.x < kTileK) {
+      const int kk = k0 + threadIdx.x;
+      double w = 0.0;
+      if (kk < K2) w = kk < nk ? kw[kk] : kw[kk - nk] * h;
+      Ws[buf][threadIdx.x] = w;
+    }
+  };
+  double av[8], bv[8];
+  if (K2 > 0) {  // a channel without pp or hh kets only writes zeros
+    load_stage(0, av, bv);
+    load_w(0, 0);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      As[0][kh * 8 + q][trow] = av[q];
+      Bs[0][kh * 8 + q][trow] = bv[q];
+    }
+  }
+  __syncthreads();
+  int cur = 0;
+  for (int k0 = 0; k0 < K2; k0 += kTileK) {
+    const bool more = k0 + kTileK < K2;
+    if (more) {
+      load_stage(k0 + kTileK, av, bv);
+      load_w(k0 + kTileK, cur ^ 1);
+    }
+#pragma unroll
+    for (int ks = 0; ks < kTileK / 4; ++ks) {
+      const int kr = ks * 4 + (lane >> 4);
+      const double wk = Ws[cur][kr];
+      double a[4], b[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        a[m] = As[cur][kr][wr + m * 16 + (lane & 15)] * wk;
+        b[m] = Bs[cur][kr][wc + m * 16 + (lane & 15)];
+      }
+#pragma unroll
+      for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 4; ++ni)
+          acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[ni], a[mi], acc[mi][ni], 0, 0, 0);
+    }
+    if (more) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        As[cur ^ 1][kh * 8 + q][trow] = av[q];
+        Bs[cur ^ 1][kh * 8 + q][trow] = bv[q];
+      }
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+  const double alpha = (double)P.hY;
+#pragma unroll
+  for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int row = R0 + wr + mi * 16 + (lane & 15);
         const int col = S0 + wc + ni * 16 + 4 * q + (lane >> 4);
-        if (row < n && col < n && row <= col) {
-          double v = alpha * acc[mi][ni][q];
-          if (MODE == 2) v *= (double)phi[col];
-          double *dp = D + (long)col * n + row;
-          *dp = accumulate ? *dp + v : v;
-        }
+        if (row < n && col < n && row <= col) D[(long)col * n + row] = alpha * acc[mi][ni][q];
       }
 }
 
@@ -342,6 +545,7 @@ __global__ __launch_bounds__(256) void k_syr2k(PairArg pa, const int4 *items, in
 //   Z(r,s) -= W(r,s) + hZ W(s,r),  r <= s,
 //   W(:,q) = sum_m [Y2(:,ind_m) x_m - X2(:,ind_m) y_m]  (x2 if ket q has p == q),
 // with W(s,r) formed from column s of X2 / Y2 (staged in LDS) via Y2(s,i) = hY Y2(i,s).
+// The coefficients x_m, y_m of this call come from k_c122_coef.
 __global__ __launch_bounds__(256) void k_c122(PairArg pa, const int4 *items) {
   extern __shared__ double lds[];
   __shared__ int sind[kMax122];
@@ -350,10 +554,9 @@ __global__ __launch_bounds__(256) void k_c122(PairArg pa, const int4 *items) {
   PairPtrs P = pa.get(it.x);
   const BasisDev &B = *P.basis;
   const int ch = it.y, s = it.z;
-  const int n = B.ch_n[ch], norb = B.norb;
+  const int n = B.ch_n[ch];
   const long off = B.ch_off[ch];
   const double *X2 = P.X + B.two_off + off, *Y2 = P.Y + B.two_off + off;
-  const double *X1 = P.X + B.one_off, *Y1 = P.Y + B.one_off;
   double *Z2 = P.Z + B.two_off + off;
   double *colX = lds, *colY = lds + n;
   for (int q = threadIdx.x; q < n; q += blockDim.x) {
@@ -361,13 +564,15 @@ __global__ __launch_bounds__(256) void k_c122(PairArg pa, const int4 *items) {
     colY[q] = Y2[(long)s * n + q];
   }
   const int ks = B.ch_kstart[ch];
-  const int e0 = B.p122_start[ks + s], ne = B.p122_start[ks + s + 1] - e0;
+  const long eo = B.ell_off[ch];
+  const int mx = B.ell_mx[ch];
+  const int ne = B.ell_cnt[ks + s];
   if (threadIdx.x < ne) {
-    int4 e = B.p122_e[e0 + threadIdx.x];
-    double f = B.p122_f[e0 + threadIdx.x];
-    sind[threadIdx.x] = e.x;
-    scx[threadIdx.x] = X1at(X1, norb, e.y, e.z) * f;
-    scy[threadIdx.x] = X1at(Y1, norb, e.y, e.z) * f;
+    const long pos = eo + (long)threadIdx.x * n + s;
+    const double2 c = P.c122[pos];
+    sind[threadIdx.x] = B.ell_ind[pos];
+    scx[threadIdx.x] = c.x;
+    scy[threadIdx.x] = c.y;
   }
   __syncthreads();
   const int kets = B.ch_kets[ks + s];
@@ -381,12 +586,11 @@ __global__ __launch_bounds__(256) void k_c122(PairArg pa, const int4 *items) {
     }
     wrs *= ds;
     double wsr = 0.0;
-    const int f0 = B.p122_start[ks + r], f1 = B.p122_start[ks + r + 1];
-    for (int e = f0; e < f1; ++e) {
-      int4 pe = B.p122_e[e];
-      double f = B.p122_f[e];
-      double cx = X1at(X1, norb, pe.y, pe.z) * f, cy = X1at(Y1, norb, pe.y, pe.z) * f;
-      wsr += hY * colY[pe.x] * cx - hX * colX[pe.x] * cy;
+    for (int m = 0; m < mx; ++m) {  // padded entries carry zero coefficients
+      const long pos = eo + (long)m * n + r;
+      const int ind = B.ell_ind[pos];
+      const double2 c = P.c122[pos];
+      wsr += hY * colY[ind] * c.x - hX * colX[ind] * c.y;
     }
     const int ketr = B.ch_kets[ks + r];
     if (B.kp[ketr] == B.kq[ketr]) wsr *= 2.0;
@@ -456,70 +660,103 @@ __global__ __launch_bounds__(256) void k_pandya(PairArg pa, const int4 *items) {
   P.Yt[o] = ybar * normf;
 }
 
-// Zbar element (row, col) of CC channel cc; only the upper triangles of the
-// head (Hermitian-folded) and tail (phase-folded) blocks are stored.
-__device__ __forceinline__ double zbar_get(const PairPtrs &P, const BasisDev &B, int cc, int row, int col) {
+// Zbar element (row, col) of CC channel cc (full storage, head | tail).
+__device__ __forceinline__ double zbar_at(const PairPtrs &P, const BasisDev &B, int cc, int row, int col) {
   const int nk = B.cc_n[cc];
-  const double *Zb = P.Zbar + B.cc_zoff[cc];
-  if (col < nk) {
-    if (row <= col) return Zb[(long)col * nk + row];
-    return (P.hZ == 1 ? 1.0 : -1.0) * Zb[(long)row * nk + col];
-  }
-  const int s = col - nk;
-  const double *T = Zb + (long)nk * nk;
-  if (row <= s) return T[(long)s * nk + row];
-  const signed char *phi = B.cc_phi + B.cc_kstart[cc];
-  return (double)(phi[row] * phi[s]) * T[(long)row * nk + s];
+  return P.Zbar[B.cc_zoff[cc] + (long)col * nk + row];
+}
+
+// wave-wide min / max of an int
+__device__ __forceinline__ int wave_min(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
 }
 
 // AddInversePandyaTransformation (src/Commutator.cc:1501-1616), upper triangle.
+// Consecutive lanes take consecutive kets (k,l) of one bra (i,j); the J' loops
+// run over the wave-wide range so that at each step the lanes of one k read
+// the same cross-coupled channel: the direct term down column kj of Zbar at
+// rows il, the exchange term (through the fold symmetry) down column ik at rows lj.
 __global__ __launch_bounds__(256) void k_invpandya(PairArg pa, const int4 *items) {
   const int4 it = items[blockIdx.x];
+  if (it.y < 0) return;  // padding item of the XCD-balanced order
   PairPtrs P = pa.get(it.x);
   const BasisDev &B = *P.basis;
   const int ch = it.y, ibra = it.z;
   const int n = B.ch_n[ch], J = ch / 6;
   const int iket = it.w + threadIdx.x;
   const int ketmin = (P.hZ == 1) ? ibra : ibra + 1;
-  if (iket < ketmin || iket >= n) return;
+  const bool active = iket >= ketmin && iket < n;
   const int ks = B.ch_kstart[ch];
-  const int kb = B.ch_kets[ks + ibra], kk = B.ch_kets[ks + iket];
+  const int kb = B.ch_kets[ks + ibra], kk = B.ch_kets[ks + (active ? iket : ibra)];
   const int i = B.kp[kb], j = B.kq[kb], k = B.kp[kk], l = B.kq[kk];
   const int jji = B.j2[i], jjj = B.j2[j], jjk = B.j2[k], jjl = B.j2[l];
-  const int nJ = B.jmax + 1;
+  const int nk = B.nkets;
+  const double *Zb = P.Zbar;
   double commij = 0.0, commji = 0.0;
   {
     const int pcc = (B.l[i] + B.l[l]) & 1, tcc = abs(B.tz2[i] - B.tz2[l]) / 2;
-    const int Jpmin = max(abs(jji - jjl), abs(jjk - jjj)) / 2, Jpmax = min(jji + jjl, jjk + jjj) / 2;
-    const int kil = ket_index(B.norb, min(i, l), max(i, l)), kjk = ket_index(B.norb, min(j, k), max(j, k));
-    for (int Jp = Jpmin; Jp <= Jpmax; ++Jp) {
-      const double sj = B.sixj[sixj_index(B.d1, B.d2, jji, jjj, J, jjk, jjl, Jp)];
-      if (fabs(sj) < 1e-8) continue;
+    int Jpmin = max(abs(jji - jjl), abs(jjk - jjj)) / 2, Jpmax = min(jji + jjl, jjk + jjj) / 2;
+    if (!active) { Jpmin = 1 << 20; Jpmax = -1; }
+    const int lo = wave_min(Jpmin), hi = wave_max(Jpmax);
+    const int kil = ket_index(B.norb, i, l);  // i <= l for ibra <= iket
+    const int kjk = ket_index(B.norb, min(j, k), max(j, k));
+    const long s6 = sixjT_index(B.d1, B.d2, jji, jjj, J, jjk, jjl, 0);
+    // all loads of an iteration depend only on Jp, so iterations overlap; the
+    // |6j| < 1e-8 skip of the reference is applied as a zero weight
+#pragma unroll 2
+    for (int Jp = lo; Jp <= hi; ++Jp) {
+      if (Jp < Jpmin || Jp > Jpmax) continue;
+      const double sj = B.sixjT[s6 + (long)Jp * B.d1];
       const int cc = 6 * Jp + 2 * (tcc + 1) + pcc;
       const int nkc = B.cc_n[cc];
-      const int il = B.cmap[(long)kil * nJ + Jp] + (i > l ? nkc : 0);
-      const int kj = B.cmap[(long)kjk * nJ + Jp] + (k > j ? nkc : 0);
-      commij -= (2 * Jp + 1) * sj * zbar_get(P, B, cc, il, kj);
+      const int il = B.cmapT[(long)Jp * nk + kil];
+      const int kj = B.cmapT[(long)Jp * nk + kjk] + (k > j ? nkc : 0);
+      const double z = Zb[B.cc_zoff[cc] + (long)kj * nkc + il];
+      const double w = (fabs(sj) < 1e-8) ? 0.0 : (2 * Jp + 1) * sj;
+      commij -= w * z;
     }
   }
   if (k == l) {
     commji = commij;
   } else if (i == j) {
     commji = sgn_parity((jji + jjj + jjk + jjl) / 2) * commij;
-  } else {
+  }
+  const bool exch = active && k != l && i != j;
+  {
     const int pcc = (B.l[i] + B.l[k]) & 1, tcc = abs(B.tz2[i] - B.tz2[k]) / 2;
-    const int Jpmin = max(abs(jjj - jjl), abs(jjk - jji)) / 2, Jpmax = min(jjj + jjl, jjk + jji) / 2;
-    const int kik = ket_index(B.norb, min(i, k), max(i, k)), klj = ket_index(B.norb, min(l, j), max(l, j));
-    for (int Jp = Jpmin; Jp <= Jpmax; ++Jp) {
-      const double sj = B.sixj[sixj_index(B.d1, B.d2, jjj, jji, J, jjk, jjl, Jp)];
-      if (fabs(sj) < 1e-8) continue;
+    int Jpmin = max(abs(jjj - jjl), abs(jjk - jji)) / 2, Jpmax = min(jjj + jjl, jjk + jji) / 2;
+    if (!exch) { Jpmin = 1 << 20; Jpmax = -1; }
+    const int lo = wave_min(Jpmin), hi = wave_max(Jpmax);
+    const int kik = ket_index(B.norb, i, k);  // i <= k for ibra <= iket
+    const int klj = ket_index(B.norb, min(l, j), max(l, j));
+    const long s6x = sixjT_index(B.d1, B.d2, jjj, jji, J, jjk, jjl, 0);
+    const double hf = (P.hZ == 1) ? 1.0 : -1.0;
+#pragma unroll 2
+    for (int Jp = lo; Jp <= hi; ++Jp) {
+      if (Jp < Jpmin || Jp > Jpmax) continue;
+      const double sj = B.sixjT[s6x + (long)Jp * B.d1];
       const int cc = 6 * Jp + 2 * (tcc + 1) + pcc;
       const int nkc = B.cc_n[cc];
-      const int ik = B.cmap[(long)kik * nJ + Jp] + (i > k ? nkc : 0);
-      const int lj = B.cmap[(long)klj * nJ + Jp] + (l > j ? nkc : 0);
-      commji -= (2 * Jp + 1) * sj * zbar_get(P, B, cc, ik, lj);
+      const int ik = B.cmapT[(long)Jp * nk + kik];
+      const int lj = B.cmapT[(long)Jp * nk + klj];
+      const double *Zc = Zb + B.cc_zoff[cc];
+      // tail element T(ik, lj) = phi_ik phi_lj T(lj, ik); head H(ik, lj) = hZ-sign H(lj, ik)
+      const double me = (l > j) ? Zc[(long)(nkc + ik) * nkc + lj] : Zc[(long)ik * nkc + lj];
+      double sym = hf;
+      if (l > j) {
+        const signed char *phi = B.cc_phi + B.cc_kstart[cc];
+        sym = (double)(phi[ik] * phi[lj]);
+      }
+      const double w = (fabs(sj) < 1e-8) ? 0.0 : (2 * Jp + 1) * sj;
+      commji -= w * (sym * me);
     }
   }
+  if (!active) return;
   const int dbra = (i == j), dket = (k == l);
   const double nrm = (dbra == dket) ? 1.0 + dbra : SQRT2_D;
   const double z = -(commij - sgn_parity((jjk + jjl) / 2 - J) * commji) / nrm;
@@ -572,9 +809,10 @@ __global__ __launch_bounds__(256) void k_zero_two(PairArg pa, const int4 *items)
 }
 
 // ---------------------------------------------------------------------------
-hipError_t launch_zerobody(const PairArg &pa, int npairs, hipStream_t s) {
+hipError_t launch_zerobody(const PairArg &pa, int npairs, int maxnch, hipStream_t s) {
   if (npairs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_zerobody, dim3(npairs), dim3(256), 0, s, pa);
+  hipLaunchKernelGGL(k_zerobody, dim3(maxnch + 1, npairs), dim3(256), 0, s, pa);
+  hipLaunchKernelGGL(k_zerobody_final, dim3(npairs), dim3(64), 0, s, pa);
   return hipGetLastError();
 }
 hipError_t launch_onebody(const PairArg &pa, const int4 *items, int nitems, hipStream_t s) {
@@ -585,7 +823,7 @@ hipError_t launch_onebody(const PairArg &pa, const int4 *items, int nitems, hipS
 hipError_t launch_pphh(const PairArg &pa, const int4 *items, int nitems, int precision, hipStream_t s) {
   (void)precision;
   if (nitems <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_syr2k<0>, dim3(nitems), dim3(256), 0, s, pa, items, 0);
+  hipLaunchKernelGGL(k_pphh_big, dim3(nitems), dim3(256), 0, s, pa, items);
   return hipGetLastError();
 }
 hipError_t launch_ph_gemm(const PairArg &pa, const int4 *items, int nitems, int precision, hipStream_t s) {
@@ -595,8 +833,11 @@ hipError_t launch_ph_gemm(const PairArg &pa, const int4 *items, int nitems, int 
   hipLaunchKernelGGL(k_syr2k<2>, dim3(nitems), dim3(256), 0, s, pa, items, 0);
   return hipGetLastError();
 }
-hipError_t launch_c122(const PairArg &pa, const int4 *items, int nitems, int maxn, hipStream_t s) {
+hipError_t launch_c122(const PairArg &pa, const int4 *items, int nitems, int maxn, int npairs, long maxn122,
+                       hipStream_t s) {
   if (nitems <= 0) return hipSuccess;
+  int cb = (int)std::max<long>(1, std::min<long>((maxn122 + 255) / 256, 4096));
+  hipLaunchKernelGGL(k_c122_coef, dim3(cb, npairs), dim3(256), 0, s, pa, npairs);
   size_t lds = (size_t)2 * maxn * sizeof(double);
   hipLaunchKernelGGL(k_c122, dim3(nitems), dim3(256), lds, s, pa, items);
   return hipGetLastError();

@@ -7,13 +7,14 @@
 namespace imsrg {
 
 // dense 6j table {jj1 jj2 J3; jj4 jj5 J6} (double-double Racah), layout of SixJCache_112112
-hipError_t launch_sixj_table(double *table, int d1, int d2, hipStream_t s);
+hipError_t launch_sixj_table(double *table, double *tableT, int d1, int d2, hipStream_t s);
 
 // commutator phases; items are device arrays of int4 work items
-hipError_t launch_zerobody(const PairArg &pa, int npairs, hipStream_t s);
+hipError_t launch_zerobody(const PairArg &pa, int npairs, int maxnch, hipStream_t s);
 hipError_t launch_onebody(const PairArg &pa, const int4 *items, int nitems, hipStream_t s);
 hipError_t launch_pphh(const PairArg &pa, const int4 *items, int nitems, int precision, hipStream_t s);
-hipError_t launch_c122(const PairArg &pa, const int4 *items, int nitems, int maxn, hipStream_t s);
+hipError_t launch_c122(const PairArg &pa, const int4 *items, int nitems, int maxn, int npairs, long maxn122,
+                       hipStream_t s);
 hipError_t launch_pandya(const PairArg &pa, const int4 *items, int nitems, hipStream_t s);
 hipError_t launch_ph_gemm(const PairArg &pa, const int4 *items, int nitems, int precision, hipStream_t s);
 hipError_t launch_invpandya(const PairArg &pa, const int4 *items, int nitems, hipStream_t s);
