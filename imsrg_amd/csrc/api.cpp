@@ -436,7 +436,7 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
   HIPCHK(ctx, mark(6));
   if (mask & MPH) HIPCHK(ctx, launch_invpandya(pa, items[4], cnt[4], s));
   HIPCHK(ctx, mark(7));
-  HIPCHK(ctx, launch_mirror(pa, items[6], cnt[6], s));
+  HIPCHK(ctx, launch_mirror(pa, items[6], cnt[6], (mask & MPH) != 0, s));
   HIPCHK(ctx, mark(8));
   if (prof) {
     HIPCHK(ctx, hipEventSynchronize(ctx->ev[8]));

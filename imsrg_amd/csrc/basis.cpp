@@ -183,7 +183,22 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
     for (int k = 0; k < hb.nkets; ++k)
       for (int J = 0; J <= hb.jmax; ++J) cT[(size_t)J * hb.nkets + k] = hb.cmap[(size_t)k * (hb.jmax + 1) + J];
     B.cmapT = dupload(*this, cT, err);
+    for (int k = 0; k < hb.nkets; ++k)
+      for (int J = 0; J <= hb.jmax; ++J) cT[(size_t)J * hb.nkets + k] = hb.kmap[(size_t)k * (hb.jmax + 1) + J];
+    B.kmapT = dupload(*this, cT, err);
   }
+  // (parity, Tz) groups of kets
+  std::vector<int> grp_start(7, 0), grp_kets;
+  for (int g = 0; g < 6; ++g) {
+    grp_start[g] = (int)grp_kets.size();
+    for (int k = 0; k < hb.nkets; ++k) {
+      const int p = hb.kp[k], q = hb.kq[k];
+      const int par = (hb.l[p] + hb.l[q]) & 1, tz = (hb.tz2[p] + hb.tz2[q]) / 2;
+      if (2 * (tz + 1) + par == g) grp_kets.push_back(k);
+    }
+  }
+  grp_start[6] = (int)grp_kets.size();
+  B.grp_start = dupload(*this, grp_start, err); B.grp_kets = dupload(*this, grp_kets, err);
 
   // ---- standard channels ----
   std::vector<int> ch_n(nch), ch_kstart(nch + 1), ch_kets;
@@ -421,41 +436,22 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
       for (int rb = 0; rb < nt; ++rb)
         for (int sb = rb; sb < nt; ++sb) wl.mirror.push_back(make_int4(0, c, rb, sb));
       for (int s = 0; s < nk; ++s) wl.c122.push_back(make_int4(0, c, s, 0));
-      for (int ib = 0; ib < nk; ++ib)
-        for (int ks = ib; ks < nk; ks += kChunk) wl.invp.push_back(make_int4(0, c, ib, ks));
       if (gcore_start[c + 1] > gcore_start[c] && gnc_start[c + 1] > gnc_start[c])
         wl.gen.push_back(make_int4(0, c, 0, 0));
     }
-    // IMSRG_HIP_INVP_ORDER: "channel" (channel-major), "bra" (all channels of one
-    // bra orbit pair (i,j) consecutively, so the Zbar blocks rows(i) x cols(j)
-    // they share are reused from cache) or "xcd" (bra-major, bras with the same
-    // first orbit i on one XCD)
+    // inverse Pandya by orbit quadruple: (pair, group g, bra b, first ket) with
+    // one lane per group ket e >= b; items in global bra-ket order, so the bras
+    // (i, j) that share 6j sub-tables and Zbar rows (i, .) run together
     {
-      const char *io = std::getenv("IMSRG_HIP_INVP_ORDER");
-      std::string mode = io ? io : "bra";
-      if (mode != "channel") {
-        auto bra_of = [&](const int4 &w) {
-          int k = hb.ch[w.y].kets[w.z];
-          return std::make_pair(hb.kp[k], hb.kq[k]);
-        };
-        std::stable_sort(wl.invp.begin(), wl.invp.end(), [&](const int4 &a, const int4 &b2) {
-          auto pa = bra_of(a), pb = bra_of(b2);
-          if (pa != pb) return pa < pb;
-          if (a.y != b2.y) return a.y < b2.y;
-          return a.w < b2.w;
-        });
-        if (mode == "xcd") {
-          std::map<int, TileGroup> gm;
-          for (auto &w : wl.invp) {
-            TileGroup &g = gm[bra_of(w).first];
-            g.tiles.push_back(w);
-            g.cost += std::min(kChunk, ch_n[w.y] - w.w);
-          }
-          std::vector<TileGroup> groups;
-          for (auto &kv : gm) groups.push_back(kv.second);
-          wl.invp = xcd_order(groups, kXcd);
-        }
+      std::vector<std::pair<int, int4>> t;
+      for (int g = 0; g < 6; ++g) {
+        const int gs = grp_start[g], ng = grp_start[g + 1] - gs;
+        for (int b = 0; b < ng; ++b)
+          for (int e0 = b; e0 < ng; e0 += kChunk) t.push_back({grp_kets[gs + b], make_int4(0, g, b, e0)});
       }
+      std::stable_sort(t.begin(), t.end(),
+                       [](const std::pair<int, int4> &x, const std::pair<int, int4> &y) { return x.first < y.first; });
+      for (auto &x : t) wl.invp.push_back(x.second);
     }
     for (int c = 0; c < nch; ++c) {
       int nk = cc_n[c], nph = cc_nph[c];

@@ -49,7 +49,7 @@ struct WorkLists {
   std::vector<int4> ph;      // (pair, cc, rb, sb), rb <= sb
   std::vector<int4> c122;    // (pair, ch, col, 0)
   std::vector<int4> pandya;  // (pair, cc, start, 0)
-  std::vector<int4> invp;    // (pair, ch, ibra, kstart)
+  std::vector<int4> invp;    // (pair, group, bra, first ket) of the (parity, Tz) ket groups
   std::vector<int4> one;     // (pair, i, j, 0)
   std::vector<int4> mirror;  // (pair, ch, rb, sb), rb < sb  and diagonal tiles
   std::vector<int4> gen;     // (pair, ch, 0, 0) channels with work for the generator

@@ -12,6 +12,7 @@ constexpr int kTileB = 128;       // output tile edge of the large pp/hh SYR2K
 constexpr int kTileK = 16;        // k-depth of one LDS stage
 constexpr int kChunk = 256;       // elements per work item of the gather kernels
 constexpr int kMax122 = 32;       // max one-body partners of a ket in comm122
+constexpr int kJC = 4;            // J values per register chunk of the inverse Pandya gather
 
 // Term bits (same values as IMSRG_COMM* in include/imsrg_hip.h)
 constexpr unsigned M110 = 1u << 0, M220 = 1u << 1, M111 = 1u << 2, M121 = 1u << 3, M221 = 1u << 4,
@@ -40,6 +41,10 @@ struct BasisDev {
   const int *kp, *kq;
   const int *kmap, *cmap;
   const int *cmapT;     // cmapT[J*nkets + k] = cmap[k*(jmax+1) + J]
+  const int *kmapT;     // kmapT[J*nkets + k] = kmap[k*(jmax+1) + J]
+  // kets of each (parity, Tz) group g = 2(Tz+1) + parity (the channels 6J + g),
+  // ascending global index: grp_kets[grp_start[g] .. grp_start[g+1])
+  const int *grp_start, *grp_kets;
   // standard channels
   const long *ch_off;
   const int *ch_n, *ch_kstart, *ch_kets;

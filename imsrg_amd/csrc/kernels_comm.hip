@@ -666,81 +666,111 @@ __device__ __forceinline__ double zbar_at(const PairPtrs &P, const BasisDev &B, 
   return P.Zbar[B.cc_zoff[cc] + (long)col * nk + row];
 }
 
-// wave-wide min / max of an int
+// wave-wide min / max of an int, returned as a wave-uniform (scalar) value
 __device__ __forceinline__ int wave_min(int v) {
   for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-  return v;
+  return __builtin_amdgcn_readfirstlane(v);
 }
 __device__ __forceinline__ int wave_max(int v) {
   for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-  return v;
+  return __builtin_amdgcn_readfirstlane(v);
 }
 
-// AddInversePandyaTransformation (src/Commutator.cc:1501-1616), upper triangle.
-// Consecutive lanes take consecutive kets (k,l) of one bra (i,j); the J' loops
-// run over the wave-wide range so that at each step the lanes of one k read
-// the same cross-coupled channel: the direct term down column kj of Zbar at
-// rows il, the exchange term (through the fold symmetry) down column ik at rows lj.
-__global__ __launch_bounds__(256) void k_invpandya(PairArg pa, const int4 *items) {
+// AddInversePandyaTransformation (src/Commutator.cc:1501-1616), by orbit quadruple.
+// One lane per pair of kets (ij) <= (kl) of one (parity, Tz) group: the
+// cross-coupled elements Zbar^{J'}[(il),(kj)] (direct) and Zbar^{J'}[(jl),(ki)]
+// (exchange) do not depend on the standard-channel J, so each is loaded once
+// and recoupled into every J of the quadruple (register chunks of kJC values):
+//   commij(J) = -sum_J' (2J'+1) {ji jj J; jk jl J'} Zbar^{J'}[il, kj]
+//   commji(J) = -sum_J' (2J'+1) {jj ji J; jk jl J'} Zbar^{J'}[jl, ki]
+//   Z^J_{ij,kl} += -(commij - (-1)^{(jk+jl)/2 - J} commji) / norm
+// with the reference's |6j| < 1e-8 skip.  Every upper element (ij <= kl, or
+// ij < kl for anti-Hermitian Z) of every channel is produced exactly once and
+// stored at its transposed slot (consecutive lanes, consecutive addresses);
+// k_mirror_fold adds it to the upper triangle, the diagonal is added in place.
+__global__ __launch_bounds__(256, 6) void k_invpandya(PairArg pa, const int4 *items) {
   const int4 it = items[blockIdx.x];
-  if (it.y < 0) return;  // padding item of the XCD-balanced order
   PairPtrs P = pa.get(it.x);
   const BasisDev &B = *P.basis;
-  const int ch = it.y, ibra = it.z;
-  const int n = B.ch_n[ch], J = ch / 6;
-  const int iket = it.w + threadIdx.x;
-  const int ketmin = (P.hZ == 1) ? ibra : ibra + 1;
-  const bool active = iket >= ketmin && iket < n;
-  const int ks = B.ch_kstart[ch];
-  const int kb = B.ch_kets[ks + ibra], kk = B.ch_kets[ks + (active ? iket : ibra)];
-  const int i = B.kp[kb], j = B.kq[kb], k = B.kp[kk], l = B.kq[kk];
+  const int g = it.y, b = it.z;
+  const int gs = B.grp_start[g], ng = B.grp_start[g + 1] - gs;
+  const int e = it.w + threadIdx.x;
+  const int emin = (P.hZ == 1) ? b : b + 1;
+  const bool active = e >= emin && e < ng;
+  const int kb = B.grp_kets[gs + b], kk = B.grp_kets[gs + (active ? e : b)];
+  const int i = B.kp[kb], j = B.kq[kb], k = B.kp[kk], l = B.kq[kk];  // i <= k <= l
   const int jji = B.j2[i], jjj = B.j2[j], jjk = B.j2[k], jjl = B.j2[l];
   const int nk = B.nkets;
+  const bool pauli = (i == j) || (k == l);  // kets (p,p) exist for even J only
+  int Jlo = max(abs(jji - jjj), abs(jjk - jjl)) / 2, Jhi = min(jji + jjj, jjk + jjl) / 2;
+  if (pauli) {
+    Jlo += Jlo & 1;
+    Jhi -= Jhi & 1;
+  }
+  const bool live = active && Jlo <= Jhi;
+  if (!live) {
+    Jlo = 1 << 20;
+    Jhi = -1;
+  }
+  const int wlo = wave_min(Jlo), whi = wave_max(Jhi);
+  if (wlo > whi) return;  // wave-uniform: no lane of this wave has a channel
+  // direct term: CC channels of (i,l) for J' in tri(ji,jl) ∩ tri(jk,jj)
+  int Dlo = max(abs(jji - jjl), abs(jjk - jjj)) / 2, Dhi = min(jji + jjl, jjk + jjj) / 2;
+  if (!live) {
+    Dlo = 1 << 20;
+    Dhi = -1;
+  }
+  const int dlo = wave_min(Dlo), dhi = wave_max(Dhi);
+  // exchange term: CC channels of (i,k) for J' in tri(jj,jl) ∩ tri(jk,ji)
+  const bool exch = live && k != l && i != j;
+  int Xlo = max(abs(jjj - jjl), abs(jjk - jji)) / 2, Xhi = min(jjj + jjl, jjk + jji) / 2;
+  if (!exch) {
+    Xlo = 1 << 20;
+    Xhi = -1;
+  }
+  const int xlo = wave_min(Xlo), xhi = wave_max(Xhi);
+  const int pd = (B.l[i] + B.l[l]) & 1, td = abs(B.tz2[i] - B.tz2[l]) / 2;
+  const int px = (B.l[i] + B.l[k]) & 1, tx = abs(B.tz2[i] - B.tz2[k]) / 2;
+  const int kil = ket_index(B.norb, i, l), kjk = ket_index(B.norb, min(j, k), max(j, k));
+  const int kik = ket_index(B.norb, i, k), klj = ket_index(B.norb, min(l, j), max(l, j));
+  const long sJ = (long)B.d1 * B.d2 * B.d1;  // stride of J3 in the sixjT layout
+  const long s6d = sixjT_index(B.d1, B.d2, jji, jjj, 0, jjk, jjl, 0);
+  const long s6x = sixjT_index(B.d1, B.d2, jjj, jji, 0, jjk, jjl, 0);
+  const double hf = (P.hZ == 1) ? 1.0 : -1.0;
   const double *Zb = P.Zbar;
-  double commij = 0.0, commji = 0.0;
-  {
-    const int pcc = (B.l[i] + B.l[l]) & 1, tcc = abs(B.tz2[i] - B.tz2[l]) / 2;
-    int Jpmin = max(abs(jji - jjl), abs(jjk - jjj)) / 2, Jpmax = min(jji + jjl, jjk + jjj) / 2;
-    if (!active) { Jpmin = 1 << 20; Jpmax = -1; }
-    const int lo = wave_min(Jpmin), hi = wave_max(Jpmax);
-    const int kil = ket_index(B.norb, i, l);  // i <= l for ibra <= iket
-    const int kjk = ket_index(B.norb, min(j, k), max(j, k));
-    const long s6 = sixjT_index(B.d1, B.d2, jji, jjj, J, jjk, jjl, 0);
-    // all loads of an iteration depend only on Jp, so iterations overlap; the
-    // |6j| < 1e-8 skip of the reference is applied as a zero weight
-#pragma unroll 2
-    for (int Jp = lo; Jp <= hi; ++Jp) {
-      if (Jp < Jpmin || Jp > Jpmax) continue;
-      const double sj = B.sixjT[s6 + (long)Jp * B.d1];
-      const int cc = 6 * Jp + 2 * (tcc + 1) + pcc;
+  const int dbra = (i == j), dket = (k == l);
+  const double nrm = (dbra == dket) ? 1.0 + dbra : SQRT2_D;
+  const double pij = sgn_parity((jji + jjj + jjk + jjl) / 2);
+  double *Ztwo = P.Z + B.two_off;
+  for (int c0 = wlo; c0 <= whi; c0 += kJC) {
+    // lane's valid J of this chunk as a bit mask
+    unsigned vm = 0;
+#pragma unroll
+    for (int t = 0; t < kJC; ++t) {
+      const int J = c0 + t;
+      if (J >= Jlo && J <= Jhi && !(pauli && (J & 1))) vm |= 1u << t;
+    }
+    double ad[kJC], ax[kJC];
+#pragma unroll
+    for (int t = 0; t < kJC; ++t) ad[t] = ax[t] = 0.0;
+    for (int Jp = dlo; Jp <= dhi; ++Jp) {
+      if (Jp < Dlo || Jp > Dhi) continue;
+      const int cc = 6 * Jp + 2 * (td + 1) + pd;
       const int nkc = B.cc_n[cc];
       const int il = B.cmapT[(long)Jp * nk + kil];
       const int kj = B.cmapT[(long)Jp * nk + kjk] + (k > j ? nkc : 0);
-      const double z = Zb[B.cc_zoff[cc] + (long)kj * nkc + il];
-      const double w = (fabs(sj) < 1e-8) ? 0.0 : (2 * Jp + 1) * sj;
-      commij -= w * z;
+      const double z = (2 * Jp + 1) * Zb[B.cc_zoff[cc] + (long)kj * nkc + il];
+      const double *s6 = B.sixjT + s6d + (long)c0 * sJ + (long)Jp * B.d1;
+#pragma unroll
+      for (int t = 0; t < kJC; ++t) {
+        if (c0 + t > whi) break;
+        const double sj = s6[t * sJ];
+        ad[t] += ((vm >> t) & 1) && fabs(sj) >= 1e-8 ? sj * z : 0.0;
+      }
     }
-  }
-  if (k == l) {
-    commji = commij;
-  } else if (i == j) {
-    commji = sgn_parity((jji + jjj + jjk + jjl) / 2) * commij;
-  }
-  const bool exch = active && k != l && i != j;
-  {
-    const int pcc = (B.l[i] + B.l[k]) & 1, tcc = abs(B.tz2[i] - B.tz2[k]) / 2;
-    int Jpmin = max(abs(jjj - jjl), abs(jjk - jji)) / 2, Jpmax = min(jjj + jjl, jjk + jji) / 2;
-    if (!exch) { Jpmin = 1 << 20; Jpmax = -1; }
-    const int lo = wave_min(Jpmin), hi = wave_max(Jpmax);
-    const int kik = ket_index(B.norb, i, k);  // i <= k for ibra <= iket
-    const int klj = ket_index(B.norb, min(l, j), max(l, j));
-    const long s6x = sixjT_index(B.d1, B.d2, jjj, jji, J, jjk, jjl, 0);
-    const double hf = (P.hZ == 1) ? 1.0 : -1.0;
-#pragma unroll 2
-    for (int Jp = lo; Jp <= hi; ++Jp) {
-      if (Jp < Jpmin || Jp > Jpmax) continue;
-      const double sj = B.sixjT[s6x + (long)Jp * B.d1];
-      const int cc = 6 * Jp + 2 * (tcc + 1) + pcc;
+    for (int Jp = xlo; Jp <= xhi; ++Jp) {
+      if (Jp < Xlo || Jp > Xhi) continue;
+      const int cc = 6 * Jp + 2 * (tx + 1) + px;
       const int nkc = B.cc_n[cc];
       const int ik = B.cmapT[(long)Jp * nk + kik];
       const int lj = B.cmapT[(long)Jp * nk + klj];
@@ -752,16 +782,33 @@ __global__ __launch_bounds__(256) void k_invpandya(PairArg pa, const int4 *items
         const signed char *phi = B.cc_phi + B.cc_kstart[cc];
         sym = (double)(phi[ik] * phi[lj]);
       }
-      const double w = (fabs(sj) < 1e-8) ? 0.0 : (2 * Jp + 1) * sj;
-      commji -= w * (sym * me);
+      const double z = (2 * Jp + 1) * (sym * me);
+      const double *s6 = B.sixjT + s6x + (long)c0 * sJ + (long)Jp * B.d1;
+#pragma unroll
+      for (int t = 0; t < kJC; ++t) {
+        if (c0 + t > whi) break;
+        const double sj = s6[t * sJ];
+        ax[t] += ((vm >> t) & 1) && fabs(sj) >= 1e-8 ? sj * z : 0.0;
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < kJC; ++t) {
+      if (c0 + t > whi) break;
+      if (!((vm >> t) & 1)) continue;
+      const int J = c0 + t;
+      const double commij = -ad[t];
+      const double commji = (k == l) ? commij : (i == j) ? pij * commij : -ax[t];
+      const double z = -(commij - sgn_parity((jjk + jjl) / 2 - J) * commji) / nrm;
+      const int ch = 6 * J + g;
+      const int n = B.ch_n[ch];
+      const int ib = B.kmapT[(long)J * nk + kb], ik = B.kmapT[(long)J * nk + kk];
+      double *Z2 = Ztwo + B.ch_off[ch];
+      if (ik == ib)
+        Z2[(long)ib * n + ib] += z;
+      else
+        Z2[(long)ib * n + ik] = z;
     }
   }
-  if (!active) return;
-  const int dbra = (i == j), dket = (k == l);
-  const double nrm = (dbra == dket) ? 1.0 + dbra : SQRT2_D;
-  const double z = -(commij - sgn_parity((jjk + jjl) / 2 - J) * commji) / nrm;
-  double *Z2 = P.Z + B.two_off + B.ch_off[ch];
-  Z2[(long)iket * n + ibra] += z;
 }
 
 // Lower triangle <- hZ * upper triangle; anti-Hermitian diagonal <- 0.
@@ -788,6 +835,41 @@ __global__ __launch_bounds__(256) void k_mirror(PairArg pa, const int4 *items) {
       Z2[(long)col * n + row] = hZ * t[rr][cc];
     else if (row == col && P.hZ == -1)
       Z2[(long)col * n + row] = 0.0;
+  }
+}
+
+// Upper triangle += transposed-slot contributions of k_invpandya, then
+// lower triangle <- hZ * upper triangle; anti-Hermitian diagonal <- 0.
+// Tile pair: U = rows R0.., cols S0.. (R0 <= S0) and its transpose position.
+__global__ __launch_bounds__(256) void k_mirror_fold(PairArg pa, const int4 *items) {
+  __shared__ double tu[kTile][kTile + 1];  // tu[col][row] of U
+  __shared__ double tl[kTile][kTile + 1];  // tl[col][row] of the transposed tile
+  const int4 it = items[blockIdx.x];
+  PairPtrs P = pa.get(it.x);
+  const BasisDev &B = *P.basis;
+  const int ch = it.y, n = B.ch_n[ch];
+  const int R0 = it.z * kTile, S0 = it.w * kTile;
+  double *Z2 = P.Z + B.two_off + B.ch_off[ch];
+  for (int e = threadIdx.x; e < kTile * kTile; e += blockDim.x) {
+    const int rr = e & (kTile - 1), cc = e / kTile;
+    if (R0 + rr < n && S0 + cc < n) tu[cc][rr] = Z2[(long)(S0 + cc) * n + R0 + rr];
+    if (S0 + rr < n && R0 + cc < n) tl[cc][rr] = Z2[(long)(R0 + cc) * n + S0 + rr];
+  }
+  __syncthreads();
+  const double hZ = P.hZ;
+  // upper element (r, c) = (R0 + rr, S0 + cc), r < c: its contribution sits at
+  // (row c, col r) = tl[rr][cc]
+  for (int e = threadIdx.x; e < kTile * kTile; e += blockDim.x) {
+    const int rr = e & (kTile - 1), cc = e / kTile;
+    const int r = R0 + rr, c = S0 + cc;
+    if (r >= n || c >= n) continue;
+    if (r < c) {
+      const double u = tu[cc][rr] + tl[rr][cc];
+      Z2[(long)c * n + r] = u;
+      Z2[(long)r * n + c] = hZ * u;
+    } else if (r == c && P.hZ == -1) {
+      Z2[(long)c * n + r] = 0.0;
+    }
   }
 }
 
@@ -852,9 +934,12 @@ hipError_t launch_invpandya(const PairArg &pa, const int4 *items, int nitems, hi
   hipLaunchKernelGGL(k_invpandya, dim3(nitems), dim3(kChunk), 0, s, pa, items);
   return hipGetLastError();
 }
-hipError_t launch_mirror(const PairArg &pa, const int4 *items, int nitems, hipStream_t s) {
+hipError_t launch_mirror(const PairArg &pa, const int4 *items, int nitems, bool fold, hipStream_t s) {
   if (nitems <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_mirror, dim3(nitems), dim3(256), 0, s, pa, items);
+  if (fold)
+    hipLaunchKernelGGL(k_mirror_fold, dim3(nitems), dim3(256), 0, s, pa, items);
+  else
+    hipLaunchKernelGGL(k_mirror, dim3(nitems), dim3(256), 0, s, pa, items);
   return hipGetLastError();
 }
 hipError_t launch_zero_two(const PairArg &pa, const int4 *items, int nitems, hipStream_t s) {

@@ -18,7 +18,9 @@ hipError_t launch_c122(const PairArg &pa, const int4 *items, int nitems, int max
 hipError_t launch_pandya(const PairArg &pa, const int4 *items, int nitems, hipStream_t s);
 hipError_t launch_ph_gemm(const PairArg &pa, const int4 *items, int nitems, int precision, hipStream_t s);
 hipError_t launch_invpandya(const PairArg &pa, const int4 *items, int nitems, hipStream_t s);
-hipError_t launch_mirror(const PairArg &pa, const int4 *items, int nitems, hipStream_t s);
+// lower triangle <- hZ * upper; with fold, first upper += the inverse-Pandya
+// contributions stored at the transposed slots
+hipError_t launch_mirror(const PairArg &pa, const int4 *items, int nitems, bool fold, hipStream_t s);
 hipError_t launch_zero_two(const PairArg &pa, const int4 *items, int nitems, hipStream_t s);
 
 // operator arithmetic
