@@ -105,7 +105,7 @@ class ModelSpace:
         return list(a)
 
     def selftest_mfma(self):
-        out = np.zeros(512)
+        out = np.zeros(1024)
         check(lib().imsrg_hip_selftest_mfma(self.h, out.ctypes.data_as(C.POINTER(C.c_double))), self.h)
         return out
 

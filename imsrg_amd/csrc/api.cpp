@@ -849,12 +849,12 @@ extern "C" int imsrg_hip_phase_times(imsrg_ctx *ctx, double *us, int n) {
 }
 
 // Lane layout self-test of v_mfma_f64_16x16x4f64 (not part of the reference interface).
-extern "C" int imsrg_hip_selftest_mfma(imsrg_ctx *ctx, double *out512) {
-  if (!ctx || !out512) return IMSRG_E_INVALID;
+extern "C" int imsrg_hip_selftest_mfma(imsrg_ctx *ctx, double *out1024) {
+  if (!ctx || !out1024) return IMSRG_E_INVALID;
   double *d = nullptr;
-  HIPCHK(ctx, hipMalloc(&d, 512 * sizeof(double)));
+  HIPCHK(ctx, hipMalloc(&d, 1024 * sizeof(double)));
   hipError_t e = launch_mfma_probe(d, ctx->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(out512, d, 512 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out1024, d, 1024 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   (void)hipFree(d);
   if (e != hipSuccess) return fail(ctx, IMSRG_E_HIP, hipGetErrorString(e));

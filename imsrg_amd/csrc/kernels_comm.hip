@@ -10,12 +10,23 @@
 // 1611-1612).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "internal.hpp"
 #include "launch.hpp"
 
 namespace imsrg {
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+// Column (within a 16x16 block) held by register q of a lane in the D^T-block
+// formulation of the GEMM kernels: f64 16x16x4 puts D[m][n] in lane n+16(m%4),
+// register m/4; f32 16x16x4 in lane n+16(m/4), register m%4.
+template <bool MIXED>
+__device__ __forceinline__ int mfma_col(int lane, int q) {
+  return MIXED ? 4 * (lane >> 4) + q : 4 * q + (lane >> 4);
+}
 
 #define SQRT2_D 1.4142135623730951
 
@@ -249,11 +260,15 @@ __global__ __launch_bounds__(256) void k_onebody(PairArg pa, const int4 *items) 
 //         B = Ybar^T, h = +1 (Z Hermitian) or -1.
 // MODE 2: tail of Zbar (src/Commutator.cc:1788-1789, 1805): B columns permuted
 //         by ab <-> ba with weights hY*phi(ph ket), cph(s) = phi(s), h = +1.
-template <int MODE>
+//
+// MIXED: operands rounded to FP32 in LDS, products on f32 16x16x4 MFMA, FP32
+// partial sums over kF32Flush k-steps added into the FP64 accumulators.
+template <int MODE, bool MIXED>
 __global__ __launch_bounds__(256) void k_syr2k(PairArg pa, const int4 *items, int accumulate) {
   constexpr int LD = kTile + 2;
-  __shared__ double As[2][kTileK][LD];
-  __shared__ double Bs[2][kTileK][LD];
+  using ST = typename std::conditional<MIXED, float, double>::type;
+  __shared__ ST As[2][kTileK][LD];
+  __shared__ ST Bs[2][kTileK][LD];
   const int4 it = items[blockIdx.x];
   PairPtrs P = pa.get(it.x);
   const BasisDev &B = *P.basis;
@@ -354,17 +369,40 @@ __global__ __launch_bounds__(256) void k_syr2k(PairArg pa, const int4 *items, in
   for (int k0 = 0; k0 < K2; k0 += kTileK) {
     const bool more = k0 + kTileK < K2;
     if (more) load_stage(k0 + kTileK, av, bv);  // in flight during the MFMAs below
+    if constexpr (MIXED) {
+      flt4 t[2][2];
 #pragma unroll
-    for (int ks = 0; ks < kTileK / 4; ++ks) {
-      const int kr = ks * 4 + (lane >> 4);
-      const double a0 = As[cur][kr][wr + (lane & 15)], a1 = As[cur][kr][wr + 16 + (lane & 15)];
-      const double b0 = Bs[cur][kr][wc + (lane & 15)], b1 = Bs[cur][kr][wc + 16 + (lane & 15)];
-      // D^T blocks; f64 16x16x4 result layout: lane l, reg q holds Dm[4q + l/16][l%16],
-      // i.e. D row (lane&15), column 4q + (lane>>4)
-      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc[0][0], 0, 0, 0);
-      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a0, acc[0][1], 0, 0, 0);
-      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a1, acc[1][0], 0, 0, 0);
-      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a1, acc[1][1], 0, 0, 0);
+      for (int ks = 0; ks < kTileK / 4; ++ks) {
+        const int kr = ks * 4 + (lane >> 4);
+        const float a0 = As[cur][kr][wr + (lane & 15)], a1 = As[cur][kr][wr + 16 + (lane & 15)];
+        const float b0 = Bs[cur][kr][wc + (lane & 15)], b1 = Bs[cur][kr][wc + 16 + (lane & 15)];
+        const bool fresh = ks % kF32Flush == 0;
+        const flt4 fz = {0.f, 0.f, 0.f, 0.f};
+        t[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(b0, a0, fresh ? fz : t[0][0], 0, 0, 0);
+        t[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(b1, a0, fresh ? fz : t[0][1], 0, 0, 0);
+        t[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(b0, a1, fresh ? fz : t[1][0], 0, 0, 0);
+        t[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(b1, a1, fresh ? fz : t[1][1], 0, 0, 0);
+        if (ks % kF32Flush == kF32Flush - 1) {
+#pragma unroll
+          for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+            for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+              for (int q = 0; q < 4; ++q) acc[mi][ni][q] += (double)t[mi][ni][q];
+        }
+      }
+    } else {
+#pragma unroll
+      for (int ks = 0; ks < kTileK / 4; ++ks) {
+        const int kr = ks * 4 + (lane >> 4);
+        const double a0 = As[cur][kr][wr + (lane & 15)], a1 = As[cur][kr][wr + 16 + (lane & 15)];
+        const double b0 = Bs[cur][kr][wc + (lane & 15)], b1 = Bs[cur][kr][wc + 16 + (lane & 15)];
+        // D^T blocks: lane l, register q holds D row (lane&15), column mfma_col(l, q)
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a0, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a1, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a1, acc[1][1], 0, 0, 0);
+      }
     }
     if (more) {
 #pragma unroll
@@ -384,7 +422,7 @@ __global__ __launch_bounds__(256) void k_syr2k(PairArg pa, const int4 *items, in
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int row = R0 + wr + mi * 16 + (lane & 15);
-          const int col = S0 + wc + ni * 16 + 4 * q + (lane >> 4);
+          const int col = S0 + wc + ni * 16 + mfma_col<MIXED>(lane, q);
           if (row < n && col < n && row <= col) {
             const double v = alpha * acc[mi][ni][q];
             double *dp = D + (long)col * n + row;
@@ -402,7 +440,7 @@ __global__ __launch_bounds__(256) void k_syr2k(PairArg pa, const int4 *items, in
       for (int ni = 0; ni < 2; ++ni)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const int rr = wr + mi * 16 + (lane & 15), cc = wc + ni * 16 + 4 * q + (lane >> 4);
+          const int rr = wr + mi * 16 + (lane & 15), cc = wc + ni * 16 + mfma_col<MIXED>(lane, q);
           double v = alpha * acc[mi][ni][q];
           if (MODE == 2) {
             const int col = S0 + cc;
@@ -432,11 +470,24 @@ __global__ __launch_bounds__(256) void k_syr2k(PairArg pa, const int4 *items, in
 // 16 are double-buffered with the next stage's global loads in flight.
 //   Z(r,s) = hY sum_k w_k [X(r,k) Y(s,k) + hZ Y(r,k) X(s,k)],  r <= s
 // (comm222_pp_hh_221ss two-body part, src/Commutator.cc:1171-1194, 1237-1238).
-__global__ __launch_bounds__(256, 2) void k_pphh_big(PairArg pa, const int4 *items) {
+// MIXED: as in k_syr2k.  The FP64 accumulators, FP32 partials and the staged
+// loads need more than the 256 registers of two waves per SIMD, so the mixed
+// instantiation runs one wave per SIMD.
+template <bool MIXED>
+__device__ __forceinline__ void pphh_tile(PairArg pa, const int4 *items);
+__global__ __launch_bounds__(256, 2) void k_pphh_big_f64(PairArg pa, const int4 *items) {
+  pphh_tile<false>(pa, items);
+}
+__global__ __launch_bounds__(256, 1) void k_pphh_big_mixed(PairArg pa, const int4 *items) {
+  pphh_tile<true>(pa, items);
+}
+template <bool MIXED>
+__device__ __forceinline__ void pphh_tile(PairArg pa, const int4 *items) {
   constexpr int T = kTileB, LD = kTileB + 4;
-  __shared__ double As[2][kTileK][LD];
-  __shared__ double Bs[2][kTileK][LD];
-  __shared__ double Ws[2][kTileK];
+  using ST = typename std::conditional<MIXED, float, double>::type;
+  __shared__ ST As[2][kTileK][LD];
+  __shared__ ST Bs[2][kTileK][LD];
+  __shared__ ST Ws[2][kTileK];
   const int4 it = items[blockIdx.x];
   if (it.y < 0) return;  // padding item of the XCD-balanced order
   PairPtrs P = pa.get(it.x);
@@ -501,21 +552,48 @@ __global__ __launch_bounds__(256, 2) void k_pphh_big(PairArg pa, const int4 *ite
       load_stage(k0 + kTileK, av, bv);
       load_w(k0 + kTileK, cur ^ 1);
     }
+    if constexpr (MIXED) {
+      // one row block mi at a time, operands re-read from LDS, so that only four
+      // FP32 partial blocks are live next to the FP64 accumulators
 #pragma unroll
-    for (int ks = 0; ks < kTileK / 4; ++ks) {
-      const int kr = ks * 4 + (lane >> 4);
-      const double wk = Ws[cur][kr];
-      double a[4], b[4];
+      for (int mi = 0; mi < 4; ++mi) {
 #pragma unroll
-      for (int m = 0; m < 4; ++m) {
-        a[m] = As[cur][kr][wr + m * 16 + (lane & 15)] * wk;
-        b[m] = Bs[cur][kr][wc + m * 16 + (lane & 15)];
+        for (int kp = 0; kp < kTileK / 4; kp += kF32Flush) {
+          flt4 t[4];
+#pragma unroll
+          for (int ni = 0; ni < 4; ++ni) t[ni] = flt4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int ks = kp; ks < kp + kF32Flush; ++ks) {
+            const int kr = ks * 4 + (lane >> 4);
+            const float am = As[cur][kr][wr + mi * 16 + (lane & 15)] * Ws[cur][kr];
+#pragma unroll
+            for (int ni = 0; ni < 4; ++ni)
+              t[ni] = __builtin_amdgcn_mfma_f32_16x16x4f32(Bs[cur][kr][wc + ni * 16 + (lane & 15)], am, t[ni], 0,
+                                                           0, 0);
+          }
+#pragma unroll
+          for (int ni = 0; ni < 4; ++ni)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[mi][ni][q] += (double)t[ni][q];
+        }
       }
+    } else {
 #pragma unroll
-      for (int mi = 0; mi < 4; ++mi)
+      for (int ks = 0; ks < kTileK / 4; ++ks) {
+        const int kr = ks * 4 + (lane >> 4);
+        const double wk = Ws[cur][kr];
+        double a[4], b[4];
 #pragma unroll
-        for (int ni = 0; ni < 4; ++ni)
-          acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[ni], a[mi], acc[mi][ni], 0, 0, 0);
+        for (int m = 0; m < 4; ++m) {
+          a[m] = As[cur][kr][wr + m * 16 + (lane & 15)] * wk;
+          b[m] = Bs[cur][kr][wc + m * 16 + (lane & 15)];
+        }
+#pragma unroll
+        for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+          for (int ni = 0; ni < 4; ++ni)
+            acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[ni], a[mi], acc[mi][ni], 0, 0, 0);
+      }
     }
     if (more) {
 #pragma unroll
@@ -535,7 +613,7 @@ __global__ __launch_bounds__(256, 2) void k_pphh_big(PairArg pa, const int4 *ite
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int row = R0 + wr + mi * 16 + (lane & 15);
-        const int col = S0 + wc + ni * 16 + 4 * q + (lane >> 4);
+        const int col = S0 + wc + ni * 16 + mfma_col<MIXED>(lane, q);
         if (row < n && col < n && row <= col) D[(long)col * n + row] = alpha * acc[mi][ni][q];
       }
 }
@@ -903,16 +981,22 @@ hipError_t launch_onebody(const PairArg &pa, const int4 *items, int nitems, hipS
   return hipGetLastError();
 }
 hipError_t launch_pphh(const PairArg &pa, const int4 *items, int nitems, int precision, hipStream_t s) {
-  (void)precision;
   if (nitems <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_pphh_big, dim3(nitems), dim3(256), 0, s, pa, items);
+  if (precision == 1)
+    hipLaunchKernelGGL(k_pphh_big_mixed, dim3(nitems), dim3(256), 0, s, pa, items);
+  else
+    hipLaunchKernelGGL(k_pphh_big_f64, dim3(nitems), dim3(256), 0, s, pa, items);
   return hipGetLastError();
 }
 hipError_t launch_ph_gemm(const PairArg &pa, const int4 *items, int nitems, int precision, hipStream_t s) {
-  (void)precision;
   if (nitems <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_syr2k<1>, dim3(nitems), dim3(256), 0, s, pa, items, 0);
-  hipLaunchKernelGGL(k_syr2k<2>, dim3(nitems), dim3(256), 0, s, pa, items, 0);
+  if (precision == 1) {
+    hipLaunchKernelGGL((k_syr2k<1, true>), dim3(nitems), dim3(256), 0, s, pa, items, 0);
+    hipLaunchKernelGGL((k_syr2k<2, true>), dim3(nitems), dim3(256), 0, s, pa, items, 0);
+  } else {
+    hipLaunchKernelGGL((k_syr2k<1, false>), dim3(nitems), dim3(256), 0, s, pa, items, 0);
+    hipLaunchKernelGGL((k_syr2k<2, false>), dim3(nitems), dim3(256), 0, s, pa, items, 0);
+  }
   return hipGetLastError();
 }
 hipError_t launch_c122(const PairArg &pa, const int4 *items, int nitems, int maxn, int npairs, long maxn122,

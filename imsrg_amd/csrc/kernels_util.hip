@@ -373,9 +373,15 @@ __global__ void k_mfma_probe(double *out) {
   dbl4u z = {0.0, 0.0, 0.0, 0.0};
   dbl4u r1 = __builtin_amdgcn_mfma_f64_16x16x4f64((double)lane, 1.0, z, 0, 0, 0);
   dbl4u r2 = __builtin_amdgcn_mfma_f64_16x16x4f64(1.0, (double)lane, z, 0, 0, 0);
+  typedef float flt4u __attribute__((ext_vector_type(4)));
+  flt4u zf = {0.f, 0.f, 0.f, 0.f};
+  flt4u f1 = __builtin_amdgcn_mfma_f32_16x16x4f32((float)lane, 1.0f, zf, 0, 0, 0);
+  flt4u f2 = __builtin_amdgcn_mfma_f32_16x16x4f32(1.0f, (float)lane, zf, 0, 0, 0);
   for (int q = 0; q < 4; ++q) {
     out[lane * 4 + q] = r1[q];
     out[256 + lane * 4 + q] = r2[q];
+    out[512 + lane * 4 + q] = f1[q];
+    out[768 + lane * 4 + q] = f2[q];
   }
 }
 hipError_t launch_mfma_probe(double *out, hipStream_t s) {

@@ -34,7 +34,7 @@ hipError_t launch_fill_random(const BasisDev *bd, const BasisDev &bh, double *op
 hipError_t launch_white(const BasisDev *bd, const BasisDev &bh, const double *H, double *Eta, const int4 *gen_items,
                         int ngen, hipStream_t s);
 
-// self-test of the MFMA f64 lane layout: out[256] receives D of a 16x16x4 product
+// self-test of the MFMA f64 and f32 lane layouts: out[1024] (see imsrg_hip_selftest_mfma)
 hipError_t launch_mfma_probe(double *out, hipStream_t s);
 
 }  // namespace imsrg

@@ -50,7 +50,9 @@ extern "C" {
 
 /* arithmetic of the channel GEMMs */
 #define IMSRG_PREC_FP64 0  /* every product and sum in FP64 */
-#define IMSRG_PREC_MIXED 1 /* pp/hh and ph GEMMs in FP32 MFMA (split operands), rest FP64 */
+#define IMSRG_PREC_MIXED 1 /* pp/hh and ph GEMMs: FP32-rounded operands on FP32 MFMA,
+                              FP32 partial sums over 16 k flushed into FP64
+                              accumulators; everything else FP64 */
 
 typedef struct imsrg_ctx imsrg_ctx; /* basis tables, 6j table, stream, workspaces */
 typedef struct imsrg_dop imsrg_dop; /* device-resident operator */
@@ -172,11 +174,12 @@ int imsrg_hip_magnus_flow(imsrg_ctx *ctx, const imsrg_dop *H0, imsrg_dop *Hs, do
 int imsrg_hip_set_profiling(imsrg_ctx *ctx, int on);
 int imsrg_hip_phase_times(imsrg_ctx *ctx, double *us, int n);
 
-/* Diagnostic with no counterpart in the reference: out512[0..255] = D of one
- * v_mfma_f64_16x16x4f64 with A = lane id, B = 1; out512[256..511] with A = 1,
- * B = lane id (lane l, register q at index 4l+q).  Used by the tests to pin the
- * MFMA lane layout the GEMM kernels assume. */
-int imsrg_hip_selftest_mfma(imsrg_ctx *ctx, double *out512);
+/* Diagnostic with no counterpart in the reference: out1024[0..255] = D of one
+ * v_mfma_f64_16x16x4f64 with A = lane id, B = 1; out1024[256..511] with A = 1,
+ * B = lane id (lane l, register q at index 4l+q); out1024[512..1023] the same
+ * two products on v_mfma_f32_16x16x4f32.  Used by the tests to pin the MFMA
+ * lane layouts the GEMM kernels assume. */
+int imsrg_hip_selftest_mfma(imsrg_ctx *ctx, double *out1024);
 
 #ifdef __cplusplus
 }

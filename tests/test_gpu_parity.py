@@ -75,6 +75,14 @@ def test_mfma_f64_lane_layout(ia):
         for q in range(4):
             assert rows[4 * l + q] == 4 * q + l // 16
             assert cols[4 * l + q] == l % 16
+    # v_mfma_f32_16x16x4f32: same operand layout, D[m][n] in lane n + 16*(m // 4),
+    # register m % 4
+    rows32 = (out[512:768] - 96) / 4
+    cols32 = (out[768:] - 96) / 4
+    for l in range(64):
+        for q in range(4):
+            assert rows32[4 * l + q] == 4 * (l // 16) + q
+            assert cols32[4 * l + q] == l % 16
 
 
 @pytest.mark.parametrize("emax", [2, 4, 6])
