@@ -28,14 +28,18 @@ PHASES = ["zerobody", "onebody", "comm122", "pphh_gemm", "pandya_gather", "ph_ge
 
 
 def algorithmic_counts(ms):
-    """Per-commutator algorithmic work of the reference algorithm:
-    F_A = sum_ch 2 n^2 (n_pp + 2 n_hh + n_ph)    pp/hh GEMMs (src/Commutator.cc:1176-1183)
-    F_C = sum_cc 8 c^2 n_ph,cc                    ph GEMM (src/Commutator.cc:1789)
-    B_B = 8 [2 sum n^2 + 4 sum c 2n_ph + 2 sum 2c^2 + 2 sum n^2]  Pandya-phase bytes
-    plus the kernel-level byte counts used for the gather kernels."""
-    occ = [o["occ"] for o in ms.orbits]
+    """Per-commutator algorithmic work of the reference algorithm over the
+    basis tables (src/ModelSpace.cc:1120-1201, src/TwoBodyChannel.cc:142-188):
+      F_A = sum_ch 2 n^2 (n_pp + 2 n_hh + n_ph)   pp/hh GEMMs (src/Commutator.cc:1176-1183)
+      F_C = sum_cc 8 c^2 n_ph,cc                   ph GEMM (src/Commutator.cc:1789)
+      B_I = 8 (sum_cc 2 c^2 + sum_ch n(n+1)/2)     inverse Pandya: one read of Zbar,
+                                                   one store per upper Z element
+    Returns (F_A, F_A executed, F_C, B_I)."""
+    orb = ms.orbits
+    occ = [o["occ"] for o in orb]
     FA = 0
     executed_A = 0
+    upper = 0
     for ch, c in enumerate(ms.channels):
         n = c["n"]
         if n == 0:
@@ -51,7 +55,34 @@ def algorithmic_counts(ms):
                 npp += 1
         FA += 2 * n * n * (npp + 2 * nhh + nph)
         executed_A += 2 * n * n * (npp + nhh)
-    return FA, executed_A
+        upper += n * (n + 1) // 2
+    # cross-coupled channels (J', parity, |tz_p - tz_q|/2): kets p <= q, no Pauli rule
+    cc = {}
+    for p in range(len(orb)):
+        for q in range(p, len(orb)):
+            a, b = orb[p], orb[q]
+            key0 = ((a["l"] + b["l"]) & 1, abs(a["tz2"] - b["tz2"]) // 2)
+            ph = (occ[p] > 1e-6) or (occ[q] > 1e-6)  # the reference's CC ph list holds hh and ph kets
+            for J in range(abs(a["j2"] - b["j2"]) // 2, (a["j2"] + b["j2"]) // 2 + 1):
+                e = cc.setdefault((J,) + key0, [0, 0])
+                e[0] += 1
+                e[1] += ph
+    FC = sum(8 * c * c * nph for c, nph in cc.values() if nph > 0)
+    BI = 8 * (sum(2 * c * c for c, _ in cc.values()) + upper)
+    return FA, executed_A, FC, BI
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE scaled by a plain-copy calibration), or
+    None when no summary for it exists."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t["kernels"][kernel]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def main():
@@ -102,6 +133,16 @@ def main():
 
     fp64_rate, t_fp64, phase64, Zref = timed(ia.PREC_FP64)
     mixed_rate, t_mixed, phasemx, Zmx = timed(ia.PREC_MIXED)
+
+    # host-buffer drop-in (operators in host memory, PCIe transfers included);
+    # reported beside the device-resident rate, never as the value
+    hX, hY = X.to_host(), Y.to_host()
+    ia.Commutator.CommutatorScalarScalar(hX, hY)
+    nh = 3
+    t0 = time.perf_counter()
+    for _ in range(nh):
+        ia.Commutator.CommutatorScalarScalar(hX, hY)
+    host_rate = nh / (time.perf_counter() - t0)
     scale = max(np.max(np.abs(Zref.TwoBody)), np.max(np.abs(Zref.OneBody)))
     rel_dz = float(max(np.max(np.abs(Zmx.TwoBody - Zref.TwoBody)), np.max(np.abs(Zmx.OneBody - Zref.OneBody)))
                    / scale)
@@ -115,18 +156,27 @@ def main():
     phase_avg = phasemx if use_mixed else phase64
 
     # ---- roofline of the dominant kernel of the reported run ----
-    FA, execA = algorithmic_counts(ms)
+    # (phase durations are HIP events recorded on the library's stream around
+    # each launch during the timed steps)
+    FA, execA, FC, BI = algorithmic_counts(ms)
     dom = int(np.argmax(phase_avg))
     t_dom = phase_avg[dom] * 1e-6
     if PHASES[dom] == "pphh_gemm":
         peak = PEAK_FP32_MFMA if use_mixed else PEAK_FP64_MFMA
+        kname = "k_pphh_big_mixed" if use_mixed else "k_pphh_big_f64"
         ach = FA / t_dom
-        roof = {"kernel": "k_pphh_big (pp/hh SYR2K, %s MFMA)" % ("FP32" if use_mixed else "FP64"), "bound": "mfma",
-                "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s", "frac": ach / peak, "traffic": None,
-                "algorithmic_flops": FA, "executed_flops": execA}
+        roof = {"kernel": kname, "bound": "mfma", "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+                "frac": ach / peak, "traffic": profiled_traffic(kname), "algorithmic_flops": FA,
+                "executed_flops": execA}
+    elif PHASES[dom] == "inverse_pandya":
+        ach = BI / t_dom
+        roof = {"kernel": "k_invpandya", "bound": "hbm", "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9,
+                "unit": "GB/s", "frac": ach / PEAK_HBM, "traffic": profiled_traffic("k_invpandya"),
+                "algorithmic_bytes": BI}
     else:
         roof = {"kernel": PHASES[dom], "bound": "hbm", "achieved": None, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                 "frac": None, "traffic": None}
+    roof["launch_us"] = phase_avg[dom]
 
     line = {
         "metric": "[X,Y] commutators/sec at emax=10: mixed vs FP64 GPU, at |dE0|<1e-6",
@@ -148,6 +198,7 @@ def main():
         "mixed_commutators_per_s": mixed_rate,
         "mixed_vs_fp64": {"speedup": mixed_rate / fp64_rate, "max_abs_dZ_rel": rel_dz, "abs_dE0": de0,
                           "meets_bar": mixed_ok},
+        "host_buffer_fp64_commutators_per_s": host_rate,
         "phase_us": dict(zip(PHASES, [round(x, 2) for x in phase_avg])),
         "phase_us_fp64": dict(zip(PHASES, [round(x, 2) for x in phase64])),
         "roofline": roof,

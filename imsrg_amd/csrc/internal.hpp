@@ -12,8 +12,8 @@ constexpr int kTileB = 128;       // output tile edge of the large pp/hh SYR2K
 constexpr int kTileK = 16;        // k-depth of one LDS stage
 constexpr int kChunk = 256;       // elements per work item of the gather kernels
 constexpr int kMax122 = 32;       // max one-body partners of a ket in comm122
-constexpr int kJC = 4;
-constexpr int kF32Flush = 4;      // MFMA k-steps (of 4) summed in FP32 before the FP64 flush (mixed precision)            // J values per register chunk of the inverse Pandya gather
+constexpr int kJC = 4;            // J values per register chunk of the inverse Pandya gather
+constexpr int kF32Flush = 4;      // MFMA k-steps (of 4) summed in FP32 before the FP64 flush (mixed precision)
 
 // Term bits (same values as IMSRG_COMM* in include/imsrg_hip.h)
 constexpr unsigned M110 = 1u << 0, M220 = 1u << 1, M111 = 1u << 2, M121 = 1u << 3, M221 = 1u << 4,

@@ -136,3 +136,58 @@ def test_white_generator_structure(oracle_ms_cache):
                 continue
             d = H["one"][i, i] - H["one"][a, a] + (ms.orbits[i]["occ"] - ms.orbits[a]["occ"]) * ms.monopole(H, i, a, i, a)
             assert abs(eta["one"][i, a] * d - H["one"][i, a]) < 1e-12
+
+
+def test_white_generator_two_body_reduces_to_moller_plesset(oracle_ms_cache):
+    # With a diagonal one-body part and two-body elements only between pp and
+    # hh kets, every monopole in the Epstein-Nesbet denominator vanishes and
+    # eta_2b(pp', hh') = H / (f_p + f_p' - f_h - f_h'); all other elements are 0.
+    ms = oracle_ms_cache(2)
+    rng = np.random.default_rng(17)
+    H = ms.zero_op(1)
+    f = rng.uniform(-20.0, 20.0, ms.norb)
+    for p in range(ms.norb):
+        H["one"][p, p] = f[p]
+    hole = [o["occ"] > 0.5 for o in ms.orbits]
+    expect = np.zeros(ms.tbsize)
+    for c in ms.channels:
+        n = c["n"]
+        if n == 0:
+            continue
+        blk = np.zeros((n, n))
+        eb = np.zeros((n, n))
+        for r, (a, b) in enumerate(c["kets"]):
+            for s, (k, l) in enumerate(c["kets"]):
+                if not hole[a] and not hole[b] and hole[k] and hole[l]:
+                    v = rng.normal()
+                    blk[r, s] = blk[s, r] = v
+                    e = v / (f[a] + f[b] - f[k] - f[l])
+                    eb[r, s], eb[s, r] = e, -e
+        H["two"][c["off"]:c["off"] + n * n] = blk.reshape(-1, order="F")
+        expect[c["off"]:c["off"] + n * n] = eb.reshape(-1, order="F")
+    assert np.count_nonzero(expect) > 0
+    eta = ms.white_generator(H)
+    assert np.max(np.abs(eta["one"])) == 0.0
+    assert np.max(np.abs(eta["two"] - expect)) < 1e-14 * np.max(np.abs(expect))
+
+
+def test_magnus_flow_of_one_body_hamiltonian_is_unitary(oracle_ms_cache):
+    # With no two-body part the flow is an exact unitary rotation of the
+    # one-body matrix: each (l, j, tz) block keeps its spectrum, and the
+    # constant of the un-normal-ordered operator, E0 - sum_a n_a (2j_a+1) f_aa,
+    # is conserved.
+    ms = oracle_ms_cache(2)
+    H = synthetic_hamiltonian(ms, 23, offdiag_sigma=2.0)
+    H["two"][:] = 0.0
+    Hs, e0, en, nc = ms.magnus_flow(H, ds=0.5, smax=20.0, max_steps=40)
+    assert len(e0) > 3 and en[len(e0)] < en[0]
+    for blk in one_body_blocks(ms.orbits):
+        a = np.linalg.eigvalsh(H["one"][np.ix_(blk, blk)])
+        b = np.linalg.eigvalsh(Hs["one"][np.ix_(blk, blk)])
+        assert np.max(np.abs(a - b)) < 1e-8 * np.max(np.abs(a))
+
+    def const(op):
+        return op["zero"] - sum(o["occ"] * (o["j2"] + 1) * op["one"][i, i] for i, o in enumerate(ms.orbits))
+
+    assert abs(const(Hs) - const(H)) < 1e-8 * abs(const(H))
+    assert abs(Hs["zero"] - H["zero"]) > 1e-3  # the flow did move E0
