@@ -6,10 +6,13 @@ N(0,1) entries (synthetic data; the library's counter-based generator).
 Operators are resident in HBM before the timed region; each step is one full
 commutator (all eight IMSRG(2) terms).
 
-Prints one JSON line (see README/DESIGN for the fields).  ``roofline`` is for
-the dominant kernel, from HIP events recorded on the library stream during
-the timed steps; ``cpu_baseline`` times the oracle (C/OpenMP restatement) on
-one commutator of the same workload on this host.
+Prints one JSON line.  ``value`` is the mixed-precision rate when the mixed
+result meets max|dZ|/max|Z| < 1e-7 and |dE0| < 1e-6 against the FP64 result,
+else the FP64 rate; both rates are reported.  ``roofline`` is for the dominant
+kernel, from HIP events recorded on the library stream during the timed steps;
+``cpu_baseline`` times the oracle (C/OpenMP restatement) on one commutator of
+the same workload on this host.  ``--workload`` selects the other measured
+workloads (batched pairs, emax=12 precision, Magnus flow), one JSON line each.
 """
 import argparse
 import json
@@ -85,6 +88,141 @@ def profiled_traffic(kernel):
         return None
 
 
+def _errors(Z, R, np):
+    scale = max(np.max(np.abs(R.TwoBody)), np.max(np.abs(R.OneBody)))
+    rel = float(max(np.max(np.abs(Z.TwoBody - R.TwoBody)), np.max(np.abs(Z.OneBody - R.OneBody))) / scale)
+    return rel, abs(Z.ZeroBody - R.ZeroBody)
+
+
+def workload_batched(args, ia, np):
+    """48 independent pairs, emax drawn uniformly from {4,5,6,7} (seed 3000),
+    pair i seeded 3000+2i+1 / 3000+2i+2: one grouped launch sequence for all
+    pairs against a loop of single-pair commutators."""
+    rng = np.random.default_rng(3000)
+    emaxes = [int(e) for e in rng.integers(4, 8, size=48)]
+    spaces = {e: ia.ModelSpace(e, "O16", device=0) for e in sorted(set(emaxes))}
+    pairs = []
+    for i, e in enumerate(emaxes):
+        ms = spaces[e]
+        pairs.append((ia.DeviceOperator(ms, -1).fill_random(-1, 3000 + 2 * i + 1),
+                      ia.DeviceOperator(ms, 1).fill_random(1, 3000 + 2 * i + 2), ia.DeviceOperator(ms, 1)))
+
+    def sync():
+        for ms in spaces.values():
+            ms.synchronize()
+
+    reps = max(args.steps, 5)
+    out = {}
+    for name, prec in (("fp64", ia.PREC_FP64), ("mixed", ia.PREC_MIXED)):
+        for _ in range(2):
+            ia.Commutator.CommutatorScalarScalarBatched(pairs, precision=prec)
+            for X, Y, Z in pairs:
+                ia.Commutator.CommutatorScalarScalar(X, Y, precision=prec, out=Z)
+        sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ia.Commutator.CommutatorScalarScalarBatched(pairs, precision=prec)
+        sync()
+        tb = time.perf_counter() - t0
+        ref = [Z.to_host() for _, _, Z in pairs]
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            for X, Y, Z in pairs:
+                ia.Commutator.CommutatorScalarScalar(X, Y, precision=prec, out=Z)
+        sync()
+        tl = time.perf_counter() - t0
+        same = all(np.array_equal(r.TwoBody, Z.to_host().TwoBody) for r, (_, _, Z) in zip(ref, pairs))
+        out[name] = {"batched_pairs_per_s": 48 * reps / tb, "looped_pairs_per_s": 48 * reps / tl,
+                     "speedup": tl / tb, "batched_equals_looped": bool(same)}
+    return {"metric": "independent [X,Y] pairs/sec, 48 pairs with emax in {4,5,6,7}: one grouped launch vs a loop",
+            "value": out["fp64"]["batched_pairs_per_s"], "unit": "pairs/s", "n_gpus": 1, "steps": reps,
+            "higher_is_better": True, "dtype": "f64", "data": "synthetic (seeded, library generator)",
+            "config": {"workload": "48 scalar IMSRG(2) commutator pairs, O16 reference, emax drawn from {4,5,6,7}",
+                       "emax_counts": {e: emaxes.count(e) for e in sorted(set(emaxes))}},
+            "fp64": out["fp64"], "mixed": out["mixed"]}
+
+
+def workload_emax12(args, ia, np):
+    """One commutator at emax=12 (182 orbits): FP64 and mixed rates and the
+    mixed-precision error against the FP64 GPU result."""
+    ms = ia.ModelSpace(12, "O16", device=0)
+    X = ia.DeviceOperator(ms, -1).fill_random(-1, 12001)
+    Y = ia.DeviceOperator(ms, 1).fill_random(1, 12002)
+    Z = ia.DeviceOperator(ms, 1)
+    res = {}
+    hosts = {}
+    for name, prec in (("fp64", ia.PREC_FP64), ("mixed", ia.PREC_MIXED)):
+        ms.set_profiling(True)
+        ia.Commutator.CommutatorScalarScalar(X, Y, precision=prec, out=Z)
+        ms.synchronize()
+        steps = max(3, min(args.steps, 5))
+        phase = np.zeros(8)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ia.Commutator.CommutatorScalarScalar(X, Y, precision=prec, out=Z)
+            phase += np.array(ms.phase_times())
+        ms.synchronize()
+        t = time.perf_counter() - t0
+        ms.set_profiling(False)
+        hosts[name] = Z.to_host()
+        res[name] = {"commutators_per_s": steps / t, "phase_us": dict(zip(PHASES, [round(x, 1) for x in phase / steps]))}
+    rel, de0 = _errors(hosts["mixed"], hosts["fp64"], np)
+    d = np.abs(hosts["mixed"].TwoBody - hosts["fp64"].TwoBody)
+    return {"metric": "mixed-precision error at emax=12 vs the FP64 GPU result", "value": rel,
+            "unit": "max-abs dZ / max |Z|", "n_gpus": 1, "higher_is_better": False, "dtype": "f32 MFMA + f64",
+            "data": "synthetic (seeds 12001/12002)",
+            "config": {"workload": "scalar IMSRG(2) commutator, HO emax=12, O16 reference", "norb": ms.norb,
+                       "two_body_doubles": ms.tbsize},
+            "abs_dE0": de0, "max_abs_dZ": float(np.max(d)),
+            "max_rel_dZ_elementwise_over_1e-3_max": float(np.max(d / np.maximum(np.abs(hosts["fp64"].TwoBody),
+                                                                                1e-3 * np.max(np.abs(hosts["fp64"].TwoBody))))),
+            "fp64": res["fp64"], "mixed": res["mixed"]}
+
+
+def workload_flow(args, ia, np):
+    """Magnus-Euler IMSRG(2) flow at emax=10: one-body diagonal 10 (2n+l+1.5)
+    plus N(0, 0.5) off-diagonals inside each (l, j, tz) block, two-body
+    (R+R^T)/sqrt(2n) (seed 5001), White generator, ds = 0.5, BCH stopped at
+    nested norm < 1e-8; E0(s) of the mixed-precision flow against the FP64 one."""
+    ms = ia.ModelSpace(10, "O16", device=0)
+    H = ia.DeviceOperator(ms, 1).fill_random(1, 5001)
+    Hh = H.to_host()
+    rng = np.random.default_rng(5001)
+    Hh.OneBody[:] = 0.0
+    Hh.ZeroBody = 0.0
+    blocks = {}
+    for i, o in enumerate(ms.orbits):
+        blocks.setdefault((o["l"], o["j2"], o["tz2"]), []).append(i)
+    for blk in blocks.values():
+        for x, a in enumerate(blk):
+            o = ms.orbits[a]
+            Hh.OneBody[a, a] = 10.0 * (2 * o["n"] + o["l"] + 1.5)
+            for b in blk[x + 1:]:
+                v = rng.normal(0.0, 0.5)
+                Hh.OneBody[a, b] = Hh.OneBody[b, a] = v
+    H.upload(Hh)
+    steps = 40
+    res = {}
+    for name, prec in (("fp64", ia.PREC_FP64), ("mixed", ia.PREC_MIXED)):
+        t0 = time.perf_counter()
+        Hs, e0, en, nc = ia.IMSRGSolver.Solve_magnus_euler(H, ds=0.5, smax=0.5 * steps, eta_criterion=1e-6,
+                                                           max_steps=steps, mode=1, transform_thr=1e-8,
+                                                           precision=prec)
+        t = time.perf_counter() - t0
+        res[name] = {"steps": len(e0), "seconds": t, "commutators": int(np.sum(nc)),
+                     "commutators_per_s": float(np.sum(nc)) / t, "E0_final": float(e0[-1]) if len(e0) else None,
+                     "eta_norm_final": float(en[len(e0)]) if len(en) > len(e0) else None,
+                     "E0": [float(x) for x in e0]}
+    n = min(len(res["fp64"]["E0"]), len(res["mixed"]["E0"]))
+    drift = max(abs(a - b) for a, b in zip(res["fp64"]["E0"][:n], res["mixed"]["E0"][:n])) if n else None
+    return {"metric": "Magnus IMSRG(2) flow at emax=10: E0(s) drift of the mixed-precision flow vs FP64",
+            "value": drift, "unit": "max_s |E0_mixed(s) - E0_fp64(s)|", "n_gpus": 1, "higher_is_better": False,
+            "dtype": "f32 MFMA + f64", "data": "synthetic Hamiltonian (seed 5001)",
+            "config": {"workload": "Magnus-Euler flow, White generator, ds=0.5, BCH nested-norm cut 1e-8, "
+                                   "HO emax=10, O16 reference", "max_steps": steps},
+            "fp64": res["fp64"], "mixed": res["mixed"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,15 +230,24 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--emax", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="commutator", choices=["commutator", "batched", "emax12", "flow"],
+                    help="commutator: the headline metric line; batched: 48 independent pairs at emax 4-7; "
+                         "emax12: mixed-precision error report at emax=12; flow: Magnus flow at emax=10")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one MI355X: this build has no multi-GPU path (see DESIGN.md, sharding)",
-              file=sys.stderr)
+        print("bench.py measures one MI355X; runs on more than one GPU are not part of this build "
+              "(independent operators shard one process per GPU, see the sharding notes)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
 
     import imsrg_amd as ia
+
+    if args.workload != "commutator":
+        line = {"batched": workload_batched, "emax12": workload_emax12, "flow": workload_flow}[args.workload](
+            args, ia, np)
+        print(json.dumps(line), flush=True)
+        return
 
     emax = args.emax
     ms = ia.ModelSpace(emax, "O16", device=0)
