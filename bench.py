@@ -213,6 +213,36 @@ def workload_flow(args, ia, np):
                      "commutators_per_s": float(np.sum(nc)) / t, "E0_final": float(e0[-1]) if len(e0) else None,
                      "eta_norm_final": float(en[len(e0)]) if len(en) > len(e0) else None,
                      "E0": [float(x) for x in e0]}
+    # the same FP64 flow with every nested commutator replayed from a captured hipGraph
+    ms.set_graphs(True)
+    t0 = time.perf_counter()
+    Hs, e0g, en, nc = ia.IMSRGSolver.Solve_magnus_euler(H, ds=0.5, smax=0.5 * steps, eta_criterion=1e-6,
+                                                        max_steps=steps, mode=1, transform_thr=1e-8,
+                                                        precision=ia.PREC_FP64)
+    tg = time.perf_counter() - t0
+    ms.set_graphs(False)
+    res["fp64_graphs"] = {"seconds": tg, "commutators_per_s": float(np.sum(nc)) / tg,
+                          "E0_equal_to_eager": bool(np.array_equal(e0g, np.array(res["fp64"]["E0"])))}
+    # launch overhead matters at small sizes: the same flow recipe at emax=4
+    ms4 = ia.ModelSpace(4, "O16", device=0)
+    H4 = ia.DeviceOperator(ms4, 1).fill_random(1, 5001)
+    H4h = H4.to_host()
+    H4h.OneBody[:] = 0.0
+    for i, o in enumerate(ms4.orbits):
+        H4h.OneBody[i, i] = 10.0 * (2 * o["n"] + o["l"] + 1.5)
+    H4.upload(H4h)
+    small = {}
+    for name, on in (("eager", False), ("graphs", True)):
+        ms4.set_graphs(on)
+        ia.IMSRGSolver.Solve_magnus_euler(H4, ds=0.5, smax=2.0, max_steps=4, mode=1, transform_thr=1e-8)
+        t0 = time.perf_counter()
+        _, e04, _, nc4 = ia.IMSRGSolver.Solve_magnus_euler(H4, ds=0.5, smax=0.5 * steps, eta_criterion=1e-6,
+                                                           max_steps=steps, mode=1, transform_thr=1e-8)
+        t4 = time.perf_counter() - t0
+        small[name] = {"seconds": t4, "steps": len(e04), "commutators": int(np.sum(nc4)),
+                       "commutators_per_s": float(np.sum(nc4)) / t4}
+    ms4.set_graphs(False)
+    res["emax4_flow"] = small
     n = min(len(res["fp64"]["E0"]), len(res["mixed"]["E0"]))
     drift = max(abs(a - b) for a, b in zip(res["fp64"]["E0"][:n], res["mixed"]["E0"][:n])) if n else None
     return {"metric": "Magnus IMSRG(2) flow at emax=10: E0(s) drift of the mixed-precision flow vs FP64",
@@ -220,7 +250,8 @@ def workload_flow(args, ia, np):
             "dtype": "f32 MFMA + f64", "data": "synthetic Hamiltonian (seed 5001)",
             "config": {"workload": "Magnus-Euler flow, White generator, ds=0.5, BCH nested-norm cut 1e-8, "
                                    "HO emax=10, O16 reference", "max_steps": steps},
-            "fp64": res["fp64"], "mixed": res["mixed"]}
+            "fp64": res["fp64"], "mixed": res["mixed"], "fp64_graphs": res["fp64_graphs"],
+            "emax4_flow_eager_vs_graphs": res["emax4_flow"]}
 
 
 def main():

@@ -104,6 +104,10 @@ class ModelSpace:
         check(lib().imsrg_hip_phase_times(self.h, a, 8), self.h)
         return list(a)
 
+    def set_graphs(self, on=True):
+        """Replay repeated commutators on this space as captured hipGraphs."""
+        check(lib().imsrg_hip_set_graphs(self.h, 1 if on else 0), self.h)
+
     def selftest_mfma(self):
         out = np.zeros(1024)
         check(lib().imsrg_hip_selftest_mfma(self.h, out.ctypes.data_as(C.POINTER(C.c_double))), self.h)

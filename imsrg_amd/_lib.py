@@ -103,6 +103,7 @@ SIGNATURES = {
     "imsrg_hip_set_profiling": (C.c_int, [_vp, C.c_int]),
     "imsrg_hip_phase_times": (C.c_int, [_vp, _dp, C.c_int]),
     "imsrg_hip_selftest_mfma": (C.c_int, [_vp, _dp]),
+    "imsrg_hip_set_graphs": (C.c_int, [_vp, C.c_int]),
 }
 
 

@@ -42,6 +42,17 @@ struct BatchCache {
   }
 };
 
+// One captured commutator launch sequence (operator buffers, Hermiticity,
+// terms and precision are baked into its kernel arguments).
+struct GraphEntry {
+  const double *x, *y, *z;
+  int hx, hy;
+  unsigned mask;
+  int prec;
+  hipGraphExec_t exec;
+};
+constexpr size_t kMaxGraphs = 16;
+
 struct imsrg_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -62,6 +73,8 @@ struct imsrg_ctx {
   imsrg_dop *tmp[4] = {nullptr, nullptr, nullptr, nullptr};
   double *pinned = nullptr;  // host staging for views
   BatchCache batch;
+  bool use_graphs = false;
+  std::vector<GraphEntry> graphs;  // oldest first
 };
 
 static int fail(imsrg_ctx *ctx, int code, const std::string &msg) {
@@ -157,6 +170,8 @@ extern "C" int imsrg_hip_ctx_destroy(imsrg_ctx *ctx) {
   if (ctx->norm_host) (void)hipHostFree(ctx->norm_host);
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   for (auto &ev : ctx->ev) if (ev) (void)hipEventDestroy(ev);
+  for (auto &g : ctx->graphs) (void)hipGraphExecDestroy(g.exec);
+  ctx->graphs.clear();
   ctx->db.release();
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -476,7 +491,47 @@ static int commutator_dev(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y
   const int4 *items[7] = {db.d_pphh, db.d_ph, db.d_c122, db.d_pandya, db.d_invp, db.d_one, db.d_mirror};
   int cnt[7] = {(int)db.wl.pphh.size(), (int)db.wl.ph.size(), (int)db.wl.c122.size(), (int)db.wl.pandya.size(),
                 (int)db.wl.invp.size(), (int)db.wl.one.size(), (int)db.wl.mirror.size()};
-  return run_commutator(ctx, pa, 1, items, cnt, mask, prec, db.h.maxn, db.h.nch, db.h.n122);
+  if (!ctx->use_graphs || ctx->profiling)
+    return run_commutator(ctx, pa, 1, items, cnt, mask, prec, db.h.maxn, db.h.nch, db.h.n122);
+  // replay a captured launch sequence for this operand set, capturing it on first use
+  for (const GraphEntry &g : ctx->graphs)
+    if (g.x == X->buf && g.y == Y->buf && g.z == Z->buf && g.hx == X->herm && g.hy == Y->herm && g.mask == mask &&
+        g.prec == prec) {
+      HIPCHK(ctx, hipGraphLaunch(g.exec, ctx->stream));
+      return IMSRG_OK;
+    }
+  HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = run_commutator(ctx, pa, 1, items, cnt, mask, prec, db.h.maxn, db.h.nch, db.h.n122);
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);  // always leave capture mode
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  HIPCHK(ctx, ec);
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  HIPCHK(ctx, ei);
+  if (ctx->graphs.size() >= kMaxGraphs) {
+    (void)hipGraphExecDestroy(ctx->graphs.front().exec);
+    ctx->graphs.erase(ctx->graphs.begin());
+  }
+  ctx->graphs.push_back(GraphEntry{X->buf, Y->buf, Z->buf, X->herm, Y->herm, mask, prec, exec});
+  HIPCHK(ctx, hipGraphLaunch(exec, ctx->stream));
+  return IMSRG_OK;
+}
+
+extern "C" int imsrg_hip_set_graphs(imsrg_ctx *ctx, int on) {
+  if (!ctx) return IMSRG_E_INVALID;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  if (!on) {
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    for (auto &g : ctx->graphs) (void)hipGraphExecDestroy(g.exec);
+    ctx->graphs.clear();
+  }
+  ctx->use_graphs = on != 0;
+  return IMSRG_OK;
 }
 
 extern "C" int imsrg_hip_commutator_ss_dev(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y, imsrg_dop *Z,

@@ -168,6 +168,14 @@ int imsrg_hip_magnus_flow(imsrg_ctx *ctx, const imsrg_dop *H0, imsrg_dop *Hs, do
                           double eta_criterion, double omega_norm_max, const imsrg_bch_opts *opts, int max_steps,
                           double *e0, double *eta_norm, int *ncomm, int *nsteps);
 
+/* No counterpart in the reference.  on != 0: every device commutator on ctx
+ * (single calls and the nested commutators of BCH_Transform, BCH_Product and
+ * the Magnus flow) replays a hipGraph of its launch sequence, captured on the
+ * first call for that set of operand buffers, Hermiticity flags, terms and
+ * precision (up to 16 kept per context, oldest dropped).  on == 0 drops the
+ * graphs.  Ignored while profiling is on. */
+int imsrg_hip_set_graphs(imsrg_ctx *ctx, int on);
+
 /* Per-phase timing of the last commutator on ctx (microseconds, HIP events):
  * [0] zero-body, [1] one-body, [2] comm122, [3] pp/hh GEMM, [4] Pandya gather,
  * [5] ph GEMM, [6] inverse Pandya, [7] mirror.  Enabled by imsrg_hip_set_profiling. */
