@@ -309,7 +309,10 @@ def main():
         ms.set_profiling(False)
         return args.steps / t, t, phase_sum / args.steps, Z.to_host()
 
+    # launch settings chosen by the autotuner on this basis, per precision (set-up, untimed)
+    tuning = {"fp64": ms.autotune(ia.PREC_FP64)}
     fp64_rate, t_fp64, phase64, Zref = timed(ia.PREC_FP64)
+    tuning["mixed"] = ms.autotune(ia.PREC_MIXED)
     mixed_rate, t_mixed, phasemx, Zmx = timed(ia.PREC_MIXED)
 
     # host-buffer drop-in (operators in host memory, PCIe transfers included);
@@ -377,6 +380,7 @@ def main():
         "mixed_vs_fp64": {"speedup": mixed_rate / fp64_rate, "max_abs_dZ_rel": rel_dz, "abs_dE0": de0,
                           "meets_bar": mixed_ok},
         "host_buffer_fp64_commutators_per_s": host_rate,
+        "tuning": tuning,
         "phase_us": dict(zip(PHASES, [round(x, 2) for x in phase_avg])),
         "phase_us_fp64": dict(zip(PHASES, [round(x, 2) for x in phase64])),
         "roofline": roof,

@@ -108,6 +108,26 @@ class ModelSpace:
         """Replay repeated commutators on this space as captured hipGraphs."""
         check(lib().imsrg_hip_set_graphs(self.h, 1 if on else 0), self.h)
 
+    def get_tuning(self):
+        t = _lib.Tuning()
+        check(lib().imsrg_hip_get_tuning(self.h, C.byref(t)), self.h)
+        return {f: getattr(t, f) for f, _ in _lib.Tuning._fields_}
+
+    def set_tuning(self, **kw):
+        t = _lib.Tuning()
+        check(lib().imsrg_hip_get_tuning(self.h, C.byref(t)), self.h)
+        for f, v in kw.items():
+            setattr(t, f, v)
+        check(lib().imsrg_hip_set_tuning(self.h, C.byref(t)), self.h)
+
+    def autotune(self, precision=PREC_FP64, cache_path=None):
+        """Time the launch settings on this basis and apply the fastest (see
+        imsrg_hip_autotune); returns the chosen setting."""
+        t = _lib.Tuning()
+        path = cache_path.encode() if cache_path else None
+        check(lib().imsrg_hip_autotune(self.h, precision, path, C.byref(t)), self.h)
+        return {f: getattr(t, f) for f, _ in _lib.Tuning._fields_}
+
     def selftest_mfma(self):
         out = np.zeros(1024)
         check(lib().imsrg_hip_selftest_mfma(self.h, out.ctypes.data_as(C.POINTER(C.c_double))), self.h)

@@ -52,6 +52,11 @@ class BCHOpts(C.Structure):
                 ("max_terms", C.c_int), ("precision", C.c_int)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("pphh_small_tile_max", C.c_int), ("pphh_order", C.c_int), ("mixed_pphh_fp32", C.c_int),
+                ("mixed_ph_fp32", C.c_int), ("us_per_commutator", C.c_double)]
+
+
 class HipError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"{ERRORS.get(code, code)}: {msg}")
@@ -104,6 +109,9 @@ SIGNATURES = {
     "imsrg_hip_phase_times": (C.c_int, [_vp, _dp, C.c_int]),
     "imsrg_hip_selftest_mfma": (C.c_int, [_vp, _dp]),
     "imsrg_hip_set_graphs": (C.c_int, [_vp, C.c_int]),
+    "imsrg_hip_get_tuning": (C.c_int, [_vp, C.POINTER(Tuning)]),
+    "imsrg_hip_set_tuning": (C.c_int, [_vp, C.POINTER(Tuning)]),
+    "imsrg_hip_autotune": (C.c_int, [_vp, C.c_int, C.c_char_p, C.POINTER(Tuning)]),
 }
 
 

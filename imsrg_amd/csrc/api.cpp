@@ -75,6 +75,7 @@ struct imsrg_ctx {
   BatchCache batch;
   bool use_graphs = false;
   std::vector<GraphEntry> graphs;  // oldest first
+  long lists_gen = 0;              // bumped whenever the work lists are rebuilt
 };
 
 static int fail(imsrg_ctx *ctx, int code, const std::string &msg) {
@@ -428,8 +429,15 @@ static long r8(long n) { return (n + 7) / 8 * 8; }  // 64-byte aligned segment l
 static int herm_rule(int hx, int hy) { return (hx == hy) ? -1 : 1; }  // src/Commutator.cc:292-297
 
 // Launch sequence of one commutator (or a batch when pa.tab is set).
+// items / cnt: [0] pp/hh 128-tiles, [1] ph tiles, [2] comm122 columns, [3] Pandya
+// chunks, [4] inverse-Pandya items, [5] one-body pairs, [6] mirror tiles,
+// [7] pp/hh 64-tiles.  Under mixed precision the context's tuning picks the
+// FP32 or FP64 kernel of each GEMM family.
 static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const int4 *const *items, const int *cnt,
                           unsigned mask, int prec, int maxn, int maxnch, long maxn122) {
+  const Tuning &tu = ctx->db.tune;
+  const int prec_pphh = (prec == IMSRG_PREC_MIXED && tu.mixed_pphh) ? IMSRG_PREC_MIXED : IMSRG_PREC_FP64;
+  const int prec_ph = (prec == IMSRG_PREC_MIXED && tu.mixed_ph) ? IMSRG_PREC_MIXED : IMSRG_PREC_FP64;
   hipStream_t s = ctx->stream;
   bool prof = ctx->profiling;
   auto mark = [&](int i) -> hipError_t { return prof ? hipEventRecord(ctx->ev[i], s) : hipSuccess; };
@@ -439,7 +447,7 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
   HIPCHK(ctx, launch_onebody(pa, items[5], cnt[5], s));
   HIPCHK(ctx, mark(2));
   if (mask & MPPHH)
-    HIPCHK(ctx, launch_pphh(pa, items[0], cnt[0], prec, s));
+    HIPCHK(ctx, launch_pphh(pa, items[0], cnt[0], items[7], cnt[7], prec_pphh, s));
   else
     HIPCHK(ctx, launch_zero_two(pa, items[6], cnt[6], s));
   HIPCHK(ctx, mark(3));
@@ -447,7 +455,7 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
   HIPCHK(ctx, mark(4));
   if (mask & MPH) HIPCHK(ctx, launch_pandya(pa, items[3], cnt[3], s));
   HIPCHK(ctx, mark(5));
-  if (mask & MPH) HIPCHK(ctx, launch_ph_gemm(pa, items[1], cnt[1], prec, s));
+  if (mask & MPH) HIPCHK(ctx, launch_ph_gemm(pa, items[1], cnt[1], prec_ph, s));
   HIPCHK(ctx, mark(6));
   if (mask & MPH) HIPCHK(ctx, launch_invpandya(pa, items[4], cnt[4], s));
   HIPCHK(ctx, mark(7));
@@ -488,9 +496,9 @@ static int commutator_dev(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y
   pa.p0.hZ = herm_rule(X->herm, Y->herm);
   pa.p0.mask = mask;
   Z->herm = pa.p0.hZ;
-  const int4 *items[7] = {db.d_pphh, db.d_ph, db.d_c122, db.d_pandya, db.d_invp, db.d_one, db.d_mirror};
-  int cnt[7] = {(int)db.wl.pphh.size(), (int)db.wl.ph.size(), (int)db.wl.c122.size(), (int)db.wl.pandya.size(),
-                (int)db.wl.invp.size(), (int)db.wl.one.size(), (int)db.wl.mirror.size()};
+  const int4 *items[8] = {db.d_pphh, db.d_ph, db.d_c122, db.d_pandya, db.d_invp, db.d_one, db.d_mirror, db.d_pphh64};
+  int cnt[8] = {(int)db.wl.pphh.size(), (int)db.wl.ph.size(), (int)db.wl.c122.size(), (int)db.wl.pandya.size(),
+                (int)db.wl.invp.size(), (int)db.wl.one.size(), (int)db.wl.mirror.size(), (int)db.wl.pphh64.size()};
   if (!ctx->use_graphs || ctx->profiling)
     return run_commutator(ctx, pa, 1, items, cnt, mask, prec, db.h.maxn, db.h.nch, db.h.n122);
   // replay a captured launch sequence for this operand set, capturing it on first use
@@ -814,6 +822,7 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
   for (int i = 0; i < npairs; ++i) {
     key.push_back(ctxs[i]); key.push_back(X[i]); key.push_back(Y[i]); key.push_back(Z[i]);
     key.push_back((const void *)(intptr_t)(X[i]->herm * 4 + Y[i]->herm + 8 * (int)mask));
+    key.push_back((const void *)(intptr_t)ctxs[i]->lists_gen);  // work lists rebuilt by a new tuning
   }
   if (key != bc.key) {
     bc.release();
@@ -844,12 +853,13 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
     HIPCHK(c0, hipMalloc(&bc.d_tab, sizeof(PairPtrs) * npairs));
     HIPCHK(c0, hipMemcpy(bc.d_tab, tab.data(), sizeof(PairPtrs) * npairs, hipMemcpyHostToDevice));
     // concatenated lists; pp/hh tiles of all pairs go through the XCD-balanced order
-    std::vector<int4> all[7];
+    std::vector<int4> all[8];
     std::vector<TileGroup> groups;
     for (int i = 0; i < npairs; ++i) {
       const WorkLists &wl = ctxs[i]->db.wl;
-      const std::vector<int4> *src[7] = {&wl.pphh, &wl.ph, &wl.c122, &wl.pandya, &wl.invp, &wl.one, &wl.mirror};
-      for (int t = 1; t < 7; ++t)
+      const std::vector<int4> *src[8] = {&wl.pphh, &wl.ph, &wl.c122, &wl.pandya, &wl.invp, &wl.one, &wl.mirror,
+                                         &wl.pphh64};
+      for (int t = 1; t < 8; ++t)
         for (int4 w : *src[t]) {
           w.x = i;
           all[t].push_back(w);
@@ -868,15 +878,22 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
       bc.maxnch = std::max(bc.maxnch, ctxs[i]->db.h.nch);
       bc.maxn122 = std::max(bc.maxn122, ctxs[i]->db.h.n122);
     }
-    all[0] = xcd_order(groups, kXcd);
+    if (c0->db.tune.pphh_order == 1) {
+      all[0] = xcd_order(groups, kXcd);
+    } else {  // groups by decreasing k-length, as for a single commutator
+      std::stable_sort(groups.begin(), groups.end(), [](const TileGroup &a, const TileGroup &b) {
+        return a.cost / (long)a.tiles.size() > b.cost / (long)b.tiles.size();
+      });
+      for (auto &g : groups) all[0].insert(all[0].end(), g.tiles.begin(), g.tiles.end());
+    }
     size_t tot = 0;
-    for (int t = 0; t < 7; ++t) {
+    for (int t = 0; t < 8; ++t) {
       bc.off[t] = tot;
       bc.cnt[t] = (int)all[t].size();
       tot += all[t].size();
     }
     HIPCHK(c0, hipMalloc(&bc.d_items, sizeof(int4) * std::max<size_t>(tot, 1)));
-    for (int t = 0; t < 7; ++t)
+    for (int t = 0; t < 8; ++t)
       if (!all[t].empty())
         HIPCHK(c0, hipMemcpy(bc.d_items + bc.off[t], all[t].data(), sizeof(int4) * all[t].size(),
                              hipMemcpyHostToDevice));
@@ -886,12 +903,176 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
   PairArg pa;
   std::memset(&pa.p0, 0, sizeof(pa.p0));
   pa.tab = bc.d_tab;
-  const int4 *items[7];
-  for (int t = 0; t < 7; ++t) items[t] = bc.d_items + bc.off[t];
+  const int4 *items[8];
+  for (int t = 0; t < 8; ++t) items[t] = bc.d_items + bc.off[t];
   return run_commutator(c0, pa, npairs, items, bc.cnt, mask, prec, bc.maxn, bc.maxnch, bc.maxn122);
 }
 
 // ---------------------------------------------------------------------------
+// launch tuning
+static void tuning_out(const Tuning &t, double us, imsrg_tuning *o) {
+  o->pphh_small_tile_max = t.pphh_small;
+  o->pphh_order = t.pphh_order;
+  o->mixed_pphh_fp32 = t.mixed_pphh;
+  o->mixed_ph_fp32 = t.mixed_ph;
+  o->us_per_commutator = us;
+}
+
+static int apply_tuning(imsrg_ctx *ctx, const Tuning &t) {
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  for (auto &g : ctx->graphs) (void)hipGraphExecDestroy(g.exec);  // their work lists are gone
+  ctx->graphs.clear();
+  ctx->batch.release();
+  const std::string e = ctx->db.build_lists(t);
+  if (!e.empty()) return fail(ctx, IMSRG_E_HIP, e);
+  ++ctx->lists_gen;
+  return IMSRG_OK;
+}
+
+extern "C" int imsrg_hip_get_tuning(imsrg_ctx *ctx, imsrg_tuning *t) {
+  if (!ctx || !t) return IMSRG_E_INVALID;
+  tuning_out(ctx->db.tune, 0.0, t);
+  return IMSRG_OK;
+}
+
+extern "C" int imsrg_hip_set_tuning(imsrg_ctx *ctx, const imsrg_tuning *t) {
+  if (!ctx || !t) return IMSRG_E_INVALID;
+  if (t->pphh_small_tile_max < 0 || (t->pphh_order != 0 && t->pphh_order != 1) ||
+      (t->mixed_pphh_fp32 != 0 && t->mixed_pphh_fp32 != 1) || (t->mixed_ph_fp32 != 0 && t->mixed_ph_fp32 != 1))
+    return fail(ctx, IMSRG_E_INVALID, "invalid tuning values");
+  Tuning n;
+  n.pphh_small = t->pphh_small_tile_max;
+  n.pphh_order = t->pphh_order;
+  n.mixed_pphh = t->mixed_pphh_fp32;
+  n.mixed_ph = t->mixed_ph_fp32;
+  return apply_tuning(ctx, n);
+}
+
+// cache key: basis (orbits and occupations), precision, GPU architecture
+static std::string tuning_key(imsrg_ctx *ctx, int precision) {
+  unsigned long long h = 1469598103934665603ull;  // FNV-1a over the orbit table
+  auto mix = [&](long long v) {
+    for (int b = 0; b < 8; ++b) {
+      h ^= (unsigned long long)((v >> (8 * b)) & 0xff);
+      h *= 1099511628211ull;
+    }
+  };
+  const HostBasis &hb = ctx->hb;
+  for (int i = 0; i < hb.norb; ++i) {
+    mix(hb.n[i]); mix(hb.l[i]); mix(hb.j2[i]); mix(hb.tz2[i]); mix(hb.cvq[i]);
+    mix((long long)std::llround(hb.occ[i] * 1e9));
+  }
+  hipDeviceProp_t prop;
+  std::string arch = "unknown";
+  if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) arch = prop.gcnArchName;
+  char buf[256];
+  std::snprintf(buf, sizeof(buf), "imsrg_hip_tuning norb=%d emax=%d basis=%016llx prec=%d arch=%s", hb.norb,
+                hb.emax, h, precision, arch.c_str());
+  return buf;
+}
+
+extern "C" int imsrg_hip_autotune(imsrg_ctx *ctx, int precision, const char *cache_path, imsrg_tuning *out) {
+  if (!ctx) return IMSRG_E_INVALID;
+  if (precision != IMSRG_PREC_FP64 && precision != IMSRG_PREC_MIXED)
+    return fail(ctx, IMSRG_E_INVALID, "unknown precision mode");
+  const std::string key = tuning_key(ctx, precision);
+  if (cache_path) {
+    if (FILE *f = std::fopen(cache_path, "r")) {
+      char line[512];
+      Tuning t;
+      double us = 0.0;
+      bool found = false;
+      while (std::fgets(line, sizeof(line), f)) {
+        if (std::strncmp(line, key.c_str(), key.size()) != 0 || line[key.size()] != ' ') continue;
+        if (std::sscanf(line + key.size(), " pphh_small=%d pphh_order=%d mixed_pphh=%d mixed_ph=%d us=%lf",
+                        &t.pphh_small, &t.pphh_order, &t.mixed_pphh, &t.mixed_ph, &us) == 5)
+          found = true;  // the last matching line wins
+      }
+      std::fclose(f);
+      if (found) {
+        int rc = apply_tuning(ctx, t);
+        if (rc) return rc;
+        if (out) tuning_out(t, us, out);
+        return IMSRG_OK;
+      }
+    }
+  }
+  // candidates: distinct 64-tile channel sets, both tile orders, and under
+  // mixed precision each GEMM family on FP32 or FP64 MFMA
+  std::vector<int> smalls = {0};
+  {
+    std::vector<int> ns(ctx->db.ch_n.begin(), ctx->db.ch_n.end());
+    for (int th : {64, 128, 256}) {
+      bool changes = false;
+      for (int n : ns)
+        if (n > smalls.back() && n <= th) changes = true;
+      if (changes) smalls.push_back(th);
+    }
+  }
+  std::vector<Tuning> cand;
+  for (int sm : smalls)
+    for (int order = 0; order < 2; ++order)
+      for (int mp = 0; mp < (precision == IMSRG_PREC_MIXED ? 2 : 1); ++mp)
+        for (int mh = 0; mh < (precision == IMSRG_PREC_MIXED ? 2 : 1); ++mh) {
+          Tuning t;
+          t.pphh_small = sm;
+          t.pphh_order = order;
+          t.mixed_pphh = precision == IMSRG_PREC_MIXED ? mp : 1;
+          t.mixed_ph = precision == IMSRG_PREC_MIXED ? mh : 1;
+          cand.push_back(t);
+        }
+  imsrg_dop *X = nullptr, *Y = nullptr, *Z = nullptr;
+  int rc = imsrg_hip_op_create(ctx, -1, &X);
+  if (!rc) rc = imsrg_hip_op_create(ctx, 1, &Y);
+  if (!rc) rc = imsrg_hip_op_create(ctx, 1, &Z);
+  if (!rc) rc = imsrg_hip_op_fill_random(X, -1, 911);
+  if (!rc) rc = imsrg_hip_op_fill_random(Y, 1, 912);
+  const bool graphs = ctx->use_graphs, prof = ctx->profiling;
+  ctx->use_graphs = false;
+  ctx->profiling = false;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = fail(ctx, IMSRG_E_HIP, "hipEventCreate failed");
+  Tuning best = ctx->db.tune;
+  double best_us = 1e300;
+  for (size_t c = 0; !rc && c < cand.size(); ++c) {
+    if ((rc = apply_tuning(ctx, cand[c]))) break;
+    if ((rc = commutator_dev(ctx, X, Y, Z, IMSRG_ALL_TERMS, precision))) break;  // warm-up
+    if (hipEventRecord(e0, ctx->stream) != hipSuccess) { rc = fail(ctx, IMSRG_E_HIP, "hipEventRecord"); break; }
+    for (int r = 0; r < 3 && !rc; ++r) rc = commutator_dev(ctx, X, Y, Z, IMSRG_ALL_TERMS, precision);
+    if (rc) break;
+    float ms = 0.f;
+    if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+      rc = fail(ctx, IMSRG_E_HIP, "timing failed");
+      break;
+    }
+    const double us = 1000.0 * ms / 3.0;
+    if (us < best_us) {
+      best_us = us;
+      best = cand[c];
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  for (imsrg_dop *o : {X, Y, Z})
+    if (o) imsrg_hip_op_destroy(o);
+  ctx->use_graphs = graphs;
+  ctx->profiling = prof;
+  if (rc) return rc;
+  if ((rc = apply_tuning(ctx, best))) return rc;
+  if (cache_path) {
+    if (FILE *f = std::fopen(cache_path, "a")) {
+      std::fprintf(f, "%s pphh_small=%d pphh_order=%d mixed_pphh=%d mixed_ph=%d us=%.3f\n", key.c_str(),
+                   best.pphh_small, best.pphh_order, best.mixed_pphh, best.mixed_ph, best_us);
+      std::fclose(f);
+    }
+  }
+  if (out) tuning_out(best, best_us, out);
+  return IMSRG_OK;
+}
+
 extern "C" int imsrg_hip_set_profiling(imsrg_ctx *ctx, int on) {
   if (!ctx) return IMSRG_E_INVALID;
   ctx->profiling = on != 0;

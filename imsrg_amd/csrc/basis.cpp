@@ -400,75 +400,18 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
   }
 
   // ---- work lists ----
-  wl = WorkLists();
-  {
-    std::vector<TileGroup> groups;
-    for (int c = 0; c < nch; ++c) {
-      int nk = ch_n[c];
-      if (nk == 0) continue;
-      int nt = (nk + kTileB - 1) / kTileB;
-      long kk = k_start[c + 1] - k_start[c];
-      for (int rb = 0; rb < nt; ++rb) {
-        TileGroup g;
-        for (int sb = rb; sb < nt; ++sb) g.tiles.push_back(make_int4(0, c, rb, sb));
-        g.cost = (kk + 1) * (long)g.tiles.size();
-        groups.push_back(g);
-      }
-    }
-    // IMSRG_HIP_PPHH_ORDER: "cost" (tiles by decreasing k-length) or "xcd"
-    // (row-block groups balanced over the XCDs)
-    const char *po = std::getenv("IMSRG_HIP_PPHH_ORDER");
-    if (po && std::string(po) == "xcd") {
-      wl.pphh = xcd_order(groups, kXcd);
-    } else {
-      std::vector<std::pair<long, int4>> t;
-      for (auto &g : groups)
-        for (auto &w : g.tiles) t.push_back({g.cost / (long)g.tiles.size(), w});
-      std::stable_sort(t.begin(), t.end(), [](const std::pair<long, int4> &a, const std::pair<long, int4> &b) {
-        return a.first > b.first;
-      });
-      for (auto &x : t) wl.pphh.push_back(x.second);
-    }
-    for (int c = 0; c < nch; ++c) {
-      int nk = ch_n[c];
-      if (nk == 0) continue;
-      int nt = (nk + kTile - 1) / kTile;
-      for (int rb = 0; rb < nt; ++rb)
-        for (int sb = rb; sb < nt; ++sb) wl.mirror.push_back(make_int4(0, c, rb, sb));
-      for (int s = 0; s < nk; ++s) wl.c122.push_back(make_int4(0, c, s, 0));
-      if (gcore_start[c + 1] > gcore_start[c] && gnc_start[c + 1] > gnc_start[c])
-        wl.gen.push_back(make_int4(0, c, 0, 0));
-    }
-    // inverse Pandya by orbit quadruple: (pair, group g, bra b, first ket) with
-    // one lane per group ket e >= b; items in global bra-ket order, so the bras
-    // (i, j) that share 6j sub-tables and Zbar rows (i, .) run together
-    {
-      std::vector<std::pair<int, int4>> t;
-      for (int g = 0; g < 6; ++g) {
-        const int gs = grp_start[g], ng = grp_start[g + 1] - gs;
-        for (int b = 0; b < ng; ++b)
-          for (int e0 = b; e0 < ng; e0 += kChunk) t.push_back({grp_kets[gs + b], make_int4(0, g, b, e0)});
-      }
-      std::stable_sort(t.begin(), t.end(),
-                       [](const std::pair<int, int4> &x, const std::pair<int, int4> &y) { return x.first < y.first; });
-      for (auto &x : t) wl.invp.push_back(x.second);
-    }
-    for (int c = 0; c < nch; ++c) {
-      int nk = cc_n[c], nph = cc_nph[c];
-      if (nk == 0 || nph == 0) continue;
-      int nt = (nk + kTile - 1) / kTile;
-      for (int rb = 0; rb < nt; ++rb)
-        for (int sb = rb; sb < nt; ++sb) wl.ph.push_back(make_int4(0, c, rb, sb));
-      long tot = (long)nk * 2 * nph;
-      for (long s = 0; s < tot; s += kChunk) wl.pandya.push_back(make_int4(0, c, (int)s, 0));
-    }
-    for (auto &pr : pairs1) wl.one.push_back(make_int4(0, pr.x, pr.y, 0));
-  }
-  d_pphh = dupload(*this, wl.pphh, err); d_ph = dupload(*this, wl.ph, err);
-  d_c122 = dupload(*this, wl.c122, err); d_pandya = dupload(*this, wl.pandya, err);
-  d_invp = dupload(*this, wl.invp, err); d_one = dupload(*this, wl.one, err);
-  d_mirror = dupload(*this, wl.mirror, err); d_gen = dupload(*this, wl.gen, err);
+  this->ch_n = ch_n;
+  this->k_start = k_start;
+  this->cc_n = cc_n;
+  this->cc_nph = cc_nph;
+  this->grp_start = grp_start;
+  this->grp_kets = grp_kets;
+  this->pairs1 = pairs1;
+  gen_ok.assign(nch, 0);
+  for (int c = 0; c < nch; ++c) gen_ok[c] = gcore_start[c + 1] > gcore_start[c] && gnc_start[c + 1] > gnc_start[c];
   if (!err.empty()) return err;
+  std::string le = build_lists(tune);
+  if (!le.empty()) return le;
 
   void *p = nullptr;
   if (hipMalloc(&p, sizeof(BasisDev)) != hipSuccess) return "hipMalloc failed for BasisDev";
@@ -478,7 +421,102 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
   return "";
 }
 
+std::string DeviceBasis::build_lists(const Tuning &t) {
+  const int nch = (int)ch_n.size();
+  wl = WorkLists();
+  {
+    // pp/hh tiles: 64x64 for channels with n <= t.pphh_small, 128x128 otherwise
+    std::vector<TileGroup> groups;
+    for (int c = 0; c < nch; ++c) {
+      const int nk = ch_n[c];
+      if (nk == 0) continue;
+      if (nk <= t.pphh_small) {
+        const int nt = (nk + kTile - 1) / kTile;
+        for (int rb = 0; rb < nt; ++rb)
+          for (int sb = rb; sb < nt; ++sb) wl.pphh64.push_back(make_int4(0, c, rb, sb));
+        continue;
+      }
+      const int nt = (nk + kTileB - 1) / kTileB;
+      const long kk = k_start[c + 1] - k_start[c];
+      for (int rb = 0; rb < nt; ++rb) {
+        TileGroup g;
+        for (int sb = rb; sb < nt; ++sb) g.tiles.push_back(make_int4(0, c, rb, sb));
+        g.cost = (kk + 1) * (long)g.tiles.size();
+        groups.push_back(g);
+      }
+    }
+    if (t.pphh_order == 1) {
+      wl.pphh = xcd_order(groups, kXcd);
+    } else {
+      std::vector<std::pair<long, int4>> ts;
+      for (auto &g : groups)
+        for (auto &w : g.tiles) ts.push_back({g.cost / (long)g.tiles.size(), w});
+      std::stable_sort(ts.begin(), ts.end(), [](const std::pair<long, int4> &a, const std::pair<long, int4> &b) {
+        return a.first > b.first;
+      });
+      for (auto &x : ts) wl.pphh.push_back(x.second);
+    }
+  }
+  for (int c = 0; c < nch; ++c) {
+    const int nk = ch_n[c];
+    if (nk == 0) continue;
+    const int nt = (nk + kTile - 1) / kTile;
+    for (int rb = 0; rb < nt; ++rb)
+      for (int sb = rb; sb < nt; ++sb) wl.mirror.push_back(make_int4(0, c, rb, sb));
+    for (int s = 0; s < nk; ++s) wl.c122.push_back(make_int4(0, c, s, 0));
+    if (gen_ok[c]) wl.gen.push_back(make_int4(0, c, 0, 0));
+  }
+  // inverse Pandya by orbit quadruple: (pair, group g, bra b, first ket) with
+  // one lane per group ket e >= b; items in global bra-ket order, so the bras
+  // (i, j) that share 6j sub-tables and Zbar rows (i, .) run together
+  {
+    std::vector<std::pair<int, int4>> ts;
+    for (int g = 0; g < 6; ++g) {
+      const int gs = grp_start[g], ng = grp_start[g + 1] - gs;
+      for (int b = 0; b < ng; ++b)
+        for (int e0 = b; e0 < ng; e0 += kChunk) ts.push_back({grp_kets[gs + b], make_int4(0, g, b, e0)});
+    }
+    std::stable_sort(ts.begin(), ts.end(),
+                     [](const std::pair<int, int4> &x, const std::pair<int, int4> &y) { return x.first < y.first; });
+    for (auto &x : ts) wl.invp.push_back(x.second);
+  }
+  for (int c = 0; c < nch; ++c) {
+    const int nk = cc_n[c], nph = cc_nph[c];
+    if (nk == 0 || nph == 0) continue;
+    const int nt = (nk + kTile - 1) / kTile;
+    for (int rb = 0; rb < nt; ++rb)
+      for (int sb = rb; sb < nt; ++sb) wl.ph.push_back(make_int4(0, c, rb, sb));
+    const long tot = (long)nk * 2 * nph;
+    for (long s = 0; s < tot; s += kChunk) wl.pandya.push_back(make_int4(0, c, (int)s, 0));
+  }
+  for (auto &pr : pairs1) wl.one.push_back(make_int4(0, pr.x, pr.y, 0));
+
+  // upload, replacing the previous lists
+  for (void *p : list_allocs) (void)hipFree(p);
+  list_allocs.clear();
+  std::string err;
+  auto up = [&](const std::vector<int4> &v) -> int4 * {
+    void *p = nullptr;
+    const size_t bytes = std::max<size_t>(v.size(), 1) * sizeof(int4);
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      err = "hipMalloc failed for a work list";
+      return nullptr;
+    }
+    list_allocs.push_back(p);
+    if (!v.empty() && hipMemcpy(p, v.data(), v.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess)
+      err = "hipMemcpy failed for a work list";
+    return (int4 *)p;
+  };
+  d_pphh = up(wl.pphh); d_pphh64 = up(wl.pphh64); d_ph = up(wl.ph);
+  d_c122 = up(wl.c122); d_pandya = up(wl.pandya); d_invp = up(wl.invp);
+  d_one = up(wl.one); d_mirror = up(wl.mirror); d_gen = up(wl.gen);
+  if (err.empty()) tune = t;
+  return err;
+}
+
 void DeviceBasis::release() {
+  for (void *p : list_allocs) (void)hipFree(p);
+  list_allocs.clear();
   for (void *p : allocs) (void)hipFree(p);
   allocs.clear();
   d = nullptr;

@@ -43,9 +43,18 @@ struct TileGroup {
 // Launch order of tile groups balanced over the XCDs (padding items have .y = -1).
 std::vector<int4> xcd_order(std::vector<TileGroup> groups, int nxcd);
 
+// Launch choices that the autotuner sets (defaults: measured best at emax=10).
+struct Tuning {
+  int pphh_small = 0;  // channels with n <= pphh_small take 64x64 pp/hh tiles, the others 128x128
+  int pphh_order = 0;  // 0: tiles by decreasing k-length, 1: row-block groups balanced over the XCDs
+  int mixed_pphh = 1;  // mixed precision: 1 = FP32-MFMA pp/hh kernel, 0 = the FP64 kernel
+  int mixed_ph = 1;    // mixed precision: 1 = FP32-MFMA ph kernel, 0 = the FP64 kernel
+};
+
 // Work-item lists (int4: pair, a, b, c) for every kernel of one commutator.
 struct WorkLists {
-  std::vector<int4> pphh;    // (pair, ch, rb, sb), rb <= sb
+  std::vector<int4> pphh;    // (pair, ch, rb, sb), rb <= sb, 128x128 tiles
+  std::vector<int4> pphh64;  // (pair, ch, rb, sb), rb <= sb, 64x64 tiles (channels n <= pphh_small)
   std::vector<int4> ph;      // (pair, cc, rb, sb), rb <= sb
   std::vector<int4> c122;    // (pair, ch, col, 0)
   std::vector<int4> pandya;  // (pair, cc, start, 0)
@@ -60,10 +69,16 @@ struct DeviceBasis {
   BasisDev h{};             // host copy holding device pointers
   BasisDev *d = nullptr;    // device copy of h
   std::vector<void *> allocs;
+  std::vector<void *> list_allocs;  // device work lists (rebuilt when the tuning changes)
   WorkLists wl;             // host lists with pair = 0
-  int4 *d_pphh = nullptr, *d_ph = nullptr, *d_c122 = nullptr, *d_pandya = nullptr, *d_invp = nullptr,
-       *d_one = nullptr, *d_mirror = nullptr, *d_gen = nullptr;
+  Tuning tune;
+  int4 *d_pphh = nullptr, *d_pphh64 = nullptr, *d_ph = nullptr, *d_c122 = nullptr, *d_pandya = nullptr,
+       *d_invp = nullptr, *d_one = nullptr, *d_mirror = nullptr, *d_gen = nullptr;
+  // host tables the work lists are built from
+  std::vector<int> ch_n, k_start, cc_n, cc_nph, grp_start, grp_kets, gen_ok;
+  std::vector<int2> pairs1;
   std::string upload(const HostBasis &hb, int emax6j);  // builds tables, lists, 6j table
+  std::string build_lists(const Tuning &t);             // (re)builds and uploads the work lists
   void release();
 };
 

@@ -980,12 +980,20 @@ hipError_t launch_onebody(const PairArg &pa, const int4 *items, int nitems, hipS
   hipLaunchKernelGGL(k_onebody, dim3(nitems), dim3(256), 0, s, pa, items);
   return hipGetLastError();
 }
-hipError_t launch_pphh(const PairArg &pa, const int4 *items, int nitems, int precision, hipStream_t s) {
-  if (nitems <= 0) return hipSuccess;
-  if (precision == 1)
-    hipLaunchKernelGGL(k_pphh_big_mixed, dim3(nitems), dim3(256), 0, s, pa, items);
-  else
-    hipLaunchKernelGGL(k_pphh_big_f64, dim3(nitems), dim3(256), 0, s, pa, items);
+hipError_t launch_pphh(const PairArg &pa, const int4 *items, int nitems, const int4 *items64, int nitems64,
+                       int precision, hipStream_t s) {
+  if (nitems > 0) {
+    if (precision == 1)
+      hipLaunchKernelGGL(k_pphh_big_mixed, dim3(nitems), dim3(256), 0, s, pa, items);
+    else
+      hipLaunchKernelGGL(k_pphh_big_f64, dim3(nitems), dim3(256), 0, s, pa, items);
+  }
+  if (nitems64 > 0) {
+    if (precision == 1)
+      hipLaunchKernelGGL((k_syr2k<0, true>), dim3(nitems64), dim3(256), 0, s, pa, items64, 0);
+    else
+      hipLaunchKernelGGL((k_syr2k<0, false>), dim3(nitems64), dim3(256), 0, s, pa, items64, 0);
+  }
   return hipGetLastError();
 }
 hipError_t launch_ph_gemm(const PairArg &pa, const int4 *items, int nitems, int precision, hipStream_t s) {

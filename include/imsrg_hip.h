@@ -176,6 +176,26 @@ int imsrg_hip_magnus_flow(imsrg_ctx *ctx, const imsrg_dop *H0, imsrg_dop *Hs, do
  * graphs.  Ignored while profiling is on. */
 int imsrg_hip_set_graphs(imsrg_ctx *ctx, int on);
 
+/* ---- launch tuning ------------------------------------------------------
+ * No counterpart in the reference (its GEMMs go to BLAS).  The choices below
+ * change only how work is tiled and ordered, and under IMSRG_PREC_MIXED which
+ * GEMM family runs on FP32 MFMA; FP64 results agree to rounding. */
+typedef struct {
+  int pphh_small_tile_max;  /* channels with n <= this use 64x64 pp/hh tiles, larger ones 128x128 */
+  int pphh_order;           /* 0: pp/hh tiles by decreasing k-length; 1: row-block groups balanced over XCDs */
+  int mixed_pphh_fp32;      /* IMSRG_PREC_MIXED: 1 = pp/hh GEMM on FP32 MFMA, 0 = on FP64 MFMA */
+  int mixed_ph_fp32;        /* IMSRG_PREC_MIXED: 1 = ph GEMM on FP32 MFMA, 0 = on FP64 MFMA */
+  double us_per_commutator; /* autotune: measured time of the chosen setting (0 when set by hand) */
+} imsrg_tuning;
+int imsrg_hip_get_tuning(imsrg_ctx *ctx, imsrg_tuning *t);
+int imsrg_hip_set_tuning(imsrg_ctx *ctx, const imsrg_tuning *t);
+/* Times every distinct setting on seeded synthetic operators of ctx's basis
+ * (one warm-up and three timed commutators each, at `precision`) and applies
+ * the fastest.  With cache_path != NULL, a setting stored there for the same
+ * basis, precision and GPU architecture is applied without timing, and a new
+ * result is appended (one text line per entry). */
+int imsrg_hip_autotune(imsrg_ctx *ctx, int precision, const char *cache_path, imsrg_tuning *out);
+
 /* Per-phase timing of the last commutator on ctx (microseconds, HIP events):
  * [0] zero-body, [1] one-body, [2] comm122, [3] pp/hh GEMM, [4] Pandya gather,
  * [5] ph GEMM, [6] inverse Pandya, [7] mirror.  Enabled by imsrg_hip_set_profiling. */
