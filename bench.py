@@ -342,11 +342,16 @@ def main():
     FA, execA, FC, BI = algorithmic_counts(ms)
     dom = int(np.argmax(phase_avg))
     t_dom = phase_avg[dom] * 1e-6
+    tun = tuning["mixed" if use_mixed else "fp64"]
     if PHASES[dom] == "pphh_gemm":
-        peak = PEAK_FP32_MFMA if use_mixed else PEAK_FP64_MFMA
-        kname = "k_pphh_big_mixed" if use_mixed else "k_pphh_big_f64"
+        # the kernel that ran: FP32 MFMA only when mixed precision and the tuning chose it
+        fp32 = use_mixed and tun["mixed_pphh_fp32"] == 1
+        peak = PEAK_FP32_MFMA if fp32 else PEAK_FP64_MFMA
+        kname = "k_pphh_big_mixed" if fp32 else "k_pphh_big_f64"
         ach = FA / t_dom
-        roof = {"kernel": kname, "bound": "mfma", "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+        roof = {"kernel": kname + (" + k_syr2k<0> (channels n <= %d)" % tun["pphh_small_tile_max"]
+                                   if tun["pphh_small_tile_max"] > 0 else ""),
+                "bound": "mfma", "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
                 "frac": ach / peak, "traffic": profiled_traffic(kname), "algorithmic_flops": FA,
                 "executed_flops": execA}
     elif PHASES[dom] == "inverse_pandya":

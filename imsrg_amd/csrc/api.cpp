@@ -31,13 +31,15 @@ struct BatchCache {
   size_t off[8] = {0};
   int cnt[8] = {0};
   double *work = nullptr;
-  int maxn = 0, maxnch = 0;
+  int maxn = 0, maxnch = 0, maxjmax = 0, maxnorb = 0;
   long maxn122 = 0;
   void release() {
     if (d_tab) (void)hipFree(d_tab);
     if (d_items) (void)hipFree(d_items);
     if (work) (void)hipFree(work);
     d_tab = nullptr; d_items = nullptr; work = nullptr;
+    maxn = maxnch = maxjmax = maxnorb = 0;
+    maxn122 = 0;
     key.clear();
   }
 };
@@ -434,7 +436,7 @@ static int herm_rule(int hx, int hy) { return (hx == hy) ? -1 : 1; }  // src/Com
 // [7] pp/hh 64-tiles.  Under mixed precision the context's tuning picks the
 // FP32 or FP64 kernel of each GEMM family.
 static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const int4 *const *items, const int *cnt,
-                          unsigned mask, int prec, int maxn, int maxnch, long maxn122) {
+                          unsigned mask, int prec, int maxn, int maxnch, long maxn122, int maxjmax, int maxnorb) {
   const Tuning &tu = ctx->db.tune;
   const int prec_pphh = (prec == IMSRG_PREC_MIXED && tu.mixed_pphh) ? IMSRG_PREC_MIXED : IMSRG_PREC_FP64;
   const int prec_ph = (prec == IMSRG_PREC_MIXED && tu.mixed_ph) ? IMSRG_PREC_MIXED : IMSRG_PREC_FP64;
@@ -457,7 +459,7 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
   HIPCHK(ctx, mark(5));
   if (mask & MPH) HIPCHK(ctx, launch_ph_gemm(pa, items[1], cnt[1], prec_ph, s));
   HIPCHK(ctx, mark(6));
-  if (mask & MPH) HIPCHK(ctx, launch_invpandya(pa, items[4], cnt[4], s));
+  if (mask & MPH) HIPCHK(ctx, launch_invpandya(pa, items[4], cnt[4], maxjmax, maxnorb, maxnch, s));
   HIPCHK(ctx, mark(7));
   HIPCHK(ctx, launch_mirror(pa, items[6], cnt[6], (mask & MPH) != 0, s));
   HIPCHK(ctx, mark(8));
@@ -500,7 +502,8 @@ static int commutator_dev(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y
   int cnt[8] = {(int)db.wl.pphh.size(), (int)db.wl.ph.size(), (int)db.wl.c122.size(), (int)db.wl.pandya.size(),
                 (int)db.wl.invp.size(), (int)db.wl.one.size(), (int)db.wl.mirror.size(), (int)db.wl.pphh64.size()};
   if (!ctx->use_graphs || ctx->profiling)
-    return run_commutator(ctx, pa, 1, items, cnt, mask, prec, db.h.maxn, db.h.nch, db.h.n122);
+    return run_commutator(ctx, pa, 1, items, cnt, mask, prec, db.h.maxn, db.h.nch, db.h.n122, db.h.jmax,
+                          db.h.norb);
   // replay a captured launch sequence for this operand set, capturing it on first use
   for (const GraphEntry &g : ctx->graphs)
     if (g.x == X->buf && g.y == Y->buf && g.z == Z->buf && g.hx == X->herm && g.hy == Y->herm && g.mask == mask &&
@@ -509,7 +512,8 @@ static int commutator_dev(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y
       return IMSRG_OK;
     }
   HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-  const int rc = run_commutator(ctx, pa, 1, items, cnt, mask, prec, db.h.maxn, db.h.nch, db.h.n122);
+  const int rc = run_commutator(ctx, pa, 1, items, cnt, mask, prec, db.h.maxn, db.h.nch, db.h.n122, db.h.jmax,
+                          db.h.norb);
   hipGraph_t graph = nullptr;
   const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);  // always leave capture mode
   if (rc) {
@@ -876,6 +880,8 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
       for (auto &kv : gm) groups.push_back(kv.second);
       bc.maxn = std::max(bc.maxn, ctxs[i]->db.h.maxn);
       bc.maxnch = std::max(bc.maxnch, ctxs[i]->db.h.nch);
+      bc.maxjmax = std::max(bc.maxjmax, ctxs[i]->db.h.jmax);
+      bc.maxnorb = std::max(bc.maxnorb, ctxs[i]->db.h.norb);
       bc.maxn122 = std::max(bc.maxn122, ctxs[i]->db.h.n122);
     }
     if (c0->db.tune.pphh_order == 1) {
@@ -905,7 +911,8 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
   pa.tab = bc.d_tab;
   const int4 *items[8];
   for (int t = 0; t < 8; ++t) items[t] = bc.d_items + bc.off[t];
-  return run_commutator(c0, pa, npairs, items, bc.cnt, mask, prec, bc.maxn, bc.maxnch, bc.maxn122);
+  return run_commutator(c0, pa, npairs, items, bc.cnt, mask, prec, bc.maxn, bc.maxnch, bc.maxn122, bc.maxjmax,
+                        bc.maxnorb);
 }
 
 // ---------------------------------------------------------------------------

@@ -388,15 +388,21 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
   // ---- 6j table on the device ----
   {
     long tot = (long)B.d1 * B.d1 * B.d2 * B.d1 * B.d1 * B.d2;
-    void *p = nullptr, *pT = nullptr;
+    B.d2q = (B.d2 + 1) & ~1;
+    // the J-innermost copy: padded rows, plus 4 entries so that a 4-wide J chunk
+    // starting at the last row stays inside the allocation
+    long totq = (long)B.d1 * B.d1 * B.d1 * B.d1 * B.d2q * B.d2q + 4;
+    void *p = nullptr, *pQ = nullptr;
     if (hipMalloc(&p, tot * sizeof(double)) != hipSuccess) return "hipMalloc failed for the 6j table";
     allocs.push_back(p);
-    if (hipMalloc(&pT, tot * sizeof(double)) != hipSuccess) return "hipMalloc failed for the 6j table";
-    allocs.push_back(pT);
-    if (launch_sixj_table((double *)p, (double *)pT, B.d1, B.d2, 0) != hipSuccess) return "6j table kernel failed";
+    if (hipMalloc(&pQ, totq * sizeof(double)) != hipSuccess) return "hipMalloc failed for the 6j table";
+    allocs.push_back(pQ);
+    if (hipMemset(pQ, 0, totq * sizeof(double)) != hipSuccess) return "hipMemset failed for the 6j table";
+    if (launch_sixj_table((double *)p, (double *)pQ, B.d1, B.d2, B.d2q, 0) != hipSuccess)
+      return "6j table kernel failed";
     if (hipDeviceSynchronize() != hipSuccess) return "6j table kernel failed";
     B.sixj = (const double *)p;
-    B.sixjT = (const double *)pT;
+    B.sixjQ = (const double *)pQ;
   }
 
   // ---- work lists ----
@@ -466,18 +472,16 @@ std::string DeviceBasis::build_lists(const Tuning &t) {
     for (int s = 0; s < nk; ++s) wl.c122.push_back(make_int4(0, c, s, 0));
     if (gen_ok[c]) wl.gen.push_back(make_int4(0, c, 0, 0));
   }
-  // inverse Pandya by orbit quadruple: (pair, group g, bra b, first ket) with
-  // one lane per group ket e >= b; items in global bra-ket order, so the bras
-  // (i, j) that share 6j sub-tables and Zbar rows (i, .) run together
+  // inverse Pandya by orbit quadruple: one item (pair, group g, bra b, 0) per
+  // bra ket, whose work-group loops over the group kets e >= b; longest first
   {
     std::vector<std::pair<int, int4>> ts;
     for (int g = 0; g < 6; ++g) {
       const int gs = grp_start[g], ng = grp_start[g + 1] - gs;
-      for (int b = 0; b < ng; ++b)
-        for (int e0 = b; e0 < ng; e0 += kChunk) ts.push_back({grp_kets[gs + b], make_int4(0, g, b, e0)});
+      for (int b = 0; b < ng; ++b) ts.push_back({ng - b, make_int4(0, g, b, 0)});
     }
     std::stable_sort(ts.begin(), ts.end(),
-                     [](const std::pair<int, int4> &x, const std::pair<int, int4> &y) { return x.first < y.first; });
+                     [](const std::pair<int, int4> &x, const std::pair<int, int4> &y) { return x.first > y.first; });
     for (auto &x : ts) wl.invp.push_back(x.second);
   }
   for (int c = 0; c < nch; ++c) {

@@ -7,6 +7,17 @@
 
 namespace imsrg {
 
+// Pointer members of the structs below address global memory.  In the device
+// pass they carry address space 1, so that every load and store through them
+// (and through pointers derived from them) compiles to global_* instructions:
+// flat_* ones would also count against the LDS wait counter and serialize LDS
+// reads with outstanding global loads.  Host code fills the same layout.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define IMSRG_G __attribute__((address_space(1)))
+#else
+#define IMSRG_G
+#endif
+
 constexpr int kTile = 64;         // SYR2K output tile edge
 constexpr int kTileB = 128;       // output tile edge of the large pp/hh SYR2K
 constexpr int kTileK = 16;        // k-depth of one LDS stage
@@ -35,73 +46,78 @@ struct BasisDev {
   long opsize;   // doubles per operator buffer
   int maxn, maxcc, maxblk;
   long n122;           // comm122 ELL entries (including padding)
-  const int *j2, *l, *tz2, *cvq;
-  const double *occ;
-  const int *holes;
-  const int *obstart, *oblist;  // one-body block of each orbit
-  const int *kp, *kq;
-  const int *kmap, *cmap;
-  const int *cmapT;     // cmapT[J*nkets + k] = cmap[k*(jmax+1) + J]
-  const int *kmapT;     // kmapT[J*nkets + k] = kmap[k*(jmax+1) + J]
+  const IMSRG_G int *j2, *l, *tz2, *cvq;
+  const IMSRG_G double *occ;
+  const IMSRG_G int *holes;
+  const IMSRG_G int *obstart, *oblist;  // one-body block of each orbit
+  const IMSRG_G int *kp, *kq;
+  const IMSRG_G int *kmap, *cmap;
+  const IMSRG_G int *cmapT;     // cmapT[J*nkets + k] = cmap[k*(jmax+1) + J]
+  const IMSRG_G int *kmapT;     // kmapT[J*nkets + k] = kmap[k*(jmax+1) + J]
   // kets of each (parity, Tz) group g = 2(Tz+1) + parity (the channels 6J + g),
   // ascending global index: grp_kets[grp_start[g] .. grp_start[g+1])
-  const int *grp_start, *grp_kets;
+  const IMSRG_G int *grp_start, *grp_kets;
   // standard channels
-  const long *ch_off;
-  const int *ch_n, *ch_kstart, *ch_kets;
-  const double *w220;  // per channel ket: (1-n_p)(1-n_q) for hh/ph kets, 1 for pp kets
-  const int *hh_start, *hh_loc;
-  const double *hh_occ;
-  const int *k_start, *k_loc;  // pp/hh product list with combined weight (w_pp - w_hh)
-  const double *k_w;
-  const int *kpp_start, *kpp_loc;  // M_pp list (weights w_pp)
-  const double *kpp_w;
-  const int *khh_start, *khh_loc;  // M_hh list (weights n_p n_q)
-  const double *khh_w;
+  const IMSRG_G long *ch_off;
+  const IMSRG_G int *ch_n, *ch_kstart, *ch_kets;
+  const IMSRG_G double *w220;  // per channel ket: (1-n_p)(1-n_q) for hh/ph kets, 1 for pp kets
+  const IMSRG_G int *hh_start, *hh_loc;
+  const IMSRG_G double *hh_occ;
+  const IMSRG_G int *k_start, *k_loc;  // pp/hh product list with combined weight (w_pp - w_hh)
+  const IMSRG_G double *k_w;
+  const IMSRG_G int *kpp_start, *kpp_loc;  // M_pp list (weights w_pp)
+  const IMSRG_G double *kpp_w;
+  const IMSRG_G int *khh_start, *khh_loc;  // M_hh list (weights n_p n_q)
+  const IMSRG_G double *khh_w;
   // comm122 partner lists in ELL layout: entry m of ket r of channel ch at
   // ell_off[ch] + m * n_ch + r, m < ell_mx[ch]; padding has a = -1
-  const long *ell_off;
-  const int *ell_mx;
-  const int *ell_cnt;     // entries of ket (ch_kstart[ch] + r)
-  const int *ell_ind;     // local ket index ind2 of the partner
-  const int2 *ell_ao;     // (a, o): coefficient X1(a, o) * f
-  const double *ell_f;
+  const IMSRG_G long *ell_off;
+  const IMSRG_G int *ell_mx;
+  const IMSRG_G int *ell_cnt;     // entries of ket (ch_kstart[ch] + r)
+  const IMSRG_G int *ell_ind;     // local ket index ind2 of the partner
+  const IMSRG_G int2 *ell_ao;     // (a, o): coefficient X1(a, o) * f
+  const IMSRG_G double *ell_f;
   // cross-coupled channels
-  const int *cc_n, *cc_nph, *cc_kstart, *cc_kets, *cc_phstart, *cc_ph;
-  const signed char *cc_phi;  // (-1)^{(j_p+j_q)} per CC ket
-  const long *cc_xoff, *cc_zoff;
+  const IMSRG_G int *cc_n, *cc_nph, *cc_kstart, *cc_kets, *cc_phstart, *cc_ph;
+  const IMSRG_G signed char *cc_phi;  // (-1)^{(j_p+j_q)} per CC ket
+  const IMSRG_G long *cc_xoff, *cc_zoff;
   long xsize, zsize;
-  const int2 *pairs1;   // one-body pairs (i<=j, same (l,j2,tz2) block)
-  const double *w1norm; // norb*norb one-body norm weights
-  const double *sixj;   // dense 6j table, layout of SixJCache_112112
-  const double *sixjT;  // same values, layout [jj1][jj2][J3][jj4][J6][jj5] (j5 innermost)
+  const IMSRG_G int2 *pairs1;   // one-body pairs (i<=j, same (l,j2,tz2) block)
+  const IMSRG_G double *w1norm; // norb*norb one-body norm weights
+  const IMSRG_G double *sixj;   // dense 6j table, layout of SixJCache_112112
+  const IMSRG_G double *sixjQ;  // same values, layout [jj1][jj2][jj4][jj5][J6][J3] with J3 innermost and rows
+                        // padded to d2q = d2 rounded up to even (16-byte aligned pairs of J3)
+  int d2q;
   // generator lists: per channel, core-core kets and (non-core)-(non-core) kets
-  const int *gcore_start, *gcore_loc, *gnc_start, *gnc_loc;
+  const IMSRG_G int *gcore_start, *gcore_loc, *gnc_start, *gnc_loc;
 };
 
 // Everything one commutator needs: operand buffers, workspaces, flags.
 struct PairPtrs {
-  const BasisDev *basis;  // device pointer
-  const double *X, *Y;    // operator buffers (zero | one | two)
-  double *Z;
-  double *Xt, *Yt, *Zbar; // Pandya workspaces
-  double *scratch;        // >= nch + 8 doubles (zero-body partial sums)
-  double2 *c122;          // per-call comm122 coefficients (X1(a,o) f, Y1(a,o) f), n122 entries
+  const IMSRG_G BasisDev *basis;  // device pointer
+  const IMSRG_G double *X, *Y;    // operator buffers (zero | one | two)
+  IMSRG_G double *Z;
+  IMSRG_G double *Xt, *Yt, *Zbar; // Pandya workspaces
+  IMSRG_G double *scratch;        // >= nch + 8 doubles (zero-body partial sums)
+  IMSRG_G double2 *c122;          // per-call comm122 coefficients (X1(a,o) f, Y1(a,o) f), n122 entries
   int hX, hY, hZ;
   unsigned mask;
 };
 
 struct PairArg {
   PairPtrs p0;           // used when tab == nullptr
-  const PairPtrs *tab;   // batched: device table
-  __device__ __forceinline__ PairPtrs get(int i) const { return tab ? tab[i] : p0; }
+  const IMSRG_G PairPtrs *tab;  // batched: device table
+  __device__ __forceinline__ PairPtrs get(int i) const {
+    if (tab) return tab[i];  // separate returns keep each load in its own address space
+    return p0;
+  }
 };
 
 __host__ __device__ inline int ket_index(int norb, int p, int q) { return p * (2 * norb - 1 - p) / 2 + q; }
 __host__ __device__ inline int sgn_parity(int x) { return (x & 1) ? -1 : 1; }
-__host__ __device__ inline long sixjT_index(int d1, int d2, int jj1, int jj2, int J3, int jj4, int jj5, int J6) {
-  return (((((long)((jj1 - 1) >> 1) * d1 + ((jj2 - 1) >> 1)) * d2 + J3) * d1 + ((jj4 - 1) >> 1)) * d2 + J6) * d1 +
-         ((jj5 - 1) >> 1);
+__host__ __device__ inline long sixjQ_index(int d1, int d2q, int jj1, int jj2, int J3, int jj4, int jj5, int J6) {
+  return (((((long)((jj1 - 1) >> 1) * d1 + ((jj2 - 1) >> 1)) * d1 + ((jj4 - 1) >> 1)) * d1 + ((jj5 - 1) >> 1)) * d2q +
+          J6) * d2q + J3;
 }
 __host__ __device__ inline long sixj_index(int d1, int d2, int jj1, int jj2, int J3, int jj4, int jj5, int J6) {
   return (((((long)((jj1 - 1) >> 1) * d1 + ((jj2 - 1) >> 1)) * d2 + J3) * d1 + ((jj4 - 1) >> 1)) * d1 +

@@ -755,136 +755,168 @@ __device__ __forceinline__ int wave_max(int v) {
 }
 
 // AddInversePandyaTransformation (src/Commutator.cc:1501-1616), by orbit quadruple.
-// One lane per pair of kets (ij) <= (kl) of one (parity, Tz) group: the
-// cross-coupled elements Zbar^{J'}[(il),(kj)] (direct) and Zbar^{J'}[(jl),(ki)]
-// (exchange) do not depend on the standard-channel J, so each is loaded once
-// and recoupled into every J of the quadruple (register chunks of kJC values):
+// One work-group per bra ket (ij) of a (parity, Tz) group; lane e takes the
+// group's ket (kl) = e >= (ij), looping over the kets in chunks of the group
+// size.  The cross-coupled elements Zbar^{J'}[(il),(kj)] (direct) and
+// Zbar^{J'}[(jl),(ki)] (exchange) do not depend on the standard-channel J, so
+// each is loaded once and recoupled into every J of the quadruple (register
+// chunks of kJC values):
 //   commij(J) = -sum_J' (2J'+1) {ji jj J; jk jl J'} Zbar^{J'}[il, kj]
 //   commji(J) = -sum_J' (2J'+1) {jj ji J; jk jl J'} Zbar^{J'}[jl, ki]
 //   Z^J_{ij,kl} += -(commij - (-1)^{(jk+jl)/2 - J} commji) / norm
-// with the reference's |6j| < 1e-8 skip.  Every upper element (ij <= kl, or
-// ij < kl for anti-Hermitian Z) of every channel is produced exactly once and
-// stored at its transposed slot (consecutive lanes, consecutive addresses);
-// k_mirror_fold adds it to the upper triangle, the diagonal is added in place.
+// with the reference's |6j| < 1e-8 skip.  The cross-coupled local indices of
+// the kets (i,x) and (j,x), the CC channel sizes/offsets and the bra's
+// standard-channel data are staged in LDS once per work-group.  Every upper
+// element (ij <= kl, or ij < kl for anti-Hermitian Z) of every channel is
+// produced exactly once and stored at its transposed slot (consecutive lanes,
+// consecutive addresses); k_mirror_fold adds it to the upper triangle, the
+// diagonal is added in place.
+__host__ __device__ inline size_t invp_lds_bytes(int jmax, int norb, int nch) {
+  const size_t nJ = (size_t)jmax + 1;
+  return 8 * (size_t)nch + 8 * nJ + 4 * (size_t)nch + 4 * nJ * (2 * (size_t)norb + 2);
+}
+
 __global__ __launch_bounds__(256, 6) void k_invpandya(PairArg pa, const int4 *items) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
   const int4 it = items[blockIdx.x];
   PairPtrs P = pa.get(it.x);
   const BasisDev &B = *P.basis;
   const int g = it.y, b = it.z;
   const int gs = B.grp_start[g], ng = B.grp_start[g + 1] - gs;
-  const int e = it.w + threadIdx.x;
-  const int emin = (P.hZ == 1) ? b : b + 1;
-  const bool active = e >= emin && e < ng;
-  const int kb = B.grp_kets[gs + b], kk = B.grp_kets[gs + (active ? e : b)];
-  const int i = B.kp[kb], j = B.kq[kb], k = B.kp[kk], l = B.kq[kk];  // i <= k <= l
-  const int jji = B.j2[i], jjj = B.j2[j], jjk = B.j2[k], jjl = B.j2[l];
-  const int nk = B.nkets;
-  const bool pauli = (i == j) || (k == l);  // kets (p,p) exist for even J only
-  int Jlo = max(abs(jji - jjj), abs(jjk - jjl)) / 2, Jhi = min(jji + jjj, jjk + jjl) / 2;
-  if (pauli) {
-    Jlo += Jlo & 1;
-    Jhi -= Jhi & 1;
+  const int nk = B.nkets, norb = B.norb, nJ = B.jmax + 1, nch = B.nch;
+  long *s_zoff = (long *)lds_raw;      // [nch] CC channel offsets in Zbar
+  long *s_choff = s_zoff + nch;        // [nJ]  offsets of the standard channels 6J + g
+  int *s_ccn = (int *)(s_choff + nJ);  // [nch] CC channel sizes
+  int *s_chn = s_ccn + nch;            // [nJ]  sizes of the standard channels 6J + g
+  int *s_ib = s_chn + nJ;              // [nJ]  local index of the bra in channel 6J + g
+  int *Ti = s_ib + nJ;                 // [nJ * norb] CC local index of ket (i,x) in channel J'
+  int *Tj = Ti + nJ * norb;            // [nJ * norb] CC local index of ket (j,x) in channel J'
+  const int kb = B.grp_kets[gs + b];
+  const int i = B.kp[kb], j = B.kq[kb];
+  for (int c = threadIdx.x; c < nch; c += blockDim.x) {
+    s_zoff[c] = B.cc_zoff[c];
+    s_ccn[c] = B.cc_n[c];
   }
-  const bool live = active && Jlo <= Jhi;
-  if (!live) {
-    Jlo = 1 << 20;
-    Jhi = -1;
+  for (int J = threadIdx.x; J < nJ; J += blockDim.x) {
+    s_choff[J] = B.ch_off[6 * J + g];
+    s_chn[J] = B.ch_n[6 * J + g];
+    s_ib[J] = B.kmapT[(long)J * nk + kb];
   }
-  const int wlo = wave_min(Jlo), whi = wave_max(Jhi);
-  if (wlo > whi) return;  // wave-uniform: no lane of this wave has a channel
-  // direct term: CC channels of (i,l) for J' in tri(ji,jl) ∩ tri(jk,jj)
-  int Dlo = max(abs(jji - jjl), abs(jjk - jjj)) / 2, Dhi = min(jji + jjl, jjk + jjj) / 2;
-  if (!live) {
-    Dlo = 1 << 20;
-    Dhi = -1;
+  for (int e = threadIdx.x; e < nJ * norb; e += blockDim.x) {
+    const int Jp = e / norb, x = e - Jp * norb;
+    Ti[e] = B.cmapT[(long)Jp * nk + ket_index(norb, min(i, x), max(i, x))];
+    Tj[e] = B.cmapT[(long)Jp * nk + ket_index(norb, min(j, x), max(j, x))];
   }
-  const int dlo = wave_min(Dlo), dhi = wave_max(Dhi);
-  // exchange term: CC channels of (i,k) for J' in tri(jj,jl) ∩ tri(jk,ji)
-  const bool exch = live && k != l && i != j;
-  int Xlo = max(abs(jjj - jjl), abs(jjk - jji)) / 2, Xhi = min(jjj + jjl, jjk + jji) / 2;
-  if (!exch) {
-    Xlo = 1 << 20;
-    Xhi = -1;
-  }
-  const int xlo = wave_min(Xlo), xhi = wave_max(Xhi);
-  const int pd = (B.l[i] + B.l[l]) & 1, td = abs(B.tz2[i] - B.tz2[l]) / 2;
-  const int px = (B.l[i] + B.l[k]) & 1, tx = abs(B.tz2[i] - B.tz2[k]) / 2;
-  const int kil = ket_index(B.norb, i, l), kjk = ket_index(B.norb, min(j, k), max(j, k));
-  const int kik = ket_index(B.norb, i, k), klj = ket_index(B.norb, min(l, j), max(l, j));
-  const long sJ = (long)B.d1 * B.d2 * B.d1;  // stride of J3 in the sixjT layout
-  const long s6d = sixjT_index(B.d1, B.d2, jji, jjj, 0, jjk, jjl, 0);
-  const long s6x = sixjT_index(B.d1, B.d2, jjj, jji, 0, jjk, jjl, 0);
+  __syncthreads();
+  const int jji = B.j2[i], jjj = B.j2[j];
+  const int d2q = B.d2q;
   const double hf = (P.hZ == 1) ? 1.0 : -1.0;
   const double *Zb = P.Zbar;
-  const int dbra = (i == j), dket = (k == l);
-  const double nrm = (dbra == dket) ? 1.0 + dbra : SQRT2_D;
-  const double pij = sgn_parity((jji + jjj + jjk + jjl) / 2);
   double *Ztwo = P.Z + B.two_off;
-  for (int c0 = wlo; c0 <= whi; c0 += kJC) {
-    // lane's valid J of this chunk as a bit mask
-    unsigned vm = 0;
-#pragma unroll
-    for (int t = 0; t < kJC; ++t) {
-      const int J = c0 + t;
-      if (J >= Jlo && J <= Jhi && !(pauli && (J & 1))) vm |= 1u << t;
+  const int emin = (P.hZ == 1) ? b : b + 1;
+  for (int e0 = emin; e0 < ng; e0 += blockDim.x) {
+    const int e = e0 + threadIdx.x;
+    const bool active = e < ng;
+    const int kk = B.grp_kets[gs + (active ? e : b)];
+    const int k = B.kp[kk], l = B.kq[kk];  // i <= k <= l
+    const int jjk = B.j2[k], jjl = B.j2[l];
+    const bool pauli = (i == j) || (k == l);  // kets (p,p) exist for even J only
+    int Jlo = max(abs(jji - jjj), abs(jjk - jjl)) / 2, Jhi = min(jji + jjj, jjk + jjl) / 2;
+    if (pauli) {
+      Jlo += Jlo & 1;
+      Jhi -= Jhi & 1;
     }
-    double ad[kJC], ax[kJC];
-#pragma unroll
-    for (int t = 0; t < kJC; ++t) ad[t] = ax[t] = 0.0;
-    for (int Jp = dlo; Jp <= dhi; ++Jp) {
-      if (Jp < Dlo || Jp > Dhi) continue;
-      const int cc = 6 * Jp + 2 * (td + 1) + pd;
-      const int nkc = B.cc_n[cc];
-      const int il = B.cmapT[(long)Jp * nk + kil];
-      const int kj = B.cmapT[(long)Jp * nk + kjk] + (k > j ? nkc : 0);
-      const double z = (2 * Jp + 1) * Zb[B.cc_zoff[cc] + (long)kj * nkc + il];
-      const double *s6 = B.sixjT + s6d + (long)c0 * sJ + (long)Jp * B.d1;
+    const bool live = active && Jlo <= Jhi;
+    if (!live) {
+      Jlo = 1 << 20;
+      Jhi = -1;
+    }
+    const int wlo = wave_min(Jlo), whi = wave_max(Jhi);
+    if (wlo > whi) continue;  // wave-uniform: no lane of this wave has a channel
+    // direct term: CC channels of (i,l) for J' in tri(ji,jl) ∩ tri(jk,jj)
+    int Dlo = max(abs(jji - jjl), abs(jjk - jjj)) / 2, Dhi = min(jji + jjl, jjk + jjj) / 2;
+    if (!live) {
+      Dlo = 1 << 20;
+      Dhi = -1;
+    }
+    const int dlo = wave_min(Dlo), dhi = wave_max(Dhi);
+    // exchange term: CC channels of (i,k) for J' in tri(jj,jl) ∩ tri(jk,ji)
+    const bool exch = live && k != l && i != j;
+    int Xlo = max(abs(jjj - jjl), abs(jjk - jji)) / 2, Xhi = min(jjj + jjl, jjk + jji) / 2;
+    if (!exch) {
+      Xlo = 1 << 20;
+      Xhi = -1;
+    }
+    const int xlo = wave_min(Xlo), xhi = wave_max(Xhi);
+    const int pd = (B.l[i] + B.l[l]) & 1, td = abs(B.tz2[i] - B.tz2[l]) / 2;
+    const int px = (B.l[i] + B.l[k]) & 1, tx = abs(B.tz2[i] - B.tz2[k]) / 2;
+    const long s6d = sixjQ_index(B.d1, d2q, jji, jjj, 0, jjk, jjl, 0);
+    const long s6x = sixjQ_index(B.d1, d2q, jjj, jji, 0, jjk, jjl, 0);
+    // tail element T(ik, lj) = phi_ik phi_lj T(lj, ik), phi_pq = (-1)^{(j_p+j_q)/2};
+    // head H(ik, lj) = hZ-sign H(lj, ik)
+    const double symx = (l > j) ? (double)sgn_parity((jji + jjk) / 2 + (jjl + jjj) / 2) : hf;
+    const int dbra = (i == j), dket = (k == l);
+    const double nrm = (dbra == dket) ? 1.0 + dbra : SQRT2_D;
+    const double pij = sgn_parity((jji + jjj + jjk + jjl) / 2);
+    // J chunks start at even J so that each pair of 6j values is one aligned 16-byte load
+    for (int c0 = wlo & ~1; c0 <= whi; c0 += kJC) {
+      unsigned vm = 0;  // lane's valid J of this chunk
 #pragma unroll
       for (int t = 0; t < kJC; ++t) {
-        if (c0 + t > whi) break;
-        const double sj = s6[t * sJ];
-        ad[t] += ((vm >> t) & 1) && fabs(sj) >= 1e-8 ? sj * z : 0.0;
+        const int J = c0 + t;
+        if (J >= Jlo && J <= Jhi && !(pauli && (J & 1))) vm |= 1u << t;
       }
-    }
-    for (int Jp = xlo; Jp <= xhi; ++Jp) {
-      if (Jp < Xlo || Jp > Xhi) continue;
-      const int cc = 6 * Jp + 2 * (tx + 1) + px;
-      const int nkc = B.cc_n[cc];
-      const int ik = B.cmapT[(long)Jp * nk + kik];
-      const int lj = B.cmapT[(long)Jp * nk + klj];
-      const double *Zc = Zb + B.cc_zoff[cc];
-      // tail element T(ik, lj) = phi_ik phi_lj T(lj, ik); head H(ik, lj) = hZ-sign H(lj, ik)
-      const double me = (l > j) ? Zc[(long)(nkc + ik) * nkc + lj] : Zc[(long)ik * nkc + lj];
-      double sym = hf;
-      if (l > j) {
-        const signed char *phi = B.cc_phi + B.cc_kstart[cc];
-        sym = (double)(phi[ik] * phi[lj]);
+      // acc(J) = sum_J' w_d z_d - (-1)^{(jk+jl)/2 - J} sum_J' w_x z_x, i.e. -(commij - sign commji)
+      // before the special cases k == l, i == j (no exchange sum; factor applied below)
+      double acc[kJC];
+#pragma unroll
+      for (int t = 0; t < kJC; ++t) acc[t] = 0.0;
+      const double sx0 = -(double)sgn_parity((jjk + jjl) / 2 - c0);  // -(-1)^{(jk+jl)/2 - J} at J = c0
+      for (int Jp = dlo; Jp <= dhi; ++Jp) {
+        if (Jp < Dlo || Jp > Dhi) continue;
+        const int cc = 6 * Jp + 2 * (td + 1) + pd;
+        const int nkc = s_ccn[cc];
+        const int il = Ti[Jp * norb + l];
+        const int kj = Tj[Jp * norb + k] + (k > j ? nkc : 0);
+        const double z = (2 * Jp + 1) * Zb[s_zoff[cc] + (long)kj * nkc + il];
+        const double2 *s6 = (const double2 *)(B.sixjQ + s6d + (long)Jp * d2q + c0);
+        const double2 w01 = s6[0], w23 = s6[1];
+        acc[0] += (vm & 1) && fabs(w01.x) >= 1e-8 ? w01.x * z : 0.0;
+        acc[1] += (vm & 2) && fabs(w01.y) >= 1e-8 ? w01.y * z : 0.0;
+        acc[2] += (vm & 4) && fabs(w23.x) >= 1e-8 ? w23.x * z : 0.0;
+        acc[3] += (vm & 8) && fabs(w23.y) >= 1e-8 ? w23.y * z : 0.0;
       }
-      const double z = (2 * Jp + 1) * (sym * me);
-      const double *s6 = B.sixjT + s6x + (long)c0 * sJ + (long)Jp * B.d1;
+      for (int Jp = xlo; Jp <= xhi; ++Jp) {
+        if (Jp < Xlo || Jp > Xhi) continue;
+        const int cc = 6 * Jp + 2 * (tx + 1) + px;
+        const int nkc = s_ccn[cc];
+        const int ik = Ti[Jp * norb + k];
+        const int lj = Tj[Jp * norb + l];
+        const double me = Zb[s_zoff[cc] + (long)((l > j ? nkc : 0) + ik) * nkc + lj];
+        const double z = (2 * Jp + 1) * (symx * me) * sx0;  // the sign alternates with J below
+        const double2 *s6 = (const double2 *)(B.sixjQ + s6x + (long)Jp * d2q + c0);
+        const double2 w01 = s6[0], w23 = s6[1];
+        acc[0] += (vm & 1) && fabs(w01.x) >= 1e-8 ? w01.x * z : 0.0;
+        acc[1] -= (vm & 2) && fabs(w01.y) >= 1e-8 ? w01.y * z : 0.0;
+        acc[2] += (vm & 4) && fabs(w23.x) >= 1e-8 ? w23.x * z : 0.0;
+        acc[3] -= (vm & 8) && fabs(w23.y) >= 1e-8 ? w23.y * z : 0.0;
+      }
 #pragma unroll
       for (int t = 0; t < kJC; ++t) {
-        if (c0 + t > whi) break;
-        const double sj = s6[t * sJ];
-        ax[t] += ((vm >> t) & 1) && fabs(sj) >= 1e-8 ? sj * z : 0.0;
+        if (!((vm >> t) & 1)) continue;
+        const int J = c0 + t;
+        // commji = commij for k == l, pij commij for i == j (no exchange sum then)
+        const double s = sgn_parity((jjk + jjl) / 2 - J);
+        const double f = (k == l) ? 1.0 - s : (i == j) ? 1.0 - s * pij : 1.0;
+        const double z = acc[t] * f / nrm;
+        const int n = s_chn[J];
+        const int ib = s_ib[J], ik = B.kmapT[(long)J * nk + kk];
+        double *Z2 = Ztwo + s_choff[J];
+        if (ik == ib)
+          Z2[(long)ib * n + ib] += z;
+        else
+          Z2[(long)ib * n + ik] = z;
       }
-    }
-#pragma unroll
-    for (int t = 0; t < kJC; ++t) {
-      if (c0 + t > whi) break;
-      if (!((vm >> t) & 1)) continue;
-      const int J = c0 + t;
-      const double commij = -ad[t];
-      const double commji = (k == l) ? commij : (i == j) ? pij * commij : -ax[t];
-      const double z = -(commij - sgn_parity((jjk + jjl) / 2 - J) * commji) / nrm;
-      const int ch = 6 * J + g;
-      const int n = B.ch_n[ch];
-      const int ib = B.kmapT[(long)J * nk + kb], ik = B.kmapT[(long)J * nk + kk];
-      double *Z2 = Ztwo + B.ch_off[ch];
-      if (ik == ib)
-        Z2[(long)ib * n + ib] += z;
-      else
-        Z2[(long)ib * n + ik] = z;
     }
   }
 }
@@ -1021,9 +1053,12 @@ hipError_t launch_pandya(const PairArg &pa, const int4 *items, int nitems, hipSt
   hipLaunchKernelGGL(k_pandya, dim3(nitems), dim3(kChunk), 0, s, pa, items);
   return hipGetLastError();
 }
-hipError_t launch_invpandya(const PairArg &pa, const int4 *items, int nitems, hipStream_t s) {
+hipError_t launch_invpandya(const PairArg &pa, const int4 *items, int nitems, int jmax, int norb, int nch,
+                            hipStream_t s) {
   if (nitems <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_invpandya, dim3(nitems), dim3(kChunk), 0, s, pa, items);
+  const size_t lds = invp_lds_bytes(jmax, norb, nch);
+  if (lds > 160 * 1024) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_invpandya, dim3(nitems), dim3(kChunk), lds, s, pa, items);
   return hipGetLastError();
 }
 hipError_t launch_mirror(const PairArg &pa, const int4 *items, int nitems, bool fold, hipStream_t s) {

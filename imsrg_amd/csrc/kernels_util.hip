@@ -107,8 +107,8 @@ __device__ double sixj_dd(const dd *F, int a, int b, int c, int d, int e, int f)
   return r.hi + r.lo;
 }
 
-__global__ __launch_bounds__(256) void k_sixj_table(const dd *F, double *table, double *tableT, int d1, int d2,
-                                                   long total) {
+__global__ __launch_bounds__(256) void k_sixj_table(const dd *F, double *table, double *tableQ, int d1, int d2,
+                                                   int d2q, long total) {
   for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
     long r = idx;
     const int J6 = r % d2; r /= d2;
@@ -119,11 +119,11 @@ __global__ __launch_bounds__(256) void k_sixj_table(const dd *F, double *table, 
     const int i1 = (int)r;
     const double v = sixj_dd(F, 2 * i1 + 1, 2 * i2 + 1, 2 * J3, 2 * i4 + 1, 2 * i5 + 1, 2 * J6);
     table[idx] = v;
-    tableT[sixjT_index(d1, d2, 2 * i1 + 1, 2 * i2 + 1, J3, 2 * i4 + 1, 2 * i5 + 1, J6)] = v;
+    tableQ[sixjQ_index(d1, d2q, 2 * i1 + 1, 2 * i2 + 1, J3, 2 * i4 + 1, 2 * i5 + 1, J6)] = v;
   }
 }
 
-hipError_t launch_sixj_table(double *table, double *tableT, int d1, int d2, hipStream_t s) {
+hipError_t launch_sixj_table(double *table, double *tableQ, int d1, int d2, int d2q, hipStream_t s) {
   if (4 * (d1 - 1) + 4 > kNFact - 4) return hipErrorInvalidValue;
   dd *F = nullptr;
   hipError_t e = hipMalloc(&F, sizeof(dd) * kNFact);
@@ -131,7 +131,7 @@ hipError_t launch_sixj_table(double *table, double *tableT, int d1, int d2, hipS
   hipLaunchKernelGGL(k_factorials, dim3(1), dim3(64), 0, s, F);
   long total = (long)d1 * d1 * d2 * d1 * d1 * d2;
   int blocks = (int)std::min<long>((total + 255) / 256, 65536);
-  hipLaunchKernelGGL(k_sixj_table, dim3(blocks), dim3(256), 0, s, F, table, tableT, d1, d2, total);
+  hipLaunchKernelGGL(k_sixj_table, dim3(blocks), dim3(256), 0, s, F, table, tableQ, d1, d2, d2q, total);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   e = hipStreamSynchronize(s);

@@ -6,8 +6,9 @@
 
 namespace imsrg {
 
-// dense 6j table {jj1 jj2 J3; jj4 jj5 J6} (double-double Racah), layout of SixJCache_112112
-hipError_t launch_sixj_table(double *table, double *tableT, int d1, int d2, hipStream_t s);
+// dense 6j table {jj1 jj2 J3; jj4 jj5 J6} (double-double Racah): `table` in the layout of
+// SixJCache_112112, `tableQ` in the sixjQ_index layout (padding entries are left untouched)
+hipError_t launch_sixj_table(double *table, double *tableQ, int d1, int d2, int d2q, hipStream_t s);
 
 // commutator phases; items are device arrays of int4 work items
 hipError_t launch_zerobody(const PairArg &pa, int npairs, int maxnch, hipStream_t s);
@@ -19,7 +20,9 @@ hipError_t launch_c122(const PairArg &pa, const int4 *items, int nitems, int max
                        hipStream_t s);
 hipError_t launch_pandya(const PairArg &pa, const int4 *items, int nitems, hipStream_t s);
 hipError_t launch_ph_gemm(const PairArg &pa, const int4 *items, int nitems, int precision, hipStream_t s);
-hipError_t launch_invpandya(const PairArg &pa, const int4 *items, int nitems, hipStream_t s);
+// jmax, norb, nch: the largest of the pairs (they size the kernel's LDS tables)
+hipError_t launch_invpandya(const PairArg &pa, const int4 *items, int nitems, int jmax, int norb, int nch,
+                            hipStream_t s);
 // lower triangle <- hZ * upper; with fold, first upper += the inverse-Pandya
 // contributions stored at the transposed slots
 hipError_t launch_mirror(const PairArg &pa, const int4 *items, int nitems, bool fold, hipStream_t s);
