@@ -884,7 +884,7 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
       bc.maxnorb = std::max(bc.maxnorb, ctxs[i]->db.h.norb);
       bc.maxn122 = std::max(bc.maxn122, ctxs[i]->db.h.n122);
     }
-    if (c0->db.tune.pphh_order == 1) {
+    if (c0->db.tune.pphh_order >= 1) {
       all[0] = xcd_order(groups, kXcd);
     } else {  // groups by decreasing k-length, as for a single commutator
       std::stable_sort(groups.begin(), groups.end(), [](const TileGroup &a, const TileGroup &b) {
@@ -945,7 +945,7 @@ extern "C" int imsrg_hip_get_tuning(imsrg_ctx *ctx, imsrg_tuning *t) {
 
 extern "C" int imsrg_hip_set_tuning(imsrg_ctx *ctx, const imsrg_tuning *t) {
   if (!ctx || !t) return IMSRG_E_INVALID;
-  if (t->pphh_small_tile_max < 0 || (t->pphh_order != 0 && t->pphh_order != 1) ||
+  if (t->pphh_small_tile_max < 0 || t->pphh_order < 0 || t->pphh_order > 2 ||
       (t->mixed_pphh_fp32 != 0 && t->mixed_pphh_fp32 != 1) || (t->mixed_ph_fp32 != 0 && t->mixed_ph_fp32 != 1))
     return fail(ctx, IMSRG_E_INVALID, "invalid tuning values");
   Tuning n;
@@ -1019,7 +1019,7 @@ extern "C" int imsrg_hip_autotune(imsrg_ctx *ctx, int precision, const char *cac
   }
   std::vector<Tuning> cand;
   for (int sm : smalls)
-    for (int order = 0; order < 2; ++order)
+    for (int order = 0; order < 3; ++order)
       for (int mp = 0; mp < (precision == IMSRG_PREC_MIXED ? 2 : 1); ++mp)
         for (int mh = 0; mh < (precision == IMSRG_PREC_MIXED ? 2 : 1); ++mh) {
           Tuning t;

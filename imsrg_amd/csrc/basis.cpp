@@ -444,6 +444,21 @@ std::string DeviceBasis::build_lists(const Tuning &t) {
       }
       const int nt = (nk + kTileB - 1) / kTileB;
       const long kk = k_start[c + 1] - k_start[c];
+      if (t.pphh_order == 2) {
+        // square super-tiles of kSuper x kSuper tiles: their work-groups run
+        // together on one XCD and share the A and B row blocks in its L2
+        for (int rs = 0; rs < nt; rs += kSuper)
+          for (int ss = rs; ss < nt; ss += kSuper) {
+            TileGroup g;
+            for (int rb = rs; rb < std::min(rs + kSuper, nt); ++rb)
+              for (int sb = std::max(rb, ss); sb < std::min(ss + kSuper, nt); ++sb)
+                g.tiles.push_back(make_int4(0, c, rb, sb));
+            if (g.tiles.empty()) continue;
+            g.cost = (kk + 1) * (long)g.tiles.size();
+            groups.push_back(g);
+          }
+        continue;
+      }
       for (int rb = 0; rb < nt; ++rb) {
         TileGroup g;
         for (int sb = rb; sb < nt; ++sb) g.tiles.push_back(make_int4(0, c, rb, sb));
@@ -451,7 +466,7 @@ std::string DeviceBasis::build_lists(const Tuning &t) {
         groups.push_back(g);
       }
     }
-    if (t.pphh_order == 1) {
+    if (t.pphh_order >= 1) {
       wl.pphh = xcd_order(groups, kXcd);
     } else {
       std::vector<std::pair<long, int4>> ts;

@@ -34,7 +34,8 @@ struct HostBasis {
                         std::vector<int> &tz2, std::vector<double> &occ, std::vector<int> &cvq);
 };
 
-constexpr int kXcd = 8;  // XCDs of an MI355X; workgroup i runs on XCD i % 8
+constexpr int kXcd = 8;    // XCDs of an MI355X; workgroup i runs on XCD i % 8
+constexpr int kSuper = 8;  // edge (in tiles) of the pp/hh super-tiles of tile order 2
 
 struct TileGroup {
   std::vector<int4> tiles;  // tiles sharing one 128-row block of one channel
@@ -46,7 +47,8 @@ std::vector<int4> xcd_order(std::vector<TileGroup> groups, int nxcd);
 // Launch choices that the autotuner sets (defaults: measured best at emax=10).
 struct Tuning {
   int pphh_small = 0;  // channels with n <= pphh_small take 64x64 pp/hh tiles, the others 128x128
-  int pphh_order = 0;  // 0: tiles by decreasing k-length, 1: row-block groups balanced over the XCDs
+  int pphh_order = 0;  // 0: tiles by decreasing k-length, 1: row-block groups balanced over the XCDs,
+                       // 2: square super-tiles balanced over the XCDs
   int mixed_pphh = 1;  // mixed precision: 1 = FP32-MFMA pp/hh kernel, 0 = the FP64 kernel
   int mixed_ph = 1;    // mixed precision: 1 = FP32-MFMA ph kernel, 0 = the FP64 kernel
 };

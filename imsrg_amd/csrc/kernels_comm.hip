@@ -308,8 +308,17 @@ __global__ __launch_bounds__(256) void k_syr2k(PairArg pa, const int4 *items, in
     phl = B.cc_ph + B.cc_phstart[cc];
   }
   const int R0 = it.z * kTile, S0 = it.w * kTile;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int wr = (wv & 1) * 32, wc = (wv >> 1) * 32;
+  // 16x16 blocks with upper-triangle elements inside the matrix (the others issue no MFMA)
+  unsigned bm = 0;
+#pragma unroll
+  for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 2; ++ni) {
+      const int r0 = R0 + wr + mi * 16, c0 = S0 + wc + ni * 16;
+      if (r0 < n && c0 < n && r0 <= c0 + 15) bm |= 1u << (mi * 2 + ni);
+    }
   dbl4 acc[2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
@@ -378,10 +387,10 @@ __global__ __launch_bounds__(256) void k_syr2k(PairArg pa, const int4 *items, in
         const float b0 = Bs[cur][kr][wc + (lane & 15)], b1 = Bs[cur][kr][wc + 16 + (lane & 15)];
         const bool fresh = ks % kF32Flush == 0;
         const flt4 fz = {0.f, 0.f, 0.f, 0.f};
-        t[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(b0, a0, fresh ? fz : t[0][0], 0, 0, 0);
-        t[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(b1, a0, fresh ? fz : t[0][1], 0, 0, 0);
-        t[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(b0, a1, fresh ? fz : t[1][0], 0, 0, 0);
-        t[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(b1, a1, fresh ? fz : t[1][1], 0, 0, 0);
+        if (bm & 1) t[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(b0, a0, fresh ? fz : t[0][0], 0, 0, 0);
+        if (bm & 2) t[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(b1, a0, fresh ? fz : t[0][1], 0, 0, 0);
+        if (bm & 4) t[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(b0, a1, fresh ? fz : t[1][0], 0, 0, 0);
+        if (bm & 8) t[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(b1, a1, fresh ? fz : t[1][1], 0, 0, 0);
         if (ks % kF32Flush == kF32Flush - 1) {
 #pragma unroll
           for (int mi = 0; mi < 2; ++mi)
@@ -398,10 +407,10 @@ __global__ __launch_bounds__(256) void k_syr2k(PairArg pa, const int4 *items, in
         const double a0 = As[cur][kr][wr + (lane & 15)], a1 = As[cur][kr][wr + 16 + (lane & 15)];
         const double b0 = Bs[cur][kr][wc + (lane & 15)], b1 = Bs[cur][kr][wc + 16 + (lane & 15)];
         // D^T blocks: lane l, register q holds D row (lane&15), column mfma_col(l, q)
-        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc[0][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a0, acc[0][1], 0, 0, 0);
-        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a1, acc[1][0], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a1, acc[1][1], 0, 0, 0);
+        if (bm & 1) acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc[0][0], 0, 0, 0);
+        if (bm & 2) acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a0, acc[0][1], 0, 0, 0);
+        if (bm & 4) acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a1, acc[1][0], 0, 0, 0);
+        if (bm & 8) acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a1, acc[1][1], 0, 0, 0);
       }
     }
     if (more) {
@@ -502,8 +511,11 @@ __device__ __forceinline__ void pphh_tile(PairArg pa, const int4 *items) {
   const double h = (double)P.hZ;
   double *D = P.Z + B.two_off + off;
   const int R0 = it.z * T, S0 = it.w * T;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int wr = (wv & 1) * 64, wc = (wv >> 1) * 64;
+  // a wave whose 64x64 sub-tile holds no upper-triangle element inside the
+  // matrix (padding past n, or below the diagonal) issues no MFMA
+  const bool wave_live = R0 + wr < n && S0 + wc < n && R0 + wr <= S0 + wc + 63;
   dbl4 acc[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
@@ -516,43 +528,61 @@ __device__ __forceinline__ void pphh_tile(PairArg pa, const int4 *items) {
   const int trow = threadIdx.x & (T - 1), kh = threadIdx.x >> 7;
   const int K2 = 2 * nk;
   const int ra = min(R0 + trow, n - 1), rb = min(S0 + trow, n - 1);
-  auto load_stage = [&](int k0, double (&av)[8], double (&bv)[8]) {
+  // the column indices kl[k] of a stage are loaded one stage before its data,
+  // so the data loads never wait on an index load
+  auto load_idx = [&](int k0, int (&ki)[8]) {
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const int kk = min(k0 + kh * 8 + q, K2 - 1);
-      const int k = kk < nk ? kk : kk - nk;
-      const long c = (long)kl[k] * n;
+      ki[q] = kl[kk < nk ? kk : kk - nk];
+    }
+  };
+  auto load_stage = [&](int k0, const int (&ki)[8], double (&av)[8], double (&bv)[8]) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int kk = min(k0 + kh * 8 + q, K2 - 1);
+      const long c = (long)ki[q] * n;
       av[q] = (kk < nk ? Am : Bm)[c + ra];
       bv[q] = (kk < nk ? Bm : Am)[c + rb];
     }
   };
-  auto load_w = [&](int k0, int buf) {
+  // the weight of k' = k0 + t (t < 16) is loaded into a register and only
+  // scaled and stored to LDS with the stage's operands
+  auto load_w = [&](int k0) -> double {
+    const int kk = min(k0 + (int)(threadIdx.x & (kTileK - 1)), K2 - 1);
+    return kw[kk < nk ? kk : kk - nk];
+  };
+  auto store_w = [&](int k0, int buf, double w) {
     if (threadIdx.x < kTileK) {
       const int kk = k0 + threadIdx.x;
-      double w = 0.0;
-      if (kk < K2) w = kk < nk ? kw[kk] : kw[kk - nk] * h;
-      Ws[buf][threadIdx.x] = w;
+      Ws[buf][threadIdx.x] = kk >= K2 ? 0.0 : kk < nk ? w : w * h;
     }
   };
   double av[8], bv[8];
+  int ki[8];
+  double wv_next = 0.0;
   if (K2 > 0) {  // a channel without pp or hh kets only writes zeros
-    load_stage(0, av, bv);
-    load_w(0, 0);
+    load_idx(0, ki);
+    load_stage(0, ki, av, bv);
+    store_w(0, 0, load_w(0));
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       As[0][kh * 8 + q][trow] = av[q];
       Bs[0][kh * 8 + q][trow] = bv[q];
     }
+    if (kTileK < K2) load_idx(kTileK, ki);
   }
   __syncthreads();
   int cur = 0;
   for (int k0 = 0; k0 < K2; k0 += kTileK) {
     const bool more = k0 + kTileK < K2;
     if (more) {
-      load_stage(k0 + kTileK, av, bv);
-      load_w(k0 + kTileK, cur ^ 1);
+      load_stage(k0 + kTileK, ki, av, bv);
+      wv_next = load_w(k0 + kTileK);
+      if (k0 + 2 * kTileK < K2) load_idx(k0 + 2 * kTileK, ki);
     }
-    if constexpr (MIXED) {
+    if (!wave_live) {
+    } else if constexpr (MIXED) {
       // one row block mi at a time, operands re-read from LDS, so that only four
       // FP32 partial blocks are live next to the FP64 accumulators
 #pragma unroll
@@ -601,6 +631,7 @@ __device__ __forceinline__ void pphh_tile(PairArg pa, const int4 *items) {
         As[cur ^ 1][kh * 8 + q][trow] = av[q];
         Bs[cur ^ 1][kh * 8 + q][trow] = bv[q];
       }
+      store_w(k0 + kTileK, cur ^ 1, wv_next);
     }
     __syncthreads();
     cur ^= 1;

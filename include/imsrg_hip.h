@@ -182,7 +182,8 @@ int imsrg_hip_set_graphs(imsrg_ctx *ctx, int on);
  * GEMM family runs on FP32 MFMA; FP64 results agree to rounding. */
 typedef struct {
   int pphh_small_tile_max;  /* channels with n <= this use 64x64 pp/hh tiles, larger ones 128x128 */
-  int pphh_order;           /* 0: pp/hh tiles by decreasing k-length; 1: row-block groups balanced over XCDs */
+  int pphh_order;           /* 0: pp/hh tiles by decreasing k-length; 1: row-block groups balanced over XCDs;
+                               2: 8x8-tile super-tiles balanced over XCDs (shared operand blocks in one L2) */
   int mixed_pphh_fp32;      /* IMSRG_PREC_MIXED: 1 = pp/hh GEMM on FP32 MFMA, 0 = on FP64 MFMA */
   int mixed_ph_fp32;        /* IMSRG_PREC_MIXED: 1 = ph GEMM on FP32 MFMA, 0 = on FP64 MFMA */
   double us_per_commutator; /* autotune: measured time of the chosen setting (0 when set by hand) */

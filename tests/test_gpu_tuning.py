@@ -41,7 +41,7 @@ def test_every_setting_matches_oracle(ia):
     default = ms.get_tuning()
     try:
         for small in (0, 64, 128):
-            for order in (0, 1):
+            for order in (0, 1, 2):
                 say(f"[gpu] tuning pphh_small={small} order={order}")
                 ms.set_tuning(pphh_small_tile_max=small, pphh_order=order)
                 Z = ia.Commutator.CommutatorScalarScalar(dX, dY).to_host()
