@@ -163,7 +163,9 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
   B.one_off = 8;
   long n2 = (long)hb.norb * hb.norb;
   B.two_off = 8 + ((n2 + 7) / 8) * 8;
-  B.opsize = B.two_off + ((hb.tbsize + 7) / 8) * 8;
+  // 8 doubles of slack after the two-body arena: the pp/hh tile loader copies
+  // row pairs and may read one element past the last column of a channel
+  B.opsize = B.two_off + ((hb.tbsize + 7) / 8) * 8 + 8;
   const int norb = hb.norb, nch = hb.nch;
 
   B.j2 = dupload(*this, hb.j2, err); B.l = dupload(*this, hb.l, err); B.tz2 = dupload(*this, hb.tz2, err);

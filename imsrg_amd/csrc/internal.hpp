@@ -43,7 +43,7 @@ struct BasisDev {
   long tbsize;   // doubles in the two-body arena
   long one_off;  // offset of the one-body block inside an operator buffer
   long two_off;  // offset of the two-body arena inside an operator buffer
-  long opsize;   // doubles per operator buffer
+  long opsize;   // doubles per operator buffer (including 8 doubles of zero slack at the end)
   int maxn, maxcc, maxblk;
   long n122;           // comm122 ELL entries (including padding)
   const IMSRG_G int *j2, *l, *tz2, *cvq;

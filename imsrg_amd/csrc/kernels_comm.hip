@@ -493,10 +493,11 @@ __global__ __launch_bounds__(256, 1) void k_pphh_big_mixed(PairArg pa, const int
 template <bool MIXED>
 __device__ __forceinline__ void pphh_tile(PairArg pa, const int4 *items) {
   constexpr int T = kTileB, LD = kTileB + 4;
-  using ST = typename std::conditional<MIXED, float, double>::type;
-  __shared__ ST As[2][kTileK][LD];
-  __shared__ ST Bs[2][kTileK][LD];
-  __shared__ ST Ws[2][kTileK];
+  // operands stay FP64 in LDS (they arrive by direct global->LDS copies); the
+  // mixed kernel rounds them to FP32 when it reads them
+  __shared__ __align__(16) double As[2][kTileK][LD];
+  __shared__ __align__(16) double Bs[2][kTileK][LD];
+  __shared__ double Ws[2][kTileK];
   const int4 it = items[blockIdx.x];
   if (it.y < 0) return;  // padding item of the XCD-balanced order
   PairPtrs P = pa.get(it.x);
@@ -521,33 +522,37 @@ __device__ __forceinline__ void pphh_tile(PairArg pa, const int4 *items) {
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
-  // loader: thread t owns row (t & 127) of the A and B row blocks for the eight
-  // k' values kh*8 .. kh*8+7 (kh = t >> 7); loads are plain copies from clamped
-  // addresses (padding rows are never written back, padding k' get weight 0),
-  // so no arithmetic waits on them before they are stored to LDS
-  const int trow = threadIdx.x & (T - 1), kh = threadIdx.x >> 7;
   const int K2 = 2 * nk;
-  const int ra = min(R0 + trow, n - 1), rb = min(S0 + trow, n - 1);
-  // the column indices kl[k] of a stage are loaded one stage before its data,
-  // so the data loads never wait on an index load
-  auto load_idx = [&](int k0, int (&ki)[8]) {
+  // Stage loader: wave w copies the k' columns 4w .. 4w+3 of the stage, for
+  // the A and the B row block, each a 128-row segment of one column, with one
+  // 16-byte-per-lane global->LDS copy (lane l: rows 2l, 2l+1).  Rows past n
+  // are never written back; their source is clamped so that a row pair reads
+  // at most one element past the column (inside the channel arena, or the
+  // operator buffer's slack).  The column indices kl[k] (wave-uniform) are
+  // loaded one stage ahead.
+  const int ra = min(R0 + 2 * lane, n - 1), rb = min(S0 + 2 * lane, n - 1);
+  auto load_idx = [&](int k0, int (&ki)[4]) {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int kk = min(k0 + kh * 8 + q, K2 - 1);
-      ki[q] = kl[kk < nk ? kk : kk - nk];
+    for (int j = 0; j < 4; ++j) {
+      const int kk = min(k0 + wv * 4 + j, K2 - 1);
+      ki[j] = kl[kk < nk ? kk : kk - nk];
     }
   };
-  auto load_stage = [&](int k0, const int (&ki)[8], double (&av)[8], double (&bv)[8]) {
+  auto issue_stage = [&](int k0, const int (&ki)[4], int buf) {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int kk = min(k0 + kh * 8 + q, K2 - 1);
-      const long c = (long)ki[q] * n;
-      av[q] = (kk < nk ? Am : Bm)[c + ra];
-      bv[q] = (kk < nk ? Bm : Am)[c + rb];
+    for (int j = 0; j < 4; ++j) {
+      const int kq = wv * 4 + j;
+      const int kk = min(k0 + kq, K2 - 1);
+      const long c = (long)ki[j] * n;
+      const double *sa = (kk < nk ? Am : Bm) + c, *sb = (kk < nk ? Bm : Am) + c;
+      __builtin_amdgcn_global_load_lds((const IMSRG_G void *)(sa + ra), (__attribute__((address_space(3))) void *)&As[buf][kq][0],
+                                       16, 0, 0);
+      __builtin_amdgcn_global_load_lds((const IMSRG_G void *)(sb + rb), (__attribute__((address_space(3))) void *)&Bs[buf][kq][0],
+                                       16, 0, 0);
     }
   };
   // the weight of k' = k0 + t (t < 16) is loaded into a register and only
-  // scaled and stored to LDS with the stage's operands
+  // scaled and stored to LDS at the end of the stage
   auto load_w = [&](int k0) -> double {
     const int kk = min(k0 + (int)(threadIdx.x & (kTileK - 1)), K2 - 1);
     return kw[kk < nk ? kk : kk - nk];
@@ -558,27 +563,22 @@ __device__ __forceinline__ void pphh_tile(PairArg pa, const int4 *items) {
       Ws[buf][threadIdx.x] = kk >= K2 ? 0.0 : kk < nk ? w : w * h;
     }
   };
-  double av[8], bv[8];
-  int ki[8];
-  double wv_next = 0.0;
+  int ki[4];
+  double w_next = 0.0;
   if (K2 > 0) {  // a channel without pp or hh kets only writes zeros
     load_idx(0, ki);
-    load_stage(0, ki, av, bv);
+    issue_stage(0, ki, 0);
     store_w(0, 0, load_w(0));
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      As[0][kh * 8 + q][trow] = av[q];
-      Bs[0][kh * 8 + q][trow] = bv[q];
-    }
     if (kTileK < K2) load_idx(kTileK, ki);
   }
+  __builtin_amdgcn_s_waitcnt(0);  // the copies into LDS have landed
   __syncthreads();
   int cur = 0;
   for (int k0 = 0; k0 < K2; k0 += kTileK) {
     const bool more = k0 + kTileK < K2;
     if (more) {
-      load_stage(k0 + kTileK, ki, av, bv);
-      wv_next = load_w(k0 + kTileK);
+      issue_stage(k0 + kTileK, ki, cur ^ 1);
+      w_next = load_w(k0 + kTileK);
       if (k0 + 2 * kTileK < K2) load_idx(k0 + 2 * kTileK, ki);
     }
     if (!wave_live) {
@@ -595,11 +595,11 @@ __device__ __forceinline__ void pphh_tile(PairArg pa, const int4 *items) {
 #pragma unroll
           for (int ks = kp; ks < kp + kF32Flush; ++ks) {
             const int kr = ks * 4 + (lane >> 4);
-            const float am = As[cur][kr][wr + mi * 16 + (lane & 15)] * Ws[cur][kr];
+            const float am = (float)(As[cur][kr][wr + mi * 16 + (lane & 15)] * Ws[cur][kr]);
 #pragma unroll
             for (int ni = 0; ni < 4; ++ni)
-              t[ni] = __builtin_amdgcn_mfma_f32_16x16x4f32(Bs[cur][kr][wc + ni * 16 + (lane & 15)], am, t[ni], 0,
-                                                           0, 0);
+              t[ni] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)Bs[cur][kr][wc + ni * 16 + (lane & 15)], am, t[ni],
+                                                           0, 0, 0);
           }
 #pragma unroll
           for (int ni = 0; ni < 4; ++ni)
@@ -625,14 +625,8 @@ __device__ __forceinline__ void pphh_tile(PairArg pa, const int4 *items) {
             acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[ni], a[mi], acc[mi][ni], 0, 0, 0);
       }
     }
-    if (more) {
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        As[cur ^ 1][kh * 8 + q][trow] = av[q];
-        Bs[cur ^ 1][kh * 8 + q][trow] = bv[q];
-      }
-      store_w(k0 + kTileK, cur ^ 1, wv_next);
-    }
+    if (more) store_w(k0 + kTileK, cur ^ 1, w_next);
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of the next stage have landed
     __syncthreads();
     cur ^= 1;
   }
