@@ -109,8 +109,9 @@ static int ctx_init(imsrg_ctx *ctx, const imsrg_basis_desc *b, int device) {
   if (!e.empty()) return fail(ctx, IMSRG_E_HIP, e);
   HIPCHK(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   const BasisDev &B = ctx->db.h;
-  HIPCHK(ctx, hipMalloc(&ctx->Xt, sizeof(double) * std::max<long>(B.xsize, 1)));
-  HIPCHK(ctx, hipMalloc(&ctx->Yt, sizeof(double) * std::max<long>(B.xsize, 1)));
+  // 8 doubles of slack: the GEMM tile loader may read one element past a column
+  HIPCHK(ctx, hipMalloc(&ctx->Xt, sizeof(double) * (std::max<long>(B.xsize, 1) + 8)));
+  HIPCHK(ctx, hipMalloc(&ctx->Yt, sizeof(double) * (std::max<long>(B.xsize, 1) + 8)));
   HIPCHK(ctx, hipMalloc(&ctx->Zbar, sizeof(double) * std::max<long>(B.zsize, 1)));
   // cross-coupled channels without ph kets are never written and must read as 0
   HIPCHK(ctx, hipMemset(ctx->Zbar, 0, sizeof(double) * std::max<long>(B.zsize, 1)));
@@ -457,7 +458,7 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
   HIPCHK(ctx, mark(4));
   if (mask & MPH) HIPCHK(ctx, launch_pandya(pa, items[3], cnt[3], s));
   HIPCHK(ctx, mark(5));
-  if (mask & MPH) HIPCHK(ctx, launch_ph_gemm(pa, items[1], cnt[1], prec_ph, s));
+  if (mask & MPH) HIPCHK(ctx, launch_ph_gemm(pa, items[1], cnt[1], prec_ph, tu.ph_big != 0, s));
   HIPCHK(ctx, mark(6));
   if (mask & MPH) HIPCHK(ctx, launch_invpandya(pa, items[4], cnt[4], maxjmax, maxnorb, maxnch, s));
   HIPCHK(ctx, mark(7));
@@ -863,7 +864,18 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
       const WorkLists &wl = ctxs[i]->db.wl;
       const std::vector<int4> *src[8] = {&wl.pphh, &wl.ph, &wl.c122, &wl.pandya, &wl.invp, &wl.one, &wl.mirror,
                                          &wl.pphh64};
-      for (int t = 1; t < 8; ++t)
+      {  // ph tiles in the tile size of the first context, whose kernels run the batch
+        const int tile = c0->db.tune.ph_big ? kTileB : kTile;
+        const DeviceBasis &dbi = ctxs[i]->db;
+        for (int c = 0; c < (int)dbi.cc_n.size(); ++c) {
+          const int nk = dbi.cc_n[c];
+          if (nk == 0 || dbi.cc_nph[c] == 0) continue;
+          const int nt = (nk + tile - 1) / tile;
+          for (int rb = 0; rb < nt; ++rb)
+            for (int sb = rb; sb < nt; ++sb) all[1].push_back(make_int4(i, c, rb, sb));
+        }
+      }
+      for (int t = 2; t < 8; ++t)
         for (int4 w : *src[t]) {
           w.x = i;
           all[t].push_back(w);
@@ -922,6 +934,7 @@ static void tuning_out(const Tuning &t, double us, imsrg_tuning *o) {
   o->pphh_order = t.pphh_order;
   o->mixed_pphh_fp32 = t.mixed_pphh;
   o->mixed_ph_fp32 = t.mixed_ph;
+  o->ph_tile_128 = t.ph_big;
   o->us_per_commutator = us;
 }
 
@@ -946,13 +959,15 @@ extern "C" int imsrg_hip_get_tuning(imsrg_ctx *ctx, imsrg_tuning *t) {
 extern "C" int imsrg_hip_set_tuning(imsrg_ctx *ctx, const imsrg_tuning *t) {
   if (!ctx || !t) return IMSRG_E_INVALID;
   if (t->pphh_small_tile_max < 0 || t->pphh_order < 0 || t->pphh_order > 2 ||
-      (t->mixed_pphh_fp32 != 0 && t->mixed_pphh_fp32 != 1) || (t->mixed_ph_fp32 != 0 && t->mixed_ph_fp32 != 1))
+      (t->mixed_pphh_fp32 != 0 && t->mixed_pphh_fp32 != 1) || (t->mixed_ph_fp32 != 0 && t->mixed_ph_fp32 != 1) ||
+      (t->ph_tile_128 != 0 && t->ph_tile_128 != 1))
     return fail(ctx, IMSRG_E_INVALID, "invalid tuning values");
   Tuning n;
   n.pphh_small = t->pphh_small_tile_max;
   n.pphh_order = t->pphh_order;
   n.mixed_pphh = t->mixed_pphh_fp32;
   n.mixed_ph = t->mixed_ph_fp32;
+  n.ph_big = t->ph_tile_128;
   return apply_tuning(ctx, n);
 }
 
@@ -992,8 +1007,8 @@ extern "C" int imsrg_hip_autotune(imsrg_ctx *ctx, int precision, const char *cac
       bool found = false;
       while (std::fgets(line, sizeof(line), f)) {
         if (std::strncmp(line, key.c_str(), key.size()) != 0 || line[key.size()] != ' ') continue;
-        if (std::sscanf(line + key.size(), " pphh_small=%d pphh_order=%d mixed_pphh=%d mixed_ph=%d us=%lf",
-                        &t.pphh_small, &t.pphh_order, &t.mixed_pphh, &t.mixed_ph, &us) == 5)
+        if (std::sscanf(line + key.size(), " pphh_small=%d pphh_order=%d mixed_pphh=%d mixed_ph=%d ph_big=%d us=%lf",
+                        &t.pphh_small, &t.pphh_order, &t.mixed_pphh, &t.mixed_ph, &t.ph_big, &us) == 6)
           found = true;  // the last matching line wins
       }
       std::fclose(f);
@@ -1021,14 +1036,16 @@ extern "C" int imsrg_hip_autotune(imsrg_ctx *ctx, int precision, const char *cac
   for (int sm : smalls)
     for (int order = 0; order < 3; ++order)
       for (int mp = 0; mp < (precision == IMSRG_PREC_MIXED ? 2 : 1); ++mp)
-        for (int mh = 0; mh < (precision == IMSRG_PREC_MIXED ? 2 : 1); ++mh) {
-          Tuning t;
-          t.pphh_small = sm;
-          t.pphh_order = order;
-          t.mixed_pphh = precision == IMSRG_PREC_MIXED ? mp : 1;
-          t.mixed_ph = precision == IMSRG_PREC_MIXED ? mh : 1;
-          cand.push_back(t);
-        }
+        for (int mh = 0; mh < (precision == IMSRG_PREC_MIXED ? 2 : 1); ++mh)
+          for (int pb = 0; pb < 2; ++pb) {
+            Tuning t;
+            t.pphh_small = sm;
+            t.pphh_order = order;
+            t.mixed_pphh = precision == IMSRG_PREC_MIXED ? mp : 1;
+            t.mixed_ph = precision == IMSRG_PREC_MIXED ? mh : 1;
+            t.ph_big = pb;
+            cand.push_back(t);
+          }
   imsrg_dop *X = nullptr, *Y = nullptr, *Z = nullptr;
   int rc = imsrg_hip_op_create(ctx, -1, &X);
   if (!rc) rc = imsrg_hip_op_create(ctx, 1, &Y);
@@ -1071,8 +1088,8 @@ extern "C" int imsrg_hip_autotune(imsrg_ctx *ctx, int precision, const char *cac
   if ((rc = apply_tuning(ctx, best))) return rc;
   if (cache_path) {
     if (FILE *f = std::fopen(cache_path, "a")) {
-      std::fprintf(f, "%s pphh_small=%d pphh_order=%d mixed_pphh=%d mixed_ph=%d us=%.3f\n", key.c_str(),
-                   best.pphh_small, best.pphh_order, best.mixed_pphh, best.mixed_ph, best_us);
+      std::fprintf(f, "%s pphh_small=%d pphh_order=%d mixed_pphh=%d mixed_ph=%d ph_big=%d us=%.3f\n", key.c_str(),
+                   best.pphh_small, best.pphh_order, best.mixed_pphh, best.mixed_ph, best.ph_big, best_us);
       std::fclose(f);
     }
   }

@@ -504,7 +504,8 @@ std::string DeviceBasis::build_lists(const Tuning &t) {
   for (int c = 0; c < nch; ++c) {
     const int nk = cc_n[c], nph = cc_nph[c];
     if (nk == 0 || nph == 0) continue;
-    const int nt = (nk + kTile - 1) / kTile;
+    const int tile = t.ph_big ? kTileB : kTile;
+    const int nt = (nk + tile - 1) / tile;
     for (int rb = 0; rb < nt; ++rb)
       for (int sb = rb; sb < nt; ++sb) wl.ph.push_back(make_int4(0, c, rb, sb));
     const long tot = (long)nk * 2 * nph;

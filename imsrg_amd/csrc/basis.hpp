@@ -51,13 +51,14 @@ struct Tuning {
                        // 2: square super-tiles balanced over the XCDs
   int mixed_pphh = 1;  // mixed precision: 1 = FP32-MFMA pp/hh kernel, 0 = the FP64 kernel
   int mixed_ph = 1;    // mixed precision: 1 = FP32-MFMA ph kernel, 0 = the FP64 kernel
+  int ph_big = 1;      // ph GEMM on 128x128 tiles (1) or 64x64 tiles (0)
 };
 
 // Work-item lists (int4: pair, a, b, c) for every kernel of one commutator.
 struct WorkLists {
   std::vector<int4> pphh;    // (pair, ch, rb, sb), rb <= sb, 128x128 tiles
   std::vector<int4> pphh64;  // (pair, ch, rb, sb), rb <= sb, 64x64 tiles (channels n <= pphh_small)
-  std::vector<int4> ph;      // (pair, cc, rb, sb), rb <= sb
+  std::vector<int4> ph;      // (pair, cc, rb, sb), rb <= sb, tiles of 128 (ph_big) or 64
   std::vector<int4> c122;    // (pair, ch, col, 0)
   std::vector<int4> pandya;  // (pair, cc, start, 0)
   std::vector<int4> invp;    // (pair, group, bra, first ket) of the (parity, Tz) ket groups

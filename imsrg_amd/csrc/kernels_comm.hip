@@ -474,24 +474,26 @@ __global__ __launch_bounds__(256) void k_syr2k(PairArg pa, const int4 *items, in
 
 
 // ---------------------------------------------------------------------------
-// pp/hh SYR2K with 128x128 output tiles: each of the 4 waves owns a 64x64
-// sub-tile (4x4 MFMA blocks), so one LDS read feeds four MFMAs; k-stages of
-// 16 are double-buffered with the next stage's global loads in flight.
-//   Z(r,s) = hY sum_k w_k [X(r,k) Y(s,k) + hZ Y(r,k) X(s,k)],  r <= s
-// (comm222_pp_hh_221ss two-body part, src/Commutator.cc:1171-1194, 1237-1238).
-// MIXED: as in k_syr2k.  The FP64 accumulators, FP32 partials and the staged
-// loads need more than the 256 registers of two waves per SIMD, so the mixed
-// instantiation runs one wave per SIMD.
-template <bool MIXED>
-__device__ __forceinline__ void pphh_tile(PairArg pa, const int4 *items);
-__global__ __launch_bounds__(256, 2) void k_pphh_big_f64(PairArg pa, const int4 *items) {
-  pphh_tile<false>(pa, items);
-}
-__global__ __launch_bounds__(256, 1) void k_pphh_big_mixed(PairArg pa, const int4 *items) {
-  pphh_tile<true>(pa, items);
-}
-template <bool MIXED>
-__device__ __forceinline__ void pphh_tile(PairArg pa, const int4 *items) {
+// SYR2K on 128x128 output tiles: each of the 4 waves owns a 64x64 sub-tile
+// (4x4 MFMA blocks), so one LDS read feeds four MFMAs; k-stages of 16 are
+// double-buffered in LDS and filled by direct global->LDS copies issued one
+// stage ahead.
+//   D(r,s) = alpha * cph(s) * sum_k w_k [ A(r,ka_k) B(s,kb_k) + h B(r,kb_k) A(s,ka_k) ],  r <= s
+// MODE 0: pp/hh term of comm222_pp_hh_221ss (src/Commutator.cc:1171-1194, 1237-1238):
+//         A = X2, B = Y2 of one channel, k over pp/hh kets with weight w_pp - w_hh,
+//         alpha = hY, h = hZ; the upper triangle of Z2 is written.
+// MODE 1: head of Zbar in comm222_phss (src/Commutator.cc:1789-1799): A = Xbar,
+//         B = Ybar^T, h = +1 (Z Hermitian) or -1.
+// MODE 2: tail of Zbar (src/Commutator.cc:1788-1789, 1805): B columns permuted
+//         by ab <-> ba with weights hY*phi(ph ket), cph(s) = phi(s), h = +1.
+// MODE 1/2 store Zbar in full: the lower triangle as (head) hZ-sign or (tail)
+// phi_r phi_s times the upper element, as the reference's folds make it
+// (src/Commutator.cc:1792-1805), written through an LDS transpose.
+// MIXED: operands rounded to FP32 when read from LDS, products on f32
+// 16x16x4 MFMA, FP32 partial sums over kF32Flush k-steps added into the FP64
+// accumulators.
+template <int MODE, bool MIXED>
+__device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
   constexpr int T = kTileB, LD = kTileB + 4;
   // operands stay FP64 in LDS (they arrive by direct global->LDS copies); the
   // mixed kernel rounds them to FP32 when it reads them
@@ -502,21 +504,48 @@ __device__ __forceinline__ void pphh_tile(PairArg pa, const int4 *items) {
   if (it.y < 0) return;  // padding item of the XCD-balanced order
   PairPtrs P = pa.get(it.x);
   const BasisDev &B = *P.basis;
-  const int ch = it.y;
-  const int n = B.ch_n[ch];
-  const long off = B.ch_off[ch];
-  const double *Am = P.X + B.two_off + off, *Bm = P.Y + B.two_off + off;
-  const int k0s = B.k_start[ch], nk = B.k_start[ch + 1] - k0s;
-  const int *kl = B.k_loc + k0s;
-  const double *kw = B.k_w + k0s;
-  const double h = (double)P.hZ;
-  double *D = P.Z + B.two_off + off;
+  int n, nk, nph = 0;
+  const double *Am, *Bm;
+  const int *kl = nullptr;
+  const double *kw = nullptr;
+  const int *phl = nullptr;
+  const signed char *phi = nullptr;
+  double alpha, h;
+  double *D;
+  if constexpr (MODE == 0) {
+    const int ch = it.y;
+    n = B.ch_n[ch];
+    const long off = B.ch_off[ch];
+    Am = P.X + B.two_off + off;
+    Bm = P.Y + B.two_off + off;
+    const int k0s = B.k_start[ch];
+    nk = B.k_start[ch + 1] - k0s;
+    kl = B.k_loc + k0s;
+    kw = B.k_w + k0s;
+    alpha = (double)P.hY;
+    h = (double)P.hZ;
+    D = P.Z + B.two_off + off;
+  } else {
+    const int cc = it.y;
+    n = B.cc_n[cc];
+    nph = B.cc_nph[cc];
+    nk = 2 * nph;
+    const long xo = B.cc_xoff[cc];
+    Am = P.Xt + xo;
+    Bm = P.Yt + xo;
+    D = P.Zbar + B.cc_zoff[cc] + (MODE == 2 ? (long)n * n : 0);
+    alpha = 1.0;
+    h = (MODE == 1) ? (P.hZ == 1 ? 1.0 : -1.0) : 1.0;
+    phi = B.cc_phi + B.cc_kstart[cc];
+    phl = B.cc_ph + B.cc_phstart[cc];
+  }
   const int R0 = it.z * T, S0 = it.w * T;
   const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int wr = (wv & 1) * 64, wc = (wv >> 1) * 64;
   // a wave whose 64x64 sub-tile holds no upper-triangle element inside the
   // matrix (padding past n, or below the diagonal) issues no MFMA
-  const bool wave_live = R0 + wr < n && S0 + wc < n && R0 + wr <= S0 + wc + 63;
+  auto live = [&](int w) { return R0 + (w & 1) * 64 < n && S0 + (w >> 1) * 64 < n && R0 + (w & 1) * 64 <= S0 + (w >> 1) * 64 + 63; };
+  const bool wave_live = live(wv);
   dbl4 acc[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
@@ -527,15 +556,15 @@ __device__ __forceinline__ void pphh_tile(PairArg pa, const int4 *items) {
   // the A and the B row block, each a 128-row segment of one column, with one
   // 16-byte-per-lane global->LDS copy (lane l: rows 2l, 2l+1).  Rows past n
   // are never written back; their source is clamped so that a row pair reads
-  // at most one element past the column (inside the channel arena, or the
-  // operator buffer's slack).  The column indices kl[k] (wave-uniform) are
-  // loaded one stage ahead.
+  // at most one element past the column (inside the arena, or the buffer's
+  // slack).  Column indices (wave-uniform) are loaded one stage ahead.
   const int ra = min(R0 + 2 * lane, n - 1), rb = min(S0 + 2 * lane, n - 1);
   auto load_idx = [&](int k0, int (&ki)[4]) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int kk = min(k0 + wv * 4 + j, K2 - 1);
-      ki[j] = kl[kk < nk ? kk : kk - nk];
+      const int k = kk < nk ? kk : kk - nk;
+      ki[j] = (MODE == 0) ? kl[k] : k;
     }
   };
   auto issue_stage = [&](int k0, const int (&ki)[4], int buf) {
@@ -543,19 +572,24 @@ __device__ __forceinline__ void pphh_tile(PairArg pa, const int4 *items) {
     for (int j = 0; j < 4; ++j) {
       const int kq = wv * 4 + j;
       const int kk = min(k0 + kq, K2 - 1);
-      const long c = (long)ki[j] * n;
-      const double *sa = (kk < nk ? Am : Bm) + c, *sb = (kk < nk ? Bm : Am) + c;
-      __builtin_amdgcn_global_load_lds((const IMSRG_G void *)(sa + ra), (__attribute__((address_space(3))) void *)&As[buf][kq][0],
-                                       16, 0, 0);
-      __builtin_amdgcn_global_load_lds((const IMSRG_G void *)(sb + rb), (__attribute__((address_space(3))) void *)&Bs[buf][kq][0],
-                                       16, 0, 0);
+      const int ca = ki[j];
+      const int cb = (MODE == 2) ? (ca < nph ? ca + nph : ca - nph) : ca;
+      const double *sa = kk < nk ? Am + (long)ca * n : Bm + (long)cb * n;
+      const double *sb = kk < nk ? Bm + (long)cb * n : Am + (long)ca * n;
+      __builtin_amdgcn_global_load_lds((const IMSRG_G void *)(sa + ra),
+                                       (__attribute__((address_space(3))) void *)&As[buf][kq][0], 16, 0, 0);
+      __builtin_amdgcn_global_load_lds((const IMSRG_G void *)(sb + rb),
+                                       (__attribute__((address_space(3))) void *)&Bs[buf][kq][0], 16, 0, 0);
     }
   };
   // the weight of k' = k0 + t (t < 16) is loaded into a register and only
   // scaled and stored to LDS at the end of the stage
   auto load_w = [&](int k0) -> double {
     const int kk = min(k0 + (int)(threadIdx.x & (kTileK - 1)), K2 - 1);
-    return kw[kk < nk ? kk : kk - nk];
+    const int k = kk < nk ? kk : kk - nk;
+    if constexpr (MODE == 0) return kw[k];
+    if constexpr (MODE == 1) return 1.0;
+    return (double)P.hY * (double)phi[phl[k < nph ? k : k - nph]];
   };
   auto store_w = [&](int k0, int buf, double w) {
     if (threadIdx.x < kTileK) {
@@ -630,17 +664,66 @@ __device__ __forceinline__ void pphh_tile(PairArg pa, const int4 *items) {
     __syncthreads();
     cur ^= 1;
   }
-  const double alpha = (double)P.hY;
+  if constexpr (MODE == 0) {
 #pragma unroll
-  for (int mi = 0; mi < 4; ++mi)
+    for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
-    for (int ni = 0; ni < 4; ++ni)
+      for (int ni = 0; ni < 4; ++ni)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int row = R0 + wr + mi * 16 + (lane & 15);
-        const int col = S0 + wc + ni * 16 + mfma_col<MIXED>(lane, q);
-        if (row < n && col < n && row <= col) D[(long)col * n + row] = alpha * acc[mi][ni][q];
+        for (int q = 0; q < 4; ++q) {
+          const int row = R0 + wr + mi * 16 + (lane & 15);
+          const int col = S0 + wc + ni * 16 + mfma_col<MIXED>(lane, q);
+          if (row < n && col < n && row <= col) D[(long)col * n + row] = alpha * acc[mi][ni][q];
+        }
+  } else {
+    // full Zbar storage through an LDS transpose, one wave sub-tile per round
+    // (the operand buffer As, idle now, holds the 64x65 tile)
+    double(*Tt)[kTile + 1] = reinterpret_cast<double(*)[kTile + 1]>(&As[0][0][0]);
+    for (int w = 0; w < 4; ++w) {
+      if (!live(w)) continue;  // block-uniform
+      const int wr2 = (w & 1) * 64, wc2 = (w >> 1) * 64;
+      if (wv == w) {
+#pragma unroll
+        for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+          for (int ni = 0; ni < 4; ++ni)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const int rr = mi * 16 + (lane & 15), cc = ni * 16 + mfma_col<MIXED>(lane, q);
+              double v = alpha * acc[mi][ni][q];
+              if (MODE == 2) {
+                const int col = S0 + wc2 + cc;
+                v *= (col < n) ? (double)phi[col] : 0.0;
+              }
+              Tt[rr][cc] = v;
+            }
       }
+      __syncthreads();
+      for (int e = threadIdx.x; e < kTile * kTile; e += blockDim.x) {
+        const int rr = e & (kTile - 1), cc = e / kTile;
+        const int row = R0 + wr2 + rr, col = S0 + wc2 + cc;
+        if (row < n && col < n && row <= col) D[(long)col * n + row] = Tt[rr][cc];
+        // mirrored element (S0+wc2+rr, R0+wr2+cc) from the upper element (R0+wr2+cc, S0+wc2+rr)
+        const int row2 = S0 + wc2 + rr, col2 = R0 + wr2 + cc;
+        if (row2 < n && col2 < n && row2 > col2) {
+          const double sym = (MODE == 1) ? h : (double)(phi[row2] * phi[col2]);
+          D[(long)col2 * n + row2] = sym * Tt[cc][rr];
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+__global__ __launch_bounds__(256, 2) void k_pphh_big_f64(PairArg pa, const int4 *items) {
+  gemm_tile<0, false>(pa, items);
+}
+__global__ __launch_bounds__(256, 2) void k_pphh_big_mixed(PairArg pa, const int4 *items) {
+  gemm_tile<0, true>(pa, items);
+}
+template <int MODE, bool MIXED>
+__global__ __launch_bounds__(256, 2) void k_ph_big(PairArg pa, const int4 *items) {
+  gemm_tile<MODE, MIXED>(pa, items);
 }
 
 // ---------------------------------------------------------------------------
@@ -1053,8 +1136,18 @@ hipError_t launch_pphh(const PairArg &pa, const int4 *items, int nitems, const i
   }
   return hipGetLastError();
 }
-hipError_t launch_ph_gemm(const PairArg &pa, const int4 *items, int nitems, int precision, hipStream_t s) {
+hipError_t launch_ph_gemm(const PairArg &pa, const int4 *items, int nitems, int precision, bool big, hipStream_t s) {
   if (nitems <= 0) return hipSuccess;
+  if (big) {
+    if (precision == 1) {
+      hipLaunchKernelGGL((k_ph_big<1, true>), dim3(nitems), dim3(256), 0, s, pa, items);
+      hipLaunchKernelGGL((k_ph_big<2, true>), dim3(nitems), dim3(256), 0, s, pa, items);
+    } else {
+      hipLaunchKernelGGL((k_ph_big<1, false>), dim3(nitems), dim3(256), 0, s, pa, items);
+      hipLaunchKernelGGL((k_ph_big<2, false>), dim3(nitems), dim3(256), 0, s, pa, items);
+    }
+    return hipGetLastError();
+  }
   if (precision == 1) {
     hipLaunchKernelGGL((k_syr2k<1, true>), dim3(nitems), dim3(256), 0, s, pa, items, 0);
     hipLaunchKernelGGL((k_syr2k<2, true>), dim3(nitems), dim3(256), 0, s, pa, items, 0);

@@ -19,7 +19,8 @@ hipError_t launch_pphh(const PairArg &pa, const int4 *items, int nitems, const i
 hipError_t launch_c122(const PairArg &pa, const int4 *items, int nitems, int maxn, int npairs, long maxn122,
                        hipStream_t s);
 hipError_t launch_pandya(const PairArg &pa, const int4 *items, int nitems, hipStream_t s);
-hipError_t launch_ph_gemm(const PairArg &pa, const int4 *items, int nitems, int precision, hipStream_t s);
+// ph GEMM (head and tail of Zbar) on 128x128 tiles (big) or 64x64 tiles
+hipError_t launch_ph_gemm(const PairArg &pa, const int4 *items, int nitems, int precision, bool big, hipStream_t s);
 // jmax, norb, nch: the largest of the pairs (they size the kernel's LDS tables)
 hipError_t launch_invpandya(const PairArg &pa, const int4 *items, int nitems, int jmax, int norb, int nch,
                             hipStream_t s);

@@ -186,6 +186,7 @@ typedef struct {
                                2: 8x8-tile super-tiles balanced over XCDs (shared operand blocks in one L2) */
   int mixed_pphh_fp32;      /* IMSRG_PREC_MIXED: 1 = pp/hh GEMM on FP32 MFMA, 0 = on FP64 MFMA */
   int mixed_ph_fp32;        /* IMSRG_PREC_MIXED: 1 = ph GEMM on FP32 MFMA, 0 = on FP64 MFMA */
+  int ph_tile_128;          /* 1: ph GEMM on 128x128 tiles, 0: on 64x64 tiles */
   double us_per_commutator; /* autotune: measured time of the chosen setting (0 when set by hand) */
 } imsrg_tuning;
 int imsrg_hip_get_tuning(imsrg_ctx *ctx, imsrg_tuning *t);
