@@ -391,9 +391,9 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
   {
     long tot = (long)B.d1 * B.d1 * B.d2 * B.d1 * B.d1 * B.d2;
     B.d2q = (B.d2 + 1) & ~1;
-    // the J-innermost copy: padded rows, plus 4 entries so that a 4-wide J chunk
-    // starting at the last row stays inside the allocation
-    long totq = (long)B.d1 * B.d1 * B.d1 * B.d1 * B.d2q * B.d2q + 4;
+    // the gather copy: padded J3 range, plus one (jj4, jj5) block of J3 pairs so
+    // that a 4-wide J chunk starting at the last pair stays inside the allocation
+    long totq = (long)B.d1 * B.d1 * B.d1 * B.d1 * B.d2q * B.d2q + 2L * B.d1 * B.d1;
     void *p = nullptr, *pQ = nullptr;
     if (hipMalloc(&p, tot * sizeof(double)) != hipSuccess) return "hipMalloc failed for the 6j table";
     allocs.push_back(p);

@@ -85,8 +85,9 @@ struct BasisDev {
   const IMSRG_G int2 *pairs1;   // one-body pairs (i<=j, same (l,j2,tz2) block)
   const IMSRG_G double *w1norm; // norb*norb one-body norm weights
   const IMSRG_G double *sixj;   // dense 6j table, layout of SixJCache_112112
-  const IMSRG_G double *sixjQ;  // same values, layout [jj1][jj2][jj4][jj5][J6][J3] with J3 innermost and rows
-                        // padded to d2q = d2 rounded up to even (16-byte aligned pairs of J3)
+  const IMSRG_G double *sixjQ;  // same values, layout [jj1][jj2][J6][J3/2][jj4][jj5][J3%2] (sixjQ_index):
+                                // lanes of one (jj1, jj2, J6, J3 pair) that differ in jj4, jj5 read
+                                // neighbouring 16-byte entries; d2q = d2 rounded up to even
   int d2q;
   // generator lists: per channel, core-core kets and (non-core)-(non-core) kets
   const IMSRG_G int *gcore_start, *gcore_loc, *gnc_start, *gnc_loc;
@@ -116,8 +117,8 @@ struct PairArg {
 __host__ __device__ inline int ket_index(int norb, int p, int q) { return p * (2 * norb - 1 - p) / 2 + q; }
 __host__ __device__ inline int sgn_parity(int x) { return (x & 1) ? -1 : 1; }
 __host__ __device__ inline long sixjQ_index(int d1, int d2q, int jj1, int jj2, int J3, int jj4, int jj5, int J6) {
-  return (((((long)((jj1 - 1) >> 1) * d1 + ((jj2 - 1) >> 1)) * d1 + ((jj4 - 1) >> 1)) * d1 + ((jj5 - 1) >> 1)) * d2q +
-          J6) * d2q + J3;
+  return ((((((long)((jj1 - 1) >> 1) * d1 + ((jj2 - 1) >> 1)) * d2q + J6) * (d2q >> 1) + (J3 >> 1)) * d1 +
+           ((jj4 - 1) >> 1)) * d1 + ((jj5 - 1) >> 1)) * 2 + (J3 & 1);
 }
 __host__ __device__ inline long sixj_index(int d1, int d2, int jj1, int jj2, int J3, int jj4, int jj5, int J6) {
   return (((((long)((jj1 - 1) >> 1) * d1 + ((jj2 - 1) >> 1)) * d2 + J3) * d1 + ((jj4 - 1) >> 1)) * d1 +

@@ -958,6 +958,9 @@ __global__ __launch_bounds__(256, 6) void k_invpandya(PairArg pa, const int4 *it
     const int xlo = wave_min(Xlo), xhi = wave_max(Xhi);
     const int pd = (B.l[i] + B.l[l]) & 1, td = abs(B.tz2[i] - B.tz2[l]) / 2;
     const int px = (B.l[i] + B.l[k]) & 1, tx = abs(B.tz2[i] - B.tz2[k]) / 2;
+    // 6j entries of J' and the J pair (c0, c0+1) sit at s6 + J' * s6J + (c0/2) * s6P; the next
+    // pair (c0+2, c0+3) one pair stride further
+    const long s6P = 2L * B.d1 * B.d1, s6J = s6P * (d2q >> 1);
     const long s6d = sixjQ_index(B.d1, d2q, jji, jjj, 0, jjk, jjl, 0);
     const long s6x = sixjQ_index(B.d1, d2q, jjj, jji, 0, jjk, jjl, 0);
     // tail element T(ik, lj) = phi_ik phi_lj T(lj, ik), phi_pq = (-1)^{(j_p+j_q)/2};
@@ -987,8 +990,8 @@ __global__ __launch_bounds__(256, 6) void k_invpandya(PairArg pa, const int4 *it
         const int il = Ti[Jp * norb + l];
         const int kj = Tj[Jp * norb + k] + (k > j ? nkc : 0);
         const double z = (2 * Jp + 1) * Zb[s_zoff[cc] + (long)kj * nkc + il];
-        const double2 *s6 = (const double2 *)(B.sixjQ + s6d + (long)Jp * d2q + c0);
-        const double2 w01 = s6[0], w23 = s6[1];
+        const double *s6 = B.sixjQ + s6d + (long)Jp * s6J + (long)(c0 >> 1) * s6P;
+        const double2 w01 = *(const double2 *)s6, w23 = *(const double2 *)(s6 + s6P);
         acc[0] += (vm & 1) && fabs(w01.x) >= 1e-8 ? w01.x * z : 0.0;
         acc[1] += (vm & 2) && fabs(w01.y) >= 1e-8 ? w01.y * z : 0.0;
         acc[2] += (vm & 4) && fabs(w23.x) >= 1e-8 ? w23.x * z : 0.0;
@@ -1002,8 +1005,8 @@ __global__ __launch_bounds__(256, 6) void k_invpandya(PairArg pa, const int4 *it
         const int lj = Tj[Jp * norb + l];
         const double me = Zb[s_zoff[cc] + (long)((l > j ? nkc : 0) + ik) * nkc + lj];
         const double z = (2 * Jp + 1) * (symx * me) * sx0;  // the sign alternates with J below
-        const double2 *s6 = (const double2 *)(B.sixjQ + s6x + (long)Jp * d2q + c0);
-        const double2 w01 = s6[0], w23 = s6[1];
+        const double *s6 = B.sixjQ + s6x + (long)Jp * s6J + (long)(c0 >> 1) * s6P;
+        const double2 w01 = *(const double2 *)s6, w23 = *(const double2 *)(s6 + s6P);
         acc[0] += (vm & 1) && fabs(w01.x) >= 1e-8 ? w01.x * z : 0.0;
         acc[1] -= (vm & 2) && fabs(w01.y) >= 1e-8 ? w01.y * z : 0.0;
         acc[2] += (vm & 4) && fabs(w23.x) >= 1e-8 ? w23.x * z : 0.0;
