@@ -450,7 +450,7 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
   HIPCHK(ctx, launch_onebody(pa, items[5], cnt[5], s));
   HIPCHK(ctx, mark(2));
   if (mask & MPPHH)
-    HIPCHK(ctx, launch_pphh(pa, items[0], cnt[0], items[7], cnt[7], prec_pphh, s));
+    HIPCHK(ctx, launch_pphh(pa, items[0], cnt[0], items[7], cnt[7], prec_pphh, maxn, s));
   else
     HIPCHK(ctx, launch_zero_two(pa, items[6], cnt[6], s));
   HIPCHK(ctx, mark(3));

@@ -10,6 +10,7 @@
 // 1611-1612).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "internal.hpp"
@@ -497,9 +498,13 @@ __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
   constexpr int T = kTileB, LD = kTileB + 4;
   // operands stay FP64 in LDS (they arrive by direct global->LDS copies); the
   // mixed kernel rounds them to FP32 when it reads them
-  __shared__ __align__(16) double As[2][kTileK][LD];
-  __shared__ __align__(16) double Bs[2][kTileK][LD];
-  __shared__ double Ws[2][kTileK];
+  // the two stage buffers are separate LDS objects and every access names one
+  // statically (the stage loop is unrolled by two), so the compiler can tell
+  // the reads of one buffer from the direct copies into the other and does
+  // not wait for the copies in flight before computing
+  __shared__ __align__(16) double A0[kTileK][LD], A1[kTileK][LD];
+  __shared__ __align__(16) double B0[kTileK][LD], B1[kTileK][LD];
+  __shared__ double W0[kTileK], W1[kTileK];
   const int4 it = items[blockIdx.x];
   if (it.y < 0) return;  // padding item of the XCD-balanced order
   PairPtrs P = pa.get(it.x);
@@ -559,15 +564,23 @@ __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
   // at most one element past the column (inside the arena, or the buffer's
   // slack).  Column indices (wave-uniform) are loaded one stage ahead.
   const int ra = min(R0 + 2 * lane, n - 1), rb = min(S0 + 2 * lane, n - 1);
+  // MODE 0: the channel's column-index list kl[0..nk) is staged in LDS (dynamic
+  // shared memory of at least nk ints), so index reads never wait behind the
+  // global->LDS copies in flight
+  extern __shared__ int KLs[];
+  if constexpr (MODE == 0) {
+    for (int e = threadIdx.x; e < nk; e += blockDim.x) KLs[e] = kl[e];
+    __syncthreads();
+  }
   auto load_idx = [&](int k0, int (&ki)[4]) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int kk = min(k0 + wv * 4 + j, K2 - 1);
       const int k = kk < nk ? kk : kk - nk;
-      ki[j] = (MODE == 0) ? kl[k] : k;
+      ki[j] = (MODE == 0) ? KLs[k] : k;
     }
   };
-  auto issue_stage = [&](int k0, const int (&ki)[4], int buf) {
+  auto issue_stage = [&](int k0, const int (&ki)[4], double (&As)[kTileK][LD], double (&Bs)[kTileK][LD]) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int kq = wv * 4 + j;
@@ -577,9 +590,9 @@ __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
       const double *sa = kk < nk ? Am + (long)ca * n : Bm + (long)cb * n;
       const double *sb = kk < nk ? Bm + (long)cb * n : Am + (long)ca * n;
       __builtin_amdgcn_global_load_lds((const IMSRG_G void *)(sa + ra),
-                                       (__attribute__((address_space(3))) void *)&As[buf][kq][0], 16, 0, 0);
+                                       (__attribute__((address_space(3))) void *)&As[kq][0], 16, 0, 0);
       __builtin_amdgcn_global_load_lds((const IMSRG_G void *)(sb + rb),
-                                       (__attribute__((address_space(3))) void *)&Bs[buf][kq][0], 16, 0, 0);
+                                       (__attribute__((address_space(3))) void *)&Bs[kq][0], 16, 0, 0);
     }
   };
   // the weight of k' = k0 + t (t < 16) is loaded into a register and only
@@ -591,30 +604,31 @@ __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
     if constexpr (MODE == 1) return 1.0;
     return (double)P.hY * (double)phi[phl[k < nph ? k : k - nph]];
   };
-  auto store_w = [&](int k0, int buf, double w) {
+  auto store_w = [&](int k0, double (&Ws)[kTileK], double w) {
     if (threadIdx.x < kTileK) {
       const int kk = k0 + threadIdx.x;
-      Ws[buf][threadIdx.x] = kk >= K2 ? 0.0 : kk < nk ? w : w * h;
+      Ws[threadIdx.x] = kk >= K2 ? 0.0 : kk < nk ? w : w * h;
     }
   };
   int ki[4];
-  double w_next = 0.0;
   if (K2 > 0) {  // a channel without pp or hh kets only writes zeros
     load_idx(0, ki);
-    issue_stage(0, ki, 0);
-    store_w(0, 0, load_w(0));
-    if (kTileK < K2) load_idx(kTileK, ki);
+    issue_stage(0, ki, A0, B0);
+    store_w(0, W0, load_w(0));
+    load_idx(kTileK, ki);  // clamped: harmless past the end
   }
   __builtin_amdgcn_s_waitcnt(0);  // the copies into LDS have landed
   __syncthreads();
-  int cur = 0;
-  for (int k0 = 0; k0 < K2; k0 += kTileK) {
+  // one stage: copies of the next stage into (An, Bn, Wn), MFMAs on (Ac, Bc, Wc)
+  auto stage = [&](int k0, double (&Ac)[kTileK][LD], double (&Bc)[kTileK][LD], double (&Wc)[kTileK],
+                   double (&An)[kTileK][LD], double (&Bn)[kTileK][LD], double (&Wn)[kTileK]) {
     const bool more = k0 + kTileK < K2;
-    if (more) {
-      issue_stage(k0 + kTileK, ki, cur ^ 1);
-      w_next = load_w(k0 + kTileK);
-      if (k0 + 2 * kTileK < K2) load_idx(k0 + 2 * kTileK, ki);
-    }
+    // loads with register results are unconditional (clamped indices): a
+    // branch around them would make the compiler wait for them, and for the
+    // copies issued with them, where the paths join
+    const double w_next = load_w(k0 + kTileK);
+    if (more) issue_stage(k0 + kTileK, ki, An, Bn);
+    load_idx(k0 + 2 * kTileK, ki);
     if (!wave_live) {
     } else if constexpr (MIXED) {
       // one row block mi at a time, operands re-read from LDS, so that only four
@@ -629,11 +643,11 @@ __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
 #pragma unroll
           for (int ks = kp; ks < kp + kF32Flush; ++ks) {
             const int kr = ks * 4 + (lane >> 4);
-            const float am = (float)(As[cur][kr][wr + mi * 16 + (lane & 15)] * Ws[cur][kr]);
+            const float am = (float)(Ac[kr][wr + mi * 16 + (lane & 15)] * Wc[kr]);
 #pragma unroll
             for (int ni = 0; ni < 4; ++ni)
-              t[ni] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)Bs[cur][kr][wc + ni * 16 + (lane & 15)], am, t[ni],
-                                                           0, 0, 0);
+              t[ni] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)Bc[kr][wc + ni * 16 + (lane & 15)], am, t[ni], 0,
+                                                           0, 0);
           }
 #pragma unroll
           for (int ni = 0; ni < 4; ++ni)
@@ -645,12 +659,12 @@ __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
 #pragma unroll
       for (int ks = 0; ks < kTileK / 4; ++ks) {
         const int kr = ks * 4 + (lane >> 4);
-        const double wk = Ws[cur][kr];
+        const double wk = Wc[kr];
         double a[4], b[4];
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
-          a[m] = As[cur][kr][wr + m * 16 + (lane & 15)] * wk;
-          b[m] = Bs[cur][kr][wc + m * 16 + (lane & 15)];
+          a[m] = Ac[kr][wr + m * 16 + (lane & 15)] * wk;
+          b[m] = Bc[kr][wc + m * 16 + (lane & 15)];
         }
 #pragma unroll
         for (int mi = 0; mi < 4; ++mi)
@@ -659,10 +673,13 @@ __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
             acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[ni], a[mi], acc[mi][ni], 0, 0, 0);
       }
     }
-    if (more) store_w(k0 + kTileK, cur ^ 1, w_next);
+    if (more) store_w(k0 + kTileK, Wn, w_next);
     __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of the next stage have landed
     __syncthreads();
-    cur ^= 1;
+  };
+  for (int k0 = 0; k0 < K2; k0 += 2 * kTileK) {
+    stage(k0, A0, B0, W0, A1, B1, W1);
+    if (k0 + kTileK < K2) stage(k0 + kTileK, A1, B1, W1, A0, B0, W0);
   }
   if constexpr (MODE == 0) {
 #pragma unroll
@@ -676,41 +693,52 @@ __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
           if (row < n && col < n && row <= col) D[(long)col * n + row] = alpha * acc[mi][ni][q];
         }
   } else {
-    // full Zbar storage through an LDS transpose, one wave sub-tile per round
-    // (the operand buffer As, idle now, holds the 64x65 tile)
-    double(*Tt)[kTile + 1] = reinterpret_cast<double(*)[kTile + 1]>(&As[0][0][0]);
+    // full Zbar storage through an LDS transpose, one 32x64 half of a wave
+    // sub-tile per round (the operand buffer A0, idle now, holds the tile)
+    double(*Tt)[kTile + 1] = reinterpret_cast<double(*)[kTile + 1]>(&A0[0][0]);
+    static_assert(32 * (kTile + 1) <= kTileK * LD, "transpose tile exceeds the operand buffer");
     for (int w = 0; w < 4; ++w) {
       if (!live(w)) continue;  // block-uniform
       const int wr2 = (w & 1) * 64, wc2 = (w >> 1) * 64;
-      if (wv == w) {
+      for (int hh = 0; hh < 2; ++hh) {
+        if (wv == w) {
 #pragma unroll
-        for (int mi = 0; mi < 4; ++mi)
+          for (int mi = 0; mi < 4; ++mi) {
+            if ((mi >> 1) != hh) continue;
 #pragma unroll
-          for (int ni = 0; ni < 4; ++ni)
+            for (int ni = 0; ni < 4; ++ni)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-              const int rr = mi * 16 + (lane & 15), cc = ni * 16 + mfma_col<MIXED>(lane, q);
-              double v = alpha * acc[mi][ni][q];
-              if (MODE == 2) {
-                const int col = S0 + wc2 + cc;
-                v *= (col < n) ? (double)phi[col] : 0.0;
+              for (int q = 0; q < 4; ++q) {
+                const int rr = (mi & 1) * 16 + (lane & 15), cc = ni * 16 + mfma_col<MIXED>(lane, q);
+                double v = alpha * acc[mi][ni][q];
+                if (MODE == 2) {
+                  const int col = S0 + wc2 + cc;
+                  v *= (col < n) ? (double)phi[col] : 0.0;
+                }
+                Tt[rr][cc] = v;
               }
-              Tt[rr][cc] = v;
-            }
-      }
-      __syncthreads();
-      for (int e = threadIdx.x; e < kTile * kTile; e += blockDim.x) {
-        const int rr = e & (kTile - 1), cc = e / kTile;
-        const int row = R0 + wr2 + rr, col = S0 + wc2 + cc;
-        if (row < n && col < n && row <= col) D[(long)col * n + row] = Tt[rr][cc];
-        // mirrored element (S0+wc2+rr, R0+wr2+cc) from the upper element (R0+wr2+cc, S0+wc2+rr)
-        const int row2 = S0 + wc2 + rr, col2 = R0 + wr2 + cc;
-        if (row2 < n && col2 < n && row2 > col2) {
-          const double sym = (MODE == 1) ? h : (double)(phi[row2] * phi[col2]);
-          D[(long)col2 * n + row2] = sym * Tt[cc][rr];
+          }
         }
+        __syncthreads();
+        const int R1 = R0 + wr2 + 32 * hh, S1 = S0 + wc2;
+        // upper elements (R1 + rr, S1 + cc), consecutive threads on consecutive rows
+        for (int e = threadIdx.x; e < 32 * kTile; e += blockDim.x) {
+          const int rr = e & 31, cc = e >> 5;
+          const int row = R1 + rr, col = S1 + cc;
+          if (row < n && col < n && row <= col) D[(long)col * n + row] = Tt[rr][cc];
+        }
+        // mirrored elements (S1 + cc, R1 + rr) = sym * upper (R1 + rr, S1 + cc),
+        // consecutive threads on consecutive rows S1 + cc
+        for (int e = threadIdx.x; e < 32 * kTile; e += blockDim.x) {
+          const int cc = e & (kTile - 1), rr = e >> 6;
+          const int row2 = S1 + cc, col2 = R1 + rr;
+          if (row2 < n && col2 < n && row2 > col2) {
+            const double sym = (MODE == 1) ? h : (double)(phi[row2] * phi[col2]);
+            D[(long)col2 * n + row2] = sym * Tt[rr][cc];
+          }
+        }
+        __syncthreads();
       }
-      __syncthreads();
     }
   }
 }
@@ -1124,12 +1152,13 @@ hipError_t launch_onebody(const PairArg &pa, const int4 *items, int nitems, hipS
   return hipGetLastError();
 }
 hipError_t launch_pphh(const PairArg &pa, const int4 *items, int nitems, const int4 *items64, int nitems64,
-                       int precision, hipStream_t s) {
+                       int precision, int maxn, hipStream_t s) {
   if (nitems > 0) {
+    const size_t lds = sizeof(int) * (size_t)std::max(maxn, 1);  // column-index list of the largest channel
     if (precision == 1)
-      hipLaunchKernelGGL(k_pphh_big_mixed, dim3(nitems), dim3(256), 0, s, pa, items);
+      hipLaunchKernelGGL(k_pphh_big_mixed, dim3(nitems), dim3(256), lds, s, pa, items);
     else
-      hipLaunchKernelGGL(k_pphh_big_f64, dim3(nitems), dim3(256), 0, s, pa, items);
+      hipLaunchKernelGGL(k_pphh_big_f64, dim3(nitems), dim3(256), lds, s, pa, items);
   }
   if (nitems64 > 0) {
     if (precision == 1)

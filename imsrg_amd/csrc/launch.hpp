@@ -14,8 +14,9 @@ hipError_t launch_sixj_table(double *table, double *tableQ, int d1, int d2, int 
 hipError_t launch_zerobody(const PairArg &pa, int npairs, int maxnch, hipStream_t s);
 hipError_t launch_onebody(const PairArg &pa, const int4 *items, int nitems, hipStream_t s);
 // pp/hh SYR2K: items on 128x128 tiles, items64 on 64x64 tiles (disjoint channels)
+// maxn: the largest channel dimension of the pairs (sizes the kernel's LDS index list)
 hipError_t launch_pphh(const PairArg &pa, const int4 *items, int nitems, const int4 *items64, int nitems64,
-                       int precision, hipStream_t s);
+                       int precision, int maxn, hipStream_t s);
 hipError_t launch_c122(const PairArg &pa, const int4 *items, int nitems, int maxn, int npairs, long maxn122,
                        hipStream_t s);
 hipError_t launch_pandya(const PairArg &pa, const int4 *items, int nitems, hipStream_t s);
