@@ -261,6 +261,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--emax", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--autotune-cache", default=None,
+                    help="text cache of the autotuner: settings found there are applied without timing trials")
     ap.add_argument("--workload", default="commutator", choices=["commutator", "batched", "emax12", "flow"],
                     help="commutator: the headline metric line; batched: 48 independent pairs at emax 4-7; "
                          "emax12: mixed-precision error report at emax=12; flow: Magnus flow at emax=10")
@@ -310,9 +312,9 @@ def main():
         return args.steps / t, t, phase_sum / args.steps, Z.to_host()
 
     # launch settings chosen by the autotuner on this basis, per precision (set-up, untimed)
-    tuning = {"fp64": ms.autotune(ia.PREC_FP64)}
+    tuning = {"fp64": ms.autotune(ia.PREC_FP64, args.autotune_cache)}
     fp64_rate, t_fp64, phase64, Zref = timed(ia.PREC_FP64)
-    tuning["mixed"] = ms.autotune(ia.PREC_MIXED)
+    tuning["mixed"] = ms.autotune(ia.PREC_MIXED, args.autotune_cache)
     mixed_rate, t_mixed, phasemx, Zmx = timed(ia.PREC_MIXED)
 
     # host-buffer drop-in (operators in host memory, PCIe transfers included);

@@ -1090,35 +1090,54 @@ __global__ __launch_bounds__(256) void k_mirror(PairArg pa, const int4 *items) {
 // Upper triangle += transposed-slot contributions of k_invpandya, then
 // lower triangle <- hZ * upper triangle; anti-Hermitian diagonal <- 0.
 // Tile pair: U = rows R0.., cols S0.. (R0 <= S0) and its transpose position.
+// The upper tile stays in registers (read and written along rows); the
+// transposed tile goes through LDS (read and written along its rows).
 __global__ __launch_bounds__(256) void k_mirror_fold(PairArg pa, const int4 *items) {
-  __shared__ double tu[kTile][kTile + 1];  // tu[col][row] of U
-  __shared__ double tl[kTile][kTile + 1];  // tl[col][row] of the transposed tile
+  __shared__ double t[kTile][kTile + 1];  // t[rr][cc]: element (S0 + cc, R0 + rr)
   const int4 it = items[blockIdx.x];
   PairPtrs P = pa.get(it.x);
   const BasisDev &B = *P.basis;
   const int ch = it.y, n = B.ch_n[ch];
   const int R0 = it.z * kTile, S0 = it.w * kTile;
   double *Z2 = P.Z + B.two_off + B.ch_off[ch];
-  for (int e = threadIdx.x; e < kTile * kTile; e += blockDim.x) {
-    const int rr = e & (kTile - 1), cc = e / kTile;
-    if (R0 + rr < n && S0 + cc < n) tu[cc][rr] = Z2[(long)(S0 + cc) * n + R0 + rr];
-    if (S0 + rr < n && R0 + cc < n) tl[cc][rr] = Z2[(long)(R0 + cc) * n + S0 + rr];
+  constexpr int kPer = kTile * kTile / 256;
+  double u[kPer];
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {  // upper element (R0 + rr, S0 + cc), consecutive rr
+    const int e = threadIdx.x + 256 * i, rr = e & (kTile - 1), cc = e / kTile;
+    const int r = R0 + rr, c = S0 + cc;
+    u[i] = (r < n && c < n) ? Z2[(long)c * n + r] : 0.0;
+  }
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {  // transposed slot (row S0 + cc, col R0 + rr), consecutive cc
+    const int e = threadIdx.x + 256 * i, cc = e & (kTile - 1), rr = e / kTile;
+    const int r = R0 + rr, c = S0 + cc;
+    if (r < n && c < n) t[rr][cc] = Z2[(long)r * n + c];
   }
   __syncthreads();
   const double hZ = P.hZ;
-  // upper element (r, c) = (R0 + rr, S0 + cc), r < c: its contribution sits at
-  // (row c, col r) = tl[rr][cc]
-  for (int e = threadIdx.x; e < kTile * kTile; e += blockDim.x) {
-    const int rr = e & (kTile - 1), cc = e / kTile;
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    const int e = threadIdx.x + 256 * i, rr = e & (kTile - 1), cc = e / kTile;
     const int r = R0 + rr, c = S0 + cc;
-    if (r >= n || c >= n) continue;
-    if (r < c) {
-      const double u = tu[cc][rr] + tl[rr][cc];
-      Z2[(long)c * n + r] = u;
-      Z2[(long)r * n + c] = hZ * u;
-    } else if (r == c && P.hZ == -1) {
-      Z2[(long)c * n + r] = 0.0;
+    if (r < n && c < n && r <= c) {
+      const double v = (r < c) ? u[i] + t[rr][cc] : (P.hZ == -1 ? 0.0 : u[i]);
+      Z2[(long)c * n + r] = v;
+      u[i] = v;
     }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {  // hand the new upper values to the thread that writes their mirror
+    const int e = threadIdx.x + 256 * i, rr = e & (kTile - 1), cc = e / kTile;
+    t[rr][cc] = u[i];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {  // lower element (S0 + cc, R0 + rr) = hZ * upper, consecutive cc
+    const int e = threadIdx.x + 256 * i, cc = e & (kTile - 1), rr = e / kTile;
+    const int r = R0 + rr, c = S0 + cc;
+    if (r < n && c < n && r < c) Z2[(long)r * n + c] = hZ * t[rr][cc];
   }
 }
 
