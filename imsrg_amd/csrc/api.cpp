@@ -432,8 +432,9 @@ static long r8(long n) { return (n + 7) / 8 * 8; }  // 64-byte aligned segment l
 static int herm_rule(int hx, int hy) { return (hx == hy) ? -1 : 1; }  // src/Commutator.cc:292-297
 
 // Launch sequence of one commutator (or a batch when pa.tab is set).
-// items / cnt: [0] pp/hh 128-tiles, [1] ph tiles, [2] comm122 columns, [3] Pandya
-// chunks, [4] inverse-Pandya items, [5] one-body pairs, [6] mirror tiles,
+// items / cnt: [0] pp/hh 128-tiles, [1] ph tiles, [2] none (comm122 runs in the
+// finishing kernel, over [6]), [3] Pandya chunks, [4] inverse-Pandya items,
+// [5] one-body pairs, [6] tile pairs of the finishing kernel,
 // [7] pp/hh 64-tiles.  Under mixed precision the context's tuning picks the
 // FP32 or FP64 kernel of each GEMM family.
 static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const int4 *const *items, const int *cnt,
@@ -454,7 +455,7 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
   else
     HIPCHK(ctx, launch_zero_two(pa, items[6], cnt[6], s));
   HIPCHK(ctx, mark(3));
-  if (mask & M122) HIPCHK(ctx, launch_c122(pa, items[2], cnt[2], maxn, npairs, maxn122, s));
+  if (mask & M122) HIPCHK(ctx, launch_c122_coef(pa, npairs, maxn122, s));
   HIPCHK(ctx, mark(4));
   if (mask & MPH) HIPCHK(ctx, launch_pandya(pa, items[3], cnt[3], s));
   HIPCHK(ctx, mark(5));
@@ -462,7 +463,7 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
   HIPCHK(ctx, mark(6));
   if (mask & MPH) HIPCHK(ctx, launch_invpandya(pa, items[4], cnt[4], maxjmax, maxnorb, maxnch, s));
   HIPCHK(ctx, mark(7));
-  HIPCHK(ctx, launch_mirror(pa, items[6], cnt[6], (mask & MPH) != 0, s));
+  HIPCHK(ctx, launch_finish(pa, items[6], cnt[6], (mask & MPH) != 0, (mask & M122) != 0, s));
   HIPCHK(ctx, mark(8));
   if (prof) {
     HIPCHK(ctx, hipEventSynchronize(ctx->ev[8]));
@@ -499,8 +500,8 @@ static int commutator_dev(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y
   pa.p0.hZ = herm_rule(X->herm, Y->herm);
   pa.p0.mask = mask;
   Z->herm = pa.p0.hZ;
-  const int4 *items[8] = {db.d_pphh, db.d_ph, db.d_c122, db.d_pandya, db.d_invp, db.d_one, db.d_mirror, db.d_pphh64};
-  int cnt[8] = {(int)db.wl.pphh.size(), (int)db.wl.ph.size(), (int)db.wl.c122.size(), (int)db.wl.pandya.size(),
+  const int4 *items[8] = {db.d_pphh, db.d_ph, nullptr, db.d_pandya, db.d_invp, db.d_one, db.d_mirror, db.d_pphh64};
+  int cnt[8] = {(int)db.wl.pphh.size(), (int)db.wl.ph.size(), 0, (int)db.wl.pandya.size(),
                 (int)db.wl.invp.size(), (int)db.wl.one.size(), (int)db.wl.mirror.size(), (int)db.wl.pphh64.size()};
   if (!ctx->use_graphs || ctx->profiling)
     return run_commutator(ctx, pa, 1, items, cnt, mask, prec, db.h.maxn, db.h.nch, db.h.n122, db.h.jmax,
@@ -862,7 +863,8 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
     std::vector<TileGroup> groups;
     for (int i = 0; i < npairs; ++i) {
       const WorkLists &wl = ctxs[i]->db.wl;
-      const std::vector<int4> *src[8] = {&wl.pphh, &wl.ph, &wl.c122, &wl.pandya, &wl.invp, &wl.one, &wl.mirror,
+      const std::vector<int4> none;
+      const std::vector<int4> *src[8] = {&wl.pphh, &wl.ph, &none, &wl.pandya, &wl.invp, &wl.one, &wl.mirror,
                                          &wl.pphh64};
       {  // ph tiles in the tile size of the first context, whose kernels run the batch
         const int tile = c0->db.tune.ph_big ? kTileB : kTile;
@@ -935,6 +937,7 @@ static void tuning_out(const Tuning &t, double us, imsrg_tuning *o) {
   o->mixed_pphh_fp32 = t.mixed_pphh;
   o->mixed_ph_fp32 = t.mixed_ph;
   o->ph_tile_128 = t.ph_big;
+  o->finish_order = t.finish_order;
   o->us_per_commutator = us;
 }
 
@@ -960,7 +963,7 @@ extern "C" int imsrg_hip_set_tuning(imsrg_ctx *ctx, const imsrg_tuning *t) {
   if (!ctx || !t) return IMSRG_E_INVALID;
   if (t->pphh_small_tile_max < 0 || t->pphh_order < 0 || t->pphh_order > 2 ||
       (t->mixed_pphh_fp32 != 0 && t->mixed_pphh_fp32 != 1) || (t->mixed_ph_fp32 != 0 && t->mixed_ph_fp32 != 1) ||
-      (t->ph_tile_128 != 0 && t->ph_tile_128 != 1))
+      (t->ph_tile_128 != 0 && t->ph_tile_128 != 1) || (t->finish_order != 0 && t->finish_order != 1))
     return fail(ctx, IMSRG_E_INVALID, "invalid tuning values");
   Tuning n;
   n.pphh_small = t->pphh_small_tile_max;
@@ -968,6 +971,7 @@ extern "C" int imsrg_hip_set_tuning(imsrg_ctx *ctx, const imsrg_tuning *t) {
   n.mixed_pphh = t->mixed_pphh_fp32;
   n.mixed_ph = t->mixed_ph_fp32;
   n.ph_big = t->ph_tile_128;
+  n.finish_order = t->finish_order;
   return apply_tuning(ctx, n);
 }
 
@@ -1007,8 +1011,10 @@ extern "C" int imsrg_hip_autotune(imsrg_ctx *ctx, int precision, const char *cac
       bool found = false;
       while (std::fgets(line, sizeof(line), f)) {
         if (std::strncmp(line, key.c_str(), key.size()) != 0 || line[key.size()] != ' ') continue;
-        if (std::sscanf(line + key.size(), " pphh_small=%d pphh_order=%d mixed_pphh=%d mixed_ph=%d ph_big=%d us=%lf",
-                        &t.pphh_small, &t.pphh_order, &t.mixed_pphh, &t.mixed_ph, &t.ph_big, &us) == 6)
+        if (std::sscanf(line + key.size(),
+                        " pphh_small=%d pphh_order=%d mixed_pphh=%d mixed_ph=%d ph_big=%d finish_order=%d us=%lf",
+                        &t.pphh_small, &t.pphh_order, &t.mixed_pphh, &t.mixed_ph, &t.ph_big, &t.finish_order,
+                        &us) == 7)
           found = true;  // the last matching line wins
       }
       std::fclose(f);
@@ -1060,23 +1066,30 @@ extern "C" int imsrg_hip_autotune(imsrg_ctx *ctx, int precision, const char *cac
     rc = fail(ctx, IMSRG_E_HIP, "hipEventCreate failed");
   Tuning best = ctx->db.tune;
   double best_us = 1e300;
-  for (size_t c = 0; !rc && c < cand.size(); ++c) {
-    if ((rc = apply_tuning(ctx, cand[c]))) break;
-    if ((rc = commutator_dev(ctx, X, Y, Z, IMSRG_ALL_TERMS, precision))) break;  // warm-up
-    if (hipEventRecord(e0, ctx->stream) != hipSuccess) { rc = fail(ctx, IMSRG_E_HIP, "hipEventRecord"); break; }
-    for (int r = 0; r < 3 && !rc; ++r) rc = commutator_dev(ctx, X, Y, Z, IMSRG_ALL_TERMS, precision);
-    if (rc) break;
+  auto trial = [&](const Tuning &t) -> int {  // time one setting; keep it if it is the fastest so far
+    int r = apply_tuning(ctx, t);
+    if (!r) r = commutator_dev(ctx, X, Y, Z, IMSRG_ALL_TERMS, precision);  // warm-up
+    if (r) return r;
+    if (hipEventRecord(e0, ctx->stream) != hipSuccess) return fail(ctx, IMSRG_E_HIP, "hipEventRecord");
+    for (int i = 0; i < 3 && !r; ++i) r = commutator_dev(ctx, X, Y, Z, IMSRG_ALL_TERMS, precision);
+    if (r) return r;
     float ms = 0.f;
     if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-        hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-      rc = fail(ctx, IMSRG_E_HIP, "timing failed");
-      break;
-    }
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+      return fail(ctx, IMSRG_E_HIP, "timing failed");
     const double us = 1000.0 * ms / 3.0;
     if (us < best_us) {
       best_us = us;
-      best = cand[c];
+      best = t;
     }
+    return IMSRG_OK;
+  };
+  for (size_t c = 0; !rc && c < cand.size(); ++c) rc = trial(cand[c]);
+  // the finishing kernel's tile order is independent of the GEMM choices: one more trial
+  if (!rc) {
+    Tuning t = best;
+    t.finish_order = 1 - best.finish_order;
+    rc = trial(t);
   }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
@@ -1088,8 +1101,9 @@ extern "C" int imsrg_hip_autotune(imsrg_ctx *ctx, int precision, const char *cac
   if ((rc = apply_tuning(ctx, best))) return rc;
   if (cache_path) {
     if (FILE *f = std::fopen(cache_path, "a")) {
-      std::fprintf(f, "%s pphh_small=%d pphh_order=%d mixed_pphh=%d mixed_ph=%d ph_big=%d us=%.3f\n", key.c_str(),
-                   best.pphh_small, best.pphh_order, best.mixed_pphh, best.mixed_ph, best.ph_big, best_us);
+      std::fprintf(f, "%s pphh_small=%d pphh_order=%d mixed_pphh=%d mixed_ph=%d ph_big=%d finish_order=%d us=%.3f\n",
+                   key.c_str(), best.pphh_small, best.pphh_order, best.mixed_pphh, best.mixed_ph, best.ph_big,
+                   best.finish_order, best_us);
       std::fclose(f);
     }
   }

@@ -208,7 +208,7 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
   std::vector<int> hh_start(nch + 1), hh_loc, k_start(nch + 1), k_loc, kpp_start(nch + 1), kpp_loc,
       khh_start(nch + 1), khh_loc, gcore_start(nch + 1), gcore_loc, gnc_start(nch + 1), gnc_loc;
   std::vector<double> hh_occ, k_w, kpp_w, khh_w;
-  // comm122 lists per ket, converted to ELL after the channel loop
+  // comm122 lists per ket (pushed in ket order), concatenated after the channel loop
   std::vector<std::vector<int4>> p122_e(nch);   // (ind2, a, o, ket)
   std::vector<std::vector<double>> p122_f(nch);
   int maxn = 0;
@@ -242,7 +242,6 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
       if (hb.cvq[p] == 0 && hb.cvq[q] == 0) gcore_loc.push_back(i);
       if (hb.cvq[p] != 0 && hb.cvq[q] != 0) gnc_loc.push_back(i);
       // comm122ss partner list, src/Commutator.cc:857-896
-      const size_t e_begin = p122_e[c].size();
       int ii = p, jj = q;
       int flip = -sgn_parity((hb.j2[ii] + hb.j2[jj]) / 2 - C.J);
       for (int a : hb.ob[ii]) {
@@ -250,7 +249,7 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
         int loc = hb.kmap[(size_t)ket_index(norb, lo, hi) * (hb.jmax + 1) + C.J];
         if (loc < 0) continue;
         double f = (a > jj) ? (double)flip : (a == jj ? sqrt2 : 1.0);
-        if (ii == jj) f /= sqrt2;
+        if (ii == jj) f = f / sqrt2 * 2.0;  // 1/sqrt2 (src/Commutator.cc:898-902) and the column's 2 (925-926)
         p122_e[c].push_back(make_int4(loc, a, ii, i));
         p122_f[c].push_back(f);
       }
@@ -263,51 +262,42 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
           p122_e[c].push_back(make_int4(loc, a, jj, i));
           p122_f[c].push_back(f);
         }
-      if (p122_e[c].size() - e_begin > (size_t)kMax122) return "comm122 partner list too long";
     }
   }
   ch_kstart[nch] = (int)ch_kets.size();
   hh_start[nch] = (int)hh_loc.size(); k_start[nch] = (int)k_loc.size();
   kpp_start[nch] = (int)kpp_loc.size(); khh_start[nch] = (int)khh_loc.size();
   gcore_start[nch] = (int)gcore_loc.size(); gnc_start[nch] = (int)gnc_loc.size();
-  // ELL layout of the comm122 lists
-  std::vector<long> ell_off(nch + 1, 0);
-  std::vector<int> ell_mx(nch, 0), ell_cnt(ch_kets.size(), 0), ell_ind;
-  std::vector<int2> ell_ao;
-  std::vector<double> ell_f;
-  {
-    long tot = 0;
-    for (int c = 0; c < nch; ++c) {
-      const int nk = ch_n[c];
-      std::vector<int> cnt(nk, 0);
-      for (auto &e : p122_e[c]) cnt[e.w]++;
-      int mx = 0;
-      for (int r = 0; r < nk; ++r) {
-        mx = std::max(mx, cnt[r]);
-        ell_cnt[ch_kstart[c] + r] = cnt[r];
+  // comm122 lists in CSR form over the kets of all channels; each ket's list is
+  // padded to a multiple of 4 entries with its first partner and factor 0
+  std::vector<int> c122_start(ch_kets.size() + 1, 0), c122_ind;
+  std::vector<int2> c122_ao;
+  std::vector<double> c122_f;
+  for (int c = 0; c < nch; ++c) {
+    const size_t ne = p122_e[c].size();
+    for (size_t t = 0; t < ne;) {
+      const int ket = p122_e[c][t].w;
+      size_t u = t;
+      for (; u < ne && p122_e[c][u].w == ket; ++u) {
+        const int4 e = p122_e[c][u];
+        c122_ind.push_back(e.x);
+        c122_ao.push_back(make_int2(e.y, e.z));
+        c122_f.push_back(p122_f[c][u]);
       }
-      ell_mx[c] = mx;
-      ell_off[c] = tot;
-      tot += (long)mx * nk;
-    }
-    ell_off[nch] = tot;
-    ell_ind.assign(tot, 0);
-    ell_ao.assign(tot, make_int2(-1, -1));
-    ell_f.assign(tot, 0.0);
-    for (int c = 0; c < nch; ++c) {
-      const int nk = ch_n[c];
-      std::vector<int> fill(nk, 0);
-      for (size_t t = 0; t < p122_e[c].size(); ++t) {
-        const int4 e = p122_e[c][t];
-        const long pos = ell_off[c] + (long)fill[e.w]++ * nk + e.w;
-        ell_ind[pos] = e.x;
-        ell_ao[pos] = make_int2(e.y, e.z);
-        ell_f[pos] = p122_f[c][t];
+      for (size_t v = u - t; v % 4 != 0; ++v) {
+        c122_ind.push_back(p122_e[c][t].x);
+        c122_ao.push_back(make_int2(p122_e[c][t].y, p122_e[c][t].z));
+        c122_f.push_back(0.0);
       }
+      c122_start[ch_kstart[c] + ket + 1] = (int)c122_ind.size();
+      t = u;
     }
-    B.n122 = tot;
   }
+  for (size_t i = 0; i + 1 < c122_start.size(); ++i)  // kets without partners
+    c122_start[i + 1] = std::max(c122_start[i + 1], c122_start[i]);
+  B.n122 = (long)c122_ind.size();
   B.maxn = maxn;
+  if ((long)maxn * maxn >= (1L << 29)) return "two-body channel too large (kernels address a channel block with 32-bit byte offsets)";
   {
     int mb = 0;
     for (int i = 0; i < norb; ++i) mb = std::max(mb, (int)hb.ob[i].size());
@@ -323,9 +313,8 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
   B.kpp_w = dupload(*this, kpp_w, err);
   B.khh_start = dupload(*this, khh_start, err); B.khh_loc = dupload(*this, khh_loc, err);
   B.khh_w = dupload(*this, khh_w, err);
-  B.ell_off = dupload(*this, ell_off, err); B.ell_mx = dupload(*this, ell_mx, err);
-  B.ell_cnt = dupload(*this, ell_cnt, err); B.ell_ind = dupload(*this, ell_ind, err);
-  B.ell_ao = dupload(*this, ell_ao, err); B.ell_f = dupload(*this, ell_f, err);
+  B.c122_start = dupload(*this, c122_start, err); B.c122_ind = dupload(*this, c122_ind, err);
+  B.c122_ao = dupload(*this, c122_ao, err); B.c122_f = dupload(*this, c122_f, err);
   B.gcore_start = dupload(*this, gcore_start, err); B.gcore_loc = dupload(*this, gcore_loc, err);
   B.gnc_start = dupload(*this, gnc_start, err); B.gnc_loc = dupload(*this, gnc_loc, err);
 
@@ -409,6 +398,8 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
 
   // ---- work lists ----
   this->ch_n = ch_n;
+  this->ch_kstart = ch_kstart;
+  this->c122_start = c122_start;
   this->k_start = k_start;
   this->cc_n = cc_n;
   this->cc_nph = cc_nph;
@@ -486,8 +477,41 @@ std::string DeviceBasis::build_lists(const Tuning &t) {
     const int nt = (nk + kTile - 1) / kTile;
     for (int rb = 0; rb < nt; ++rb)
       for (int sb = rb; sb < nt; ++sb) wl.mirror.push_back(make_int4(0, c, rb, sb));
-    for (int s = 0; s < nk; ++s) wl.c122.push_back(make_int4(0, c, s, 0));
     if (gen_ok[c]) wl.gen.push_back(make_int4(0, c, 0, 0));
+  }
+  if (t.finish_order == 1) {
+    // XCD-contiguous order: workgroup i runs on XCD i % 8, so the list is cut
+    // into 8 runs of consecutive tiles (whole block rows, mostly) of about equal
+    // cost, padded with empty items to equal length, and the runs are
+    // interleaved; each XCD then sweeps its own block rows, whose partner-column
+    // segments (comm122) stay in that XCD's L2.  Cost of a tile pair: partner
+    // entries of its row- and column-block kets plus its columns.
+    const std::vector<int4> &L = wl.mirror;
+    std::vector<double> acc(L.size() + 1, 0.0);
+    for (size_t i = 0; i < L.size(); ++i) {
+      const int c = L[i].y, n = ch_n[c], ks = ch_kstart[c];
+      double w = 0.0;
+      for (int blk : {L[i].z, L[i].w}) {
+        const int lo = blk * kTile, hi = std::min(n, lo + kTile);
+        w += (c122_start[ks + hi] - c122_start[ks + lo]) + 0.5 * (hi - lo);
+      }
+      acc[i + 1] = acc[i] + w;
+    }
+    size_t bound[9];
+    bound[0] = 0;
+    bound[8] = L.size();
+    for (int x = 1; x < 8; ++x) {
+      const double target = acc[L.size()] * x / 8;
+      size_t j = bound[x - 1];
+      while (j < L.size() && acc[j] < target) ++j;
+      bound[x] = j;
+    }
+    size_t len = 0;
+    for (int x = 0; x < 8; ++x) len = std::max(len, bound[x + 1] - bound[x]);
+    std::vector<int4> v(8 * len, make_int4(0, -1, 0, 0));
+    for (int x = 0; x < 8; ++x)
+      for (size_t j = bound[x]; j < bound[x + 1]; ++j) v[8 * (j - bound[x]) + x] = L[j];
+    wl.mirror.swap(v);
   }
   // inverse Pandya by orbit quadruple: one item (pair, group g, bra b, 0) per
   // bra ket, whose work-group loops over the group kets e >= b; longest first
@@ -530,7 +554,7 @@ std::string DeviceBasis::build_lists(const Tuning &t) {
     return (int4 *)p;
   };
   d_pphh = up(wl.pphh); d_pphh64 = up(wl.pphh64); d_ph = up(wl.ph);
-  d_c122 = up(wl.c122); d_pandya = up(wl.pandya); d_invp = up(wl.invp);
+  d_pandya = up(wl.pandya); d_invp = up(wl.invp);
   d_one = up(wl.one); d_mirror = up(wl.mirror); d_gen = up(wl.gen);
   if (err.empty()) tune = t;
   return err;

@@ -52,6 +52,7 @@ struct Tuning {
   int mixed_pphh = 1;  // mixed precision: 1 = FP32-MFMA pp/hh kernel, 0 = the FP64 kernel
   int mixed_ph = 1;    // mixed precision: 1 = FP32-MFMA ph kernel, 0 = the FP64 kernel
   int ph_big = 1;      // ph GEMM on 128x128 tiles (1) or 64x64 tiles (0)
+  int finish_order = 1;  // tile pairs of the finishing kernel: 0 channel/row order, 1 XCD-contiguous runs
 };
 
 // Work-item lists (int4: pair, a, b, c) for every kernel of one commutator.
@@ -59,11 +60,10 @@ struct WorkLists {
   std::vector<int4> pphh;    // (pair, ch, rb, sb), rb <= sb, 128x128 tiles
   std::vector<int4> pphh64;  // (pair, ch, rb, sb), rb <= sb, 64x64 tiles (channels n <= pphh_small)
   std::vector<int4> ph;      // (pair, cc, rb, sb), rb <= sb, tiles of 128 (ph_big) or 64
-  std::vector<int4> c122;    // (pair, ch, col, 0)
   std::vector<int4> pandya;  // (pair, cc, start, 0)
   std::vector<int4> invp;    // (pair, group, bra, first ket) of the (parity, Tz) ket groups
   std::vector<int4> one;     // (pair, i, j, 0)
-  std::vector<int4> mirror;  // (pair, ch, rb, sb), rb < sb  and diagonal tiles
+  std::vector<int4> mirror;  // (pair, ch, rb, sb), rb < sb  and diagonal tiles; ch = -1: empty item
   std::vector<int4> gen;     // (pair, ch, 0, 0) channels with work for the generator
 };
 
@@ -75,10 +75,11 @@ struct DeviceBasis {
   std::vector<void *> list_allocs;  // device work lists (rebuilt when the tuning changes)
   WorkLists wl;             // host lists with pair = 0
   Tuning tune;
-  int4 *d_pphh = nullptr, *d_pphh64 = nullptr, *d_ph = nullptr, *d_c122 = nullptr, *d_pandya = nullptr,
+  int4 *d_pphh = nullptr, *d_pphh64 = nullptr, *d_ph = nullptr, *d_pandya = nullptr,
        *d_invp = nullptr, *d_one = nullptr, *d_mirror = nullptr, *d_gen = nullptr;
   // host tables the work lists are built from
   std::vector<int> ch_n, k_start, cc_n, cc_nph, grp_start, grp_kets, gen_ok;
+  std::vector<int> ch_kstart, c122_start;
   std::vector<int2> pairs1;
   std::string upload(const HostBasis &hb, int emax6j);  // builds tables, lists, 6j table
   std::string build_lists(const Tuning &t);             // (re)builds and uploads the work lists

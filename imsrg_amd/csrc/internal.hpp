@@ -22,7 +22,6 @@ constexpr int kTile = 64;         // SYR2K output tile edge
 constexpr int kTileB = 128;       // output tile edge of the large pp/hh SYR2K
 constexpr int kTileK = 16;        // k-depth of one LDS stage
 constexpr int kChunk = 256;       // elements per work item of the gather kernels
-constexpr int kMax122 = 32;       // max one-body partners of a ket in comm122
 constexpr int kJC = 4;            // J values per register chunk of the inverse Pandya gather
 constexpr int kF32Flush = 4;      // MFMA k-steps (of 4) summed in FP32 before the FP64 flush (mixed precision)
 
@@ -45,7 +44,7 @@ struct BasisDev {
   long two_off;  // offset of the two-body arena inside an operator buffer
   long opsize;   // doubles per operator buffer (including 8 doubles of zero slack at the end)
   int maxn, maxcc, maxblk;
-  long n122;           // comm122 ELL entries (including padding)
+  long n122;           // comm122 partner entries (including the padding to multiples of 4 per ket)
   const IMSRG_G int *j2, *l, *tz2, *cvq;
   const IMSRG_G double *occ;
   const IMSRG_G int *holes;
@@ -69,14 +68,13 @@ struct BasisDev {
   const IMSRG_G double *kpp_w;
   const IMSRG_G int *khh_start, *khh_loc;  // M_hh list (weights n_p n_q)
   const IMSRG_G double *khh_w;
-  // comm122 partner lists in ELL layout: entry m of ket r of channel ch at
-  // ell_off[ch] + m * n_ch + r, m < ell_mx[ch]; padding has a = -1
-  const IMSRG_G long *ell_off;
-  const IMSRG_G int *ell_mx;
-  const IMSRG_G int *ell_cnt;     // entries of ket (ch_kstart[ch] + r)
-  const IMSRG_G int *ell_ind;     // local ket index ind2 of the partner
-  const IMSRG_G int2 *ell_ao;     // (a, o): coefficient X1(a, o) * f
-  const IMSRG_G double *ell_f;
+  // comm122 partner lists (CSR): the partners of local ket r of channel ch are
+  // the entries c122_start[ch_kstart[ch] + r] .. c122_start[ch_kstart[ch] + r + 1],
+  // a multiple of 4 (padding: the first partner again with factor 0)
+  const IMSRG_G int *c122_start;
+  const IMSRG_G int *c122_ind;     // local ket index ind2 of the partner
+  const IMSRG_G int2 *c122_ao;     // (a, o): coefficient X1(a, o) * f
+  const IMSRG_G double *c122_f;    // phase / sqrt2 factors, times 2 for kets with p == q
   // cross-coupled channels
   const IMSRG_G int *cc_n, *cc_nph, *cc_kstart, *cc_kets, *cc_phstart, *cc_ph;
   const IMSRG_G signed char *cc_phi;  // (-1)^{(j_p+j_q)} per CC ket

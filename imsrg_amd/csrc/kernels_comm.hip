@@ -143,10 +143,9 @@ __global__ __launch_bounds__(256) void k_c122_coef(PairArg pa, int npairs) {
   const int norb = B.norb;
   const double *X1 = P.X + B.one_off, *Y1 = P.Y + B.one_off;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < B.n122; e += (long)gridDim.x * blockDim.x) {
-    const int2 ao = B.ell_ao[e];
-    const double f = B.ell_f[e];
-    P.c122[e] = ao.x < 0 ? make_double2(0.0, 0.0)
-                         : make_double2(X1at(X1, norb, ao.x, ao.y) * f, X1at(Y1, norb, ao.x, ao.y) * f);
+    const int2 ao = B.c122_ao[e];
+    const double f = B.c122_f[e];
+    P.c122[e] = make_double2(X1at(X1, norb, ao.x, ao.y) * f, X1at(Y1, norb, ao.x, ao.y) * f);
   }
 }
 
@@ -755,64 +754,6 @@ __global__ __launch_bounds__(256, 2) void k_ph_big(PairArg pa, const int4 *items
 }
 
 // ---------------------------------------------------------------------------
-// comm122ss (src/Commutator.cc:837-932): column s of one channel.
-//   Z(r,s) -= W(r,s) + hZ W(s,r),  r <= s,
-//   W(:,q) = sum_m [Y2(:,ind_m) x_m - X2(:,ind_m) y_m]  (x2 if ket q has p == q),
-// with W(s,r) formed from column s of X2 / Y2 (staged in LDS) via Y2(s,i) = hY Y2(i,s).
-// The coefficients x_m, y_m of this call come from k_c122_coef.
-__global__ __launch_bounds__(256) void k_c122(PairArg pa, const int4 *items) {
-  extern __shared__ double lds[];
-  __shared__ int sind[kMax122];
-  __shared__ double scx[kMax122], scy[kMax122];
-  const int4 it = items[blockIdx.x];
-  PairPtrs P = pa.get(it.x);
-  const BasisDev &B = *P.basis;
-  const int ch = it.y, s = it.z;
-  const int n = B.ch_n[ch];
-  const long off = B.ch_off[ch];
-  const double *X2 = P.X + B.two_off + off, *Y2 = P.Y + B.two_off + off;
-  double *Z2 = P.Z + B.two_off + off;
-  double *colX = lds, *colY = lds + n;
-  for (int q = threadIdx.x; q < n; q += blockDim.x) {
-    colX[q] = X2[(long)s * n + q];
-    colY[q] = Y2[(long)s * n + q];
-  }
-  const int ks = B.ch_kstart[ch];
-  const long eo = B.ell_off[ch];
-  const int mx = B.ell_mx[ch];
-  const int ne = B.ell_cnt[ks + s];
-  if (threadIdx.x < ne) {
-    const long pos = eo + (long)threadIdx.x * n + s;
-    const double2 c = P.c122[pos];
-    sind[threadIdx.x] = B.ell_ind[pos];
-    scx[threadIdx.x] = c.x;
-    scy[threadIdx.x] = c.y;
-  }
-  __syncthreads();
-  const int kets = B.ch_kets[ks + s];
-  const double ds = (B.kp[kets] == B.kq[kets]) ? 2.0 : 1.0;
-  const double hX = P.hX, hY = P.hY, hZ = P.hZ;
-  for (int r = threadIdx.x; r <= s; r += blockDim.x) {
-    double wrs = 0.0;
-    for (int m = 0; m < ne; ++m) {
-      const long c = (long)sind[m] * n + r;
-      wrs += Y2[c] * scx[m] - X2[c] * scy[m];
-    }
-    wrs *= ds;
-    double wsr = 0.0;
-    for (int m = 0; m < mx; ++m) {  // padded entries carry zero coefficients
-      const long pos = eo + (long)m * n + r;
-      const int ind = B.ell_ind[pos];
-      const double2 c = P.c122[pos];
-      wsr += hY * colY[ind] * c.x - hX * colX[ind] * c.y;
-    }
-    const int ketr = B.ch_kets[ks + r];
-    if (B.kp[ketr] == B.kq[ketr]) wsr *= 2.0;
-    Z2[(long)s * n + r] -= wrs + hZ * wsr;
-  }
-}
-
-// ---------------------------------------------------------------------------
 // DoPandyaTransformation_SingleChannel_XandY (src/Commutator.cc:1392-1485):
 // Xbar (nk x 2nph) and Ybar^T (nk x 2nph), one element each per thread.
 __global__ __launch_bounds__(256) void k_pandya(PairArg pa, const int4 *items) {
@@ -905,7 +846,7 @@ __device__ __forceinline__ int wave_max(int v) {
 // standard-channel data are staged in LDS once per work-group.  Every upper
 // element (ij <= kl, or ij < kl for anti-Hermitian Z) of every channel is
 // produced exactly once and stored at its transposed slot (consecutive lanes,
-// consecutive addresses); k_mirror_fold adds it to the upper triangle, the
+// consecutive addresses); k_finish adds it to the upper triangle, the
 // diagonal is added in place.
 __host__ __device__ inline size_t invp_lds_bytes(int jmax, int norb, int nch) {
   const size_t nJ = (size_t)jmax + 1;
@@ -1064,6 +1005,7 @@ __global__ __launch_bounds__(256, 6) void k_invpandya(PairArg pa, const int4 *it
 __global__ __launch_bounds__(256) void k_mirror(PairArg pa, const int4 *items) {
   __shared__ double t[kTile][kTile + 1];
   const int4 it = items[blockIdx.x];
+  if (it.y < 0) return;  // empty item (padding of an XCD run)
   PairPtrs P = pa.get(it.x);
   const BasisDev &B = *P.basis;
   const int ch = it.y, n = B.ch_n[ch];
@@ -1087,63 +1029,113 @@ __global__ __launch_bounds__(256) void k_mirror(PairArg pa, const int4 *items) {
   }
 }
 
-// Upper triangle += transposed-slot contributions of k_invpandya, then
-// lower triangle <- hZ * upper triangle; anti-Hermitian diagonal <- 0.
-// Tile pair: U = rows R0.., cols S0.. (R0 <= S0) and its transpose position.
-// The upper tile stays in registers (read and written along rows); the
-// transposed tile goes through LDS (read and written along its rows).
-__global__ __launch_bounds__(256) void k_mirror_fold(PairArg pa, const int4 *items) {
-  __shared__ double t[kTile][kTile + 1];  // t[rr][cc]: element (S0 + cc, R0 + rr)
+// Element idx of a channel block through a 32-bit byte offset (global loads and
+// stores then take the block base in SGPRs); a block holds < 2^29 doubles.
+template <class T>
+__device__ __forceinline__ T &blk_at(T *base, unsigned idx) {
+  return *(T *)((char *)base + (idx << 3));
+}
+template <class T>
+__device__ __forceinline__ const T &blk_at(const T *base, unsigned idx) {
+  return *(const T *)((const char *)base + (idx << 3));
+}
+
+// W(row, q) of comm122ss (src/Commutator.cc:908-926) for one column q of a
+// channel: sum over the partner entries e of ket q of Y2(row, ind_e) x_e -
+// X2(row, ind_e) y_e.  q (hence e0, e1) is uniform over the wave; row is the
+// lane's.  Every column's list holds a multiple of 4 entries (padded with its
+// first partner and a zero factor), read in blocks of 4 whose loads are all
+// issued before the first use.
+__device__ __forceinline__ double c122_w(const double *X2, const double *Y2, const IMSRG_G double2 *coef,
+                                         const IMSRG_G int *ind, int e0, int e1, int n, int row) {
+  constexpr int kB = 4;
+  double w = 0.0;
+  for (int eb = e0; eb < e1; eb += kB) {
+    double y[kB], x[kB];
+    double2 cf[kB];
+#pragma unroll
+    for (int m = 0; m < kB; ++m) {
+      const unsigned o = (unsigned)(ind[eb + m] * n + row);
+      y[m] = blk_at(Y2, o);
+      x[m] = blk_at(X2, o);
+      cf[m] = coef[eb + m];
+    }
+#pragma unroll
+    for (int m = 0; m < kB; ++m) w += y[m] * cf[m].x - x[m] * cf[m].y;
+  }
+  return w;
+}
+
+// Last kernel of a commutator, over the tile pairs of every two-body block:
+//   FOLD: upper triangle += the transposed-slot contributions of k_invpandya;
+//   C122: comm122ss (src/Commutator.cc:837-932), Z2 -= W + hZ W^T, with W formed
+//         here from full columns of X2 / Y2 (the partner coefficients of this
+//         call from k_c122_coef);
+// then lower triangle <- hZ * upper triangle, anti-Hermitian diagonal <- 0.
+// Tile pair: U = rows R0.., cols S0.. (R0 <= S0) and its transposed position.
+// U stays in registers: one column per wave, lane = row.  The contributions
+// held at the transposed position go through LDS: one column R0 + rr of Z2
+// per wave, lane = row S0 + cc.
+template <bool FOLD, bool C122>
+__global__ __launch_bounds__(256, 4) void k_finish(PairArg pa, const int4 *items) {
+  __shared__ double t[kTile][kTile + 1];  // t[rr][cc]: contribution to upper (R0 + rr, S0 + cc)
   const int4 it = items[blockIdx.x];
+  if (it.y < 0) return;  // empty item (padding of an XCD run)
   PairPtrs P = pa.get(it.x);
   const BasisDev &B = *P.basis;
   const int ch = it.y, n = B.ch_n[ch];
   const int R0 = it.z * kTile, S0 = it.w * kTile;
-  double *Z2 = P.Z + B.two_off + B.ch_off[ch];
+  const long off = B.ch_off[ch];
+  double *Z2 = P.Z + B.two_off + off;
+  const double *X2 = P.X + B.two_off + off, *Y2 = P.Y + B.two_off + off;
+  const IMSRG_G int *cst = B.c122_start + B.ch_kstart[ch];
+  const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const double hZ = P.hZ;
   constexpr int kPer = kTile * kTile / 256;
   double u[kPer];
 #pragma unroll
-  for (int i = 0; i < kPer; ++i) {  // upper element (R0 + rr, S0 + cc), consecutive rr
-    const int e = threadIdx.x + 256 * i, rr = e & (kTile - 1), cc = e / kTile;
-    const int r = R0 + rr, c = S0 + cc;
-    u[i] = (r < n && c < n) ? Z2[(long)c * n + r] : 0.0;
+  for (int i = 0; i < kPer; ++i) {  // upper element (R0 + lane, S0 + cc)
+    const int cc = wv + 4 * i, r = R0 + lane, c = S0 + cc;
+    const double z = blk_at(Z2, (unsigned)(min(c, n - 1) * n + min(r, n - 1)));  // clamped: no branch around the load
+    u[i] = (r < n && c < n) ? z : 0.0;
+    if (C122 && c < n) u[i] -= c122_w(X2, Y2, P.c122, B.c122_ind, cst[c], cst[c + 1], n, min(r, n - 1));
   }
 #pragma unroll
-  for (int i = 0; i < kPer; ++i) {  // transposed slot (row S0 + cc, col R0 + rr), consecutive cc
-    const int e = threadIdx.x + 256 * i, cc = e & (kTile - 1), rr = e / kTile;
-    const int r = R0 + rr, c = S0 + cc;
-    if (r < n && c < n) t[rr][cc] = Z2[(long)r * n + c];
+  for (int i = 0; i < kPer; ++i) {  // transposed position (S0 + lane, R0 + rr)
+    const int rr = wv + 4 * i, r = R0 + rr, c = S0 + lane;
+    double v = 0.0;
+    if (FOLD) {
+      const double z = blk_at(Z2, (unsigned)(min(r, n - 1) * n + min(c, n - 1)));
+      v = (r < n && c < n && c != r) ? z : 0.0;
+    }
+    if (C122 && r < n) v -= hZ * c122_w(X2, Y2, P.c122, B.c122_ind, cst[r], cst[r + 1], n, min(c, n - 1));
+    t[rr][lane] = v;
   }
   __syncthreads();
-  const double hZ = P.hZ;
 #pragma unroll
   for (int i = 0; i < kPer; ++i) {
-    const int e = threadIdx.x + 256 * i, rr = e & (kTile - 1), cc = e / kTile;
-    const int r = R0 + rr, c = S0 + cc;
+    const int cc = wv + 4 * i, r = R0 + lane, c = S0 + cc;
     if (r < n && c < n && r <= c) {
-      const double v = (r < c) ? u[i] + t[rr][cc] : (P.hZ == -1 ? 0.0 : u[i]);
-      Z2[(long)c * n + r] = v;
+      const double v = (r == c && P.hZ == -1) ? 0.0 : u[i] + t[lane][cc];
+      blk_at(Z2, (unsigned)(c * n + r)) = v;
       u[i] = v;
     }
   }
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < kPer; ++i) {  // hand the new upper values to the thread that writes their mirror
-    const int e = threadIdx.x + 256 * i, rr = e & (kTile - 1), cc = e / kTile;
-    t[rr][cc] = u[i];
-  }
+  for (int i = 0; i < kPer; ++i) t[lane][wv + 4 * i] = u[i];  // new upper values to the threads writing their mirror
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < kPer; ++i) {  // lower element (S0 + cc, R0 + rr) = hZ * upper, consecutive cc
-    const int e = threadIdx.x + 256 * i, cc = e & (kTile - 1), rr = e / kTile;
-    const int r = R0 + rr, c = S0 + cc;
-    if (r < n && c < n && r < c) Z2[(long)r * n + c] = hZ * t[rr][cc];
+  for (int i = 0; i < kPer; ++i) {  // lower element (S0 + lane, R0 + rr) = hZ * upper
+    const int rr = wv + 4 * i, r = R0 + rr, c = S0 + lane;
+    if (r < n && c < n && r < c) blk_at(Z2, (unsigned)(r * n + c)) = hZ * t[rr][lane];
   }
 }
 
 // Z2 <- 0 over whole tiles (used when the pp/hh term is switched off)
 __global__ __launch_bounds__(256) void k_zero_two(PairArg pa, const int4 *items) {
   const int4 it = items[blockIdx.x];
+  if (it.y < 0) return;  // empty item (padding of an XCD run)
   PairPtrs P = pa.get(it.x);
   const BasisDev &B = *P.basis;
   const int ch = it.y, n = B.ch_n[ch];
@@ -1208,13 +1200,10 @@ hipError_t launch_ph_gemm(const PairArg &pa, const int4 *items, int nitems, int 
   }
   return hipGetLastError();
 }
-hipError_t launch_c122(const PairArg &pa, const int4 *items, int nitems, int maxn, int npairs, long maxn122,
-                       hipStream_t s) {
-  if (nitems <= 0) return hipSuccess;
+hipError_t launch_c122_coef(const PairArg &pa, int npairs, long maxn122, hipStream_t s) {
+  if (maxn122 <= 0) return hipSuccess;
   int cb = (int)std::max<long>(1, std::min<long>((maxn122 + 255) / 256, 4096));
   hipLaunchKernelGGL(k_c122_coef, dim3(cb, npairs), dim3(256), 0, s, pa, npairs);
-  size_t lds = (size_t)2 * maxn * sizeof(double);
-  hipLaunchKernelGGL(k_c122, dim3(nitems), dim3(256), lds, s, pa, items);
   return hipGetLastError();
 }
 hipError_t launch_pandya(const PairArg &pa, const int4 *items, int nitems, hipStream_t s) {
@@ -1230,10 +1219,14 @@ hipError_t launch_invpandya(const PairArg &pa, const int4 *items, int nitems, in
   hipLaunchKernelGGL(k_invpandya, dim3(nitems), dim3(kChunk), lds, s, pa, items);
   return hipGetLastError();
 }
-hipError_t launch_mirror(const PairArg &pa, const int4 *items, int nitems, bool fold, hipStream_t s) {
+hipError_t launch_finish(const PairArg &pa, const int4 *items, int nitems, bool fold, bool c122, hipStream_t s) {
   if (nitems <= 0) return hipSuccess;
-  if (fold)
-    hipLaunchKernelGGL(k_mirror_fold, dim3(nitems), dim3(256), 0, s, pa, items);
+  if (fold && c122)
+    hipLaunchKernelGGL((k_finish<true, true>), dim3(nitems), dim3(256), 0, s, pa, items);
+  else if (fold)
+    hipLaunchKernelGGL((k_finish<true, false>), dim3(nitems), dim3(256), 0, s, pa, items);
+  else if (c122)
+    hipLaunchKernelGGL((k_finish<false, true>), dim3(nitems), dim3(256), 0, s, pa, items);
   else
     hipLaunchKernelGGL(k_mirror, dim3(nitems), dim3(256), 0, s, pa, items);
   return hipGetLastError();

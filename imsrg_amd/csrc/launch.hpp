@@ -17,17 +17,18 @@ hipError_t launch_onebody(const PairArg &pa, const int4 *items, int nitems, hipS
 // maxn: the largest channel dimension of the pairs (sizes the kernel's LDS index list)
 hipError_t launch_pphh(const PairArg &pa, const int4 *items, int nitems, const int4 *items64, int nitems64,
                        int precision, int maxn, hipStream_t s);
-hipError_t launch_c122(const PairArg &pa, const int4 *items, int nitems, int maxn, int npairs, long maxn122,
-                       hipStream_t s);
+// comm122 partner coefficients of this call (used by launch_finish with c122)
+hipError_t launch_c122_coef(const PairArg &pa, int npairs, long maxn122, hipStream_t s);
 hipError_t launch_pandya(const PairArg &pa, const int4 *items, int nitems, hipStream_t s);
 // ph GEMM (head and tail of Zbar) on 128x128 tiles (big) or 64x64 tiles
 hipError_t launch_ph_gemm(const PairArg &pa, const int4 *items, int nitems, int precision, bool big, hipStream_t s);
 // jmax, norb, nch: the largest of the pairs (they size the kernel's LDS tables)
 hipError_t launch_invpandya(const PairArg &pa, const int4 *items, int nitems, int jmax, int norb, int nch,
                             hipStream_t s);
-// lower triangle <- hZ * upper; with fold, first upper += the inverse-Pandya
-// contributions stored at the transposed slots
-hipError_t launch_mirror(const PairArg &pa, const int4 *items, int nitems, bool fold, hipStream_t s);
+// last phase: with fold, upper += the inverse-Pandya contributions stored at the
+// transposed slots; with c122, Z2 -= W + hZ W^T (comm122ss); then lower triangle
+// <- hZ * upper
+hipError_t launch_finish(const PairArg &pa, const int4 *items, int nitems, bool fold, bool c122, hipStream_t s);
 hipError_t launch_zero_two(const PairArg &pa, const int4 *items, int nitems, hipStream_t s);
 
 // operator arithmetic

@@ -187,6 +187,8 @@ typedef struct {
   int mixed_pphh_fp32;      /* IMSRG_PREC_MIXED: 1 = pp/hh GEMM on FP32 MFMA, 0 = on FP64 MFMA */
   int mixed_ph_fp32;        /* IMSRG_PREC_MIXED: 1 = ph GEMM on FP32 MFMA, 0 = on FP64 MFMA */
   int ph_tile_128;          /* 1: ph GEMM on 128x128 tiles, 0: on 64x64 tiles */
+  int finish_order;         /* tile pairs of the last kernel (fold, comm122, mirror): 0 in channel and
+                               row-block order, 1 cut into 8 runs of consecutive tiles, one per XCD */
   double us_per_commutator; /* autotune: measured time of the chosen setting (0 when set by hand) */
 } imsrg_tuning;
 int imsrg_hip_get_tuning(imsrg_ctx *ctx, imsrg_tuning *t);
@@ -199,8 +201,9 @@ int imsrg_hip_set_tuning(imsrg_ctx *ctx, const imsrg_tuning *t);
 int imsrg_hip_autotune(imsrg_ctx *ctx, int precision, const char *cache_path, imsrg_tuning *out);
 
 /* Per-phase timing of the last commutator on ctx (microseconds, HIP events):
- * [0] zero-body, [1] one-body, [2] comm122, [3] pp/hh GEMM, [4] Pandya gather,
- * [5] ph GEMM, [6] inverse Pandya, [7] mirror.  Enabled by imsrg_hip_set_profiling. */
+ * [0] zero-body, [1] one-body, [2] comm122 coefficients, [3] pp/hh GEMM,
+ * [4] Pandya gather, [5] ph GEMM, [6] inverse Pandya, [7] last kernel (fold of
+ * the inverse-Pandya terms, comm122, mirror).  Enabled by imsrg_hip_set_profiling. */
 int imsrg_hip_set_profiling(imsrg_ctx *ctx, int on);
 int imsrg_hip_phase_times(imsrg_ctx *ctx, double *us, int n);
 

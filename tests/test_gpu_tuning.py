@@ -47,6 +47,11 @@ def test_every_setting_matches_oracle(ia):
                     ms.set_tuning(pphh_small_tile_max=small, pphh_order=order, ph_tile_128=ph128)
                     Z = ia.Commutator.CommutatorScalarScalar(dX, dY).to_host()
                     assert rel(Z, ref) < 1e-12 and abs(Z.ZeroBody - ref["zero"]) < 1e-10
+        for fo in (0, 1):
+            say(f"[gpu] tuning finish_order={fo}")
+            ms.set_tuning(finish_order=fo)
+            Z = ia.Commutator.CommutatorScalarScalar(dX, dY).to_host()
+            assert rel(Z, ref) < 1e-12 and abs(Z.ZeroBody - ref["zero"]) < 1e-10
     finally:
         ms.set_tuning(**{k: v for k, v in default.items() if k != "us_per_commutator"})
 
@@ -97,7 +102,7 @@ def test_autotune_and_cache(ia, tmp_path):
 def test_batched_with_differently_tuned_contexts(ia):
     say("[gpu] batched with tuned contexts")
     msa, msb = ia.ModelSpace(4, "O16"), ia.ModelSpace(5, "O16")
-    msb.set_tuning(pphh_small_tile_max=128, pphh_order=1, ph_tile_128=0)
+    msb.set_tuning(pphh_small_tile_max=128, pphh_order=1, ph_tile_128=0, finish_order=1)
     pairs, refs = [], []
     for i, ms in enumerate([msa, msb, msa, msb]):
         X = ia.DeviceOperator(ms, -1).fill_random(-1, 700 + i)
