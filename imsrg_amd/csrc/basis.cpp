@@ -269,7 +269,7 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
   kpp_start[nch] = (int)kpp_loc.size(); khh_start[nch] = (int)khh_loc.size();
   gcore_start[nch] = (int)gcore_loc.size(); gnc_start[nch] = (int)gnc_loc.size();
   // comm122 lists in CSR form over the kets of all channels; each ket's list is
-  // padded to a multiple of 4 entries with its first partner and factor 0
+  // padded to a multiple of kC122Blk entries with its first partner and factor 0
   std::vector<int> c122_start(ch_kets.size() + 1, 0), c122_ind;
   std::vector<int2> c122_ao;
   std::vector<double> c122_f;
@@ -284,7 +284,7 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
         c122_ao.push_back(make_int2(e.y, e.z));
         c122_f.push_back(p122_f[c][u]);
       }
-      for (size_t v = u - t; v % 4 != 0; ++v) {
+      for (size_t v = u - t; v % kC122Blk != 0; ++v) {
         c122_ind.push_back(p122_e[c][t].x);
         c122_ao.push_back(make_int2(p122_e[c][t].y, p122_e[c][t].z));
         c122_f.push_back(0.0);
@@ -380,9 +380,9 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
   {
     long tot = (long)B.d1 * B.d1 * B.d2 * B.d1 * B.d1 * B.d2;
     B.d2q = (B.d2 + 1) & ~1;
-    // the gather copy: padded J3 range, plus one (jj4, jj5) block of J3 pairs so
-    // that a 4-wide J chunk starting at the last pair stays inside the allocation
-    long totq = (long)B.d1 * B.d1 * B.d1 * B.d1 * B.d2q * B.d2q + 2L * B.d1 * B.d1;
+    // the gather copy: padded J3 range, plus kJC / 2 (jj4, jj5) blocks of J3 pairs
+    // so that a kJC-wide J chunk starting at the last pair stays inside the allocation
+    long totq = (long)B.d1 * B.d1 * B.d1 * B.d1 * B.d2q * B.d2q + (long)(kJC / 2) * 2 * B.d1 * B.d1;
     void *p = nullptr, *pQ = nullptr;
     if (hipMalloc(&p, tot * sizeof(double)) != hipSuccess) return "hipMalloc failed for the 6j table";
     allocs.push_back(p);

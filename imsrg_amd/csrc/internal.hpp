@@ -23,6 +23,7 @@ constexpr int kTileB = 128;       // output tile edge of the large pp/hh SYR2K
 constexpr int kTileK = 16;        // k-depth of one LDS stage
 constexpr int kChunk = 256;       // elements per work item of the gather kernels
 constexpr int kJC = 4;            // J values per register chunk of the inverse Pandya gather
+constexpr int kC122Blk = 4;       // comm122 partner lists are padded to multiples of this (one load block)
 constexpr int kF32Flush = 4;      // MFMA k-steps (of 4) summed in FP32 before the FP64 flush (mixed precision)
 
 // Term bits (same values as IMSRG_COMM* in include/imsrg_hip.h)
@@ -44,7 +45,7 @@ struct BasisDev {
   long two_off;  // offset of the two-body arena inside an operator buffer
   long opsize;   // doubles per operator buffer (including 8 doubles of zero slack at the end)
   int maxn, maxcc, maxblk;
-  long n122;           // comm122 partner entries (including the padding to multiples of 4 per ket)
+  long n122;           // comm122 partner entries (including the padding to multiples of kC122Blk per ket)
   const IMSRG_G int *j2, *l, *tz2, *cvq;
   const IMSRG_G double *occ;
   const IMSRG_G int *holes;
@@ -70,7 +71,7 @@ struct BasisDev {
   const IMSRG_G double *khh_w;
   // comm122 partner lists (CSR): the partners of local ket r of channel ch are
   // the entries c122_start[ch_kstart[ch] + r] .. c122_start[ch_kstart[ch] + r + 1],
-  // a multiple of 4 (padding: the first partner again with factor 0)
+  // a multiple of kC122Blk (padding: the first partner again with factor 0)
   const IMSRG_G int *c122_start;
   const IMSRG_G int *c122_ind;     // local ket index ind2 of the partner
   const IMSRG_G int2 *c122_ao;     // (a, o): coefficient X1(a, o) * f

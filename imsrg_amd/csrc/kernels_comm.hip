@@ -890,6 +890,8 @@ __global__ __launch_bounds__(256, 6) void k_invpandya(PairArg pa, const int4 *it
   const double hf = (P.hZ == 1) ? 1.0 : -1.0;
   const double *Zb = P.Zbar;
   double *Ztwo = P.Z + B.two_off;
+  const IMSRG_G double *sixq = B.sixjQ;  // held in registers: the stores below could alias B
+  const IMSRG_G int *kmapT = B.kmapT;
   const int emin = (P.hZ == 1) ? b : b + 1;
   for (int e0 = emin; e0 < ng; e0 += blockDim.x) {
     const int e = e0 + threadIdx.x;
@@ -959,12 +961,13 @@ __global__ __launch_bounds__(256, 6) void k_invpandya(PairArg pa, const int4 *it
         const int il = Ti[Jp * norb + l];
         const int kj = Tj[Jp * norb + k] + (k > j ? nkc : 0);
         const double z = (2 * Jp + 1) * Zb[s_zoff[cc] + (long)kj * nkc + il];
-        const double *s6 = B.sixjQ + s6d + (long)Jp * s6J + (long)(c0 >> 1) * s6P;
+        const double *s6 = sixq + s6d + (long)Jp * s6J + (long)(c0 >> 1) * s6P;
         const double2 w01 = *(const double2 *)s6, w23 = *(const double2 *)(s6 + s6P);
-        acc[0] += (vm & 1) && fabs(w01.x) >= 1e-8 ? w01.x * z : 0.0;
-        acc[1] += (vm & 2) && fabs(w01.y) >= 1e-8 ? w01.y * z : 0.0;
-        acc[2] += (vm & 4) && fabs(w23.x) >= 1e-8 ? w23.x * z : 0.0;
-        acc[3] += (vm & 8) && fabs(w23.y) >= 1e-8 ? w23.y * z : 0.0;
+        // 0/1 factors rather than selects keep both 16-byte 6j loads unconditional
+        acc[0] += ((vm & 1) && fabs(w01.x) >= 1e-8 ? 1.0 : 0.0) * (w01.x * z);
+        acc[1] += ((vm & 2) && fabs(w01.y) >= 1e-8 ? 1.0 : 0.0) * (w01.y * z);
+        acc[2] += ((vm & 4) && fabs(w23.x) >= 1e-8 ? 1.0 : 0.0) * (w23.x * z);
+        acc[3] += ((vm & 8) && fabs(w23.y) >= 1e-8 ? 1.0 : 0.0) * (w23.y * z);
       }
       for (int Jp = xlo; Jp <= xhi; ++Jp) {
         if (Jp < Xlo || Jp > Xhi) continue;
@@ -974,12 +977,12 @@ __global__ __launch_bounds__(256, 6) void k_invpandya(PairArg pa, const int4 *it
         const int lj = Tj[Jp * norb + l];
         const double me = Zb[s_zoff[cc] + (long)((l > j ? nkc : 0) + ik) * nkc + lj];
         const double z = (2 * Jp + 1) * (symx * me) * sx0;  // the sign alternates with J below
-        const double *s6 = B.sixjQ + s6x + (long)Jp * s6J + (long)(c0 >> 1) * s6P;
+        const double *s6 = sixq + s6x + (long)Jp * s6J + (long)(c0 >> 1) * s6P;
         const double2 w01 = *(const double2 *)s6, w23 = *(const double2 *)(s6 + s6P);
-        acc[0] += (vm & 1) && fabs(w01.x) >= 1e-8 ? w01.x * z : 0.0;
-        acc[1] -= (vm & 2) && fabs(w01.y) >= 1e-8 ? w01.y * z : 0.0;
-        acc[2] += (vm & 4) && fabs(w23.x) >= 1e-8 ? w23.x * z : 0.0;
-        acc[3] -= (vm & 8) && fabs(w23.y) >= 1e-8 ? w23.y * z : 0.0;
+        acc[0] += ((vm & 1) && fabs(w01.x) >= 1e-8 ? 1.0 : 0.0) * (w01.x * z);
+        acc[1] -= ((vm & 2) && fabs(w01.y) >= 1e-8 ? 1.0 : 0.0) * (w01.y * z);
+        acc[2] += ((vm & 4) && fabs(w23.x) >= 1e-8 ? 1.0 : 0.0) * (w23.x * z);
+        acc[3] -= ((vm & 8) && fabs(w23.y) >= 1e-8 ? 1.0 : 0.0) * (w23.y * z);
       }
 #pragma unroll
       for (int t = 0; t < kJC; ++t) {
@@ -990,7 +993,7 @@ __global__ __launch_bounds__(256, 6) void k_invpandya(PairArg pa, const int4 *it
         const double f = (k == l) ? 1.0 - s : (i == j) ? 1.0 - s * pij : 1.0;
         const double z = acc[t] * f / nrm;
         const int n = s_chn[J];
-        const int ib = s_ib[J], ik = B.kmapT[(long)J * nk + kk];
+        const int ib = s_ib[J], ik = kmapT[(long)J * nk + kk];
         double *Z2 = Ztwo + s_choff[J];
         if (ik == ib)
           Z2[(long)ib * n + ib] += z;
@@ -1043,12 +1046,12 @@ __device__ __forceinline__ const T &blk_at(const T *base, unsigned idx) {
 // W(row, q) of comm122ss (src/Commutator.cc:908-926) for one column q of a
 // channel: sum over the partner entries e of ket q of Y2(row, ind_e) x_e -
 // X2(row, ind_e) y_e.  q (hence e0, e1) is uniform over the wave; row is the
-// lane's.  Every column's list holds a multiple of 4 entries (padded with its
-// first partner and a zero factor), read in blocks of 4 whose loads are all
-// issued before the first use.
+// lane's.  Every column's list holds a multiple of kC122Blk entries (padded
+// with its first partner and a zero factor), read in blocks of kC122Blk whose
+// loads are all issued before the first use.
 __device__ __forceinline__ double c122_w(const double *X2, const double *Y2, const IMSRG_G double2 *coef,
                                          const IMSRG_G int *ind, int e0, int e1, int n, int row) {
-  constexpr int kB = 4;
+  constexpr int kB = kC122Blk;
   double w = 0.0;
   for (int eb = e0; eb < e1; eb += kB) {
     double y[kB], x[kB];
