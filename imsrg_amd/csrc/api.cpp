@@ -31,14 +31,14 @@ struct BatchCache {
   size_t off[8] = {0};
   int cnt[8] = {0};
   double *work = nullptr;
-  int maxn = 0, maxnch = 0, maxjmax = 0, maxnorb = 0;
+  int maxn = 0, maxnch = 0, maxjmax = 0, maxnorb = 0, maxn1pairs = 0;
   long maxn122 = 0;
   void release() {
     if (d_tab) (void)hipFree(d_tab);
     if (d_items) (void)hipFree(d_items);
     if (work) (void)hipFree(work);
     d_tab = nullptr; d_items = nullptr; work = nullptr;
-    maxn = maxnch = maxjmax = maxnorb = 0;
+    maxn = maxnch = maxjmax = maxnorb = maxn1pairs = 0;
     maxn122 = 0;
     key.clear();
   }
@@ -63,6 +63,7 @@ struct imsrg_ctx {
   double *Xt = nullptr, *Yt = nullptr, *Zbar = nullptr;
   double *scratch = nullptr;
   double2 *c122 = nullptr;
+  double *obpart = nullptr;  // one-body partial sums (nobpart)
   double *partials = nullptr;
   double *norm_dev = nullptr;
   double *norm_host = nullptr;  // pinned
@@ -117,6 +118,7 @@ static int ctx_init(imsrg_ctx *ctx, const imsrg_basis_desc *b, int device) {
   HIPCHK(ctx, hipMemset(ctx->Zbar, 0, sizeof(double) * std::max<long>(B.zsize, 1)));
   HIPCHK(ctx, hipMalloc(&ctx->scratch, sizeof(double) * (B.nch + 8)));
   HIPCHK(ctx, hipMalloc(&ctx->c122, sizeof(double2) * std::max<long>(B.n122, 1)));
+  HIPCHK(ctx, hipMalloc(&ctx->obpart, sizeof(double) * std::max(B.nobpart, 1)));
   ctx->npart = 257;
   HIPCHK(ctx, hipMalloc(&ctx->partials, sizeof(double) * ctx->npart));
   HIPCHK(ctx, hipMalloc(&ctx->norm_dev, sizeof(double) * 2));
@@ -169,6 +171,7 @@ extern "C" int imsrg_hip_ctx_destroy(imsrg_ctx *ctx) {
   if (ctx->Zbar) (void)hipFree(ctx->Zbar);
   if (ctx->scratch) (void)hipFree(ctx->scratch);
   if (ctx->c122) (void)hipFree(ctx->c122);
+  if (ctx->obpart) (void)hipFree(ctx->obpart);
   if (ctx->partials) (void)hipFree(ctx->partials);
   if (ctx->norm_dev) (void)hipFree(ctx->norm_dev);
   if (ctx->norm_host) (void)hipHostFree(ctx->norm_host);
@@ -432,13 +435,14 @@ static long r8(long n) { return (n + 7) / 8 * 8; }  // 64-byte aligned segment l
 static int herm_rule(int hx, int hy) { return (hx == hy) ? -1 : 1; }  // src/Commutator.cc:292-297
 
 // Launch sequence of one commutator (or a batch when pa.tab is set).
-// items / cnt: [0] pp/hh 128-tiles, [1] ph tiles, [2] none (comm122 runs in the
-// finishing kernel, over [6]), [3] Pandya chunks, [4] inverse-Pandya items,
+// items / cnt: [0] pp/hh 128-tiles, [1] ph tiles, [2] comm221 one-body items,
+// [3] Pandya chunks, [4] inverse-Pandya items,
 // [5] one-body pairs, [6] tile pairs of the finishing kernel,
 // [7] pp/hh 64-tiles.  Under mixed precision the context's tuning picks the
 // FP32 or FP64 kernel of each GEMM family.
 static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const int4 *const *items, const int *cnt,
-                          unsigned mask, int prec, int maxn, int maxnch, long maxn122, int maxjmax, int maxnorb) {
+                          unsigned mask, int prec, int maxn, int maxnch, long maxn122, int maxjmax, int maxnorb,
+                          int maxn1pairs) {
   const Tuning &tu = ctx->db.tune;
   const int prec_pphh = (prec == IMSRG_PREC_MIXED && tu.mixed_pphh) ? IMSRG_PREC_MIXED : IMSRG_PREC_FP64;
   const int prec_ph = (prec == IMSRG_PREC_MIXED && tu.mixed_ph) ? IMSRG_PREC_MIXED : IMSRG_PREC_FP64;
@@ -448,7 +452,8 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
   HIPCHK(ctx, mark(0));
   HIPCHK(ctx, launch_zerobody(pa, npairs, maxnch, s));
   HIPCHK(ctx, mark(1));
-  HIPCHK(ctx, launch_onebody(pa, items[5], cnt[5], s));
+  HIPCHK(ctx, launch_onebody(pa, items[5], cnt[5], (mask & M221) ? items[2] : nullptr, (mask & M221) ? cnt[2] : 0,
+                             npairs, maxn1pairs, s));
   HIPCHK(ctx, mark(2));
   if (mask & MPPHH)
     HIPCHK(ctx, launch_pphh(pa, items[0], cnt[0], items[7], cnt[7], prec_pphh, maxn, s));
@@ -495,17 +500,18 @@ static int commutator_dev(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y
   pa.p0.Zbar = ctx->Zbar;
   pa.p0.scratch = ctx->scratch;
   pa.p0.c122 = ctx->c122;
+  pa.p0.obpart = ctx->obpart;
   pa.p0.hX = X->herm;
   pa.p0.hY = Y->herm;
   pa.p0.hZ = herm_rule(X->herm, Y->herm);
   pa.p0.mask = mask;
   Z->herm = pa.p0.hZ;
-  const int4 *items[8] = {db.d_pphh, db.d_ph, nullptr, db.d_pandya, db.d_invp, db.d_one, db.d_mirror, db.d_pphh64};
-  int cnt[8] = {(int)db.wl.pphh.size(), (int)db.wl.ph.size(), 0, (int)db.wl.pandya.size(),
+  const int4 *items[8] = {db.d_pphh, db.d_ph, db.d_ob221, db.d_pandya, db.d_invp, db.d_one, db.d_mirror, db.d_pphh64};
+  int cnt[8] = {(int)db.wl.pphh.size(), (int)db.wl.ph.size(), (int)db.wl.ob221.size(), (int)db.wl.pandya.size(),
                 (int)db.wl.invp.size(), (int)db.wl.one.size(), (int)db.wl.mirror.size(), (int)db.wl.pphh64.size()};
   if (!ctx->use_graphs || ctx->profiling)
     return run_commutator(ctx, pa, 1, items, cnt, mask, prec, db.h.maxn, db.h.nch, db.h.n122, db.h.jmax,
-                          db.h.norb);
+                          db.h.norb, db.h.n1pairs);
   // replay a captured launch sequence for this operand set, capturing it on first use
   for (const GraphEntry &g : ctx->graphs)
     if (g.x == X->buf && g.y == Y->buf && g.z == Z->buf && g.hx == X->herm && g.hy == Y->herm && g.mask == mask &&
@@ -515,7 +521,7 @@ static int commutator_dev(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y
     }
   HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
   const int rc = run_commutator(ctx, pa, 1, items, cnt, mask, prec, db.h.maxn, db.h.nch, db.h.n122, db.h.jmax,
-                          db.h.norb);
+                                db.h.norb, db.h.n1pairs);
   hipGraph_t graph = nullptr;
   const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);  // always leave capture mode
   if (rc) {
@@ -838,7 +844,7 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
     for (int i = 0; i < npairs; ++i) {
       woff[i] = wtot;
       const BasisDev &Bi = ctxs[i]->db.h;
-      wtot += 2 * r8(Bi.xsize) + r8(Bi.zsize) + r8(Bi.nch + 8) + 2 * r8(Bi.n122) + 64;
+      wtot += 2 * r8(Bi.xsize) + r8(Bi.zsize) + r8(Bi.nch + 8) + 2 * r8(Bi.n122) + r8(Bi.nobpart) + 64;
     }
     HIPCHK(c0, hipMalloc(&bc.work, sizeof(double) * std::max<long>(wtot, 1)));
     HIPCHK(c0, hipMemset(bc.work, 0, sizeof(double) * std::max<long>(wtot, 1)));
@@ -853,6 +859,7 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
       p.Zbar = p.Yt + r8(B.xsize);
       p.scratch = p.Zbar + r8(B.zsize);
       p.c122 = (double2 *)(p.scratch + r8(B.nch + 8));
+      p.obpart = (double *)(p.c122) + 2 * r8(B.n122);
       p.hX = X[i]->herm; p.hY = Y[i]->herm; p.hZ = herm_rule(X[i]->herm, Y[i]->herm);
       p.mask = mask;
     }
@@ -863,8 +870,7 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
     std::vector<TileGroup> groups;
     for (int i = 0; i < npairs; ++i) {
       const WorkLists &wl = ctxs[i]->db.wl;
-      const std::vector<int4> none;
-      const std::vector<int4> *src[8] = {&wl.pphh, &wl.ph, &none, &wl.pandya, &wl.invp, &wl.one, &wl.mirror,
+      const std::vector<int4> *src[8] = {&wl.pphh, &wl.ph, &wl.ob221, &wl.pandya, &wl.invp, &wl.one, &wl.mirror,
                                          &wl.pphh64};
       {  // ph tiles in the tile size of the first context, whose kernels run the batch
         const int tile = c0->db.tune.ph_big ? kTileB : kTile;
@@ -896,6 +902,7 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
       bc.maxnch = std::max(bc.maxnch, ctxs[i]->db.h.nch);
       bc.maxjmax = std::max(bc.maxjmax, ctxs[i]->db.h.jmax);
       bc.maxnorb = std::max(bc.maxnorb, ctxs[i]->db.h.norb);
+      bc.maxn1pairs = std::max(bc.maxn1pairs, ctxs[i]->db.h.n1pairs);
       bc.maxn122 = std::max(bc.maxn122, ctxs[i]->db.h.n122);
     }
     if (c0->db.tune.pphh_order >= 1) {
@@ -926,7 +933,7 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
   const int4 *items[8];
   for (int t = 0; t < 8; ++t) items[t] = bc.d_items + bc.off[t];
   return run_commutator(c0, pa, npairs, items, bc.cnt, mask, prec, bc.maxn, bc.maxnch, bc.maxn122, bc.maxjmax,
-                        bc.maxnorb);
+                        bc.maxnorb, bc.maxn1pairs);
 }
 
 // ---------------------------------------------------------------------------

@@ -374,6 +374,36 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
   B.n1pairs = (int)pairs1.size();
   B.pairs1 = dupload(*this, pairs1, err);
   B.w1norm = dupload(*this, w1, err);
+  // comm221 one-body work (src/Commutator.cc:1246-1294) in partial sums: pair p
+  // owns the slots ob_start[p] .. ob_start[p + 1] - 1 of the per-call buffer;
+  // the first takes its comm111 + comm121 part, every item one more slot.
+  //  kind 0 (n_c != 0, M_pp over the long pp lists): one item per (c, J);
+  //  kind 1 (1 - n_c != 0, M_hh over the short hh lists): one item per 4 entries
+  //  of the list cpart of such orbits c (one wave each, all J)
+  std::vector<int> ob_start(pairs1.size() + 1, 0), cpart;
+  for (int c = 0; c < norb; ++c)
+    if (std::fabs(1.0 - hb.occ[c]) > 1e-9) cpart.push_back(c);
+  ob221.clear();
+  {
+    int slot = 0;
+    for (size_t p = 0; p < pairs1.size(); ++p) {
+      ob_start[p] = slot++;
+      const int i = pairs1[p].x, j = pairs1[p].y;
+      for (int c = 0; c < norb; ++c) {
+        if (!(std::fabs(hb.occ[c]) > 1e-9)) continue;
+        const int Jmin = std::max(std::abs(hb.j2[c] - hb.j2[i]), std::abs(hb.j2[c] - hb.j2[j])) / 2;
+        const int Jmax = (hb.j2[c] + std::min(hb.j2[i], hb.j2[j])) / 2;
+        for (int J = Jmin; J <= Jmax; ++J) ob221.push_back(make_int4(0, (int)p, (c << 8) | J, slot++));
+      }
+      for (size_t q = 0; q < cpart.size(); q += 4)
+        ob221.push_back(make_int4(0, (int)p, (1 << 30) | (int)q, slot++));
+    }
+    ob_start[pairs1.size()] = slot;
+    B.nobpart = slot;
+  }
+  B.ncpart = (int)cpart.size();
+  B.ob_start = dupload(*this, ob_start, err);
+  B.cpart = dupload(*this, cpart, err);
   if (!err.empty()) return err;
 
   // ---- 6j table on the device ----
@@ -535,7 +565,8 @@ std::string DeviceBasis::build_lists(const Tuning &t) {
     const long tot = (long)nk * 2 * nph;
     for (long s = 0; s < tot; s += kChunk) wl.pandya.push_back(make_int4(0, c, (int)s, 0));
   }
-  for (auto &pr : pairs1) wl.one.push_back(make_int4(0, pr.x, pr.y, 0));
+  for (size_t p = 0; p < pairs1.size(); ++p) wl.one.push_back(make_int4(0, pairs1[p].x, pairs1[p].y, (int)p));
+  wl.ob221 = ob221;
 
   // upload, replacing the previous lists
   for (void *p : list_allocs) (void)hipFree(p);
@@ -555,7 +586,7 @@ std::string DeviceBasis::build_lists(const Tuning &t) {
   };
   d_pphh = up(wl.pphh); d_pphh64 = up(wl.pphh64); d_ph = up(wl.ph);
   d_pandya = up(wl.pandya); d_invp = up(wl.invp);
-  d_one = up(wl.one); d_mirror = up(wl.mirror); d_gen = up(wl.gen);
+  d_one = up(wl.one); d_mirror = up(wl.mirror); d_gen = up(wl.gen); d_ob221 = up(wl.ob221);
   if (err.empty()) tune = t;
   return err;
 }

@@ -62,7 +62,8 @@ struct WorkLists {
   std::vector<int4> ph;      // (pair, cc, rb, sb), rb <= sb, tiles of 128 (ph_big) or 64
   std::vector<int4> pandya;  // (pair, cc, start, 0)
   std::vector<int4> invp;    // (pair, group, bra, first ket) of the (parity, Tz) ket groups
-  std::vector<int4> one;     // (pair, i, j, 0)
+  std::vector<int4> one;     // (pair, i, j, index of the one-body pair)
+  std::vector<int4> ob221;   // (pair, one-body pair, kind/c/J, partial-sum slot) of the comm221 one-body part
   std::vector<int4> mirror;  // (pair, ch, rb, sb), rb < sb  and diagonal tiles; ch = -1: empty item
   std::vector<int4> gen;     // (pair, ch, 0, 0) channels with work for the generator
 };
@@ -76,11 +77,12 @@ struct DeviceBasis {
   WorkLists wl;             // host lists with pair = 0
   Tuning tune;
   int4 *d_pphh = nullptr, *d_pphh64 = nullptr, *d_ph = nullptr, *d_pandya = nullptr,
-       *d_invp = nullptr, *d_one = nullptr, *d_mirror = nullptr, *d_gen = nullptr;
+       *d_invp = nullptr, *d_one = nullptr, *d_mirror = nullptr, *d_gen = nullptr, *d_ob221 = nullptr;
   // host tables the work lists are built from
   std::vector<int> ch_n, k_start, cc_n, cc_nph, grp_start, grp_kets, gen_ok;
   std::vector<int> ch_kstart, c122_start;
   std::vector<int2> pairs1;
+  std::vector<int4> ob221;
   std::string upload(const HostBasis &hb, int emax6j);  // builds tables, lists, 6j table
   std::string build_lists(const Tuning &t);             // (re)builds and uploads the work lists
   void release();

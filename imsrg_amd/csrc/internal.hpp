@@ -82,6 +82,10 @@ struct BasisDev {
   const IMSRG_G long *cc_xoff, *cc_zoff;
   long xsize, zsize;
   const IMSRG_G int2 *pairs1;   // one-body pairs (i<=j, same (l,j2,tz2) block)
+  // partial sums of the one-body part: slots ob_start[p] .. ob_start[p+1]-1 of pair p
+  // (the first: comm111 + comm121, the others: comm221 items); cpart: orbits with 1 - n_c != 0
+  const IMSRG_G int *ob_start, *cpart;
+  int nobpart, ncpart;
   const IMSRG_G double *w1norm; // norb*norb one-body norm weights
   const IMSRG_G double *sixj;   // dense 6j table, layout of SixJCache_112112
   const IMSRG_G double *sixjQ;  // same values, layout [jj1][jj2][J6][J3/2][jj4][jj5][J3%2] (sixjQ_index):
@@ -100,6 +104,7 @@ struct PairPtrs {
   IMSRG_G double *Xt, *Yt, *Zbar; // Pandya workspaces
   IMSRG_G double *scratch;        // >= nch + 8 doubles (zero-body partial sums)
   IMSRG_G double2 *c122;          // per-call comm122 coefficients (X1(a,o) f, Y1(a,o) f), n122 entries
+  IMSRG_G double *obpart;         // per-call one-body partial sums, nobpart entries
   int hX, hY, hZ;
   unsigned mask;
 };

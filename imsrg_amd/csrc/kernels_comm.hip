@@ -172,11 +172,13 @@ __global__ __launch_bounds__(256) void k_onebody(PairArg pa, const int4 *items) 
   const int norb = B.norb, i = it.y, j = it.z;
   const double *X1 = P.X + B.one_off, *Y1 = P.Y + B.one_off;
   const double *X2 = P.X + B.two_off, *Y2 = P.Y + B.two_off;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  // (X1 Y1)(i,j) and (Y1 X1)(i,j) summed apart, as the reference's two matrix
+  // products: commuting blocks then give exactly 0
+  double acc[3] = {0.0, 0.0, 0.0};
   if (P.mask & M111) {
     for (int k = threadIdx.x; k < norb; k += blockDim.x) {
       acc[0] += X1at(X1, norb, i, k) * X1at(Y1, norb, k, j);
-      acc[1] += X1at(Y1, norb, i, k) * X1at(X1, norb, k, j);
+      acc[2] += X1at(Y1, norb, i, k) * X1at(X1, norb, k, j);
     }
   }
   if (P.mask & M121) {
@@ -194,59 +196,84 @@ __global__ __launch_bounds__(256) void k_onebody(PairArg pa, const int4 *items) 
       else if (term == 1) v = -(B.j2[a] + 1) * nanb * X1at(X1, norb, b, a) * tbme_monopole(B, Y2, a, i, b, j);
       else if (term == 2) v = -(B.j2[b] + 1) * nanb * X1at(Y1, norb, a, b) * tbme_monopole(B, X2, b, i, a, j);
       else v = (B.j2[a] + 1) * nanb * X1at(Y1, norb, b, a) * tbme_monopole(B, X2, a, i, b, j);
-      acc[2] += v;
+      acc[1] += v;
     }
   }
-  if (P.mask & M221) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int nJ = B.jmax + 1;
-    const double hX = P.hX, hZ = P.hZ;
-    // (c, J) combos; M_pp parts (long k lists) one wave each, M_hh parts (short
-    // lists) one lane each
-    for (int pass = 0; pass < 2; ++pass) {
-      const int ncombo = norb * nJ;
-      const int first = pass == 0 ? w : threadIdx.x;
-      const int step = pass == 0 ? 4 : (int)blockDim.x;
-      for (int t = first; t < ncombo; t += step) {
-        const int c = t / nJ, J = t % nJ;
-        const int Jmin = max(abs(B.j2[c] - B.j2[i]), abs(B.j2[c] - B.j2[j])) / 2;
-        const int Jmax = (B.j2[c] + min(B.j2[i], B.j2[j])) / 2;
-        if (J < Jmin || J > Jmax) continue;
-        const double nc = B.occ[c], nbc = 1.0 - nc;
-        if (pass == 0 ? !(fabs(nc) > 1e-9) : !(fabs(nbc) > 1e-9)) continue;
-        const int kci = ket_index(norb, min(c, i), max(c, i)), kcj = ket_index(norb, min(c, j), max(c, j));
-        const int r = B.kmap[(long)kci * nJ + J], s = B.kmap[(long)kcj * nJ + J];
-        if (r < 0 || s < 0) continue;
-        const int ch = 6 * J + 2 * ((B.tz2[c] + B.tz2[i]) / 2 + 1) + ((B.l[c] + B.l[i]) & 1);
-        const int n = B.ch_n[ch];
-        const long off = B.ch_off[ch];
-        double fac = (c == i ? SQRT2_D : 1.0) * (c == j ? SQRT2_D : 1.0);
-        if (c > i) fac *= sgn_parity((B.j2[c] + B.j2[i]) / 2 + 1 + J);
-        if (c > j) fac *= sgn_parity((B.j2[c] + B.j2[j]) / 2 + 1 + J);
-        const double *xr = X2 + off + (long)r * n, *xs = X2 + off + (long)s * n;
-        const double *yr = Y2 + off + (long)r * n, *ys = Y2 + off + (long)s * n;
-        if (pass == 0) {
-          const int k0 = B.kpp_start[ch];
-          const double m = fold_dot(xr, xs, yr, ys, B.kpp_loc + k0, B.kpp_w + k0, B.kpp_start[ch + 1] - k0, lane, 64, hZ);
-          acc[3] += (2 * J + 1) * nc * fac * hX * m;
-        } else {
-          const int k0 = B.khh_start[ch];
-          const double m = fold_dot(xr, xs, yr, ys, B.khh_loc + k0, B.khh_w + k0, B.khh_start[ch + 1] - k0, 0, 1, hZ);
-          acc[3] += (2 * J + 1) * nbc * fac * hX * m;
-        }
-      }
+  block_reduce<3>(acc);
+  if (threadIdx.x == 0) P.obpart[B.ob_start[it.w]] = (acc[0] - acc[2]) + acc[1];
+}
+
+// Lane's share of (2J+1) fac M(ci, cj) for the comm221 one-body part
+// (src/Commutator.cc:1246-1294): M = M_pp (pp = true, ConstructScalarMpp_Mhh's
+// pp list) or M_hh, formed as the folded dot product M(r,s) + hZ M(s,r) over
+// the list entries start, start + stride, ...; 0 when a ket is missing.
+__device__ __forceinline__ double m221_part(const BasisDev &B, const double *X2, const double *Y2, int i, int j,
+                                            int c, int J, bool pp, int start, int stride, double hZ) {
+  const int norb = B.norb, nJ = B.jmax + 1;
+  const int kci = ket_index(norb, min(c, i), max(c, i)), kcj = ket_index(norb, min(c, j), max(c, j));
+  const int r = B.kmap[(long)kci * nJ + J], s = B.kmap[(long)kcj * nJ + J];
+  if (r < 0 || s < 0) return 0.0;
+  const int ch = 6 * J + 2 * ((B.tz2[c] + B.tz2[i]) / 2 + 1) + ((B.l[c] + B.l[i]) & 1);
+  const int n = B.ch_n[ch];
+  const long off = B.ch_off[ch];
+  double fac = (c == i ? SQRT2_D : 1.0) * (c == j ? SQRT2_D : 1.0);
+  if (c > i) fac *= sgn_parity((B.j2[c] + B.j2[i]) / 2 + 1 + J);
+  if (c > j) fac *= sgn_parity((B.j2[c] + B.j2[j]) / 2 + 1 + J);
+  const double *xr = X2 + off + (long)r * n, *xs = X2 + off + (long)s * n;
+  const double *yr = Y2 + off + (long)r * n, *ys = Y2 + off + (long)s * n;
+  const int k0 = pp ? B.kpp_start[ch] : B.khh_start[ch], k1 = pp ? B.kpp_start[ch + 1] : B.khh_start[ch + 1];
+  const double m = pp ? fold_dot(xr, xs, yr, ys, B.kpp_loc + k0, B.kpp_w + k0, k1 - k0, start, stride, hZ)
+                      : fold_dot(xr, xs, yr, ys, B.khh_loc + k0, B.khh_w + k0, k1 - k0, start, stride, hZ);
+  return (2 * J + 1) * fac * m;
+}
+
+// comm221 one-body part in partial sums, one slot per item (it.w):
+//  kind 0 (n_c != 0): n_c M_pp for one (c, J), the block's lanes over the pp list;
+//  kind 1 (1 - n_c != 0): (1 - n_c) M_hh for 4 orbits c of the list cpart
+//  (one per wave), all J, the wave's lanes over the hh list.
+__global__ __launch_bounds__(256) void k_onebody221(PairArg pa, const int4 *items) {
+  const int4 it = items[blockIdx.x];
+  PairPtrs P = pa.get(it.x);
+  const BasisDev &B = *P.basis;
+  const int2 pr = B.pairs1[it.y];
+  const int i = pr.x, j = pr.y;
+  const double *X2 = P.X + B.two_off, *Y2 = P.Y + B.two_off;
+  const double hZ = P.hZ;
+  double acc[1] = {0.0};
+  if (!(it.z >> 30)) {
+    const int c = (it.z >> 8) & 0xffff, J = it.z & 0xff;
+    acc[0] = B.occ[c] * m221_part(B, X2, Y2, i, j, c, J, true, threadIdx.x, blockDim.x, hZ);
+  } else {
+    const int q = (it.z & 0x3fffffff) + (threadIdx.x >> 6);
+    if (q < B.ncpart) {
+      const int c = B.cpart[q], lane = threadIdx.x & 63;
+      const int Jmin = max(abs(B.j2[c] - B.j2[i]), abs(B.j2[c] - B.j2[j])) / 2;
+      const int Jmax = (B.j2[c] + min(B.j2[i], B.j2[j])) / 2;
+      double m = 0.0;
+      for (int J = Jmin; J <= Jmax; ++J) m += m221_part(B, X2, Y2, i, j, c, J, false, lane, 64, hZ);
+      acc[0] = (1.0 - B.occ[c]) * m;
     }
   }
-  block_reduce<4>(acc);
-  if (threadIdx.x == 0) {
-    double z = 0.0;
-    if (P.mask & M111) z += acc[0] - acc[1];
-    z += acc[2];
-    z += acc[3] / (B.j2[i] + 1.0);
-    double *Z1 = P.Z + B.one_off;
-    Z1[(long)j * norb + i] = z;
-    if (i != j) Z1[(long)i * norb + j] = P.hZ * z;
-  }
+  block_reduce<1>(acc);
+  if (threadIdx.x == 0) P.obpart[it.w] = P.hX * acc[0];
+}
+
+// Z1(i, j) = comm111 + comm121 + comm221 / (j_i + 1) from the partial sums of
+// one-body pair p (fixed order), and Z1(j, i) = hZ Z1(i, j).
+__global__ __launch_bounds__(256) void k_onebody_final(PairArg pa) {
+  PairPtrs P = pa.get(blockIdx.y);
+  const BasisDev &B = *P.basis;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= B.n1pairs) return;
+  const int2 pr = B.pairs1[p];
+  const int i = pr.x, j = pr.y, s0 = B.ob_start[p], s1 = B.ob_start[p + 1];
+  double z221 = 0.0;
+  if (P.mask & M221)
+    for (int s = s0 + 1; s < s1; ++s) z221 += P.obpart[s];
+  const double z = P.obpart[s0] + z221 / (B.j2[i] + 1.0);
+  double *Z1 = P.Z + B.one_off;
+  Z1[(long)j * B.norb + i] = z;
+  if (i != j) Z1[(long)i * B.norb + j] = P.hZ * z;
 }
 
 // ---------------------------------------------------------------------------
@@ -1160,9 +1187,12 @@ hipError_t launch_zerobody(const PairArg &pa, int npairs, int maxnch, hipStream_
   hipLaunchKernelGGL(k_zerobody_final, dim3(npairs), dim3(64), 0, s, pa);
   return hipGetLastError();
 }
-hipError_t launch_onebody(const PairArg &pa, const int4 *items, int nitems, hipStream_t s) {
+hipError_t launch_onebody(const PairArg &pa, const int4 *items, int nitems, const int4 *items221, int n221,
+                          int npairs, int maxn1pairs, hipStream_t s) {
   if (nitems <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_onebody, dim3(nitems), dim3(256), 0, s, pa, items);
+  if (n221 > 0) hipLaunchKernelGGL(k_onebody221, dim3(n221), dim3(256), 0, s, pa, items221);
+  hipLaunchKernelGGL(k_onebody_final, dim3((maxn1pairs + 255) / 256, npairs), dim3(256), 0, s, pa);
   return hipGetLastError();
 }
 hipError_t launch_pphh(const PairArg &pa, const int4 *items, int nitems, const int4 *items64, int nitems64,

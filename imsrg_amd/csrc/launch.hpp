@@ -12,7 +12,10 @@ hipError_t launch_sixj_table(double *table, double *tableQ, int d1, int d2, int 
 
 // commutator phases; items are device arrays of int4 work items
 hipError_t launch_zerobody(const PairArg &pa, int npairs, int maxnch, hipStream_t s);
-hipError_t launch_onebody(const PairArg &pa, const int4 *items, int nitems, hipStream_t s);
+// one-body terms: comm111 + comm121 per pair (items), comm221 in partial sums
+// (items221, none when that term is off), then the sums into Z1
+hipError_t launch_onebody(const PairArg &pa, const int4 *items, int nitems, const int4 *items221, int n221,
+                          int npairs, int maxn1pairs, hipStream_t s);
 // pp/hh SYR2K: items on 128x128 tiles, items64 on 64x64 tiles (disjoint channels)
 // maxn: the largest channel dimension of the pairs (sizes the kernel's LDS index list)
 hipError_t launch_pphh(const PairArg &pa, const int4 *items, int nitems, const int4 *items64, int nitems64,
