@@ -21,6 +21,10 @@ namespace imsrg {
 constexpr int kTile = 64;         // SYR2K output tile edge
 constexpr int kTileB = 128;       // output tile edge of the large pp/hh SYR2K
 constexpr int kTileK = 16;        // k-depth of one LDS stage
+constexpr int kGWaves = 8;        // waves of a 128x128 GEMM work-group (2 x 4 grid of 64 x kGW sub-tiles)
+constexpr int kGW = 32;           // columns of a wave's sub-tile in the 128x128 GEMM
+constexpr int kGN = kGW / 16;     // 16-column MFMA blocks per wave sub-tile
+constexpr int kGC = kTileK / kGWaves;  // k columns each wave copies into LDS per stage
 constexpr int kChunk = 256;       // elements per work item of the gather kernels
 constexpr int kJC = 4;            // J values per register chunk of the inverse Pandya gather
 constexpr int kC122Blk = 4;       // comm122 partner lists are padded to multiples of this (one load block)

@@ -572,18 +572,21 @@ __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
   }
   const int R0 = it.z * T, S0 = it.w * T;
   const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int wr = (wv & 1) * 64, wc = (wv >> 1) * 64;
-  // a wave whose 64x64 sub-tile holds no upper-triangle element inside the
-  // matrix (padding past n, or below the diagonal) issues no MFMA
-  auto live = [&](int w) { return R0 + (w & 1) * 64 < n && S0 + (w >> 1) * 64 < n && R0 + (w & 1) * 64 <= S0 + (w >> 1) * 64 + 63; };
+  // 8 waves, each a 64x32 sub-tile: rows wr .. wr+63, columns wc .. wc+31
+  const int wr = (wv & 1) * 64, wc = (wv >> 1) * kGW;
+  // a wave whose sub-tile holds no upper-triangle element inside the matrix
+  // (padding past n, or below the diagonal) issues no MFMA
+  auto live = [&](int w) {
+    return R0 + (w & 1) * 64 < n && S0 + (w >> 1) * kGW < n && R0 + (w & 1) * 64 <= S0 + (w >> 1) * kGW + kGW - 1;
+  };
   const bool wave_live = live(wv);
-  dbl4 acc[4][4];
+  dbl4 acc[4][kGN];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
+    for (int b = 0; b < kGN; ++b) acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
   const int K2 = 2 * nk;
-  // Stage loader: wave w copies the k' columns 4w .. 4w+3 of the stage, for
+  // Stage loader: wave w copies the k' columns 2w, 2w+1 of the stage, for
   // the A and the B row block, each a 128-row segment of one column, with one
   // 16-byte-per-lane global->LDS copy (lane l: rows 2l, 2l+1).  Rows past n
   // are never written back; their source is clamped so that a row pair reads
@@ -598,18 +601,18 @@ __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
     for (int e = threadIdx.x; e < nk; e += blockDim.x) KLs[e] = kl[e];
     __syncthreads();
   }
-  auto load_idx = [&](int k0, int (&ki)[4]) {
+  auto load_idx = [&](int k0, int (&ki)[kGC]) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int kk = min(k0 + wv * 4 + j, K2 - 1);
+    for (int j = 0; j < kGC; ++j) {
+      const int kk = min(k0 + wv * kGC + j, K2 - 1);
       const int k = kk < nk ? kk : kk - nk;
       ki[j] = (MODE == 0) ? KLs[k] : k;
     }
   };
-  auto issue_stage = [&](int k0, const int (&ki)[4], double (&As)[kTileK][LD], double (&Bs)[kTileK][LD]) {
+  auto issue_stage = [&](int k0, const int (&ki)[kGC], double (&As)[kTileK][LD], double (&Bs)[kTileK][LD]) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int kq = wv * 4 + j;
+    for (int j = 0; j < kGC; ++j) {
+      const int kq = wv * kGC + j;
       const int kk = min(k0 + kq, K2 - 1);
       const int ca = ki[j];
       const int cb = (MODE == 2) ? (ca < nph ? ca + nph : ca - nph) : ca;
@@ -636,7 +639,7 @@ __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
       Ws[threadIdx.x] = kk >= K2 ? 0.0 : kk < nk ? w : w * h;
     }
   };
-  int ki[4];
+  int ki[kGC];
   if (K2 > 0) {  // a channel without pp or hh kets only writes zeros
     load_idx(0, ki);
     issue_stage(0, ki, A0, B0);
@@ -657,45 +660,49 @@ __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
     load_idx(k0 + 2 * kTileK, ki);
     if (!wave_live) {
     } else if constexpr (MIXED) {
-      // one row block mi at a time, operands re-read from LDS, so that only four
-      // FP32 partial blocks are live next to the FP64 accumulators
+      // FP32 partial sums of all 4 x kGN blocks over the stage's 16 k, then one
+      // flush into the FP64 accumulators
+      static_assert(kF32Flush == kTileK / 4, "one FP32 flush per stage");
+      flt4 t[4][kGN];
 #pragma unroll
-      for (int mi = 0; mi < 4; ++mi) {
+      for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
-        for (int kp = 0; kp < kTileK / 4; kp += kF32Flush) {
-          flt4 t[4];
+        for (int ni = 0; ni < kGN; ++ni) t[mi][ni] = flt4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-          for (int ni = 0; ni < 4; ++ni) t[ni] = flt4{0.f, 0.f, 0.f, 0.f};
+      for (int ks = 0; ks < kTileK / 4; ++ks) {
+        const int kr = ks * 4 + (lane >> 4);
+        const double wk = Wc[kr];
+        float am[4], bm[kGN];
 #pragma unroll
-          for (int ks = kp; ks < kp + kF32Flush; ++ks) {
-            const int kr = ks * 4 + (lane >> 4);
-            const float am = (float)(Ac[kr][wr + mi * 16 + (lane & 15)] * Wc[kr]);
+        for (int m = 0; m < 4; ++m) am[m] = (float)(Ac[kr][wr + m * 16 + (lane & 15)] * wk);
 #pragma unroll
-            for (int ni = 0; ni < 4; ++ni)
-              t[ni] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)Bc[kr][wc + ni * 16 + (lane & 15)], am, t[ni], 0,
-                                                           0, 0);
-          }
+        for (int m = 0; m < kGN; ++m) bm[m] = (float)Bc[kr][wc + m * 16 + (lane & 15)];
 #pragma unroll
-          for (int ni = 0; ni < 4; ++ni)
+        for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) acc[mi][ni][q] += (double)t[ni][q];
-        }
+          for (int ni = 0; ni < kGN; ++ni)
+            t[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x4f32(bm[ni], am[mi], t[mi][ni], 0, 0, 0);
       }
+#pragma unroll
+      for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < kGN; ++ni)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) acc[mi][ni][q] += (double)t[mi][ni][q];
     } else {
 #pragma unroll
       for (int ks = 0; ks < kTileK / 4; ++ks) {
         const int kr = ks * 4 + (lane >> 4);
         const double wk = Wc[kr];
-        double a[4], b[4];
+        double a[4], b[kGN];
 #pragma unroll
-        for (int m = 0; m < 4; ++m) {
-          a[m] = Ac[kr][wr + m * 16 + (lane & 15)] * wk;
-          b[m] = Bc[kr][wc + m * 16 + (lane & 15)];
-        }
+        for (int m = 0; m < 4; ++m) a[m] = Ac[kr][wr + m * 16 + (lane & 15)] * wk;
+#pragma unroll
+        for (int m = 0; m < kGN; ++m) b[m] = Bc[kr][wc + m * 16 + (lane & 15)];
 #pragma unroll
         for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
-          for (int ni = 0; ni < 4; ++ni)
+          for (int ni = 0; ni < kGN; ++ni)
             acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[ni], a[mi], acc[mi][ni], 0, 0, 0);
       }
     }
@@ -711,7 +718,7 @@ __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
 #pragma unroll
     for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
-      for (int ni = 0; ni < 4; ++ni)
+      for (int ni = 0; ni < kGN; ++ni)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int row = R0 + wr + mi * 16 + (lane & 15);
@@ -719,64 +726,60 @@ __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
           if (row < n && col < n && row <= col) D[(long)col * n + row] = alpha * acc[mi][ni][q];
         }
   } else {
-    // full Zbar storage through an LDS transpose, one 32x64 half of a wave
-    // sub-tile per round (the operand buffer A0, idle now, holds the tile)
-    double(*Tt)[kTile + 1] = reinterpret_cast<double(*)[kTile + 1]>(&A0[0][0]);
-    static_assert(32 * (kTile + 1) <= kTileK * LD, "transpose tile exceeds the operand buffer");
-    for (int w = 0; w < 4; ++w) {
+    // full Zbar storage through an LDS transpose of one 64x32 wave sub-tile per
+    // round (the operand buffer A0, idle now, holds the tile)
+    double(*Tt)[kGW + 1] = reinterpret_cast<double(*)[kGW + 1]>(&A0[0][0]);
+    static_assert(64 * (kGW + 1) <= kTileK * LD, "transpose tile exceeds the operand buffer");
+    for (int w = 0; w < kGWaves; ++w) {
       if (!live(w)) continue;  // block-uniform
-      const int wr2 = (w & 1) * 64, wc2 = (w >> 1) * 64;
-      for (int hh = 0; hh < 2; ++hh) {
-        if (wv == w) {
+      const int wr2 = (w & 1) * 64, wc2 = (w >> 1) * kGW;
+      if (wv == w) {
 #pragma unroll
-          for (int mi = 0; mi < 4; ++mi) {
-            if ((mi >> 1) != hh) continue;
+        for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
-            for (int ni = 0; ni < 4; ++ni)
+          for (int ni = 0; ni < kGN; ++ni)
 #pragma unroll
-              for (int q = 0; q < 4; ++q) {
-                const int rr = (mi & 1) * 16 + (lane & 15), cc = ni * 16 + mfma_col<MIXED>(lane, q);
-                double v = alpha * acc[mi][ni][q];
-                if (MODE == 2) {
-                  const int col = S0 + wc2 + cc;
-                  v *= (col < n) ? (double)phi[col] : 0.0;
-                }
-                Tt[rr][cc] = v;
+            for (int q = 0; q < 4; ++q) {
+              const int rr = mi * 16 + (lane & 15), cc = ni * 16 + mfma_col<MIXED>(lane, q);
+              double v = alpha * acc[mi][ni][q];
+              if (MODE == 2) {
+                const int col = S0 + wc2 + cc;
+                v *= (col < n) ? (double)phi[col] : 0.0;
               }
-          }
-        }
-        __syncthreads();
-        const int R1 = R0 + wr2 + 32 * hh, S1 = S0 + wc2;
-        // upper elements (R1 + rr, S1 + cc), consecutive threads on consecutive rows
-        for (int e = threadIdx.x; e < 32 * kTile; e += blockDim.x) {
-          const int rr = e & 31, cc = e >> 5;
-          const int row = R1 + rr, col = S1 + cc;
-          if (row < n && col < n && row <= col) D[(long)col * n + row] = Tt[rr][cc];
-        }
-        // mirrored elements (S1 + cc, R1 + rr) = sym * upper (R1 + rr, S1 + cc),
-        // consecutive threads on consecutive rows S1 + cc
-        for (int e = threadIdx.x; e < 32 * kTile; e += blockDim.x) {
-          const int cc = e & (kTile - 1), rr = e >> 6;
-          const int row2 = S1 + cc, col2 = R1 + rr;
-          if (row2 < n && col2 < n && row2 > col2) {
-            const double sym = (MODE == 1) ? h : (double)(phi[row2] * phi[col2]);
-            D[(long)col2 * n + row2] = sym * Tt[rr][cc];
-          }
-        }
-        __syncthreads();
+              Tt[rr][cc] = v;
+            }
       }
+      __syncthreads();
+      const int R1 = R0 + wr2, S1 = S0 + wc2;
+      // upper elements (R1 + rr, S1 + cc), consecutive threads on consecutive rows
+      for (int e = threadIdx.x; e < 64 * kGW; e += blockDim.x) {
+        const int rr = e & 63, cc = e >> 6;
+        const int row = R1 + rr, col = S1 + cc;
+        if (row < n && col < n && row <= col) D[(long)col * n + row] = Tt[rr][cc];
+      }
+      // mirrored elements (S1 + cc, R1 + rr) = sym * upper (R1 + rr, S1 + cc),
+      // consecutive threads on consecutive rows S1 + cc
+      for (int e = threadIdx.x; e < 64 * kGW; e += blockDim.x) {
+        const int cc = e & (kGW - 1), rr = e / kGW;
+        const int row2 = S1 + cc, col2 = R1 + rr;
+        if (row2 < n && col2 < n && row2 > col2) {
+          const double sym = (MODE == 1) ? h : (double)(phi[row2] * phi[col2]);
+          D[(long)col2 * n + row2] = sym * Tt[rr][cc];
+        }
+      }
+      __syncthreads();
     }
   }
 }
 
-__global__ __launch_bounds__(256, 2) void k_pphh_big_f64(PairArg pa, const int4 *items) {
+__global__ __launch_bounds__(kGWaves * 64, 4) void k_pphh_big_f64(PairArg pa, const int4 *items) {
   gemm_tile<0, false>(pa, items);
 }
-__global__ __launch_bounds__(256, 2) void k_pphh_big_mixed(PairArg pa, const int4 *items) {
+__global__ __launch_bounds__(kGWaves * 64, 4) void k_pphh_big_mixed(PairArg pa, const int4 *items) {
   gemm_tile<0, true>(pa, items);
 }
 template <int MODE, bool MIXED>
-__global__ __launch_bounds__(256, 2) void k_ph_big(PairArg pa, const int4 *items) {
+__global__ __launch_bounds__(kGWaves * 64, 4) void k_ph_big(PairArg pa, const int4 *items) {
   gemm_tile<MODE, MIXED>(pa, items);
 }
 
@@ -1200,9 +1203,9 @@ hipError_t launch_pphh(const PairArg &pa, const int4 *items, int nitems, const i
   if (nitems > 0) {
     const size_t lds = sizeof(int) * (size_t)std::max(maxn, 1);  // column-index list of the largest channel
     if (precision == 1)
-      hipLaunchKernelGGL(k_pphh_big_mixed, dim3(nitems), dim3(256), lds, s, pa, items);
+      hipLaunchKernelGGL(k_pphh_big_mixed, dim3(nitems), dim3(kGWaves * 64), lds, s, pa, items);
     else
-      hipLaunchKernelGGL(k_pphh_big_f64, dim3(nitems), dim3(256), lds, s, pa, items);
+      hipLaunchKernelGGL(k_pphh_big_f64, dim3(nitems), dim3(kGWaves * 64), lds, s, pa, items);
   }
   if (nitems64 > 0) {
     if (precision == 1)
@@ -1216,11 +1219,11 @@ hipError_t launch_ph_gemm(const PairArg &pa, const int4 *items, int nitems, int 
   if (nitems <= 0) return hipSuccess;
   if (big) {
     if (precision == 1) {
-      hipLaunchKernelGGL((k_ph_big<1, true>), dim3(nitems), dim3(256), 0, s, pa, items);
-      hipLaunchKernelGGL((k_ph_big<2, true>), dim3(nitems), dim3(256), 0, s, pa, items);
+      hipLaunchKernelGGL((k_ph_big<1, true>), dim3(nitems), dim3(kGWaves * 64), 0, s, pa, items);
+      hipLaunchKernelGGL((k_ph_big<2, true>), dim3(nitems), dim3(kGWaves * 64), 0, s, pa, items);
     } else {
-      hipLaunchKernelGGL((k_ph_big<1, false>), dim3(nitems), dim3(256), 0, s, pa, items);
-      hipLaunchKernelGGL((k_ph_big<2, false>), dim3(nitems), dim3(256), 0, s, pa, items);
+      hipLaunchKernelGGL((k_ph_big<1, false>), dim3(nitems), dim3(kGWaves * 64), 0, s, pa, items);
+      hipLaunchKernelGGL((k_ph_big<2, false>), dim3(nitems), dim3(kGWaves * 64), 0, s, pa, items);
     }
     return hipGetLastError();
   }
