@@ -462,7 +462,7 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
   HIPCHK(ctx, mark(3));
   if (mask & M122) HIPCHK(ctx, launch_c122_coef(pa, npairs, maxn122, s));
   HIPCHK(ctx, mark(4));
-  if (mask & MPH) HIPCHK(ctx, launch_pandya(pa, items[3], cnt[3], s));
+  if (mask & MPH) HIPCHK(ctx, launch_pandya(pa, items[3], cnt[3], maxnch, s));
   HIPCHK(ctx, mark(5));
   if (mask & MPH) HIPCHK(ctx, launch_ph_gemm(pa, items[1], cnt[1], prec_ph, tu.ph_big != 0, s));
   HIPCHK(ctx, mark(6));

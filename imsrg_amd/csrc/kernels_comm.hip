@@ -787,9 +787,19 @@ __global__ __launch_bounds__(kGWaves * 64, 4) void k_ph_big(PairArg pa, const in
 // DoPandyaTransformation_SingleChannel_XandY (src/Commutator.cc:1392-1485):
 // Xbar (nk x 2nph) and Ybar^T (nk x 2nph), one element each per thread.
 __global__ __launch_bounds__(256) void k_pandya(PairArg pa, const int4 *items) {
+  extern __shared__ __align__(8) unsigned char lds_raw[];
   const int4 it = items[blockIdx.x];
   PairPtrs P = pa.get(it.x);
   const BasisDev &B = *P.basis;
+  // channel offsets and sizes in LDS: the gather of each J then waits on one
+  // LDS read instead of two dependent global loads
+  long *s_choff = (long *)lds_raw;
+  int *s_chn = (int *)(s_choff + B.nch);
+  for (int c = threadIdx.x; c < B.nch; c += blockDim.x) {
+    s_choff[c] = B.ch_off[c];
+    s_chn[c] = B.ch_n[c];
+  }
+  __syncthreads();
   const int cc = it.y;
   const int nk = B.cc_n[cc], nph = B.cc_nph[cc];
   const long tot = (long)nk * 2 * nph;
@@ -825,15 +835,15 @@ __global__ __launch_bounds__(256) void k_pandya(PairArg pa, const int4 *items) {
     const double hY = P.hY;
     if (pcb == pad) {
       for (int J = jmin; J <= jmax; J += dJ) {
+        // the 6j value and both ket maps are loaded together (no branch between them)
         const double sj = B.sixj[sixj_index(B.d1, B.d2, ja, jb, Jcc, jc, jd, J)];
-        if (fabs(sj) < 1e-8) continue;
         const int i1 = B.kmap[(long)k1 * (B.jmax + 1) + J], i2 = B.kmap[(long)k2 * (B.jmax + 1) + J];
-        if (i1 < 0 || i2 < 0) continue;
+        if (fabs(sj) < 1e-8 || i1 < 0 || i2 < 0) continue;
         const int ch = 6 * J + 2 * (tcb + 1) + pcb;
         double ph = 1.0;
         if (c > b) ph *= sgn_parity((jc + jb) / 2 + 1 + J);
         if (a > d) ph *= sgn_parity((ja + jd) / 2 + 1 + J);
-        const long idx = B.ch_off[ch] + (long)i2 * B.ch_n[ch] + i1;
+        const long idx = s_choff[ch] + (long)i2 * s_chn[ch] + i1;
         const double w = (2 * J + 1) * sj;
         xbar -= w * (ph * X2[idx]);
         ybar -= w * (ph * Y2[idx]) * hY;
@@ -1242,9 +1252,10 @@ hipError_t launch_c122_coef(const PairArg &pa, int npairs, long maxn122, hipStre
   hipLaunchKernelGGL(k_c122_coef, dim3(cb, npairs), dim3(256), 0, s, pa, npairs);
   return hipGetLastError();
 }
-hipError_t launch_pandya(const PairArg &pa, const int4 *items, int nitems, hipStream_t s) {
+hipError_t launch_pandya(const PairArg &pa, const int4 *items, int nitems, int maxnch, hipStream_t s) {
   if (nitems <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_pandya, dim3(nitems), dim3(kChunk), 0, s, pa, items);
+  const size_t lds = (sizeof(long) + sizeof(int)) * (size_t)std::max(maxnch, 1);
+  hipLaunchKernelGGL(k_pandya, dim3(nitems), dim3(kChunk), lds, s, pa, items);
   return hipGetLastError();
 }
 hipError_t launch_invpandya(const PairArg &pa, const int4 *items, int nitems, int jmax, int norb, int nch,

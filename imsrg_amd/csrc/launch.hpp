@@ -22,7 +22,8 @@ hipError_t launch_pphh(const PairArg &pa, const int4 *items, int nitems, const i
                        int precision, int maxn, hipStream_t s);
 // comm122 partner coefficients of this call (used by launch_finish with c122)
 hipError_t launch_c122_coef(const PairArg &pa, int npairs, long maxn122, hipStream_t s);
-hipError_t launch_pandya(const PairArg &pa, const int4 *items, int nitems, hipStream_t s);
+// maxnch: the largest channel count of the pairs (sizes the kernel's LDS channel table)
+hipError_t launch_pandya(const PairArg &pa, const int4 *items, int nitems, int maxnch, hipStream_t s);
 // ph GEMM (head and tail of Zbar) on 128x128 tiles (big) or 64x64 tiles
 hipError_t launch_ph_gemm(const PairArg &pa, const int4 *items, int nitems, int precision, bool big, hipStream_t s);
 // jmax, norb, nch: the largest of the pairs (they size the kernel's LDS tables)
