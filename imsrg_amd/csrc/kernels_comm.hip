@@ -259,17 +259,20 @@ __global__ __launch_bounds__(256) void k_onebody221(PairArg pa, const int4 *item
 }
 
 // Z1(i, j) = comm111 + comm121 + comm221 / (j_i + 1) from the partial sums of
-// one-body pair p (fixed order), and Z1(j, i) = hZ Z1(i, j).
+// one-body pair p, one wave per pair (lane-strided sums and a fixed shuffle
+// tree: the same order every call), and Z1(j, i) = hZ Z1(i, j).
 __global__ __launch_bounds__(256) void k_onebody_final(PairArg pa) {
   PairPtrs P = pa.get(blockIdx.y);
   const BasisDev &B = *P.basis;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= B.n1pairs) return;
+  const int p = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (p >= B.n1pairs) return;  // wave-uniform
   const int2 pr = B.pairs1[p];
   const int i = pr.x, j = pr.y, s0 = B.ob_start[p], s1 = B.ob_start[p + 1];
   double z221 = 0.0;
   if (P.mask & M221)
-    for (int s = s0 + 1; s < s1; ++s) z221 += P.obpart[s];
+    for (int s = s0 + 1 + lane; s < s1; s += 64) z221 += P.obpart[s];
+  for (int o = 32; o > 0; o >>= 1) z221 += __shfl_down(z221, o, 64);
+  if (lane != 0) return;
   const double z = P.obpart[s0] + z221 / (B.j2[i] + 1.0);
   double *Z1 = P.Z + B.one_off;
   Z1[(long)j * B.norb + i] = z;
@@ -1205,7 +1208,7 @@ hipError_t launch_onebody(const PairArg &pa, const int4 *items, int nitems, cons
   if (nitems <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_onebody, dim3(nitems), dim3(256), 0, s, pa, items);
   if (n221 > 0) hipLaunchKernelGGL(k_onebody221, dim3(n221), dim3(256), 0, s, pa, items221);
-  hipLaunchKernelGGL(k_onebody_final, dim3((maxn1pairs + 255) / 256, npairs), dim3(256), 0, s, pa);
+  hipLaunchKernelGGL(k_onebody_final, dim3((maxn1pairs + 3) / 4, npairs), dim3(256), 0, s, pa);
   return hipGetLastError();
 }
 hipError_t launch_pphh(const PairArg &pa, const int4 *items, int nitems, const int4 *items64, int nitems64,
