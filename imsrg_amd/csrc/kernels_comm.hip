@@ -896,7 +896,7 @@ __host__ __device__ inline size_t invp_lds_bytes(int jmax, int norb, int nch) {
   return 8 * (size_t)nch + 8 * nJ + 4 * (size_t)nch + 4 * nJ * (2 * (size_t)norb + 2);
 }
 
-__global__ __launch_bounds__(256, 6) void k_invpandya(PairArg pa, const int4 *items) {
+__global__ __launch_bounds__(256, 5) void k_invpandya(PairArg pa, const int4 *items) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const int4 it = items[blockIdx.x];
   PairPtrs P = pa.get(it.x);
@@ -1005,12 +1005,15 @@ __global__ __launch_bounds__(256, 6) void k_invpandya(PairArg pa, const int4 *it
         const int kj = Tj[Jp * norb + k] + (k > j ? nkc : 0);
         const double z = (2 * Jp + 1) * Zb[s_zoff[cc] + (long)kj * nkc + il];
         const double *s6 = sixq + s6d + (long)Jp * s6J + (long)(c0 >> 1) * s6P;
-        const double2 w01 = *(const double2 *)s6, w23 = *(const double2 *)(s6 + s6P);
-        // 0/1 factors rather than selects keep both 16-byte 6j loads unconditional
-        acc[0] += ((vm & 1) && fabs(w01.x) >= 1e-8 ? 1.0 : 0.0) * (w01.x * z);
-        acc[1] += ((vm & 2) && fabs(w01.y) >= 1e-8 ? 1.0 : 0.0) * (w01.y * z);
-        acc[2] += ((vm & 4) && fabs(w23.x) >= 1e-8 ? 1.0 : 0.0) * (w23.x * z);
-        acc[3] += ((vm & 8) && fabs(w23.y) >= 1e-8 ? 1.0 : 0.0) * (w23.y * z);
+        // 0/1 factors rather than selects keep the 16-byte 6j loads unconditional
+        double2 w[kJC / 2];
+#pragma unroll
+        for (int hh = 0; hh < kJC / 2; ++hh) w[hh] = *(const double2 *)(s6 + hh * s6P);
+#pragma unroll
+        for (int hh = 0; hh < kJC / 2; ++hh) {
+          acc[2 * hh] += (((vm >> (2 * hh)) & 1) && fabs(w[hh].x) >= 1e-8 ? 1.0 : 0.0) * (w[hh].x * z);
+          acc[2 * hh + 1] += (((vm >> (2 * hh + 1)) & 1) && fabs(w[hh].y) >= 1e-8 ? 1.0 : 0.0) * (w[hh].y * z);
+        }
       }
       for (int Jp = xlo; Jp <= xhi; ++Jp) {
         if (Jp < Xlo || Jp > Xhi) continue;
@@ -1021,11 +1024,14 @@ __global__ __launch_bounds__(256, 6) void k_invpandya(PairArg pa, const int4 *it
         const double me = Zb[s_zoff[cc] + (long)((l > j ? nkc : 0) + ik) * nkc + lj];
         const double z = (2 * Jp + 1) * (symx * me) * sx0;  // the sign alternates with J below
         const double *s6 = sixq + s6x + (long)Jp * s6J + (long)(c0 >> 1) * s6P;
-        const double2 w01 = *(const double2 *)s6, w23 = *(const double2 *)(s6 + s6P);
-        acc[0] += ((vm & 1) && fabs(w01.x) >= 1e-8 ? 1.0 : 0.0) * (w01.x * z);
-        acc[1] -= ((vm & 2) && fabs(w01.y) >= 1e-8 ? 1.0 : 0.0) * (w01.y * z);
-        acc[2] += ((vm & 4) && fabs(w23.x) >= 1e-8 ? 1.0 : 0.0) * (w23.x * z);
-        acc[3] -= ((vm & 8) && fabs(w23.y) >= 1e-8 ? 1.0 : 0.0) * (w23.y * z);
+        double2 w[kJC / 2];
+#pragma unroll
+        for (int hh = 0; hh < kJC / 2; ++hh) w[hh] = *(const double2 *)(s6 + hh * s6P);
+#pragma unroll
+        for (int hh = 0; hh < kJC / 2; ++hh) {
+          acc[2 * hh] += (((vm >> (2 * hh)) & 1) && fabs(w[hh].x) >= 1e-8 ? 1.0 : 0.0) * (w[hh].x * z);
+          acc[2 * hh + 1] -= (((vm >> (2 * hh + 1)) & 1) && fabs(w[hh].y) >= 1e-8 ? 1.0 : 0.0) * (w[hh].y * z);
+        }
       }
 #pragma unroll
       for (int t = 0; t < kJC; ++t) {
