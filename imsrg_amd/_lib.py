@@ -55,7 +55,7 @@ class BCHOpts(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("pphh_small_tile_max", C.c_int), ("pphh_order", C.c_int), ("mixed_pphh_fp32", C.c_int),
                 ("mixed_ph_fp32", C.c_int), ("ph_tile_128", C.c_int),
-                ("finish_order", C.c_int), ("us_per_commutator", C.c_double)]
+                ("finish_order", C.c_int), ("overlap_pphh", C.c_int), ("us_per_commutator", C.c_double)]
 
 
 class HipError(RuntimeError):

@@ -58,12 +58,17 @@ constexpr size_t kMaxGraphs = 16;
 struct imsrg_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // second stream for the pp/hh GEMM when it overlaps the one-body, Pandya and
+  // ph phases (tuning field overlap_pphh); default priority, like `stream`
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
   HostBasis hb;
   DeviceBasis db;
   double *Xt = nullptr, *Yt = nullptr, *Zbar = nullptr;
   double *scratch = nullptr;
   double2 *c122 = nullptr;
   double *obpart = nullptr;  // one-body partial sums (nobpart)
+  double *diag = nullptr;    // inverse-Pandya diagonal terms (ndiag)
   double *partials = nullptr;
   double *norm_dev = nullptr;
   double *norm_host = nullptr;  // pinned
@@ -109,6 +114,9 @@ static int ctx_init(imsrg_ctx *ctx, const imsrg_basis_desc *b, int device) {
   e = ctx->db.upload(ctx->hb, b->emax);
   if (!e.empty()) return fail(ctx, IMSRG_E_HIP, e);
   HIPCHK(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  HIPCHK(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+  HIPCHK(ctx, hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming));
+  HIPCHK(ctx, hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming));
   const BasisDev &B = ctx->db.h;
   // 8 doubles of slack: the GEMM tile loader may read one element past a column
   HIPCHK(ctx, hipMalloc(&ctx->Xt, sizeof(double) * (std::max<long>(B.xsize, 1) + 8)));
@@ -119,6 +127,7 @@ static int ctx_init(imsrg_ctx *ctx, const imsrg_basis_desc *b, int device) {
   HIPCHK(ctx, hipMalloc(&ctx->scratch, sizeof(double) * (B.nch + 8)));
   HIPCHK(ctx, hipMalloc(&ctx->c122, sizeof(double2) * std::max<long>(B.n122, 1)));
   HIPCHK(ctx, hipMalloc(&ctx->obpart, sizeof(double) * std::max(B.nobpart, 1)));
+  HIPCHK(ctx, hipMalloc(&ctx->diag, sizeof(double) * std::max(B.ndiag, 1)));
   ctx->npart = 257;
   HIPCHK(ctx, hipMalloc(&ctx->partials, sizeof(double) * ctx->npart));
   HIPCHK(ctx, hipMalloc(&ctx->norm_dev, sizeof(double) * 2));
@@ -163,6 +172,7 @@ extern "C" int imsrg_hip_ctx_create_ho(int emax, double A, double Z, int device,
 extern "C" int imsrg_hip_ctx_destroy(imsrg_ctx *ctx) {
   if (!ctx) return IMSRG_OK;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->side) (void)hipStreamSynchronize(ctx->side);
   for (auto *&o : ctx->stage) if (o) { imsrg_hip_op_destroy(o); o = nullptr; }
   for (auto *&o : ctx->tmp) if (o) { imsrg_hip_op_destroy(o); o = nullptr; }
   ctx->batch.release();
@@ -172,6 +182,7 @@ extern "C" int imsrg_hip_ctx_destroy(imsrg_ctx *ctx) {
   if (ctx->scratch) (void)hipFree(ctx->scratch);
   if (ctx->c122) (void)hipFree(ctx->c122);
   if (ctx->obpart) (void)hipFree(ctx->obpart);
+  if (ctx->diag) (void)hipFree(ctx->diag);
   if (ctx->partials) (void)hipFree(ctx->partials);
   if (ctx->norm_dev) (void)hipFree(ctx->norm_dev);
   if (ctx->norm_host) (void)hipHostFree(ctx->norm_host);
@@ -181,6 +192,9 @@ extern "C" int imsrg_hip_ctx_destroy(imsrg_ctx *ctx) {
   ctx->graphs.clear();
   ctx->db.release();
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  if (ctx->side) (void)hipStreamDestroy(ctx->side);
+  if (ctx->fork) (void)hipEventDestroy(ctx->fork);
+  if (ctx->join) (void)hipEventDestroy(ctx->join);
   delete ctx;
   return IMSRG_OK;
 }
@@ -449,16 +463,30 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
   hipStream_t s = ctx->stream;
   bool prof = ctx->profiling;
   auto mark = [&](int i) -> hipError_t { return prof ? hipEventRecord(ctx->ev[i], s) : hipSuccess; };
+  // With overlap (never while profiling), the pp/hh GEMM runs on the side
+  // stream next to the zero-body, one-body, comm122-coefficient, Pandya, ph
+  // GEMM and inverse Pandya kernels (none of them writes the upper triangle
+  // of Z's two-body blocks); the main stream waits for it before the last
+  // kernel.
+  const bool overlap = !prof && tu.overlap_pphh && (mask & MPPHH);
+  if (overlap) {
+    HIPCHK(ctx, hipEventRecord(ctx->fork, s));
+    HIPCHK(ctx, hipStreamWaitEvent(ctx->side, ctx->fork, 0));
+    HIPCHK(ctx, launch_pphh(pa, items[0], cnt[0], items[7], cnt[7], prec_pphh, maxn, ctx->side));
+    HIPCHK(ctx, hipEventRecord(ctx->join, ctx->side));
+  }
   HIPCHK(ctx, mark(0));
   HIPCHK(ctx, launch_zerobody(pa, npairs, maxnch, s));
   HIPCHK(ctx, mark(1));
   HIPCHK(ctx, launch_onebody(pa, items[5], cnt[5], (mask & M221) ? items[2] : nullptr, (mask & M221) ? cnt[2] : 0,
                              npairs, maxn1pairs, s));
   HIPCHK(ctx, mark(2));
-  if (mask & MPPHH)
+  if (overlap) {
+  } else if (mask & MPPHH) {
     HIPCHK(ctx, launch_pphh(pa, items[0], cnt[0], items[7], cnt[7], prec_pphh, maxn, s));
-  else
+  } else {
     HIPCHK(ctx, launch_zero_two(pa, items[6], cnt[6], s));
+  }
   HIPCHK(ctx, mark(3));
   if (mask & M122) HIPCHK(ctx, launch_c122_coef(pa, npairs, maxn122, s));
   HIPCHK(ctx, mark(4));
@@ -467,6 +495,7 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
   if (mask & MPH) HIPCHK(ctx, launch_ph_gemm(pa, items[1], cnt[1], prec_ph, tu.ph_big != 0, s));
   HIPCHK(ctx, mark(6));
   if (mask & MPH) HIPCHK(ctx, launch_invpandya(pa, items[4], cnt[4], maxjmax, maxnorb, maxnch, s));
+  if (overlap) HIPCHK(ctx, hipStreamWaitEvent(s, ctx->join, 0));
   HIPCHK(ctx, mark(7));
   HIPCHK(ctx, launch_finish(pa, items[6], cnt[6], (mask & MPH) != 0, (mask & M122) != 0, s));
   HIPCHK(ctx, mark(8));
@@ -501,6 +530,7 @@ static int commutator_dev(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y
   pa.p0.scratch = ctx->scratch;
   pa.p0.c122 = ctx->c122;
   pa.p0.obpart = ctx->obpart;
+  pa.p0.diag = ctx->diag;
   pa.p0.hX = X->herm;
   pa.p0.hY = Y->herm;
   pa.p0.hZ = herm_rule(X->herm, Y->herm);
@@ -844,7 +874,7 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
     for (int i = 0; i < npairs; ++i) {
       woff[i] = wtot;
       const BasisDev &Bi = ctxs[i]->db.h;
-      wtot += 2 * r8(Bi.xsize) + r8(Bi.zsize) + r8(Bi.nch + 8) + 2 * r8(Bi.n122) + r8(Bi.nobpart) + 64;
+      wtot += 2 * r8(Bi.xsize) + r8(Bi.zsize) + r8(Bi.nch + 8) + 2 * r8(Bi.n122) + r8(Bi.nobpart) + r8(Bi.ndiag) + 64;
     }
     HIPCHK(c0, hipMalloc(&bc.work, sizeof(double) * std::max<long>(wtot, 1)));
     HIPCHK(c0, hipMemset(bc.work, 0, sizeof(double) * std::max<long>(wtot, 1)));
@@ -860,6 +890,7 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
       p.scratch = p.Zbar + r8(B.zsize);
       p.c122 = (double2 *)(p.scratch + r8(B.nch + 8));
       p.obpart = (double *)(p.c122) + 2 * r8(B.n122);
+      p.diag = p.obpart + r8(B.nobpart);
       p.hX = X[i]->herm; p.hY = Y[i]->herm; p.hZ = herm_rule(X[i]->herm, Y[i]->herm);
       p.mask = mask;
     }
@@ -945,6 +976,7 @@ static void tuning_out(const Tuning &t, double us, imsrg_tuning *o) {
   o->mixed_ph_fp32 = t.mixed_ph;
   o->ph_tile_128 = t.ph_big;
   o->finish_order = t.finish_order;
+  o->overlap_pphh = t.overlap_pphh;
   o->us_per_commutator = us;
 }
 
@@ -970,7 +1002,8 @@ extern "C" int imsrg_hip_set_tuning(imsrg_ctx *ctx, const imsrg_tuning *t) {
   if (!ctx || !t) return IMSRG_E_INVALID;
   if (t->pphh_small_tile_max < 0 || t->pphh_order < 0 || t->pphh_order > 2 ||
       (t->mixed_pphh_fp32 != 0 && t->mixed_pphh_fp32 != 1) || (t->mixed_ph_fp32 != 0 && t->mixed_ph_fp32 != 1) ||
-      (t->ph_tile_128 != 0 && t->ph_tile_128 != 1) || (t->finish_order != 0 && t->finish_order != 1))
+      (t->ph_tile_128 != 0 && t->ph_tile_128 != 1) || (t->finish_order != 0 && t->finish_order != 1) ||
+      (t->overlap_pphh != 0 && t->overlap_pphh != 1))
     return fail(ctx, IMSRG_E_INVALID, "invalid tuning values");
   Tuning n;
   n.pphh_small = t->pphh_small_tile_max;
@@ -979,6 +1012,7 @@ extern "C" int imsrg_hip_set_tuning(imsrg_ctx *ctx, const imsrg_tuning *t) {
   n.mixed_ph = t->mixed_ph_fp32;
   n.ph_big = t->ph_tile_128;
   n.finish_order = t->finish_order;
+  n.overlap_pphh = t->overlap_pphh;
   return apply_tuning(ctx, n);
 }
 
@@ -1019,9 +1053,10 @@ extern "C" int imsrg_hip_autotune(imsrg_ctx *ctx, int precision, const char *cac
       while (std::fgets(line, sizeof(line), f)) {
         if (std::strncmp(line, key.c_str(), key.size()) != 0 || line[key.size()] != ' ') continue;
         if (std::sscanf(line + key.size(),
-                        " pphh_small=%d pphh_order=%d mixed_pphh=%d mixed_ph=%d ph_big=%d finish_order=%d us=%lf",
+                        " pphh_small=%d pphh_order=%d mixed_pphh=%d mixed_ph=%d ph_big=%d finish_order=%d"
+                        " overlap_pphh=%d us=%lf",
                         &t.pphh_small, &t.pphh_order, &t.mixed_pphh, &t.mixed_ph, &t.ph_big, &t.finish_order,
-                        &us) == 7)
+                        &t.overlap_pphh, &us) == 8)
           found = true;  // the last matching line wins
       }
       std::fclose(f);
@@ -1092,10 +1127,16 @@ extern "C" int imsrg_hip_autotune(imsrg_ctx *ctx, int precision, const char *cac
     return IMSRG_OK;
   };
   for (size_t c = 0; !rc && c < cand.size(); ++c) rc = trial(cand[c]);
-  // the finishing kernel's tile order is independent of the GEMM choices: one more trial
+  // the last kernel's tile order and the pp/hh overlap are independent of the
+  // GEMM choices: one more trial each
   if (!rc) {
     Tuning t = best;
     t.finish_order = 1 - best.finish_order;
+    rc = trial(t);
+  }
+  if (!rc) {
+    Tuning t = best;
+    t.overlap_pphh = 1 - best.overlap_pphh;
     rc = trial(t);
   }
   if (e0) (void)hipEventDestroy(e0);
@@ -1108,9 +1149,11 @@ extern "C" int imsrg_hip_autotune(imsrg_ctx *ctx, int precision, const char *cac
   if ((rc = apply_tuning(ctx, best))) return rc;
   if (cache_path) {
     if (FILE *f = std::fopen(cache_path, "a")) {
-      std::fprintf(f, "%s pphh_small=%d pphh_order=%d mixed_pphh=%d mixed_ph=%d ph_big=%d finish_order=%d us=%.3f\n",
+      std::fprintf(f,
+                   "%s pphh_small=%d pphh_order=%d mixed_pphh=%d mixed_ph=%d ph_big=%d finish_order=%d"
+                   " overlap_pphh=%d us=%.3f\n",
                    key.c_str(), best.pphh_small, best.pphh_order, best.mixed_pphh, best.mixed_ph, best.ph_big,
-                   best.finish_order, best_us);
+                   best.finish_order, best.overlap_pphh, best_us);
       std::fclose(f);
     }
   }

@@ -265,6 +265,7 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
     }
   }
   ch_kstart[nch] = (int)ch_kets.size();
+  B.ndiag = ch_kstart[nch];
   hh_start[nch] = (int)hh_loc.size(); k_start[nch] = (int)k_loc.size();
   kpp_start[nch] = (int)kpp_loc.size(); khh_start[nch] = (int)khh_loc.size();
   gcore_start[nch] = (int)gcore_loc.size(); gnc_start[nch] = (int)gnc_loc.size();

@@ -90,6 +90,7 @@ struct BasisDev {
   // (the first: comm111 + comm121, the others: comm221 items); cpart: orbits with 1 - n_c != 0
   const IMSRG_G int *ob_start, *cpart;
   int nobpart, ncpart;
+  int ndiag;  // kets summed over all channels (ch_kstart[nch])
   const IMSRG_G double *w1norm; // norb*norb one-body norm weights
   const IMSRG_G double *sixj;   // dense 6j table, layout of SixJCache_112112
   const IMSRG_G double *sixjQ;  // same values, layout [jj1][jj2][J6][J3/2][jj4][jj5][J3%2] (sixjQ_index):
@@ -109,6 +110,7 @@ struct PairPtrs {
   IMSRG_G double *scratch;        // >= nch + 8 doubles (zero-body partial sums)
   IMSRG_G double2 *c122;          // per-call comm122 coefficients (X1(a,o) f, Y1(a,o) f), n122 entries
   IMSRG_G double *obpart;         // per-call one-body partial sums, nobpart entries
+  IMSRG_G double *diag;           // per-call inverse-Pandya diagonal terms, one per ket of every channel (ndiag)
   int hX, hY, hZ;
   unsigned mask;
 };

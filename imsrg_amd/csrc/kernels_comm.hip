@@ -893,7 +893,7 @@ __device__ __forceinline__ int wave_max(int v) {
 // diagonal is added in place.
 __host__ __device__ inline size_t invp_lds_bytes(int jmax, int norb, int nch) {
   const size_t nJ = (size_t)jmax + 1;
-  return 8 * (size_t)nch + 8 * nJ + 4 * (size_t)nch + 4 * nJ * (2 * (size_t)norb + 2);
+  return 8 * (size_t)nch + 8 * nJ + 4 * (size_t)nch + 4 * nJ * (2 * (size_t)norb + 3);
 }
 
 __global__ __launch_bounds__(256, 5) void k_invpandya(PairArg pa, const int4 *items) {
@@ -909,7 +909,8 @@ __global__ __launch_bounds__(256, 5) void k_invpandya(PairArg pa, const int4 *it
   int *s_ccn = (int *)(s_choff + nJ);  // [nch] CC channel sizes
   int *s_chn = s_ccn + nch;            // [nJ]  sizes of the standard channels 6J + g
   int *s_ib = s_chn + nJ;              // [nJ]  local index of the bra in channel 6J + g
-  int *Ti = s_ib + nJ;                 // [nJ * norb] CC local index of ket (i,x) in channel J'
+  int *s_chks = s_ib + nJ;             // [nJ]  first ket (all-channel index) of channel 6J + g
+  int *Ti = s_chks + nJ;               // [nJ * norb] CC local index of ket (i,x) in channel J'
   int *Tj = Ti + nJ * norb;            // [nJ * norb] CC local index of ket (j,x) in channel J'
   const int kb = B.grp_kets[gs + b];
   const int i = B.kp[kb], j = B.kq[kb];
@@ -921,6 +922,7 @@ __global__ __launch_bounds__(256, 5) void k_invpandya(PairArg pa, const int4 *it
     s_choff[J] = B.ch_off[6 * J + g];
     s_chn[J] = B.ch_n[6 * J + g];
     s_ib[J] = B.kmapT[(long)J * nk + kb];
+    s_chks[J] = B.ch_kstart[6 * J + g];
   }
   for (int e = threadIdx.x; e < nJ * norb; e += blockDim.x) {
     const int Jp = e / norb, x = e - Jp * norb;
@@ -1043,11 +1045,10 @@ __global__ __launch_bounds__(256, 5) void k_invpandya(PairArg pa, const int4 *it
         const double z = acc[t] * f / nrm;
         const int n = s_chn[J];
         const int ib = s_ib[J], ik = kmapT[(long)J * nk + kk];
-        double *Z2 = Ztwo + s_choff[J];
-        if (ik == ib)
-          Z2[(long)ib * n + ib] += z;
+        if (ik == ib)  // diagonal: kept apart (the pp/hh GEMM may still be writing Z's diagonal)
+          P.diag[s_chks[J] + max(ib, 0)] = z;
         else
-          Z2[(long)ib * n + ik] = z;
+          Ztwo[s_choff[J] + (long)ib * n + ik] = z;
       }
     }
   }
@@ -1119,7 +1120,8 @@ __device__ __forceinline__ double c122_w(const double *X2, const double *Y2, con
 }
 
 // Last kernel of a commutator, over the tile pairs of every two-body block:
-//   FOLD: upper triangle += the transposed-slot contributions of k_invpandya;
+//   FOLD: upper triangle += the transposed-slot contributions of k_invpandya,
+//         diagonal += its diagonal terms (kept in P.diag);
 //   C122: comm122ss (src/Commutator.cc:837-932), Z2 -= W + hZ W^T, with W formed
 //         here from full columns of X2 / Y2 (the partner coefficients of this
 //         call from k_c122_coef);
@@ -1141,6 +1143,7 @@ __global__ __launch_bounds__(256, 4) void k_finish(PairArg pa, const int4 *items
   double *Z2 = P.Z + B.two_off + off;
   const double *X2 = P.X + B.two_off + off, *Y2 = P.Y + B.two_off + off;
   const IMSRG_G int *cst = B.c122_start + B.ch_kstart[ch];
+  const IMSRG_G double *dg = P.diag + B.ch_kstart[ch];  // inverse-Pandya diagonal terms of this channel
   const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const double hZ = P.hZ;
   constexpr int kPer = kTile * kTile / 256;
@@ -1168,7 +1171,7 @@ __global__ __launch_bounds__(256, 4) void k_finish(PairArg pa, const int4 *items
   for (int i = 0; i < kPer; ++i) {
     const int cc = wv + 4 * i, r = R0 + lane, c = S0 + cc;
     if (r < n && c < n && r <= c) {
-      const double v = (r == c && P.hZ == -1) ? 0.0 : u[i] + t[lane][cc];
+      const double v = (r != c) ? u[i] + t[lane][cc] : (P.hZ == -1 ? 0.0 : u[i] + t[lane][cc] + (FOLD ? dg[r] : 0.0));
       blk_at(Z2, (unsigned)(c * n + r)) = v;
       u[i] = v;
     }
