@@ -152,20 +152,20 @@ def workload_emax12(args, ia, np):
     res = {}
     hosts = {}
     for name, prec in (("fp64", ia.PREC_FP64), ("mixed", ia.PREC_MIXED)):
-        ms.set_profiling(True)
         ia.Commutator.CommutatorScalarScalar(X, Y, precision=prec, out=Z)
         ms.synchronize()
         steps = max(3, min(args.steps, 5))
-        phase = np.zeros(8)
+        ms.set_profiling(True)
+        ms.phase_times()
         t0 = time.perf_counter()
         for _ in range(steps):
             ia.Commutator.CommutatorScalarScalar(X, Y, precision=prec, out=Z)
-            phase += np.array(ms.phase_times())
         ms.synchronize()
         t = time.perf_counter() - t0
+        phase = np.array(ms.phase_times())
         ms.set_profiling(False)
         hosts[name] = Z.to_host()
-        res[name] = {"commutators_per_s": steps / t, "phase_us": dict(zip(PHASES, [round(x, 1) for x in phase / steps]))}
+        res[name] = {"commutators_per_s": steps / t, "phase_us": dict(zip(PHASES, [round(x, 1) for x in phase]))}
     rel, de0 = _errors(hosts["mixed"], hosts["fp64"], np)
     d = np.abs(hosts["mixed"].TwoBody - hosts["fp64"].TwoBody)
     return {"metric": "mixed-precision error at emax=12 vs the FP64 GPU result", "value": rel,
@@ -295,21 +295,23 @@ def main():
     ms.synchronize()
 
     def timed(precision):
-        """warm-up, then the timed steps with per-phase HIP events; returns the
+        """warm-up, then the timed steps; the library records HIP events around
+        every phase of the timed steps on the stream that phase runs on, with no
+        host synchronisation per step, and averages them afterwards; returns the
         rate, the per-phase averages (us) and the result on the host"""
-        ms.set_profiling(True)
         for _ in range(args.warmup):
             ia.Commutator.CommutatorScalarScalar(X, Y, precision=precision, out=Z)
         ms.synchronize()
-        phase_sum = np.zeros(8)
+        ms.set_profiling(True)
+        ms.phase_times()  # nothing pending from earlier calls
         t0 = time.perf_counter()
         for _ in range(args.steps):
             ia.Commutator.CommutatorScalarScalar(X, Y, precision=precision, out=Z)
-            phase_sum += np.array(ms.phase_times())
         ms.synchronize()
         t = time.perf_counter() - t0
+        phases = np.array(ms.phase_times())
         ms.set_profiling(False)
-        return args.steps / t, t, phase_sum / args.steps, Z.to_host()
+        return args.steps / t, t, phases, Z.to_host()
 
     # launch settings chosen by the autotuner on this basis, per precision (set-up, untimed)
     tuning = {"fp64": ms.autotune(ia.PREC_FP64, args.autotune_cache)}
@@ -338,13 +340,31 @@ def main():
     t_run = t_mixed if use_mixed else t_fp64
     phase_avg = phasemx if use_mixed else phase64
 
+    # the same phases run one at a time (pp/hh GEMM not overlapped): each
+    # kernel's own duration without a concurrent kernel on the GPU
+    def isolated(precision, tun):
+        ms.set_tuning(**{**{k: v for k, v in tun.items() if k != "us_per_commutator"}, "overlap_pphh": 0})
+        ia.Commutator.CommutatorScalarScalar(X, Y, precision=precision, out=Z)
+        ms.synchronize()
+        ms.set_profiling(True)
+        ms.phase_times()
+        for _ in range(max(3, min(args.steps, 5))):
+            ia.Commutator.CommutatorScalarScalar(X, Y, precision=precision, out=Z)
+        ph = np.array(ms.phase_times())
+        ms.set_profiling(False)
+        ms.set_tuning(**{k: v for k, v in tun.items() if k != "us_per_commutator"})
+        return ph
+
+    tun = tuning["mixed" if use_mixed else "fp64"]
+    phase_iso = isolated(ia.PREC_MIXED if use_mixed else ia.PREC_FP64, tun)
+
     # ---- roofline of the dominant kernel of the reported run ----
-    # (phase durations are HIP events recorded on the library's stream around
-    # each launch during the timed steps)
+    # (phase durations are HIP events recorded around each launch during the
+    # timed steps, on the stream the kernel runs on; with the pp/hh GEMM
+    # overlapping other phases, durations include the sharing of the GPU)
     FA, execA, FC, BI = algorithmic_counts(ms)
     dom = int(np.argmax(phase_avg))
     t_dom = phase_avg[dom] * 1e-6
-    tun = tuning["mixed" if use_mixed else "fp64"]
     if PHASES[dom] == "pphh_gemm":
         # the kernel that ran: FP32 MFMA only when mixed precision and the tuning chose it
         fp32 = use_mixed and tun["mixed_pphh_fp32"] == 1
@@ -365,6 +385,11 @@ def main():
         roof = {"kernel": PHASES[dom], "bound": "hbm", "achieved": None, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                 "frac": None, "traffic": None}
     roof["launch_us"] = phase_avg[dom]
+    if roof["achieved"] is not None:  # the same kernel run alone (no overlapping kernel)
+        t_iso = phase_iso[dom] * 1e-6
+        work = FA if PHASES[dom] == "pphh_gemm" else BI
+        ach_iso = work / t_iso / (1e12 if roof["unit"] == "TFLOP/s" else 1e9)
+        roof["isolated"] = {"launch_us": phase_iso[dom], "achieved": ach_iso, "frac": ach_iso / roof["peak"]}
 
     line = {
         "metric": "[X,Y] commutators/sec at emax=10: mixed vs FP64 GPU, at |dE0|<1e-6",
@@ -390,6 +415,7 @@ def main():
         "tuning": tuning,
         "phase_us": dict(zip(PHASES, [round(x, 2) for x in phase_avg])),
         "phase_us_fp64": dict(zip(PHASES, [round(x, 2) for x in phase64])),
+        "phase_us_isolated": dict(zip(PHASES, [round(x, 2) for x in phase_iso])),
         "roofline": roof,
     }
 

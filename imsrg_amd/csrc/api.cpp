@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -75,8 +76,15 @@ struct imsrg_ctx {
   int npart = 0;
   std::string err;
   bool profiling = false;
-  hipEvent_t ev[9] = {};
-  double phase_us[8] = {0};
+  // phase events of profiled commutators: a ring of sets recorded without host
+  // synchronisation, read (and summed) by imsrg_hip_phase_times; marks 0..8 on
+  // the main stream, 9..10 around the pp/hh GEMM on the side stream
+  static constexpr int kRing = 32, kMarks = 11;
+  std::vector<std::array<hipEvent_t, kMarks>> ring;
+  int ring_head = 0, ring_count = 0;
+  bool ring_side[kRing] = {};  // set recorded with the pp/hh GEMM on the side stream
+  double phase_sum[8] = {0};
+  int phase_n = 0;
   imsrg_dop *stage[3] = {nullptr, nullptr, nullptr};
   imsrg_dop *tmp[4] = {nullptr, nullptr, nullptr, nullptr};
   double *pinned = nullptr;  // host staging for views
@@ -132,7 +140,9 @@ static int ctx_init(imsrg_ctx *ctx, const imsrg_basis_desc *b, int device) {
   HIPCHK(ctx, hipMalloc(&ctx->partials, sizeof(double) * ctx->npart));
   HIPCHK(ctx, hipMalloc(&ctx->norm_dev, sizeof(double) * 2));
   HIPCHK(ctx, hipHostMalloc(&ctx->norm_host, sizeof(double) * 2, hipHostMallocDefault));
-  for (auto &ev : ctx->ev) HIPCHK(ctx, hipEventCreate(&ev));
+  ctx->ring.resize(imsrg_ctx::kRing);
+  for (auto &set : ctx->ring)
+    for (auto &ev : set) HIPCHK(ctx, hipEventCreate(&ev));
   return IMSRG_OK;
 }
 
@@ -187,7 +197,8 @@ extern "C" int imsrg_hip_ctx_destroy(imsrg_ctx *ctx) {
   if (ctx->norm_dev) (void)hipFree(ctx->norm_dev);
   if (ctx->norm_host) (void)hipHostFree(ctx->norm_host);
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
-  for (auto &ev : ctx->ev) if (ev) (void)hipEventDestroy(ev);
+  for (auto &set : ctx->ring)
+    for (auto &ev : set) if (ev) (void)hipEventDestroy(ev);
   for (auto &g : ctx->graphs) (void)hipGraphExecDestroy(g.exec);
   ctx->graphs.clear();
   ctx->db.release();
@@ -454,6 +465,27 @@ static int herm_rule(int hx, int hy) { return (hx == hy) ? -1 : 1; }  // src/Com
 // [5] one-body pairs, [6] tile pairs of the finishing kernel,
 // [7] pp/hh 64-tiles.  Under mixed precision the context's tuning picks the
 // FP32 or FP64 kernel of each GEMM family.
+// Sums the phase durations of the oldest recorded event set (waits for it).
+static int harvest_oldest(imsrg_ctx *ctx) {
+  auto &set = ctx->ring[ctx->ring_head];
+  HIPCHK(ctx, hipEventSynchronize(set[8]));
+  // sequence order of the phases: zero-body, one-body, pp/hh, 122 coefficients,
+  // Pandya, ph GEMM, inverse Pandya, last kernel; reported in the header's order
+  const int map[8] = {0, 1, 3, 2, 4, 5, 6, 7};
+  for (int i = 0; i < 8; ++i) {
+    float ms = 0.f;
+    if (i == 2 && ctx->ring_side[ctx->ring_head])
+      HIPCHK(ctx, hipEventElapsedTime(&ms, set[9], set[10]));
+    else
+      HIPCHK(ctx, hipEventElapsedTime(&ms, set[i], set[i + 1]));
+    ctx->phase_sum[map[i]] += ms * 1000.0;
+  }
+  ++ctx->phase_n;
+  ctx->ring_head = (ctx->ring_head + 1) % imsrg_ctx::kRing;
+  --ctx->ring_count;
+  return IMSRG_OK;
+}
+
 static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const int4 *const *items, const int *cnt,
                           unsigned mask, int prec, int maxn, int maxnch, long maxn122, int maxjmax, int maxnorb,
                           int maxn1pairs) {
@@ -461,18 +493,28 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
   const int prec_pphh = (prec == IMSRG_PREC_MIXED && tu.mixed_pphh) ? IMSRG_PREC_MIXED : IMSRG_PREC_FP64;
   const int prec_ph = (prec == IMSRG_PREC_MIXED && tu.mixed_ph) ? IMSRG_PREC_MIXED : IMSRG_PREC_FP64;
   hipStream_t s = ctx->stream;
-  bool prof = ctx->profiling;
-  auto mark = [&](int i) -> hipError_t { return prof ? hipEventRecord(ctx->ev[i], s) : hipSuccess; };
-  // With overlap (never while profiling), the pp/hh GEMM runs on the side
+  // With overlap, the pp/hh GEMM runs on the side
   // stream next to the zero-body, one-body, comm122-coefficient, Pandya, ph
   // GEMM and inverse Pandya kernels (none of them writes the upper triangle
   // of Z's two-body blocks); the main stream waits for it before the last
   // kernel.
-  const bool overlap = !prof && tu.overlap_pphh && (mask & MPPHH);
+  const bool overlap = tu.overlap_pphh && (mask & MPPHH);
+  // profiling: this commutator's phase events go to the next free set of the ring
+  std::array<hipEvent_t, imsrg_ctx::kMarks> *set = nullptr;
+  if (ctx->profiling) {
+    if (ctx->ring_count == imsrg_ctx::kRing)
+      if (int rc = harvest_oldest(ctx)) return rc;
+    const int slot = (ctx->ring_head + ctx->ring_count) % imsrg_ctx::kRing;
+    set = &ctx->ring[slot];
+    ctx->ring_side[slot] = overlap;
+  }
+  auto mark = [&](int i) -> hipError_t { return set ? hipEventRecord((*set)[i], s) : hipSuccess; };
   if (overlap) {
     HIPCHK(ctx, hipEventRecord(ctx->fork, s));
     HIPCHK(ctx, hipStreamWaitEvent(ctx->side, ctx->fork, 0));
+    if (set) HIPCHK(ctx, hipEventRecord((*set)[9], ctx->side));
     HIPCHK(ctx, launch_pphh(pa, items[0], cnt[0], items[7], cnt[7], prec_pphh, maxn, ctx->side));
+    if (set) HIPCHK(ctx, hipEventRecord((*set)[10], ctx->side));
     HIPCHK(ctx, hipEventRecord(ctx->join, ctx->side));
   }
   HIPCHK(ctx, mark(0));
@@ -499,16 +541,7 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
   HIPCHK(ctx, mark(7));
   HIPCHK(ctx, launch_finish(pa, items[6], cnt[6], (mask & MPH) != 0, (mask & M122) != 0, s));
   HIPCHK(ctx, mark(8));
-  if (prof) {
-    HIPCHK(ctx, hipEventSynchronize(ctx->ev[8]));
-    // order of phases in the sequence: zero-body, one-body, pp/hh, 122, Pandya, ph GEMM, inverse, mirror
-    const int map[8] = {0, 1, 3, 2, 4, 5, 6, 7};
-    for (int i = 0; i < 8; ++i) {
-      float ms = 0.f;
-      HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[i], ctx->ev[i + 1]));
-      ctx->phase_us[map[i]] = ms * 1000.0;
-    }
-  }
+  if (set) ++ctx->ring_count;
   return IMSRG_OK;
 }
 
@@ -1168,7 +1201,11 @@ extern "C" int imsrg_hip_set_profiling(imsrg_ctx *ctx, int on) {
 }
 extern "C" int imsrg_hip_phase_times(imsrg_ctx *ctx, double *us, int n) {
   if (!ctx || !us) return IMSRG_E_INVALID;
-  for (int i = 0; i < n && i < 8; ++i) us[i] = ctx->phase_us[i];
+  while (ctx->ring_count > 0)
+    if (int rc = harvest_oldest(ctx)) return rc;
+  for (int i = 0; i < n && i < 8; ++i) us[i] = ctx->phase_n ? ctx->phase_sum[i] / ctx->phase_n : 0.0;
+  for (double &v : ctx->phase_sum) v = 0.0;
+  ctx->phase_n = 0;
   return IMSRG_OK;
 }
 
