@@ -27,7 +27,7 @@ PEAK_FP64_MFMA = 78.6e12   # FLOP/s, MI355X dense FP64 matrix
 PEAK_FP32_MFMA = 157.3e12  # FLOP/s, MI355X dense FP32 matrix
 PEAK_HBM = 8.0e12          # B/s
 
-PHASES = ["zerobody", "onebody", "comm122_coef", "pphh_gemm", "pandya_gather", "ph_gemm", "inverse_pandya", "comm122_fold_mirror"]
+PHASES = ["zerobody", "onebody", "comm122", "pphh_gemm", "pandya_gather", "ph_gemm", "inverse_pandya", "fold_mirror"]
 
 
 def algorithmic_counts(ms):

@@ -98,9 +98,9 @@ class ModelSpace:
         check(lib().imsrg_hip_set_profiling(self.h, 1 if on else 0), self.h)
 
     def phase_times(self):
-        """Per-phase microseconds (zero-body, one-body, comm122 coefficients,
-        pp/hh GEMM, Pandya gather, ph GEMM, inverse Pandya, last kernel) averaged
-        over the commutators profiled since the previous call."""
+        """Per-phase microseconds (zero-body, one-body, comm122, pp/hh GEMM,
+        Pandya gather, ph GEMM, inverse Pandya, last kernel) averaged over the
+        commutators profiled since the previous call."""
         a = (C.c_double * 8)()
         check(lib().imsrg_hip_phase_times(self.h, a, 8), self.h)
         return list(a)

@@ -78,8 +78,8 @@ struct imsrg_ctx {
   bool profiling = false;
   // phase events of profiled commutators: a ring of sets recorded without host
   // synchronisation, read (and summed) by imsrg_hip_phase_times; marks 0..8 on
-  // the main stream, 9..10 around the pp/hh GEMM on the side stream
-  static constexpr int kRing = 32, kMarks = 11;
+  // the main stream, 9, 10, 11 around the pp/hh GEMM and comm122 on the side stream
+  static constexpr int kRing = 32, kMarks = 12;
   std::vector<std::array<hipEvent_t, kMarks>> ring;
   int ring_head = 0, ring_count = 0;
   bool ring_side[kRing] = {};  // set recorded with the pp/hh GEMM on the side stream
@@ -476,6 +476,8 @@ static int harvest_oldest(imsrg_ctx *ctx) {
     float ms = 0.f;
     if (i == 2 && ctx->ring_side[ctx->ring_head])
       HIPCHK(ctx, hipEventElapsedTime(&ms, set[9], set[10]));
+    else if (i == 3 && ctx->ring_side[ctx->ring_head])  // comm122 on the side stream
+      HIPCHK(ctx, hipEventElapsedTime(&ms, set[10], set[11]));
     else
       HIPCHK(ctx, hipEventElapsedTime(&ms, set[i], set[i + 1]));
     ctx->phase_sum[map[i]] += ms * 1000.0;
@@ -493,11 +495,11 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
   const int prec_pphh = (prec == IMSRG_PREC_MIXED && tu.mixed_pphh) ? IMSRG_PREC_MIXED : IMSRG_PREC_FP64;
   const int prec_ph = (prec == IMSRG_PREC_MIXED && tu.mixed_ph) ? IMSRG_PREC_MIXED : IMSRG_PREC_FP64;
   hipStream_t s = ctx->stream;
-  // With overlap, the pp/hh GEMM runs on the side
-  // stream next to the zero-body, one-body, comm122-coefficient, Pandya, ph
-  // GEMM and inverse Pandya kernels (none of them writes the upper triangle
-  // of Z's two-body blocks); the main stream waits for it before the last
-  // kernel.
+  // With overlap, the pp/hh GEMM and then comm122 (coefficients and an
+  // upper-triangle pass) run on the side stream next to the zero-body,
+  // one-body, Pandya, ph GEMM and inverse Pandya kernels (none of them touches
+  // the upper triangle of Z's two-body blocks); the main stream waits for them
+  // before the last kernel, which then only folds and mirrors.
   const bool overlap = tu.overlap_pphh && (mask & MPPHH);
   // profiling: this commutator's phase events go to the next free set of the ring
   std::array<hipEvent_t, imsrg_ctx::kMarks> *set = nullptr;
@@ -515,6 +517,11 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
     if (set) HIPCHK(ctx, hipEventRecord((*set)[9], ctx->side));
     HIPCHK(ctx, launch_pphh(pa, items[0], cnt[0], items[7], cnt[7], prec_pphh, maxn, ctx->side));
     if (set) HIPCHK(ctx, hipEventRecord((*set)[10], ctx->side));
+    if (mask & M122) {
+      HIPCHK(ctx, launch_c122_coef(pa, npairs, maxn122, ctx->side));
+      HIPCHK(ctx, launch_c122_upper(pa, items[6], cnt[6], ctx->side));
+    }
+    if (set) HIPCHK(ctx, hipEventRecord((*set)[11], ctx->side));
     HIPCHK(ctx, hipEventRecord(ctx->join, ctx->side));
   }
   HIPCHK(ctx, mark(0));
@@ -530,7 +537,7 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
     HIPCHK(ctx, launch_zero_two(pa, items[6], cnt[6], s));
   }
   HIPCHK(ctx, mark(3));
-  if (mask & M122) HIPCHK(ctx, launch_c122_coef(pa, npairs, maxn122, s));
+  if ((mask & M122) && !overlap) HIPCHK(ctx, launch_c122_coef(pa, npairs, maxn122, s));
   HIPCHK(ctx, mark(4));
   if (mask & MPH) HIPCHK(ctx, launch_pandya(pa, items[3], cnt[3], maxnch, s));
   HIPCHK(ctx, mark(5));
@@ -539,7 +546,7 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
   if (mask & MPH) HIPCHK(ctx, launch_invpandya(pa, items[4], cnt[4], maxjmax, maxnorb, maxnch, s));
   if (overlap) HIPCHK(ctx, hipStreamWaitEvent(s, ctx->join, 0));
   HIPCHK(ctx, mark(7));
-  HIPCHK(ctx, launch_finish(pa, items[6], cnt[6], (mask & MPH) != 0, (mask & M122) != 0, s));
+  HIPCHK(ctx, launch_finish(pa, items[6], cnt[6], (mask & MPH) != 0, (mask & M122) != 0 && !overlap, s));
   HIPCHK(ctx, mark(8));
   if (set) ++ctx->ring_count;
   return IMSRG_OK;

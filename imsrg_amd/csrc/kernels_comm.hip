@@ -1125,12 +1125,14 @@ __device__ __forceinline__ double c122_w(const double *X2, const double *Y2, con
 //   C122: comm122ss (src/Commutator.cc:837-932), Z2 -= W + hZ W^T, with W formed
 //         here from full columns of X2 / Y2 (the partner coefficients of this
 //         call from k_c122_coef);
-// then lower triangle <- hZ * upper triangle, anti-Hermitian diagonal <- 0.
+// then (MIRROR) lower triangle <- hZ * upper triangle; anti-Hermitian diagonal <- 0.
+// Without MIRROR only the upper triangle is read and written, so that pass can
+// run while other kernels write the transposed slots.
 // Tile pair: U = rows R0.., cols S0.. (R0 <= S0) and its transposed position.
 // U stays in registers: one column per wave, lane = row.  The contributions
 // held at the transposed position go through LDS: one column R0 + rr of Z2
 // per wave, lane = row S0 + cc.
-template <bool FOLD, bool C122>
+template <bool FOLD, bool C122, bool MIRROR = true>
 __global__ __launch_bounds__(256, 4) void k_finish(PairArg pa, const int4 *items) {
   __shared__ double t[kTile][kTile + 1];  // t[rr][cc]: contribution to upper (R0 + rr, S0 + cc)
   const int4 it = items[blockIdx.x];
@@ -1176,6 +1178,7 @@ __global__ __launch_bounds__(256, 4) void k_finish(PairArg pa, const int4 *items
       u[i] = v;
     }
   }
+  if constexpr (!MIRROR) return;
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < kPer; ++i) t[lane][wv + 4 * i] = u[i];  // new upper values to the threads writing their mirror
@@ -1288,6 +1291,11 @@ hipError_t launch_finish(const PairArg &pa, const int4 *items, int nitems, bool 
     hipLaunchKernelGGL((k_finish<false, true>), dim3(nitems), dim3(256), 0, s, pa, items);
   else
     hipLaunchKernelGGL(k_mirror, dim3(nitems), dim3(256), 0, s, pa, items);
+  return hipGetLastError();
+}
+hipError_t launch_c122_upper(const PairArg &pa, const int4 *items, int nitems, hipStream_t s) {
+  if (nitems <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_finish<false, true, false>), dim3(nitems), dim3(256), 0, s, pa, items);
   return hipGetLastError();
 }
 hipError_t launch_zero_two(const PairArg &pa, const int4 *items, int nitems, hipStream_t s) {

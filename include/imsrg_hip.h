@@ -203,9 +203,10 @@ int imsrg_hip_set_tuning(imsrg_ctx *ctx, const imsrg_tuning *t);
 int imsrg_hip_autotune(imsrg_ctx *ctx, int precision, const char *cache_path, imsrg_tuning *out);
 
 /* Per-phase timing of commutators on ctx (microseconds, HIP events):
- * [0] zero-body, [1] one-body, [2] comm122 coefficients, [3] pp/hh GEMM,
- * [4] Pandya gather, [5] ph GEMM, [6] inverse Pandya, [7] last kernel (fold of
- * the inverse-Pandya terms, comm122, mirror).  Enabled by imsrg_hip_set_profiling:
+ * [0] zero-body, [1] one-body, [2] comm122 (coefficients, and with the pp/hh
+ * overlap its upper-triangle pass), [3] pp/hh GEMM, [4] Pandya gather, [5] ph
+ * GEMM, [6] inverse Pandya, [7] last kernel (fold of the inverse-Pandya terms,
+ * comm122 when not overlapped, mirror).  Enabled by imsrg_hip_set_profiling:
  * each commutator records HIP events on the stream its phases run on (the
  * pp/hh GEMM's own stream when it overlaps the others) without a host
  * synchronisation; imsrg_hip_phase_times returns the average over the

@@ -56,6 +56,51 @@ def test_every_setting_matches_oracle(ia):
         ms.set_tuning(**{k: v for k, v in default.items() if k != "us_per_commutator"})
 
 
+def test_overlap_matches_serial(ia):
+    """With overlap_pphh the pp/hh GEMM and comm122's upper-triangle pass run on
+    a second stream while other kernels write disjoint outputs; the result must
+    equal the serial launch order to rounding (comm122 is then added in its own
+    pass, so the sums differ in order), and repeat exactly from call to call."""
+    ms = ia.ModelSpace(6, "O16")
+    X = ia.DeviceOperator(ms, -1).fill_random(-1, 331)
+    Y = ia.DeviceOperator(ms, 1).fill_random(1, 332)
+    default = ms.get_tuning()
+    try:
+        for prec in (ia.PREC_FP64, ia.PREC_MIXED):
+            say(f"[gpu] overlap on/off emax=6 precision={prec}")
+            ms.set_tuning(overlap_pphh=0)
+            Zs = ia.Commutator.CommutatorScalarScalar(X, Y, precision=prec).to_host()
+            ms.set_tuning(overlap_pphh=1)
+            first = None
+            for rep in range(3):
+                Zo = ia.Commutator.CommutatorScalarScalar(X, Y, precision=prec).to_host()
+                assert Zo.ZeroBody == Zs.ZeroBody and np.array_equal(Zo.OneBody, Zs.OneBody)
+                scale = np.max(np.abs(Zs.TwoBody))
+                assert np.max(np.abs(Zo.TwoBody - Zs.TwoBody)) < 1e-13 * scale, rep
+                if first is None:
+                    first = Zo.TwoBody.copy()
+                assert np.array_equal(Zo.TwoBody, first), rep
+    finally:
+        ms.set_tuning(**{k: v for k, v in default.items() if k != "us_per_commutator"})
+
+
+def test_phase_times_average_and_reset(ia):
+    ms = ia.ModelSpace(4, "O16")
+    X = ia.DeviceOperator(ms, -1).fill_random(-1, 341)
+    Y = ia.DeviceOperator(ms, 1).fill_random(1, 342)
+    say("[gpu] phase timing ring")
+    ms.set_profiling(True)
+    try:
+        ms.phase_times()
+        for _ in range(40):  # more than the ring holds: older sets are summed on the way
+            ia.Commutator.CommutatorScalarScalar(X, Y)
+        t = ms.phase_times()
+        assert len(t) == 8 and all(v >= 0.0 for v in t) and t[3] > 0.0 and t[7] > 0.0
+        assert ms.phase_times() == [0.0] * 8  # read resets
+    finally:
+        ms.set_profiling(False)
+
+
 def test_mixed_switches(ia):
     ms = ia.ModelSpace(6, "O16")
     X = ia.DeviceOperator(ms, -1).fill_random(-1, 311)
