@@ -28,7 +28,7 @@ constexpr int kGC = kTileK / kGWaves;  // k columns each wave copies into LDS pe
 constexpr int kChunk = 256;       // elements per work item of the gather kernels
 constexpr int kJC = 6;            // J values per register chunk of the inverse Pandya gather (even)
 constexpr int kC122Blk = 4;       // comm122 partner lists are padded to multiples of this (one load block)
-constexpr int kF32Flush = 4;      // MFMA k-steps (of 4) summed in FP32 before the FP64 flush (mixed precision)
+constexpr int kF32Flush = 4;      // MFMA k-steps (of 4) summed in FP32 before the FP64 flush (mixed, 64x64 tiles)
 
 // Term bits (same values as IMSRG_COMM* in include/imsrg_hip.h)
 constexpr unsigned M110 = 1u << 0, M220 = 1u << 1, M111 = 1u << 2, M121 = 1u << 3, M221 = 1u << 4,

@@ -520,7 +520,7 @@ __global__ __launch_bounds__(256) void k_syr2k(PairArg pa, const int4 *items, in
 // phi_r phi_s times the upper element, as the reference's folds make it
 // (src/Commutator.cc:1792-1805), written through an LDS transpose.
 // MIXED: operands rounded to FP32 when read from LDS, products on f32
-// 16x16x4 MFMA, FP32 partial sums over kF32Flush k-steps added into the FP64
+// 16x16x4 MFMA, FP32 partial sums over two stages (32 k) added into the FP64
 // accumulators.
 template <int MODE, bool MIXED>
 __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
@@ -652,8 +652,25 @@ __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
   __builtin_amdgcn_s_waitcnt(0);  // the copies into LDS have landed
   __syncthreads();
   // one stage: copies of the next stage into (An, Bn, Wn), MFMAs on (Ac, Bc, Wc)
+  // mixed precision: FP32 partial sums of the wave's 4 x kGN blocks, flushed into
+  // the FP64 accumulators every second stage (32 k)
+  flt4 t[4][kGN];
+#pragma unroll
+  for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < kGN; ++ni) t[mi][ni] = flt4{0.f, 0.f, 0.f, 0.f};
+  auto flush_t = [&]() {
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < kGN; ++ni) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[mi][ni][q] += (double)t[mi][ni][q];
+        t[mi][ni] = flt4{0.f, 0.f, 0.f, 0.f};
+      }
+  };
   auto stage = [&](int k0, double (&Ac)[kTileK][LD], double (&Bc)[kTileK][LD], double (&Wc)[kTileK],
-                   double (&An)[kTileK][LD], double (&Bn)[kTileK][LD], double (&Wn)[kTileK]) {
+                   double (&An)[kTileK][LD], double (&Bn)[kTileK][LD], double (&Wn)[kTileK], bool flush) {
     const bool more = k0 + kTileK < K2;
     // loads with register results are unconditional (clamped indices): a
     // branch around them would make the compiler wait for them, and for the
@@ -663,14 +680,6 @@ __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
     load_idx(k0 + 2 * kTileK, ki);
     if (!wave_live) {
     } else if constexpr (MIXED) {
-      // FP32 partial sums of all 4 x kGN blocks over the stage's 16 k, then one
-      // flush into the FP64 accumulators
-      static_assert(kF32Flush == kTileK / 4, "one FP32 flush per stage");
-      flt4 t[4][kGN];
-#pragma unroll
-      for (int mi = 0; mi < 4; ++mi)
-#pragma unroll
-        for (int ni = 0; ni < kGN; ++ni) t[mi][ni] = flt4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int ks = 0; ks < kTileK / 4; ++ks) {
         const int kr = ks * 4 + (lane >> 4);
@@ -686,12 +695,7 @@ __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
           for (int ni = 0; ni < kGN; ++ni)
             t[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x4f32(bm[ni], am[mi], t[mi][ni], 0, 0, 0);
       }
-#pragma unroll
-      for (int mi = 0; mi < 4; ++mi)
-#pragma unroll
-        for (int ni = 0; ni < kGN; ++ni)
-#pragma unroll
-          for (int q = 0; q < 4; ++q) acc[mi][ni][q] += (double)t[mi][ni][q];
+      if (flush) flush_t();
     } else {
 #pragma unroll
       for (int ks = 0; ks < kTileK / 4; ++ks) {
@@ -714,9 +718,10 @@ __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
     __syncthreads();
   };
   for (int k0 = 0; k0 < K2; k0 += 2 * kTileK) {
-    stage(k0, A0, B0, W0, A1, B1, W1);
-    if (k0 + kTileK < K2) stage(k0 + kTileK, A1, B1, W1, A0, B0, W0);
+    stage(k0, A0, B0, W0, A1, B1, W1, false);
+    if (k0 + kTileK < K2) stage(k0 + kTileK, A1, B1, W1, A0, B0, W0, true);
   }
+  if constexpr (MIXED) flush_t();  // a last stage without its pair
   if constexpr (MODE == 0) {
 #pragma unroll
     for (int mi = 0; mi < 4; ++mi)

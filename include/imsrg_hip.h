@@ -51,8 +51,8 @@ extern "C" {
 /* arithmetic of the channel GEMMs */
 #define IMSRG_PREC_FP64 0  /* every product and sum in FP64 */
 #define IMSRG_PREC_MIXED 1 /* pp/hh and ph GEMMs: FP32-rounded operands on FP32 MFMA,
-                              FP32 partial sums over 16 k flushed into FP64
-                              accumulators; everything else FP64 */
+                              FP32 partial sums over 32 k (16 k on 64x64 tiles)
+                              flushed into FP64 accumulators; everything else FP64 */
 
 typedef struct imsrg_ctx imsrg_ctx; /* basis tables, 6j table, stream, workspaces */
 typedef struct imsrg_dop imsrg_dop; /* device-resident operator */
