@@ -501,6 +501,9 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
   // the upper triangle of Z's two-body blocks); the main stream waits for them
   // before the last kernel, which then only folds and mirrors.
   const bool overlap = tu.overlap_pphh && (mask & MPPHH);
+  // overlap_pphh 1: fork before the first kernel; 2: fork after the ph GEMM, so
+  // that the pp/hh GEMM shares the GPU with the inverse Pandya gather only
+  const bool fork_late = overlap && tu.overlap_pphh == 2;
   // profiling: this commutator's phase events go to the next free set of the ring
   std::array<hipEvent_t, imsrg_ctx::kMarks> *set = nullptr;
   if (ctx->profiling) {
@@ -511,7 +514,7 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
     ctx->ring_side[slot] = overlap;
   }
   auto mark = [&](int i) -> hipError_t { return set ? hipEventRecord((*set)[i], s) : hipSuccess; };
-  if (overlap) {
+  auto fork_side = [&]() -> int {
     HIPCHK(ctx, hipEventRecord(ctx->fork, s));
     HIPCHK(ctx, hipStreamWaitEvent(ctx->side, ctx->fork, 0));
     if (set) HIPCHK(ctx, hipEventRecord((*set)[9], ctx->side));
@@ -523,7 +526,10 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
     }
     if (set) HIPCHK(ctx, hipEventRecord((*set)[11], ctx->side));
     HIPCHK(ctx, hipEventRecord(ctx->join, ctx->side));
-  }
+    return IMSRG_OK;
+  };
+  if (overlap && !fork_late)
+    if (int rc = fork_side()) return rc;
   HIPCHK(ctx, mark(0));
   HIPCHK(ctx, launch_zerobody(pa, npairs, maxnch, s));
   HIPCHK(ctx, mark(1));
@@ -542,6 +548,8 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
   if (mask & MPH) HIPCHK(ctx, launch_pandya(pa, items[3], cnt[3], maxnch, s));
   HIPCHK(ctx, mark(5));
   if (mask & MPH) HIPCHK(ctx, launch_ph_gemm(pa, items[1], cnt[1], prec_ph, tu.ph_big != 0, s));
+  if (fork_late)
+    if (int rc = fork_side()) return rc;
   HIPCHK(ctx, mark(6));
   if (mask & MPH) HIPCHK(ctx, launch_invpandya(pa, items[4], cnt[4], maxjmax, maxnorb, maxnch, s));
   if (overlap) HIPCHK(ctx, hipStreamWaitEvent(s, ctx->join, 0));
@@ -1043,7 +1051,7 @@ extern "C" int imsrg_hip_set_tuning(imsrg_ctx *ctx, const imsrg_tuning *t) {
   if (t->pphh_small_tile_max < 0 || t->pphh_order < 0 || t->pphh_order > 2 ||
       (t->mixed_pphh_fp32 != 0 && t->mixed_pphh_fp32 != 1) || (t->mixed_ph_fp32 != 0 && t->mixed_ph_fp32 != 1) ||
       (t->ph_tile_128 != 0 && t->ph_tile_128 != 1) || (t->finish_order != 0 && t->finish_order != 1) ||
-      (t->overlap_pphh != 0 && t->overlap_pphh != 1))
+      t->overlap_pphh < 0 || t->overlap_pphh > 2)
     return fail(ctx, IMSRG_E_INVALID, "invalid tuning values");
   Tuning n;
   n.pphh_small = t->pphh_small_tile_max;
@@ -1174,9 +1182,10 @@ extern "C" int imsrg_hip_autotune(imsrg_ctx *ctx, int precision, const char *cac
     t.finish_order = 1 - best.finish_order;
     rc = trial(t);
   }
-  if (!rc) {
+  for (int ov = 0; ov <= 2 && !rc; ++ov) {
+    if (ov == best.overlap_pphh) continue;
     Tuning t = best;
-    t.overlap_pphh = 1 - best.overlap_pphh;
+    t.overlap_pphh = ov;
     rc = trial(t);
   }
   if (e0) (void)hipEventDestroy(e0);

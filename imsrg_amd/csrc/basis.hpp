@@ -53,7 +53,7 @@ struct Tuning {
   int mixed_ph = 1;    // mixed precision: 1 = FP32-MFMA ph kernel, 0 = the FP64 kernel
   int ph_big = 1;      // ph GEMM on 128x128 tiles (1) or 64x64 tiles (0)
   int finish_order = 1;  // tile pairs of the finishing kernel: 0 channel/row order, 1 XCD-contiguous runs
-  int overlap_pphh = 1;  // pp/hh GEMM on a second stream beside the one-body, Pandya and ph phases
+  int overlap_pphh = 1;  // pp/hh GEMM on a second stream: 0 no, 1 forked first, 2 forked after the ph GEMM
 };
 
 // Work-item lists (int4: pair, a, b, c) for every kernel of one commutator.

@@ -189,8 +189,10 @@ typedef struct {
   int ph_tile_128;          /* 1: ph GEMM on 128x128 tiles, 0: on 64x64 tiles */
   int finish_order;         /* tile pairs of the last kernel (fold, comm122, mirror): 0 in channel and
                                row-block order, 1 cut into 8 runs of consecutive tiles, one per XCD */
-  int overlap_pphh;         /* 1: the pp/hh GEMM runs on a second stream of the context, concurrently with
-                               the one-body, Pandya and ph GEMM kernels (never while profiling) */
+  int overlap_pphh;         /* pp/hh GEMM (then comm122) on a second stream of the context: 0 no (one
+                               stream); 1 forked before the first kernel, beside the one-body, Pandya,
+                               ph GEMM and inverse Pandya kernels; 2 forked after the ph GEMM, beside
+                               the inverse Pandya kernel only */
   double us_per_commutator; /* autotune: measured time of the chosen setting (0 when set by hand) */
 } imsrg_tuning;
 int imsrg_hip_get_tuning(imsrg_ctx *ctx, imsrg_tuning *t);

@@ -70,16 +70,17 @@ def test_overlap_matches_serial(ia):
             say(f"[gpu] overlap on/off emax=6 precision={prec}")
             ms.set_tuning(overlap_pphh=0)
             Zs = ia.Commutator.CommutatorScalarScalar(X, Y, precision=prec).to_host()
-            ms.set_tuning(overlap_pphh=1)
-            first = None
-            for rep in range(3):
+            for ov, rep in ((1, 0), (1, 1), (1, 2), (2, 0)):
+                ms.set_tuning(overlap_pphh=ov)
+                if rep == 0:
+                    first = None
                 Zo = ia.Commutator.CommutatorScalarScalar(X, Y, precision=prec).to_host()
                 assert Zo.ZeroBody == Zs.ZeroBody and np.array_equal(Zo.OneBody, Zs.OneBody)
                 scale = np.max(np.abs(Zs.TwoBody))
-                assert np.max(np.abs(Zo.TwoBody - Zs.TwoBody)) < 1e-13 * scale, rep
+                assert np.max(np.abs(Zo.TwoBody - Zs.TwoBody)) < 1e-13 * scale, (ov, rep)
                 if first is None:
                     first = Zo.TwoBody.copy()
-                assert np.array_equal(Zo.TwoBody, first), rep
+                assert np.array_equal(Zo.TwoBody, first), (ov, rep)
     finally:
         ms.set_tuning(**{k: v for k, v in default.items() if k != "us_per_commutator"})
 
