@@ -157,12 +157,15 @@ __global__ __launch_bounds__(256) void k_c122_coef(PairArg pa, int npairs) {
 __device__ __forceinline__ double fold_dot(const double *xr, const double *xs, const double *yr, const double *ys,
                                            const int *loc, const double *w, int nk, int start, int stride,
                                            double hZ) {
-  double m = 0.0;
+  // the two products summed apart, as the reference's M and its transpose: for
+  // r == s and hZ = -1 they cancel exactly (an anti-Hermitian diagonal is 0)
+  double m1 = 0.0, m2 = 0.0;
   for (int q = start; q < nk; q += stride) {
     const int k = loc[q];
-    m += w[q] * (xr[k] * ys[k] + hZ * (xs[k] * yr[k]));
+    m1 += w[q] * (xr[k] * ys[k]);
+    m2 += w[q] * (xs[k] * yr[k]);
   }
-  return m;
+  return m1 + hZ * m2;
 }
 
 __global__ __launch_bounds__(256) void k_onebody(PairArg pa, const int4 *items) {

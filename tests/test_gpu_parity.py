@@ -127,6 +127,23 @@ def test_each_term_matches_oracle(ia, emax, A, Z):
             assert_close(Z, R, what=f"{name} e{emax} A{A} h{hx}{hy}")
 
 
+@pytest.mark.parametrize("emax,A,Z", [(0, 4, 2), (0, 1, 1), (1, 4, 2), (1, 2, 1), (1, 16, 8)])
+def test_tiny_spaces_match_oracle(ia, emax, A, Z):
+    """Degenerate bases: every orbit a hole (no pp or ph kets at all: He4 at
+    emax 0, O16 at emax 1), channels holding only ph kets (He4 at emax 1), and
+    half-filled orbits (H1, H2: occupations 1/2)."""
+    ms, orc = spaces(ia, emax, A, Z)
+    for hx, hy in ((-1, 1), (1, 1), (-1, -1)):
+        X = random_op(orc, hx, 5000 + 10 * emax + A)
+        Y = random_op(orc, hy, 6000 + 10 * emax + A)
+        dX, dY = to_dev(ia, ms, X), to_dev(ia, ms, Y)
+        for name, bit in list(TERMS.items()) + [("all", 0xFF)]:
+            say(f"[gpu] tiny space term={name} emax={emax} A={A} Z={Z} hx={hx} hy={hy}")
+            Zd = from_dev(ia.Commutator.CommutatorScalarScalar(dX, dY, mask=bit))
+            R = orc.commutator(X, Y, bit)
+            assert_close(Zd, R, what=f"{name} e{emax} A{A} Z{Z} h{hx}{hy}")
+
+
 @pytest.mark.parametrize("emax", [6, 8])
 def test_full_commutator_matches_oracle(ia, emax):
     ms, orc = spaces(ia, emax)
