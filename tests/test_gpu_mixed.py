@@ -1,7 +1,8 @@
 """Mixed-precision commutator (IMSRG_PREC_MIXED) against FP64.
 
 The pp/hh and ph GEMMs run on FP32 MFMA with FP32-rounded operands and FP32
-partial sums over 8 values of k flushed into FP64 accumulators; every other
+partial sums over 32 values of k (16 on 64x64 tiles) flushed into FP64
+accumulators; every other
 term, E0 included, stays FP64.  The accuracy bar is max |dZ| <= 1e-7 times
 max |Z| (one- and two-body parts) and |dE0| < 1e-6; a lower bound on the
 error shows that the FP32 path actually ran.
