@@ -2,8 +2,8 @@
 
 The pp/hh and ph GEMMs run on FP32 MFMA with FP32-rounded operands and FP32
 partial sums over 32 values of k (16 on 64x64 tiles) flushed into FP64
-accumulators; every other
-term, E0 included, stays FP64.  The accuracy bar is max |dZ| <= 1e-7 times
+accumulators; every other term, E0 included, stays FP64.  The accuracy bar
+is max |dZ| <= 1e-7 times
 max |Z| (one- and two-body parts) and |dE0| < 1e-6; a lower bound on the
 error shows that the FP32 path actually ran.
 """
