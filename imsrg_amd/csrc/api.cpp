@@ -32,14 +32,14 @@ struct BatchCache {
   size_t off[8] = {0};
   int cnt[8] = {0};
   double *work = nullptr;
-  int maxn = 0, maxnch = 0, maxjmax = 0, maxnorb = 0, maxn1pairs = 0;
+  int maxn = 0, maxnch = 0, maxjmax = 0, maxnorb = 0, maxn1pairs = 0, maxncls = 0;
   long maxn122 = 0;
   void release() {
     if (d_tab) (void)hipFree(d_tab);
     if (d_items) (void)hipFree(d_items);
     if (work) (void)hipFree(work);
     d_tab = nullptr; d_items = nullptr; work = nullptr;
-    maxn = maxnch = maxjmax = maxnorb = maxn1pairs = 0;
+    maxn = maxnch = maxjmax = maxnorb = maxn1pairs = maxncls = 0;
     maxn122 = 0;
     key.clear();
   }
@@ -490,7 +490,7 @@ static int harvest_oldest(imsrg_ctx *ctx) {
 
 static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const int4 *const *items, const int *cnt,
                           unsigned mask, int prec, int maxn, int maxnch, long maxn122, int maxjmax, int maxnorb,
-                          int maxn1pairs) {
+                          int maxn1pairs, int maxncls) {
   const Tuning &tu = ctx->db.tune;
   const int prec_pphh = (prec == IMSRG_PREC_MIXED && tu.mixed_pphh) ? IMSRG_PREC_MIXED : IMSRG_PREC_FP64;
   const int prec_ph = (prec == IMSRG_PREC_MIXED && tu.mixed_ph) ? IMSRG_PREC_MIXED : IMSRG_PREC_FP64;
@@ -551,7 +551,7 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
   if (fork_late)
     if (int rc = fork_side()) return rc;
   HIPCHK(ctx, mark(6));
-  if (mask & MPH) HIPCHK(ctx, launch_invpandya(pa, items[4], cnt[4], maxjmax, maxnorb, maxnch, s));
+  if (mask & MPH) HIPCHK(ctx, launch_invpandya(pa, items[4], cnt[4], maxjmax, maxnorb, maxnch, maxncls, s));
   if (overlap) HIPCHK(ctx, hipStreamWaitEvent(s, ctx->join, 0));
   HIPCHK(ctx, mark(7));
   HIPCHK(ctx, launch_finish(pa, items[6], cnt[6], (mask & MPH) != 0, (mask & M122) != 0 && !overlap, s));
@@ -589,7 +589,7 @@ static int commutator_dev(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y
                 (int)db.wl.invp.size(), (int)db.wl.one.size(), (int)db.wl.mirror.size(), (int)db.wl.pphh64.size()};
   if (!ctx->use_graphs || ctx->profiling)
     return run_commutator(ctx, pa, 1, items, cnt, mask, prec, db.h.maxn, db.h.nch, db.h.n122, db.h.jmax,
-                          db.h.norb, db.h.n1pairs);
+                          db.h.norb, db.h.n1pairs, db.h.ncls);
   // replay a captured launch sequence for this operand set, capturing it on first use
   for (const GraphEntry &g : ctx->graphs)
     if (g.x == X->buf && g.y == Y->buf && g.z == Z->buf && g.hx == X->herm && g.hy == Y->herm && g.mask == mask &&
@@ -599,7 +599,7 @@ static int commutator_dev(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y
     }
   HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
   const int rc = run_commutator(ctx, pa, 1, items, cnt, mask, prec, db.h.maxn, db.h.nch, db.h.n122, db.h.jmax,
-                                db.h.norb, db.h.n1pairs);
+                                db.h.norb, db.h.n1pairs, db.h.ncls);
   hipGraph_t graph = nullptr;
   const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);  // always leave capture mode
   if (rc) {
@@ -983,6 +983,7 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
       bc.maxnorb = std::max(bc.maxnorb, ctxs[i]->db.h.norb);
       bc.maxn1pairs = std::max(bc.maxn1pairs, ctxs[i]->db.h.n1pairs);
       bc.maxn122 = std::max(bc.maxn122, ctxs[i]->db.h.n122);
+      bc.maxncls = std::max(bc.maxncls, ctxs[i]->db.h.ncls);
     }
     if (c0->db.tune.pphh_order >= 1) {
       all[0] = xcd_order(groups, kXcd);
@@ -1012,7 +1013,7 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
   const int4 *items[8];
   for (int t = 0; t < 8; ++t) items[t] = bc.d_items + bc.off[t];
   return run_commutator(c0, pa, npairs, items, bc.cnt, mask, prec, bc.maxn, bc.maxnch, bc.maxn122, bc.maxjmax,
-                        bc.maxnorb, bc.maxn1pairs);
+                        bc.maxnorb, bc.maxn1pairs, bc.maxncls);
 }
 
 // ---------------------------------------------------------------------------

@@ -427,6 +427,27 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
     B.sixjQ = (const double *)pQ;
   }
 
+  // ---- (j_k, j_l) class lists of the ket groups (inverse Pandya ket order) ----
+  {
+    const int ncls = B.d1 * B.d1;
+    std::vector<int> cls_start(6 * ncls + 1, 0), cls_e;
+    for (int g = 0; g < 6; ++g) {
+      std::vector<std::vector<int>> per(ncls);
+      for (int e = 0; e < grp_start[g + 1] - grp_start[g]; ++e) {
+        const int kk = grp_kets[grp_start[g] + e];
+        per[((hb.j2[hb.kp[kk]] - 1) / 2) * B.d1 + (hb.j2[hb.kq[kk]] - 1) / 2].push_back(e);
+      }
+      for (int c = 0; c < ncls; ++c) {
+        cls_start[g * ncls + c] = (int)cls_e.size();
+        cls_e.insert(cls_e.end(), per[c].begin(), per[c].end());
+      }
+    }
+    cls_start[6 * ncls] = (int)cls_e.size();
+    B.ncls = ncls;
+    B.cls_start = dupload(*this, cls_start, err);
+    B.cls_e = dupload(*this, cls_e, err);
+  }
+
   // ---- work lists ----
   this->ch_n = ch_n;
   this->ch_kstart = ch_kstart;

@@ -85,6 +85,10 @@ struct BasisDev {
   const IMSRG_G signed char *cc_phi;  // (-1)^{(j_p+j_q)} per CC ket
   const IMSRG_G long *cc_xoff, *cc_zoff;
   long xsize, zsize;
+  // kets of each (parity, Tz) group g by (j_k, j_l) class c = ((j2k-1)/2) d1 + (j2l-1)/2: the group
+  // indices e, ascending, in cls_e[cls_start[g ncls + c] .. cls_start[g ncls + c + 1]), ncls = d1 * d1
+  const IMSRG_G int *cls_start, *cls_e;
+  int ncls;
   const IMSRG_G int2 *pairs1;   // one-body pairs (i<=j, same (l,j2,tz2) block)
   // partial sums of the one-body part: slots ob_start[p] .. ob_start[p+1]-1 of pair p
   // (the first: comm111 + comm121, the others: comm221 items); cpart: orbits with 1 - n_c != 0

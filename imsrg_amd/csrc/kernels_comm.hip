@@ -883,9 +883,10 @@ __device__ __forceinline__ int wave_max(int v) {
 }
 
 // AddInversePandyaTransformation (src/Commutator.cc:1501-1616), by orbit quadruple.
-// One work-group per bra ket (ij) of a (parity, Tz) group; lane e takes the
-// group's ket (kl) = e >= (ij), looping over the kets in chunks of the group
-// size.  The cross-coupled elements Zbar^{J'}[(il),(kj)] (direct) and
+// One work-group per bra ket (ij) of a (parity, Tz) group; its lanes take the
+// group's kets (kl) = e >= (ij) in (j_k, j_l) class order, so that the lanes of
+// a wave mostly share their J and J' ranges (one lane per ket, in chunks of the
+// work-group size).  The cross-coupled elements Zbar^{J'}[(il),(kj)] (direct) and
 // Zbar^{J'}[(jl),(ki)] (exchange) do not depend on the standard-channel J, so
 // each is loaded once and recoupled into every J of the quadruple (register
 // chunks of kJC values):
@@ -896,12 +897,12 @@ __device__ __forceinline__ int wave_max(int v) {
 // the kets (i,x) and (j,x), the CC channel sizes/offsets and the bra's
 // standard-channel data are staged in LDS once per work-group.  Every upper
 // element (ij <= kl, or ij < kl for anti-Hermitian Z) of every channel is
-// produced exactly once and stored at its transposed slot (consecutive lanes,
-// consecutive addresses); k_finish adds it to the upper triangle, the
-// diagonal is added in place.
-__host__ __device__ inline size_t invp_lds_bytes(int jmax, int norb, int nch) {
+// produced exactly once and stored at its transposed slot (row ib of channel J,
+// written by this work-group only, so the L2 merges the lanes' scattered
+// stores); k_finish adds it to the upper triangle, the diagonal is kept apart.
+__host__ __device__ inline size_t invp_lds_bytes(int jmax, int norb, int nch, int ncls) {
   const size_t nJ = (size_t)jmax + 1;
-  return 8 * (size_t)nch + 8 * nJ + 4 * (size_t)nch + 4 * nJ * (2 * (size_t)norb + 3);
+  return 8 * (size_t)nch + 8 * nJ + 4 * (size_t)nch + 4 * nJ * (2 * (size_t)norb + 3) + 4 * (2 * (size_t)ncls + 1);
 }
 
 __global__ __launch_bounds__(256, 5) void k_invpandya(PairArg pa, const int4 *items) {
@@ -910,7 +911,7 @@ __global__ __launch_bounds__(256, 5) void k_invpandya(PairArg pa, const int4 *it
   PairPtrs P = pa.get(it.x);
   const BasisDev &B = *P.basis;
   const int g = it.y, b = it.z;
-  const int gs = B.grp_start[g], ng = B.grp_start[g + 1] - gs;
+  const int gs = B.grp_start[g];
   const int nk = B.nkets, norb = B.norb, nJ = B.jmax + 1, nch = B.nch;
   long *s_zoff = (long *)lds_raw;      // [nch] CC channel offsets in Zbar
   long *s_choff = s_zoff + nch;        // [nJ]  offsets of the standard channels 6J + g
@@ -920,7 +921,11 @@ __global__ __launch_bounds__(256, 5) void k_invpandya(PairArg pa, const int4 *it
   int *s_chks = s_ib + nJ;             // [nJ]  first ket (all-channel index) of channel 6J + g
   int *Ti = s_chks + nJ;               // [nJ * norb] CC local index of ket (i,x) in channel J'
   int *Tj = Ti + nJ * norb;            // [nJ * norb] CC local index of ket (j,x) in channel J'
+  const int ncls = B.ncls;
+  int *s_us = Tj + nJ * norb;          // [ncls] first position of class c with e >= emin
+  int *s_pre = s_us + ncls;            // [ncls + 1] positions of the classes in the bra's ket order
   const int kb = B.grp_kets[gs + b];
+  const int emin = (P.hZ == 1) ? b : b + 1;
   const int i = B.kp[kb], j = B.kq[kb];
   for (int c = threadIdx.x; c < nch; c += blockDim.x) {
     s_zoff[c] = B.cc_zoff[c];
@@ -937,7 +942,26 @@ __global__ __launch_bounds__(256, 5) void k_invpandya(PairArg pa, const int4 *it
     Ti[e] = B.cmapT[(long)Jp * nk + ket_index(norb, min(i, x), max(i, x))];
     Tj[e] = B.cmapT[(long)Jp * nk + ket_index(norb, min(j, x), max(j, x))];
   }
+  // kets e >= emin taken class by class: the lanes of a wave then mostly share
+  // j_k and j_l, hence their J and J' ranges
+  for (int c = threadIdx.x; c < ncls; c += blockDim.x) {
+    int lo = B.cls_start[g * ncls + c];
+    const int end = B.cls_start[g * ncls + c + 1];
+    int hi = end;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (B.cls_e[mid] < emin) lo = mid + 1; else hi = mid;
+    }
+    s_us[c] = lo;
+    s_pre[c + 1] = end - lo;
+  }
   __syncthreads();
+  if (threadIdx.x == 0) {
+    s_pre[0] = 0;
+    for (int c = 0; c < ncls; ++c) s_pre[c + 1] += s_pre[c];
+  }
+  __syncthreads();
+  const int total = s_pre[ncls];
   const int jji = B.j2[i], jjj = B.j2[j];
   const int d2q = B.d2q;
   const double hf = (P.hZ == 1) ? 1.0 : -1.0;
@@ -945,11 +969,17 @@ __global__ __launch_bounds__(256, 5) void k_invpandya(PairArg pa, const int4 *it
   double *Ztwo = P.Z + B.two_off;
   const IMSRG_G double *sixq = B.sixjQ;  // held in registers: the stores below could alias B
   const IMSRG_G int *kmapT = B.kmapT;
-  const int emin = (P.hZ == 1) ? b : b + 1;
-  for (int e0 = emin; e0 < ng; e0 += blockDim.x) {
-    const int e = e0 + threadIdx.x;
-    const bool active = e < ng;
-    const int kk = B.grp_kets[gs + (active ? e : b)];
+  const IMSRG_G int *cls_e = B.cls_e;
+  for (int q0 = 0; q0 < total; q0 += blockDim.x) {
+    const int q = q0 + threadIdx.x;
+    const bool active = q < total;
+    int clo = 0, chi = ncls;  // class of position q: s_pre[clo] <= q < s_pre[chi]
+    while (chi - clo > 1) {
+      const int mid = (clo + chi) >> 1;
+      if (s_pre[mid] <= q) clo = mid; else chi = mid;
+    }
+    const int e = active ? cls_e[s_us[clo] + q - s_pre[clo]] : b;
+    const int kk = B.grp_kets[gs + e];
     const int k = B.kp[kk], l = B.kq[kk];  // i <= k <= l
     const int jjk = B.j2[k], jjl = B.j2[l];
     const bool pauli = (i == j) || (k == l);  // kets (p,p) exist for even J only
@@ -1282,9 +1312,9 @@ hipError_t launch_pandya(const PairArg &pa, const int4 *items, int nitems, int m
   return hipGetLastError();
 }
 hipError_t launch_invpandya(const PairArg &pa, const int4 *items, int nitems, int jmax, int norb, int nch,
-                            hipStream_t s) {
+                            int ncls, hipStream_t s) {
   if (nitems <= 0) return hipSuccess;
-  const size_t lds = invp_lds_bytes(jmax, norb, nch);
+  const size_t lds = invp_lds_bytes(jmax, norb, nch, ncls);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_invpandya, dim3(nitems), dim3(kChunk), lds, s, pa, items);
   return hipGetLastError();

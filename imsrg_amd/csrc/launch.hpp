@@ -26,9 +26,9 @@ hipError_t launch_c122_coef(const PairArg &pa, int npairs, long maxn122, hipStre
 hipError_t launch_pandya(const PairArg &pa, const int4 *items, int nitems, int maxnch, hipStream_t s);
 // ph GEMM (head and tail of Zbar) on 128x128 tiles (big) or 64x64 tiles
 hipError_t launch_ph_gemm(const PairArg &pa, const int4 *items, int nitems, int precision, bool big, hipStream_t s);
-// jmax, norb, nch: the largest of the pairs (they size the kernel's LDS tables)
+// jmax, norb, nch, ncls: the largest of the pairs (they size the kernel's LDS tables)
 hipError_t launch_invpandya(const PairArg &pa, const int4 *items, int nitems, int jmax, int norb, int nch,
-                            hipStream_t s);
+                            int ncls, hipStream_t s);
 // last phase: with fold, upper += the inverse-Pandya contributions stored at the
 // transposed slots; with c122, Z2 -= W + hZ W^T (comm122ss); then lower triangle
 // <- hZ * upper
