@@ -845,19 +845,39 @@ __global__ __launch_bounds__(256) void k_pandya(PairArg pa, const int4 *items) {
     const int k1 = ket_index(B.norb, min(c, b), max(c, b)), k2 = ket_index(B.norb, min(a, d), max(a, d));
     const double hY = P.hY;
     if (pcb == pad) {
-      for (int J = jmin; J <= jmax; J += dJ) {
-        // the 6j value and both ket maps are loaded together (no branch between them)
-        const double sj = B.sixj[sixj_index(B.d1, B.d2, ja, jb, Jcc, jc, jd, J)];
-        const int i1 = B.kmap[(long)k1 * (B.jmax + 1) + J], i2 = B.kmap[(long)k2 * (B.jmax + 1) + J];
-        if (fabs(sj) < 1e-8 || i1 < 0 || i2 < 0) continue;
-        const int ch = 6 * J + 2 * (tcb + 1) + pcb;
-        double ph = 1.0;
-        if (c > b) ph *= sgn_parity((jc + jb) / 2 + 1 + J);
-        if (a > d) ph *= sgn_parity((ja + jd) / 2 + 1 + J);
-        const long idx = s_choff[ch] + (long)i2 * s_chn[ch] + i1;
-        const double w = (2 * J + 1) * sj;
-        xbar -= w * (ph * X2[idx]);
-        ybar -= w * (ph * Y2[idx]) * hY;
+      // two J per pass, every load of both issued before use; an invalid term
+      // reads a clamped in-bounds element and is weighted by 0
+      const IMSRG_G double *sixt = B.sixj;
+      const IMSRG_G int *kmap = B.kmap;
+      const int nJ = B.jmax + 1;
+      for (int J0 = jmin; J0 <= jmax; J0 += 2 * dJ) {
+        double sj[2], xv[2], yv[2], wv2[2];
+        int i1[2], i2[2], ch[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int J = min(J0 + u * dJ, jmax);
+          sj[u] = sixt[sixj_index(B.d1, B.d2, ja, jb, Jcc, jc, jd, J)];
+          i1[u] = kmap[(long)k1 * nJ + J];
+          i2[u] = kmap[(long)k2 * nJ + J];
+          ch[u] = 6 * J + 2 * (tcb + 1) + pcb;
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int J = J0 + u * dJ;
+          const bool ok = J <= jmax && fabs(sj[u]) >= 1e-8 && i1[u] >= 0 && i2[u] >= 0;
+          double ph = 1.0;
+          if (c > b) ph *= sgn_parity((jc + jb) / 2 + 1 + J);
+          if (a > d) ph *= sgn_parity((ja + jd) / 2 + 1 + J);
+          const long idx = s_choff[ch[u]] + (long)max(i2[u], 0) * s_chn[ch[u]] + max(i1[u], 0);
+          xv[u] = X2[idx];
+          yv[u] = Y2[idx];
+          wv2[u] = ok ? (2 * J + 1) * sj[u] * ph : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          xbar -= wv2[u] * xv[u];
+          ybar -= wv2[u] * yv[u] * hY;
+        }
       }
     }
   }
