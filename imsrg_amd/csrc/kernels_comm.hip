@@ -737,14 +737,17 @@ __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
           if (row < n && col < n && row <= col) D[(long)col * n + row] = alpha * acc[mi][ni][q];
         }
   } else {
-    // full Zbar storage through an LDS transpose of one 64x32 wave sub-tile per
-    // round (the operand buffer A0, idle now, holds the tile)
-    double(*Tt)[kGW + 1] = reinterpret_cast<double(*)[kGW + 1]>(&A0[0][0]);
+    // full Zbar storage through LDS transposes of the 64x32 wave sub-tiles, four
+    // per round (the four operand buffers, idle now, hold one sub-tile each)
     static_assert(64 * (kGW + 1) <= kTileK * LD, "transpose tile exceeds the operand buffer");
-    for (int w = 0; w < kGWaves; ++w) {
-      if (!live(w)) continue;  // block-uniform
-      const int wr2 = (w & 1) * 64, wc2 = (w >> 1) * kGW;
-      if (wv == w) {
+    static_assert(kGWaves == 8, "two rounds of four sub-tiles");
+    auto Tb = [&](int j) {
+      double *p = j == 0 ? &A0[0][0] : j == 1 ? &A1[0][0] : j == 2 ? &B0[0][0] : &B1[0][0];
+      return reinterpret_cast<double(*)[kGW + 1]>(p);
+    };
+    for (int rd = 0; rd < 2; ++rd) {
+      if ((wv >> 2) == rd && wave_live) {
+        double(*Tt)[kGW + 1] = Tb(wv & 3);
 #pragma unroll
         for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
@@ -754,28 +757,33 @@ __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
               const int rr = mi * 16 + (lane & 15), cc = ni * 16 + mfma_col<MIXED>(lane, q);
               double v = alpha * acc[mi][ni][q];
               if (MODE == 2) {
-                const int col = S0 + wc2 + cc;
+                const int col = S0 + wc + cc;
                 v *= (col < n) ? (double)phi[col] : 0.0;
               }
               Tt[rr][cc] = v;
             }
       }
       __syncthreads();
-      const int R1 = R0 + wr2, S1 = S0 + wc2;
-      // upper elements (R1 + rr, S1 + cc), consecutive threads on consecutive rows
-      for (int e = threadIdx.x; e < 64 * kGW; e += blockDim.x) {
-        const int rr = e & 63, cc = e >> 6;
-        const int row = R1 + rr, col = S1 + cc;
-        if (row < n && col < n && row <= col) D[(long)col * n + row] = Tt[rr][cc];
-      }
-      // mirrored elements (S1 + cc, R1 + rr) = sym * upper (R1 + rr, S1 + cc),
-      // consecutive threads on consecutive rows S1 + cc
-      for (int e = threadIdx.x; e < 64 * kGW; e += blockDim.x) {
-        const int cc = e & (kGW - 1), rr = e / kGW;
-        const int row2 = S1 + cc, col2 = R1 + rr;
-        if (row2 < n && col2 < n && row2 > col2) {
-          const double sym = (MODE == 1) ? h : (double)(phi[row2] * phi[col2]);
-          D[(long)col2 * n + row2] = sym * Tt[rr][cc];
+      for (int j = 0; j < 4; ++j) {
+        const int w = 4 * rd + j;
+        if (!live(w)) continue;  // block-uniform
+        double(*Tt)[kGW + 1] = Tb(j);
+        const int R1 = R0 + (w & 1) * 64, S1 = S0 + (w >> 1) * kGW;
+        // upper elements (R1 + rr, S1 + cc), consecutive threads on consecutive rows
+        for (int e = threadIdx.x; e < 64 * kGW; e += blockDim.x) {
+          const int rr = e & 63, cc = e >> 6;
+          const int row = R1 + rr, col = S1 + cc;
+          if (row < n && col < n && row <= col) D[(long)col * n + row] = Tt[rr][cc];
+        }
+        // mirrored elements (S1 + cc, R1 + rr) = sym * upper (R1 + rr, S1 + cc),
+        // consecutive threads on consecutive rows S1 + cc
+        for (int e = threadIdx.x; e < 64 * kGW; e += blockDim.x) {
+          const int cc = e & (kGW - 1), rr = e / kGW;
+          const int row2 = S1 + cc, col2 = R1 + rr;
+          if (row2 < n && col2 < n && row2 > col2) {
+            const double sym = (MODE == 1) ? h : (double)(phi[row2] * phi[col2]);
+            D[(long)col2 * n + row2] = sym * Tt[rr][cc];
+          }
         }
       }
       __syncthreads();
