@@ -1161,28 +1161,45 @@ __device__ __forceinline__ const T &blk_at(const T *base, unsigned idx) {
 
 // W(row, q) of comm122ss (src/Commutator.cc:908-926) for one column q of a
 // channel: sum over the partner entries e of ket q of Y2(row, ind_e) x_e -
-// X2(row, ind_e) y_e.  q (hence e0, e1) is uniform over the wave; row is the
-// lane's.  Every column's list holds a multiple of kC122Blk entries (padded
+// X2(row, ind_e) y_e.  q (hence e0, e1) is uniform over the wave; lane l
+// returns row row0 + l.  Every column's list holds a multiple of kC122Blk entries (padded
 // with its first partner and a zero factor), read in blocks of kC122Blk whose
 // loads are all issued before the first use.
+//
+// The gathers are bound by address processing per load instruction, so each
+// half-wave takes every second partner and loads 16 bytes per lane: lane
+// (h, p) = (lane >> 5, lane & 31) reads rows row0 + 2p, row0 + 2p + 1 of the
+// partners eb + h, eb + h + 2 of each block (a pair may read one element past
+// its column: the next column or the arena's slack).  The halves are summed
+// and lane l gets row row0 + l; lanes whose row lies past n get an unused value.
 __device__ __forceinline__ double c122_w(const double *X2, const double *Y2, const IMSRG_G double2 *coef,
-                                         const IMSRG_G int *ind, int e0, int e1, int n, int row) {
+                                         const IMSRG_G int *ind, int e0, int e1, int n, int row0) {
   constexpr int kB = kC122Blk;
-  double w = 0.0;
+  static_assert(kB % 2 == 0, "partner blocks split between the two half-waves");
+  const int lane = threadIdx.x & 63, h = lane >> 5;
+  const int r2 = max(min(row0 + 2 * (lane & 31), n - 1), 0);
+  double2 w = {0.0, 0.0};
   for (int eb = e0; eb < e1; eb += kB) {
-    double y[kB], x[kB];
-    double2 cf[kB];
+    double2 y[kB / 2], x[kB / 2], cf[kB / 2];
 #pragma unroll
-    for (int m = 0; m < kB; ++m) {
-      const unsigned o = (unsigned)(ind[eb + m] * n + row);
-      y[m] = blk_at(Y2, o);
-      x[m] = blk_at(X2, o);
-      cf[m] = coef[eb + m];
+    for (int m = 0; m < kB / 2; ++m) {
+      const int i0 = ind[eb + 2 * m], i1 = ind[eb + 2 * m + 1];  // wave-uniform (scalar) loads
+      const double2 c0 = coef[eb + 2 * m], c1 = coef[eb + 2 * m + 1];
+      const unsigned o = (unsigned)((h ? i1 : i0) * n + r2);
+      y[m] = *reinterpret_cast<const double2 *>(&blk_at(Y2, o));
+      x[m] = *reinterpret_cast<const double2 *>(&blk_at(X2, o));
+      cf[m] = h ? c1 : c0;
     }
 #pragma unroll
-    for (int m = 0; m < kB; ++m) w += y[m] * cf[m].x - x[m] * cf[m].y;
+    for (int m = 0; m < kB / 2; ++m) {
+      w.x += y[m].x * cf[m].x - x[m].x * cf[m].y;
+      w.y += y[m].y * cf[m].x - x[m].y * cf[m].y;
+    }
   }
-  return w;
+  w.x += __shfl_xor(w.x, 32, 64);
+  w.y += __shfl_xor(w.y, 32, 64);
+  const double a = __shfl(w.x, lane >> 1, 64), b = __shfl(w.y, lane >> 1, 64);
+  return (lane & 1) ? b : a;
 }
 
 // Last kernel of a commutator, over the tile pairs of every two-body block:
@@ -1221,7 +1238,7 @@ __global__ __launch_bounds__(256, 4) void k_finish(PairArg pa, const int4 *items
     const int cc = wv + 4 * i, r = R0 + lane, c = S0 + cc;
     const double z = blk_at(Z2, (unsigned)(min(c, n - 1) * n + min(r, n - 1)));  // clamped: no branch around the load
     u[i] = (r < n && c < n) ? z : 0.0;
-    if (C122 && c < n) u[i] -= c122_w(X2, Y2, P.c122, B.c122_ind, cst[c], cst[c + 1], n, min(r, n - 1));
+    if (C122 && c < n) u[i] -= c122_w(X2, Y2, P.c122, B.c122_ind, cst[c], cst[c + 1], n, R0);
   }
 #pragma unroll
   for (int i = 0; i < kPer; ++i) {  // transposed position (S0 + lane, R0 + rr)
@@ -1231,7 +1248,7 @@ __global__ __launch_bounds__(256, 4) void k_finish(PairArg pa, const int4 *items
       const double z = blk_at(Z2, (unsigned)(min(r, n - 1) * n + min(c, n - 1)));
       v = (r < n && c < n && c != r) ? z : 0.0;
     }
-    if (C122 && r < n) v -= hZ * c122_w(X2, Y2, P.c122, B.c122_ind, cst[r], cst[r + 1], n, min(c, n - 1));
+    if (C122 && r < n) v -= hZ * c122_w(X2, Y2, P.c122, B.c122_ind, cst[r], cst[r + 1], n, S0);
     t[rr][lane] = v;
   }
   __syncthreads();
