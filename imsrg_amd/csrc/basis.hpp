@@ -46,7 +46,7 @@ std::vector<int4> xcd_order(std::vector<TileGroup> groups, int nxcd);
 
 // Launch choices that the autotuner sets (defaults: measured best at emax=10).
 struct Tuning {
-  int pphh_small = 0;  // channels with n <= pphh_small take 64x64 pp/hh tiles, the others 128x128
+  int pphh_small = 64;  // channels with n <= pphh_small take 64x64 pp/hh tiles, the others 128x128
   int pphh_order = 0;  // 0: tiles by decreasing k-length, 1: row-block groups balanced over the XCDs,
                        // 2: square super-tiles balanced over the XCDs
   int mixed_pphh = 1;  // mixed precision: 1 = FP32-MFMA pp/hh kernel, 0 = the FP64 kernel
