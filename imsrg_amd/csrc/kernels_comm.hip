@@ -769,20 +769,36 @@ __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
         if (!live(w)) continue;  // block-uniform
         double(*Tt)[kGW + 1] = Tb(j);
         const int R1 = R0 + (w & 1) * 64, S1 = S0 + (w >> 1) * kGW;
-        // upper elements (R1 + rr, S1 + cc), consecutive threads on consecutive rows
-        for (int e = threadIdx.x; e < 64 * kGW; e += blockDim.x) {
-          const int rr = e & 63, cc = e >> 6;
+        // Two rows per thread and one 16-byte store where both are written (store
+        // instructions, not bytes, bound this epilogue); R1 and S1 are even.
+        // upper elements (R1 + rr, S1 + cc), consecutive threads on consecutive row pairs
+        for (int e = threadIdx.x; e < 32 * kGW; e += blockDim.x) {
+          const int rr = 2 * (e & 31), cc = e >> 5;
           const int row = R1 + rr, col = S1 + cc;
-          if (row < n && col < n && row <= col) D[(long)col * n + row] = Tt[rr][cc];
+          if (col >= n || row >= n || row > col) continue;
+          double *dp = D + (long)col * n + row;
+          if (row + 1 < n && row + 1 <= col)
+            *reinterpret_cast<double2 *>(dp) = double2{Tt[rr][cc], Tt[rr + 1][cc]};
+          else
+            *dp = Tt[rr][cc];
         }
         // mirrored elements (S1 + cc, R1 + rr) = sym * upper (R1 + rr, S1 + cc),
-        // consecutive threads on consecutive rows S1 + cc
-        for (int e = threadIdx.x; e < 64 * kGW; e += blockDim.x) {
-          const int cc = e & (kGW - 1), rr = e / kGW;
+        // consecutive threads on consecutive row pairs S1 + cc
+        for (int e = threadIdx.x; e < 32 * kGW; e += blockDim.x) {
+          const int cc = 2 * (e & (kGW / 2 - 1)), rr = e / (kGW / 2);
           const int row2 = S1 + cc, col2 = R1 + rr;
-          if (row2 < n && col2 < n && row2 > col2) {
-            const double sym = (MODE == 1) ? h : (double)(phi[row2] * phi[col2]);
-            D[(long)col2 * n + row2] = sym * Tt[rr][cc];
+          if (col2 >= n || row2 >= n) continue;
+          const bool lo = row2 > col2, hi = row2 + 1 < n && row2 + 1 > col2;
+          const double s0 = (MODE == 1) ? h : (double)(phi[row2] * phi[col2]);
+          double *dp = D + (long)col2 * n + row2;
+          if (lo && hi) {
+            const double s1 = (MODE == 1) ? h : (double)(phi[row2 + 1] * phi[col2]);
+            *reinterpret_cast<double2 *>(dp) = double2{s0 * Tt[rr][cc], s1 * Tt[rr][cc + 1]};
+          } else if (lo) {
+            *dp = s0 * Tt[rr][cc];
+          } else if (hi) {
+            const double s1 = (MODE == 1) ? h : (double)(phi[row2 + 1] * phi[col2]);
+            dp[1] = s1 * Tt[rr][cc + 1];
           }
         }
       }
