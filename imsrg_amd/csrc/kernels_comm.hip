@@ -1177,8 +1177,8 @@ __device__ __forceinline__ const T &blk_at(const T *base, unsigned idx) {
 
 // W(row, q) of comm122ss (src/Commutator.cc:908-926) for one column q of a
 // channel: sum over the partner entries e of ket q of Y2(row, ind_e) x_e -
-// X2(row, ind_e) y_e.  q (hence e0, e1) is uniform over the wave; lane l
-// returns row row0 + l.  Every column's list holds a multiple of kC122Blk entries (padded
+// X2(row, ind_e) y_e.  q (hence e0, e1) is uniform over the wave.  Every
+// column's list holds a multiple of kC122Blk entries (padded
 // with its first partner and a zero factor), read in blocks of kC122Blk whose
 // loads are all issued before the first use.
 //
@@ -1186,10 +1186,11 @@ __device__ __forceinline__ const T &blk_at(const T *base, unsigned idx) {
 // half-wave takes every second partner and loads 16 bytes per lane: lane
 // (h, p) = (lane >> 5, lane & 31) reads rows row0 + 2p, row0 + 2p + 1 of the
 // partners eb + h, eb + h + 2 of each block (a pair may read one element past
-// its column: the next column or the arena's slack).  The halves are summed
-// and lane l gets row row0 + l; lanes whose row lies past n get an unused value.
-__device__ __forceinline__ double c122_w(const double *X2, const double *Y2, const IMSRG_G double2 *coef,
-                                         const IMSRG_G int *ind, int e0, int e1, int n, int row0) {
+// its column: the next column or the arena's slack).  The halves are summed,
+// so both lanes p and p + 32 return rows row0 + 2p, row0 + 2p + 1; rows past n
+// get unused values.
+__device__ __forceinline__ double2 c122_w2(const double *X2, const double *Y2, const IMSRG_G double2 *coef,
+                                           const IMSRG_G int *ind, int e0, int e1, int n, int row0) {
   constexpr int kB = kC122Blk;
   static_assert(kB % 2 == 0, "partner blocks split between the two half-waves");
   const int lane = threadIdx.x & 63, h = lane >> 5;
@@ -1214,8 +1215,15 @@ __device__ __forceinline__ double c122_w(const double *X2, const double *Y2, con
   }
   w.x += __shfl_xor(w.x, 32, 64);
   w.y += __shfl_xor(w.y, 32, 64);
-  const double a = __shfl(w.x, lane >> 1, 64), b = __shfl(w.y, lane >> 1, 64);
-  return (lane & 1) ? b : a;
+  return w;
+}
+
+// 16-byte access to elements idx, idx + 1 of a channel block (8-byte aligned)
+__device__ __forceinline__ double2 blk_ld2(const double *base, unsigned idx) {
+  return *reinterpret_cast<const double2 *>(&blk_at(base, idx));
+}
+__device__ __forceinline__ void blk_st2(double *base, unsigned idx, double2 v) {
+  *reinterpret_cast<double2 *>(&blk_at(base, idx)) = v;
 }
 
 // Last kernel of a commutator, over the tile pairs of every two-body block:
@@ -1246,46 +1254,83 @@ __global__ __launch_bounds__(256, 4) void k_finish(PairArg pa, const int4 *items
   const IMSRG_G int *cst = B.c122_start + B.ch_kstart[ch];
   const IMSRG_G double *dg = P.diag + B.ch_kstart[ch];  // inverse-Pandya diagonal terms of this channel
   const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  // lane (h, p): rows p2, p2 + 1 (p2 = 2p) of the tile, in the columns (U) or
+  // rows (T) wv + 4 (2 j + h), j < kPairs: two elements per load and store
+  const int h = lane >> 5, p2 = 2 * (lane & 31);
   const double hZ = P.hZ;
-  constexpr int kPer = kTile * kTile / 256;
-  double u[kPer];
+  constexpr int kPairs = kTile * kTile / 512;
+  // acc -= f * W for the wave's two columns (or rows) of pass j: one
+  // wave-uniform partner list each, the half-wave h keeps its own
+  auto sub122 = [&](double2 &acc, int cA, int row0, double f) {
 #pragma unroll
-  for (int i = 0; i < kPer; ++i) {  // upper element (R0 + lane, S0 + cc)
-    const int cc = wv + 4 * i, r = R0 + lane, c = S0 + cc;
-    const double z = blk_at(Z2, (unsigned)(min(c, n - 1) * n + min(r, n - 1)));  // clamped: no branch around the load
-    u[i] = (r < n && c < n) ? z : 0.0;
-    if (C122 && c < n) u[i] -= c122_w(X2, Y2, P.c122, B.c122_ind, cst[c], cst[c + 1], n, R0);
+    for (int hh = 0; hh < 2; ++hh) {
+      const int cq = cA + 4 * hh;
+      if (cq >= n) break;  // wave-uniform
+      const double2 w = c122_w2(X2, Y2, P.c122, B.c122_ind, cst[cq], cst[cq + 1], n, row0);
+      if (h == hh) {
+        acc.x -= f * w.x;
+        acc.y -= f * w.y;
+      }
+    }
+  };
+  double2 u[kPairs];
+#pragma unroll
+  for (int j = 0; j < kPairs; ++j) {  // upper elements (R0 + p2 (+1), S0 + cc)
+    const int cc = wv + 4 * (2 * j + h), r = R0 + p2, c = S0 + cc;
+    // clamped: no branch around the load (a pair may read one element past its column)
+    const double2 z = blk_ld2(Z2, (unsigned)(min(c, n - 1) * n + min(r, n - 1)));
+    u[j].x = (r < n && c < n) ? z.x : 0.0;
+    u[j].y = (r + 1 < n && c < n) ? z.y : 0.0;
+    if (C122) sub122(u[j], S0 + wv + 8 * j, R0, 1.0);
   }
 #pragma unroll
-  for (int i = 0; i < kPer; ++i) {  // transposed position (S0 + lane, R0 + rr)
-    const int rr = wv + 4 * i, r = R0 + rr, c = S0 + lane;
-    double v = 0.0;
+  for (int j = 0; j < kPairs; ++j) {  // transposed positions (S0 + p2 (+1), R0 + rr)
+    const int rr = wv + 4 * (2 * j + h), r = R0 + rr, c = S0 + p2;
+    double2 v = {0.0, 0.0};
     if (FOLD) {
-      const double z = blk_at(Z2, (unsigned)(min(r, n - 1) * n + min(c, n - 1)));
-      v = (r < n && c < n && c != r) ? z : 0.0;
+      const double2 z = blk_ld2(Z2, (unsigned)(min(r, n - 1) * n + min(c, n - 1)));
+      v.x = (r < n && c < n && c != r) ? z.x : 0.0;
+      v.y = (r < n && c + 1 < n && c + 1 != r) ? z.y : 0.0;
     }
-    if (C122 && r < n) v -= hZ * c122_w(X2, Y2, P.c122, B.c122_ind, cst[r], cst[r + 1], n, S0);
-    t[rr][lane] = v;
+    if (C122) sub122(v, R0 + wv + 8 * j, S0, hZ);
+    t[rr][p2] = v.x;
+    t[rr][p2 + 1] = v.y;
   }
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < kPer; ++i) {
-    const int cc = wv + 4 * i, r = R0 + lane, c = S0 + cc;
-    if (r < n && c < n && r <= c) {
-      const double v = (r != c) ? u[i] + t[lane][cc] : (P.hZ == -1 ? 0.0 : u[i] + t[lane][cc] + (FOLD ? dg[r] : 0.0));
-      blk_at(Z2, (unsigned)(c * n + r)) = v;
-      u[i] = v;
-    }
+  for (int j = 0; j < kPairs; ++j) {
+    const int cc = wv + 4 * (2 * j + h), r = R0 + p2, c = S0 + cc;
+    double2 v = {u[j].x + t[p2][cc], u[j].y + t[p2 + 1][cc]};
+    const bool ok0 = r < n && c < n && r <= c, ok1 = r + 1 < n && c < n && r + 1 <= c;  // ok1 implies ok0
+    // diagonal elements inside the block only: dg holds n entries of this channel
+    if (ok0 && r == c) v.x = (P.hZ == -1) ? 0.0 : v.x + (FOLD ? dg[r] : 0.0);
+    if (ok1 && r + 1 == c) v.y = (P.hZ == -1) ? 0.0 : v.y + (FOLD ? dg[r + 1] : 0.0);
+    if (ok1)
+      blk_st2(Z2, (unsigned)(c * n + r), v);
+    else if (ok0)
+      blk_at(Z2, (unsigned)(c * n + r)) = v.x;
+    u[j] = v;
   }
   if constexpr (!MIRROR) return;
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < kPer; ++i) t[lane][wv + 4 * i] = u[i];  // new upper values to the threads writing their mirror
+  for (int j = 0; j < kPairs; ++j) {  // new upper values to the threads writing their mirror
+    const int cc = wv + 4 * (2 * j + h);
+    t[p2][cc] = u[j].x;
+    t[p2 + 1][cc] = u[j].y;
+  }
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < kPer; ++i) {  // lower element (S0 + lane, R0 + rr) = hZ * upper
-    const int rr = wv + 4 * i, r = R0 + rr, c = S0 + lane;
-    if (r < n && c < n && r < c) blk_at(Z2, (unsigned)(r * n + c)) = hZ * t[rr][lane];
+  for (int j = 0; j < kPairs; ++j) {  // lower elements (S0 + p2 (+1), R0 + rr) = hZ * upper
+    const int rr = wv + 4 * (2 * j + h), r = R0 + rr, c = S0 + p2;
+    const bool ok0 = r < n && c < n && r < c, ok1 = r < n && c + 1 < n && r < c + 1;
+    const double2 v = {hZ * t[rr][p2], hZ * t[rr][p2 + 1]};
+    if (ok0 && ok1)
+      blk_st2(Z2, (unsigned)(r * n + c), v);
+    else if (ok0)
+      blk_at(Z2, (unsigned)(r * n + c)) = v.x;
+    else if (ok1)
+      blk_at(Z2, (unsigned)(r * n + c + 1)) = v.y;
   }
 }
 
