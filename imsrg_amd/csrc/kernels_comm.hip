@@ -1236,9 +1236,11 @@ __device__ __forceinline__ void blk_st2(double *base, unsigned idx, double2 v) {
 // Without MIRROR only the upper triangle is read and written, so that pass can
 // run while other kernels write the transposed slots.
 // Tile pair: U = rows R0.., cols S0.. (R0 <= S0) and its transposed position.
-// U stays in registers: one column per wave, lane = row.  The contributions
-// held at the transposed position go through LDS: one column R0 + rr of Z2
-// per wave, lane = row S0 + cc.
+// U stays in registers, two rows per thread in 8 of the wave's 16 columns.  The
+// contributions held at the transposed position go through LDS, two rows per
+// thread of 8 of the wave's 16 columns R0 + rr of Z2.  Loads and stores of
+// Z2 then move 16 bytes per lane (address processing per instruction, not
+// bytes, bounds these accesses).
 template <bool FOLD, bool C122, bool MIRROR = true>
 __global__ __launch_bounds__(256, 4) void k_finish(PairArg pa, const int4 *items) {
   __shared__ double t[kTile][kTile + 1];  // t[rr][cc]: contribution to upper (R0 + rr, S0 + cc)
