@@ -254,8 +254,11 @@ def test_batched_matches_single(ia):
     pairs = []
     refs = []
     say("[gpu] batched commutators")
-    for i, emax in enumerate([2, 4, 3, 4, 2]):
-        ms, _ = spaces(ia, emax)
+    # includes degenerate bases (He4 at emax 0: no pp or ph kets; H2 at emax 1:
+    # half-filled orbits) inside the grouped launch
+    for i, (emax, A, Z_) in enumerate([(2, 16, 8), (4, 16, 8), (0, 4, 2), (3, 16, 8), (1, 2, 1), (4, 16, 8),
+                                       (2, 16, 8)]):
+        ms, _ = spaces(ia, emax, A, Z_)
         X = ia.DeviceOperator(ms, -1).fill_random(-1, 500 + i)
         Y = ia.DeviceOperator(ms, 1).fill_random(1, 600 + i)
         Z = ia.DeviceOperator(ms, 1)
