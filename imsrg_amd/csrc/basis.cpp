@@ -321,7 +321,7 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
 
   // ---- cross-coupled channels ----
   std::vector<int> cc_n(nch), cc_nph(nch), cc_kstart(nch + 1), cc_kets, cc_phstart(nch + 1), cc_ph;
-  std::vector<signed char> cc_phi;
+  std::vector<signed char> cc_phi, cc_phph;
   std::vector<long> cc_xoff(nch + 1), cc_zoff(nch + 1);
   long xo = 0, zo = 0;
   int maxcc = 0;
@@ -345,6 +345,8 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
     // kets_ph = join_cols(hh, ph), src/Commutator.cc:1398
     cc_ph.insert(cc_ph.end(), hh.begin(), hh.end());
     cc_ph.insert(cc_ph.end(), ph.begin(), ph.end());
+    for (int i : hh) cc_phph.push_back(cc_phi[cc_kstart[c] + i]);
+    for (int i : ph) cc_phph.push_back(cc_phi[cc_kstart[c] + i]);
     cc_nph[c] = (int)(hh.size() + ph.size());
     cc_xoff[c] = xo;
     cc_zoff[c] = zo;
@@ -362,6 +364,7 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
   B.cc_kstart = dupload(*this, cc_kstart, err); B.cc_kets = dupload(*this, cc_kets, err);
   B.cc_phstart = dupload(*this, cc_phstart, err); B.cc_ph = dupload(*this, cc_ph, err);
   B.cc_phi = dupload(*this, cc_phi, err);
+  B.cc_phph = dupload(*this, cc_phph, err);
   B.cc_xoff = dupload(*this, cc_xoff, err); B.cc_zoff = dupload(*this, cc_zoff, err);
 
   // ---- one-body pairs and norm weights ----

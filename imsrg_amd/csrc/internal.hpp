@@ -83,6 +83,7 @@ struct BasisDev {
   // cross-coupled channels
   const IMSRG_G int *cc_n, *cc_nph, *cc_kstart, *cc_kets, *cc_phstart, *cc_ph;
   const IMSRG_G signed char *cc_phi;  // (-1)^{(j_p+j_q)} per CC ket
+  const IMSRG_G signed char *cc_phph; // cc_phi of the ket at each position of the cc_ph lists
   const IMSRG_G long *cc_xoff, *cc_zoff;
   long xsize, zsize;
   // kets of each (parity, Tz) group g by (j_k, j_l) class c = ((j2k-1)/2) d1 + (j2l-1)/2: the group

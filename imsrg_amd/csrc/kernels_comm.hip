@@ -545,8 +545,7 @@ __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
   const double *Am, *Bm;
   const int *kl = nullptr;
   const double *kw = nullptr;
-  const int *phl = nullptr;
-  const signed char *phi = nullptr;
+  const signed char *phi = nullptr, *phph = nullptr;
   double alpha, h;
   double *D;
   if constexpr (MODE == 0) {
@@ -574,10 +573,19 @@ __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
     alpha = 1.0;
     h = (MODE == 1) ? (P.hZ == 1 ? 1.0 : -1.0) : 1.0;
     phi = B.cc_phi + B.cc_kstart[cc];
-    phl = B.cc_ph + B.cc_phstart[cc];
+    phph = B.cc_phph + B.cc_phstart[cc];
   }
   const int R0 = it.z * T, S0 = it.w * T;
   const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  // MODE 2: the phases of the tile's 128 rows and 128 columns, staged once for
+  // the epilogue (the prologue's barrier below publishes them)
+  __shared__ signed char s_ph[2][T];
+  if constexpr (MODE == 2) {
+    if (threadIdx.x < 2 * T) {
+      const int sd = threadIdx.x / T, e = threadIdx.x - sd * T;
+      s_ph[sd][e] = phi[min((sd ? S0 : R0) + e, n - 1)];
+    }
+  }
   // 8 waves, each a 64x32 sub-tile: rows wr .. wr+63, columns wc .. wc+31
   const int wr = (wv & 1) * 64, wc = (wv >> 1) * kGW;
   // a wave whose sub-tile holds no upper-triangle element inside the matrix
@@ -637,7 +645,7 @@ __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
     const int k = kk < nk ? kk : kk - nk;
     if constexpr (MODE == 0) return kw[k];
     if constexpr (MODE == 1) return 1.0;
-    return (double)P.hY * (double)phi[phl[k < nph ? k : k - nph]];
+    return (double)P.hY * (double)phph[k < nph ? k : k - nph];
   };
   auto store_w = [&](int k0, double (&Ws)[kTileK], double w) {
     if (threadIdx.x < kTileK) {
@@ -758,7 +766,7 @@ __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
               double v = alpha * acc[mi][ni][q];
               if (MODE == 2) {
                 const int col = S0 + wc + cc;
-                v *= (col < n) ? (double)phi[col] : 0.0;
+                v *= (col < n) ? (double)s_ph[1][wc + cc] : 0.0;
               }
               Tt[rr][cc] = v;
             }
@@ -789,15 +797,16 @@ __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
           const int row2 = S1 + cc, col2 = R1 + rr;
           if (col2 >= n || row2 >= n) continue;
           const bool lo = row2 > col2, hi = row2 + 1 < n && row2 + 1 > col2;
-          const double s0 = (MODE == 1) ? h : (double)(phi[row2] * phi[col2]);
+          // phases from LDS: row2 - S0 and col2 - R0 lie in [0, 128)
+          const double s0 = (MODE == 1) ? h : (double)(s_ph[1][row2 - S0] * s_ph[0][col2 - R0]);
           double *dp = D + (long)col2 * n + row2;
           if (lo && hi) {
-            const double s1 = (MODE == 1) ? h : (double)(phi[row2 + 1] * phi[col2]);
+            const double s1 = (MODE == 1) ? h : (double)(s_ph[1][row2 + 1 - S0] * s_ph[0][col2 - R0]);
             *reinterpret_cast<double2 *>(dp) = double2{s0 * Tt[rr][cc], s1 * Tt[rr][cc + 1]};
           } else if (lo) {
             *dp = s0 * Tt[rr][cc];
           } else if (hi) {
-            const double s1 = (MODE == 1) ? h : (double)(phi[row2 + 1] * phi[col2]);
+            const double s1 = (MODE == 1) ? h : (double)(s_ph[1][row2 + 1 - S0] * s_ph[0][col2 - R0]);
             dp[1] = s1 * Tt[rr][cc + 1];
           }
         }
