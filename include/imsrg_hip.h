@@ -181,7 +181,8 @@ int imsrg_hip_set_graphs(imsrg_ctx *ctx, int on);
  * change only how work is tiled and ordered, and under IMSRG_PREC_MIXED which
  * GEMM family runs on FP32 MFMA; FP64 results agree to rounding. */
 typedef struct {
-  int pphh_small_tile_max;  /* channels with n <= this use 64x64 pp/hh tiles, larger ones 128x128 */
+  int pphh_small_tile_max;  /* channels with n <= this use 64x64 pp/hh tiles, larger ones 128x128
+                               (default 64) */
   int pphh_order;           /* 0: pp/hh tiles by decreasing k-length; 1: row-block groups balanced over XCDs;
                                2: 8x8-tile super-tiles balanced over XCDs (shared operand blocks in one L2) */
   int mixed_pphh_fp32;      /* IMSRG_PREC_MIXED: 1 = pp/hh GEMM on FP32 MFMA, 0 = on FP64 MFMA */
@@ -199,7 +200,8 @@ int imsrg_hip_get_tuning(imsrg_ctx *ctx, imsrg_tuning *t);
 int imsrg_hip_set_tuning(imsrg_ctx *ctx, const imsrg_tuning *t);
 /* Times every distinct setting on seeded synthetic operators of ctx's basis
  * (one warm-up and three timed commutators each, at `precision`) and applies
- * the fastest.  With cache_path != NULL, a setting stored there for the same
+ * the fastest; the default setting is timed first and another replaces the
+ * best so far only when more than 1% faster.  With cache_path != NULL, a setting stored there for the same
  * basis, precision and GPU architecture is applied without timing, and a new
  * result is appended (one text line per entry). */
 int imsrg_hip_autotune(imsrg_ctx *ctx, int precision, const char *cache_path, imsrg_tuning *out);
