@@ -528,7 +528,8 @@ __global__ __launch_bounds__(256) void k_syr2k(PairArg pa, const int4 *items, in
 // Non-temporal stores of Zbar: the inverse Pandya gather reads it only after
 // the whole ph GEMM, so keeping its lines out of the L2 spares that gather
 // the write-back of dirty lines (inverse Pandya 880 -> 835 us, ph GEMM
-// 398 -> 437 us: a net 1% on the overlapped commutator).
+// 398 -> 437 us: a net 1% on the overlapped commutator).  The last kernel's
+// stores of Z use them too (Z is read again only by a later call).
 __device__ __forceinline__ void st_nt(double *p, double v) { __builtin_nontemporal_store(v, p); }
 __device__ __forceinline__ void st_nt2(double *p, double a, double b) {
   typedef double d2v __attribute__((ext_vector_type(2)));
@@ -1328,9 +1329,9 @@ __global__ __launch_bounds__(256, 4) void k_finish(PairArg pa, const int4 *items
     if (ok0 && r == c) v.x = (P.hZ == -1) ? 0.0 : v.x + (FOLD ? dg[r] : 0.0);
     if (ok1 && r + 1 == c) v.y = (P.hZ == -1) ? 0.0 : v.y + (FOLD ? dg[r + 1] : 0.0);
     if (ok1)
-      blk_st2(Z2, (unsigned)(c * n + r), v);
+      st_nt2(&blk_at(Z2, (unsigned)(c * n + r)), v.x, v.y);
     else if (ok0)
-      blk_at(Z2, (unsigned)(c * n + r)) = v.x;
+      st_nt(&blk_at(Z2, (unsigned)(c * n + r)), v.x);
     u[j] = v;
   }
   if constexpr (!MIRROR) return;
@@ -1348,11 +1349,11 @@ __global__ __launch_bounds__(256, 4) void k_finish(PairArg pa, const int4 *items
     const bool ok0 = r < n && c < n && r < c, ok1 = r < n && c + 1 < n && r < c + 1;
     const double2 v = {hZ * t[rr][p2], hZ * t[rr][p2 + 1]};
     if (ok0 && ok1)
-      blk_st2(Z2, (unsigned)(r * n + c), v);
+      st_nt2(&blk_at(Z2, (unsigned)(r * n + c)), v.x, v.y);
     else if (ok0)
-      blk_at(Z2, (unsigned)(r * n + c)) = v.x;
+      st_nt(&blk_at(Z2, (unsigned)(r * n + c)), v.x);
     else if (ok1)
-      blk_at(Z2, (unsigned)(r * n + c + 1)) = v.y;
+      st_nt(&blk_at(Z2, (unsigned)(r * n + c + 1)), v.y);
   }
 }
 
