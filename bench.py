@@ -143,15 +143,16 @@ def workload_batched(args, ia, np):
 
 
 def workload_emax12(args, ia, np):
-    """One commutator at emax=12 (182 orbits): FP64 and mixed rates and the
-    mixed-precision error against the FP64 GPU result."""
+    """One commutator at emax=12 (182 orbits): FP64, mixed and split-product
+    rates and the errors of both reduced-precision modes against the FP64 GPU
+    result."""
     ms = ia.ModelSpace(12, "O16", device=0)
     X = ia.DeviceOperator(ms, -1).fill_random(-1, 12001)
     Y = ia.DeviceOperator(ms, 1).fill_random(1, 12002)
     Z = ia.DeviceOperator(ms, 1)
     res = {}
     hosts = {}
-    for name, prec in (("fp64", ia.PREC_FP64), ("mixed", ia.PREC_MIXED)):
+    for name, prec in (("fp64", ia.PREC_FP64), ("mixed", ia.PREC_MIXED), ("split", ia.PREC_SPLIT)):
         ia.Commutator.CommutatorScalarScalar(X, Y, precision=prec, out=Z)
         ms.synchronize()
         steps = max(3, min(args.steps, 5))
@@ -167,6 +168,8 @@ def workload_emax12(args, ia, np):
         hosts[name] = Z.to_host()
         res[name] = {"commutators_per_s": steps / t, "phase_us": dict(zip(PHASES, [round(x, 1) for x in phase]))}
     rel, de0 = _errors(hosts["mixed"], hosts["fp64"], np)
+    rel_s, de0_s = _errors(hosts["split"], hosts["fp64"], np)
+    res["split"].update({"rel_dZ": rel_s, "abs_dE0": de0_s})
     d = np.abs(hosts["mixed"].TwoBody - hosts["fp64"].TwoBody)
     return {"metric": "mixed-precision error at emax=12 vs the FP64 GPU result", "value": rel,
             "unit": "max-abs dZ / max |Z|", "n_gpus": 1, "higher_is_better": False, "dtype": "f32 MFMA + f64",
@@ -176,7 +179,7 @@ def workload_emax12(args, ia, np):
             "abs_dE0": de0, "max_abs_dZ": float(np.max(d)),
             "max_rel_dZ_elementwise_over_1e-3_max": float(np.max(d / np.maximum(np.abs(hosts["fp64"].TwoBody),
                                                                                 1e-3 * np.max(np.abs(hosts["fp64"].TwoBody))))),
-            "fp64": res["fp64"], "mixed": res["mixed"]}
+            "fp64": res["fp64"], "mixed": res["mixed"], "split": res["split"]}
 
 
 def workload_flow(args, ia, np):

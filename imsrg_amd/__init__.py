@@ -20,10 +20,10 @@ import types
 import numpy as np
 
 from . import _lib
-from ._lib import ALL_TERMS, PREC_FP64, PREC_MIXED, TERMS, HipError, check, lib
+from ._lib import ALL_TERMS, PREC_FP64, PREC_MIXED, PREC_SPLIT, TERMS, HipError, check, lib
 
 __all__ = ["ModelSpace", "Operator", "DeviceOperator", "Commutator", "BCH", "IMSRGSolver", "TERMS", "ALL_TERMS",
-           "PREC_FP64", "PREC_MIXED", "HipError"]
+           "PREC_FP64", "PREC_MIXED", "PREC_SPLIT", "HipError"]
 
 _ELEM = {"H": 1, "He": 2, "Li": 3, "Be": 4, "B": 5, "C": 6, "N": 7, "O": 8, "F": 9, "Ne": 10, "Na": 11, "Mg": 12,
          "Al": 13, "Si": 14, "P": 15, "S": 16, "Cl": 17, "Ar": 18, "K": 19, "Ca": 20}

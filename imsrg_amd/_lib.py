@@ -26,6 +26,7 @@ TERMS = {
 ALL_TERMS = 0xFF
 PREC_FP64 = 0
 PREC_MIXED = 1
+PREC_SPLIT = 2
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)

@@ -448,7 +448,7 @@ static int comm_opts_ok(imsrg_ctx *ctx, const imsrg_comm_opts *o, unsigned *mask
   if (o) {
     if (o->use_imsrg3) return fail(ctx, IMSRG_E_UNSUPPORTED, "IMSRG(3) terms are not provided");
     if (o->term_mask) *mask = o->term_mask & IMSRG_ALL_TERMS;
-    if (o->precision != IMSRG_PREC_FP64 && o->precision != IMSRG_PREC_MIXED)
+    if (o->precision != IMSRG_PREC_FP64 && o->precision != IMSRG_PREC_MIXED && o->precision != IMSRG_PREC_SPLIT)
       return fail(ctx, IMSRG_E_INVALID, "unknown precision mode");
     *prec = o->precision;
   }
@@ -492,8 +492,9 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
                           unsigned mask, int prec, int maxn, int maxnch, long maxn122, int maxjmax, int maxnorb,
                           int maxn1pairs, int maxncls) {
   const Tuning &tu = ctx->db.tune;
-  const int prec_pphh = (prec == IMSRG_PREC_MIXED && tu.mixed_pphh) ? IMSRG_PREC_MIXED : IMSRG_PREC_FP64;
-  const int prec_ph = (prec == IMSRG_PREC_MIXED && tu.mixed_ph) ? IMSRG_PREC_MIXED : IMSRG_PREC_FP64;
+  // split products take both GEMM families whatever the mixed-precision choices
+  const int prec_pphh = (prec == IMSRG_PREC_MIXED && !tu.mixed_pphh) ? IMSRG_PREC_FP64 : prec;
+  const int prec_ph = (prec == IMSRG_PREC_MIXED && !tu.mixed_ph) ? IMSRG_PREC_FP64 : prec;
   hipStream_t s = ctx->stream;
   // With overlap, the pp/hh GEMM and then comm122 (coefficients and an
   // upper-triangle pass) run on the side stream next to the zero-body,
@@ -1090,7 +1091,7 @@ static std::string tuning_key(imsrg_ctx *ctx, int precision) {
 
 extern "C" int imsrg_hip_autotune(imsrg_ctx *ctx, int precision, const char *cache_path, imsrg_tuning *out) {
   if (!ctx) return IMSRG_E_INVALID;
-  if (precision != IMSRG_PREC_FP64 && precision != IMSRG_PREC_MIXED)
+  if (precision != IMSRG_PREC_FP64 && precision != IMSRG_PREC_MIXED && precision != IMSRG_PREC_SPLIT)
     return fail(ctx, IMSRG_E_INVALID, "unknown precision mode");
   const std::string key = tuning_key(ctx, precision);
   if (cache_path) {

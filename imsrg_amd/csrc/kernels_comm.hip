@@ -525,6 +525,10 @@ __global__ __launch_bounds__(256) void k_syr2k(PairArg pa, const int4 *items, in
 // MIXED: operands rounded to FP32 when read from LDS, products on f32
 // 16x16x4 MFMA, FP32 partial sums over two stages (32 k) added into the FP64
 // accumulators.
+// SPLIT (with MIXED): each operand is split into two FP32 values, a = ah + al
+// with ah = fl32(a), al = fl32(a - ah), and a product takes three f32 MFMAs,
+// ah*bh + ah*bl + al*bh (the dropped al*bl is 2^-48 of it); the FP32 partial
+// sums are flushed into FP64 after every 4 k.
 // Non-temporal stores of Zbar: the inverse Pandya gather reads it only after
 // the whole ph GEMM, so keeping its lines out of the L2 spares that gather
 // the write-back of dirty lines (inverse Pandya 880 -> 835 us, ph GEMM
@@ -537,8 +541,9 @@ __device__ __forceinline__ void st_nt2(double *p, double a, double b) {
   __builtin_nontemporal_store(v, reinterpret_cast<d2v *>(p));
 }
 
-template <int MODE, bool MIXED>
+template <int MODE, bool MIXED, bool SPLIT = false>
 __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
+  static_assert(MIXED || !SPLIT, "split products are a mixed-precision mode");
   constexpr int T = kTileB, LD = kTileB + 4;
   // operands stay FP64 in LDS (they arrive by direct global->LDS copies); the
   // mixed kernel rounds them to FP32 when it reads them
@@ -707,18 +712,34 @@ __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
       for (int ks = 0; ks < kTileK / 4; ++ks) {
         const int kr = ks * 4 + (lane >> 4);
         const double wk = Wc[kr];
-        float am[4], bm[kGN];
+        float am[4], bm[kGN], al[4], bl[kGN];
 #pragma unroll
-        for (int m = 0; m < 4; ++m) am[m] = (float)(Ac[kr][wr + m * 16 + (lane & 15)] * wk);
+        for (int m = 0; m < 4; ++m) {
+          const double a = Ac[kr][wr + m * 16 + (lane & 15)] * wk;
+          am[m] = (float)a;
+          if constexpr (SPLIT) al[m] = (float)(a - (double)am[m]);
+        }
 #pragma unroll
-        for (int m = 0; m < kGN; ++m) bm[m] = (float)Bc[kr][wc + m * 16 + (lane & 15)];
+        for (int m = 0; m < kGN; ++m) {
+          const double b = Bc[kr][wc + m * 16 + (lane & 15)];
+          bm[m] = (float)b;
+          if constexpr (SPLIT) bl[m] = (float)(b - (double)bm[m]);
+        }
 #pragma unroll
         for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
-          for (int ni = 0; ni < kGN; ++ni)
+          for (int ni = 0; ni < kGN; ++ni) {
+            if constexpr (SPLIT) {
+              t[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x4f32(bl[ni], am[mi], t[mi][ni], 0, 0, 0);
+              t[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x4f32(bm[ni], al[mi], t[mi][ni], 0, 0, 0);
+            }
             t[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x4f32(bm[ni], am[mi], t[mi][ni], 0, 0, 0);
+          }
+        // split products flush every 4 k: with the operand rounding gone, the
+        // FP32 partial sums over 32 k would set the error
+        if constexpr (SPLIT) flush_t();
       }
-      if (flush) flush_t();
+      if (!SPLIT && flush) flush_t();
     } else {
 #pragma unroll
       for (int ks = 0; ks < kTileK / 4; ++ks) {
@@ -834,9 +855,12 @@ __global__ __launch_bounds__(kGWaves * 64, 4) void k_pphh_big_f64(PairArg pa, co
 __global__ __launch_bounds__(kGWaves * 64, 4) void k_pphh_big_mixed(PairArg pa, const int4 *items) {
   gemm_tile<0, true>(pa, items);
 }
-template <int MODE, bool MIXED>
+__global__ __launch_bounds__(kGWaves * 64, 4) void k_pphh_big_split(PairArg pa, const int4 *items) {
+  gemm_tile<0, true, true>(pa, items);
+}
+template <int MODE, bool MIXED, bool SPLIT = false>
 __global__ __launch_bounds__(kGWaves * 64, 4) void k_ph_big(PairArg pa, const int4 *items) {
-  gemm_tile<MODE, MIXED>(pa, items);
+  gemm_tile<MODE, MIXED, SPLIT>(pa, items);
 }
 
 // ---------------------------------------------------------------------------
@@ -1396,6 +1420,8 @@ hipError_t launch_pphh(const PairArg &pa, const int4 *items, int nitems, const i
     const size_t lds = sizeof(int) * (size_t)std::max(maxn, 1);  // column-index list of the largest channel
     if (precision == 1)
       hipLaunchKernelGGL(k_pphh_big_mixed, dim3(nitems), dim3(kGWaves * 64), lds, s, pa, items);
+    else if (precision == 2)
+      hipLaunchKernelGGL(k_pphh_big_split, dim3(nitems), dim3(kGWaves * 64), lds, s, pa, items);
     else
       hipLaunchKernelGGL(k_pphh_big_f64, dim3(nitems), dim3(kGWaves * 64), lds, s, pa, items);
   }
@@ -1413,6 +1439,9 @@ hipError_t launch_ph_gemm(const PairArg &pa, const int4 *items, int nitems, int 
     if (precision == 1) {
       hipLaunchKernelGGL((k_ph_big<1, true>), dim3(nitems), dim3(kGWaves * 64), 0, s, pa, items);
       hipLaunchKernelGGL((k_ph_big<2, true>), dim3(nitems), dim3(kGWaves * 64), 0, s, pa, items);
+    } else if (precision == 2) {
+      hipLaunchKernelGGL((k_ph_big<1, true, true>), dim3(nitems), dim3(kGWaves * 64), 0, s, pa, items);
+      hipLaunchKernelGGL((k_ph_big<2, true, true>), dim3(nitems), dim3(kGWaves * 64), 0, s, pa, items);
     } else {
       hipLaunchKernelGGL((k_ph_big<1, false>), dim3(nitems), dim3(kGWaves * 64), 0, s, pa, items);
       hipLaunchKernelGGL((k_ph_big<2, false>), dim3(nitems), dim3(kGWaves * 64), 0, s, pa, items);

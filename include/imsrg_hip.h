@@ -53,6 +53,12 @@ extern "C" {
 #define IMSRG_PREC_MIXED 1 /* pp/hh and ph GEMMs: FP32-rounded operands on FP32 MFMA,
                               FP32 partial sums over 32 k (16 k on 64x64 tiles)
                               flushed into FP64 accumulators; everything else FP64 */
+#define IMSRG_PREC_SPLIT 2 /* pp/hh and ph GEMMs on 128x128 tiles: each FP64 operand split into
+                              two FP32 values (hi + lo), three FP32 MFMA products per
+                              product (hi*hi + hi*lo + lo*hi), FP32 partial sums flushed
+                              into FP64 every 4 k; 64x64-tile GEMMs and everything else FP64.
+                              Not faster than FP64 on MI355X (three FP32 MFMAs cost 1.5x
+                              one FP64 MFMA): an accuracy reference for the mixed mode */
 
 typedef struct imsrg_ctx imsrg_ctx; /* basis tables, 6j table, stream, workspaces */
 typedef struct imsrg_dop imsrg_dop; /* device-resident operator */

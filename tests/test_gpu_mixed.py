@@ -93,3 +93,29 @@ def test_mixed_bch_transform_close_to_fp64(ia):
     e0, e1, e2 = errors(omx, o64)
     say(f"  dE0={e0:.3e} rel1={e1:.3e} rel2={e2:.3e}")
     assert e0 < MIXED_E0 and e1 < MIXED_REL and e2 < MIXED_REL
+
+
+# IMSRG_PREC_SPLIT: the 128x128-tile GEMMs take each FP64 operand as two FP32
+# values (hi + lo) and three FP32 MFMA products, which removes the operand
+# rounding that sets the mixed mode's error floor; what is left is the FP32
+# accumulation, which split products flush into FP64 after every 4 k.
+# Measured: 1.9e-8 (emax=4) and 7.8e-9 (emax=10) where the mixed mode gives
+# 7.2e-8 and 9.7e-8.
+SPLIT_REL = 4e-8
+
+
+@pytest.mark.parametrize("emax,sx,sy", [(4, 901, 902), (10, 77, 78)])
+def test_split_products_beat_mixed(ia, emax, sx, sy):
+    ms = ia.ModelSpace(emax, "O16")
+    say(f"[gpu] split vs mixed vs FP64 commutator emax={emax} seeds {sx},{sy}")
+    X = ia.DeviceOperator(ms, -1).fill_random(-1, sx)
+    Y = ia.DeviceOperator(ms, 1).fill_random(1, sy)
+    Z64 = from_dev(ia.Commutator.CommutatorScalarScalar(X, Y, precision=ia.PREC_FP64))
+    Zmx = from_dev(ia.Commutator.CommutatorScalarScalar(X, Y, precision=ia.PREC_MIXED))
+    Zsp = from_dev(ia.Commutator.CommutatorScalarScalar(X, Y, precision=ia.PREC_SPLIT))
+    m0, m1, m2 = errors(Zmx, Z64)
+    s0, s1, s2 = errors(Zsp, Z64)
+    say(f"  mixed dE0={m0:.3e} rel={max(m1, m2):.3e}  split dE0={s0:.3e} rel={max(s1, s2):.3e}")
+    assert s0 < 1e-10 and max(s1, s2) < SPLIT_REL
+    assert max(s1, s2) < 0.5 * max(m1, m2)
+    assert s2 > 0.0  # the FP32 products ran
