@@ -671,7 +671,7 @@ extern "C" int imsrg_hip_commutator_ss(imsrg_ctx *ctx, const imsrg_op_view *X, c
 
 // ---------------------------------------------------------------------------
 // BCH on device operators
-static void bch_defaults(const imsrg_bch_opts *o, imsrg_bch_opts *d) {
+static int bch_defaults(imsrg_ctx *ctx, const imsrg_bch_opts *o, imsrg_bch_opts *d) {
   d->mode = 0;
   d->transform_thr = 1e-9;
   d->product_thr = 1e-4;
@@ -682,8 +682,11 @@ static void bch_defaults(const imsrg_bch_opts *o, imsrg_bch_opts *d) {
     if (o->transform_thr > 0) d->transform_thr = o->transform_thr;
     if (o->product_thr > 0) d->product_thr = o->product_thr;
     if (o->max_terms > 0) d->max_terms = o->max_terms;
+    if (o->precision != IMSRG_PREC_FP64 && o->precision != IMSRG_PREC_MIXED && o->precision != IMSRG_PREC_SPLIT)
+      return fail(ctx, IMSRG_E_INVALID, "unknown precision mode");
     d->precision = o->precision;
   }
+  return IMSRG_OK;
 }
 
 // Standard_BCH_Transform, src/BCH.cc:87-201
@@ -773,7 +776,7 @@ extern "C" int imsrg_hip_bch_transform_dev(imsrg_ctx *ctx, const imsrg_dop *Op, 
   if (Op->ctx != ctx || Omega->ctx != ctx || Out->ctx != ctx) return fail(ctx, IMSRG_E_INVALID, "foreign operator");
   if (Out == Omega) return fail(ctx, IMSRG_E_INVALID, "Out must not alias Omega");
   imsrg_bch_opts o;
-  bch_defaults(opts, &o);
+  if (int e = bch_defaults(ctx, opts, &o)) return e;
   return bch_transform(ctx, Op, Omega, Out, o, ncomm);
 }
 extern "C" int imsrg_hip_bch_product_dev(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y, imsrg_dop *Out,
@@ -781,7 +784,7 @@ extern "C" int imsrg_hip_bch_product_dev(imsrg_ctx *ctx, const imsrg_dop *X, con
   if (!ctx || !X || !Y || !Out) return IMSRG_E_INVALID;
   if (X->ctx != ctx || Y->ctx != ctx || Out->ctx != ctx) return fail(ctx, IMSRG_E_INVALID, "foreign operator");
   imsrg_bch_opts o;
-  bch_defaults(opts, &o);
+  if (int e = bch_defaults(ctx, opts, &o)) return e;
   return bch_product(ctx, X, Y, Out, o, ncomm);
 }
 
@@ -794,7 +797,7 @@ extern "C" int imsrg_hip_bch_transform(imsrg_ctx *ctx, const imsrg_op_view *Op, 
   if ((rc = imsrg_hip_op_upload(ctx->stage[0], Op))) return rc;
   if ((rc = imsrg_hip_op_upload(ctx->stage[1], Omega))) return rc;
   imsrg_bch_opts o;
-  bch_defaults(opts, &o);
+  if (int e = bch_defaults(ctx, opts, &o)) return e;
   rc = bch_transform(ctx, ctx->stage[0], ctx->stage[1], ctx->stage[2], o, ncomm);
   if (rc && rc != IMSRG_E_NOT_CONVERGED) return rc;
   int rc2 = imsrg_hip_op_download(ctx->stage[2], Out);
@@ -809,7 +812,7 @@ extern "C" int imsrg_hip_bch_product(imsrg_ctx *ctx, const imsrg_op_view *X, con
   if ((rc = imsrg_hip_op_upload(ctx->stage[0], X))) return rc;
   if ((rc = imsrg_hip_op_upload(ctx->stage[1], Y))) return rc;
   imsrg_bch_opts o;
-  bch_defaults(opts, &o);
+  if (int e = bch_defaults(ctx, opts, &o)) return e;
   if ((rc = bch_product(ctx, ctx->stage[0], ctx->stage[1], ctx->stage[2], o, ncomm))) return rc;
   return imsrg_hip_op_download(ctx->stage[2], Out);
 }
@@ -830,7 +833,7 @@ extern "C" int imsrg_hip_magnus_flow(imsrg_ctx *ctx, const imsrg_dop *H0, imsrg_
   if (!ctx || !H0 || !Hs || H0 == Hs || H0->ctx != ctx || Hs->ctx != ctx || ds <= 0 || max_steps < 0)
     return IMSRG_E_INVALID;
   imsrg_bch_opts o;
-  bch_defaults(opts, &o);
+  if (int e = bch_defaults(ctx, opts, &o)) return e;
   imsrg_dop *Eta = nullptr, *Om = nullptr, *Om2 = nullptr, *Hsaved = nullptr;
   int rc = IMSRG_OK;
   auto cleanup = [&]() {

@@ -191,7 +191,8 @@ typedef struct {
                                (default 64) */
   int pphh_order;           /* 0: pp/hh tiles by decreasing k-length; 1: row-block groups balanced over XCDs;
                                2: 8x8-tile super-tiles balanced over XCDs (shared operand blocks in one L2) */
-  int mixed_pphh_fp32;      /* IMSRG_PREC_MIXED: 1 = pp/hh GEMM on FP32 MFMA, 0 = on FP64 MFMA */
+  int mixed_pphh_fp32;      /* IMSRG_PREC_MIXED: 1 = pp/hh GEMM on FP32 MFMA, 0 = on FP64 MFMA
+                               (IMSRG_PREC_SPLIT ignores both flags) */
   int mixed_ph_fp32;        /* IMSRG_PREC_MIXED: 1 = ph GEMM on FP32 MFMA, 0 = on FP64 MFMA */
   int ph_tile_128;          /* 1: ph GEMM on 128x128 tiles, 0: on 64x64 tiles */
   int finish_order;         /* tile pairs of the last kernel (fold, comm122, mirror): 0 in channel and

@@ -119,3 +119,16 @@ def test_split_products_beat_mixed(ia, emax, sx, sy):
     assert s0 < 1e-10 and max(s1, s2) < SPLIT_REL
     assert max(s1, s2) < 0.5 * max(m1, m2)
     assert s2 > 0.0  # the FP32 products ran
+
+
+def test_unknown_precision_is_rejected(ia):
+    ms = ia.ModelSpace(2, "O16")
+    say("[gpu] unknown precision modes are rejected")
+    X = ia.DeviceOperator(ms, -1).fill_random(-1, 1)
+    Y = ia.DeviceOperator(ms, 1).fill_random(1, 2)
+    with pytest.raises(ia.HipError):
+        ia.Commutator.CommutatorScalarScalar(X, Y, precision=7)
+    with pytest.raises(ia.HipError):
+        ia.BCH.BCH_Transform(Y, X, precision=7)
+    with pytest.raises(ia.HipError):
+        ia.BCH.BCH_Product(X, X, precision=-1)
