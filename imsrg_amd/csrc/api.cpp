@@ -1134,7 +1134,7 @@ extern "C" int imsrg_hip_autotune(imsrg_ctx *ctx, int precision, const char *cac
     }
   }
   // the default setting first; a later candidate is kept only if it is more than
-  // 1% faster than the best so far, so that timing noise between near-equal
+  // 2% faster than the best so far, so that timing noise between near-equal
   // settings does not decide
   std::vector<Tuning> cand{Tuning{}};
   for (int sm : smalls)
@@ -1168,19 +1168,21 @@ extern "C" int imsrg_hip_autotune(imsrg_ctx *ctx, int precision, const char *cac
     rc = fail(ctx, IMSRG_E_HIP, "hipEventCreate failed");
   Tuning best = ctx->db.tune;
   double best_us = 1e300;
+  constexpr int kTimed = 8;          // timed commutators per setting, after one warm-up
+  constexpr double kMargin = 0.02;   // a setting must beat the best so far by this fraction
   auto trial = [&](const Tuning &t) -> int {  // time one setting; keep it if it is the fastest so far
     int r = apply_tuning(ctx, t);
     if (!r) r = commutator_dev(ctx, X, Y, Z, IMSRG_ALL_TERMS, precision);  // warm-up
     if (r) return r;
     if (hipEventRecord(e0, ctx->stream) != hipSuccess) return fail(ctx, IMSRG_E_HIP, "hipEventRecord");
-    for (int i = 0; i < 3 && !r; ++i) r = commutator_dev(ctx, X, Y, Z, IMSRG_ALL_TERMS, precision);
+    for (int i = 0; i < kTimed && !r; ++i) r = commutator_dev(ctx, X, Y, Z, IMSRG_ALL_TERMS, precision);
     if (r) return r;
     float ms = 0.f;
     if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
         hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
       return fail(ctx, IMSRG_E_HIP, "timing failed");
-    const double us = 1000.0 * ms / 3.0;
-    if (us < 0.99 * best_us) {
+    const double us = 1000.0 * ms / kTimed;
+    if (us < (1.0 - kMargin) * best_us) {
       best_us = us;
       best = t;
     }

@@ -206,9 +206,9 @@ typedef struct {
 int imsrg_hip_get_tuning(imsrg_ctx *ctx, imsrg_tuning *t);
 int imsrg_hip_set_tuning(imsrg_ctx *ctx, const imsrg_tuning *t);
 /* Times every distinct setting on seeded synthetic operators of ctx's basis
- * (one warm-up and three timed commutators each, at `precision`) and applies
+ * (one warm-up and eight timed commutators each, at `precision`) and applies
  * the fastest; the default setting is timed first and another replaces the
- * best so far only when more than 1% faster.  With cache_path != NULL, a setting stored there for the same
+ * best so far only when more than 2% faster.  With cache_path != NULL, a setting stored there for the same
  * basis, precision and GPU architecture is applied without timing, and a new
  * result is appended (one text line per entry). */
 int imsrg_hip_autotune(imsrg_ctx *ctx, int precision, const char *cache_path, imsrg_tuning *out);
