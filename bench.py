@@ -12,7 +12,7 @@ else the FP64 rate; both rates are reported.  ``roofline`` is for the dominant
 kernel, from HIP events recorded on the library stream during the timed steps;
 ``cpu_baseline`` times the oracle (C/OpenMP restatement) on one commutator of
 the same workload on this host.  ``--workload`` selects the other measured
-workloads (batched pairs, emax=12 precision, Magnus flow), one JSON line each.
+workloads (batched pairs, emax=12 and emax=14 precision, Magnus flow), one JSON line each.
 """
 import argparse
 import json
@@ -142,13 +142,13 @@ def workload_batched(args, ia, np):
             "fp64": out["fp64"], "mixed": out["mixed"]}
 
 
-def workload_emax12(args, ia, np):
-    """One commutator at emax=12 (182 orbits): FP64, mixed and split-product
-    rates and the errors of both reduced-precision modes against the FP64 GPU
-    result."""
-    ms = ia.ModelSpace(12, "O16", device=0)
-    X = ia.DeviceOperator(ms, -1).fill_random(-1, 12001)
-    Y = ia.DeviceOperator(ms, 1).fill_random(1, 12002)
+def workload_emax12(args, ia, np, emax=12):
+    """One commutator at emax=12 (182 orbits; or emax=14, 240 orbits): FP64,
+    mixed and split-product rates and the errors of both reduced-precision
+    modes against the FP64 GPU result.  Operators seeded 1000*emax+1 / +2."""
+    ms = ia.ModelSpace(emax, "O16", device=0)
+    X = ia.DeviceOperator(ms, -1).fill_random(-1, 1000 * emax + 1)
+    Y = ia.DeviceOperator(ms, 1).fill_random(1, 1000 * emax + 2)
     Z = ia.DeviceOperator(ms, 1)
     res = {}
     hosts = {}
@@ -171,10 +171,10 @@ def workload_emax12(args, ia, np):
     rel_s, de0_s = _errors(hosts["split"], hosts["fp64"], np)
     res["split"].update({"rel_dZ": rel_s, "abs_dE0": de0_s})
     d = np.abs(hosts["mixed"].TwoBody - hosts["fp64"].TwoBody)
-    return {"metric": "mixed-precision error at emax=12 vs the FP64 GPU result", "value": rel,
+    return {"metric": f"mixed-precision error at emax={emax} vs the FP64 GPU result", "value": rel,
             "unit": "max-abs dZ / max |Z|", "n_gpus": 1, "higher_is_better": False, "dtype": "f32 MFMA + f64",
-            "data": "synthetic (seeds 12001/12002)",
-            "config": {"workload": "scalar IMSRG(2) commutator, HO emax=12, O16 reference", "norb": ms.norb,
+            "data": f"synthetic (seeds {1000 * emax + 1}/{1000 * emax + 2})",
+            "config": {"workload": f"scalar IMSRG(2) commutator, HO emax={emax}, O16 reference", "norb": ms.norb,
                        "two_body_doubles": ms.tbsize},
             "abs_dE0": de0, "max_abs_dZ": float(np.max(d)),
             "max_rel_dZ_elementwise_over_1e-3_max": float(np.max(d / np.maximum(np.abs(hosts["fp64"].TwoBody),
@@ -266,9 +266,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--autotune-cache", default=None,
                     help="text cache of the autotuner: settings found there are applied without timing trials")
-    ap.add_argument("--workload", default="commutator", choices=["commutator", "batched", "emax12", "flow"],
+    ap.add_argument("--workload", default="commutator", choices=["commutator", "batched", "emax12", "emax14", "flow"],
                     help="commutator: the headline metric line; batched: 48 independent pairs at emax 4-7; "
-                         "emax12: mixed-precision error report at emax=12; flow: Magnus flow at emax=10")
+                         "emax12 / emax14: precision report at emax=12 / 14; flow: Magnus flow at emax=10")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X; runs on more than one GPU are not part of this build "
@@ -280,7 +280,8 @@ def main():
     import imsrg_amd as ia
 
     if args.workload != "commutator":
-        line = {"batched": workload_batched, "emax12": workload_emax12, "flow": workload_flow}[args.workload](
+        line = {"batched": workload_batched, "emax12": workload_emax12,
+                "emax14": lambda a, i, n: workload_emax12(a, i, n, emax=14), "flow": workload_flow}[args.workload](
             args, ia, np)
         print(json.dumps(line), flush=True)
         return

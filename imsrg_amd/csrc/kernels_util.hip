@@ -81,15 +81,16 @@ __device__ __forceinline__ bool tri_ok(int a, int b, int c) {
 __device__ double sixj_dd(const dd *F, int a, int b, int c, int d, int e, int f) {
   if (!tri_ok(a, b, c) || !tri_ok(a, e, f) || !tri_ok(d, b, f) || !tri_ok(d, e, c)) return 0.0;
   const int tri[4][3] = {{a, b, c}, {a, e, f}, {d, b, f}, {d, e, c}};
-  dd num = {1.0, 0.0}, den = {1.0, 0.0};
+  // product of the four triangle coefficients, each formed as its own ratio
+  // (at most 1): the four denominators multiplied first overflow a double
+  // for j = 29/2, J = 29 (emax = 14)
+  dd d2 = {1.0, 0.0};
   for (int k = 0; k < 4; ++k) {
     int x = tri[k][0], y = tri[k][1], z = tri[k][2];
-    num = dd_mul(num, F[(x + y - z) / 2]);
-    num = dd_mul(num, F[(x - y + z) / 2]);
-    num = dd_mul(num, F[(-x + y + z) / 2]);
-    den = dd_mul(den, F[(x + y + z) / 2 + 1]);
+    const dd num = dd_mul(dd_mul(F[(x + y - z) / 2], F[(x - y + z) / 2]), F[(-x + y + z) / 2]);
+    d2 = dd_mul(d2, dd_div(num, F[(x + y + z) / 2 + 1]));
   }
-  dd pref = dd_sqrt(dd_div(num, den));
+  dd pref = dd_sqrt(d2);
   const int al[4] = {(a + b + c) / 2, (a + e + f) / 2, (d + b + f) / 2, (d + e + c) / 2};
   const int be[3] = {(a + b + d + e) / 2, (b + c + e + f) / 2, (a + c + d + f) / 2};
   int tmin = max(max(al[0], al[1]), max(al[2], al[3]));

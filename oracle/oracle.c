@@ -157,15 +157,16 @@ double orc_sixj_exact(int a, int b, int c, int d, int e, int f) {
   if (a < 0 || b < 0 || c < 0 || d < 0 || e < 0 || f < 0) return 0.0;
   if (!tri_ok(a, b, c) || !tri_ok(a, e, f) || !tri_ok(d, b, f) || !tri_ok(d, e, c)) return 0.0;
   int tri[4][3] = {{a, b, c}, {a, e, f}, {d, b, f}, {d, e, c}};
-  dd_t num = {1.0, 0.0}, den = {1.0, 0.0};
+  /* product of the four triangle coefficients, each formed as its own ratio
+   * (at most 1): the four denominators multiplied first overflow a double
+   * for j = 29/2, J = 29 (emax = 14) */
+  dd_t d2 = {1.0, 0.0};
   for (int k = 0; k < 4; ++k) {
     int x = tri[k][0], y = tri[k][1], z = tri[k][2];
-    num = dd_mul(num, g_fact[(x + y - z) / 2]);
-    num = dd_mul(num, g_fact[(x - y + z) / 2]);
-    num = dd_mul(num, g_fact[(-x + y + z) / 2]);
-    den = dd_mul(den, g_fact[(x + y + z) / 2 + 1]);
+    dd_t num = dd_mul(dd_mul(g_fact[(x + y - z) / 2], g_fact[(x - y + z) / 2]), g_fact[(-x + y + z) / 2]);
+    d2 = dd_mul(d2, dd_div(num, g_fact[(x + y + z) / 2 + 1]));
   }
-  dd_t pref = dd_sqrt(dd_div(num, den));
+  dd_t pref = dd_sqrt(d2);
   int al[4] = {(a + b + c) / 2, (a + e + f) / 2, (d + b + f) / 2, (d + e + c) / 2};
   int be[3] = {(a + b + d + e) / 2, (b + c + e + f) / 2, (a + c + d + f) / 2};
   int tmin = al[0], tmax = be[0];

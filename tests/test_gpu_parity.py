@@ -7,6 +7,7 @@ matrix element (relative to the largest element of the reference result) and
 """
 import json
 import os
+import random
 import sys
 
 import numpy as np
@@ -269,3 +270,21 @@ def test_batched_matches_single(ia):
         z = from_dev(Z)
         assert z["herm"] == R["herm"]
         assert np.array_equal(z["two"], R["two"]) and np.array_equal(z["one"], R["one"]) and z["zero"] == R["zero"]
+
+
+def test_device_sixj_table_large_j(ia):
+    """emax=14 (j up to 29/2, J up to 29): the device table is finite and
+    equals the oracle's exact 6j, which the CPU tests pin against exact
+    rational arithmetic."""
+    from oracle.pyoracle import sixj_exact
+    ms = ia.ModelSpace(14, "O16")
+    rng = random.Random(14)
+    n = 0
+    while n < 400:
+        a, b, d, e = (2 * rng.randint(0, 14) + 1 for _ in range(4))
+        J3, J6 = rng.randint(0, 29), rng.randint(0, 29)
+        v = ms.sixj(a, b, J3, d, e, J6)
+        ref = sixj_exact(a, b, 2 * J3, d, e, 2 * J6)
+        assert np.isfinite(v)
+        assert abs(v - ref) <= 4e-16 * abs(ref) + 1e-300, (a, b, J3, d, e, J6, v, ref)
+        n += 1

@@ -1,6 +1,7 @@
 """Pins of the oracle's 6j (replaces GSL gsl_sf_coupling_6j, src/AngMom.cc:94-97)
 and of the dense 6j table layout (src/AngMomCache.hh:15-22)."""
 import json
+import math
 import os
 import random
 
@@ -72,3 +73,38 @@ def test_cached_table_equals_exact(oracle_ms_cache):
         jj = [rng.randrange(1, 10, 2) for _ in range(4)]
         J3, J6 = rng.randrange(0, 10), rng.randrange(0, 10)
         assert ms.sixj(jj[0], jj[1], J3, jj[2], jj[3], J6) == sixj_exact(jj[0], jj[1], 2 * J3, jj[2], jj[3], 2 * J6)
+
+
+def _sixj_rational(a, b, c, d, e, f):
+    """Racah formula in exact integer arithmetic: returns (sign * sqrt(P) * S)
+    as a 40-digit Decimal, P and S exact rationals (doubled arguments)."""
+    from decimal import Decimal, getcontext
+    from fractions import Fraction
+    from math import factorial as F
+    getcontext().prec = 40
+    P = Fraction(1)
+    for x, y, z in ((a, b, c), (a, e, f), (d, b, f), (d, e, c)):
+        P *= Fraction(F((x + y - z) // 2) * F((x - y + z) // 2) * F((-x + y + z) // 2), F((x + y + z) // 2 + 1))
+    al = [(a + b + c) // 2, (a + e + f) // 2, (d + b + f) // 2, (d + e + c) // 2]
+    be = [(a + b + d + e) // 2, (b + c + e + f) // 2, (a + c + d + f) // 2]
+    S = Fraction(0)
+    for t in range(max(al), min(be) + 1):
+        den = 1
+        for v in al:
+            den *= F(t - v)
+        for v in be:
+            den *= F(v - t)
+        S += Fraction((-1) ** t * F(t + 1), den)
+    r = (Decimal(P.numerator) / Decimal(P.denominator)).sqrt() * (Decimal(S.numerator) / Decimal(S.denominator))
+    return float(r)
+
+
+@pytest.mark.parametrize("args", [(29, 29, 58, 29, 29, 58), (29, 27, 56, 29, 27, 2), (29, 29, 0, 29, 29, 58),
+                                  (25, 29, 54, 27, 29, 52), (31, 31, 62, 31, 31, 40), (41, 41, 82, 41, 41, 80)])
+def test_sixj_large_arguments_exact(args):
+    """emax = 14 and beyond (j = 29/2 .. 41/2, J up to 41): no overflow, and
+    agreement with exact rational arithmetic."""
+    v = sixj_exact(*args)
+    ref = _sixj_rational(*args)
+    assert math.isfinite(v)
+    assert abs(v - ref) <= 4e-16 * abs(ref) + 1e-300
