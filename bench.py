@@ -10,8 +10,8 @@ Prints one JSON line.  ``value`` is the mixed-precision rate when the mixed
 result meets max|dZ|/max|Z| < 1e-7 and |dE0| < 1e-6 against the FP64 result,
 else the FP64 rate; both rates are reported.  ``roofline`` is for the dominant
 kernel, from HIP events recorded on the library stream during the timed steps;
-``cpu_baseline`` times the oracle (C/OpenMP restatement) on one commutator of
-the same workload on this host.  ``--workload`` selects the other measured
+``cpu_baseline`` times the oracle (C/OpenMP restatement, BLAS GEMMs) on the
+same workload on this host (median of five commutators).  ``--workload`` selects the other measured
 workloads (batched pairs, emax=12 and emax=14 precision, Magnus flow), one JSON line each.
 """
 import argparse
@@ -25,6 +25,8 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64_MFMA = 78.6e12   # FLOP/s, MI355X dense FP64 matrix
 PEAK_FP32_MFMA = 157.3e12  # FLOP/s, MI355X dense FP32 matrix
+PEAK_BF16_MFMA = 2.5e15    # FLOP/s, MI355X dense bf16 matrix
+B3_PRODUCTS = 6            # bf16 MFMA products per FP64 product in the bf16x3 kernels
 PEAK_HBM = 8.0e12          # B/s
 
 PHASES = ["zerobody", "onebody", "comm122", "pphh_gemm", "pandya_gather", "ph_gemm", "inverse_pandya", "fold_mirror"]
@@ -370,10 +372,12 @@ def main():
     dom = int(np.argmax(phase_avg))
     t_dom = phase_avg[dom] * 1e-6
     if PHASES[dom] == "pphh_gemm":
-        # the kernel that ran: FP32 MFMA only when mixed precision and the tuning chose it
-        fp32 = use_mixed and tun["mixed_pphh_fp32"] == 1
-        peak = PEAK_FP32_MFMA if fp32 else PEAK_FP64_MFMA
-        kname = "k_pphh_big_mixed" if fp32 else "k_pphh_big_f64"
+        # the kernel that ran: under mixed precision the tuning picks bf16x3 (2),
+        # FP32 (1) or FP64 (0) MFMA.  The bf16x3 kernel's peak for the FP64
+        # algorithm's flops is the bf16 matrix peak over its six products.
+        fam = tun["mixed_pphh_fp32"] if use_mixed else 0
+        peak = {0: PEAK_FP64_MFMA, 1: PEAK_FP32_MFMA, 2: PEAK_BF16_MFMA / B3_PRODUCTS}[fam]
+        kname = {0: "k_pphh_big_f64", 1: "k_pphh_big_mixed", 2: "k_pphh_big_b3"}[fam]
         ach = FA / t_dom
         roof = {"kernel": kname + (" + k_syr2k<0> (channels n <= %d)" % tun["pphh_small_tile_max"]
                                    if tun["pphh_small_tile_max"] > 0 else ""),
@@ -406,7 +410,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 MFMA products + f64 accumulation" if use_mixed else "f64",
+        "dtype": ({2: "bf16x3 MFMA products + f32/f64 accumulation", 1: "f32 MFMA products + f64 accumulation",
+                   0: "f64"}[tuning["mixed"]["mixed_pphh_fp32"]] if use_mixed else "f64"),
         "data": "synthetic (seeded N(0,1)/sqrt(dim) operators, X anti-Hermitian, Y Hermitian)",
         "config": {"workload": f"scalar IMSRG(2) commutator [X,Y], HO emax={emax}, O16 reference, "
                                f"{ms.norb} orbits", "emax": emax, "norb": ms.norb,
@@ -423,26 +428,41 @@ def main():
         "roofline": roof,
     }
 
-    # ---- CPU baseline: the oracle on one commutator of the same workload ----
+    # ---- CPU baseline: the oracle on the same workload, on this host ----
+    # The oracle restates the reference's OpenMP path; its channel GEMMs go
+    # through BLAS dgemm (scipy's OpenBLAS, one thread per OpenMP thread) as the
+    # reference's go through Armadillo.  One warm-up, then the median of five
+    # commutators, with per-phase times under the reference's timer names.
     if not args.no_cpu_baseline:
         try:
-            sys.path.insert(0, os.path.join(ROOT, "tests"))
-            from oracle.pyoracle import OracleModelSpace, lib as orc_lib
-            orc = OracleModelSpace(emax, 16, 8)
+            from oracle import pyoracle
+            blas = pyoracle.use_blas(True)
+            orc = pyoracle.OracleModelSpace(emax, 16, 8)
             Xh, Yh = X.to_host(), Y.to_host()
             xo = {"zero": Xh.ZeroBody, "one": Xh.OneBody, "two": Xh.TwoBody, "herm": -1}
             yo = {"zero": Yh.ZeroBody, "one": Yh.OneBody, "two": Yh.TwoBody, "herm": 1}
-            t0 = time.perf_counter()
             R = orc.commutator(xo, yo)
-            tc = time.perf_counter() - t0
+            pyoracle.timers(reset=True)
+            samples = []
+            for _ in range(5):
+                t0 = time.perf_counter()
+                R = orc.commutator(xo, yo)
+                samples.append(time.perf_counter() - t0)
+            ncalls, phase_s = pyoracle.timers(reset=True)
+            tc = float(np.median(samples))
             scale = max(np.max(np.abs(R["two"])), np.max(np.abs(R["one"])))
             err = float(max(np.max(np.abs(Zref.TwoBody - R["two"])), np.max(np.abs(Zref.OneBody - R["one"]))) / scale)
-            line["cpu_baseline"] = {"value": 1.0 / tc, "unit": "commutators/s", "cores": orc_lib().orc_num_threads(),
-                                    "kind": "port",
-                                    "sample": f"1 FP64 commutator of the same emax={emax} workload by the "
-                                              f"OpenMP C restatement (oracle/), {tc:.1f} s",
-                                    "gpu_vs_cpu_rel_maxabs": err,
-                                    "gpu_vs_cpu_dE0": abs(Zref.ZeroBody - R["zero"])}
+            line["cpu_baseline"] = {
+                "value": 1.0 / tc, "unit": "commutators/s", "cores": pyoracle.lib().orc_num_threads(),
+                "kind": "port",
+                "sample": f"median of 5 FP64 commutators of the same emax={emax} workload (after one warm-up) by "
+                          "the OpenMP C restatement of the reference path (oracle/), channel GEMMs through BLAS "
+                          f"dgemm ({blas or 'plain loops: scipy BLAS unavailable'})",
+                "samples_s": [round(x, 4) for x in samples],
+                "gemm": f"BLAS dgemm, {blas}" if blas else "plain loops",
+                "phase_us": {k: round(1e6 * v / max(ncalls, 1), 1) for k, v in phase_s.items()},
+                "gpu_vs_cpu_rel_maxabs": err,
+                "gpu_vs_cpu_dE0": abs(Zref.ZeroBody - R["zero"])}
         except Exception as e:  # the baseline is reported, never the measured value
             line["cpu_baseline"] = {"value": None, "unit": "commutators/s", "cores": None, "kind": "port",
                                     "sample": f"failed: {e}"}

@@ -247,6 +247,27 @@ class DeviceOperator:
     def TwoBodyNorm(self):
         return self.norms()[1]
 
+    def WriteBinary(self, path):
+        """Operator::WriteBinary (src/Operator.cc:341-362): the reference's byte layout."""
+        check(lib().imsrg_hip_op_write_binary(self.h, str(path).encode()), self.ms.h)
+
+    def ReadBinary(self, path):
+        """Operator::ReadBinary (src/Operator.cc:364-388)."""
+        check(lib().imsrg_hip_op_read_binary(self.h, str(path).encode()), self.ms.h)
+        return self
+
+    def to_binary(self):
+        n = C.c_long()
+        check(lib().imsrg_hip_op_binary_size(self.h, C.byref(n)), self.ms.h)
+        buf = C.create_string_buffer(n.value)
+        check(lib().imsrg_hip_op_to_binary(self.h, buf, n.value), self.ms.h)
+        return buf.raw
+
+    def from_binary(self, data):
+        buf = C.create_string_buffer(bytes(data), len(data))
+        check(lib().imsrg_hip_op_from_binary(self.h, buf, len(data)), self.ms.h)
+        return self
+
     def close(self):
         if getattr(self, "h", None):
             lib().imsrg_hip_op_destroy(self.h)
@@ -359,4 +380,72 @@ def _magnus(H0, ds=0.5, smax=50.0, eta_criterion=1e-6, omega_norm_max=2.0, max_s
     return Hs, e0[:n].copy(), en[:n + 1].copy(), nc[:n].copy()
 
 
-IMSRGSolver = types.SimpleNamespace(WhiteGenerator=_white, Solve_magnus_euler=_magnus)
+class _Solver:
+    """IMSRGSolver (src/IMSRGSolver.hh) for the Magnus-Euler flow: keeps the
+    list of Omega segments on the device (or in scratch files) so that other
+    operators can be transformed with the same flow."""
+
+    def __init__(self, ms):
+        self.ms = ms
+        h = C.c_void_p()
+        check(lib().imsrg_hip_solver_create(ms.h, C.byref(h)), ms.h)
+        self.h = h
+
+    def SetScratchDir(self, path):
+        check(lib().imsrg_hip_solver_set_scratch(self.h, str(path).encode() if path else None), self.ms.h)
+
+    def Solve_magnus_euler(self, H0, ds=0.5, smax=50.0, eta_criterion=1e-6, omega_norm_max=2.0, max_steps=200,
+                           **kw):
+        ms = self.ms
+        Hs = DeviceOperator(ms, 1)
+        e0 = np.zeros(max_steps + 1)
+        en = np.zeros(max_steps + 1)
+        nc = np.zeros(max_steps + 1, dtype=np.int32)
+        ns = C.c_int()
+        dp = C.POINTER(C.c_double)
+        check(lib().imsrg_hip_solver_magnus_euler(self.h, H0.h, Hs.h, ds, smax, eta_criterion, omega_norm_max,
+                                                  C.byref(_bch_opts(**kw)), max_steps, e0.ctypes.data_as(dp),
+                                                  en.ctypes.data_as(dp), nc.ctypes.data_as(C.POINTER(C.c_int)),
+                                                  C.byref(ns)), ms.h)
+        n = ns.value
+        self.H_s = Hs
+        return Hs, e0[:n].copy(), en[:n + 1].copy(), nc[:n].copy()
+
+    def GetNumberOmega(self):
+        n = C.c_int()
+        check(lib().imsrg_hip_solver_num_omega(self.h, C.byref(n)), self.ms.h)
+        return n.value
+
+    def GetOmega(self, i):
+        out = DeviceOperator(self.ms, -1)
+        check(lib().imsrg_hip_solver_get_omega(self.h, i, out.h), self.ms.h)
+        return out
+
+    def Transform(self, Op, first=0, **kw):
+        """Transform_Partial(Op, first); Transform(Op) is first = 0."""
+        out = DeviceOperator(self.ms, Op.herm)
+        n = C.c_int()
+        check(lib().imsrg_hip_solver_transform(self.h, Op.h, out.h, first, C.byref(_bch_opts(**kw)), C.byref(n)),
+              self.ms.h)
+        return out
+
+    def InverseTransform(self, Op, **kw):
+        out = DeviceOperator(self.ms, Op.herm)
+        n = C.c_int()
+        check(lib().imsrg_hip_solver_inverse_transform(self.h, Op.h, out.h, C.byref(_bch_opts(**kw)), C.byref(n)),
+              self.ms.h)
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().imsrg_hip_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+IMSRGSolver = types.SimpleNamespace(WhiteGenerator=_white, Solve_magnus_euler=_magnus, Solver=_Solver)

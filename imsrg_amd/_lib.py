@@ -100,6 +100,7 @@ SIGNATURES = {
     "imsrg_hip_op_zero": (C.c_int, [_vp]),
     "imsrg_hip_op_norm": (C.c_int, [_vp, _dp, _dp, _dp]),
     "imsrg_hip_commutator_ss_dev": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(CommOpts)]),
+    "imsrg_hip_commutator_ss_dev_async": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(CommOpts), _vp]),
     "imsrg_hip_bch_transform_dev": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(BCHOpts), _ip]),
     "imsrg_hip_bch_product_dev": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(BCHOpts), _ip]),
     "imsrg_hip_commutator_ss_batched": (C.c_int, [C.c_int, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp),
@@ -107,9 +108,25 @@ SIGNATURES = {
     "imsrg_hip_white_generator_dev": (C.c_int, [_vp, _vp, _vp]),
     "imsrg_hip_magnus_flow": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, C.c_double, C.c_double,
                                         C.POINTER(BCHOpts), C.c_int, _dp, _dp, _ip, _ip]),
+    "imsrg_hip_op_binary_size": (C.c_int, [_vp, C.POINTER(C.c_long)]),
+    "imsrg_hip_op_to_binary": (C.c_int, [_vp, _vp, C.c_long]),
+    "imsrg_hip_op_from_binary": (C.c_int, [_vp, _vp, C.c_long]),
+    "imsrg_hip_op_write_binary": (C.c_int, [_vp, C.c_char_p]),
+    "imsrg_hip_op_read_binary": (C.c_int, [_vp, C.c_char_p]),
+    "imsrg_hip_solver_create": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "imsrg_hip_solver_destroy": (C.c_int, [_vp]),
+    "imsrg_hip_solver_set_scratch": (C.c_int, [_vp, C.c_char_p]),
+    "imsrg_hip_solver_magnus_euler": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, C.c_double, C.c_double,
+                                                C.POINTER(BCHOpts), C.c_int, _dp, _dp, _ip, _ip]),
+    "imsrg_hip_solver_num_omega": (C.c_int, [_vp, _ip]),
+    "imsrg_hip_solver_get_omega": (C.c_int, [_vp, C.c_int, _vp]),
+    "imsrg_hip_solver_transform": (C.c_int, [_vp, _vp, _vp, C.c_int, C.POINTER(BCHOpts), _ip]),
+    "imsrg_hip_solver_inverse_transform": (C.c_int, [_vp, _vp, _vp, C.POINTER(BCHOpts), _ip]),
     "imsrg_hip_set_profiling": (C.c_int, [_vp, C.c_int]),
     "imsrg_hip_phase_times": (C.c_int, [_vp, _dp, C.c_int]),
     "imsrg_hip_selftest_mfma": (C.c_int, [_vp, _dp]),
+    "imsrg_hip_selftest_mfma_bf16": (C.c_int, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                               C.POINTER(C.c_float)]),
     "imsrg_hip_set_graphs": (C.c_int, [_vp, C.c_int]),
     "imsrg_hip_get_tuning": (C.c_int, [_vp, C.POINTER(Tuning)]),
     "imsrg_hip_set_tuning": (C.c_int, [_vp, C.POINTER(Tuning)]),

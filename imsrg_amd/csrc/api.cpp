@@ -6,6 +6,11 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
+#include <thread>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -13,6 +18,8 @@
 #include <memory>
 #include <string>
 #include <vector>
+
+#include <unistd.h>
 
 #include "basis.hpp"
 #include "launch.hpp"
@@ -26,7 +33,7 @@ struct imsrg_dop {
 };
 
 struct BatchCache {
-  std::vector<const void *> key;
+  std::vector<long long> key;
   PairPtrs *d_tab = nullptr;
   int4 *d_items = nullptr;
   size_t off[8] = {0};
@@ -43,6 +50,67 @@ struct BatchCache {
     maxn122 = 0;
     key.clear();
   }
+};
+
+// A small pool of host threads for the memcpy side of view transfers:
+// run(n, f) calls f(0) .. f(n-1) on the workers and the caller and returns
+// when all are done.
+class CopyPool {
+ public:
+  explicit CopyPool(int nworkers) {
+    for (int i = 0; i < nworkers; ++i) th_.emplace_back([this] { work(); });
+  }
+  ~CopyPool() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto &t : th_) t.join();
+  }
+  void run(int n, const std::function<void(int)> &f) {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      job_ = &f;
+      n_ = n;
+      next_ = 0;
+      busy_ = (int)th_.size();
+      ++gen_;
+    }
+    cv_.notify_all();
+    for (int i; (i = next_.fetch_add(1)) < n;) f(i);
+    std::unique_lock<std::mutex> lk(mu_);
+    done_.wait(lk, [this] { return busy_ == 0; });
+    job_ = nullptr;
+  }
+
+ private:
+  void work() {
+    long seen = 0;
+    for (;;) {
+      const std::function<void(int)> *job;
+      int n;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+        if (stop_) return;
+        seen = gen_;
+        job = job_;
+        n = n_;
+      }
+      for (int i; (i = next_.fetch_add(1)) < n;) (*job)(i);
+      std::lock_guard<std::mutex> lk(mu_);
+      if (--busy_ == 0) done_.notify_one();
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_;
+  const std::function<void(int)> *job_ = nullptr;
+  int n_ = 0, busy_ = 0;
+  long gen_ = 0;
+  bool stop_ = false;
+  std::atomic<int> next_{0};
 };
 
 // One captured commutator launch sequence (operator buffers, Hermiticity,
@@ -63,6 +131,11 @@ struct imsrg_ctx {
   // ph phases (tuning field overlap_pphh); default priority, like `stream`
   hipStream_t side = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
+  // orders a batched call against the streams of every context in the batch
+  hipEvent_t batch_ev = nullptr;
+  // order imsrg_hip_commutator_ss_dev_async against the caller's stream
+  hipEvent_t async_in = nullptr, async_out = nullptr;
+  long long serial = 0;  // unique per context of the process (batch cache keys)
   HostBasis hb;
   DeviceBasis db;
   double *Xt = nullptr, *Yt = nullptr, *Zbar = nullptr;
@@ -87,7 +160,11 @@ struct imsrg_ctx {
   int phase_n = 0;
   imsrg_dop *stage[3] = {nullptr, nullptr, nullptr};
   imsrg_dop *tmp[4] = {nullptr, nullptr, nullptr, nullptr};
-  double *pinned = nullptr;  // host staging for views
+  // host staging of view transfers: two pinned buffers (the host copy of one
+  // chunk overlaps the DMA of the other) and a pool of host copy threads
+  char *stage_buf[2] = {nullptr, nullptr};
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
+  std::unique_ptr<CopyPool> copy_pool;
   BatchCache batch;
   bool use_graphs = false;
   std::vector<GraphEntry> graphs;  // oldest first
@@ -125,6 +202,11 @@ static int ctx_init(imsrg_ctx *ctx, const imsrg_basis_desc *b, int device) {
   HIPCHK(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
   HIPCHK(ctx, hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming));
   HIPCHK(ctx, hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming));
+  HIPCHK(ctx, hipEventCreateWithFlags(&ctx->batch_ev, hipEventDisableTiming));
+  HIPCHK(ctx, hipEventCreateWithFlags(&ctx->async_in, hipEventDisableTiming));
+  HIPCHK(ctx, hipEventCreateWithFlags(&ctx->async_out, hipEventDisableTiming));
+  static std::atomic<long long> next_serial{1};
+  ctx->serial = next_serial++;
   const BasisDev &B = ctx->db.h;
   // 8 doubles of slack: the GEMM tile loader may read one element past a column
   HIPCHK(ctx, hipMalloc(&ctx->Xt, sizeof(double) * (std::max<long>(B.xsize, 1) + 8)));
@@ -196,7 +278,11 @@ extern "C" int imsrg_hip_ctx_destroy(imsrg_ctx *ctx) {
   if (ctx->partials) (void)hipFree(ctx->partials);
   if (ctx->norm_dev) (void)hipFree(ctx->norm_dev);
   if (ctx->norm_host) (void)hipHostFree(ctx->norm_host);
-  if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+  for (int i = 0; i < 2; ++i) {
+    if (ctx->stage_buf[i]) (void)hipHostFree(ctx->stage_buf[i]);
+    if (ctx->stage_ev[i]) (void)hipEventDestroy(ctx->stage_ev[i]);
+  }
+  ctx->copy_pool.reset();
   for (auto &set : ctx->ring)
     for (auto &ev : set) if (ev) (void)hipEventDestroy(ev);
   for (auto &g : ctx->graphs) (void)hipGraphExecDestroy(g.exec);
@@ -206,6 +292,9 @@ extern "C" int imsrg_hip_ctx_destroy(imsrg_ctx *ctx) {
   if (ctx->side) (void)hipStreamDestroy(ctx->side);
   if (ctx->fork) (void)hipEventDestroy(ctx->fork);
   if (ctx->join) (void)hipEventDestroy(ctx->join);
+  if (ctx->batch_ev) (void)hipEventDestroy(ctx->batch_ev);
+  if (ctx->async_in) (void)hipEventDestroy(ctx->async_in);
+  if (ctx->async_out) (void)hipEventDestroy(ctx->async_out);
   delete ctx;
   return IMSRG_OK;
 }
@@ -293,12 +382,6 @@ extern "C" int imsrg_hip_op_set_herm(imsrg_dop *op, int h) {
   return IMSRG_OK;
 }
 
-static int ensure_pinned(imsrg_ctx *ctx) {
-  if (ctx->pinned) return IMSRG_OK;
-  HIPCHK(ctx, hipHostMalloc(&ctx->pinned, sizeof(double) * ctx->db.h.opsize, hipHostMallocDefault));
-  return IMSRG_OK;
-}
-
 static int view_ok(imsrg_ctx *ctx, const imsrg_op_view *v, bool need_two) {
   if (!v) return fail(ctx, IMSRG_E_INVALID, "null operator view");
   if (v->rank_J != 0 || v->rank_T != 0 || v->parity != 0 || v->reduced)
@@ -312,26 +395,117 @@ static int view_ok(imsrg_ctx *ctx, const imsrg_op_view *v, bool need_two) {
   return IMSRG_OK;
 }
 
+// View transfers.  The pieces of an operator view (ZeroBody, the one-body
+// matrix, one block per channel) map to device ranges of the operator buffer;
+// they are packed into chunks of at most kStageBytes that are contiguous on
+// the device, and each chunk moves through one of two pinned buffers: while the
+// DMA of one chunk runs, the host threads copy the next one.
+constexpr size_t kStageBytes = size_t(32) << 20;
+
+struct Piece {
+  char *host;
+  size_t dev;  // byte offset in the operator buffer
+  size_t bytes;
+};
+struct Chunk {
+  size_t dev = 0, bytes = 0;
+  std::vector<Piece> pieces;  // host ranges, consecutive on the device
+};
+
+static std::vector<Chunk> make_chunks(const std::vector<Piece> &in) {
+  std::vector<Chunk> out;
+  for (Piece p : in) {
+    while (p.bytes > 0) {
+      if (out.empty() || out.back().bytes == kStageBytes || out.back().dev + out.back().bytes != p.dev) {
+        out.emplace_back();
+        out.back().dev = p.dev;
+      }
+      Chunk &c = out.back();
+      const size_t take = std::min(p.bytes, kStageBytes - c.bytes);
+      c.pieces.push_back(Piece{p.host, p.dev, take});
+      c.bytes += take;
+      p.host += take;
+      p.dev += take;
+      p.bytes -= take;
+    }
+  }
+  return out;
+}
+
+static int ensure_staging(imsrg_ctx *ctx) {
+  for (int i = 0; i < 2; ++i) {
+    if (!ctx->stage_buf[i]) HIPCHK(ctx, hipHostMalloc((void **)&ctx->stage_buf[i], kStageBytes, hipHostMallocDefault));
+    if (!ctx->stage_ev[i]) HIPCHK(ctx, hipEventCreateWithFlags(&ctx->stage_ev[i], hipEventDisableTiming));
+  }
+  if (!ctx->copy_pool) {
+    int nt = 7;  // workers besides the calling thread
+    if (const char *e = std::getenv("IMSRG_HIP_COPY_THREADS")) nt = std::max(0, std::atoi(e) - 1);
+    ctx->copy_pool.reset(new CopyPool(nt));
+  }
+  return IMSRG_OK;
+}
+
+// host memcpy of one chunk's pieces (to_stage: host -> pinned, else pinned -> host),
+// split into parts of about 1 MiB over the copy threads
+static void copy_chunk(imsrg_ctx *ctx, const Chunk &c, char *stage, bool to_stage) {
+  struct Part {
+    char *host;
+    char *pinned;
+    size_t bytes;
+  };
+  std::vector<Part> parts;
+  constexpr size_t kPart = size_t(1) << 20;
+  for (const Piece &p : c.pieces)
+    for (size_t o = 0; o < p.bytes; o += kPart)
+      parts.push_back(Part{p.host + o, stage + (p.dev - c.dev) + o, std::min(kPart, p.bytes - o)});
+  ctx->copy_pool->run((int)parts.size(), [&](int i) {
+    const Part &q = parts[i];
+    if (to_stage)
+      std::memcpy(q.pinned, q.host, q.bytes);
+    else
+      std::memcpy(q.host, q.pinned, q.bytes);
+  });
+}
+
+static std::vector<Piece> view_pieces(imsrg_ctx *ctx, const imsrg_op_view *v, double *zero, bool two) {
+  const BasisDev &B = ctx->db.h;
+  std::vector<Piece> pc;
+  pc.push_back(Piece{(char *)zero, 0, sizeof(double)});
+  pc.push_back(Piece{(char *)v->one_body, sizeof(double) * B.one_off, sizeof(double) * B.norb * B.norb});
+  if (two)
+    for (int c = 0; c < ctx->hb.nch; ++c) {
+      const size_t n = ctx->hb.ch[c].kets.size();
+      if (n == 0 || !v->two_body[c]) continue;
+      pc.push_back(Piece{(char *)v->two_body[c], sizeof(double) * (B.two_off + ctx->hb.ch_off[c]),
+                         sizeof(double) * n * n});
+    }
+  return pc;
+}
+
 extern "C" int imsrg_hip_op_upload(imsrg_dop *op, const imsrg_op_view *v) {
   if (!op) return IMSRG_E_INVALID;
   imsrg_ctx *ctx = op->ctx;
   int rc = view_ok(ctx, v, true);
   if (rc) return rc;
-  rc = ensure_pinned(ctx);
-  if (rc) return rc;
   const BasisDev &B = ctx->db.h;
-  double *h = ctx->pinned;
-  std::memset(h, 0, sizeof(double) * B.opsize);
-  h[0] = v->zero_body;
-  std::memcpy(h + B.one_off, v->one_body, sizeof(double) * B.norb * B.norb);
-  if (v->particle_rank == 2)
-    for (int c = 0; c < ctx->hb.nch; ++c) {
-      size_t n = ctx->hb.ch[c].kets.size();
-      if (n == 0) continue;
-      if (!v->two_body[c]) return fail(ctx, IMSRG_E_INVALID, "null two-body block for a non-empty channel");
-      std::memcpy(h + B.two_off + ctx->hb.ch_off[c], v->two_body[c], sizeof(double) * n * n);
-    }
-  HIPCHK(ctx, hipMemcpyAsync(op->buf, h, sizeof(double) * B.opsize, hipMemcpyHostToDevice, ctx->stream));
+  const bool two = v->particle_rank == 2;
+  if (two)
+    for (int c = 0; c < ctx->hb.nch; ++c)
+      if (!ctx->hb.ch[c].kets.empty() && !v->two_body[c])
+        return fail(ctx, IMSRG_E_INVALID, "null two-body block for a non-empty channel");
+  if ((rc = ensure_staging(ctx))) return rc;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  double zero = v->zero_body;
+  if (!two) HIPCHK(ctx, launch_fill(op->buf + B.two_off, 0.0, B.tbsize, ctx->stream));
+  const std::vector<Chunk> ch = make_chunks(view_pieces(ctx, v, &zero, two));
+  char *dev = reinterpret_cast<char *>(op->buf);
+  for (size_t c = 0; c < ch.size(); ++c) {
+    const int b = c & 1;
+    if (c >= 2) HIPCHK(ctx, hipEventSynchronize(ctx->stage_ev[b]));  // its previous DMA is done
+    copy_chunk(ctx, ch[c], ctx->stage_buf[b], true);
+    HIPCHK(ctx, hipMemcpyAsync(dev + ch[c].dev, ctx->stage_buf[b], ch[c].bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipEventRecord(ctx->stage_ev[b], ctx->stream));
+  }
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   op->herm = v->herm;
   return IMSRG_OK;
@@ -340,20 +514,26 @@ extern "C" int imsrg_hip_op_upload(imsrg_dop *op, const imsrg_op_view *v) {
 extern "C" int imsrg_hip_op_download(imsrg_dop *op, imsrg_op_view *v) {
   if (!op || !v || !v->one_body) return IMSRG_E_INVALID;
   imsrg_ctx *ctx = op->ctx;
-  int rc = ensure_pinned(ctx);
-  if (rc) return rc;
-  const BasisDev &B = ctx->db.h;
-  double *h = ctx->pinned;
-  HIPCHK(ctx, hipMemcpyAsync(h, op->buf, sizeof(double) * B.opsize, hipMemcpyDeviceToHost, ctx->stream));
+  if (int rc = ensure_staging(ctx)) return rc;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  double zero = 0.0;
+  const std::vector<Chunk> ch = make_chunks(view_pieces(ctx, v, &zero, v->two_body != nullptr));
+  const char *dev = reinterpret_cast<const char *>(op->buf);
+  auto issue = [&](size_t c) -> hipError_t {
+    const int b = c & 1;
+    hipError_t e = hipMemcpyAsync(ctx->stage_buf[b], dev + ch[c].dev, ch[c].bytes, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(ctx->stage_ev[b], ctx->stream);
+    return e;
+  };
+  if (!ch.empty()) HIPCHK(ctx, issue(0));
+  for (size_t c = 0; c < ch.size(); ++c) {
+    // the other buffer's host copy (chunk c - 1) is done: its next DMA may start
+    if (c + 1 < ch.size()) HIPCHK(ctx, issue(c + 1));
+    HIPCHK(ctx, hipEventSynchronize(ctx->stage_ev[c & 1]));
+    copy_chunk(ctx, ch[c], ctx->stage_buf[c & 1], false);
+  }
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  v->zero_body = h[0];
-  std::memcpy(v->one_body, h + B.one_off, sizeof(double) * B.norb * B.norb);
-  if (v->two_body)
-    for (int c = 0; c < ctx->hb.nch; ++c) {
-      size_t n = ctx->hb.ch[c].kets.size();
-      if (n == 0 || !v->two_body[c]) continue;
-      std::memcpy(v->two_body[c], h + B.two_off + ctx->hb.ch_off[c], sizeof(double) * n * n);
-    }
+  v->zero_body = zero;
   v->herm = op->herm;
   return IMSRG_OK;
 }
@@ -419,7 +599,7 @@ extern "C" int imsrg_hip_op_axpy(imsrg_dop *y, double a, const imsrg_dop *x) {
 }
 extern "C" int imsrg_hip_op_zero(imsrg_dop *op) {
   if (!op) return IMSRG_E_INVALID;
-  HIPCHK(op->ctx, launch_scale(op->buf, 0.0, op->ctx->db.h.opsize, op->ctx->stream));
+  HIPCHK(op->ctx, launch_fill(op->buf, 0.0, op->ctx->db.h.opsize, op->ctx->stream));
   return IMSRG_OK;
 }
 
@@ -493,8 +673,11 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
                           int maxn1pairs, int maxncls) {
   const Tuning &tu = ctx->db.tune;
   // split products take both GEMM families whatever the mixed-precision choices
-  const int prec_pphh = (prec == IMSRG_PREC_MIXED && !tu.mixed_pphh) ? IMSRG_PREC_FP64 : prec;
-  const int prec_ph = (prec == IMSRG_PREC_MIXED && !tu.mixed_ph) ? IMSRG_PREC_FP64 : prec;
+  // GEMM kernel family of each phase: under mixed precision the tuning picks
+  // FP64 (0), FP32 MFMA (1) or bf16x3 MFMA (2)
+  auto family = [&](int flag) { return prec != IMSRG_PREC_MIXED ? prec : flag == 0 ? 0 : flag == 1 ? 1 : kGemmB3; };
+  const int prec_pphh = family(tu.mixed_pphh);
+  const int prec_ph = family(tu.mixed_ph);
   hipStream_t s = ctx->stream;
   // With overlap, the pp/hh GEMM and then comm122 (coefficients and an
   // upper-triangle pass) run on the side stream next to the zero-body,
@@ -561,8 +744,31 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
   return IMSRG_OK;
 }
 
+// Per-call workspaces of one commutator inside one buffer of `doubles` doubles
+// (the layout of the batched path's per-pair workspaces).
+struct Workspace {
+  double *Xt, *Yt, *Zbar, *scratch;
+  double2 *c122;
+  double *obpart, *diag;
+};
+static long workspace_doubles(const BasisDev &B) {
+  return 2 * r8(B.xsize + 8) + r8(B.zsize) + r8(B.nch + 8) + 2 * r8(B.n122) + r8(B.nobpart) + r8(B.ndiag) + 64;
+}
+static Workspace carve_workspace(const BasisDev &B, double *base) {
+  Workspace w;
+  // 8 doubles of slack after X and Y bar: the GEMM tile loader may read one element past a column
+  w.Xt = base;
+  w.Yt = w.Xt + r8(B.xsize + 8);
+  w.Zbar = w.Yt + r8(B.xsize + 8);
+  w.scratch = w.Zbar + r8(B.zsize);
+  w.c122 = (double2 *)(w.scratch + r8(B.nch + 8));
+  w.obpart = (double *)(w.c122) + 2 * r8(B.n122);
+  w.diag = w.obpart + r8(B.nobpart);
+  return w;
+}
+
 static int commutator_dev(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y, imsrg_dop *Z, unsigned mask,
-                          int prec) {
+                          int prec, const Workspace *ws = nullptr) {
   if (X->herm != 1 && X->herm != -1) return fail(ctx, IMSRG_E_UNSUPPORTED, "X must be Hermitian or anti-Hermitian");
   if (Y->herm != 1 && Y->herm != -1) return fail(ctx, IMSRG_E_UNSUPPORTED, "Y must be Hermitian or anti-Hermitian");
   if (Z == X || Z == Y) return fail(ctx, IMSRG_E_INVALID, "Z must not alias X or Y");
@@ -573,13 +779,13 @@ static int commutator_dev(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y
   pa.p0.X = X->buf;
   pa.p0.Y = Y->buf;
   pa.p0.Z = Z->buf;
-  pa.p0.Xt = ctx->Xt;
-  pa.p0.Yt = ctx->Yt;
-  pa.p0.Zbar = ctx->Zbar;
-  pa.p0.scratch = ctx->scratch;
-  pa.p0.c122 = ctx->c122;
-  pa.p0.obpart = ctx->obpart;
-  pa.p0.diag = ctx->diag;
+  pa.p0.Xt = ws ? ws->Xt : ctx->Xt;
+  pa.p0.Yt = ws ? ws->Yt : ctx->Yt;
+  pa.p0.Zbar = ws ? ws->Zbar : ctx->Zbar;
+  pa.p0.scratch = ws ? ws->scratch : ctx->scratch;
+  pa.p0.c122 = ws ? ws->c122 : ctx->c122;
+  pa.p0.obpart = ws ? ws->obpart : ctx->obpart;
+  pa.p0.diag = ws ? ws->diag : ctx->diag;
   pa.p0.hX = X->herm;
   pa.p0.hY = Y->herm;
   pa.p0.hZ = herm_rule(X->herm, Y->herm);
@@ -588,7 +794,7 @@ static int commutator_dev(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y
   const int4 *items[8] = {db.d_pphh, db.d_ph, db.d_ob221, db.d_pandya, db.d_invp, db.d_one, db.d_mirror, db.d_pphh64};
   int cnt[8] = {(int)db.wl.pphh.size(), (int)db.wl.ph.size(), (int)db.wl.ob221.size(), (int)db.wl.pandya.size(),
                 (int)db.wl.invp.size(), (int)db.wl.one.size(), (int)db.wl.mirror.size(), (int)db.wl.pphh64.size()};
-  if (!ctx->use_graphs || ctx->profiling)
+  if (!ctx->use_graphs || ctx->profiling || ws)
     return run_commutator(ctx, pa, 1, items, cnt, mask, prec, db.h.maxn, db.h.nch, db.h.n122, db.h.jmax,
                           db.h.norb, db.h.n1pairs, db.h.ncls);
   // replay a captured launch sequence for this operand set, capturing it on first use
@@ -644,6 +850,24 @@ extern "C" int imsrg_hip_commutator_ss_dev(imsrg_ctx *ctx, const imsrg_dop *X, c
   return commutator_dev(ctx, X, Y, Z, mask, prec);
 }
 
+extern "C" int imsrg_hip_commutator_ss_dev_async(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y,
+                                                 imsrg_dop *Z, const imsrg_comm_opts *opts, void *stream) {
+  if (!ctx || !X || !Y || !Z) return IMSRG_E_INVALID;
+  if (X->ctx != ctx || Y->ctx != ctx || Z->ctx != ctx) return fail(ctx, IMSRG_E_INVALID, "operator from another context");
+  unsigned mask;
+  int prec;
+  int rc = comm_opts_ok(ctx, opts, &mask, &prec);
+  if (rc) return rc;
+  hipStream_t cs = static_cast<hipStream_t>(stream);
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, hipEventRecord(ctx->async_in, cs));
+  HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ctx->async_in, 0));
+  if ((rc = commutator_dev(ctx, X, Y, Z, mask, prec))) return rc;
+  HIPCHK(ctx, hipEventRecord(ctx->async_out, ctx->stream));
+  HIPCHK(ctx, hipStreamWaitEvent(cs, ctx->async_out, 0));
+  return IMSRG_OK;
+}
+
 static int ensure_ops(imsrg_ctx *ctx, imsrg_dop **arr, int n) {
   for (int i = 0; i < n; ++i)
     if (!arr[i]) {
@@ -678,6 +902,7 @@ static int bch_defaults(imsrg_ctx *ctx, const imsrg_bch_opts *o, imsrg_bch_opts 
   d->max_terms = 40;
   d->precision = IMSRG_PREC_FP64;
   if (o) {
+    if (o->mode != 0 && o->mode != 1) return fail(ctx, IMSRG_E_INVALID, "unknown BCH truncation mode (0 or 1)");
     d->mode = o->mode;
     if (o->transform_thr > 0) d->transform_thr = o->transform_thr;
     if (o->product_thr > 0) d->product_thr = o->product_thr;
@@ -820,26 +1045,243 @@ extern "C" int imsrg_hip_bch_product(imsrg_ctx *ctx, const imsrg_op_view *X, con
 // ---------------------------------------------------------------------------
 extern "C" int imsrg_hip_white_generator_dev(imsrg_ctx *ctx, const imsrg_dop *H, imsrg_dop *Eta) {
   if (!ctx || !H || !Eta || H->ctx != ctx || Eta->ctx != ctx || H == Eta) return IMSRG_E_INVALID;
-  HIPCHK(ctx, launch_scale(Eta->buf, 0.0, ctx->db.h.opsize, ctx->stream));
+  HIPCHK(ctx, launch_fill(Eta->buf, 0.0, ctx->db.h.opsize, ctx->stream));
   HIPCHK(ctx, launch_white(ctx->db.d, ctx->db.h, H->buf, Eta->buf, ctx->db.d_gen, (int)ctx->db.wl.gen.size(),
                            ctx->stream));
   Eta->herm = -1;
   return IMSRG_OK;
 }
 
-extern "C" int imsrg_hip_magnus_flow(imsrg_ctx *ctx, const imsrg_dop *H0, imsrg_dop *Hs, double ds, double smax,
-                                     double eta_crit, double omega_norm_max, const imsrg_bch_opts *opts,
-                                     int max_steps, double *e0, double *eta_norm, int *ncomm, int *nsteps) {
-  if (!ctx || !H0 || !Hs || H0 == Hs || H0->ctx != ctx || Hs->ctx != ctx || ds <= 0 || max_steps < 0)
-    return IMSRG_E_INVALID;
-  imsrg_bch_opts o;
-  if (int e = bch_defaults(ctx, opts, &o)) return e;
-  imsrg_dop *Eta = nullptr, *Om = nullptr, *Om2 = nullptr, *Hsaved = nullptr;
+// ---------------------------------------------------------------------------
+// Operator binary format: Operator::WriteBinary / ReadBinary
+// (src/Operator.cc:341-388) with TwoBodyME::WriteBinary / ReadBinary
+// (src/TwoBodyME.cc:812-838).  Fields are written one after another, with no
+// padding: int rank_J, rank_T, parity, particle_rank, legs, E2max, E3max;
+// bool hermitian, antihermitian; int nChannels; double ZeroBody; the norb x
+// norb one-body matrix (column-major); then, for legs > 3, size_t nChannels,
+// bool hermitian, antihermitian, int rank_J, rank_T, parity and every
+// (ch, ch) block of TwoBodyME::MatEl in map order (ascending ch, column-major).
+namespace {
+struct BinHeader {
+  int rank_J = 0, rank_T = 0, parity = 0, particle_rank = 2, legs = 4, E2max = 0, E3max = 0;
+  bool herm = false, antiherm = false;
+  int nch = 0;
+};
+constexpr long kBinOpHead = 7 * 4 + 2 + 4;       // Operator fields before ZeroBody
+constexpr long kBinTwoHead = 8 + 2 + 3 * 4;      // TwoBodyME fields before the blocks
+template <class T>
+void put(std::vector<char> &b, long &o, const T &v) {
+  std::memcpy(b.data() + o, &v, sizeof(T));
+  o += sizeof(T);
+}
+template <class T>
+T get(const char *b, long &o) {
+  T v;
+  std::memcpy(&v, b + o, sizeof(T));
+  o += sizeof(T);
+  return v;
+}
+}  // namespace
+
+static long binary_size(const imsrg_ctx *ctx) {
+  const BasisDev &B = ctx->db.h;
+  return kBinOpHead + 8 + 8L * B.norb * B.norb + kBinTwoHead + 8L * B.tbsize;
+}
+
+extern "C" int imsrg_hip_op_binary_size(const imsrg_dop *op, long *bytes) {
+  if (!op || !bytes) return IMSRG_E_INVALID;
+  *bytes = binary_size(op->ctx);
+  return IMSRG_OK;
+}
+
+extern "C" int imsrg_hip_op_to_binary(imsrg_dop *op, void *buf, long size) {
+  if (!op || !buf) return IMSRG_E_INVALID;
+  imsrg_ctx *ctx = op->ctx;
+  const BasisDev &B = ctx->db.h;
+  const long need = binary_size(ctx);
+  if (size < need) return fail(ctx, IMSRG_E_INVALID, "binary buffer too small");
+  std::vector<char> b(kBinOpHead + kBinTwoHead + 8);
+  long o = 0;
+  const int emax = ctx->hb.emax;
+  for (int v : {0, 0, 0, 2, 4, 2 * emax, std::min(14, 3 * emax)}) put(b, o, v);  // rank_J .. E3max
+  put(b, o, (bool)(op->herm == 1));
+  put(b, o, (bool)(op->herm == -1));
+  put(b, o, (int)B.nch);
+  char *out = static_cast<char *>(buf);
+  std::memcpy(out, b.data(), o);
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, hipMemcpyAsync(out + o, op->buf, 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(out + o + 8, op->buf + B.one_off, 8L * B.norb * B.norb, hipMemcpyDeviceToHost,
+                             ctx->stream));
+  long t = o + 8 + 8L * B.norb * B.norb;
+  long o2 = 0;
+  put(b, o2, (size_t)B.nch);
+  put(b, o2, (bool)(op->herm == 1));
+  put(b, o2, (bool)(op->herm == -1));
+  for (int v : {0, 0, 0}) put(b, o2, v);  // rank_J, rank_T, parity
+  std::memcpy(out + t, b.data(), o2);
+  HIPCHK(ctx, hipMemcpyAsync(out + t + o2, op->buf + B.two_off, 8L * B.tbsize, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return IMSRG_OK;
+}
+
+extern "C" int imsrg_hip_op_from_binary(imsrg_dop *op, const void *buf, long size) {
+  if (!op || !buf) return IMSRG_E_INVALID;
+  imsrg_ctx *ctx = op->ctx;
+  const BasisDev &B = ctx->db.h;
+  const char *in = static_cast<const char *>(buf);
+  if (size < kBinOpHead + 8 + 8L * B.norb * B.norb) return fail(ctx, IMSRG_E_INVALID, "binary operator truncated");
+  long o = 0;
+  BinHeader h;
+  h.rank_J = get<int>(in, o); h.rank_T = get<int>(in, o); h.parity = get<int>(in, o);
+  h.particle_rank = get<int>(in, o); h.legs = get<int>(in, o); h.E2max = get<int>(in, o); h.E3max = get<int>(in, o);
+  h.herm = get<bool>(in, o); h.antiherm = get<bool>(in, o);
+  h.nch = get<int>(in, o);
+  if (h.rank_J != 0 || h.rank_T != 0 || h.parity != 0)
+    return fail(ctx, IMSRG_E_UNSUPPORTED, "binary operator is not a parity-conserving scalar");
+  if (h.legs != 2 && h.legs != 4) return fail(ctx, IMSRG_E_UNSUPPORTED, "binary operator has a particle rank above 2");
+  if (h.nch != B.nch) return fail(ctx, IMSRG_E_INVALID, "binary operator has another channel count");
+  const int herm = h.herm ? 1 : (h.antiherm ? -1 : 0);
+  const long n1 = 8L * B.norb * B.norb;
+  const long two_at = o + 8 + n1;
+  if (h.legs > 3) {
+    if (size < two_at + kBinTwoHead + 8L * B.tbsize) return fail(ctx, IMSRG_E_INVALID, "binary operator truncated");
+    long t = two_at;
+    const size_t nch2 = get<size_t>(in, t);
+    get<bool>(in, t);
+    get<bool>(in, t);
+    const int rj = get<int>(in, t), rt = get<int>(in, t), par = get<int>(in, t);
+    if (nch2 != (size_t)B.nch || rj != 0 || rt != 0 || par != 0)
+      return fail(ctx, IMSRG_E_INVALID, "binary two-body header does not match the basis");
+  }
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, launch_fill(op->buf, 0.0, B.opsize, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(op->buf, in + o, 8, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(op->buf + B.one_off, in + o + 8, n1, hipMemcpyHostToDevice, ctx->stream));
+  if (h.legs > 3)
+    HIPCHK(ctx, hipMemcpyAsync(op->buf + B.two_off, in + two_at + kBinTwoHead, 8L * B.tbsize, hipMemcpyHostToDevice,
+                               ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  op->herm = herm;
+  return IMSRG_OK;
+}
+
+extern "C" int imsrg_hip_op_write_binary(imsrg_dop *op, const char *path) {
+  if (!op || !path) return IMSRG_E_INVALID;
+  imsrg_ctx *ctx = op->ctx;
+  std::vector<char> b(binary_size(ctx));
+  if (int rc = imsrg_hip_op_to_binary(op, b.data(), (long)b.size())) return rc;
+  FILE *f = std::fopen(path, "wb");
+  if (!f) return fail(ctx, IMSRG_E_INVALID, std::string("cannot open ") + path + " for writing");
+  const size_t w = std::fwrite(b.data(), 1, b.size(), f);
+  const int c = std::fclose(f);
+  if (w != b.size() || c != 0) return fail(ctx, IMSRG_E_INVALID, std::string("short write to ") + path);
+  return IMSRG_OK;
+}
+
+extern "C" int imsrg_hip_op_read_binary(imsrg_dop *op, const char *path) {
+  if (!op || !path) return IMSRG_E_INVALID;
+  imsrg_ctx *ctx = op->ctx;
+  FILE *f = std::fopen(path, "rb");
+  if (!f) return fail(ctx, IMSRG_E_INVALID, std::string("cannot open ") + path);
+  std::vector<char> b;
+  char chunk[1 << 16];
+  size_t r;
+  while ((r = std::fread(chunk, 1, sizeof(chunk), f)) > 0) b.insert(b.end(), chunk, chunk + r);
+  std::fclose(f);
+  return imsrg_hip_op_from_binary(op, b.data(), (long)b.size());
+}
+
+// ---------------------------------------------------------------------------
+// IMSRGSolver: the Magnus flow with its list of Omega segments
+// (src/IMSRGSolver.cc:44-84 NewOmega, :236-321 Solve_magnus_euler,
+// :928-1004 Transform / Transform_Partial, :939-951 InverseTransform).
+struct imsrg_solver {
+  imsrg_ctx *ctx;
+  std::vector<imsrg_dop *> omega;  // finished segments held in HBM (nullptr when written to scratch)
+  std::vector<std::string> files;  // segment i written to files[i] (scratch mode), else ""
+  imsrg_dop *current = nullptr;    // the segment being accumulated (Omega.back())
+  std::string scratch;             // SetScratchDir: finished segments go to files
+  int written = 0;
+};
+
+static void solver_clear(imsrg_solver *sv) {
+  for (imsrg_dop *o : sv->omega) imsrg_hip_op_destroy(o);
+  for (const std::string &f : sv->files)
+    if (!f.empty()) std::remove(f.c_str());
+  sv->omega.clear();
+  sv->files.clear();
+  imsrg_hip_op_destroy(sv->current);
+  sv->current = nullptr;
+}
+
+extern "C" int imsrg_hip_solver_create(imsrg_ctx *ctx, imsrg_solver **out) {
+  if (!ctx || !out) return IMSRG_E_INVALID;
+  *out = new imsrg_solver{ctx, {}, {}, nullptr, "", 0};
+  return IMSRG_OK;
+}
+extern "C" int imsrg_hip_solver_destroy(imsrg_solver *sv) {
+  if (!sv) return IMSRG_OK;
+  solver_clear(sv);
+  delete sv;
+  return IMSRG_OK;
+}
+extern "C" int imsrg_hip_solver_set_scratch(imsrg_solver *sv, const char *dir) {
+  if (!sv) return IMSRG_E_INVALID;
+  sv->scratch = dir ? dir : "";
+  return IMSRG_OK;
+}
+extern "C" int imsrg_hip_solver_num_omega(const imsrg_solver *sv, int *n) {
+  if (!sv || !n) return IMSRG_E_INVALID;
+  *n = (int)sv->omega.size() + (sv->current ? 1 : 0);
+  return IMSRG_OK;
+}
+// segment i into out (the last one is the current segment)
+static int solver_omega(imsrg_solver *sv, int i, imsrg_dop *out) {
+  imsrg_ctx *ctx = sv->ctx;
+  const int nfin = (int)sv->omega.size();
+  if (i == nfin && sv->current) return imsrg_hip_op_copy(out, sv->current);
+  if (i < 0 || i >= nfin) return fail(ctx, IMSRG_E_INVALID, "Omega index out of range");
+  if (sv->omega[i]) return imsrg_hip_op_copy(out, sv->omega[i]);
+  return imsrg_hip_op_read_binary(out, sv->files[i].c_str());
+}
+extern "C" int imsrg_hip_solver_get_omega(imsrg_solver *sv, int i, imsrg_dop *out) {
+  if (!sv || !out || out->ctx != sv->ctx) return IMSRG_E_INVALID;
+  return solver_omega(sv, i, out);
+}
+
+// NewOmega: the current segment joins the list (in HBM, or written to the
+// scratch directory as OMEGA_<pid><n>, the reference's file name) and a
+// cleared one takes its place.
+static int new_omega(imsrg_solver *sv) {
+  imsrg_ctx *ctx = sv->ctx;
+  if (sv->scratch.empty()) {
+    sv->omega.push_back(sv->current);
+    sv->files.emplace_back();
+    sv->current = nullptr;
+    if (int rc = imsrg_hip_op_create(ctx, -1, &sv->current)) return rc;
+    return IMSRG_OK;
+  }
+  char name[64];
+  std::snprintf(name, sizeof(name), "/OMEGA_%06d%03d", (int)getpid(), sv->written);
+  std::string path = sv->scratch + name;
+  if (int rc = imsrg_hip_op_write_binary(sv->current, path.c_str())) return rc;
+  sv->omega.push_back(nullptr);
+  sv->files.push_back(path);
+  sv->written++;
+  return imsrg_hip_op_zero(sv->current);
+}
+
+static int magnus_run(imsrg_ctx *ctx, imsrg_solver *sv, const imsrg_dop *H0, imsrg_dop *Hs, double ds, double smax,
+                      double eta_crit, double omega_norm_max, const imsrg_bch_opts &o, int max_steps, double *e0,
+                      double *eta_norm, int *ncomm, int *nsteps) {
+  imsrg_dop *Eta = nullptr, *Om2 = nullptr, *Hsaved = nullptr;
   int rc = IMSRG_OK;
   auto cleanup = [&]() {
-    imsrg_hip_op_destroy(Eta); imsrg_hip_op_destroy(Om); imsrg_hip_op_destroy(Om2); imsrg_hip_op_destroy(Hsaved);
+    imsrg_hip_op_destroy(Eta); imsrg_hip_op_destroy(Om2); imsrg_hip_op_destroy(Hsaved);
   };
-  if ((rc = imsrg_hip_op_create(ctx, -1, &Eta)) || (rc = imsrg_hip_op_create(ctx, -1, &Om)) ||
+  solver_clear(sv);
+  if ((rc = imsrg_hip_op_create(ctx, -1, &Eta)) || (rc = imsrg_hip_op_create(ctx, -1, &sv->current)) ||
       (rc = imsrg_hip_op_create(ctx, -1, &Om2)) || (rc = imsrg_hip_op_create(ctx, 1, &Hsaved))) {
     cleanup();
     return rc;
@@ -850,7 +1292,8 @@ extern "C" int imsrg_hip_magnus_flow(imsrg_ctx *ctx, const imsrg_dop *H0, imsrg_
   int nsplit = 0, step = 0;
   double s = 0.0, Elast = 0.0;
   if (!rc) {
-    HIPCHK(ctx, hipMemcpy(&Elast, H0->buf, sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(ctx, hipMemcpyAsync(&Elast, H0->buf, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   }
   while (!rc && s < smax && step < max_steps) {
     double ne;
@@ -862,20 +1305,23 @@ extern "C" int imsrg_hip_magnus_flow(imsrg_ctx *ctx, const imsrg_dop *H0, imsrg_
       break;
     }
     double no;
-    if ((rc = op_norm(Om, nullptr, nullptr, &no))) break;
+    if ((rc = op_norm(sv->current, nullptr, nullptr, &no))) break;
     if (no > omega_norm_max) {  // NewOmega, src/IMSRGSolver.cc:44-84
       if ((rc = imsrg_hip_op_copy(Hsaved, Hs))) break;
-      if ((rc = imsrg_hip_op_zero(Om))) break;
+      if ((rc = new_omega(sv))) break;
       nsplit++;
     }
     double dss = std::min(ds, smax - s);
     s += dss;
     if ((rc = imsrg_hip_op_scale(Eta, dss))) break;
     int nc1 = 0, nc2 = 0;
-    if ((rc = bch_product(ctx, Eta, Om, Om2, o, &nc1))) break;
-    std::swap(Om, Om2);
-    Om->herm = -1;
-    rc = bch_transform(ctx, nsplit == 0 ? H0 : Hsaved, Om, Hs, o, &nc2);
+    if ((rc = bch_product(ctx, Eta, sv->current, Om2, o, &nc1))) break;
+    std::swap(sv->current, Om2);
+    sv->current->herm = -1;
+    rc = bch_transform(ctx, nsplit == 0 ? H0 : Hsaved, sv->current, Hs, o, &nc2);
+    // a transform that reaches max_terms only warns in the reference
+    // (src/BCH.cc:192-195), and the flow goes on
+    if (rc == IMSRG_E_NOT_CONVERGED) rc = IMSRG_OK;
     if (rc) break;
     if ((rc = imsrg_hip_white_generator_dev(ctx, Hs, Eta))) break;
     double ez;
@@ -889,6 +1335,74 @@ extern "C" int imsrg_hip_magnus_flow(imsrg_ctx *ctx, const imsrg_dop *H0, imsrg_
   if (nsteps) *nsteps = step;
   cleanup();
   return rc;
+}
+
+extern "C" int imsrg_hip_solver_magnus_euler(imsrg_solver *sv, const imsrg_dop *H0, imsrg_dop *Hs, double ds,
+                                             double smax, double eta_crit, double omega_norm_max,
+                                             const imsrg_bch_opts *opts, int max_steps, double *e0,
+                                             double *eta_norm, int *ncomm, int *nsteps) {
+  if (!sv) return IMSRG_E_INVALID;
+  imsrg_ctx *ctx = sv->ctx;
+  if (!H0 || !Hs || H0 == Hs || H0->ctx != ctx || Hs->ctx != ctx || ds <= 0 || max_steps < 0)
+    return fail(ctx, IMSRG_E_INVALID, "invalid flow arguments");
+  imsrg_bch_opts o;
+  if (int e = bch_defaults(ctx, opts, &o)) return e;
+  return magnus_run(ctx, sv, H0, Hs, ds, smax, eta_crit, omega_norm_max, o, max_steps, e0, eta_norm, ncomm, nsteps);
+}
+
+extern "C" int imsrg_hip_magnus_flow(imsrg_ctx *ctx, const imsrg_dop *H0, imsrg_dop *Hs, double ds, double smax,
+                                     double eta_crit, double omega_norm_max, const imsrg_bch_opts *opts,
+                                     int max_steps, double *e0, double *eta_norm, int *ncomm, int *nsteps) {
+  if (!ctx) return IMSRG_E_INVALID;
+  imsrg_solver sv{ctx, {}, {}, nullptr, "", 0};
+  const int rc = imsrg_hip_solver_magnus_euler(&sv, H0, Hs, ds, smax, eta_crit, omega_norm_max, opts, max_steps, e0,
+                                               eta_norm, ncomm, nsteps);
+  solver_clear(&sv);
+  return rc;
+}
+
+// Transform_Partial(OpIn, first): Out = BCH_Transform over the segments
+// first .. n-1 in turn; InverseTransform: over -Omega_i from the last to the first.
+static int solver_apply(imsrg_solver *sv, const imsrg_dop *Op, imsrg_dop *Out, int first, bool inverse,
+                        const imsrg_bch_opts *opts, int *ncomm) {
+  imsrg_ctx *ctx = sv->ctx;
+  if (!Op || !Out || Op->ctx != ctx || Out->ctx != ctx) return fail(ctx, IMSRG_E_INVALID, "invalid operator");
+  imsrg_bch_opts o;
+  if (int e = bch_defaults(ctx, opts, &o)) return e;
+  int n = 0;
+  imsrg_hip_solver_num_omega(sv, &n);
+  if (first < 0) return fail(ctx, IMSRG_E_INVALID, "first segment index is negative");
+  imsrg_dop *om = nullptr, *cur = nullptr;
+  int rc = imsrg_hip_op_create(ctx, -1, &om);
+  if (!rc) rc = imsrg_hip_op_create(ctx, Op->herm, &cur);
+  if (!rc) rc = imsrg_hip_op_copy(Out, Op);
+  int total = 0;
+  for (int k = 0; !rc && k < n - (inverse ? 0 : first); ++k) {
+    const int i = inverse ? n - 1 - k : first + k;
+    if ((rc = solver_omega(sv, i, om))) break;
+    if (inverse && (rc = imsrg_hip_op_scale(om, -1.0))) break;
+    if ((rc = imsrg_hip_op_copy(cur, Out))) break;
+    int nc = 0;
+    rc = bch_transform(ctx, cur, om, Out, o, &nc);
+    if (rc == IMSRG_E_NOT_CONVERGED) rc = IMSRG_OK;  // warned only (src/BCH.cc:192-195)
+    total += nc;
+  }
+  if (ncomm) *ncomm = total;
+  imsrg_hip_op_destroy(om);
+  imsrg_hip_op_destroy(cur);
+  if (!rc) HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return rc;
+}
+
+extern "C" int imsrg_hip_solver_transform(imsrg_solver *sv, const imsrg_dop *Op, imsrg_dop *Out, int first,
+                                          const imsrg_bch_opts *opts, int *ncomm) {
+  if (!sv) return IMSRG_E_INVALID;
+  return solver_apply(sv, Op, Out, first, false, opts, ncomm);
+}
+extern "C" int imsrg_hip_solver_inverse_transform(imsrg_solver *sv, const imsrg_dop *Op, imsrg_dop *Out,
+                                                  const imsrg_bch_opts *opts, int *ncomm) {
+  if (!sv) return IMSRG_E_INVALID;
+  return solver_apply(sv, Op, Out, 0, true, opts, ncomm);
 }
 
 // ---------------------------------------------------------------------------
@@ -912,11 +1426,17 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
     if (Z[i] == X[i] || Z[i] == Y[i]) return fail(c0, IMSRG_E_INVALID, "Z aliases X or Y");
   }
   BatchCache &bc = c0->batch;
-  std::vector<const void *> key;
+  // The cached device table holds the operator buffers and each context's
+  // basis tables, so the key names those (and each context by its serial
+  // number, never reused), not the handle addresses a new object may take over.
+  std::vector<long long> key;
   for (int i = 0; i < npairs; ++i) {
-    key.push_back(ctxs[i]); key.push_back(X[i]); key.push_back(Y[i]); key.push_back(Z[i]);
-    key.push_back((const void *)(intptr_t)(X[i]->herm * 4 + Y[i]->herm + 8 * (int)mask));
-    key.push_back((const void *)(intptr_t)ctxs[i]->lists_gen);  // work lists rebuilt by a new tuning
+    key.push_back(ctxs[i]->serial);
+    key.push_back((long long)(intptr_t)X[i]->buf);
+    key.push_back((long long)(intptr_t)Y[i]->buf);
+    key.push_back((long long)(intptr_t)Z[i]->buf);
+    key.push_back(X[i]->herm * 4 + Y[i]->herm + 8 * (long long)mask);
+    key.push_back(ctxs[i]->lists_gen);  // work lists rebuilt by a new tuning
   }
   if (key != bc.key) {
     bc.release();
@@ -925,8 +1445,7 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
     std::vector<long> woff(npairs);
     for (int i = 0; i < npairs; ++i) {
       woff[i] = wtot;
-      const BasisDev &Bi = ctxs[i]->db.h;
-      wtot += 2 * r8(Bi.xsize) + r8(Bi.zsize) + r8(Bi.nch + 8) + 2 * r8(Bi.n122) + r8(Bi.nobpart) + r8(Bi.ndiag) + 64;
+      wtot += workspace_doubles(ctxs[i]->db.h);
     }
     HIPCHK(c0, hipMalloc(&bc.work, sizeof(double) * std::max<long>(wtot, 1)));
     HIPCHK(c0, hipMemset(bc.work, 0, sizeof(double) * std::max<long>(wtot, 1)));
@@ -936,13 +1455,9 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
       PairPtrs &p = tab[i];
       p.basis = ctxs[i]->db.d;
       p.X = X[i]->buf; p.Y = Y[i]->buf; p.Z = Z[i]->buf;
-      p.Xt = bc.work + woff[i];
-      p.Yt = p.Xt + r8(B.xsize);
-      p.Zbar = p.Yt + r8(B.xsize);
-      p.scratch = p.Zbar + r8(B.zsize);
-      p.c122 = (double2 *)(p.scratch + r8(B.nch + 8));
-      p.obpart = (double *)(p.c122) + 2 * r8(B.n122);
-      p.diag = p.obpart + r8(B.nobpart);
+      const Workspace w = carve_workspace(B, bc.work + woff[i]);
+      p.Xt = w.Xt; p.Yt = w.Yt; p.Zbar = w.Zbar; p.scratch = w.scratch;
+      p.c122 = w.c122; p.obpart = w.obpart; p.diag = w.diag;
       p.hX = X[i]->herm; p.hY = Y[i]->herm; p.hZ = herm_rule(X[i]->herm, Y[i]->herm);
       p.mask = mask;
     }
@@ -1016,8 +1531,24 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
   pa.tab = bc.d_tab;
   const int4 *items[8];
   for (int t = 0; t < 8; ++t) items[t] = bc.d_items + bc.off[t];
-  return run_commutator(c0, pa, npairs, items, bc.cnt, mask, prec, bc.maxn, bc.maxnch, bc.maxn122, bc.maxjmax,
-                        bc.maxnorb, bc.maxn1pairs, bc.maxncls);
+  // The batch runs on c0's stream.  Work queued on the other contexts' streams
+  // (uploads, copies, axpy, generators of X_i / Y_i) finishes before it starts,
+  // and their later work (downloads, norms of Z_i) waits for it to end.
+  std::vector<imsrg_ctx *> others;
+  for (int i = 0; i < npairs; ++i)
+    if (ctxs[i] != c0 && std::find(others.begin(), others.end(), ctxs[i]) == others.end()) others.push_back(ctxs[i]);
+  for (imsrg_ctx *c : others) {
+    HIPCHK(c0, hipEventRecord(c->batch_ev, c->stream));
+    HIPCHK(c0, hipStreamWaitEvent(c0->stream, c->batch_ev, 0));
+  }
+  rc = run_commutator(c0, pa, npairs, items, bc.cnt, mask, prec, bc.maxn, bc.maxnch, bc.maxn122, bc.maxjmax,
+                      bc.maxnorb, bc.maxn1pairs, bc.maxncls);
+  if (rc) return rc;
+  if (!others.empty()) {
+    HIPCHK(c0, hipEventRecord(c0->batch_ev, c0->stream));
+    for (imsrg_ctx *c : others) HIPCHK(c0, hipStreamWaitEvent(c->stream, c0->batch_ev, 0));
+  }
+  return IMSRG_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -1054,7 +1585,7 @@ extern "C" int imsrg_hip_get_tuning(imsrg_ctx *ctx, imsrg_tuning *t) {
 extern "C" int imsrg_hip_set_tuning(imsrg_ctx *ctx, const imsrg_tuning *t) {
   if (!ctx || !t) return IMSRG_E_INVALID;
   if (t->pphh_small_tile_max < 0 || t->pphh_order < 0 || t->pphh_order > 2 ||
-      (t->mixed_pphh_fp32 != 0 && t->mixed_pphh_fp32 != 1) || (t->mixed_ph_fp32 != 0 && t->mixed_ph_fp32 != 1) ||
+      t->mixed_pphh_fp32 < 0 || t->mixed_pphh_fp32 > 2 || t->mixed_ph_fp32 < 0 || t->mixed_ph_fp32 > 2 ||
       (t->ph_tile_128 != 0 && t->ph_tile_128 != 1) || (t->finish_order != 0 && t->finish_order != 1) ||
       t->overlap_pphh < 0 || t->overlap_pphh > 2)
     return fail(ctx, IMSRG_E_INVALID, "invalid tuning values");
@@ -1137,16 +1668,19 @@ extern "C" int imsrg_hip_autotune(imsrg_ctx *ctx, int precision, const char *cac
   // 2% faster than the best so far, so that timing noise between near-equal
   // settings does not decide
   std::vector<Tuning> cand{Tuning{}};
+  // under mixed precision each GEMM family runs FP32 (1) or bf16x3 (2) MFMA;
+  // the FP64 kernels (0) stay a setting by hand
+  const bool mixed = precision == IMSRG_PREC_MIXED;
   for (int sm : smalls)
     for (int order = 0; order < 3; ++order)
-      for (int mp = 0; mp < (precision == IMSRG_PREC_MIXED ? 2 : 1); ++mp)
-        for (int mh = 0; mh < (precision == IMSRG_PREC_MIXED ? 2 : 1); ++mh)
+      for (int mp = mixed ? 1 : 2; mp <= 2; ++mp)
+        for (int mh = mixed ? 1 : 2; mh <= 2; ++mh)
           for (int pb = 0; pb < 2; ++pb) {
             Tuning t;
             t.pphh_small = sm;
             t.pphh_order = order;
-            t.mixed_pphh = precision == IMSRG_PREC_MIXED ? mp : 1;
-            t.mixed_ph = precision == IMSRG_PREC_MIXED ? mh : 1;
+            t.mixed_pphh = mp;
+            t.mixed_ph = mh;
             t.ph_big = pb;
             const Tuning d{};
             if (t.pphh_small == d.pphh_small && t.pphh_order == d.pphh_order && t.mixed_pphh == d.mixed_pphh &&
@@ -1246,6 +1780,22 @@ extern "C" int imsrg_hip_selftest_mfma(imsrg_ctx *ctx, double *out1024) {
   HIPCHK(ctx, hipMalloc(&d, 1024 * sizeof(double)));
   hipError_t e = launch_mfma_probe(d, ctx->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(out1024, d, 1024 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(ctx, IMSRG_E_HIP, hipGetErrorString(e));
+  return IMSRG_OK;
+}
+
+// Lane layout self-test of v_mfma_f32_16x16x32_bf16 (not part of the reference interface).
+extern "C" int imsrg_hip_selftest_mfma_bf16(imsrg_ctx *ctx, const float *a512, const float *b512, float *d256) {
+  if (!ctx || !a512 || !b512 || !d256) return IMSRG_E_INVALID;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  float *d = nullptr;
+  HIPCHK(ctx, hipMalloc(&d, 1280 * sizeof(float)));
+  hipError_t e = hipMemcpyAsync(d, a512, 512 * sizeof(float), hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(d + 512, b512, 512 * sizeof(float), hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = launch_mfma_bf16_probe(d, d + 512, d + 1024, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(d256, d + 1024, 256 * sizeof(float), hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   (void)hipFree(d);
   if (e != hipSuccess) return fail(ctx, IMSRG_E_HIP, hipGetErrorString(e));

@@ -49,8 +49,8 @@ struct Tuning {
   int pphh_small = 64;  // channels with n <= pphh_small take 64x64 pp/hh tiles, the others 128x128
   int pphh_order = 0;  // 0: tiles by decreasing k-length, 1: row-block groups balanced over the XCDs,
                        // 2: square super-tiles balanced over the XCDs
-  int mixed_pphh = 1;  // mixed precision: 1 = FP32-MFMA pp/hh kernel, 0 = the FP64 kernel
-  int mixed_ph = 1;    // mixed precision: 1 = FP32-MFMA ph kernel, 0 = the FP64 kernel
+  int mixed_pphh = 2;  // mixed precision pp/hh kernel: 0 FP64, 1 FP32 MFMA, 2 bf16x3 MFMA
+  int mixed_ph = 2;    // mixed precision ph kernel: 0 FP64, 1 FP32 MFMA, 2 bf16x3 MFMA
   int ph_big = 1;      // ph GEMM on 128x128 tiles (1) or 64x64 tiles (0)
   int finish_order = 1;  // tile pairs of the finishing kernel: 0 channel/row order, 1 XCD-contiguous runs
   int overlap_pphh = 1;  // pp/hh GEMM on a second stream: 0 no, 1 forked first, 2 forked after the ph GEMM

@@ -30,6 +30,11 @@ constexpr int kJC = 6;            // J values per register chunk of the inverse 
 constexpr int kC122Blk = 4;       // comm122 partner lists are padded to multiples of this (one load block)
 constexpr int kF32Flush = 4;      // MFMA k-steps (of 4) summed in FP32 before the FP64 flush (mixed, 64x64 tiles)
 
+// GEMM kernel families passed to launch_pphh / launch_ph_gemm: 0 FP64,
+// 1 FP32 MFMA on FP32-rounded operands, 2 split FP32 products, kGemmB3 bf16
+// MFMA on three-way split operands
+constexpr int kGemmB3 = 3;
+
 // Term bits (same values as IMSRG_COMM* in include/imsrg_hip.h)
 constexpr unsigned M110 = 1u << 0, M220 = 1u << 1, M111 = 1u << 2, M121 = 1u << 3, M221 = 1u << 4,
                    M122 = 1u << 5, MPPHH = 1u << 6, MPH = 1u << 7;

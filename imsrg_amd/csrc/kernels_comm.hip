@@ -541,6 +541,108 @@ __device__ __forceinline__ void st_nt2(double *p, double a, double b) {
   __builtin_nontemporal_store(v, reinterpret_cast<d2v *>(p));
 }
 
+// A wave of a 128x128 tile owns the 64 x kGW sub-tile at rows (w & 1) * 64,
+// columns (w >> 1) * kGW; it has work when that sub-tile holds an upper-
+// triangle element inside the n x n matrix.
+__device__ __forceinline__ bool tile_wave_live(int R0, int S0, int n, int w) {
+  return R0 + (w & 1) * 64 < n && S0 + (w >> 1) * kGW < n && R0 + (w & 1) * 64 <= S0 + (w >> 1) * kGW + kGW - 1;
+}
+
+// Stores a 128x128 tile from the waves' FP64 accumulators (lane layout of the
+// f32 MFMAs when F32L, else of the f64 16x16x4 MFMA).
+// MODE 0: the upper triangle of D = alpha * acc.
+// MODE 1/2: Zbar in full through LDS transposes of the 64 x kGW wave sub-tiles,
+// four per round in the buffers tb0..tb3 (each >= 64 (kGW + 1) doubles): the
+// lower triangle as (head) h or (tail) phi_r phi_s times the upper element, as
+// the reference's folds make it (src/Commutator.cc:1792-1805); s_ph holds the
+// tile's row (0) and column (1) phases.
+template <int MODE, bool F32L>
+__device__ __forceinline__ void gemm_epilogue(const dbl4 (&acc)[4][kGN], double *D, int n, int R0, int S0,
+                                              double alpha, double h, const signed char (*s_ph)[kTileB],
+                                              double *tb0, double *tb1, double *tb2, double *tb3) {
+  const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wr = (wv & 1) * 64, wc = (wv >> 1) * kGW;
+  const bool wave_live = tile_wave_live(R0, S0, n, wv);
+  if constexpr (MODE == 0) {
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < kGN; ++ni)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int row = R0 + wr + mi * 16 + (lane & 15);
+          const int col = S0 + wc + ni * 16 + mfma_col<F32L>(lane, q);
+          if (row < n && col < n && row <= col) D[(long)col * n + row] = alpha * acc[mi][ni][q];
+        }
+  } else {
+    static_assert(kGWaves == 8, "two rounds of four sub-tiles");
+    auto Tb = [&](int j) {
+      double *p = j == 0 ? tb0 : j == 1 ? tb1 : j == 2 ? tb2 : tb3;
+      return reinterpret_cast<double(*)[kGW + 1]>(p);
+    };
+    for (int rd = 0; rd < 2; ++rd) {
+      if ((wv >> 2) == rd && wave_live) {
+        double(*Tt)[kGW + 1] = Tb(wv & 3);
+#pragma unroll
+        for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+          for (int ni = 0; ni < kGN; ++ni)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const int rr = mi * 16 + (lane & 15), cc = ni * 16 + mfma_col<F32L>(lane, q);
+              double v = alpha * acc[mi][ni][q];
+              if (MODE == 2) {
+                const int col = S0 + wc + cc;
+                v *= (col < n) ? (double)s_ph[1][wc + cc] : 0.0;
+              }
+              Tt[rr][cc] = v;
+            }
+      }
+      __syncthreads();
+      for (int j = 0; j < 4; ++j) {
+        const int w = 4 * rd + j;
+        if (!tile_wave_live(R0, S0, n, w)) continue;  // block-uniform
+        double(*Tt)[kGW + 1] = Tb(j);
+        const int R1 = R0 + (w & 1) * 64, S1 = S0 + (w >> 1) * kGW;
+        // Two rows per thread and one 16-byte store where both are written (store
+        // instructions, not bytes, bound this epilogue); R1 and S1 are even.
+        // upper elements (R1 + rr, S1 + cc), consecutive threads on consecutive row pairs
+        for (int e = threadIdx.x; e < 32 * kGW; e += blockDim.x) {
+          const int rr = 2 * (e & 31), cc = e >> 5;
+          const int row = R1 + rr, col = S1 + cc;
+          if (col >= n || row >= n || row > col) continue;
+          double *dp = D + (long)col * n + row;
+          if (row + 1 < n && row + 1 <= col)
+            st_nt2(dp, Tt[rr][cc], Tt[rr + 1][cc]);
+          else
+            st_nt(dp, Tt[rr][cc]);
+        }
+        // mirrored elements (S1 + cc, R1 + rr) = sym * upper (R1 + rr, S1 + cc),
+        // consecutive threads on consecutive row pairs S1 + cc
+        for (int e = threadIdx.x; e < 32 * kGW; e += blockDim.x) {
+          const int cc = 2 * (e & (kGW / 2 - 1)), rr = e / (kGW / 2);
+          const int row2 = S1 + cc, col2 = R1 + rr;
+          if (col2 >= n || row2 >= n) continue;
+          const bool lo = row2 > col2, hi = row2 + 1 < n && row2 + 1 > col2;
+          // phases from LDS: row2 - S0 and col2 - R0 lie in [0, 128)
+          const double s0 = (MODE == 1) ? h : (double)(s_ph[1][row2 - S0] * s_ph[0][col2 - R0]);
+          double *dp = D + (long)col2 * n + row2;
+          if (lo && hi) {
+            const double s1 = (MODE == 1) ? h : (double)(s_ph[1][row2 + 1 - S0] * s_ph[0][col2 - R0]);
+            st_nt2(dp, s0 * Tt[rr][cc], s1 * Tt[rr][cc + 1]);
+          } else if (lo) {
+            st_nt(dp, s0 * Tt[rr][cc]);
+          } else if (hi) {
+            const double s1 = (MODE == 1) ? h : (double)(s_ph[1][row2 + 1 - S0] * s_ph[0][col2 - R0]);
+            st_nt(dp + 1, s1 * Tt[rr][cc + 1]);
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
 template <int MODE, bool MIXED, bool SPLIT = false>
 __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
   static_assert(MIXED || !SPLIT, "split products are a mixed-precision mode");
@@ -607,10 +709,7 @@ __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
   const int wr = (wv & 1) * 64, wc = (wv >> 1) * kGW;
   // a wave whose sub-tile holds no upper-triangle element inside the matrix
   // (padding past n, or below the diagonal) issues no MFMA
-  auto live = [&](int w) {
-    return R0 + (w & 1) * 64 < n && S0 + (w >> 1) * kGW < n && R0 + (w & 1) * 64 <= S0 + (w >> 1) * kGW + kGW - 1;
-  };
-  const bool wave_live = live(wv);
+  const bool wave_live = tile_wave_live(R0, S0, n, wv);
   dbl4 acc[4][kGN];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
@@ -769,87 +868,231 @@ __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
     if (k0 + kTileK < K2) stage(k0 + kTileK, A1, B1, W1, A0, B0, W0, true);
   }
   if constexpr (MIXED) flush_t();  // a last stage without its pair
+  static_assert(64 * (kGW + 1) <= kTileK * LD, "transpose tile exceeds the operand buffer");
+  gemm_epilogue<MODE, MIXED>(acc, D, n, R0, S0, alpha, h, s_ph, &A0[0][0], &A1[0][0], &B0[0][0], &B1[0][0]);
+}
+
+// ---------------------------------------------------------------------------
+// The same 128x128 SYR2K tiles on bf16 MFMA with three-way split operands
+// (mixed precision, "bf16x3").  Each FP64 operand x (A values already scaled
+// by their weight w_k) is held as three bf16 values x = p0 + p1 + p2:
+// p0 = bf16(fl32(x)), p1 = bf16(fl32(x) - p0) and p2 = bf16(fl32(x - (p0 + p1)))
+// from the FP64 residual, so |x - (p0 + p1 + p2)| <= 2^-26 |x| (the FP32
+// mixed mode rounds each operand by 2^-24).  A product takes six
+// v_mfma_f32_16x16x32_bf16 (p0q0, p0q1, p1q0, p1q1, p0q2, p2q0; the dropped
+// p1q2 + p2q1 + p2q2 are <= 2^-26 of it), accumulated in FP32 over one stage
+// of 32 k and flushed into the FP64 accumulators after every stage.  bf16
+// MFMA runs at 16x the FP32 matrix rate on MI355X, so six products cost
+// 3/8 of one FP32 product.
+// Operands are split once per tile when a stage is staged: each wave loads
+// 8 consecutive k of two rows of one operand (16-byte global loads, lanes on
+// consecutive row pairs), splits them in registers and stores the three
+// bf16x8 vectors in the MFMA operand layout (lane l of a 16-row block reads
+// row l % 16, k octet l / 16: one 16-byte LDS read per fragment).  The next
+// stage's global loads are issued before the current stage's MFMAs.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef double d2u __attribute__((ext_vector_type(2), aligned(8)));
+
+// LDS of one stage: [operand A/B][piece 0..2][k octet 0..3][row 0..127], 16 bytes each
+typedef bf16x8 StageB3[2][3][4][kTileB];
+
+// The FP32 intermediates pass through empty asm statements (no instructions):
+// otherwise the compiler fuses the double -> float -> bf16 conversions into one
+// correctly rounded double -> bf16 conversion (round-to-odd emulation, eight
+// instructions), where any nearby p0 serves, the residuals being exact.
+__device__ __forceinline__ void split3(double x, __bf16 &p0, __bf16 &p1, __bf16 &p2) {
+  float f = (float)x;
+  asm("" : "+v"(f));
+  p0 = (__bf16)f;
+  const float f0 = (float)p0;
+  p1 = (__bf16)(f - f0);
+  const float s01 = f0 + (float)p1;  // exact: p0 and p1 span at most 18 bits
+  float r2 = (float)(x - (double)s01);
+  asm("" : "+v"(r2));
+  p2 = (__bf16)r2;
+}
+
+template <int MODE>
+__device__ __forceinline__ void gemm_tile_b3(PairArg pa, const int4 *items) {
+  constexpr int T = kTileB, KS = 32;
+  __shared__ __align__(16) StageB3 L0, L1;
+  const int4 it = items[blockIdx.x];
+  if (it.y < 0) return;  // padding item of the XCD-balanced order
+  PairPtrs P = pa.get(it.x);
+  const BasisDev &B = *P.basis;
+  int n, nk, nph = 0;
+  const double *Am, *Bm;
+  const int *kl = nullptr;
+  const double *kw = nullptr;
+  const signed char *phi = nullptr, *phph = nullptr;
+  double alpha, h;
+  double *D;
   if constexpr (MODE == 0) {
+    const int ch = it.y;
+    n = B.ch_n[ch];
+    const long off = B.ch_off[ch];
+    Am = P.X + B.two_off + off;
+    Bm = P.Y + B.two_off + off;
+    const int k0s = B.k_start[ch];
+    nk = B.k_start[ch + 1] - k0s;
+    kl = B.k_loc + k0s;
+    kw = B.k_w + k0s;
+    alpha = (double)P.hY;
+    h = (double)P.hZ;
+    D = P.Z + B.two_off + off;
+  } else {
+    const int cc = it.y;
+    n = B.cc_n[cc];
+    nph = B.cc_nph[cc];
+    nk = 2 * nph;
+    const long xo = B.cc_xoff[cc];
+    Am = P.Xt + xo;
+    Bm = P.Yt + xo;
+    D = P.Zbar + B.cc_zoff[cc] + (MODE == 2 ? (long)n * n : 0);
+    alpha = 1.0;
+    h = (MODE == 1) ? (P.hZ == 1 ? 1.0 : -1.0) : 1.0;
+    phi = B.cc_phi + B.cc_kstart[cc];
+    phph = B.cc_phph + B.cc_phstart[cc];
+  }
+  const int R0 = it.z * T, S0 = it.w * T;
+  const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  __shared__ signed char s_ph[2][T];
+  if constexpr (MODE == 2) {
+    if (threadIdx.x < 2 * T) {
+      const int sd = threadIdx.x / T, e = threadIdx.x - sd * T;
+      s_ph[sd][e] = phi[min((sd ? S0 : R0) + e, n - 1)];
+    }
+  }
+  extern __shared__ int KLs[];
+  if constexpr (MODE == 0) {
+    for (int e = threadIdx.x; e < nk; e += blockDim.x) KLs[e] = kl[e];
+  }
+  __syncthreads();
+  const int K2 = 2 * nk;
+  // loader role of this wave: operand (0: rows R0.. of A-side values, 1: rows
+  // S0.. of B-side values), k octet, and the lane's two rows
+  const int opnd = wv >> 2, oct = wv & 3;
+  const int rbase = opnd ? S0 : R0;
+  const int row = min(rbase + 2 * lane, n - 1);  // row, row + 1 (one past a column at most: slack)
+  d2u v[8];
+  auto load_stage = [&](int k0) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int kk = min(k0 + 8 * oct + j, K2 - 1);
+      const int k = kk < nk ? kk : kk - nk;
+      const int ca = (MODE == 0) ? __builtin_amdgcn_readfirstlane(KLs[k]) : k;  // wave-uniform
+      const int cb = (MODE == 2) ? (ca < nph ? ca + nph : ca - nph) : ca;
+      // A side: A(., ca) for kk < nk, else B(., cb); B side the other one
+      const bool first = kk < nk;
+      const double *src = (first != (opnd != 0)) ? Am + (long)ca * n : Bm + (long)cb * n;
+      v[j] = *reinterpret_cast<const d2u *>(src + row);
+    }
+  };
+  auto weight = [&](int kk) -> double {  // wave-uniform
+    if (kk >= K2) return 0.0;
+    const int k = kk < nk ? kk : kk - nk;
+    double w;
+    if constexpr (MODE == 0) w = kw[k];
+    else if constexpr (MODE == 1) w = 1.0;
+    else w = (double)P.hY * (double)phph[k < nph ? k : k - nph];
+    return kk < nk ? w : w * h;
+  };
+  auto store_stage = [&](int k0, StageB3 &L) {
+    bf16x8 q[2][3];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      double a = v[j].x, b = v[j].y;
+      if (opnd == 0) {
+        const double w = weight(k0 + 8 * oct + j);
+        a *= w;
+        b *= w;
+      } else if (k0 + 8 * oct + j >= K2) {
+        a = 0.0;
+        b = 0.0;
+      }
+      __bf16 x0, x1, x2, y0, y1, y2;
+      split3(a, x0, x1, x2);
+      split3(b, y0, y1, y2);
+      q[0][0][j] = x0; q[0][1][j] = x1; q[0][2][j] = x2;
+      q[1][0][j] = y0; q[1][1][j] = y1; q[1][2][j] = y2;
+    }
+    const int r = 2 * lane;
+#pragma unroll
+    for (int pc = 0; pc < 3; ++pc) {
+      L[opnd][pc][oct][r] = q[0][pc];
+      L[opnd][pc][oct][r + 1] = q[1][pc];
+    }
+  };
+  const int wr = (wv & 1) * 64, wc = (wv >> 1) * kGW;
+  const bool wave_live = tile_wave_live(R0, S0, n, wv);
+  dbl4 acc[4][kGN];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < kGN; ++b) acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
+  auto compute = [&](const StageB3 &L) {
+    if (!wave_live) return;
+    bf16x8 af[4][3], bfr[kGN][3];
+    const int oc = lane >> 4, rl = lane & 15;
+#pragma unroll
+    for (int pc = 0; pc < 3; ++pc) {
+#pragma unroll
+      for (int mi = 0; mi < 4; ++mi) af[mi][pc] = L[0][pc][oc][wr + mi * 16 + rl];
+#pragma unroll
+      for (int ni = 0; ni < kGN; ++ni) bfr[ni][pc] = L[1][pc][oc][wc + ni * 16 + rl];
+    }
+    flt4 t[4][kGN];
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < kGN; ++ni) t[mi][ni] = flt4{0.f, 0.f, 0.f, 0.f};
+    // small products first; one product for all 8 blocks before the next, so
+    // that consecutive MFMAs into one accumulator are 8 instructions apart
+    constexpr int pa_[6] = {0, 2, 1, 1, 0, 0}, pb_[6] = {2, 0, 1, 0, 1, 0};
+#pragma unroll
+    for (int p = 0; p < 6; ++p)
+#pragma unroll
+      for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < kGN; ++ni)
+          t[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[ni][pb_[p]], af[mi][pa_[p]], t[mi][ni], 0, 0, 0);
 #pragma unroll
     for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
       for (int ni = 0; ni < kGN; ++ni)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int row = R0 + wr + mi * 16 + (lane & 15);
-          const int col = S0 + wc + ni * 16 + mfma_col<MIXED>(lane, q);
-          if (row < n && col < n && row <= col) D[(long)col * n + row] = alpha * acc[mi][ni][q];
-        }
-  } else {
-    // full Zbar storage through LDS transposes of the 64x32 wave sub-tiles, four
-    // per round (the four operand buffers, idle now, hold one sub-tile each)
-    static_assert(64 * (kGW + 1) <= kTileK * LD, "transpose tile exceeds the operand buffer");
-    static_assert(kGWaves == 8, "two rounds of four sub-tiles");
-    auto Tb = [&](int j) {
-      double *p = j == 0 ? &A0[0][0] : j == 1 ? &A1[0][0] : j == 2 ? &B0[0][0] : &B1[0][0];
-      return reinterpret_cast<double(*)[kGW + 1]>(p);
-    };
-    for (int rd = 0; rd < 2; ++rd) {
-      if ((wv >> 2) == rd && wave_live) {
-        double(*Tt)[kGW + 1] = Tb(wv & 3);
-#pragma unroll
-        for (int mi = 0; mi < 4; ++mi)
-#pragma unroll
-          for (int ni = 0; ni < kGN; ++ni)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-              const int rr = mi * 16 + (lane & 15), cc = ni * 16 + mfma_col<MIXED>(lane, q);
-              double v = alpha * acc[mi][ni][q];
-              if (MODE == 2) {
-                const int col = S0 + wc + cc;
-                v *= (col < n) ? (double)s_ph[1][wc + cc] : 0.0;
-              }
-              Tt[rr][cc] = v;
-            }
-      }
+        for (int q = 0; q < 4; ++q) acc[mi][ni][q] += (double)t[mi][ni][q];
+  };
+  if (K2 > 0) {  // a channel without pp or hh kets only writes zeros
+    load_stage(0);
+    store_stage(0, L0);
+    __syncthreads();
+    for (int k0 = 0; k0 < K2; k0 += 2 * KS) {
+      // stage k0 in L0; the next one goes to L1
+      const bool m1 = k0 + KS < K2;
+      if (m1) load_stage(k0 + KS);
+      compute(L0);
+      if (m1) store_stage(k0 + KS, L1);
       __syncthreads();
-      for (int j = 0; j < 4; ++j) {
-        const int w = 4 * rd + j;
-        if (!live(w)) continue;  // block-uniform
-        double(*Tt)[kGW + 1] = Tb(j);
-        const int R1 = R0 + (w & 1) * 64, S1 = S0 + (w >> 1) * kGW;
-        // Two rows per thread and one 16-byte store where both are written (store
-        // instructions, not bytes, bound this epilogue); R1 and S1 are even.
-        // upper elements (R1 + rr, S1 + cc), consecutive threads on consecutive row pairs
-        for (int e = threadIdx.x; e < 32 * kGW; e += blockDim.x) {
-          const int rr = 2 * (e & 31), cc = e >> 5;
-          const int row = R1 + rr, col = S1 + cc;
-          if (col >= n || row >= n || row > col) continue;
-          double *dp = D + (long)col * n + row;
-          if (row + 1 < n && row + 1 <= col)
-            st_nt2(dp, Tt[rr][cc], Tt[rr + 1][cc]);
-          else
-            st_nt(dp, Tt[rr][cc]);
-        }
-        // mirrored elements (S1 + cc, R1 + rr) = sym * upper (R1 + rr, S1 + cc),
-        // consecutive threads on consecutive row pairs S1 + cc
-        for (int e = threadIdx.x; e < 32 * kGW; e += blockDim.x) {
-          const int cc = 2 * (e & (kGW / 2 - 1)), rr = e / (kGW / 2);
-          const int row2 = S1 + cc, col2 = R1 + rr;
-          if (col2 >= n || row2 >= n) continue;
-          const bool lo = row2 > col2, hi = row2 + 1 < n && row2 + 1 > col2;
-          // phases from LDS: row2 - S0 and col2 - R0 lie in [0, 128)
-          const double s0 = (MODE == 1) ? h : (double)(s_ph[1][row2 - S0] * s_ph[0][col2 - R0]);
-          double *dp = D + (long)col2 * n + row2;
-          if (lo && hi) {
-            const double s1 = (MODE == 1) ? h : (double)(s_ph[1][row2 + 1 - S0] * s_ph[0][col2 - R0]);
-            st_nt2(dp, s0 * Tt[rr][cc], s1 * Tt[rr][cc + 1]);
-          } else if (lo) {
-            st_nt(dp, s0 * Tt[rr][cc]);
-          } else if (hi) {
-            const double s1 = (MODE == 1) ? h : (double)(s_ph[1][row2 + 1 - S0] * s_ph[0][col2 - R0]);
-            st_nt(dp + 1, s1 * Tt[rr][cc + 1]);
-          }
-        }
-      }
+      if (!m1) break;
+      const bool m2 = k0 + 2 * KS < K2;
+      if (m2) load_stage(k0 + 2 * KS);
+      compute(L1);
+      if (m2) store_stage(k0 + 2 * KS, L0);
       __syncthreads();
     }
   }
+  static_assert(2 * 64 * (kGW + 1) * sizeof(double) <= sizeof(StageB3), "transpose tiles exceed the stage buffer");
+  double *t0 = reinterpret_cast<double *>(&L0[0][0][0][0]), *t2 = reinterpret_cast<double *>(&L1[0][0][0][0]);
+  gemm_epilogue<MODE, true>(acc, D, n, R0, S0, alpha, h, s_ph, t0, t0 + 64 * (kGW + 1), t2, t2 + 64 * (kGW + 1));
+}
+
+__global__ __launch_bounds__(kGWaves * 64, 2) void k_pphh_big_b3(PairArg pa, const int4 *items) {
+  gemm_tile_b3<0>(pa, items);
+}
+template <int MODE>
+__global__ __launch_bounds__(kGWaves * 64, 2) void k_ph_big_b3(PairArg pa, const int4 *items) {
+  gemm_tile_b3<MODE>(pa, items);
 }
 
 __global__ __launch_bounds__(kGWaves * 64, 4) void k_pphh_big_f64(PairArg pa, const int4 *items) {
@@ -997,7 +1240,7 @@ __host__ __device__ inline size_t invp_lds_bytes(int jmax, int norb, int nch, in
   return 8 * (size_t)nch + 8 * nJ + 4 * (size_t)nch + 4 * nJ * (2 * (size_t)norb + 3) + 4 * (2 * (size_t)ncls + 1);
 }
 
-__global__ __launch_bounds__(256, 5) void k_invpandya(PairArg pa, const int4 *items) {
+__global__ __launch_bounds__(256, 4) void k_invpandya(PairArg pa, const int4 *items) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const int4 it = items[blockIdx.x];
   PairPtrs P = pa.get(it.x);
@@ -1059,7 +1302,11 @@ __global__ __launch_bounds__(256, 5) void k_invpandya(PairArg pa, const int4 *it
   const double hf = (P.hZ == 1) ? 1.0 : -1.0;
   const double *Zb = P.Zbar;
   double *Ztwo = P.Z + B.two_off;
-  const IMSRG_G double *sixq = B.sixjQ;  // held in registers: the stores below could alias B
+  // held in registers: the stores below could alias B
+  typedef const __attribute__((address_space(4))) double c6;
+  typedef double c62v __attribute__((ext_vector_type(2)));
+  typedef const __attribute__((address_space(4))) c62v c62;
+  c6 *const sixc = (c6 *)B.sixjQ;
   const IMSRG_G int *kmapT = B.kmapT;
   const IMSRG_G int *cls_e = B.cls_e;
   for (int q0 = 0; q0 < total; q0 += blockDim.x) {
@@ -1105,82 +1352,98 @@ __global__ __launch_bounds__(256, 5) void k_invpandya(PairArg pa, const int4 *it
     const int pd = (B.l[i] + B.l[l]) & 1, td = abs(B.tz2[i] - B.tz2[l]) / 2;
     const int px = (B.l[i] + B.l[k]) & 1, tx = abs(B.tz2[i] - B.tz2[k]) / 2;
     // 6j entries of J' and the J pair (c0, c0+1) sit at s6 + J' * s6J + (c0/2) * s6P; the next
-    // pair (c0+2, c0+3) one pair stride further
+    // pair (c0+2, c0+3) one pair stride further.  When every live lane of the
+    // wave has the same (j_k, j_l) (the class order makes that the common case)
+    // the 6j addresses are wave-uniform and come from scalar loads through the
+    // constant address space (the table is read-only here); otherwise each lane
+    // loads its own.  Both paths do the same arithmetic in the same order.
     const long s6P = 2L * B.d1 * B.d1, s6J = s6P * (d2q >> 1);
-    const long s6d = sixjQ_index(B.d1, d2q, jji, jjj, 0, jjk, jjl, 0);
-    const long s6x = sixjQ_index(B.d1, d2q, jjj, jji, 0, jjk, jjl, 0);
+    const int jk0 = wave_max(live ? jjk : -1), jl0 = wave_max(live ? jjl : -1);
+    const bool uni = wave_min((!live || (jjk == jk0 && jjl == jl0)) ? 1 : 0) == 1;
     // tail element T(ik, lj) = phi_ik phi_lj T(lj, ik), phi_pq = (-1)^{(j_p+j_q)/2};
     // head H(ik, lj) = hZ-sign H(lj, ik)
     const double symx = (l > j) ? (double)sgn_parity((jji + jjk) / 2 + (jjl + jjj) / 2) : hf;
     const int dbra = (i == j), dket = (k == l);
     const double nrm = (dbra == dket) ? 1.0 + dbra : SQRT2_D;
     const double pij = sgn_parity((jji + jjj + jjk + jjl) / 2);
-    // J chunks start at even J so that each pair of 6j values is one aligned 16-byte load
-    for (int c0 = wlo & ~1; c0 <= whi; c0 += kJC) {
-      unsigned vm = 0;  // lane's valid J of this chunk
-#pragma unroll
-      for (int t = 0; t < kJC; ++t) {
-        const int J = c0 + t;
-        if (J >= Jlo && J <= Jhi && !(pauli && (J & 1))) vm |= 1u << t;
-      }
-      // acc(J) = sum_J' w_d z_d - (-1)^{(jk+jl)/2 - J} sum_J' w_x z_x, i.e. -(commij - sign commji)
-      // before the special cases k == l, i == j (no exchange sum; factor applied below)
-      double acc[kJC];
-#pragma unroll
-      for (int t = 0; t < kJC; ++t) acc[t] = 0.0;
-      const double sx0 = -(double)sgn_parity((jjk + jjl) / 2 - c0);  // -(-1)^{(jk+jl)/2 - J} at J = c0
-      for (int Jp = dlo; Jp <= dhi; ++Jp) {
-        if (Jp < Dlo || Jp > Dhi) continue;
-        const int cc = 6 * Jp + 2 * (td + 1) + pd;
-        const int nkc = s_ccn[cc];
-        const int il = Ti[Jp * norb + l];
-        const int kj = Tj[Jp * norb + k] + (k > j ? nkc : 0);
-        const double z = (2 * Jp + 1) * Zb[s_zoff[cc] + (long)kj * nkc + il];
-        const double *s6 = sixq + s6d + (long)Jp * s6J + (long)(c0 >> 1) * s6P;
-        // 0/1 factors rather than selects keep the 16-byte 6j loads unconditional
-        double2 w[kJC / 2];
-#pragma unroll
-        for (int hh = 0; hh < kJC / 2; ++hh) w[hh] = *(const double2 *)(s6 + hh * s6P);
-#pragma unroll
-        for (int hh = 0; hh < kJC / 2; ++hh) {
-          acc[2 * hh] += (((vm >> (2 * hh)) & 1) && fabs(w[hh].x) >= 1e-8 ? 1.0 : 0.0) * (w[hh].x * z);
-          acc[2 * hh + 1] += (((vm >> (2 * hh + 1)) & 1) && fabs(w[hh].y) >= 1e-8 ? 1.0 : 0.0) * (w[hh].y * z);
+    auto chunks = [&](auto uniform) {
+      constexpr bool UNI = decltype(uniform)::value;
+      // UNI: (j_k, j_l) from the wave's first live lane, in scalar registers
+      const long s6d = UNI ? sixjQ_index(B.d1, d2q, jji, jjj, 0, jk0, jl0, 0)
+                           : sixjQ_index(B.d1, d2q, jji, jjj, 0, jjk, jjl, 0);
+      const long s6x = UNI ? sixjQ_index(B.d1, d2q, jjj, jji, 0, jk0, jl0, 0)
+                           : sixjQ_index(B.d1, d2q, jjj, jji, 0, jjk, jjl, 0);
+      // J chunks start at even J so that each pair of 6j values is one aligned 16-byte load
+      for (int c0 = wlo & ~1; c0 <= whi; c0 += kJC) {
+        unsigned vm = 0;  // lane's valid J of this chunk
+  #pragma unroll
+        for (int t = 0; t < kJC; ++t) {
+          const int J = c0 + t;
+          if (J >= Jlo && J <= Jhi && !(pauli && (J & 1))) vm |= 1u << t;
+        }
+        // acc(J) = sum_J' w_d z_d - (-1)^{(jk+jl)/2 - J} sum_J' w_x z_x, i.e. -(commij - sign commji)
+        // before the special cases k == l, i == j (no exchange sum; factor applied below)
+        double acc[kJC];
+  #pragma unroll
+        for (int t = 0; t < kJC; ++t) acc[t] = 0.0;
+        const double sx0 = -(double)sgn_parity((jjk + jjl) / 2 - c0);  // -(-1)^{(jk+jl)/2 - J} at J = c0
+        for (int Jp = dlo; Jp <= dhi; ++Jp) {
+          if (Jp < Dlo || Jp > Dhi) continue;
+          const int cc = 6 * Jp + 2 * (td + 1) + pd;
+          const int nkc = s_ccn[cc];
+          const int il = Ti[Jp * norb + l];
+          const int kj = Tj[Jp * norb + k] + (k > j ? nkc : 0);
+          const double z = (2 * Jp + 1) * Zb[s_zoff[cc] + (long)kj * nkc + il];
+          const c6 *s6 = sixc + s6d + (long)Jp * s6J + (long)(c0 >> 1) * s6P;
+          // 0/1 factors rather than selects keep the 16-byte 6j loads unconditional
+          c62v w[kJC / 2];
+  #pragma unroll
+          for (int hh = 0; hh < kJC / 2; ++hh) w[hh] = *(const c62 *)(s6 + hh * s6P);
+  #pragma unroll
+          for (int hh = 0; hh < kJC / 2; ++hh) {
+            acc[2 * hh] += (((vm >> (2 * hh)) & 1) && fabs(w[hh][0]) >= 1e-8 ? 1.0 : 0.0) * (w[hh][0] * z);
+            acc[2 * hh + 1] += (((vm >> (2 * hh + 1)) & 1) && fabs(w[hh][1]) >= 1e-8 ? 1.0 : 0.0) * (w[hh][1] * z);
+          }
+        }
+        for (int Jp = xlo; Jp <= xhi; ++Jp) {
+          if (Jp < Xlo || Jp > Xhi) continue;
+          const int cc = 6 * Jp + 2 * (tx + 1) + px;
+          const int nkc = s_ccn[cc];
+          const int ik = Ti[Jp * norb + k];
+          const int lj = Tj[Jp * norb + l];
+          const double me = Zb[s_zoff[cc] + (long)((l > j ? nkc : 0) + ik) * nkc + lj];
+          const double z = (2 * Jp + 1) * (symx * me) * sx0;  // the sign alternates with J below
+          const c6 *s6 = sixc + s6x + (long)Jp * s6J + (long)(c0 >> 1) * s6P;
+          c62v w[kJC / 2];
+  #pragma unroll
+          for (int hh = 0; hh < kJC / 2; ++hh) w[hh] = *(const c62 *)(s6 + hh * s6P);
+  #pragma unroll
+          for (int hh = 0; hh < kJC / 2; ++hh) {
+            acc[2 * hh] += (((vm >> (2 * hh)) & 1) && fabs(w[hh][0]) >= 1e-8 ? 1.0 : 0.0) * (w[hh][0] * z);
+            acc[2 * hh + 1] -= (((vm >> (2 * hh + 1)) & 1) && fabs(w[hh][1]) >= 1e-8 ? 1.0 : 0.0) * (w[hh][1] * z);
+          }
+        }
+  #pragma unroll
+        for (int t = 0; t < kJC; ++t) {
+          if (!((vm >> t) & 1)) continue;
+          const int J = c0 + t;
+          // commji = commij for k == l, pij commij for i == j (no exchange sum then)
+          const double s = sgn_parity((jjk + jjl) / 2 - J);
+          const double f = (k == l) ? 1.0 - s : (i == j) ? 1.0 - s * pij : 1.0;
+          const double z = acc[t] * f / nrm;
+          const int n = s_chn[J];
+          const int ib = s_ib[J], ik = kmapT[(long)J * nk + kk];
+          if (ik == ib)  // diagonal: kept apart (the pp/hh GEMM may still be writing Z's diagonal)
+            P.diag[s_chks[J] + max(ib, 0)] = z;
+          else
+            Ztwo[s_choff[J] + (long)ib * n + ik] = z;
         }
       }
-      for (int Jp = xlo; Jp <= xhi; ++Jp) {
-        if (Jp < Xlo || Jp > Xhi) continue;
-        const int cc = 6 * Jp + 2 * (tx + 1) + px;
-        const int nkc = s_ccn[cc];
-        const int ik = Ti[Jp * norb + k];
-        const int lj = Tj[Jp * norb + l];
-        const double me = Zb[s_zoff[cc] + (long)((l > j ? nkc : 0) + ik) * nkc + lj];
-        const double z = (2 * Jp + 1) * (symx * me) * sx0;  // the sign alternates with J below
-        const double *s6 = sixq + s6x + (long)Jp * s6J + (long)(c0 >> 1) * s6P;
-        double2 w[kJC / 2];
-#pragma unroll
-        for (int hh = 0; hh < kJC / 2; ++hh) w[hh] = *(const double2 *)(s6 + hh * s6P);
-#pragma unroll
-        for (int hh = 0; hh < kJC / 2; ++hh) {
-          acc[2 * hh] += (((vm >> (2 * hh)) & 1) && fabs(w[hh].x) >= 1e-8 ? 1.0 : 0.0) * (w[hh].x * z);
-          acc[2 * hh + 1] -= (((vm >> (2 * hh + 1)) & 1) && fabs(w[hh].y) >= 1e-8 ? 1.0 : 0.0) * (w[hh].y * z);
-        }
-      }
-#pragma unroll
-      for (int t = 0; t < kJC; ++t) {
-        if (!((vm >> t) & 1)) continue;
-        const int J = c0 + t;
-        // commji = commij for k == l, pij commij for i == j (no exchange sum then)
-        const double s = sgn_parity((jjk + jjl) / 2 - J);
-        const double f = (k == l) ? 1.0 - s : (i == j) ? 1.0 - s * pij : 1.0;
-        const double z = acc[t] * f / nrm;
-        const int n = s_chn[J];
-        const int ib = s_ib[J], ik = kmapT[(long)J * nk + kk];
-        if (ik == ib)  // diagonal: kept apart (the pp/hh GEMM may still be writing Z's diagonal)
-          P.diag[s_chks[J] + max(ib, 0)] = z;
-        else
-          Ztwo[s_choff[J] + (long)ib * n + ik] = z;
-      }
-    }
+    };
+    if (uni)
+      chunks(std::true_type());
+    else
+      chunks(std::false_type());
   }
 }
 
@@ -1421,14 +1684,16 @@ hipError_t launch_pphh(const PairArg &pa, const int4 *items, int nitems, const i
                        int precision, int maxn, hipStream_t s) {
   if (nitems > 0) {
     const size_t lds = sizeof(int) * (size_t)std::max(maxn, 1);  // column-index list of the largest channel
-    if (precision == 1)
+    if (precision == kGemmB3)
+      hipLaunchKernelGGL(k_pphh_big_b3, dim3(nitems), dim3(kGWaves * 64), lds, s, pa, items);
+    else if (precision == 1)
       hipLaunchKernelGGL(k_pphh_big_mixed, dim3(nitems), dim3(kGWaves * 64), lds, s, pa, items);
     else if (precision == 2)
       hipLaunchKernelGGL(k_pphh_big_split, dim3(nitems), dim3(kGWaves * 64), lds, s, pa, items);
     else
       hipLaunchKernelGGL(k_pphh_big_f64, dim3(nitems), dim3(kGWaves * 64), lds, s, pa, items);
   }
-  if (nitems64 > 0) {
+  if (nitems64 > 0) {  // split and bf16x3 products take FP64 64x64 tiles
     if (precision == 1)
       hipLaunchKernelGGL((k_syr2k<0, true>), dim3(nitems64), dim3(256), 0, s, pa, items64, 0);
     else
@@ -1439,7 +1704,10 @@ hipError_t launch_pphh(const PairArg &pa, const int4 *items, int nitems, const i
 hipError_t launch_ph_gemm(const PairArg &pa, const int4 *items, int nitems, int precision, bool big, hipStream_t s) {
   if (nitems <= 0) return hipSuccess;
   if (big) {
-    if (precision == 1) {
+    if (precision == kGemmB3) {
+      hipLaunchKernelGGL((k_ph_big_b3<1>), dim3(nitems), dim3(kGWaves * 64), 0, s, pa, items);
+      hipLaunchKernelGGL((k_ph_big_b3<2>), dim3(nitems), dim3(kGWaves * 64), 0, s, pa, items);
+    } else if (precision == 1) {
       hipLaunchKernelGGL((k_ph_big<1, true>), dim3(nitems), dim3(kGWaves * 64), 0, s, pa, items);
       hipLaunchKernelGGL((k_ph_big<2, true>), dim3(nitems), dim3(kGWaves * 64), 0, s, pa, items);
     } else if (precision == 2) {

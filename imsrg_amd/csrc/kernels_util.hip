@@ -150,7 +150,15 @@ __global__ __launch_bounds__(256) void k_axpy(double *y, double a, const double 
 __global__ __launch_bounds__(256) void k_scale(double *y, double a, long n) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] *= a;
 }
+// y[i] = v (clears memory that may hold NaN or Inf, which a scale by 0 keeps)
+__global__ __launch_bounds__(256) void k_fill(double *y, double v, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] = v;
+}
 static int grid_for(long n) { return (int)std::max<long>(1, std::min<long>((n + 255) / 256, 8192)); }
+hipError_t launch_fill(double *y, double v, long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(256), 0, s, y, v, n);
+  return hipGetLastError();
+}
 hipError_t launch_axpy(double *y, double a, const double *x, long n, hipStream_t s) {
   hipLaunchKernelGGL(k_axpy, dim3(grid_for(n)), dim3(256), 0, s, y, a, x, n);
   return hipGetLastError();
@@ -387,6 +395,27 @@ __global__ void k_mfma_probe(double *out) {
 }
 hipError_t launch_mfma_probe(double *out, hipStream_t s) {
   hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, s, out);
+  return hipGetLastError();
+}
+
+// One v_mfma_f32_16x16x32_bf16 with lane l's operand slots j = 0..7 taken from
+// a[8l + j], b[8l + j] (rounded to bf16) and C = 0; d[4l + q] = register q of
+// lane l.  One wave, 512 + 512 inputs, 256 outputs.
+__global__ void k_mfma_bf16_probe(const float *a, const float *b, float *d) {
+  typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+  typedef float flt4u __attribute__((ext_vector_type(4)));
+  const int lane = threadIdx.x;
+  bf16x8 va, vb;
+  for (int j = 0; j < 8; ++j) {
+    va[j] = (__bf16)a[lane * 8 + j];
+    vb[j] = (__bf16)b[lane * 8 + j];
+  }
+  const flt4u z = {0.f, 0.f, 0.f, 0.f};
+  const flt4u r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va, vb, z, 0, 0, 0);
+  for (int q = 0; q < 4; ++q) d[lane * 4 + q] = r[q];
+}
+hipError_t launch_mfma_bf16_probe(const float *a, const float *b, float *d, hipStream_t s) {
+  hipLaunchKernelGGL(k_mfma_bf16_probe, dim3(1), dim3(64), 0, s, a, b, d);
   return hipGetLastError();
 }
 

@@ -41,6 +41,7 @@ hipError_t launch_zero_two(const PairArg &pa, const int4 *items, int nitems, hip
 // operator arithmetic
 hipError_t launch_axpy(double *y, double a, const double *x, long n, hipStream_t s);
 hipError_t launch_scale(double *y, double a, long n, hipStream_t s);
+hipError_t launch_fill(double *y, double v, long n, hipStream_t s);  // y[i] = v
 // partial sums of squares; out[0] = one-body, out[1] = two-body (weighted)
 hipError_t launch_norm(const BasisDev *bd, const BasisDev &bh, const double *op, double *partials, int npart,
                        double *out, hipStream_t s);
@@ -51,5 +52,7 @@ hipError_t launch_white(const BasisDev *bd, const BasisDev &bh, const double *H,
 
 // self-test of the MFMA f64 and f32 lane layouts: out[1024] (see imsrg_hip_selftest_mfma)
 hipError_t launch_mfma_probe(double *out, hipStream_t s);
+// one v_mfma_f32_16x16x32_bf16 on host-given operand slots (a, b: 512 floats, d: 256)
+hipError_t launch_mfma_bf16_probe(const float *a, const float *b, float *d, hipStream_t s);
 
 }  // namespace imsrg

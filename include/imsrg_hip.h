@@ -50,9 +50,14 @@ extern "C" {
 
 /* arithmetic of the channel GEMMs */
 #define IMSRG_PREC_FP64 0  /* every product and sum in FP64 */
-#define IMSRG_PREC_MIXED 1 /* pp/hh and ph GEMMs: FP32-rounded operands on FP32 MFMA,
-                              FP32 partial sums over 32 k (16 k on 64x64 tiles)
-                              flushed into FP64 accumulators; everything else FP64 */
+#define IMSRG_PREC_MIXED 1 /* pp/hh and ph GEMMs on reduced-precision MFMA, everything else
+                              FP64.  Default (tuning mixed_*_fp32 = 2): each FP64 operand
+                              split into three bf16 values (relative error 2^-26), six
+                              bf16 MFMA products per product, FP32 partial sums over 32 k
+                              flushed into FP64 accumulators; channels of up to
+                              pphh_small_tile_max kets on FP64 64x64 tiles.  Setting 1:
+                              FP32-rounded operands on FP32 MFMA, FP32 partial sums over
+                              32 k (16 k on 64x64 tiles) flushed into FP64 */
 #define IMSRG_PREC_SPLIT 2 /* pp/hh and ph GEMMs on 128x128 tiles: each FP64 operand split into
                               two FP32 values (hi + lo), three FP32 MFMA products per
                               product (hi*hi + hi*lo + lo*hi), FP32 partial sums flushed
@@ -89,7 +94,8 @@ typedef struct {
 } imsrg_comm_opts;
 
 typedef struct {
-  int mode;            /* 0: reference rule eps = ||Op|| e^{-2||Omega||} thr / (2||Omega||) growing by (i+1)
+  int mode;            /* (0 or 1; others are IMSRG_E_INVALID)
+                          0: reference rule eps = ||Op|| e^{-2||Omega||} thr / (2||Omega||) growing by (i+1)
                           (src/BCH.cc:132,189-191); 1: stop when ||nested|| < thr */
   double transform_thr;/* bch_transform_threshold, default 1e-9 (src/BCH.cc:16) */
   double product_thr;  /* bch_product_threshold, default 1e-4 (src/BCH.cc:17) */
@@ -152,6 +158,15 @@ int imsrg_hip_op_norm(imsrg_dop *op, double *norm1, double *norm2, double *norm)
 /* device-resident versions of the drop-ins (no PCIe traffic) */
 int imsrg_hip_commutator_ss_dev(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y, imsrg_dop *Z,
                                 const imsrg_comm_opts *opts);
+/* Stream-ordered form of imsrg_hip_commutator_ss_dev for callers that manage
+ * their own streams (one context per host thread, see the thread-safety notes
+ * of the reference's single-caller design): `stream` is a hipStream_t of the
+ * context's device (NULL: the null stream).  The commutator starts after the
+ * work queued on `stream` so far, and work queued on `stream` later starts
+ * after it; the call returns without a host synchronisation.  Z's Hermiticity
+ * is set at once. */
+int imsrg_hip_commutator_ss_dev_async(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y, imsrg_dop *Z,
+                                      const imsrg_comm_opts *opts, void *stream);
 int imsrg_hip_bch_transform_dev(imsrg_ctx *ctx, const imsrg_dop *Op, const imsrg_dop *Omega, imsrg_dop *Out,
                                 const imsrg_bch_opts *opts, int *n_commutators);
 int imsrg_hip_bch_product_dev(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y, imsrg_dop *Out,
@@ -169,10 +184,59 @@ int imsrg_hip_white_generator_dev(imsrg_ctx *ctx, const imsrg_dop *H, imsrg_dop 
 /* Magnus-Euler flow (IMSRGSolver::Solve_magnus_euler, src/IMSRGSolver.cc:236-321)
  * with fixed ds, Omega splitting at omega_norm_max (NewOmega, :44-84).
  * Hs receives H(s).  e0/eta_norm/ncomm (length max_steps, may be NULL) receive
- * E0(s), ||eta|| and commutators per step; *nsteps the number of steps. */
+ * E0(s), ||eta|| and commutators per step; *nsteps the number of steps.  A
+ * BCH_Transform that reaches max_terms does not stop the flow (the reference
+ * only warns, src/BCH.cc:192-195).  The Omega segments are dropped at the end;
+ * imsrg_hip_solver_magnus_euler keeps them. */
 int imsrg_hip_magnus_flow(imsrg_ctx *ctx, const imsrg_dop *H0, imsrg_dop *Hs, double ds, double smax,
                           double eta_criterion, double omega_norm_max, const imsrg_bch_opts *opts, int max_steps,
                           double *e0, double *eta_norm, int *ncomm, int *nsteps);
+
+/* ---- Operator binary format ------------------------------------------- */
+/* Operator::WriteBinary / ReadBinary (src/Operator.cc:341-388) with
+ * TwoBodyME::WriteBinary / ReadBinary (src/TwoBodyME.cc:812-838): the byte
+ * stream the reference writes to an std::ofstream (Omega checkpoints,
+ * src/IMSRGSolver.cc:57-78).  Fields are packed with no padding: int rank_J,
+ * rank_T, parity, particle_rank, legs, E2max, E3max; bool hermitian,
+ * antihermitian; int nChannels; double ZeroBody; norb*norb one-body doubles
+ * (column-major); for legs > 3: size_t nChannels, bool hermitian,
+ * antihermitian, int rank_J, rank_T, parity, then every channel block in
+ * ascending ch (column-major).  E2max/E3max are written as the reference
+ * ModelSpace defaults 2*emax and min(14, 3*emax) and ignored when read.  Reading
+ * accepts parity-conserving scalars of particle rank 1 or 2 of the same
+ * channel count, and sets the operator's Hermiticity from the flags. */
+int imsrg_hip_op_binary_size(const imsrg_dop *op, long *bytes);
+int imsrg_hip_op_to_binary(imsrg_dop *op, void *buf, long size);
+int imsrg_hip_op_from_binary(imsrg_dop *op, const void *buf, long size);
+int imsrg_hip_op_write_binary(imsrg_dop *op, const char *path);
+int imsrg_hip_op_read_binary(imsrg_dop *op, const char *path);
+
+/* ---- IMSRGSolver: Magnus flow with its Omega list ----------------------- */
+/* The state of IMSRGSolver (src/IMSRGSolver.hh) for the Magnus-Euler flow:
+ * the list of Omega segments (the finished ones, then the current one). */
+typedef struct imsrg_solver imsrg_solver;
+int imsrg_hip_solver_create(imsrg_ctx *ctx, imsrg_solver **out);
+int imsrg_hip_solver_destroy(imsrg_solver *s);
+/* IMSRGSolver::SetScratchDir: with dir != NULL and not empty, each finished
+ * segment is written to dir/OMEGA_<pid><n> in the binary format above and
+ * freed from HBM (NewOmega, src/IMSRGSolver.cc:44-84); NULL keeps them in HBM. */
+int imsrg_hip_solver_set_scratch(imsrg_solver *s, const char *dir);
+/* IMSRGSolver::Solve_magnus_euler (src/IMSRGSolver.cc:236-321), arguments as
+ * imsrg_hip_magnus_flow.  A previous flow's segments are dropped first. */
+int imsrg_hip_solver_magnus_euler(imsrg_solver *s, const imsrg_dop *H0, imsrg_dop *Hs, double ds, double smax,
+                                  double eta_criterion, double omega_norm_max, const imsrg_bch_opts *opts,
+                                  int max_steps, double *e0, double *eta_norm, int *ncomm, int *nsteps);
+/* Omega.size(); segment i copied into out (the last is the current one) */
+int imsrg_hip_solver_num_omega(const imsrg_solver *s, int *n);
+int imsrg_hip_solver_get_omega(imsrg_solver *s, int i, imsrg_dop *out);
+/* IMSRGSolver::Transform_Partial(Op, first) (src/IMSRGSolver.cc:959-1004; Transform
+ * is first = 0): Out = BCH_Transform over segments first .. n-1 in turn.
+ * IMSRGSolver::InverseTransform (:939-951): over -Omega_i from the last to the
+ * first.  *ncomm receives the nested commutators used (may be NULL). */
+int imsrg_hip_solver_transform(imsrg_solver *s, const imsrg_dop *Op, imsrg_dop *Out, int first,
+                               const imsrg_bch_opts *opts, int *ncomm);
+int imsrg_hip_solver_inverse_transform(imsrg_solver *s, const imsrg_dop *Op, imsrg_dop *Out,
+                                       const imsrg_bch_opts *opts, int *ncomm);
 
 /* No counterpart in the reference.  on != 0: every device commutator on ctx
  * (single calls and the nested commutators of BCH_Transform, BCH_Product and
@@ -191,9 +255,10 @@ typedef struct {
                                (default 64) */
   int pphh_order;           /* 0: pp/hh tiles by decreasing k-length; 1: row-block groups balanced over XCDs;
                                2: 8x8-tile super-tiles balanced over XCDs (shared operand blocks in one L2) */
-  int mixed_pphh_fp32;      /* IMSRG_PREC_MIXED: 1 = pp/hh GEMM on FP32 MFMA, 0 = on FP64 MFMA
-                               (IMSRG_PREC_SPLIT ignores both flags) */
-  int mixed_ph_fp32;        /* IMSRG_PREC_MIXED: 1 = ph GEMM on FP32 MFMA, 0 = on FP64 MFMA */
+  int mixed_pphh_fp32;      /* IMSRG_PREC_MIXED: pp/hh GEMM on 2 = bf16 MFMA with three-way split
+                               operands (default), 1 = FP32 MFMA on FP32-rounded operands, 0 = FP64
+                               MFMA (IMSRG_PREC_SPLIT ignores both flags) */
+  int mixed_ph_fp32;        /* IMSRG_PREC_MIXED: ph GEMM, same choices (default 2) */
   int ph_tile_128;          /* 1: ph GEMM on 128x128 tiles, 0: on 64x64 tiles */
   int finish_order;         /* tile pairs of the last kernel (fold, comm122, mirror): 0 in channel and
                                row-block order, 1 cut into 8 runs of consecutive tiles, one per XCD */
@@ -231,6 +296,10 @@ int imsrg_hip_phase_times(imsrg_ctx *ctx, double *us, int n);
  * two products on v_mfma_f32_16x16x4f32.  Used by the tests to pin the MFMA
  * lane layouts the GEMM kernels assume. */
 int imsrg_hip_selftest_mfma(imsrg_ctx *ctx, double *out1024);
+/* Diagnostic with no counterpart in the reference: one v_mfma_f32_16x16x32_bf16
+ * with C = 0 whose lane l takes operand slots j = 0..7 from a512[8l + j] and
+ * b512[8l + j] (rounded to bf16); d256[4l + q] receives register q of lane l. */
+int imsrg_hip_selftest_mfma_bf16(imsrg_ctx *ctx, const float *a512, const float *b512, float *d256);
 
 #ifdef __cplusplus
 }

@@ -60,7 +60,10 @@ static inline int phase_of(int x) { return (x % 2 == 0) ? 1 : -1; }
 /* Planted-mistake switch for the checker tests (0 = off).  1: drop the exchange
  * term of the inverse Pandya gather; 2: wrong sign of the swapped-ket phase;
  * 3: transpose of the one-body factor in comm122ss; 4: drop the (2J'+1) weight
- * in the Pandya transform. */
+ * in the Pandya transform; 5: two-body norm weighted by (2J+1) instead of
+ * (2J+1)^2; 6: one-body norm without the (2j+1) weight; 7: Omega not cleared at
+ * a split; 8: BCH_Transform of H(s) in place of H0 / H_saved; 9: BCH_Product
+ * without its 1/12 [[Y,X],X] term. */
 static int g_fault = 0;
 void orc_set_fault(int f) { g_fault = f; }
 
@@ -715,8 +718,35 @@ static void comm122ss(const oms_t *ms, const orc_op *X, const orc_op *Y, orc_op 
 }
 
 /* C(:, :) += A(:, K) diag(w) B(K, :) for n x n column-major A, B, C */
+/* Optional BLAS for the channel GEMMs (the reference runs them through
+ * Armadillo -> dgemm, src/Commutator.cc:1176-1183,1789).  The host sets a
+ * Fortran-convention dgemm (orc_set_dgemm); without one the loops below run. */
+typedef void (*dgemm_fn)(const char *ta, const char *tb, const int *m, const int *n, const int *k,
+                         const double *alpha, const double *a, const int *lda, const double *b, const int *ldb,
+                         const double *beta, double *c, const int *ldc);
+static dgemm_fn g_dgemm = NULL;
+void orc_set_dgemm(void *fn) { g_dgemm = (dgemm_fn)fn; }
+int orc_has_dgemm(void) { return g_dgemm != NULL; }
+
+/* C(n x n) += sign * A(n x n)[:, K] diag(w) B[K, :] */
 static void gemm_sub(int n, const double *A, const double *B, double *C, const int *K, const double *w, int nk,
                      double sign) {
+  if (g_dgemm && nk > 0) {
+    double *Ag = (double *)malloc(sizeof(double) * (size_t)n * nk);
+    double *Bg = (double *)malloc(sizeof(double) * (size_t)n * nk);
+    for (int t = 0; t < nk; ++t) {
+      const double wt = w ? w[t] : 1.0;
+      const double *ak = A + (long)K[t] * n;
+      for (int r = 0; r < n; ++r) Ag[(long)t * n + r] = ak[r] * wt;
+    }
+    for (int s = 0; s < n; ++s)
+      for (int t = 0; t < nk; ++t) Bg[(long)s * nk + t] = B[(long)s * n + K[t]];
+    const double one = 1.0;
+    g_dgemm("N", "N", &n, &n, &nk, &sign, Ag, &n, Bg, &nk, &one, C, &n);
+    free(Ag);
+    free(Bg);
+    return;
+  }
   for (int s = 0; s < n; ++s) {
     double *cs = C + (long)s * n;
     for (int t = 0; t < nk; ++t) {
@@ -876,7 +906,11 @@ static void pandya_xy(const oms_t *ms, const orc_op *X, const orc_op *Y, int chc
 }
 
 /* comm222_phss, src/Commutator.cc:1681-1823 and AddInversePandyaTransformation 1501-1616 */
+static double g_timer[8];
+static int g_ncalls;
+
 static void comm222_phss(const oms_t *ms, const orc_op *X, const orc_op *Y, orc_op *Z) {
+  double t_ph = omp_get_wtime();
   int nch = ms->nch;
   int hy = (Y->herm == 1) ? 1 : -1;
   double **Zbar = (double **)calloc(nch, sizeof(double *));
@@ -907,6 +941,11 @@ static void comm222_phss(const oms_t *ms, const orc_op *X, const orc_op *Y, orc_
         Yf[(long)s * K + nph + m] = Yb[(long)s * K + m] * pm;
       }
     double *Zb = Zbar[c]; /* nk x 2nk column-major */
+    if (g_dgemm) {
+      const double one = 1.0, zero = 0.0;
+      g_dgemm("N", "N", &nk, &nk, &K, &one, Xt, &nk, Yb, &K, &zero, Zb, &nk);
+      g_dgemm("N", "N", &nk, &nk, &K, &one, Xt, &nk, Yf, &K, &zero, Zb + (long)nk * nk, &nk);
+    } else
     for (int s = 0; s < nk; ++s) {
       double *zh = Zb + (long)s * nk, *zt = Zb + (long)(nk + s) * nk;
       for (int m = 0; m < K; ++m) {
@@ -944,6 +983,8 @@ static void comm222_phss(const oms_t *ms, const orc_op *X, const orc_op *Y, orc_
     free(phi);
   }
 
+  g_timer[6] += omp_get_wtime() - t_ph;
+  t_ph = omp_get_wtime();
   /* inverse Pandya */
   int hZ = (Z->herm == 1) ? 1 : -1;
   long npairs = 0;
@@ -1024,6 +1065,7 @@ static void comm222_phss(const oms_t *ms, const orc_op *X, const orc_op *Y, orc_
   free(pbr);
   for (int c = 0; c < nch; ++c) free(Zbar[c]);
   free(Zbar);
+  g_timer[7] += omp_get_wtime() - t_ph;
 }
 
 /* CommutatorScalarScalar, src/Commutator.cc:280-428 (IMSRG(2) part) */
@@ -1038,15 +1080,40 @@ int orc_commutator(const oms_t *ms, const orc_op *X, const orc_op *Y, orc_op *Z,
   *Z->zero = 0.0;
   memset(Z->one, 0, sizeof(double) * ms->norb * ms->norb);
   memset(Z->two, 0, sizeof(double) * ms->tbsize);
-  if (mask & ORC_COMM110) comm110ss(ms, X, Y, Z);
-  if (mask & ORC_COMM220) comm220ss(ms, X, Y, Z);
-  if (mask & ORC_COMM111) comm111ss(ms, X, Y, Z);
-  if (mask & ORC_COMM121) comm121ss(ms, X, Y, Z);
-  if (mask & ORC_COMM122) comm122ss(ms, X, Y, Z);
+  double t = omp_get_wtime();
+#define TIMED(slot, call)                  \
+  do {                                     \
+    call;                                  \
+    double t2_ = omp_get_wtime();          \
+    g_timer[slot] += t2_ - t;              \
+    t = t2_;                               \
+  } while (0)
+  if (mask & ORC_COMM110) TIMED(0, comm110ss(ms, X, Y, Z));
+  if (mask & ORC_COMM220) TIMED(1, comm220ss(ms, X, Y, Z));
+  if (mask & ORC_COMM111) TIMED(2, comm111ss(ms, X, Y, Z));
+  if (mask & ORC_COMM121) TIMED(3, comm121ss(ms, X, Y, Z));
+  if (mask & ORC_COMM122) TIMED(4, comm122ss(ms, X, Y, Z));
   int pphh = (mask & ORC_COMM222_PPHH) != 0, c221 = (mask & ORC_COMM221) != 0;
-  if (pphh || c221) comm222_pphh_221(ms, X, Y, Z, pphh, c221);
+  if (pphh || c221) TIMED(5, comm222_pphh_221(ms, X, Y, Z, pphh, c221));
   if (mask & ORC_COMM222_PH) comm222_phss(ms, X, Y, Z);
+#undef TIMED
+  g_ncalls++;
   return 0;
+}
+
+/* Accumulated seconds per phase since the last reset, named as the reference's
+ * IMSRGProfiler timers: [0] comm110ss [1] comm220ss [2] comm111ss [3] comm121ss
+ * [4] comm122ss [5] comm222_pp_hh_221ss (ConstructScalarMpp_Mhh + comm221ss)
+ * [6] DoPandyaTransformation + ph GEMM ("Build Z_bar", src/Commutator.cc:1808)
+ * [7] InversePandyaTransformation (:1821); returns the commutators timed. */
+int orc_timers(double *out, int n, int reset) {
+  for (int i = 0; i < n && i < 8; ++i) out[i] = g_timer[i];
+  int c = g_ncalls;
+  if (reset) {
+    for (int i = 0; i < 8; ++i) g_timer[i] = 0.0;
+    g_ncalls = 0;
+  }
+  return c;
 }
 
 /* ------------------------------------------------------------------ */
@@ -1060,7 +1127,7 @@ double orc_onebody_norm(const oms_t *ms, const orc_op *op) {
     for (int iq = 0; iq < ms->obn[p]; ++iq) {
       int q = ms->obl[p][iq];
       double v = ONE(op, p, q);
-      nrm += v * v * (ms->j2[p] + 1);
+      nrm += v * v * (g_fault == 6 ? 1 : ms->j2[p] + 1);
     }
   return sqrt(nrm);
 }
@@ -1074,6 +1141,7 @@ double orc_twobody_norm(const oms_t *ms, const orc_op *op) {
     double f = 0.0;
     for (long e = 0; e < nn; ++e) f += op->two[ch->off + e] * op->two[ch->off + e];
     double n2 = sqrt(f) * (2 * ch->J + 1);
+    if (g_fault == 5) n2 = sqrt(f * (2 * ch->J + 1));
     nrm += n2 * n2;
   }
   return sqrt(nrm);
@@ -1181,7 +1249,7 @@ int orc_bch_product(const oms_t *ms, const orc_op *X, const orc_op *Y, orc_op *O
   if (nxy * nx > thr) {
     orc_commutator(ms, nested, X, next, ORC_ALL_TERMS);
     ncomm++;
-    op_axpy(ms, Out, 1. / 12, next);
+    if (g_fault != 9) op_axpy(ms, Out, 1. / 12, next);
   }
   int k = 1;
   while (nxy > thr) {
@@ -1257,10 +1325,15 @@ void orc_white_generator(const oms_t *ms, const orc_op *H, orc_op *Eta) {
 }
 
 /* Solve_magnus_euler, src/IMSRGSolver.cc:236-321, with magnus_adaptive=false
- * (fixed ds) and NewOmega splitting (src/IMSRGSolver.cc:44-84). */
+ * (fixed ds) and NewOmega splitting (src/IMSRGSolver.cc:44-84): a finished
+ * Omega stays in the solver's list (Omega.emplace_back, then the new last
+ * entry is erased), and the flow continues from H_saved = H(s).  When omegas
+ * is not NULL, the list (finished segments, then the current one) is copied
+ * into omegas[0 .. *n_omega-1], at most max_omega of them. */
 int orc_magnus_flow(const oms_t *ms, const orc_op *H0, orc_op *Hs, double ds, double smax, double eta_crit,
-                    double omega_norm_max, int bch_mode, double bch_thr, int max_steps, double *e0_out,
-                    double *eta_norm_out, int *ncomm_out) {
+                    double omega_norm_max, int bch_mode, double bch_thr, double product_thr, int max_steps,
+                    double *e0_out, double *eta_norm_out, int *ncomm_out, orc_op *omegas, int max_omega,
+                    int *n_omega) {
   ownop Eta, Omega, Omega2, Hsaved;
   own_alloc(ms, &Eta, -1);
   own_alloc(ms, &Omega, -1);
@@ -1281,9 +1354,12 @@ int orc_magnus_flow(const oms_t *ms, const orc_op *H0, orc_op *Hs, double ds, do
     double norm_omega = orc_norm(ms, &Omega.v);
     if (norm_omega > omega_norm_max) {
       op_copy(ms, &Hsaved.v, Hs);
-      memset(Omega.one, 0, sizeof(double) * ms->norb * ms->norb);
-      memset(Omega.two, 0, sizeof(double) * ms->tbsize);
-      Omega.zero = 0;
+      if (omegas && nsplit < max_omega) op_copy(ms, &omegas[nsplit], &Omega.v);
+      if (g_fault != 7) {
+        memset(Omega.one, 0, sizeof(double) * ms->norb * ms->norb);
+        memset(Omega.two, 0, sizeof(double) * ms->tbsize);
+        Omega.zero = 0;
+      }
       nsplit++;
     }
     double dss = ds;
@@ -1293,16 +1369,19 @@ int orc_magnus_flow(const oms_t *ms, const orc_op *H0, orc_op *Hs, double ds, do
     Eta.zero *= dss;
     for (long e = 0; e < n1; ++e) Eta.one[e] *= dss;
     for (long e = 0; e < ms->tbsize; ++e) Eta.two[e] *= dss;
-    int nc = orc_bch_product(ms, &Eta.v, &Omega.v, &Omega2.v, 1e-4);
+    int nc = orc_bch_product(ms, &Eta.v, &Omega.v, &Omega2.v, product_thr);
     op_copy(ms, &Omega.v, &Omega2.v);
     Omega.v.herm = -1;
-    nc += orc_bch_transform(ms, (nsplit == 0) ? H0 : &Hsaved.v, &Omega.v, Hs, bch_mode, bch_thr);
+    if (g_fault == 8) op_copy(ms, &Hsaved.v, Hs);
+    nc += orc_bch_transform(ms, (nsplit == 0 && g_fault != 8) ? H0 : &Hsaved.v, &Omega.v, Hs, bch_mode, bch_thr);
     orc_white_generator(ms, Hs, &Eta.v);
     if (e0_out) e0_out[step] = *Hs->zero;
     if (ncomm_out) ncomm_out[step] = nc;
     Elast = *Hs->zero;
     step++;
   }
+  if (omegas && nsplit < max_omega) op_copy(ms, &omegas[nsplit], &Omega.v);
+  if (n_omega) *n_omega = nsplit + 1;
   own_free(&Eta);
   own_free(&Omega);
   own_free(&Omega2);

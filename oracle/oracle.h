@@ -88,10 +88,26 @@ int orc_bch_product(const oms_t *ms, const orc_op *X, const orc_op *Y, orc_op *O
 void orc_white_generator(const oms_t *ms, const orc_op *H, orc_op *Eta);
 
 /* Magnus-Euler flow (reference src/IMSRGSolver.cc:236-321), fixed ds.
- * Writes E0 after each step into e0_out[0..max_steps-1]; returns number of steps. */
+ * Writes E0 after each step into e0_out[0..max_steps-1]; returns number of steps.
+ * product_thr is the BCH_Product threshold (reference default 1e-4).  omegas
+ * (may be NULL) receives the Omega list: the segments finished at each split
+ * (src/IMSRGSolver.cc:44-84), then the current one; *n_omega = splits + 1. */
 int orc_magnus_flow(const oms_t *ms, const orc_op *H0, orc_op *Hs, double ds, double smax,
                     double eta_crit, double omega_norm_max, int bch_mode, double bch_thr,
-                    int max_steps, double *e0_out, double *eta_norm_out, int *ncomm_out);
+                    double product_thr, int max_steps, double *e0_out, double *eta_norm_out,
+                    int *ncomm_out, orc_op *omegas, int max_omega, int *n_omega);
+
+/* Channel GEMMs through a Fortran-convention dgemm (the reference calls BLAS
+ * through Armadillo); NULL restores the plain loops.  orc_has_dgemm: 1 if set. */
+void orc_set_dgemm(void *fn);
+int orc_has_dgemm(void);
+
+/* Seconds per phase accumulated by orc_commutator since the last reset, named
+ * as the reference's profiler timers: [0] comm110ss [1] comm220ss [2] comm111ss
+ * [3] comm121ss [4] comm122ss [5] comm222_pp_hh_221ss [6] Pandya transform and
+ * ph GEMM ("Build Z_bar") [7] InversePandyaTransformation.  Returns the number
+ * of commutators timed; reset != 0 clears the sums. */
+int orc_timers(double *out, int n, int reset);
 
 /* planted-mistake switch used by the checker tests (0 = off) */
 void orc_set_fault(int f);

@@ -80,12 +80,61 @@ def lib():
         L.orc_white_generator.argtypes = [C.c_void_p, C.POINTER(_Op), C.POINTER(_Op)]
         L.orc_magnus_flow.restype = C.c_int
         L.orc_magnus_flow.argtypes = [C.c_void_p, C.POINTER(_Op), C.POINTER(_Op), C.c_double, C.c_double,
-                                      C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, _dp, _dp, _ip]
+                                      C.c_double, C.c_double, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp,
+                                      _ip, C.POINTER(_Op), C.c_int, _ip]
+        L.orc_timers.restype = C.c_int
+        L.orc_timers.argtypes = [_dp, C.c_int, C.c_int]
+        L.orc_set_dgemm.argtypes = [C.c_void_p]
+        L.orc_has_dgemm.restype = C.c_int
         L.orc_set_fault.argtypes = [C.c_int]
         L.orc_num_threads.restype = C.c_int
         L.orc_set_num_threads.argtypes = [C.c_int]
         _LIB = L
     return _LIB
+
+
+TIMER_NAMES = ["comm110ss", "comm220ss", "comm111ss", "comm121ss", "comm122ss", "comm222_pp_hh_221ss",
+               "Build Z_bar", "InversePandyaTransformation"]
+
+
+def timers(reset=True):
+    """(commutators timed, {timer name: seconds}) accumulated since the last reset."""
+    out = (C.c_double * 8)()
+    n = lib().orc_timers(out, 8, 1 if reset else 0)
+    return n, dict(zip(TIMER_NAMES, list(out)))
+
+
+_BLAS_LIMIT = None
+
+
+def use_blas(on=True):
+    """Route the oracle's channel GEMMs through scipy's BLAS dgemm (the
+    reference calls BLAS through Armadillo), one BLAS thread per OpenMP thread
+    as in the reference's OpenMP loop over channels.  Returns a label of the
+    BLAS used, or None when scipy's BLAS is not available (the plain loops
+    stay in use)."""
+    global _BLAS_LIMIT
+    L = lib()
+    if not on:
+        L.orc_set_dgemm(None)
+        return None
+    try:
+        import scipy.linalg.cython_blas as cb
+        from threadpoolctl import threadpool_info, threadpool_limits
+    except ImportError:
+        return None
+    cap = cb.__pyx_capi__["dgemm"]
+    get_name = C.pythonapi.PyCapsule_GetName
+    get_name.restype = C.c_char_p
+    get_name.argtypes = [C.py_object]
+    get_ptr = C.pythonapi.PyCapsule_GetPointer
+    get_ptr.restype = C.c_void_p
+    get_ptr.argtypes = [C.py_object, C.c_char_p]
+    L.orc_set_dgemm(get_ptr(cap, get_name(cap)))
+    if _BLAS_LIMIT is None:
+        _BLAS_LIMIT = threadpool_limits(limits=1, user_api="blas")
+    info = [i for i in threadpool_info() if i.get("user_api") == "blas"]
+    return ";".join(sorted({f"{i.get('internal_api')} {i.get('version')}" for i in info})) or "scipy BLAS"
 
 
 def set_fault(f):
@@ -215,12 +264,28 @@ class OracleModelSpace:
         return o.op()
 
     def magnus_flow(self, H0, ds=0.5, smax=50.0, eta_crit=1e-6, omega_norm_max=2.0, bch_mode=0, bch_thr=1e-9,
-                    max_steps=200):
+                    max_steps=200, product_thr=1e-4, max_omega=0):
+        """Returns (H(s), E0 per step, ||eta|| per step (+1), commutators per
+        step); with max_omega > 0 also the Omega list (finished segments, then
+        the current one)."""
         a, o = _Holder(H0), _Holder(self.zero_op(1))
         e0 = np.zeros(max_steps + 1)
         en = np.zeros(max_steps + 1)
         nc = np.zeros(max_steps + 1, dtype=np.int32)
+        hold = [_Holder(self.zero_op(-1)) for _ in range(max_omega)]
+        arr = (_Op * max(max_omega, 1))(*[h.s for h in hold]) if max_omega else None
+        nom = C.c_int()
         n = lib().orc_magnus_flow(self.h, C.byref(a.s), C.byref(o.s), ds, smax, eta_crit, omega_norm_max,
-                                  bch_mode, bch_thr, max_steps, e0.ctypes.data_as(_dp), en.ctypes.data_as(_dp),
-                                  nc.ctypes.data_as(_ip))
-        return o.op(), e0[:n].copy(), en[:n + 1].copy(), nc[:n].copy()
+                                  bch_mode, bch_thr, product_thr, max_steps, e0.ctypes.data_as(_dp),
+                                  en.ctypes.data_as(_dp), nc.ctypes.data_as(_ip), arr, max_omega, C.byref(nom))
+        out = (o.op(), e0[:n].copy(), en[:n + 1].copy(), nc[:n].copy())
+        if not max_omega:
+            return out
+        oms = []
+        for i in range(min(nom.value, max_omega)):
+            h = hold[i]
+            h.zero[0] = arr[i].zero[0]
+            op = h.op()
+            op["herm"] = -1
+            oms.append(op)
+        return out + (oms,)

@@ -86,6 +86,37 @@ def test_mfma_f64_lane_layout(ia):
             assert cols32[4 * l + q] == l % 16
 
 
+def test_mfma_bf16_lane_layout(ia):
+    # v_mfma_f32_16x16x32_bf16: A[m][k] from lane m + 16 (k // 8), slot k % 8;
+    # B[k][n] from lane n + 16 (k // 8), slot k % 8; D[m][n] in lane n + 16 (m // 4),
+    # register m % 4 (the first operand holds the rows of D).  Small integers
+    # make every sum exact, so the layout is confirmed only by equality.
+    import ctypes as C
+    from imsrg_amd import _lib
+    ms, _ = spaces(ia, 2)
+    say("[gpu] bf16 mfma probe")
+    rng = np.random.default_rng(5)
+    a = rng.integers(-3, 4, size=512).astype(np.float32)
+    b = rng.integers(-3, 4, size=512).astype(np.float32)
+    d = np.zeros(256, dtype=np.float32)
+    fp = C.POINTER(C.c_float)
+    _lib.check(_lib.lib().imsrg_hip_selftest_mfma_bf16(ms.h, a.ctypes.data_as(fp), b.ctypes.data_as(fp),
+                                                        d.ctypes.data_as(fp)), ms.h)
+    A = np.zeros((16, 32))
+    B = np.zeros((32, 16))
+    for lane in range(64):
+        for j in range(8):
+            k = 8 * (lane // 16) + j
+            A[lane % 16, k] = a[8 * lane + j]
+            B[k, lane % 16] = b[8 * lane + j]
+    D = A @ B
+    got = np.zeros((16, 16))
+    for lane in range(64):
+        for q in range(4):
+            got[4 * (lane // 16) + q, lane % 16] = d[4 * lane + q]
+    assert np.array_equal(got, D)
+
+
 @pytest.mark.parametrize("emax", [2, 4, 6])
 def test_basis_tables_match_oracle(ia, emax):
     ms, orc = spaces(ia, emax)

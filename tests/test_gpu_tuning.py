@@ -113,7 +113,7 @@ def test_mixed_switches(ia):
         ms.set_tuning(mixed_pphh_fp32=0, mixed_ph_fp32=0)
         Zm = ia.Commutator.CommutatorScalarScalar(X, Y, precision=ia.PREC_MIXED).to_host()
         assert np.array_equal(Zm.TwoBody, Z64.TwoBody) and np.array_equal(Zm.OneBody, Z64.OneBody)
-        for mp, mh in ((1, 0), (0, 1), (1, 1)):
+        for mp, mh in ((1, 0), (0, 1), (1, 1), (2, 0), (0, 2), (2, 2), (2, 1), (1, 2)):
             ms.set_tuning(mixed_pphh_fp32=mp, mixed_ph_fp32=mh)
             Zm = ia.Commutator.CommutatorScalarScalar(X, Y, precision=ia.PREC_MIXED).to_host()
             scale = np.max(np.abs(Z64.TwoBody))

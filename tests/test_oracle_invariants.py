@@ -6,6 +6,7 @@ import pytest
 from scipy.linalg import expm, logm
 
 from opgen import one_body_blocks, random_op, rel_maxabs, synthetic_hamiltonian
+from oracle.pyoracle import set_fault
 
 
 def _lin(a, A, b, B):
@@ -191,3 +192,183 @@ def test_magnus_flow_of_one_body_hamiltonian_is_unitary(oracle_ms_cache):
 
     assert abs(const(Hs) - const(H)) < 1e-8 * abs(const(H))
     assert abs(Hs["zero"] - H["zero"]) > 1e-3  # the flow did move E0
+
+
+# ---- norms pinned by the m-scheme -------------------------------------------
+# The J-coupled norms of src/Operator.cc:1630-1646 and src/TwoBodyME.cc:703-724
+# are fixed by the uncoupled (m-scheme) matrices of the same operator:
+#  * one-body: sum_pq (2j_p+1) O_pq^2 is the Frobenius norm^2 of the m-scheme
+#    one-body matrix (each (p,q) element appears once per m = -j..j);
+#  * two-body: for an operator that lives in one channel (J, parity, Tz), CG
+#    orthogonality gives ||V_m||_F^2 = sum_{abcd ordered} (2J+1) |V~^J_abcd|^2
+#    = 4 (2J+1) ||M||_F^2 for the stored normalized block M (each ket, p<q in
+#    both orders or p=q with its sqrt(2), counts twice), while the reference's
+#    norm^2 is (2J+1)^2 ||M||_F^2.  So twobody_norm^2 = (2J+1) ||V_m||^2 / 4.
+
+
+def test_onebody_norm_is_mscheme_frobenius(oracle_ms_cache):
+    from mscheme import jcoupled_to_mscheme
+    ms = oracle_ms_cache(2)
+    for herm, seed in ((1, 31), (-1, 32)):
+        H = random_op(ms, herm, seed)
+        H["two"][:] = 0.0
+        _, x1, _ = jcoupled_to_mscheme(ms, H)
+        ref = np.linalg.norm(x1)
+        assert abs(ms.onebody_norm(H) - ref) < 1e-13 * ref
+        assert abs(ms.norm(H) - ref) < 1e-13 * ref
+
+
+def _single_channel_op(ms, ch, seed):
+    H = random_op(ms, 1, seed)
+    H["one"][:] = 0.0
+    c = ms.channels[ch]
+    keep = H["two"][c["off"]:c["off"] + c["n"] ** 2].copy()
+    H["two"][:] = 0.0
+    H["two"][c["off"]:c["off"] + c["n"] ** 2] = keep
+    return H
+
+
+def _pick_channels(ms):
+    """One channel for each J = 0..3 (J = 0 and 2 hold p = q kets, odd J none)."""
+    out = {}
+    for ch, c in enumerate(ms.channels):
+        if c["n"] > 1 and c["J"] not in out and c["J"] <= 3:
+            out[c["J"]] = ch
+    return [out[J] for J in sorted(out)]
+
+
+def test_twobody_norm_is_mscheme_frobenius_per_channel(oracle_ms_cache):
+    from mscheme import jcoupled_to_mscheme
+    ms = oracle_ms_cache(2)
+    chans = _pick_channels(ms)
+    assert len(chans) == 4
+    assert any(p == q for p, q in ms.channels[chans[0]]["kets"])  # J=0 holds p=q kets
+    for i, ch in enumerate(chans):
+        J = ms.channels[ch]["J"]
+        H = _single_channel_op(ms, ch, 40 + i)
+        _, _, x2 = jcoupled_to_mscheme(ms, H)
+        ref2 = (2 * J + 1) * np.sum(x2 ** 2) / 4.0
+        got = ms.twobody_norm(H)
+        assert abs(got ** 2 - ref2) < 1e-12 * ref2, (J, got ** 2, ref2)
+
+
+@pytest.mark.parametrize("fault", [5, 6])
+def test_norm_pins_detect_planted_mistakes(oracle_ms_cache, fault):
+    from mscheme import jcoupled_to_mscheme
+    ms = oracle_ms_cache(2)
+    set_fault(fault)
+    try:
+        if fault == 6:
+            H = random_op(ms, 1, 31)
+            H["two"][:] = 0.0
+            _, x1, _ = jcoupled_to_mscheme(ms, H)
+            assert abs(ms.onebody_norm(H) - np.linalg.norm(x1)) > 1e-3 * np.linalg.norm(x1)
+        else:
+            ch = _pick_channels(ms)[2]  # J = 2
+            H = _single_channel_op(ms, ch, 42)
+            _, _, x2 = jcoupled_to_mscheme(ms, H)
+            ref2 = 5 * np.sum(x2 ** 2) / 4.0
+            assert abs(ms.twobody_norm(H) ** 2 - ref2) > 1e-3 * ref2
+    finally:
+        set_fault(0)
+
+
+# ---- Magnus flow pinned by exact rotations ----------------------------------
+# For a Hamiltonian with no two-body part every commutator of the flow is a
+# matrix commutator of one-body matrices (plus its weighted trace in E0), so
+# the Magnus-Euler step of src/IMSRGSolver.cc:236-321 is an exact rotation:
+# with Omega_k = log(e^{eta ds} e^{Omega_{k-1}}) (BCH_Product) and
+# H(s_k) = e^{Omega_k} H_ref e^{-Omega_k} (BCH_Transform of H0, or of H_saved
+# after a split), H(s_k) = e^{eta_{k-1} ds} H(s_{k-1}) e^{-eta_{k-1} ds} whether
+# Omega was split or not.  eta is White's one-body generator,
+# eta_ia = H_ia / (H_ii - H_aa) for i outside the core and a in it (the
+# monopole terms of the Epstein-Nesbet denominator vanish without a two-body
+# part).  E0 follows the conserved E0 - sum_a n_a (2j_a+1) H_aa.
+
+
+def _white_one_body(ms, one):
+    core = [i for i, o in enumerate(ms.orbits) if o["cvq"] == 0]
+    eta = np.zeros_like(one)
+    for a in core:
+        for i in range(ms.norb):
+            if i in core or one[i, a] == 0.0:
+                continue
+            d = one[i, i] - one[a, a]
+            d = d if abs(d) >= 1e-6 else 1e-6
+            eta[i, a] = one[i, a] / d
+            eta[a, i] = -eta[i, a]
+    return eta
+
+
+def _exact_rotation_flow(ms, H, ds, nsteps):
+    from scipy.linalg import expm
+    w = np.array([o["occ"] * (o["j2"] + 1) for o in ms.orbits])
+    const = H["zero"] - np.sum(w * np.diag(H["one"]))
+    one = H["one"].copy()
+    e0 = []
+    for _ in range(nsteps):
+        U = expm(ds * _white_one_body(ms, one))
+        one = U @ one @ U.T
+        e0.append(const + np.sum(w * np.diag(one)))
+    return np.array(e0), one
+
+
+def _one_body_flow_case(ms):
+    H = synthetic_hamiltonian(ms, 23, offdiag_sigma=4.0)
+    H["two"][:] = 0.0
+    return H
+
+
+# emax=4: the s1/2 blocks hold three orbits, so the rotations of successive
+# steps do not commute and every BCH_Product term matters (at emax <= 3 every
+# block that mixes core and non-core orbits is 2x2, where they all commute).
+FLOW_PIN = dict(ds=0.5, smax=6.0, max_steps=12, omega_norm_max=0.2, product_thr=1e-12, bch_thr=1e-15)
+
+
+def test_magnus_flow_matches_exact_rotations_with_splits(oracle_ms_cache):
+    ms = oracle_ms_cache(4)
+    H = _one_body_flow_case(ms)
+    Hs, e0, en, nc, oms = ms.magnus_flow(H, max_omega=16, **FLOW_PIN)
+    assert len(e0) == 12
+    assert len(oms) >= 3  # Omega was split (NewOmega) at least twice
+    ref, one = _exact_rotation_flow(ms, H, 0.5, 12)
+    assert np.max(np.abs(e0 - ref)) < 1e-9 * np.max(np.abs(ref))
+    assert rel_maxabs(Hs["one"], one) < 1e-8
+    # the Omega list reproduces H(s): transforming H0 by each segment in turn
+    # (IMSRGSolver::Transform, src/IMSRGSolver.cc:959-1004) gives H(s)
+    out = H
+    for om in oms:
+        out, _ = ms.bch_transform(out, om, thr=1e-15)
+    assert rel_maxabs(out["one"], Hs["one"]) < 1e-10 and abs(out["zero"] - Hs["zero"]) < 1e-9
+
+
+def test_magnus_flow_end_point_is_decoupled_spectrum(oracle_ms_cache):
+    # converged flow: the core is decoupled, so each (l, j, tz) block's core
+    # orbits carry its lowest eigenvalues and E0 is fixed by the spectrum
+    ms = oracle_ms_cache(2)
+    H = _one_body_flow_case(ms)
+    Hs, e0, en, nc = ms.magnus_flow(H, ds=0.5, smax=200.0, max_steps=400, eta_crit=1e-9, omega_norm_max=0.5)
+    assert en[len(e0)] < 1e-9
+    w = np.array([o["occ"] * (o["j2"] + 1) for o in ms.orbits])
+    const = H["zero"] - np.sum(w * np.diag(H["one"]))
+    expect = const
+    for blk in one_body_blocks(ms.orbits):
+        ncore = sum(1 for i in blk if ms.orbits[i]["cvq"] == 0)
+        ev = np.linalg.eigvalsh(H["one"][np.ix_(blk, blk)])
+        expect += (ms.orbits[blk[0]]["j2"] + 1) * np.sum(ev[:ncore])
+    assert abs(e0[-1] - expect) < 1e-8 * abs(expect)
+
+
+@pytest.mark.parametrize("fault", [7, 8, 9])
+def test_flow_pin_detects_planted_mistakes(oracle_ms_cache, fault):
+    # 7: Omega kept at a split, 8: H(s) transformed in place of H0 / H_saved,
+    # 9: BCH_Product without its 1/12 term (the smallest of the three, 1e-8)
+    ms = oracle_ms_cache(4)
+    H = _one_body_flow_case(ms)
+    ref, _ = _exact_rotation_flow(ms, H, 0.5, 12)
+    set_fault(fault)
+    try:
+        _, e0, _, _ = ms.magnus_flow(H, **FLOW_PIN)
+    finally:
+        set_fault(0)
+    assert np.max(np.abs(e0 - ref)) > 5e-9 * np.max(np.abs(ref))
