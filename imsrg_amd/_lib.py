@@ -101,6 +101,7 @@ SIGNATURES = {
     "imsrg_hip_op_norm": (C.c_int, [_vp, _dp, _dp, _dp]),
     "imsrg_hip_commutator_ss_dev": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(CommOpts)]),
     "imsrg_hip_commutator_ss_dev_async": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(CommOpts), _vp]),
+    "imsrg_hip_commutator_ss_dev_noalloc": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(CommOpts)]),
     "imsrg_hip_bch_transform_dev": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(BCHOpts), _ip]),
     "imsrg_hip_bch_product_dev": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(BCHOpts), _ip]),
     "imsrg_hip_commutator_ss_batched": (C.c_int, [C.c_int, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp),

@@ -23,6 +23,7 @@
 
 #include "basis.hpp"
 #include "launch.hpp"
+#include "noalloc.hpp"
 
 using namespace imsrg;
 
@@ -205,6 +206,10 @@ static int ctx_init(imsrg_ctx *ctx, const imsrg_basis_desc *b, int device) {
   HIPCHK(ctx, hipEventCreateWithFlags(&ctx->batch_ev, hipEventDisableTiming));
   HIPCHK(ctx, hipEventCreateWithFlags(&ctx->async_in, hipEventDisableTiming));
   HIPCHK(ctx, hipEventCreateWithFlags(&ctx->async_out, hipEventDisableTiming));
+  {  // this device's copy of the allocation-free commutator's static workspace
+    std::string se;
+    if (int rc = noalloc_setup(device, se)) return fail(ctx, rc, se);
+  }
   static std::atomic<long long> next_serial{1};
   ctx->serial = next_serial++;
   const BasisDev &B = ctx->db.h;
@@ -825,6 +830,36 @@ static int commutator_dev(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y
   ctx->graphs.push_back(GraphEntry{X->buf, Y->buf, Z->buf, X->herm, Y->herm, mask, prec, exec});
   HIPCHK(ctx, hipGraphLaunch(exec, ctx->stream));
   return IMSRG_OK;
+}
+
+int imsrg::ctx_device(const imsrg_ctx *ctx) { return ctx ? ctx->device : -1; }
+int imsrg::ctx_fail(imsrg_ctx *ctx, int code, const char *msg) { return fail(ctx, code, msg); }
+
+// The allocation-free commutator: the context's own workspaces are replaced by
+// ones carved from ws.  Cross-coupled channels with no ph kets are never
+// written by the ph GEMM and must read as 0, so that part (all of Zbar) is
+// cleared by a fill kernel on every call: ws keeps the previous call's
+// contents, possibly of another basis.  Every other workspace is written
+// before it is read.
+int imsrg::commutator_in_workspace(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y, imsrg_dop *Z,
+                                   const imsrg_comm_opts *opts, double *ws, long cap) {
+  if (!X || !Y || !Z) return IMSRG_E_INVALID;
+  if (X->ctx != ctx || Y->ctx != ctx || Z->ctx != ctx) return fail(ctx, IMSRG_E_INVALID, "operator from another context");
+  unsigned mask;
+  int prec;
+  if (int rc = comm_opts_ok(ctx, opts, &mask, &prec)) return rc;
+  const BasisDev &B = ctx->db.h;
+  const long need = workspace_doubles(B);
+  if (need > cap) {
+    char m[160];
+    std::snprintf(m, sizeof(m), "basis needs %ld bytes of workspace; the static workspace holds %ld (emax <= 6 fits)",
+                  8 * need, 8 * cap);
+    return fail(ctx, IMSRG_E_INVALID, m);
+  }
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const Workspace w = carve_workspace(B, ws);
+  HIPCHK(ctx, launch_fill(w.Zbar, 0.0, std::max<long>(B.zsize, 1), ctx->stream));
+  return commutator_dev(ctx, X, Y, Z, mask, prec, &w);
 }
 
 extern "C" int imsrg_hip_set_graphs(imsrg_ctx *ctx, int on) {

@@ -167,6 +167,17 @@ int imsrg_hip_commutator_ss_dev(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_
  * is set at once. */
 int imsrg_hip_commutator_ss_dev_async(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y, imsrg_dop *Z,
                                       const imsrg_comm_opts *opts, void *stream);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  The same
+ * commutator as imsrg_hip_commutator_ss_dev, with every per-call workspace
+ * (Pandya matrices, partial sums, comm122 coefficients) carved from one static
+ * device array of 32 MiB per GPU, held from module load until the process
+ * ends; the call allocates nothing.  Bases whose workspaces exceed it (above
+ * emax = 6 for the O16 reference) return IMSRG_E_INVALID.  All calls on one
+ * GPU share the array, so calls that overlap on one GPU (other host threads,
+ * other contexts' streams) race; calls ordered on one stream do not. */
+int imsrg_hip_commutator_ss_dev_noalloc(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y, imsrg_dop *Z,
+                                        const imsrg_comm_opts *opts);
 int imsrg_hip_bch_transform_dev(imsrg_ctx *ctx, const imsrg_dop *Op, const imsrg_dop *Omega, imsrg_dop *Out,
                                 const imsrg_bch_opts *opts, int *n_commutators);
 int imsrg_hip_bch_product_dev(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y, imsrg_dop *Out,
