@@ -37,8 +37,8 @@ struct BatchCache {
   std::vector<long long> key;
   PairPtrs *d_tab = nullptr;
   int4 *d_items = nullptr;
-  size_t off[8] = {0};
-  int cnt[8] = {0};
+  size_t off[9] = {0};
+  int cnt[9] = {0};
   double *work = nullptr;
   int maxn = 0, maxnch = 0, maxjmax = 0, maxnorb = 0, maxn1pairs = 0, maxncls = 0;
   long maxn122 = 0;
@@ -144,6 +144,7 @@ struct imsrg_ctx {
   double2 *c122 = nullptr;
   double *obpart = nullptr;  // one-body partial sums (nobpart)
   double *diag = nullptr;    // inverse-Pandya diagonal terms (ndiag)
+  double *panel = nullptr;   // bf16x3 operand panels of the mixed pp/hh GEMM (allocated on first use)
   double *partials = nullptr;
   double *norm_dev = nullptr;
   double *norm_host = nullptr;  // pinned
@@ -280,6 +281,7 @@ extern "C" int imsrg_hip_ctx_destroy(imsrg_ctx *ctx) {
   if (ctx->c122) (void)hipFree(ctx->c122);
   if (ctx->obpart) (void)hipFree(ctx->obpart);
   if (ctx->diag) (void)hipFree(ctx->diag);
+  if (ctx->panel) (void)hipFree(ctx->panel);
   if (ctx->partials) (void)hipFree(ctx->partials);
   if (ctx->norm_dev) (void)hipFree(ctx->norm_dev);
   if (ctx->norm_host) (void)hipHostFree(ctx->norm_host);
@@ -648,8 +650,8 @@ static int herm_rule(int hx, int hy) { return (hx == hy) ? -1 : 1; }  // src/Com
 // items / cnt: [0] pp/hh 128-tiles, [1] ph tiles, [2] comm221 one-body items,
 // [3] Pandya chunks, [4] inverse-Pandya items,
 // [5] one-body pairs, [6] tile pairs of the finishing kernel,
-// [7] pp/hh 64-tiles.  Under mixed precision the context's tuning picks the
-// FP32 or FP64 kernel of each GEMM family.
+// [7] pp/hh 64-tiles, [8] bf16x3 panel packing.  Under mixed precision the
+// context's tuning picks the kernel of each GEMM family.
 // Sums the phase durations of the oldest recorded event set (waits for it).
 static int harvest_oldest(imsrg_ctx *ctx) {
   auto &set = ctx->ring[ctx->ring_head];
@@ -673,16 +675,22 @@ static int harvest_oldest(imsrg_ctx *ctx) {
   return IMSRG_OK;
 }
 
+// GEMM kernel family of a phase: under mixed precision the tuning picks FP64
+// (0), FP32 MFMA (1) or bf16x3 MFMA (2); other precisions keep theirs
+static int gemm_family(int prec, int flag) {
+  return prec != IMSRG_PREC_MIXED ? prec : flag == 0 ? 0 : flag == 1 ? 1 : kGemmB3;
+}
+static bool uses_panels(const imsrg_ctx *ctx, int prec) {
+  return gemm_family(prec, ctx->db.tune.mixed_pphh) == kGemmB3 && !ctx->db.wl.pack.empty();
+}
+
 static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const int4 *const *items, const int *cnt,
                           unsigned mask, int prec, int maxn, int maxnch, long maxn122, int maxjmax, int maxnorb,
                           int maxn1pairs, int maxncls) {
   const Tuning &tu = ctx->db.tune;
   // split products take both GEMM families whatever the mixed-precision choices
-  // GEMM kernel family of each phase: under mixed precision the tuning picks
-  // FP64 (0), FP32 MFMA (1) or bf16x3 MFMA (2)
-  auto family = [&](int flag) { return prec != IMSRG_PREC_MIXED ? prec : flag == 0 ? 0 : flag == 1 ? 1 : kGemmB3; };
-  const int prec_pphh = family(tu.mixed_pphh);
-  const int prec_ph = family(tu.mixed_ph);
+  const int prec_pphh = gemm_family(prec, tu.mixed_pphh);
+  const int prec_ph = gemm_family(prec, tu.mixed_ph);
   hipStream_t s = ctx->stream;
   // With overlap, the pp/hh GEMM and then comm122 (coefficients and an
   // upper-triangle pass) run on the side stream next to the zero-body,
@@ -707,7 +715,7 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
     HIPCHK(ctx, hipEventRecord(ctx->fork, s));
     HIPCHK(ctx, hipStreamWaitEvent(ctx->side, ctx->fork, 0));
     if (set) HIPCHK(ctx, hipEventRecord((*set)[9], ctx->side));
-    HIPCHK(ctx, launch_pphh(pa, items[0], cnt[0], items[7], cnt[7], prec_pphh, maxn, ctx->side));
+    HIPCHK(ctx, launch_pphh(pa, items[0], cnt[0], items[7], cnt[7], items[8], cnt[8], prec_pphh, maxn, ctx->side));
     if (set) HIPCHK(ctx, hipEventRecord((*set)[10], ctx->side));
     if (mask & M122) {
       HIPCHK(ctx, launch_c122_coef(pa, npairs, maxn122, ctx->side));
@@ -727,7 +735,7 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
   HIPCHK(ctx, mark(2));
   if (overlap) {
   } else if (mask & MPPHH) {
-    HIPCHK(ctx, launch_pphh(pa, items[0], cnt[0], items[7], cnt[7], prec_pphh, maxn, s));
+    HIPCHK(ctx, launch_pphh(pa, items[0], cnt[0], items[7], cnt[7], items[8], cnt[8], prec_pphh, maxn, s));
   } else {
     HIPCHK(ctx, launch_zero_two(pa, items[6], cnt[6], s));
   }
@@ -754,10 +762,13 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
 struct Workspace {
   double *Xt, *Yt, *Zbar, *scratch;
   double2 *c122;
-  double *obpart, *diag;
+  double *obpart, *diag, *panel;
 };
-static long workspace_doubles(const BasisDev &B) {
-  return 2 * r8(B.xsize + 8) + r8(B.zsize) + r8(B.nch + 8) + 2 * r8(B.n122) + r8(B.nobpart) + r8(B.ndiag) + 64;
+// panels: room for the bf16x3 operand panels (last region, used by the mixed
+// pp/hh GEMM only)
+static long workspace_doubles(const BasisDev &B, bool panels) {
+  return 2 * r8(B.xsize + 8) + r8(B.zsize) + r8(B.nch + 8) + 2 * r8(B.n122) + r8(B.nobpart) + r8(B.ndiag) +
+         (panels ? r8(2 * B.pk_size) : 0) + 64;
 }
 static Workspace carve_workspace(const BasisDev &B, double *base) {
   Workspace w;
@@ -769,6 +780,7 @@ static Workspace carve_workspace(const BasisDev &B, double *base) {
   w.c122 = (double2 *)(w.scratch + r8(B.nch + 8));
   w.obpart = (double *)(w.c122) + 2 * r8(B.n122);
   w.diag = w.obpart + r8(B.nobpart);
+  w.panel = w.diag + r8(B.ndiag);
   return w;
 }
 
@@ -778,6 +790,13 @@ static int commutator_dev(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y
   if (Y->herm != 1 && Y->herm != -1) return fail(ctx, IMSRG_E_UNSUPPORTED, "Y must be Hermitian or anti-Hermitian");
   if (Z == X || Z == Y) return fail(ctx, IMSRG_E_INVALID, "Z must not alias X or Y");
   const DeviceBasis &db = ctx->db;
+  if (!ws && !ctx->panel && uses_panels(ctx, prec)) {
+    const size_t bytes = 16 * std::max<size_t>(db.h.pk_size, 1);
+    if (hipMalloc(&ctx->panel, bytes) != hipSuccess) {
+      ctx->panel = nullptr;
+      return fail(ctx, IMSRG_E_NOMEM, "hipMalloc of the bf16x3 operand panels failed");
+    }
+  }
   PairArg pa;
   pa.tab = nullptr;
   pa.p0.basis = db.d;
@@ -791,14 +810,17 @@ static int commutator_dev(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y
   pa.p0.c122 = ws ? ws->c122 : ctx->c122;
   pa.p0.obpart = ws ? ws->obpart : ctx->obpart;
   pa.p0.diag = ws ? ws->diag : ctx->diag;
+  pa.p0.panel = ws ? ws->panel : ctx->panel;
   pa.p0.hX = X->herm;
   pa.p0.hY = Y->herm;
   pa.p0.hZ = herm_rule(X->herm, Y->herm);
   pa.p0.mask = mask;
   Z->herm = pa.p0.hZ;
-  const int4 *items[8] = {db.d_pphh, db.d_ph, db.d_ob221, db.d_pandya, db.d_invp, db.d_one, db.d_mirror, db.d_pphh64};
-  int cnt[8] = {(int)db.wl.pphh.size(), (int)db.wl.ph.size(), (int)db.wl.ob221.size(), (int)db.wl.pandya.size(),
-                (int)db.wl.invp.size(), (int)db.wl.one.size(), (int)db.wl.mirror.size(), (int)db.wl.pphh64.size()};
+  const int4 *items[9] = {db.d_pphh, db.d_ph,     db.d_ob221,  db.d_pandya, db.d_invp,
+                          db.d_one,  db.d_mirror, db.d_pphh64, db.d_pack};
+  int cnt[9] = {(int)db.wl.pphh.size(),   (int)db.wl.ph.size(),  (int)db.wl.ob221.size(),
+                (int)db.wl.pandya.size(), (int)db.wl.invp.size(), (int)db.wl.one.size(),
+                (int)db.wl.mirror.size(), (int)db.wl.pphh64.size(), (int)db.wl.pack.size()};
   if (!ctx->use_graphs || ctx->profiling || ws)
     return run_commutator(ctx, pa, 1, items, cnt, mask, prec, db.h.maxn, db.h.nch, db.h.n122, db.h.jmax,
                           db.h.norb, db.h.n1pairs, db.h.ncls);
@@ -849,7 +871,7 @@ int imsrg::commutator_in_workspace(imsrg_ctx *ctx, const imsrg_dop *X, const ims
   int prec;
   if (int rc = comm_opts_ok(ctx, opts, &mask, &prec)) return rc;
   const BasisDev &B = ctx->db.h;
-  const long need = workspace_doubles(B);
+  const long need = workspace_doubles(B, uses_panels(ctx, prec));
   if (need > cap) {
     char m[160];
     std::snprintf(m, sizeof(m), "basis needs %ld bytes of workspace; the static workspace holds %ld (emax <= 6 fits)",
@@ -1480,7 +1502,7 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
     std::vector<long> woff(npairs);
     for (int i = 0; i < npairs; ++i) {
       woff[i] = wtot;
-      wtot += workspace_doubles(ctxs[i]->db.h);
+      wtot += workspace_doubles(ctxs[i]->db.h, true);
     }
     HIPCHK(c0, hipMalloc(&bc.work, sizeof(double) * std::max<long>(wtot, 1)));
     HIPCHK(c0, hipMemset(bc.work, 0, sizeof(double) * std::max<long>(wtot, 1)));
@@ -1492,19 +1514,19 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
       p.X = X[i]->buf; p.Y = Y[i]->buf; p.Z = Z[i]->buf;
       const Workspace w = carve_workspace(B, bc.work + woff[i]);
       p.Xt = w.Xt; p.Yt = w.Yt; p.Zbar = w.Zbar; p.scratch = w.scratch;
-      p.c122 = w.c122; p.obpart = w.obpart; p.diag = w.diag;
+      p.c122 = w.c122; p.obpart = w.obpart; p.diag = w.diag; p.panel = w.panel;
       p.hX = X[i]->herm; p.hY = Y[i]->herm; p.hZ = herm_rule(X[i]->herm, Y[i]->herm);
       p.mask = mask;
     }
     HIPCHK(c0, hipMalloc(&bc.d_tab, sizeof(PairPtrs) * npairs));
     HIPCHK(c0, hipMemcpy(bc.d_tab, tab.data(), sizeof(PairPtrs) * npairs, hipMemcpyHostToDevice));
     // concatenated lists; pp/hh tiles of all pairs go through the XCD-balanced order
-    std::vector<int4> all[8];
+    std::vector<int4> all[9];
     std::vector<TileGroup> groups;
     for (int i = 0; i < npairs; ++i) {
       const WorkLists &wl = ctxs[i]->db.wl;
-      const std::vector<int4> *src[8] = {&wl.pphh, &wl.ph, &wl.ob221, &wl.pandya, &wl.invp, &wl.one, &wl.mirror,
-                                         &wl.pphh64};
+      const std::vector<int4> *src[9] = {&wl.pphh, &wl.ph,     &wl.ob221,  &wl.pandya, &wl.invp,
+                                         &wl.one,  &wl.mirror, &wl.pphh64, &wl.pack};
       {  // ph tiles in the tile size of the first context, whose kernels run the batch
         const int tile = c0->db.tune.ph_big ? kTileB : kTile;
         const DeviceBasis &dbi = ctxs[i]->db;
@@ -1516,7 +1538,7 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
             for (int sb = rb; sb < nt; ++sb) all[1].push_back(make_int4(i, c, rb, sb));
         }
       }
-      for (int t = 2; t < 8; ++t)
+      for (int t = 2; t < 9; ++t)
         for (int4 w : *src[t]) {
           w.x = i;
           all[t].push_back(w);
@@ -1548,13 +1570,13 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
       for (auto &g : groups) all[0].insert(all[0].end(), g.tiles.begin(), g.tiles.end());
     }
     size_t tot = 0;
-    for (int t = 0; t < 8; ++t) {
+    for (int t = 0; t < 9; ++t) {
       bc.off[t] = tot;
       bc.cnt[t] = (int)all[t].size();
       tot += all[t].size();
     }
     HIPCHK(c0, hipMalloc(&bc.d_items, sizeof(int4) * std::max<size_t>(tot, 1)));
-    for (int t = 0; t < 8; ++t)
+    for (int t = 0; t < 9; ++t)
       if (!all[t].empty())
         HIPCHK(c0, hipMemcpy(bc.d_items + bc.off[t], all[t].data(), sizeof(int4) * all[t].size(),
                              hipMemcpyHostToDevice));
@@ -1564,8 +1586,8 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
   PairArg pa;
   std::memset(&pa.p0, 0, sizeof(pa.p0));
   pa.tab = bc.d_tab;
-  const int4 *items[8];
-  for (int t = 0; t < 8; ++t) items[t] = bc.d_items + bc.off[t];
+  const int4 *items[9];
+  for (int t = 0; t < 9; ++t) items[t] = bc.d_items + bc.off[t];
   // The batch runs on c0's stream.  Work queued on the other contexts' streams
   // (uploads, copies, axpy, generators of X_i / Y_i) finishes before it starts,
   // and their later work (downloads, norms of Z_i) waits for it to end.

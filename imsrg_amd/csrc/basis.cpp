@@ -188,6 +188,24 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
     for (int k = 0; k < hb.nkets; ++k)
       for (int J = 0; J <= hb.jmax; ++J) cT[(size_t)J * hb.nkets + k] = hb.kmap[(size_t)k * (hb.jmax + 1) + J];
     B.kmapT = dupload(*this, cT, err);
+    // ccorb[(i (jmax+1) + J) norb + x]: cross-coupled local index of the ket of
+    // orbits i and x in channel J (of its parity and |tz_i - tz_x|/2), or -1
+    std::vector<int> co((size_t)norb * (hb.jmax + 1) * norb);
+    for (int i = 0; i < norb; ++i)
+      for (int J = 0; J <= hb.jmax; ++J)
+        for (int x = 0; x < norb; ++x) {
+          const int k = ket_index(norb, std::min(i, x), std::max(i, x));
+          co[((size_t)i * (hb.jmax + 1) + J) * norb + x] = hb.cmap[(size_t)k * (hb.jmax + 1) + J];
+        }
+    B.ccorb = dupload(*this, co, err);
+    // kmorb: the same for the standard channels (kmap)
+    for (int i = 0; i < norb; ++i)
+      for (int J = 0; J <= hb.jmax; ++J)
+        for (int x = 0; x < norb; ++x) {
+          const int k = ket_index(norb, std::min(i, x), std::max(i, x));
+          co[((size_t)i * (hb.jmax + 1) + J) * norb + x] = hb.kmap[(size_t)k * (hb.jmax + 1) + J];
+        }
+    B.kmorb = dupload(*this, co, err);
   }
   // (parity, Tz) groups of kets
   std::vector<int> grp_start(7, 0), grp_kets;
@@ -310,6 +328,21 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
   B.hh_start = dupload(*this, hh_start, err); B.hh_loc = dupload(*this, hh_loc, err);
   B.hh_occ = dupload(*this, hh_occ, err);
   B.k_start = dupload(*this, k_start, err); B.k_loc = dupload(*this, k_loc, err); B.k_w = dupload(*this, k_w, err);
+  {
+    // bf16x3 operand panels of the mixed pp/hh GEMM: per channel, two panels
+    // (X W and Y over the pp/hh list) of nkp/8 k octets x 3 pieces x nrp rows,
+    // 16 bytes each, nkp = nk rounded up to 32 and nrp = n rounded up to 128
+    std::vector<long> pk(nch + 1, 0);
+    long o = 0;
+    for (int c = 0; c < nch; ++c) {
+      pk[c] = o;
+      const long nk = k_start[c + 1] - k_start[c], n = ch_n[c];
+      if (n > 0 && nk > 0) o += 2 * ((nk + kB3K - 1) / kB3K) * (kB3K / 8) * 3 * ((n + kTileB - 1) / kTileB * kTileB);
+    }
+    pk[nch] = o;
+    B.pk_off = dupload(*this, pk, err);
+    B.pk_size = o;
+  }
   B.kpp_start = dupload(*this, kpp_start, err); B.kpp_loc = dupload(*this, kpp_loc, err);
   B.kpp_w = dupload(*this, kpp_w, err);
   B.khh_start = dupload(*this, khh_start, err); B.khh_loc = dupload(*this, khh_loc, err);
@@ -514,6 +547,15 @@ std::string DeviceBasis::build_lists(const Tuning &t) {
         groups.push_back(g);
       }
     }
+    // pack items of the mixed pp/hh GEMM: (pair, ch, k octet, block of 256 rows)
+    // for the channels on 128x128 tiles
+    for (int c = 0; c < nch; ++c) {
+      const int n = ch_n[c], nk = k_start[c + 1] - k_start[c];
+      if (n == 0 || n <= t.pphh_small || nk == 0) continue;
+      const int noct = (nk + kB3K - 1) / kB3K * (kB3K / 8), nrp = (n + kTileB - 1) / kTileB * kTileB;
+      for (int o = 0; o < noct; ++o)
+        for (int r = 0; r < nrp; r += 256) wl.pack.push_back(make_int4(0, c, o, r / 256));
+    }
     if (t.pphh_order >= 1) {
       wl.pphh = xcd_order(groups, kXcd);
     } else {
@@ -609,7 +651,7 @@ std::string DeviceBasis::build_lists(const Tuning &t) {
       err = "hipMemcpy failed for a work list";
     return (int4 *)p;
   };
-  d_pphh = up(wl.pphh); d_pphh64 = up(wl.pphh64); d_ph = up(wl.ph);
+  d_pphh = up(wl.pphh); d_pphh64 = up(wl.pphh64); d_ph = up(wl.ph); d_pack = up(wl.pack);
   d_pandya = up(wl.pandya); d_invp = up(wl.invp);
   d_one = up(wl.one); d_mirror = up(wl.mirror); d_gen = up(wl.gen); d_ob221 = up(wl.ob221);
   if (err.empty()) tune = t;

@@ -50,7 +50,7 @@ struct Tuning {
   int pphh_order = 0;  // 0: tiles by decreasing k-length, 1: row-block groups balanced over the XCDs,
                        // 2: square super-tiles balanced over the XCDs
   int mixed_pphh = 2;  // mixed precision pp/hh kernel: 0 FP64, 1 FP32 MFMA, 2 bf16x3 MFMA
-  int mixed_ph = 2;    // mixed precision ph kernel: 0 FP64, 1 FP32 MFMA, 2 bf16x3 MFMA
+  int mixed_ph = 1;    // mixed precision ph kernel: 0 FP64, 1 FP32 MFMA, 2 bf16x3 MFMA
   int ph_big = 1;      // ph GEMM on 128x128 tiles (1) or 64x64 tiles (0)
   int finish_order = 1;  // tile pairs of the finishing kernel: 0 channel/row order, 1 XCD-contiguous runs
   int overlap_pphh = 1;  // pp/hh GEMM on a second stream: 0 no, 1 forked first, 2 forked after the ph GEMM
@@ -60,6 +60,7 @@ struct Tuning {
 struct WorkLists {
   std::vector<int4> pphh;    // (pair, ch, rb, sb), rb <= sb, 128x128 tiles
   std::vector<int4> pphh64;  // (pair, ch, rb, sb), rb <= sb, 64x64 tiles (channels n <= pphh_small)
+  std::vector<int4> pack;    // (pair, ch, k octet, 256-row block): bf16x3 panels of the 128x128-tile channels
   std::vector<int4> ph;      // (pair, cc, rb, sb), rb <= sb, tiles of 128 (ph_big) or 64
   std::vector<int4> pandya;  // (pair, cc, start, 0)
   std::vector<int4> invp;    // (pair, group, bra, first ket) of the (parity, Tz) ket groups
@@ -77,7 +78,7 @@ struct DeviceBasis {
   std::vector<void *> list_allocs;  // device work lists (rebuilt when the tuning changes)
   WorkLists wl;             // host lists with pair = 0
   Tuning tune;
-  int4 *d_pphh = nullptr, *d_pphh64 = nullptr, *d_ph = nullptr, *d_pandya = nullptr,
+  int4 *d_pphh = nullptr, *d_pphh64 = nullptr, *d_ph = nullptr, *d_pandya = nullptr, *d_pack = nullptr,
        *d_invp = nullptr, *d_one = nullptr, *d_mirror = nullptr, *d_gen = nullptr, *d_ob221 = nullptr;
   // host tables the work lists are built from
   std::vector<int> ch_n, k_start, cc_n, cc_nph, grp_start, grp_kets, gen_ok;

@@ -34,6 +34,7 @@ constexpr int kF32Flush = 4;      // MFMA k-steps (of 4) summed in FP32 before t
 // 1 FP32 MFMA on FP32-rounded operands, 2 split FP32 products, kGemmB3 bf16
 // MFMA on three-way split operands
 constexpr int kGemmB3 = 3;
+constexpr int kB3K = 32;          // k-depth of one stage of the bf16x3 GEMMs (one v_mfma_f32_16x16x32_bf16)
 
 // Term bits (same values as IMSRG_COMM* in include/imsrg_hip.h)
 constexpr unsigned M110 = 1u << 0, M220 = 1u << 1, M111 = 1u << 2, M121 = 1u << 3, M221 = 1u << 4,
@@ -63,6 +64,8 @@ struct BasisDev {
   const IMSRG_G int *kmap, *cmap;
   const IMSRG_G int *cmapT;     // cmapT[J*nkets + k] = cmap[k*(jmax+1) + J]
   const IMSRG_G int *kmapT;     // kmapT[J*nkets + k] = kmap[k*(jmax+1) + J]
+  const IMSRG_G int *ccorb;     // ccorb[(i*(jmax+1) + J)*norb + x] = cmap of the ket (min(i,x), max(i,x)) at J
+  const IMSRG_G int *kmorb;     // kmorb: the same layout over kmap (standard channels)
   // kets of each (parity, Tz) group g = 2(Tz+1) + parity (the channels 6J + g),
   // ascending global index: grp_kets[grp_start[g] .. grp_start[g+1])
   const IMSRG_G int *grp_start, *grp_kets;
@@ -74,6 +77,12 @@ struct BasisDev {
   const IMSRG_G double *hh_occ;
   const IMSRG_G int *k_start, *k_loc;  // pp/hh product list with combined weight (w_pp - w_hh)
   const IMSRG_G double *k_w;
+  // bf16x3 operand panels of the mixed pp/hh GEMM (16-byte units): channel ch
+  // starts at pk_off[ch] with the panel of X W, then the panel of Y, each
+  // [nkp/8 k octets][3 pieces][nrp rows] (nkp = nk rounded up to kB3K, nrp = n
+  // rounded up to kTileB); pk_size units in all
+  const IMSRG_G long *pk_off;
+  long pk_size;
   const IMSRG_G int *kpp_start, *kpp_loc;  // M_pp list (weights w_pp)
   const IMSRG_G double *kpp_w;
   const IMSRG_G int *khh_start, *khh_loc;  // M_hh list (weights n_p n_q)
@@ -121,6 +130,7 @@ struct PairPtrs {
   IMSRG_G double2 *c122;          // per-call comm122 coefficients (X1(a,o) f, Y1(a,o) f), n122 entries
   IMSRG_G double *obpart;         // per-call one-body partial sums, nobpart entries
   IMSRG_G double *diag;           // per-call inverse-Pandya diagonal terms, one per ket of every channel (ndiag)
+  IMSRG_G double *panel;          // per-call bf16x3 panels of the mixed pp/hh GEMM (2 pk_size doubles)
   int hX, hY, hZ;
   unsigned mask;
 };

@@ -900,16 +900,20 @@ typedef bf16x8 StageB3[2][3][4][kTileB];
 // otherwise the compiler fuses the double -> float -> bf16 conversions into one
 // correctly rounded double -> bf16 conversion (round-to-odd emulation, eight
 // instructions), where any nearby p0 serves, the residuals being exact.
-__device__ __forceinline__ void split3(double x, __bf16 &p0, __bf16 &p1, __bf16 &p2) {
-  float f = (float)x;
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef __bf16 b2v __attribute__((ext_vector_type(2)));
+// The three pieces of two values (consecutive k of one row) at once: each
+// rounding to bf16 is one v_cvt_pk_bf16_f32 for the pair.
+__device__ __forceinline__ void split3x2(double x0, double x1, b2v &p0, b2v &p1, b2v &p2) {
+  f2v f = {(float)x0, (float)x1};
   asm("" : "+v"(f));
-  p0 = (__bf16)f;
-  const float f0 = (float)p0;
-  p1 = (__bf16)(f - f0);
-  const float s01 = f0 + (float)p1;  // exact: p0 and p1 span at most 18 bits
-  float r2 = (float)(x - (double)s01);
+  p0 = __builtin_convertvector(f, b2v);
+  const f2v f0 = __builtin_convertvector(p0, f2v);
+  p1 = __builtin_convertvector(f - f0, b2v);
+  const f2v s01 = f0 + __builtin_convertvector(p1, f2v);  // exact: p0 and p1 span at most 18 bits
+  f2v r2 = {(float)(x0 - (double)s01[0]), (float)(x1 - (double)s01[1])};
   asm("" : "+v"(r2));
-  p2 = (__bf16)r2;
+  p2 = __builtin_convertvector(r2, b2v);
 }
 
 template <int MODE>
@@ -963,64 +967,76 @@ __device__ __forceinline__ void gemm_tile_b3(PairArg pa, const int4 *items) {
       s_ph[sd][e] = phi[min((sd ? S0 : R0) + e, n - 1)];
     }
   }
-  extern __shared__ int KLs[];
-  if constexpr (MODE == 0) {
-    for (int e = threadIdx.x; e < nk; e += blockDim.x) KLs[e] = kl[e];
-  }
-  __syncthreads();
+  __syncthreads();  // s_ph
   const int K2 = 2 * nk;
+  // column indices and weights are read by wave-uniform scalar loads through
+  // the constant address space (read-only here): the eight loads of a stage
+  // issue back to back and are waited for once, with no LDS round trip
+  typedef const __attribute__((address_space(4))) int cint;
+  typedef const __attribute__((address_space(4))) double cdbl;
+  cint *klc = (cint *)kl;
+  cdbl *kwc = (cdbl *)kw;
   // loader role of this wave: operand (0: rows R0.. of A-side values, 1: rows
   // S0.. of B-side values), k octet, and the lane's two rows
   const int opnd = wv >> 2, oct = wv & 3;
   const int rbase = opnd ? S0 : R0;
   const int row = min(rbase + 2 * lane, n - 1);  // row, row + 1 (one past a column at most: slack)
-  d2u v[8];
-  auto load_stage = [&](int k0) {
+  // two register sets: the global loads of a stage are issued two stages
+  // before it is computed (one stage of compute between issue and use left the
+  // load latency exposed at two waves per SIMD)
+  d2u va[8], vb[8];
+  // global loads of one stage and the factors of its 8 k slots (the weights on
+  // the A side, 1 on the B side, 0 past the end).  Indices and weights are
+  // wave-uniform; all their scalar loads are issued before any is used, with
+  // no branch between them
+  auto load_stage = [&](int k0, d2u (&v)[8], double (&wf)[8]) {
+    const int kb = k0 + 8 * oct;
+    int kk[8], k[8], ca[8];
+    double wr[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      const int kk = min(k0 + 8 * oct + j, K2 - 1);
-      const int k = kk < nk ? kk : kk - nk;
-      const int ca = (MODE == 0) ? __builtin_amdgcn_readfirstlane(KLs[k]) : k;  // wave-uniform
-      const int cb = (MODE == 2) ? (ca < nph ? ca + nph : ca - nph) : ca;
+      kk[j] = min(kb + j, K2 - 1);
+      k[j] = kk[j] < nk ? kk[j] : kk[j] - nk;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      if constexpr (MODE == 0) {
+        ca[j] = klc[k[j]];
+        wr[j] = kwc[k[j]];
+      } else {
+        ca[j] = k[j];
+        wr[j] = (MODE == 1) ? 1.0 : (double)P.hY * (double)phph[k[j] < nph ? k[j] : k[j] - nph];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int cb = (MODE == 2) ? (ca[j] < nph ? ca[j] + nph : ca[j] - nph) : ca[j];
       // A side: A(., ca) for kk < nk, else B(., cb); B side the other one
-      const bool first = kk < nk;
-      const double *src = (first != (opnd != 0)) ? Am + (long)ca * n : Bm + (long)cb * n;
+      const bool first = kk[j] < nk;
+      const double *src = (first != (opnd != 0)) ? Am + (long)ca[j] * n : Bm + (long)cb * n;
       v[j] = *reinterpret_cast<const d2u *>(src + row);
+      const double w = first ? wr[j] : wr[j] * h;
+      wf[j] = kb + j >= K2 ? 0.0 : opnd ? 1.0 : w;
     }
   };
-  auto weight = [&](int kk) -> double {  // wave-uniform
-    if (kk >= K2) return 0.0;
-    const int k = kk < nk ? kk : kk - nk;
-    double w;
-    if constexpr (MODE == 0) w = kw[k];
-    else if constexpr (MODE == 1) w = 1.0;
-    else w = (double)P.hY * (double)phph[k < nph ? k : k - nph];
-    return kk < nk ? w : w * h;
-  };
-  auto store_stage = [&](int k0, StageB3 &L) {
-    bf16x8 q[2][3];
+  auto store_stage = [&](const d2u (&v)[8], const double (&wf)[8], StageB3 &L) {
+    union {
+      bf16x8 v8;
+      b2v h[4];
+    } q[2][3];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      double a = v[j].x, b = v[j].y;
-      if (opnd == 0) {
-        const double w = weight(k0 + 8 * oct + j);
-        a *= w;
-        b *= w;
-      } else if (k0 + 8 * oct + j >= K2) {
-        a = 0.0;
-        b = 0.0;
-      }
-      __bf16 x0, x1, x2, y0, y1, y2;
-      split3(a, x0, x1, x2);
-      split3(b, y0, y1, y2);
-      q[0][0][j] = x0; q[0][1][j] = x1; q[0][2][j] = x2;
-      q[1][0][j] = y0; q[1][1][j] = y1; q[1][2][j] = y2;
+    for (int j = 0; j < 8; j += 2) {
+      b2v x0, x1, x2, y0, y1, y2;
+      split3x2(v[j].x * wf[j], v[j + 1].x * wf[j + 1], x0, x1, x2);  // row 2 lane
+      split3x2(v[j].y * wf[j], v[j + 1].y * wf[j + 1], y0, y1, y2);  // row 2 lane + 1
+      q[0][0].h[j / 2] = x0; q[0][1].h[j / 2] = x1; q[0][2].h[j / 2] = x2;
+      q[1][0].h[j / 2] = y0; q[1][1].h[j / 2] = y1; q[1][2].h[j / 2] = y2;
     }
     const int r = 2 * lane;
 #pragma unroll
     for (int pc = 0; pc < 3; ++pc) {
-      L[opnd][pc][oct][r] = q[0][pc];
-      L[opnd][pc][oct][r + 1] = q[1][pc];
+      L[opnd][pc][oct][r] = q[0][pc].v8;
+      L[opnd][pc][oct][r + 1] = q[1][pc].v8;
     }
   };
   const int wr = (wv & 1) * 64, wc = (wv >> 1) * kGW;
@@ -1032,30 +1048,35 @@ __device__ __forceinline__ void gemm_tile_b3(PairArg pa, const int4 *items) {
     for (int b = 0; b < kGN; ++b) acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
   auto compute = [&](const StageB3 &L) {
     if (!wave_live) return;
-    bf16x8 af[4][3], bfr[kGN][3];
     const int oc = lane >> 4, rl = lane & 15;
+    // the B-side fragments of all three pieces stay in registers; the A side
+    // is read one piece at a time (fewer registers live at once)
+    bf16x8 bfr[kGN][3];
 #pragma unroll
-    for (int pc = 0; pc < 3; ++pc) {
-#pragma unroll
-      for (int mi = 0; mi < 4; ++mi) af[mi][pc] = L[0][pc][oc][wr + mi * 16 + rl];
+    for (int pc = 0; pc < 3; ++pc)
 #pragma unroll
       for (int ni = 0; ni < kGN; ++ni) bfr[ni][pc] = L[1][pc][oc][wc + ni * 16 + rl];
-    }
     flt4 t[4][kGN];
 #pragma unroll
     for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
       for (int ni = 0; ni < kGN; ++ni) t[mi][ni] = flt4{0.f, 0.f, 0.f, 0.f};
-    // small products first; one product for all 8 blocks before the next, so
-    // that consecutive MFMAs into one accumulator are 8 instructions apart
-    constexpr int pa_[6] = {0, 2, 1, 1, 0, 0}, pb_[6] = {2, 0, 1, 0, 1, 0};
+    // products p_a q_b with a + b <= 2: (2,0), then (1,0), (1,1), then (0,2), (0,1), (0,0);
+    // one product for all 8 blocks before the next, so that consecutive MFMAs
+    // into one accumulator are 8 instructions apart
 #pragma unroll
-    for (int p = 0; p < 6; ++p)
+    for (int pa3 = 2; pa3 >= 0; --pa3) {
+      bf16x8 af[4];
 #pragma unroll
-      for (int mi = 0; mi < 4; ++mi)
+      for (int mi = 0; mi < 4; ++mi) af[mi] = L[0][pa3][oc][wr + mi * 16 + rl];
 #pragma unroll
-        for (int ni = 0; ni < kGN; ++ni)
-          t[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[ni][pb_[p]], af[mi][pa_[p]], t[mi][ni], 0, 0, 0);
+      for (int pb3 = 2 - pa3; pb3 >= 0; --pb3)
+#pragma unroll
+        for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+          for (int ni = 0; ni < kGN; ++ni)
+            t[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[ni][pb3], af[mi], t[mi][ni], 0, 0, 0);
+    }
 #pragma unroll
     for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
@@ -1064,21 +1085,23 @@ __device__ __forceinline__ void gemm_tile_b3(PairArg pa, const int4 *items) {
         for (int q = 0; q < 4; ++q) acc[mi][ni][q] += (double)t[mi][ni][q];
   };
   if (K2 > 0) {  // a channel without pp or hh kets only writes zeros
-    load_stage(0);
-    store_stage(0, L0);
+    const int nst = (K2 + KS - 1) / KS;
+    double wa[8], wb[8];
+    load_stage(0, va, wa);
+    if (nst > 1) load_stage(KS, vb, wb);
+    store_stage(va, wa, L0);
     __syncthreads();
-    for (int k0 = 0; k0 < K2; k0 += 2 * KS) {
-      // stage k0 in L0; the next one goes to L1
-      const bool m1 = k0 + KS < K2;
-      if (m1) load_stage(k0 + KS);
+    for (int st = 0; st < nst; st += 2) {
+      // stage st in L0, stage st+1 in flight in vb, va free
+      if (st + 2 < nst) load_stage((st + 2) * KS, va, wa);
       compute(L0);
-      if (m1) store_stage(k0 + KS, L1);
+      if (st + 1 < nst) store_stage(vb, wb, L1);
       __syncthreads();
-      if (!m1) break;
-      const bool m2 = k0 + 2 * KS < K2;
-      if (m2) load_stage(k0 + 2 * KS);
+      if (st + 1 >= nst) break;
+      // stage st+1 in L1, stage st+2 in flight in va, vb free
+      if (st + 3 < nst) load_stage((st + 3) * KS, vb, wb);
       compute(L1);
-      if (m2) store_stage(k0 + 2 * KS, L0);
+      if (st + 2 < nst) store_stage(va, wa, L0);
       __syncthreads();
     }
   }
@@ -1087,9 +1110,168 @@ __device__ __forceinline__ void gemm_tile_b3(PairArg pa, const int4 *items) {
   gemm_epilogue<MODE, true>(acc, D, n, R0, S0, alpha, h, s_ph, t0, t0 + 64 * (kGW + 1), t2, t2 + 64 * (kGW + 1));
 }
 
-__global__ __launch_bounds__(kGWaves * 64, 2) void k_pphh_big_b3(PairArg pa, const int4 *items) {
-  gemm_tile_b3<0>(pa, items);
+// ---------------------------------------------------------------------------
+// Mixed pp/hh GEMM from packed panels.  k_pack_pphh splits each operand element
+// once per commutator into its three bf16 pieces (split3x2, X already scaled by
+// its weight) and stores them in the MFMA operand layout; the tile kernel then
+// fills its LDS stages with direct global->LDS copies of those panels (no
+// register staging, no conversion per tile) and runs six bf16 products per
+// product as gemm_tile_b3 does.  The SYR2K's two halves are
+//   sum_k (X w)(r,k) Y(s,k)  and  h * sum_k Y(r,k) (X w)(s,k),
+// so the second half reads the same two panels with the sides swapped and its
+// FP32 partial sums are flushed with the factor h.
+
+// one work item: k octet it.z of channel it.y, rows 256 it.w .. + 255
+__global__ __launch_bounds__(256) void k_pack_pphh(PairArg pa, const int4 *items) {
+  const int4 it = items[blockIdx.x];
+  PairPtrs P = pa.get(it.x);
+  const BasisDev &B = *P.basis;
+  const int ch = it.y, oct = it.z;
+  const int n = B.ch_n[ch];
+  const int k0s = B.k_start[ch], nk = B.k_start[ch + 1] - k0s;
+  const int nrp = (n + kTileB - 1) / kTileB * kTileB;
+  const int noct = (nk + kB3K - 1) / kB3K * (kB3K / 8);
+  const int r = it.w * 256 + (int)threadIdx.x;
+  if (r >= nrp) return;
+  const long off = B.ch_off[ch];
+  const double *X2 = P.X + B.two_off + off, *Y2 = P.Y + B.two_off + off;
+  const int rr = min(r, n - 1);
+  double xv[8], yv[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int k = 8 * oct + j;
+    const int kc = min(k, nk - 1);
+    const int col = B.k_loc[k0s + kc];  // wave-uniform
+    const double w = B.k_w[k0s + kc];
+    const bool ok = k < nk && r < n;
+    const double x = X2[(long)col * n + rr], y = Y2[(long)col * n + rr];
+    xv[j] = ok ? x * w : 0.0;
+    yv[j] = ok ? y : 0.0;
+  }
+  union {
+    bf16x8 v8;
+    b2v h[4];
+  } qx[3], qy[3];
+#pragma unroll
+  for (int j = 0; j < 8; j += 2) {
+    split3x2(xv[j], xv[j + 1], qx[0].h[j / 2], qx[1].h[j / 2], qx[2].h[j / 2]);
+    split3x2(yv[j], yv[j + 1], qy[0].h[j / 2], qy[1].h[j / 2], qy[2].h[j / 2]);
+  }
+  bf16x8 *px = reinterpret_cast<bf16x8 *>(P.panel) + B.pk_off[ch];
+  bf16x8 *py = px + (long)noct * 3 * nrp;
+#pragma unroll
+  for (int pc = 0; pc < 3; ++pc) {
+    px[((long)oct * 3 + pc) * nrp + r] = qx[pc].v8;
+    py[((long)oct * 3 + pc) * nrp + r] = qy[pc].v8;
+  }
 }
+
+// Measured alternatives at emax = 10 (pp/hh phase with the packing, 740 us
+// for this kernel): register-staged copies two stages ahead 751 us; three LDS
+// stages with the copies' waits written by hand 790 us; two wave groups that
+// alternate between LDS reads and MFMA 800 us.  The kernel streams 4.0 GB of
+// panels per commutator through L2 into LDS, and that stream sets its time:
+// the copies alone take 510 us, 205 us when every tile re-reads data that
+// stays in L2.
+__global__ __launch_bounds__(kGWaves * 64, 2) void k_pphh_big_b3p(PairArg pa, const int4 *items) {
+  constexpr int T = kTileB;
+  __shared__ __align__(16) StageB3 L0, L1;
+  __shared__ signed char s_ph[2][T];  // unused in this mode (epilogue signature)
+  const int4 it = items[blockIdx.x];
+  if (it.y < 0) return;  // padding item of the XCD-balanced order
+  PairPtrs P = pa.get(it.x);
+  const BasisDev &B = *P.basis;
+  const int ch = it.y;
+  const int n = B.ch_n[ch];
+  const int nk = B.k_start[ch + 1] - B.k_start[ch];
+  const int nrp = (n + kTileB - 1) / kTileB * kTileB;
+  const int noct = (nk + kB3K - 1) / kB3K * (kB3K / 8);
+  const bf16x8 *px = reinterpret_cast<const bf16x8 *>(P.panel) + B.pk_off[ch];
+  const bf16x8 *py = px + (long)noct * 3 * nrp;
+  const double alpha = (double)P.hY, h = (double)P.hZ;
+  double *D = P.Z + B.two_off + B.ch_off[ch];
+  const int R0 = it.z * T, S0 = it.w * T;
+  const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nst1 = noct / 4, nst = 2 * nst1;  // stages of 32 k per half, both halves
+  // stage copies: 2 sides x 3 pieces x 4 k octets x 2 halves of 64 rows = 48
+  // wave copies of 1 KiB (16 bytes per lane), six per wave
+  auto issue = [&](int st, StageB3 &L) {
+    const bool second = st >= nst1;
+    const int o0 = (second ? st - nst1 : st) * 4;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      const int q = wv * 6 + i;
+      const int side = q / 24, pc = (q / 8) % 3, oct = (q / 2) % 4, half = q & 1;
+      const bf16x8 *src = ((side == 0) != second) ? px : py;
+      const int row0 = (side ? S0 : R0) + half * 64;
+      const bf16x8 *g = src + ((long)(o0 + oct) * 3 + pc) * nrp + row0 + lane;
+      __builtin_amdgcn_global_load_lds((const IMSRG_G void *)g,
+                                       (__attribute__((address_space(3))) void *)&L[side][pc][oct][half * 64], 16, 0,
+                                       0);
+    }
+  };
+  const int wr = (wv & 1) * 64, wc = (wv >> 1) * kGW;
+  const bool wave_live = tile_wave_live(R0, S0, n, wv);
+  dbl4 acc[4][kGN];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < kGN; ++b) acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
+  // six bf16 products of a stage summed in FP32 (smallest first), then
+  // flushed into FP64 with the half's factor (1, or h for the second half)
+  auto compute = [&](const StageB3 &L, double fl) {
+    if (!wave_live) return;
+    const int oc = lane >> 4, rl = lane & 15;
+    bf16x8 bfr[kGN][3];
+#pragma unroll
+    for (int pc = 0; pc < 3; ++pc)
+#pragma unroll
+      for (int ni = 0; ni < kGN; ++ni) bfr[ni][pc] = L[1][pc][oc][wc + ni * 16 + rl];
+    flt4 t[4][kGN];
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < kGN; ++ni) t[mi][ni] = flt4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int pa3 = 2; pa3 >= 0; --pa3) {
+      bf16x8 af[4];
+#pragma unroll
+      for (int mi = 0; mi < 4; ++mi) af[mi] = L[0][pa3][oc][wr + mi * 16 + rl];
+#pragma unroll
+      for (int pb3 = 2 - pa3; pb3 >= 0; --pb3)
+#pragma unroll
+        for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+          for (int ni = 0; ni < kGN; ++ni)
+            t[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[ni][pb3], af[mi], t[mi][ni], 0, 0, 0);
+    }
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < kGN; ++ni)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[mi][ni][q] += fl * (double)t[mi][ni][q];
+  };
+  if (nst > 0) {  // a channel without pp or hh kets only writes zeros
+    issue(0, L0);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    for (int st = 0; st < nst; st += 2) {
+      if (st + 1 < nst) issue(st + 1, L1);
+      compute(L0, st >= nst1 ? h : 1.0);
+      __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of the next stage have landed
+      __syncthreads();
+      if (st + 1 >= nst) break;
+      if (st + 2 < nst) issue(st + 2, L0);
+      compute(L1, st + 1 >= nst1 ? h : 1.0);
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+    }
+  }
+  double *t0 = reinterpret_cast<double *>(&L0[0][0][0][0]), *t2 = reinterpret_cast<double *>(&L1[0][0][0][0]);
+  gemm_epilogue<0, true>(acc, D, n, R0, S0, alpha, h, s_ph, t0, t0 + 64 * (kGW + 1), t2, t2 + 64 * (kGW + 1));
+}
+
 template <int MODE>
 __global__ __launch_bounds__(kGWaves * 64, 2) void k_ph_big_b3(PairArg pa, const int4 *items) {
   gemm_tile_b3<MODE>(pa, items);
@@ -1157,14 +1339,16 @@ __global__ __launch_bounds__(256) void k_pandya(PairArg pa, const int4 *items) {
     // index work common to X and Y (GetTBME_J_norm_twoOps, src/TwoBodyME.cc:368-409)
     const int pcb = (B.l[c] + B.l[b]) & 1, pad = (B.l[a] + B.l[d]) & 1;
     const int tcb = (B.tz2[c] + B.tz2[b]) / 2;
-    const int k1 = ket_index(B.norb, min(c, b), max(c, b)), k2 = ket_index(B.norb, min(a, d), max(a, d));
     const double hY = P.hY;
     if (pcb == pad) {
       // two J per pass, every load of both issued before use; an invalid term
-      // reads a clamped in-bounds element and is weighted by 0
+      // reads a clamped in-bounds element and is weighted by 0.  Local indices of
+      // the kets (c,b) and (a,d) come from the per-orbit rows of b and a, where
+      // the lanes' consecutive c and d read neighbouring entries.
       const IMSRG_G double *sixt = B.sixj;
-      const IMSRG_G int *kmap = B.kmap;
-      const int nJ = B.jmax + 1;
+      const int nJ = B.jmax + 1, norb = B.norb;
+      const IMSRG_G int *kb1 = B.kmorb + (long)b * nJ * norb + c;
+      const IMSRG_G int *kb2 = B.kmorb + (long)a * nJ * norb + d;
       for (int J0 = jmin; J0 <= jmax; J0 += 2 * dJ) {
         double sj[2], xv[2], yv[2], wv2[2];
         int i1[2], i2[2], ch[2];
@@ -1172,8 +1356,8 @@ __global__ __launch_bounds__(256) void k_pandya(PairArg pa, const int4 *items) {
         for (int u = 0; u < 2; ++u) {
           const int J = min(J0 + u * dJ, jmax);
           sj[u] = sixt[sixj_index(B.d1, B.d2, ja, jb, Jcc, jc, jd, J)];
-          i1[u] = kmap[(long)k1 * nJ + J];
-          i2[u] = kmap[(long)k2 * nJ + J];
+          i1[u] = kb1[J * norb];
+          i2[u] = kb2[J * norb];
           ch[u] = 6 * J + 2 * (tcb + 1) + pcb;
         }
 #pragma unroll
@@ -1240,7 +1424,7 @@ __host__ __device__ inline size_t invp_lds_bytes(int jmax, int norb, int nch, in
   return 8 * (size_t)nch + 8 * nJ + 4 * (size_t)nch + 4 * nJ * (2 * (size_t)norb + 3) + 4 * (2 * (size_t)ncls + 1);
 }
 
-__global__ __launch_bounds__(256, 4) void k_invpandya(PairArg pa, const int4 *items) {
+__global__ __launch_bounds__(256, 5) void k_invpandya(PairArg pa, const int4 *items) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const int4 it = items[blockIdx.x];
   PairPtrs P = pa.get(it.x);
@@ -1272,10 +1456,11 @@ __global__ __launch_bounds__(256, 4) void k_invpandya(PairArg pa, const int4 *it
     s_ib[J] = B.kmapT[(long)J * nk + kb];
     s_chks[J] = B.ch_kstart[6 * J + g];
   }
+  // the bra's rows of the per-orbit cross-coupled index table: contiguous reads
+  const IMSRG_G int *ci = B.ccorb + (long)i * nJ * norb, *cj = B.ccorb + (long)j * nJ * norb;
   for (int e = threadIdx.x; e < nJ * norb; e += blockDim.x) {
-    const int Jp = e / norb, x = e - Jp * norb;
-    Ti[e] = B.cmapT[(long)Jp * nk + ket_index(norb, min(i, x), max(i, x))];
-    Tj[e] = B.cmapT[(long)Jp * nk + ket_index(norb, min(j, x), max(j, x))];
+    Ti[e] = ci[e];
+    Tj[e] = cj[e];
   }
   // kets e >= emin taken class by class: the lanes of a wave then mostly share
   // j_k and j_l, hence their J and J' ranges
@@ -1291,9 +1476,20 @@ __global__ __launch_bounds__(256, 4) void k_invpandya(PairArg pa, const int4 *it
     s_pre[c + 1] = end - lo;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    s_pre[0] = 0;
-    for (int c = 0; c < ncls; ++c) s_pre[c + 1] += s_pre[c];
+  // inclusive prefix sum of s_pre[1..ncls] by the first wave, 64 classes per pass
+  if (threadIdx.x < 64) {
+    if (threadIdx.x == 0) s_pre[0] = 0;
+    int carry = 0;
+    for (int c0 = 0; c0 < ncls; c0 += 64) {
+      const int c = c0 + (int)threadIdx.x;
+      int v = c < ncls ? s_pre[c + 1] : 0;
+      for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(v, o, 64);
+        if ((int)threadIdx.x >= o) v += u;
+      }
+      if (c < ncls) s_pre[c + 1] = v + carry;
+      carry += __shfl(v, 63, 64);
+    }
   }
   __syncthreads();
   const int total = s_pre[ncls];
@@ -1302,11 +1498,7 @@ __global__ __launch_bounds__(256, 4) void k_invpandya(PairArg pa, const int4 *it
   const double hf = (P.hZ == 1) ? 1.0 : -1.0;
   const double *Zb = P.Zbar;
   double *Ztwo = P.Z + B.two_off;
-  // held in registers: the stores below could alias B
-  typedef const __attribute__((address_space(4))) double c6;
-  typedef double c62v __attribute__((ext_vector_type(2)));
-  typedef const __attribute__((address_space(4))) c62v c62;
-  c6 *const sixc = (c6 *)B.sixjQ;
+  const IMSRG_G double *sixq = B.sixjQ;  // held in registers: the stores below could alias B
   const IMSRG_G int *kmapT = B.kmapT;
   const IMSRG_G int *cls_e = B.cls_e;
   for (int q0 = 0; q0 < total; q0 += blockDim.x) {
@@ -1352,98 +1544,82 @@ __global__ __launch_bounds__(256, 4) void k_invpandya(PairArg pa, const int4 *it
     const int pd = (B.l[i] + B.l[l]) & 1, td = abs(B.tz2[i] - B.tz2[l]) / 2;
     const int px = (B.l[i] + B.l[k]) & 1, tx = abs(B.tz2[i] - B.tz2[k]) / 2;
     // 6j entries of J' and the J pair (c0, c0+1) sit at s6 + J' * s6J + (c0/2) * s6P; the next
-    // pair (c0+2, c0+3) one pair stride further.  When every live lane of the
-    // wave has the same (j_k, j_l) (the class order makes that the common case)
-    // the 6j addresses are wave-uniform and come from scalar loads through the
-    // constant address space (the table is read-only here); otherwise each lane
-    // loads its own.  Both paths do the same arithmetic in the same order.
+    // pair (c0+2, c0+3) one pair stride further
     const long s6P = 2L * B.d1 * B.d1, s6J = s6P * (d2q >> 1);
-    const int jk0 = wave_max(live ? jjk : -1), jl0 = wave_max(live ? jjl : -1);
-    const bool uni = wave_min((!live || (jjk == jk0 && jjl == jl0)) ? 1 : 0) == 1;
+    const long s6d = sixjQ_index(B.d1, d2q, jji, jjj, 0, jjk, jjl, 0);
+    const long s6x = sixjQ_index(B.d1, d2q, jjj, jji, 0, jjk, jjl, 0);
     // tail element T(ik, lj) = phi_ik phi_lj T(lj, ik), phi_pq = (-1)^{(j_p+j_q)/2};
     // head H(ik, lj) = hZ-sign H(lj, ik)
     const double symx = (l > j) ? (double)sgn_parity((jji + jjk) / 2 + (jjl + jjj) / 2) : hf;
     const int dbra = (i == j), dket = (k == l);
     const double nrm = (dbra == dket) ? 1.0 + dbra : SQRT2_D;
     const double pij = sgn_parity((jji + jjj + jjk + jjl) / 2);
-    auto chunks = [&](auto uniform) {
-      constexpr bool UNI = decltype(uniform)::value;
-      // UNI: (j_k, j_l) from the wave's first live lane, in scalar registers
-      const long s6d = UNI ? sixjQ_index(B.d1, d2q, jji, jjj, 0, jk0, jl0, 0)
-                           : sixjQ_index(B.d1, d2q, jji, jjj, 0, jjk, jjl, 0);
-      const long s6x = UNI ? sixjQ_index(B.d1, d2q, jjj, jji, 0, jk0, jl0, 0)
-                           : sixjQ_index(B.d1, d2q, jjj, jji, 0, jjk, jjl, 0);
-      // J chunks start at even J so that each pair of 6j values is one aligned 16-byte load
-      for (int c0 = wlo & ~1; c0 <= whi; c0 += kJC) {
-        unsigned vm = 0;  // lane's valid J of this chunk
-  #pragma unroll
-        for (int t = 0; t < kJC; ++t) {
-          const int J = c0 + t;
-          if (J >= Jlo && J <= Jhi && !(pauli && (J & 1))) vm |= 1u << t;
-        }
-        // acc(J) = sum_J' w_d z_d - (-1)^{(jk+jl)/2 - J} sum_J' w_x z_x, i.e. -(commij - sign commji)
-        // before the special cases k == l, i == j (no exchange sum; factor applied below)
-        double acc[kJC];
-  #pragma unroll
-        for (int t = 0; t < kJC; ++t) acc[t] = 0.0;
-        const double sx0 = -(double)sgn_parity((jjk + jjl) / 2 - c0);  // -(-1)^{(jk+jl)/2 - J} at J = c0
-        for (int Jp = dlo; Jp <= dhi; ++Jp) {
-          if (Jp < Dlo || Jp > Dhi) continue;
-          const int cc = 6 * Jp + 2 * (td + 1) + pd;
-          const int nkc = s_ccn[cc];
-          const int il = Ti[Jp * norb + l];
-          const int kj = Tj[Jp * norb + k] + (k > j ? nkc : 0);
-          const double z = (2 * Jp + 1) * Zb[s_zoff[cc] + (long)kj * nkc + il];
-          const c6 *s6 = sixc + s6d + (long)Jp * s6J + (long)(c0 >> 1) * s6P;
-          // 0/1 factors rather than selects keep the 16-byte 6j loads unconditional
-          c62v w[kJC / 2];
-  #pragma unroll
-          for (int hh = 0; hh < kJC / 2; ++hh) w[hh] = *(const c62 *)(s6 + hh * s6P);
-  #pragma unroll
-          for (int hh = 0; hh < kJC / 2; ++hh) {
-            acc[2 * hh] += (((vm >> (2 * hh)) & 1) && fabs(w[hh][0]) >= 1e-8 ? 1.0 : 0.0) * (w[hh][0] * z);
-            acc[2 * hh + 1] += (((vm >> (2 * hh + 1)) & 1) && fabs(w[hh][1]) >= 1e-8 ? 1.0 : 0.0) * (w[hh][1] * z);
-          }
-        }
-        for (int Jp = xlo; Jp <= xhi; ++Jp) {
-          if (Jp < Xlo || Jp > Xhi) continue;
-          const int cc = 6 * Jp + 2 * (tx + 1) + px;
-          const int nkc = s_ccn[cc];
-          const int ik = Ti[Jp * norb + k];
-          const int lj = Tj[Jp * norb + l];
-          const double me = Zb[s_zoff[cc] + (long)((l > j ? nkc : 0) + ik) * nkc + lj];
-          const double z = (2 * Jp + 1) * (symx * me) * sx0;  // the sign alternates with J below
-          const c6 *s6 = sixc + s6x + (long)Jp * s6J + (long)(c0 >> 1) * s6P;
-          c62v w[kJC / 2];
-  #pragma unroll
-          for (int hh = 0; hh < kJC / 2; ++hh) w[hh] = *(const c62 *)(s6 + hh * s6P);
-  #pragma unroll
-          for (int hh = 0; hh < kJC / 2; ++hh) {
-            acc[2 * hh] += (((vm >> (2 * hh)) & 1) && fabs(w[hh][0]) >= 1e-8 ? 1.0 : 0.0) * (w[hh][0] * z);
-            acc[2 * hh + 1] -= (((vm >> (2 * hh + 1)) & 1) && fabs(w[hh][1]) >= 1e-8 ? 1.0 : 0.0) * (w[hh][1] * z);
-          }
-        }
-  #pragma unroll
-        for (int t = 0; t < kJC; ++t) {
-          if (!((vm >> t) & 1)) continue;
-          const int J = c0 + t;
-          // commji = commij for k == l, pij commij for i == j (no exchange sum then)
-          const double s = sgn_parity((jjk + jjl) / 2 - J);
-          const double f = (k == l) ? 1.0 - s : (i == j) ? 1.0 - s * pij : 1.0;
-          const double z = acc[t] * f / nrm;
-          const int n = s_chn[J];
-          const int ib = s_ib[J], ik = kmapT[(long)J * nk + kk];
-          if (ik == ib)  // diagonal: kept apart (the pp/hh GEMM may still be writing Z's diagonal)
-            P.diag[s_chks[J] + max(ib, 0)] = z;
-          else
-            Ztwo[s_choff[J] + (long)ib * n + ik] = z;
+    // J chunks start at even J so that each pair of 6j values is one aligned 16-byte load
+    for (int c0 = wlo & ~1; c0 <= whi; c0 += kJC) {
+      unsigned vm = 0;  // lane's valid J of this chunk
+#pragma unroll
+      for (int t = 0; t < kJC; ++t) {
+        const int J = c0 + t;
+        if (J >= Jlo && J <= Jhi && !(pauli && (J & 1))) vm |= 1u << t;
+      }
+      // acc(J) = sum_J' w_d z_d - (-1)^{(jk+jl)/2 - J} sum_J' w_x z_x, i.e. -(commij - sign commji)
+      // before the special cases k == l, i == j (no exchange sum; factor applied below)
+      double acc[kJC];
+#pragma unroll
+      for (int t = 0; t < kJC; ++t) acc[t] = 0.0;
+      const double sx0 = -(double)sgn_parity((jjk + jjl) / 2 - c0);  // -(-1)^{(jk+jl)/2 - J} at J = c0
+      for (int Jp = dlo; Jp <= dhi; ++Jp) {
+        if (Jp < Dlo || Jp > Dhi) continue;
+        const int cc = 6 * Jp + 2 * (td + 1) + pd;
+        const int nkc = s_ccn[cc];
+        const int il = Ti[Jp * norb + l];
+        const int kj = Tj[Jp * norb + k] + (k > j ? nkc : 0);
+        const double z = (2 * Jp + 1) * Zb[s_zoff[cc] + (long)kj * nkc + il];
+        const double *s6 = sixq + s6d + (long)Jp * s6J + (long)(c0 >> 1) * s6P;
+        // 0/1 factors rather than selects keep the 16-byte 6j loads unconditional
+        double2 w[kJC / 2];
+#pragma unroll
+        for (int hh = 0; hh < kJC / 2; ++hh) w[hh] = *(const double2 *)(s6 + hh * s6P);
+#pragma unroll
+        for (int hh = 0; hh < kJC / 2; ++hh) {
+          acc[2 * hh] += (((vm >> (2 * hh)) & 1) && fabs(w[hh].x) >= 1e-8 ? 1.0 : 0.0) * (w[hh].x * z);
+          acc[2 * hh + 1] += (((vm >> (2 * hh + 1)) & 1) && fabs(w[hh].y) >= 1e-8 ? 1.0 : 0.0) * (w[hh].y * z);
         }
       }
-    };
-    if (uni)
-      chunks(std::true_type());
-    else
-      chunks(std::false_type());
+      for (int Jp = xlo; Jp <= xhi; ++Jp) {
+        if (Jp < Xlo || Jp > Xhi) continue;
+        const int cc = 6 * Jp + 2 * (tx + 1) + px;
+        const int nkc = s_ccn[cc];
+        const int ik = Ti[Jp * norb + k];
+        const int lj = Tj[Jp * norb + l];
+        const double me = Zb[s_zoff[cc] + (long)((l > j ? nkc : 0) + ik) * nkc + lj];
+        const double z = (2 * Jp + 1) * (symx * me) * sx0;  // the sign alternates with J below
+        const double *s6 = sixq + s6x + (long)Jp * s6J + (long)(c0 >> 1) * s6P;
+        double2 w[kJC / 2];
+#pragma unroll
+        for (int hh = 0; hh < kJC / 2; ++hh) w[hh] = *(const double2 *)(s6 + hh * s6P);
+#pragma unroll
+        for (int hh = 0; hh < kJC / 2; ++hh) {
+          acc[2 * hh] += (((vm >> (2 * hh)) & 1) && fabs(w[hh].x) >= 1e-8 ? 1.0 : 0.0) * (w[hh].x * z);
+          acc[2 * hh + 1] -= (((vm >> (2 * hh + 1)) & 1) && fabs(w[hh].y) >= 1e-8 ? 1.0 : 0.0) * (w[hh].y * z);
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < kJC; ++t) {
+        if (!((vm >> t) & 1)) continue;
+        const int J = c0 + t;
+        // commji = commij for k == l, pij commij for i == j (no exchange sum then)
+        const double s = sgn_parity((jjk + jjl) / 2 - J);
+        const double f = (k == l) ? 1.0 - s : (i == j) ? 1.0 - s * pij : 1.0;
+        const double z = acc[t] * f / nrm;
+        const int n = s_chn[J];
+        const int ib = s_ib[J], ik = kmapT[(long)J * nk + kk];
+        if (ik == ib)  // diagonal: kept apart (the pp/hh GEMM may still be writing Z's diagonal)
+          P.diag[s_chks[J] + max(ib, 0)] = z;
+        else
+          Ztwo[s_choff[J] + (long)ib * n + ik] = z;
+      }
+    }
   }
 }
 
@@ -1681,11 +1857,13 @@ hipError_t launch_onebody(const PairArg &pa, const int4 *items, int nitems, cons
   return hipGetLastError();
 }
 hipError_t launch_pphh(const PairArg &pa, const int4 *items, int nitems, const int4 *items64, int nitems64,
-                       int precision, int maxn, hipStream_t s) {
+                       const int4 *pack, int npack, int precision, int maxn, hipStream_t s) {
   if (nitems > 0) {
     const size_t lds = sizeof(int) * (size_t)std::max(maxn, 1);  // column-index list of the largest channel
-    if (precision == kGemmB3)
-      hipLaunchKernelGGL(k_pphh_big_b3, dim3(nitems), dim3(kGWaves * 64), lds, s, pa, items);
+    if (precision == kGemmB3) {
+      if (npack > 0) hipLaunchKernelGGL(k_pack_pphh, dim3(npack), dim3(256), 0, s, pa, pack);
+      hipLaunchKernelGGL(k_pphh_big_b3p, dim3(nitems), dim3(kGWaves * 64), 0, s, pa, items);
+    }
     else if (precision == 1)
       hipLaunchKernelGGL(k_pphh_big_mixed, dim3(nitems), dim3(kGWaves * 64), lds, s, pa, items);
     else if (precision == 2)

@@ -16,10 +16,11 @@ hipError_t launch_zerobody(const PairArg &pa, int npairs, int maxnch, hipStream_
 // (items221, none when that term is off), then the sums into Z1
 hipError_t launch_onebody(const PairArg &pa, const int4 *items, int nitems, const int4 *items221, int n221,
                           int npairs, int maxn1pairs, hipStream_t s);
-// pp/hh SYR2K: items on 128x128 tiles, items64 on 64x64 tiles (disjoint channels)
+// pp/hh SYR2K: items on 128x128 tiles, items64 on 64x64 tiles (disjoint channels);
+// under bf16x3 products (kGemmB3) the pack items first write the operand panels
 // maxn: the largest channel dimension of the pairs (sizes the kernel's LDS index list)
 hipError_t launch_pphh(const PairArg &pa, const int4 *items, int nitems, const int4 *items64, int nitems64,
-                       int precision, int maxn, hipStream_t s);
+                       const int4 *pack, int npack, int precision, int maxn, hipStream_t s);
 // comm122 partner coefficients of this call (used by launch_finish with c122)
 hipError_t launch_c122_coef(const PairArg &pa, int npairs, long maxn122, hipStream_t s);
 // maxnch: the largest channel count of the pairs (sizes the kernel's LDS channel table)

@@ -171,7 +171,7 @@ int imsrg_hip_commutator_ss_dev_async(imsrg_ctx *ctx, const imsrg_dop *X, const 
  * memory (code under HIP graph capture, tight serving loops).  The same
  * commutator as imsrg_hip_commutator_ss_dev, with every per-call workspace
  * (Pandya matrices, partial sums, comm122 coefficients) carved from one static
- * device array of 32 MiB per GPU, held from module load until the process
+ * device array of 48 MiB per GPU, held from module load until the process
  * ends; the call allocates nothing.  Bases whose workspaces exceed it (above
  * emax = 6 for the O16 reference) return IMSRG_E_INVALID.  All calls on one
  * GPU share the array, so calls that overlap on one GPU (other host threads,
@@ -267,9 +267,11 @@ typedef struct {
   int pphh_order;           /* 0: pp/hh tiles by decreasing k-length; 1: row-block groups balanced over XCDs;
                                2: 8x8-tile super-tiles balanced over XCDs (shared operand blocks in one L2) */
   int mixed_pphh_fp32;      /* IMSRG_PREC_MIXED: pp/hh GEMM on 2 = bf16 MFMA with three-way split
-                               operands (default), 1 = FP32 MFMA on FP32-rounded operands, 0 = FP64
-                               MFMA (IMSRG_PREC_SPLIT ignores both flags) */
-  int mixed_ph_fp32;        /* IMSRG_PREC_MIXED: ph GEMM, same choices (default 2) */
+                               operands (default; channels on 64x64 tiles stay FP64), 1 = FP32 MFMA
+                               on FP32-rounded operands, 0 = FP64 MFMA (IMSRG_PREC_SPLIT ignores both
+                               flags) */
+  int mixed_ph_fp32;        /* IMSRG_PREC_MIXED: ph GEMM, same choices (default 1: its time is set by
+                               the FP64 Zbar it writes, and the bf16 kernel is slower there) */
   int ph_tile_128;          /* 1: ph GEMM on 128x128 tiles, 0: on 64x64 tiles */
   int finish_order;         /* tile pairs of the last kernel (fold, comm122, mirror): 0 in channel and
                                row-block order, 1 cut into 8 runs of consecutive tiles, one per XCD */

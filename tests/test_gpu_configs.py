@@ -407,24 +407,23 @@ def test_solver_omega_list_matches_oracle(ia, po, tmp_path, scratch):
 def test_async_commutator_on_a_caller_stream(ia):
     import ctypes as C
 
-    import torch
     from imsrg_amd import _lib
     ms = ia.ModelSpace(4, "O16")
-    say("[gpu] stream-ordered commutator on a torch stream")
+    say("[gpu] stream-ordered commutator on a caller stream")
     X = ia.DeviceOperator(ms, -1).fill_random(-1, 61)
     Y = ia.DeviceOperator(ms, 1).fill_random(1, 62)
     ref = host(ia.Commutator.CommutatorScalarScalar(X, Y))
-    st = torch.cuda.Stream(device=0)
-    Z = ia.DeviceOperator(ms, -1)
-    torch.cuda._sleep(1 << 22)  # work queued ahead on the null stream, not on st
-    with torch.cuda.stream(st):
-        torch.cuda._sleep(1 << 22)  # the commutator must start after this
-        _lib.check(_lib.lib().imsrg_hip_commutator_ss_dev_async(ms.h, X.h, Y.h, Z.h,
-                                                                C.byref(_lib.CommOpts(0xFF, 0, 0)),
-                                                                C.c_void_p(st.cuda_stream)), ms.h)
-        marker = torch.zeros(1, device="cuda")
-        marker += 1  # queued after the commutator on st
-    st.synchronize()
-    assert Z.herm == 1 and marker.item() == 1.0
-    got = host(Z)
-    assert np.array_equal(got["two"], ref["two"]) and np.array_equal(got["one"], ref["one"])
+    hip = C.CDLL("libamdhip64.so")  # the HIP runtime the library already loaded
+    st = C.c_void_p()
+    assert hip.hipStreamCreate(C.byref(st)) == 0
+    try:
+        f = _lib.lib().imsrg_hip_commutator_ss_dev_async
+        for stream in (st, C.c_void_p(None)):  # a created stream and the null stream
+            Z = ia.DeviceOperator(ms, -1)
+            _lib.check(f(ms.h, X.h, Y.h, Z.h, C.byref(_lib.CommOpts(0xFF, 0, 0)), stream), ms.h)
+            assert Z.herm == 1  # set at once
+            assert hip.hipStreamSynchronize(stream) == 0
+            got = host(Z)
+            assert np.array_equal(got["two"], ref["two"]) and np.array_equal(got["one"], ref["one"])
+    finally:
+        hip.hipStreamDestroy(st)
