@@ -90,6 +90,10 @@ def profiled_traffic(kernel):
         return None
 
 
+def _sum_or_none(*v):
+    return None if any(x is None for x in v) else sum(v)
+
+
 def _errors(Z, R, np):
     scale = max(np.max(np.abs(R.TwoBody)), np.max(np.abs(R.OneBody)))
     rel = float(max(np.max(np.abs(Z.TwoBody - R.TwoBody)), np.max(np.abs(Z.OneBody - R.OneBody))) / scale)
@@ -174,7 +178,8 @@ def workload_emax12(args, ia, np, emax=12):
     res["split"].update({"rel_dZ": rel_s, "abs_dE0": de0_s})
     d = np.abs(hosts["mixed"].TwoBody - hosts["fp64"].TwoBody)
     return {"metric": f"mixed-precision error at emax={emax} vs the FP64 GPU result", "value": rel,
-            "unit": "max-abs dZ / max |Z|", "n_gpus": 1, "higher_is_better": False, "dtype": "f32 MFMA + f64",
+            "unit": "max-abs dZ / max |Z|", "n_gpus": 1, "higher_is_better": False,
+            "dtype": "mixed: bf16x3 MFMA pp/hh + f32 MFMA ph + f64; split: 3 f32 MFMA products + f64",
             "data": f"synthetic (seeds {1000 * emax + 1}/{1000 * emax + 2})",
             "config": {"workload": f"scalar IMSRG(2) commutator, HO emax={emax}, O16 reference", "norb": ms.norb,
                        "two_body_doubles": ms.tbsize},
@@ -252,7 +257,7 @@ def workload_flow(args, ia, np):
     drift = max(abs(a - b) for a, b in zip(res["fp64"]["E0"][:n], res["mixed"]["E0"][:n])) if n else None
     return {"metric": "Magnus IMSRG(2) flow at emax=10: E0(s) drift of the mixed-precision flow vs FP64",
             "value": drift, "unit": "max_s |E0_mixed(s) - E0_fp64(s)|", "n_gpus": 1, "higher_is_better": False,
-            "dtype": "f32 MFMA + f64", "data": "synthetic Hamiltonian (seed 5001)",
+            "dtype": "mixed: bf16x3 MFMA pp/hh + f32 MFMA ph + f64", "data": "synthetic Hamiltonian (seed 5001)",
             "config": {"workload": "Magnus-Euler flow, White generator, ds=0.5, BCH nested-norm cut 1e-8, "
                                    "HO emax=10, O16 reference", "max_steps": steps},
             "fp64": res["fp64"], "mixed": res["mixed"], "fp64_graphs": res["fp64_graphs"],
@@ -377,12 +382,16 @@ def main():
         # algorithm's flops is the bf16 matrix peak over its six products.
         fam = tun["mixed_pphh_fp32"] if use_mixed else 0
         peak = {0: PEAK_FP64_MFMA, 1: PEAK_FP32_MFMA, 2: PEAK_BF16_MFMA / B3_PRODUCTS}[fam]
-        kname = {0: "k_pphh_big_f64", 1: "k_pphh_big_mixed", 2: "k_pphh_big_b3"}[fam]
+        kname = {0: "k_pphh_big_f64", 1: "k_pphh_big_mixed", 2: "k_pphh_big_b3p"}[fam]
         ach = FA / t_dom
-        roof = {"kernel": kname + (" + k_syr2k<0> (channels n <= %d)" % tun["pphh_small_tile_max"]
-                                   if tun["pphh_small_tile_max"] > 0 else ""),
+        roof = {"kernel": ("k_pack_pphh + " if fam == 2 else "") + kname +
+                          (" + k_syr2k<0> (channels n <= %d)" % tun["pphh_small_tile_max"]
+                           if tun["pphh_small_tile_max"] > 0 else ""),
                 "bound": "mfma", "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-                "frac": ach / peak, "traffic": profiled_traffic(kname), "algorithmic_flops": FA,
+                "frac": ach / peak,
+                "traffic": (profiled_traffic(kname) if fam != 2 else
+                            _sum_or_none(profiled_traffic("k_pack_pphh"), profiled_traffic(kname))),
+                "algorithmic_flops": FA,
                 "executed_flops": execA}
     elif PHASES[dom] == "inverse_pandya":
         ach = BI / t_dom
