@@ -121,6 +121,31 @@ def test_batched_cache_follows_buffers_not_handles(ia):
         del X, Y, Z
 
 
+def test_batched_mixed_equals_loop_with_bf16x3_panels(ia):
+    # mixed precision with every channel on 128x128 pp/hh tiles: each pair's
+    # bf16x3 operand panels live in its own batch workspace; the grouped launch
+    # must equal single commutators (the context's own panel buffer) bit for bit
+    emaxes = [6, 5, 7, 6, 5, 7]
+    spaces = {e: ia.ModelSpace(e, "O16") for e in sorted(set(emaxes))}
+    for ms in spaces.values():
+        ms.set_tuning(pphh_small_tile_max=0, mixed_pphh_fp32=2)
+    say(f"[gpu] batched mixed pairs, emax {emaxes}, bf16x3 pp/hh on 128-tiles")
+    pairs = []
+    for i, e in enumerate(emaxes):
+        ms = spaces[e]
+        X = ia.DeviceOperator(ms, -1).fill_random(-1, 4100 + 2 * i)
+        Y = ia.DeviceOperator(ms, 1).fill_random(1, 4101 + 2 * i)
+        pairs.append((X, Y, ia.DeviceOperator(ms, 1)))
+    ia.Commutator.CommutatorScalarScalarBatched(pairs, precision=ia.PREC_MIXED)
+    for i, (X, Y, Z) in enumerate(pairs):
+        got = host(Z)
+        ref = host(ia.Commutator.CommutatorScalarScalar(X, Y, precision=ia.PREC_MIXED))
+        r64 = host(ia.Commutator.CommutatorScalarScalar(X, Y, precision=ia.PREC_FP64))
+        assert np.array_equal(ref["two"], got["two"]) and np.array_equal(ref["one"], got["one"]), f"pair {i}"
+        e0, e1, e2 = errors(got, r64)
+        assert e0 < 1e-6 and max(e1, e2) < 1e-7 and e2 > 1e-13, f"pair {i}: rel={max(e1, e2):.3e}"
+
+
 # ---- emax = 12 --------------------------------------------------------------
 
 @pytest.fixture(scope="module")

@@ -62,15 +62,20 @@ def test_mixed_commutator_meets_accuracy_bar(ia, emax):
         assert np.array_equal(blk, blk.T)
 
 
+# small = 0 puts every pp/hh channel on the 128x128 tiles, so the bf16x3 panel
+# kernel meets channels whose sizes are not multiples of the tile (n < 128) or
+# of the 32-k stage
+@pytest.mark.parametrize("small", [64, 0])
 @pytest.mark.parametrize("term", ["comm222_pp_hhss", "comm222_phss"])
-def test_mixed_gemm_terms_against_oracle(ia, term):
+def test_mixed_gemm_terms_against_oracle(ia, term, small):
     from oracle.pyoracle import TERMS, OracleModelSpace
     ms, orc = ia.ModelSpace(4, "O16"), OracleModelSpace(4, 16, 8)
     X = random_op(orc, -1, 901)
     Y = random_op(orc, 1, 902)
-    say(f"[gpu] mixed {term} term vs oracle emax=4")
+    say(f"[gpu] mixed {term} term vs oracle emax=4, small-tile limit {small}")
     mask = TERMS[term]
     ref = orc.commutator(X, Y, mask=mask)
+    ms.set_tuning(pphh_small_tile_max=small)
     dX = ia.Operator(ms, -1, X["zero"], X["one"], X["two"]).to_device()
     dY = ia.Operator(ms, 1, Y["zero"], Y["one"], Y["two"]).to_device()
     Z = from_dev(ia.Commutator.CommutatorScalarScalar(dX, dY, mask=mask, precision=ia.PREC_MIXED))
