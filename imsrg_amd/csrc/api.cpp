@@ -37,17 +37,17 @@ struct BatchCache {
   std::vector<long long> key;
   PairPtrs *d_tab = nullptr;
   int4 *d_items = nullptr;
-  size_t off[9] = {0};
-  int cnt[9] = {0};
+  size_t off[10] = {0};
+  int cnt[10] = {0};
   double *work = nullptr;
-  int maxn = 0, maxnch = 0, maxjmax = 0, maxnorb = 0, maxn1pairs = 0, maxncls = 0;
+  int maxn = 0, maxnch = 0, maxjmax = 0, maxnorb = 0, maxn1pairs = 0, maxncls = 0, maxbin = 0, maxent = 0;
   long maxn122 = 0;
   void release() {
     if (d_tab) (void)hipFree(d_tab);
     if (d_items) (void)hipFree(d_items);
     if (work) (void)hipFree(work);
     d_tab = nullptr; d_items = nullptr; work = nullptr;
-    maxn = maxnch = maxjmax = maxnorb = maxn1pairs = maxncls = 0;
+    maxn = maxnch = maxjmax = maxnorb = maxn1pairs = maxncls = maxbin = maxent = 0;
     maxn122 = 0;
     key.clear();
   }
@@ -145,6 +145,7 @@ struct imsrg_ctx {
   double *obpart = nullptr;  // one-body partial sums (nobpart)
   double *diag = nullptr;    // inverse-Pandya diagonal terms (ndiag)
   double *panel = nullptr;   // bf16x3 operand panels of the mixed pp/hh GEMM (allocated on first use)
+  double *c122w = nullptr;   // comm122 W in the two-body layout (tbsize + 8)
   double *partials = nullptr;
   double *norm_dev = nullptr;
   double *norm_host = nullptr;  // pinned
@@ -224,6 +225,7 @@ static int ctx_init(imsrg_ctx *ctx, const imsrg_basis_desc *b, int device) {
   HIPCHK(ctx, hipMalloc(&ctx->c122, sizeof(double2) * std::max<long>(B.n122, 1)));
   HIPCHK(ctx, hipMalloc(&ctx->obpart, sizeof(double) * std::max(B.nobpart, 1)));
   HIPCHK(ctx, hipMalloc(&ctx->diag, sizeof(double) * std::max(B.ndiag, 1)));
+  HIPCHK(ctx, hipMalloc(&ctx->c122w, sizeof(double) * (B.tbsize + 8)));
   ctx->npart = 257;
   HIPCHK(ctx, hipMalloc(&ctx->partials, sizeof(double) * ctx->npart));
   HIPCHK(ctx, hipMalloc(&ctx->norm_dev, sizeof(double) * 2));
@@ -282,6 +284,7 @@ extern "C" int imsrg_hip_ctx_destroy(imsrg_ctx *ctx) {
   if (ctx->obpart) (void)hipFree(ctx->obpart);
   if (ctx->diag) (void)hipFree(ctx->diag);
   if (ctx->panel) (void)hipFree(ctx->panel);
+  if (ctx->c122w) (void)hipFree(ctx->c122w);
   if (ctx->partials) (void)hipFree(ctx->partials);
   if (ctx->norm_dev) (void)hipFree(ctx->norm_dev);
   if (ctx->norm_host) (void)hipHostFree(ctx->norm_host);
@@ -650,7 +653,8 @@ static int herm_rule(int hx, int hy) { return (hx == hy) ? -1 : 1; }  // src/Com
 // items / cnt: [0] pp/hh 128-tiles, [1] ph tiles, [2] comm221 one-body items,
 // [3] Pandya chunks, [4] inverse-Pandya items,
 // [5] one-body pairs, [6] tile pairs of the finishing kernel,
-// [7] pp/hh 64-tiles, [8] bf16x3 panel packing.  Under mixed precision the
+// [7] pp/hh 64-tiles, [8] bf16x3 panel packing, [9] comm122 W column bins.
+// Under mixed precision the
 // context's tuning picks the kernel of each GEMM family.
 // Sums the phase durations of the oldest recorded event set (waits for it).
 static int harvest_oldest(imsrg_ctx *ctx) {
@@ -686,7 +690,7 @@ static bool uses_panels(const imsrg_ctx *ctx, int prec) {
 
 static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const int4 *const *items, const int *cnt,
                           unsigned mask, int prec, int maxn, int maxnch, long maxn122, int maxjmax, int maxnorb,
-                          int maxn1pairs, int maxncls) {
+                          int maxn1pairs, int maxncls, int maxbin, int maxent) {
   const Tuning &tu = ctx->db.tune;
   // split products take both GEMM families whatever the mixed-precision choices
   const int prec_pphh = gemm_family(prec, tu.mixed_pphh);
@@ -719,6 +723,7 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
     if (set) HIPCHK(ctx, hipEventRecord((*set)[10], ctx->side));
     if (mask & M122) {
       HIPCHK(ctx, launch_c122_coef(pa, npairs, maxn122, ctx->side));
+      HIPCHK(ctx, launch_c122_w(pa, items[9], cnt[9], maxbin, maxent, ctx->side));
       HIPCHK(ctx, launch_c122_upper(pa, items[6], cnt[6], ctx->side));
     }
     if (set) HIPCHK(ctx, hipEventRecord((*set)[11], ctx->side));
@@ -740,7 +745,10 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
     HIPCHK(ctx, launch_zero_two(pa, items[6], cnt[6], s));
   }
   HIPCHK(ctx, mark(3));
-  if ((mask & M122) && !overlap) HIPCHK(ctx, launch_c122_coef(pa, npairs, maxn122, s));
+  if ((mask & M122) && !overlap) {
+    HIPCHK(ctx, launch_c122_coef(pa, npairs, maxn122, s));
+    HIPCHK(ctx, launch_c122_w(pa, items[9], cnt[9], maxbin, maxent, s));
+  }
   HIPCHK(ctx, mark(4));
   if (mask & MPH) HIPCHK(ctx, launch_pandya(pa, items[3], cnt[3], maxnch, s));
   HIPCHK(ctx, mark(5));
@@ -762,13 +770,13 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
 struct Workspace {
   double *Xt, *Yt, *Zbar, *scratch;
   double2 *c122;
-  double *obpart, *diag, *panel;
+  double *obpart, *diag, *c122w, *panel;
 };
 // panels: room for the bf16x3 operand panels (last region, used by the mixed
 // pp/hh GEMM only)
 static long workspace_doubles(const BasisDev &B, bool panels) {
   return 2 * r8(B.xsize + 8) + r8(B.zsize) + r8(B.nch + 8) + 2 * r8(B.n122) + r8(B.nobpart) + r8(B.ndiag) +
-         (panels ? r8(2 * B.pk_size) : 0) + 64;
+         r8(B.tbsize + 8) + (panels ? r8(2 * B.pk_size) : 0) + 64;
 }
 static Workspace carve_workspace(const BasisDev &B, double *base) {
   Workspace w;
@@ -780,7 +788,8 @@ static Workspace carve_workspace(const BasisDev &B, double *base) {
   w.c122 = (double2 *)(w.scratch + r8(B.nch + 8));
   w.obpart = (double *)(w.c122) + 2 * r8(B.n122);
   w.diag = w.obpart + r8(B.nobpart);
-  w.panel = w.diag + r8(B.ndiag);
+  w.c122w = w.diag + r8(B.ndiag);
+  w.panel = w.c122w + r8(B.tbsize + 8);
   return w;
 }
 
@@ -811,19 +820,21 @@ static int commutator_dev(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y
   pa.p0.obpart = ws ? ws->obpart : ctx->obpart;
   pa.p0.diag = ws ? ws->diag : ctx->diag;
   pa.p0.panel = ws ? ws->panel : ctx->panel;
+  pa.p0.c122w = ws ? ws->c122w : ctx->c122w;
   pa.p0.hX = X->herm;
   pa.p0.hY = Y->herm;
   pa.p0.hZ = herm_rule(X->herm, Y->herm);
   pa.p0.mask = mask;
   Z->herm = pa.p0.hZ;
-  const int4 *items[9] = {db.d_pphh, db.d_ph,     db.d_ob221,  db.d_pandya, db.d_invp,
-                          db.d_one,  db.d_mirror, db.d_pphh64, db.d_pack};
-  int cnt[9] = {(int)db.wl.pphh.size(),   (int)db.wl.ph.size(),  (int)db.wl.ob221.size(),
-                (int)db.wl.pandya.size(), (int)db.wl.invp.size(), (int)db.wl.one.size(),
-                (int)db.wl.mirror.size(), (int)db.wl.pphh64.size(), (int)db.wl.pack.size()};
+  const int4 *items[10] = {db.d_pphh, db.d_ph,     db.d_ob221,  db.d_pandya, db.d_invp,
+                           db.d_one,  db.d_mirror, db.d_pphh64, db.d_pack,   db.d_c122w};
+  int cnt[10] = {(int)db.wl.pphh.size(),   (int)db.wl.ph.size(),   (int)db.wl.ob221.size(),
+                 (int)db.wl.pandya.size(), (int)db.wl.invp.size(),  (int)db.wl.one.size(),
+                 (int)db.wl.mirror.size(), (int)db.wl.pphh64.size(), (int)db.wl.pack.size(),
+                 (int)db.wl.c122w.size()};
   if (!ctx->use_graphs || ctx->profiling || ws)
     return run_commutator(ctx, pa, 1, items, cnt, mask, prec, db.h.maxn, db.h.nch, db.h.n122, db.h.jmax,
-                          db.h.norb, db.h.n1pairs, db.h.ncls);
+                          db.h.norb, db.h.n1pairs, db.h.ncls, db.h.c122_maxbin, db.h.c122_maxent);
   // replay a captured launch sequence for this operand set, capturing it on first use
   for (const GraphEntry &g : ctx->graphs)
     if (g.x == X->buf && g.y == Y->buf && g.z == Z->buf && g.hx == X->herm && g.hy == Y->herm && g.mask == mask &&
@@ -833,7 +844,7 @@ static int commutator_dev(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y
     }
   HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
   const int rc = run_commutator(ctx, pa, 1, items, cnt, mask, prec, db.h.maxn, db.h.nch, db.h.n122, db.h.jmax,
-                                db.h.norb, db.h.n1pairs, db.h.ncls);
+                                db.h.norb, db.h.n1pairs, db.h.ncls, db.h.c122_maxbin, db.h.c122_maxent);
   hipGraph_t graph = nullptr;
   const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);  // always leave capture mode
   if (rc) {
@@ -1514,19 +1525,19 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
       p.X = X[i]->buf; p.Y = Y[i]->buf; p.Z = Z[i]->buf;
       const Workspace w = carve_workspace(B, bc.work + woff[i]);
       p.Xt = w.Xt; p.Yt = w.Yt; p.Zbar = w.Zbar; p.scratch = w.scratch;
-      p.c122 = w.c122; p.obpart = w.obpart; p.diag = w.diag; p.panel = w.panel;
+      p.c122 = w.c122; p.obpart = w.obpart; p.diag = w.diag; p.panel = w.panel; p.c122w = w.c122w;
       p.hX = X[i]->herm; p.hY = Y[i]->herm; p.hZ = herm_rule(X[i]->herm, Y[i]->herm);
       p.mask = mask;
     }
     HIPCHK(c0, hipMalloc(&bc.d_tab, sizeof(PairPtrs) * npairs));
     HIPCHK(c0, hipMemcpy(bc.d_tab, tab.data(), sizeof(PairPtrs) * npairs, hipMemcpyHostToDevice));
     // concatenated lists; pp/hh tiles of all pairs go through the XCD-balanced order
-    std::vector<int4> all[9];
+    std::vector<int4> all[10];
     std::vector<TileGroup> groups;
     for (int i = 0; i < npairs; ++i) {
       const WorkLists &wl = ctxs[i]->db.wl;
-      const std::vector<int4> *src[9] = {&wl.pphh, &wl.ph,     &wl.ob221,  &wl.pandya, &wl.invp,
-                                         &wl.one,  &wl.mirror, &wl.pphh64, &wl.pack};
+      const std::vector<int4> *src[10] = {&wl.pphh, &wl.ph,     &wl.ob221,  &wl.pandya, &wl.invp,
+                                          &wl.one,  &wl.mirror, &wl.pphh64, &wl.pack,   &wl.c122w};
       {  // ph tiles in the tile size of the first context, whose kernels run the batch
         const int tile = c0->db.tune.ph_big ? kTileB : kTile;
         const DeviceBasis &dbi = ctxs[i]->db;
@@ -1538,7 +1549,7 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
             for (int sb = rb; sb < nt; ++sb) all[1].push_back(make_int4(i, c, rb, sb));
         }
       }
-      for (int t = 2; t < 9; ++t)
+      for (int t = 2; t < 10; ++t)
         for (int4 w : *src[t]) {
           w.x = i;
           all[t].push_back(w);
@@ -1560,6 +1571,8 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
       bc.maxn1pairs = std::max(bc.maxn1pairs, ctxs[i]->db.h.n1pairs);
       bc.maxn122 = std::max(bc.maxn122, ctxs[i]->db.h.n122);
       bc.maxncls = std::max(bc.maxncls, ctxs[i]->db.h.ncls);
+      bc.maxbin = std::max(bc.maxbin, ctxs[i]->db.h.c122_maxbin);
+      bc.maxent = std::max(bc.maxent, ctxs[i]->db.h.c122_maxent);
     }
     if (c0->db.tune.pphh_order >= 1) {
       all[0] = xcd_order(groups, kXcd);
@@ -1570,13 +1583,13 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
       for (auto &g : groups) all[0].insert(all[0].end(), g.tiles.begin(), g.tiles.end());
     }
     size_t tot = 0;
-    for (int t = 0; t < 9; ++t) {
+    for (int t = 0; t < 10; ++t) {
       bc.off[t] = tot;
       bc.cnt[t] = (int)all[t].size();
       tot += all[t].size();
     }
     HIPCHK(c0, hipMalloc(&bc.d_items, sizeof(int4) * std::max<size_t>(tot, 1)));
-    for (int t = 0; t < 9; ++t)
+    for (int t = 0; t < 10; ++t)
       if (!all[t].empty())
         HIPCHK(c0, hipMemcpy(bc.d_items + bc.off[t], all[t].data(), sizeof(int4) * all[t].size(),
                              hipMemcpyHostToDevice));
@@ -1586,8 +1599,8 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
   PairArg pa;
   std::memset(&pa.p0, 0, sizeof(pa.p0));
   pa.tab = bc.d_tab;
-  const int4 *items[9];
-  for (int t = 0; t < 9; ++t) items[t] = bc.d_items + bc.off[t];
+  const int4 *items[10];
+  for (int t = 0; t < 10; ++t) items[t] = bc.d_items + bc.off[t];
   // The batch runs on c0's stream.  Work queued on the other contexts' streams
   // (uploads, copies, axpy, generators of X_i / Y_i) finishes before it starts,
   // and their later work (downloads, norms of Z_i) waits for it to end.
@@ -1599,7 +1612,7 @@ extern "C" int imsrg_hip_commutator_ss_batched(int npairs, imsrg_ctx *const *ctx
     HIPCHK(c0, hipStreamWaitEvent(c0->stream, c->batch_ev, 0));
   }
   rc = run_commutator(c0, pa, npairs, items, bc.cnt, mask, prec, bc.maxn, bc.maxnch, bc.maxn122, bc.maxjmax,
-                      bc.maxnorb, bc.maxn1pairs, bc.maxncls);
+                      bc.maxnorb, bc.maxn1pairs, bc.maxncls, bc.maxbin, bc.maxent);
   if (rc) return rc;
   if (!others.empty()) {
     HIPCHK(c0, hipEventRecord(c0->batch_ev, c0->stream));

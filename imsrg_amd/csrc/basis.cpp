@@ -315,6 +315,70 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
   for (size_t i = 0; i + 1 < c122_start.size(); ++i)  // kets without partners
     c122_start[i + 1] = std::max(c122_start[i + 1], c122_start[i]);
   B.n122 = (long)c122_ind.size();
+  // comm122 column bins.  The partners of ket (p, q) are kets (a, q) and (p, a)
+  // with a in the one-body block of p or q, so the kets of one unordered pair
+  // of one-body blocks form a group closed under partners.  Groups are packed
+  // into bins of at most kC122Cols kets (first fit, largest first); the W
+  // kernel stages a bin's columns of X2 and Y2 in LDS, and c122_gpos gives each
+  // partner entry's position within its bin.
+  std::vector<int> c122_bcols, c122_boff(1, 0), c122_gpos(c122_ind.size(), 0);
+  // bin entries: c122_bent[c122_beoff[b] ..] the global entry indices of bin b,
+  // column by column; c122_bcend[c122_boff[b] + cp] the end of column cp's
+  // entries within the bin
+  std::vector<int> c122_bent, c122_beoff(1, 0), c122_bcend;
+  c122w_items.clear();
+  int c122_maxbin = 1, c122_maxent = 4;
+  {
+    std::vector<int> blk(norb);
+    for (int o = 0; o < norb; ++o) blk[o] = *std::min_element(hb.ob[o].begin(), hb.ob[o].end());
+    for (int c = 0; c < nch; ++c) {
+      const int n = ch_n[c];
+      if (n == 0) continue;
+      std::map<std::pair<int, int>, std::vector<int>> groups;
+      for (int i = 0; i < n; ++i) {
+        const int k = ch_kets[ch_kstart[c] + i];
+        const int b1 = blk[hb.kp[k]], b2 = blk[hb.kq[k]];
+        groups[{std::min(b1, b2), std::max(b1, b2)}].push_back(i);
+      }
+      std::vector<std::vector<int>> gl;
+      for (auto &g : groups) gl.push_back(g.second);
+      std::stable_sort(gl.begin(), gl.end(),
+                       [](const std::vector<int> &a, const std::vector<int> &b) { return a.size() > b.size(); });
+      std::vector<std::vector<int>> bins;
+      for (auto &g : gl) {
+        // a group larger than the bin capacity (emax > 14) takes a bin of its own
+        if ((int)g.size() > kC122MaxCols) return "comm122 ket group larger than the W kernel's LDS";
+        size_t t = 0;
+        while (t < bins.size() && bins[t].size() + g.size() > (size_t)kC122Cols) ++t;
+        if (t == bins.size()) bins.emplace_back();
+        bins[t].insert(bins[t].end(), g.begin(), g.end());
+      }
+      std::vector<int> pos(n, -1);
+      for (auto &bn : bins) c122_maxbin = std::max(c122_maxbin, (int)bn.size());
+      for (auto &bn : bins) {
+        for (size_t t = 0; t < bn.size(); ++t) pos[bn[t]] = (int)t;
+        const int ent0 = (int)c122_bent.size();
+        for (int q : bn) {
+          c122_bcols.push_back(q);
+          for (int e = c122_start[ch_kstart[c] + q]; e < c122_start[ch_kstart[c] + q + 1]; ++e) {
+            c122_gpos[e] = pos[c122_ind[e]];
+            c122_bent.push_back(e);
+          }
+          c122_bcend.push_back((int)c122_bent.size() - ent0);
+        }
+        c122_beoff.push_back((int)c122_bent.size());
+        c122_maxent = std::max(c122_maxent, (int)c122_bent.size() - ent0);
+        const int bi = (int)c122_boff.size() - 1;
+        c122_boff.push_back((int)c122_bcols.size());
+        for (int r = 0; r < n; r += kC122Rows) c122w_items.push_back(make_int4(0, c, bi, r / kC122Rows));
+      }
+      for (auto &bn : bins)  // every partner of a bin's column lies in the bin
+        for (int q : bn)
+          for (int e = c122_start[ch_kstart[c] + q]; e < c122_start[ch_kstart[c] + q + 1]; ++e)
+            if (pos[c122_ind[e]] < 0 || (int)bn.size() <= pos[c122_ind[e]] || bn[pos[c122_ind[e]]] != c122_ind[e])
+              return "comm122 partner outside its ket group";
+    }
+  }
   B.maxn = maxn;
   if ((long)maxn * maxn >= (1L << 29)) return "two-body channel too large (kernels address a channel block with 32-bit byte offsets)";
   {
@@ -348,6 +412,13 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
   B.khh_start = dupload(*this, khh_start, err); B.khh_loc = dupload(*this, khh_loc, err);
   B.khh_w = dupload(*this, khh_w, err);
   B.c122_start = dupload(*this, c122_start, err); B.c122_ind = dupload(*this, c122_ind, err);
+  B.c122_bcols = dupload(*this, c122_bcols, err); B.c122_boff = dupload(*this, c122_boff, err);
+  B.c122_gpos = dupload(*this, c122_gpos, err);
+  B.c122_bent = dupload(*this, c122_bent, err);
+  B.c122_beoff = dupload(*this, c122_beoff, err);
+  B.c122_bcend = dupload(*this, c122_bcend, err);
+  B.c122_maxbin = c122_maxbin;
+  B.c122_maxent = c122_maxent;
   B.c122_ao = dupload(*this, c122_ao, err); B.c122_f = dupload(*this, c122_f, err);
   B.gcore_start = dupload(*this, gcore_start, err); B.gcore_loc = dupload(*this, gcore_loc, err);
   B.gnc_start = dupload(*this, gnc_start, err); B.gnc_loc = dupload(*this, gnc_loc, err);
@@ -576,6 +647,7 @@ std::string DeviceBasis::build_lists(const Tuning &t) {
       for (int sb = rb; sb < nt; ++sb) wl.mirror.push_back(make_int4(0, c, rb, sb));
     if (gen_ok[c]) wl.gen.push_back(make_int4(0, c, 0, 0));
   }
+  wl.c122w = c122w_items;
   if (t.finish_order == 1) {
     // XCD-contiguous order: workgroup i runs on XCD i % 8, so the list is cut
     // into 8 runs of consecutive tiles (whole block rows, mostly) of about equal
@@ -654,6 +726,7 @@ std::string DeviceBasis::build_lists(const Tuning &t) {
   d_pphh = up(wl.pphh); d_pphh64 = up(wl.pphh64); d_ph = up(wl.ph); d_pack = up(wl.pack);
   d_pandya = up(wl.pandya); d_invp = up(wl.invp);
   d_one = up(wl.one); d_mirror = up(wl.mirror); d_gen = up(wl.gen); d_ob221 = up(wl.ob221);
+  d_c122w = up(wl.c122w);
   if (err.empty()) tune = t;
   return err;
 }

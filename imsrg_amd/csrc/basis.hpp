@@ -68,6 +68,7 @@ struct WorkLists {
   std::vector<int4> ob221;   // (pair, one-body pair, kind/c/J, partial-sum slot) of the comm221 one-body part
   std::vector<int4> mirror;  // (pair, ch, rb, sb), rb < sb  and diagonal tiles; ch = -1: empty item
   std::vector<int4> gen;     // (pair, ch, 0, 0) channels with work for the generator
+  std::vector<int4> c122w;   // (pair, ch, comm122 column bin, block of kC122Rows rows) of the W kernel
 };
 
 // Device copy of the tables; owns the allocations.
@@ -79,11 +80,13 @@ struct DeviceBasis {
   WorkLists wl;             // host lists with pair = 0
   Tuning tune;
   int4 *d_pphh = nullptr, *d_pphh64 = nullptr, *d_ph = nullptr, *d_pandya = nullptr, *d_pack = nullptr,
-       *d_invp = nullptr, *d_one = nullptr, *d_mirror = nullptr, *d_gen = nullptr, *d_ob221 = nullptr;
+       *d_invp = nullptr, *d_one = nullptr, *d_mirror = nullptr, *d_gen = nullptr, *d_ob221 = nullptr,
+       *d_c122w = nullptr;
   // host tables the work lists are built from
   std::vector<int> ch_n, k_start, cc_n, cc_nph, grp_start, grp_kets, gen_ok;
   std::vector<int> ch_kstart, c122_start;
   std::vector<int2> pairs1;
+  std::vector<int4> c122w_items;  // the W kernel's items (basis only, not tuned)
   std::vector<int4> ob221;
   std::string upload(const HostBasis &hb, int emax6j);  // builds tables, lists, 6j table
   std::string build_lists(const Tuning &t);             // (re)builds and uploads the work lists

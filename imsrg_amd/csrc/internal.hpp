@@ -28,6 +28,9 @@ constexpr int kGC = kTileK / kGWaves;  // k columns each wave copies into LDS pe
 constexpr int kChunk = 256;       // elements per work item of the gather kernels
 constexpr int kJC = 6;            // J values per register chunk of the inverse Pandya gather (even)
 constexpr int kC122Blk = 4;       // comm122 partner lists are padded to multiples of this (one load block)
+constexpr int kC122Cols = 32;     // kets (columns) a bin of the comm122 W kernel is filled to
+constexpr int kC122MaxCols = 150; // largest bin (LDS: 16 * 64 bytes per column)
+constexpr int kC122Rows = 64;     // rows of one W-kernel work item
 constexpr int kF32Flush = 4;      // MFMA k-steps (of 4) summed in FP32 before the FP64 flush (mixed, 64x64 tiles)
 
 // GEMM kernel families passed to launch_pphh / launch_ph_gemm: 0 FP64,
@@ -94,6 +97,21 @@ struct BasisDev {
   const IMSRG_G int *c122_ind;     // local ket index ind2 of the partner
   const IMSRG_G int2 *c122_ao;     // (a, o): coefficient X1(a, o) * f
   const IMSRG_G double *c122_f;    // phase / sqrt2 factors, times 2 for kets with p == q
+  // comm122 column bins (kets of whole one-body block pairs, closed under
+  // partners): bin b holds the channel-local kets c122_bcols[c122_boff[b] ..
+  // c122_boff[b + 1]); c122_gpos[e] is the position of entry e's partner in
+  // its column's bin
+  const IMSRG_G int *c122_bcols;
+  const IMSRG_G int *c122_boff;
+  const IMSRG_G int *c122_gpos;
+  // the bin's partner entries, column by column: global entry indices
+  // c122_bent[c122_beoff[b] ..]; column cp's entries end at c122_bcend[c122_boff[b] + cp]
+  // within the bin
+  const IMSRG_G int *c122_bent;
+  const IMSRG_G int *c122_beoff;
+  const IMSRG_G int *c122_bcend;
+  int c122_maxbin;                 // kets of the largest bin
+  int c122_maxent;                 // partner entries of the largest bin
   // cross-coupled channels
   const IMSRG_G int *cc_n, *cc_nph, *cc_kstart, *cc_kets, *cc_phstart, *cc_ph;
   const IMSRG_G signed char *cc_phi;  // (-1)^{(j_p+j_q)} per CC ket
@@ -131,6 +149,7 @@ struct PairPtrs {
   IMSRG_G double *obpart;         // per-call one-body partial sums, nobpart entries
   IMSRG_G double *diag;           // per-call inverse-Pandya diagonal terms, one per ket of every channel (ndiag)
   IMSRG_G double *panel;          // per-call bf16x3 panels of the mixed pp/hh GEMM (2 pk_size doubles)
+  IMSRG_G double *c122w;          // per-call comm122 W, two-body layout (tbsize + 8 doubles)
   int hX, hY, hZ;
   unsigned mask;
 };

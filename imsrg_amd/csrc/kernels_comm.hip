@@ -1662,47 +1662,92 @@ __device__ __forceinline__ const T &blk_at(const T *base, unsigned idx) {
   return *(const T *)((const char *)base + (idx << 3));
 }
 
-// W(row, q) of comm122ss (src/Commutator.cc:908-926) for one column q of a
-// channel: sum over the partner entries e of ket q of Y2(row, ind_e) x_e -
-// X2(row, ind_e) y_e.  q (hence e0, e1) is uniform over the wave.  Every
-// column's list holds a multiple of kC122Blk entries (padded
-// with its first partner and a zero factor), read in blocks of kC122Blk whose
-// loads are all issued before the first use.
-//
-// The gathers are bound by address processing per load instruction, so each
-// half-wave takes every second partner and loads 16 bytes per lane: lane
-// (h, p) = (lane >> 5, lane & 31) reads rows row0 + 2p, row0 + 2p + 1 of the
-// partners eb + h, eb + h + 2 of each block (a pair may read one element past
-// its column: the next column or the arena's slack).  The halves are summed,
-// so both lanes p and p + 32 return rows row0 + 2p, row0 + 2p + 1; rows past n
-// get unused values.
-__device__ __forceinline__ double2 c122_w2(const double *X2, const double *Y2, const IMSRG_G double2 *coef,
-                                           const IMSRG_G int *ind, int e0, int e1, int n, int row0) {
-  constexpr int kB = kC122Blk;
-  static_assert(kB % 2 == 0, "partner blocks split between the two half-waves");
-  const int lane = threadIdx.x & 63, h = lane >> 5;
-  const int r2 = max(min(row0 + 2 * (lane & 31), n - 1), 0);
-  double2 w = {0.0, 0.0};
-  for (int eb = e0; eb < e1; eb += kB) {
-    double2 y[kB / 2], x[kB / 2], cf[kB / 2];
+// W of comm122ss (src/Commutator.cc:908-926) by column bins:
+//   W(r, q) = sum over the partner entries e of ket q of Y2(r, ind_e) x_e - X2(r, ind_e) y_e
+// (x_e, y_e from k_c122_coef).  Every partner of a bin's kets lies in the bin,
+// so a work-group stages the bin's columns of X2 and Y2 for kC122Rows rows in
+// LDS (each column segment read once, coalesced), together with the bin's
+// partner entries (position in the bin, coefficients); each lane then forms W
+// for one row of every column of the bin, reading only LDS, and W is stored in
+// the two-body layout.
+__host__ __device__ inline size_t c122w_lds_bytes(int maxbin, int maxent) {
+  return 2 * sizeof(double) * kC122Rows * (size_t)maxbin + (sizeof(double2) + sizeof(int)) * (size_t)maxent +
+         sizeof(int) * (size_t)maxbin;
+}
+__global__ __launch_bounds__(256) void k_c122_w(PairArg pa, const int4 *items, int maxbin, int maxent) {
+  static_assert(kC122Rows == 64, "one row per lane of a wave");
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  double(*sx)[kC122Rows] = reinterpret_cast<double(*)[kC122Rows]>(lds_raw);  // [maxbin][kC122Rows]
+  double(*sy)[kC122Rows] = sx + maxbin;
+  double2 *sc = reinterpret_cast<double2 *>(sy + maxbin);  // [maxent] coefficients (x_e, y_e)
+  int *sp = reinterpret_cast<int *>(sc + maxent);          // [maxent] partner position in the bin
+  int *se = sp + maxent;                                   // [maxbin] end of each column's entries
+  const int4 it = items[blockIdx.x];
+  PairPtrs P = pa.get(it.x);
+  const BasisDev &B = *P.basis;
+  const int ch = it.y, n = B.ch_n[ch];
+  const int b0 = B.c122_boff[it.z], nc = B.c122_boff[it.z + 1] - b0;
+  const int eo = B.c122_beoff[it.z], ne = B.c122_beoff[it.z + 1] - eo;
+  const IMSRG_G int *cols = B.c122_bcols + b0;
+  const int R0 = it.w * kC122Rows;
+  const long off = B.ch_off[ch];
+  const double *X2 = P.X + B.two_off + off, *Y2 = P.Y + B.two_off + off;
+  double *W = P.c122w + off;
+  const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const unsigned r = (unsigned)min(R0 + lane, n - 1);
+  for (int t = threadIdx.x; t < ne; t += blockDim.x) {
+    const int e = B.c122_bent[eo + t];
+    sp[t] = B.c122_gpos[e];
+    sc[t] = P.c122[e];
+  }
+  for (int t = threadIdx.x; t < nc; t += blockDim.x) se[t] = B.c122_bcend[b0 + t];
+  // columns: up to 16 per wave and pass, every load issued before the LDS
+  // writes (a loop that wrote each column before loading the next waited for
+  // each load in turn)
+  for (int cb = 0; cb < nc; cb += 64) {
+    double xr[16], yr[16];
 #pragma unroll
-    for (int m = 0; m < kB / 2; ++m) {
-      const int i0 = ind[eb + 2 * m], i1 = ind[eb + 2 * m + 1];  // wave-uniform (scalar) loads
-      const double2 c0 = coef[eb + 2 * m], c1 = coef[eb + 2 * m + 1];
-      const unsigned o = (unsigned)((h ? i1 : i0) * n + r2);
-      y[m] = *reinterpret_cast<const double2 *>(&blk_at(Y2, o));
-      x[m] = *reinterpret_cast<const double2 *>(&blk_at(X2, o));
-      cf[m] = h ? c1 : c0;
+    for (int m = 0; m < 16; ++m) {
+      const int c = min(cb + wv + 4 * m, nc - 1);
+      const unsigned q = (unsigned)cols[c];
+      xr[m] = blk_at(X2, q * n + r);
+      yr[m] = blk_at(Y2, q * n + r);
     }
 #pragma unroll
-    for (int m = 0; m < kB / 2; ++m) {
-      w.x += y[m].x * cf[m].x - x[m].x * cf[m].y;
-      w.y += y[m].y * cf[m].x - x[m].y * cf[m].y;
+    for (int m = 0; m < 16; ++m) {
+      const int c = cb + wv + 4 * m;
+      if (c < nc) {
+        sx[c][lane] = xr[m];
+        sy[c][lane] = yr[m];
+      }
     }
   }
-  w.x += __shfl_xor(w.x, 32, 64);
-  w.y += __shfl_xor(w.y, 32, 64);
-  return w;
+  __syncthreads();
+  const bool row_ok = R0 + lane < n;
+  static_assert(kC122Blk == 4, "partner blocks of four entries");
+  for (int cp = wv; cp < nc; cp += 4) {
+    const int q = cols[cp];
+    const int ea = cp ? se[cp - 1] : 0, eb = se[cp];
+    double w = 0.0;
+    for (int t = ea; t < eb; t += 4) {  // a column's list is a multiple of four entries
+      int pp[4];
+      double2 cf[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        pp[m] = sp[t + m];
+        cf[m] = sc[t + m];
+      }
+      double yv[4], xv[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        yv[m] = sy[pp[m]][lane];
+        xv[m] = sx[pp[m]][lane];
+      }
+#pragma unroll
+      for (int m = 0; m < 4; ++m) w += yv[m] * cf[m].x - xv[m] * cf[m].y;
+    }
+    if (row_ok) blk_at(W, (unsigned)q * n + r) = w;
+  }
 }
 
 // 16-byte access to elements idx, idx + 1 of a channel block (8-byte aligned)
@@ -1716,9 +1761,8 @@ __device__ __forceinline__ void blk_st2(double *base, unsigned idx, double2 v) {
 // Last kernel of a commutator, over the tile pairs of every two-body block:
 //   FOLD: upper triangle += the transposed-slot contributions of k_invpandya,
 //         diagonal += its diagonal terms (kept in P.diag);
-//   C122: comm122ss (src/Commutator.cc:837-932), Z2 -= W + hZ W^T, with W formed
-//         here from full columns of X2 / Y2 (the partner coefficients of this
-//         call from k_c122_coef);
+//   C122: comm122ss (src/Commutator.cc:837-932), Z2 -= W + hZ W^T, with W
+//         formed beforehand by k_c122_w;
 // then (MIRROR) lower triangle <- hZ * upper triangle; anti-Hermitian diagonal <- 0.
 // Without MIRROR only the upper triangle is read and written, so that pass can
 // run while other kernels write the transposed slots.
@@ -1739,8 +1783,7 @@ __global__ __launch_bounds__(256, 4) void k_finish(PairArg pa, const int4 *items
   const int R0 = it.z * kTile, S0 = it.w * kTile;
   const long off = B.ch_off[ch];
   double *Z2 = P.Z + B.two_off + off;
-  const double *X2 = P.X + B.two_off + off, *Y2 = P.Y + B.two_off + off;
-  const IMSRG_G int *cst = B.c122_start + B.ch_kstart[ch];
+  const double *W2 = P.c122w + off;                    // comm122 W of this channel (k_c122_w)
   const IMSRG_G double *dg = P.diag + B.ch_kstart[ch];  // inverse-Pandya diagonal terms of this channel
   const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   // lane (h, p): rows p2, p2 + 1 (p2 = 2p) of the tile, in the columns (U) or
@@ -1748,20 +1791,6 @@ __global__ __launch_bounds__(256, 4) void k_finish(PairArg pa, const int4 *items
   const int h = lane >> 5, p2 = 2 * (lane & 31);
   const double hZ = P.hZ;
   constexpr int kPairs = kTile * kTile / 512;
-  // acc -= f * W for the wave's two columns (or rows) of pass j: one
-  // wave-uniform partner list each, the half-wave h keeps its own
-  auto sub122 = [&](double2 &acc, int cA, int row0, double f) {
-#pragma unroll
-    for (int hh = 0; hh < 2; ++hh) {
-      const int cq = cA + 4 * hh;
-      if (cq >= n) break;  // wave-uniform
-      const double2 w = c122_w2(X2, Y2, P.c122, B.c122_ind, cst[cq], cst[cq + 1], n, row0);
-      if (h == hh) {
-        acc.x -= f * w.x;
-        acc.y -= f * w.y;
-      }
-    }
-  };
   double2 u[kPairs];
 #pragma unroll
   for (int j = 0; j < kPairs; ++j) {  // upper elements (R0 + p2 (+1), S0 + cc)
@@ -1770,7 +1799,11 @@ __global__ __launch_bounds__(256, 4) void k_finish(PairArg pa, const int4 *items
     const double2 z = blk_ld2(Z2, (unsigned)(min(c, n - 1) * n + min(r, n - 1)));
     u[j].x = (r < n && c < n) ? z.x : 0.0;
     u[j].y = (r + 1 < n && c < n) ? z.y : 0.0;
-    if (C122) sub122(u[j], S0 + wv + 8 * j, R0, 1.0);
+    if (C122) {  // Z2 -= W at the upper element
+      const double2 w = blk_ld2(W2, (unsigned)(min(c, n - 1) * n + min(r, n - 1)));
+      u[j].x -= w.x;
+      u[j].y -= w.y;
+    }
   }
 #pragma unroll
   for (int j = 0; j < kPairs; ++j) {  // transposed positions (S0 + p2 (+1), R0 + rr)
@@ -1781,7 +1814,11 @@ __global__ __launch_bounds__(256, 4) void k_finish(PairArg pa, const int4 *items
       v.x = (r < n && c < n && c != r) ? z.x : 0.0;
       v.y = (r < n && c + 1 < n && c + 1 != r) ? z.y : 0.0;
     }
-    if (C122) sub122(v, R0 + wv + 8 * j, S0, hZ);
+    if (C122) {  // and -= hZ W at the transposed position
+      const double2 w = blk_ld2(W2, (unsigned)(min(r, n - 1) * n + min(c, n - 1)));
+      v.x -= hZ * w.x;
+      v.y -= hZ * w.y;
+    }
     t[rr][p2] = v.x;
     t[rr][p2 + 1] = v.y;
   }
@@ -1936,6 +1973,13 @@ hipError_t launch_finish(const PairArg &pa, const int4 *items, int nitems, bool 
     hipLaunchKernelGGL((k_finish<false, true>), dim3(nitems), dim3(256), 0, s, pa, items);
   else
     hipLaunchKernelGGL(k_mirror, dim3(nitems), dim3(256), 0, s, pa, items);
+  return hipGetLastError();
+}
+hipError_t launch_c122_w(const PairArg &pa, const int4 *items, int nitems, int maxbin, int maxent, hipStream_t s) {
+  if (nitems <= 0) return hipSuccess;
+  const size_t lds = c122w_lds_bytes(maxbin, maxent);
+  if (lds > 160 * 1024) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_c122_w, dim3(nitems), dim3(256), lds, s, pa, items, maxbin, maxent);
   return hipGetLastError();
 }
 hipError_t launch_c122_upper(const PairArg &pa, const int4 *items, int nitems, hipStream_t s) {

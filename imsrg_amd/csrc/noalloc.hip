@@ -16,10 +16,10 @@
 #include "noalloc.hpp"
 
 namespace {
-// 48 MiB: the workspaces of bases up to emax = 6 (O16 reference: 24.7 MiB of
-// Pandya matrices plus small lists, and 15.1 MiB of bf16x3 operand panels in
-// mixed precision).  Larger bases are rejected.
-constexpr long kScratchDoubles = 6L << 20;
+// 56 MiB: the workspaces of bases up to emax = 6 (O16 reference: 24.7 MiB of
+// Pandya matrices plus small lists, 7.1 MiB for comm122's W, and 15.1 MiB of
+// bf16x3 operand panels in mixed precision).  Larger bases are rejected.
+constexpr long kScratchDoubles = 7L << 20;
 __device__ __align__(256) double scratch[kScratchDoubles];
 
 constexpr int kMaxDevices = 64;

@@ -171,7 +171,7 @@ int imsrg_hip_commutator_ss_dev_async(imsrg_ctx *ctx, const imsrg_dop *X, const 
  * memory (code under HIP graph capture, tight serving loops).  The same
  * commutator as imsrg_hip_commutator_ss_dev, with every per-call workspace
  * (Pandya matrices, partial sums, comm122 coefficients) carved from one static
- * device array of 48 MiB per GPU, held from module load until the process
+ * device array of 56 MiB per GPU, held from module load until the process
  * ends; the call allocates nothing.  Bases whose workspaces exceed it (above
  * emax = 6 for the O16 reference) return IMSRG_E_INVALID.  All calls on one
  * GPU share the array, so calls that overlap on one GPU (other host threads,
