@@ -696,11 +696,11 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
   const int prec_pphh = gemm_family(prec, tu.mixed_pphh);
   const int prec_ph = gemm_family(prec, tu.mixed_ph);
   hipStream_t s = ctx->stream;
-  // With overlap, the pp/hh GEMM and then comm122 (coefficients and an
-  // upper-triangle pass) run on the side stream next to the zero-body,
-  // one-body, Pandya, ph GEMM and inverse Pandya kernels (none of them touches
-  // the upper triangle of Z's two-body blocks); the main stream waits for them
-  // before the last kernel, which then only folds and mirrors.
+  // With overlap, the pp/hh GEMM and then comm122's coefficients and W run on
+  // the side stream next to the zero-body, one-body, Pandya, ph GEMM and
+  // inverse Pandya kernels (none of them touches the upper triangle of Z's
+  // two-body blocks); the main stream waits for them before the last kernel,
+  // which folds, applies W and mirrors.
   const bool overlap = tu.overlap_pphh && (mask & MPPHH);
   // overlap_pphh 1: fork before the first kernel; 2: fork after the ph GEMM, so
   // that the pp/hh GEMM shares the GPU with the inverse Pandya gather only
@@ -724,7 +724,6 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
     if (mask & M122) {
       HIPCHK(ctx, launch_c122_coef(pa, npairs, maxn122, ctx->side));
       HIPCHK(ctx, launch_c122_w(pa, items[9], cnt[9], maxbin, maxent, ctx->side));
-      HIPCHK(ctx, launch_c122_upper(pa, items[6], cnt[6], ctx->side));
     }
     if (set) HIPCHK(ctx, hipEventRecord((*set)[11], ctx->side));
     HIPCHK(ctx, hipEventRecord(ctx->join, ctx->side));
@@ -759,7 +758,7 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
   if (mask & MPH) HIPCHK(ctx, launch_invpandya(pa, items[4], cnt[4], maxjmax, maxnorb, maxnch, maxncls, s));
   if (overlap) HIPCHK(ctx, hipStreamWaitEvent(s, ctx->join, 0));
   HIPCHK(ctx, mark(7));
-  HIPCHK(ctx, launch_finish(pa, items[6], cnt[6], (mask & MPH) != 0, (mask & M122) != 0 && !overlap, s));
+  HIPCHK(ctx, launch_finish(pa, items[6], cnt[6], (mask & MPH) != 0, (mask & M122) != 0, s));
   HIPCHK(ctx, mark(8));
   if (set) ++ctx->ring_count;
   return IMSRG_OK;

@@ -1763,16 +1763,14 @@ __device__ __forceinline__ void blk_st2(double *base, unsigned idx, double2 v) {
 //         diagonal += its diagonal terms (kept in P.diag);
 //   C122: comm122ss (src/Commutator.cc:837-932), Z2 -= W + hZ W^T, with W
 //         formed beforehand by k_c122_w;
-// then (MIRROR) lower triangle <- hZ * upper triangle; anti-Hermitian diagonal <- 0.
-// Without MIRROR only the upper triangle is read and written, so that pass can
-// run while other kernels write the transposed slots.
+// then lower triangle <- hZ * upper triangle; anti-Hermitian diagonal <- 0.
 // Tile pair: U = rows R0.., cols S0.. (R0 <= S0) and its transposed position.
 // U stays in registers, two rows per thread in 8 of the wave's 16 columns.  The
 // contributions held at the transposed position go through LDS, two rows per
 // thread of 8 of the wave's 16 columns R0 + rr of Z2.  Loads and stores of
 // Z2 then move 16 bytes per lane (address processing per instruction, not
 // bytes, bounds these accesses).
-template <bool FOLD, bool C122, bool MIRROR = true>
+template <bool FOLD, bool C122>
 __global__ __launch_bounds__(256, 4) void k_finish(PairArg pa, const int4 *items) {
   __shared__ double t[kTile][kTile + 1];  // t[rr][cc]: contribution to upper (R0 + rr, S0 + cc)
   const int4 it = items[blockIdx.x];
@@ -1837,7 +1835,6 @@ __global__ __launch_bounds__(256, 4) void k_finish(PairArg pa, const int4 *items
       st_nt(&blk_at(Z2, (unsigned)(c * n + r)), v.x);
     u[j] = v;
   }
-  if constexpr (!MIRROR) return;
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < kPairs; ++j) {  // new upper values to the threads writing their mirror
@@ -1980,11 +1977,6 @@ hipError_t launch_c122_w(const PairArg &pa, const int4 *items, int nitems, int m
   const size_t lds = c122w_lds_bytes(maxbin, maxent);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_c122_w, dim3(nitems), dim3(256), lds, s, pa, items, maxbin, maxent);
-  return hipGetLastError();
-}
-hipError_t launch_c122_upper(const PairArg &pa, const int4 *items, int nitems, hipStream_t s) {
-  if (nitems <= 0) return hipSuccess;
-  hipLaunchKernelGGL((k_finish<false, true, false>), dim3(nitems), dim3(256), 0, s, pa, items);
   return hipGetLastError();
 }
 hipError_t launch_zero_two(const PairArg &pa, const int4 *items, int nitems, hipStream_t s) {

@@ -34,11 +34,8 @@ hipError_t launch_invpandya(const PairArg &pa, const int4 *items, int nitems, in
 // transposed slots; with c122, Z2 -= W + hZ W^T (comm122ss); then lower triangle
 // <- hZ * upper
 hipError_t launch_finish(const PairArg &pa, const int4 *items, int nitems, bool fold, bool c122, hipStream_t s);
-// comm122 on the upper triangle only (Z2 -= W + hZ W^T for r <= s), without the
-// fold and the mirror; it can run while the inverse Pandya kernel writes the slots
-hipError_t launch_c122_upper(const PairArg &pa, const int4 *items, int nitems, hipStream_t s);
 // comm122's W (two-body layout, P.c122w) from the partner coefficients of this
-// call, by column bins; before launch_finish / launch_c122_upper with c122
+// call, by column bins; before launch_finish with c122
 hipError_t launch_c122_w(const PairArg &pa, const int4 *items, int nitems, int maxbin, int maxent, hipStream_t s);
 hipError_t launch_zero_two(const PairArg &pa, const int4 *items, int nitems, hipStream_t s);
 
