@@ -407,6 +407,11 @@ def main():
         work = FA if PHASES[dom] == "pphh_gemm" else BI
         ach_iso = work / t_iso / (1e12 if roof["unit"] == "TFLOP/s" else 1e9)
         roof["isolated"] = {"launch_us": phase_iso[dom], "achieved": ach_iso, "frac": ach_iso / roof["peak"]}
+        if roof.get("traffic"):
+            # the profiled HBM bytes per launch over the same launch run alone:
+            # how close the kernel's actual traffic comes to the HBM peak
+            roof["isolated"]["hbm_traffic_GBps"] = roof["traffic"] / t_iso / 1e9
+            roof["isolated"]["hbm_traffic_frac"] = roof["traffic"] / t_iso / PEAK_HBM
 
     line = {
         "metric": "[X,Y] commutators/sec at emax=10: mixed vs FP64 GPU, at |dE0|<1e-6",
