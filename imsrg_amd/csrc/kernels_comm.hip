@@ -884,12 +884,14 @@ __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
 // of 32 k and flushed into the FP64 accumulators after every stage.  bf16
 // MFMA runs at 16x the FP32 matrix rate on MI355X, so six products cost
 // 3/8 of one FP32 product.
-// Operands are split once per tile when a stage is staged: each wave loads
-// 8 consecutive k of two rows of one operand (16-byte global loads, lanes on
-// consecutive row pairs), splits them in registers and stores the three
-// bf16x8 vectors in the MFMA operand layout (lane l of a 16-row block reads
-// row l % 16, k octet l / 16: one 16-byte LDS read per fragment).  The next
-// stage's global loads are issued before the current stage's MFMAs.
+// The ph GEMM's kernel (gemm_tile_b3) splits operands once per tile when a
+// stage is staged: each wave loads 8 consecutive k of two rows of one operand
+// (16-byte global loads, lanes on consecutive row pairs), splits them in
+// registers and stores the three bf16x8 vectors in the MFMA operand layout
+// (lane l of a 16-row block reads row l % 16, k octet l / 16: one 16-byte LDS
+// read per fragment).  The next stage's global loads are issued before the
+// current stage's MFMAs.  The pp/hh GEMM splits its operands once per
+// commutator instead (k_pack_pphh, below).
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef double d2u __attribute__((ext_vector_type(2), aligned(8)));
 
@@ -926,25 +928,11 @@ __device__ __forceinline__ void gemm_tile_b3(PairArg pa, const int4 *items) {
   const BasisDev &B = *P.basis;
   int n, nk, nph = 0;
   const double *Am, *Bm;
-  const int *kl = nullptr;
-  const double *kw = nullptr;
   const signed char *phi = nullptr, *phph = nullptr;
   double alpha, h;
   double *D;
-  if constexpr (MODE == 0) {
-    const int ch = it.y;
-    n = B.ch_n[ch];
-    const long off = B.ch_off[ch];
-    Am = P.X + B.two_off + off;
-    Bm = P.Y + B.two_off + off;
-    const int k0s = B.k_start[ch];
-    nk = B.k_start[ch + 1] - k0s;
-    kl = B.k_loc + k0s;
-    kw = B.k_w + k0s;
-    alpha = (double)P.hY;
-    h = (double)P.hZ;
-    D = P.Z + B.two_off + off;
-  } else {
+  static_assert(MODE == 1 || MODE == 2, "the ph GEMM's head (1) and tail (2); the pp/hh GEMM packs its panels");
+  {
     const int cc = it.y;
     n = B.cc_n[cc];
     nph = B.cc_nph[cc];
@@ -969,13 +957,6 @@ __device__ __forceinline__ void gemm_tile_b3(PairArg pa, const int4 *items) {
   }
   __syncthreads();  // s_ph
   const int K2 = 2 * nk;
-  // column indices and weights are read by wave-uniform scalar loads through
-  // the constant address space (read-only here): the eight loads of a stage
-  // issue back to back and are waited for once, with no LDS round trip
-  typedef const __attribute__((address_space(4))) int cint;
-  typedef const __attribute__((address_space(4))) double cdbl;
-  cint *klc = (cint *)kl;
-  cdbl *kwc = (cdbl *)kw;
   // loader role of this wave: operand (0: rows R0.. of A-side values, 1: rows
   // S0.. of B-side values), k octet, and the lane's two rows
   const int opnd = wv >> 2, oct = wv & 3;
@@ -1000,13 +981,8 @@ __device__ __forceinline__ void gemm_tile_b3(PairArg pa, const int4 *items) {
     }
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      if constexpr (MODE == 0) {
-        ca[j] = klc[k[j]];
-        wr[j] = kwc[k[j]];
-      } else {
-        ca[j] = k[j];
-        wr[j] = (MODE == 1) ? 1.0 : (double)P.hY * (double)phph[k[j] < nph ? k[j] : k[j] - nph];
-      }
+      ca[j] = k[j];
+      wr[j] = (MODE == 1) ? 1.0 : (double)P.hY * (double)phph[k[j] < nph ? k[j] : k[j] - nph];
     }
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
