@@ -553,6 +553,16 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
     B.ncls = ncls;
     B.cls_start = dupload(*this, cls_start, err);
     B.cls_e = dupload(*this, cls_e, err);
+    // per class entry: the ket and its orbits' quantum numbers in one record
+    std::vector<int4> rec(cls_e.size());
+    for (int g = 0; g < 6; ++g)
+      for (int t = cls_start[g * ncls]; t < cls_start[(g + 1) * ncls]; ++t) {
+        const int kk = grp_kets[grp_start[g] + cls_e[t]];
+        const int k = hb.kp[kk], l = hb.kq[kk];
+        rec[t] = make_int4(kk, (k << 16) | l, (hb.j2[k] << 16) | hb.j2[l],
+                           (hb.l[k] << 24) | (hb.l[l] << 16) | ((hb.tz2[k] + 1) << 8) | (hb.tz2[l] + 1));
+      }
+    B.cls_rec = dupload(*this, rec, err);
   }
 
   // ---- work lists ----

@@ -121,6 +121,9 @@ struct BasisDev {
   // kets of each (parity, Tz) group g by (j_k, j_l) class c = ((j2k-1)/2) d1 + (j2l-1)/2: the group
   // indices e, ascending, in cls_e[cls_start[g ncls + c] .. cls_start[g ncls + c + 1]), ncls = d1 * d1
   const IMSRG_G int *cls_start, *cls_e;
+  // per cls_e entry, the ket and its orbits in one record: (ket, k << 16 | l,
+  // j2k << 16 | j2l, l_k << 24 | l_l << 16 | (tz2k + 1) << 8 | (tz2l + 1))
+  const IMSRG_G int4 *cls_rec;
   int ncls;
   const IMSRG_G int2 *pairs1;   // one-body pairs (i<=j, same (l,j2,tz2) block)
   // partial sums of the one-body part: slots ob_start[p] .. ob_start[p+1]-1 of pair p

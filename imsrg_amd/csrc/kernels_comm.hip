@@ -1470,13 +1470,13 @@ __global__ __launch_bounds__(256, 5) void k_invpandya(PairArg pa, const int4 *it
   __syncthreads();
   const int total = s_pre[ncls];
   const int jji = B.j2[i], jjj = B.j2[j];
+  const int li = B.l[i], tzi = B.tz2[i];
   const int d2q = B.d2q;
   const double hf = (P.hZ == 1) ? 1.0 : -1.0;
   const double *Zb = P.Zbar;
   double *Ztwo = P.Z + B.two_off;
   const IMSRG_G double *sixq = B.sixjQ;  // held in registers: the stores below could alias B
   const IMSRG_G int *kmapT = B.kmapT;
-  const IMSRG_G int *cls_e = B.cls_e;
   for (int q0 = 0; q0 < total; q0 += blockDim.x) {
     const int q = q0 + threadIdx.x;
     const bool active = q < total;
@@ -1485,10 +1485,12 @@ __global__ __launch_bounds__(256, 5) void k_invpandya(PairArg pa, const int4 *it
       const int mid = (clo + chi) >> 1;
       if (s_pre[mid] <= q) clo = mid; else chi = mid;
     }
-    const int e = active ? cls_e[s_us[clo] + q - s_pre[clo]] : b;
-    const int kk = B.grp_kets[gs + e];
-    const int k = B.kp[kk], l = B.kq[kk];  // i <= k <= l
-    const int jjk = B.j2[k], jjl = B.j2[l];
+    const int4 rc = B.cls_rec[active ? s_us[clo] + q - s_pre[clo] : 0];
+    const int kk = active ? rc.x : kb;
+    const int k = active ? rc.y >> 16 : i, l = active ? rc.y & 0xFFFF : j;  // i <= k <= l
+    const int jjk = active ? rc.z >> 16 : jji, jjl = active ? rc.z & 0xFFFF : jjj;
+    const int lk = active ? rc.w >> 24 : B.l[i], ll = active ? (rc.w >> 16) & 0xFF : B.l[j];
+    const int tzk = active ? ((rc.w >> 8) & 0xFF) - 1 : B.tz2[i], tzl = active ? (rc.w & 0xFF) - 1 : B.tz2[j];
     const bool pauli = (i == j) || (k == l);  // kets (p,p) exist for even J only
     int Jlo = max(abs(jji - jjj), abs(jjk - jjl)) / 2, Jhi = min(jji + jjj, jjk + jjl) / 2;
     if (pauli) {
@@ -1517,8 +1519,8 @@ __global__ __launch_bounds__(256, 5) void k_invpandya(PairArg pa, const int4 *it
       Xhi = -1;
     }
     const int xlo = wave_min(Xlo), xhi = wave_max(Xhi);
-    const int pd = (B.l[i] + B.l[l]) & 1, td = abs(B.tz2[i] - B.tz2[l]) / 2;
-    const int px = (B.l[i] + B.l[k]) & 1, tx = abs(B.tz2[i] - B.tz2[k]) / 2;
+    const int pd = (li + ll) & 1, td = abs(tzi - tzl) / 2;
+    const int px = (li + lk) & 1, tx = abs(tzi - tzk) / 2;
     // 6j entries of J' and the J pair (c0, c0+1) sit at s6 + J' * s6J + (c0/2) * s6P; the next
     // pair (c0+2, c0+3) one pair stride further
     const long s6P = 2L * B.d1 * B.d1, s6J = s6P * (d2q >> 1);
