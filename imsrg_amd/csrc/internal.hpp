@@ -133,7 +133,8 @@ struct BasisDev {
   int ndiag;  // kets summed over all channels (ch_kstart[nch])
   const IMSRG_G double *w1norm; // norb*norb one-body norm weights
   const IMSRG_G double *sixj;   // dense 6j table, layout of SixJCache_112112
-  const IMSRG_G double *sixjQ;  // same values, layout [jj1][jj2][J6][J3/2][jj4][jj5][J3%2] (sixjQ_index):
+  const IMSRG_G double *sixjQ;  // (2 J6 + 1) times the values (0 where |6j| < 1e-8), layout
+                                // [jj1][jj2][J6][J3/2][jj4][jj5][J3%2] (sixjQ_index):
                                 // lanes of one (jj1, jj2, J6, J3 pair) that differ in jj4, jj5 read
                                 // neighbouring 16-byte entries; d2q = d2 rounded up to even
   int d2q;

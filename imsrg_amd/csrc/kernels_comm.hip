@@ -1388,7 +1388,10 @@ __device__ __forceinline__ int wave_max(int v) {
 //   commij(J) = -sum_J' (2J'+1) {ji jj J; jk jl J'} Zbar^{J'}[il, kj]
 //   commji(J) = -sum_J' (2J'+1) {jj ji J; jk jl J'} Zbar^{J'}[jl, ki]
 //   Z^J_{ij,kl} += -(commij - (-1)^{(jk+jl)/2 - J} commji) / norm
-// with the reference's |6j| < 1e-8 skip.  The cross-coupled local indices of
+// with the reference's |6j| < 1e-8 skip, which the 6j copy sixjQ holds as zeros.
+// Every J of a chunk is accumulated unmasked: a J outside the lane's triangles
+// has a zero 6j, and the accumulators of J values the lane does not own (odd J
+// of a (p,p) ket, J past the range) are never stored.  The cross-coupled local indices of
 // the kets (i,x) and (j,x), the CC channel sizes/offsets and the bra's
 // standard-channel data are staged in LDS once per work-group.  Every upper
 // element (ij <= kl, or ij < kl for anti-Hermitian Z) of every channel is
@@ -1545,23 +1548,24 @@ __global__ __launch_bounds__(256, 5) void k_invpandya(PairArg pa, const int4 *it
       double acc[kJC];
 #pragma unroll
       for (int t = 0; t < kJC; ++t) acc[t] = 0.0;
-      const double sx0 = -(double)sgn_parity((jjk + jjl) / 2 - c0);  // -(-1)^{(jk+jl)/2 - J} at J = c0
+      // symx times -(-1)^{(jk+jl)/2 - J} at J = c0
+      const double fx = symx * -(double)sgn_parity((jjk + jjl) / 2 - c0);
       for (int Jp = dlo; Jp <= dhi; ++Jp) {
         if (Jp < Dlo || Jp > Dhi) continue;
         const int cc = 6 * Jp + 2 * (td + 1) + pd;
         const int nkc = s_ccn[cc];
         const int il = Ti[Jp * norb + l];
         const int kj = Tj[Jp * norb + k] + (k > j ? nkc : 0);
-        const double z = (2 * Jp + 1) * Zb[s_zoff[cc] + (long)kj * nkc + il];
+        const double z = Zb[s_zoff[cc] + (long)kj * nkc + il];
         const double *s6 = sixq + s6d + (long)Jp * s6J + (long)(c0 >> 1) * s6P;
-        // 0/1 factors rather than selects keep the 16-byte 6j loads unconditional
         double2 w[kJC / 2];
 #pragma unroll
         for (int hh = 0; hh < kJC / 2; ++hh) w[hh] = *(const double2 *)(s6 + hh * s6P);
+        __builtin_amdgcn_sched_barrier(0);  // all four loads in flight before the first use
 #pragma unroll
         for (int hh = 0; hh < kJC / 2; ++hh) {
-          acc[2 * hh] += (((vm >> (2 * hh)) & 1) && fabs(w[hh].x) >= 1e-8 ? 1.0 : 0.0) * (w[hh].x * z);
-          acc[2 * hh + 1] += (((vm >> (2 * hh + 1)) & 1) && fabs(w[hh].y) >= 1e-8 ? 1.0 : 0.0) * (w[hh].y * z);
+          acc[2 * hh] += w[hh].x * z;
+          acc[2 * hh + 1] += w[hh].y * z;
         }
       }
       for (int Jp = xlo; Jp <= xhi; ++Jp) {
@@ -1571,15 +1575,16 @@ __global__ __launch_bounds__(256, 5) void k_invpandya(PairArg pa, const int4 *it
         const int ik = Ti[Jp * norb + k];
         const int lj = Tj[Jp * norb + l];
         const double me = Zb[s_zoff[cc] + (long)((l > j ? nkc : 0) + ik) * nkc + lj];
-        const double z = (2 * Jp + 1) * (symx * me) * sx0;  // the sign alternates with J below
         const double *s6 = sixq + s6x + (long)Jp * s6J + (long)(c0 >> 1) * s6P;
         double2 w[kJC / 2];
 #pragma unroll
         for (int hh = 0; hh < kJC / 2; ++hh) w[hh] = *(const double2 *)(s6 + hh * s6P);
+        __builtin_amdgcn_sched_barrier(0);
+        const double z = fx * me;  // the sign alternates with J below
 #pragma unroll
         for (int hh = 0; hh < kJC / 2; ++hh) {
-          acc[2 * hh] += (((vm >> (2 * hh)) & 1) && fabs(w[hh].x) >= 1e-8 ? 1.0 : 0.0) * (w[hh].x * z);
-          acc[2 * hh + 1] -= (((vm >> (2 * hh + 1)) & 1) && fabs(w[hh].y) >= 1e-8 ? 1.0 : 0.0) * (w[hh].y * z);
+          acc[2 * hh] += w[hh].x * z;
+          acc[2 * hh + 1] -= w[hh].y * z;
         }
       }
 #pragma unroll

@@ -120,7 +120,11 @@ __global__ __launch_bounds__(256) void k_sixj_table(const dd *F, double *table, 
     const int i1 = (int)r;
     const double v = sixj_dd(F, 2 * i1 + 1, 2 * i2 + 1, 2 * J3, 2 * i4 + 1, 2 * i5 + 1, 2 * J6);
     table[idx] = v;
-    tableQ[sixjQ_index(d1, d2q, 2 * i1 + 1, 2 * i2 + 1, J3, 2 * i4 + 1, 2 * i5 + 1, J6)] = v;
+    // the inverse Pandya copy holds the weights (2 J6 + 1) {...}, and 0 where
+    // |6j| < 1e-8, the values whose terms the reference skips (src/Commutator.cc:1564-1572,
+    // 1593-1604)
+    tableQ[sixjQ_index(d1, d2q, 2 * i1 + 1, 2 * i2 + 1, J3, 2 * i4 + 1, 2 * i5 + 1, J6)] =
+        fabs(v) >= 1e-8 ? (2 * J6 + 1) * v : 0.0;
   }
 }
 

@@ -7,7 +7,8 @@
 namespace imsrg {
 
 // dense 6j table {jj1 jj2 J3; jj4 jj5 J6} (double-double Racah): `table` in the layout of
-// SixJCache_112112, `tableQ` in the sixjQ_index layout (padding entries are left untouched)
+// SixJCache_112112, `tableQ` in the sixjQ_index layout as the weights (2 J6 + 1) {6j},
+// 0 in place of |6j| < 1e-8 (padding entries are left untouched)
 hipError_t launch_sixj_table(double *table, double *tableQ, int d1, int d2, int d2q, hipStream_t s);
 
 // commutator phases; items are device arrays of int4 work items
