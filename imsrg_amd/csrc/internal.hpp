@@ -26,7 +26,7 @@ constexpr int kGW = 32;           // columns of a wave's sub-tile in the 128x128
 constexpr int kGN = kGW / 16;     // 16-column MFMA blocks per wave sub-tile
 constexpr int kGC = kTileK / kGWaves;  // k columns each wave copies into LDS per stage
 constexpr int kChunk = 256;       // elements per work item of the gather kernels
-constexpr int kJC = 6;            // J values per register chunk of the inverse Pandya gather (even)
+constexpr int kJC = 8;            // J values per register chunk of the inverse Pandya gather (even)
 constexpr int kC122Blk = 4;       // comm122 partner lists are padded to multiples of this (one load block)
 constexpr int kC122Cols = 32;     // kets (columns) a bin of the comm122 W kernel is filled to
 constexpr int kC122MaxCols = 150; // largest bin (LDS: 16 * 64 bytes per column)

@@ -1550,43 +1550,74 @@ __global__ __launch_bounds__(256, 5) void k_invpandya(PairArg pa, const int4 *it
       for (int t = 0; t < kJC; ++t) acc[t] = 0.0;
       // symx times -(-1)^{(jk+jl)/2 - J} at J = c0
       const double fx = symx * -(double)sgn_parity((jjk + jjl) / 2 - c0);
-      for (int Jp = dlo; Jp <= dhi; ++Jp) {
-        if (Jp < Dlo || Jp > Dhi) continue;
-        const int cc = 6 * Jp + 2 * (td + 1) + pd;
-        const int nkc = s_ccn[cc];
-        const int il = Ti[Jp * norb + l];
-        const int kj = Tj[Jp * norb + k] + (k > j ? nkc : 0);
-        const double z = Zb[s_zoff[cc] + (long)kj * nkc + il];
-        const double *s6 = sixq + s6d + (long)Jp * s6J + (long)(c0 >> 1) * s6P;
-        double2 w[kJC / 2];
+      // two J' per step, the loads of both (Zbar element, three 16-byte 6j pairs each)
+      // issued before the first use; a J' outside the lane's range loads from a
+      // valid address and adds nothing
+      for (int Jp0 = dlo; Jp0 <= dhi; Jp0 += 2) {
+        if (Jp0 + 1 < Dlo || Jp0 > Dhi) continue;
+        bool v[2];
+        double z[2];
+        double2 w[2][kJC / 2];
 #pragma unroll
-        for (int hh = 0; hh < kJC / 2; ++hh) w[hh] = *(const double2 *)(s6 + hh * s6P);
-        __builtin_amdgcn_sched_barrier(0);  // all four loads in flight before the first use
+        for (int u = 0; u < 2; ++u) {
+          const int Jp = min(Jp0 + u, dhi);
+          v[u] = Jp0 + u <= dhi && Jp >= Dlo && Jp <= Dhi;
+          const int cc = 6 * Jp + 2 * (td + 1) + pd;
+          const int nkc = s_ccn[cc];
+          const int il = Ti[Jp * norb + l];
+          const int kj = Tj[Jp * norb + k] + (k > j ? nkc : 0);
+          z[u] = Zb[v[u] ? s_zoff[cc] + (long)kj * nkc + il : 0];
+          const double *s6 = sixq + s6d + (long)Jp * s6J + (long)(c0 >> 1) * s6P;
 #pragma unroll
-        for (int hh = 0; hh < kJC / 2; ++hh) {
-          acc[2 * hh] += w[hh].x * z;
-          acc[2 * hh + 1] += w[hh].y * z;
+          for (int hh = 0; hh < kJC / 2; ++hh) w[u][hh] = *(const double2 *)(s6 + hh * s6P);
         }
-      }
-      for (int Jp = xlo; Jp <= xhi; ++Jp) {
-        if (Jp < Xlo || Jp > Xhi) continue;
-        const int cc = 6 * Jp + 2 * (tx + 1) + px;
-        const int nkc = s_ccn[cc];
-        const int ik = Ti[Jp * norb + k];
-        const int lj = Tj[Jp * norb + l];
-        const double me = Zb[s_zoff[cc] + (long)((l > j ? nkc : 0) + ik) * nkc + lj];
-        const double *s6 = sixq + s6x + (long)Jp * s6J + (long)(c0 >> 1) * s6P;
-        double2 w[kJC / 2];
-#pragma unroll
-        for (int hh = 0; hh < kJC / 2; ++hh) w[hh] = *(const double2 *)(s6 + hh * s6P);
         __builtin_amdgcn_sched_barrier(0);
-        const double z = fx * me;  // the sign alternates with J below
 #pragma unroll
-        for (int hh = 0; hh < kJC / 2; ++hh) {
-          acc[2 * hh] += w[hh].x * z;
-          acc[2 * hh + 1] -= w[hh].y * z;
+        for (int u = 0; u < 2; ++u) {
+          const double zz = v[u] ? z[u] : 0.0;
+#pragma unroll
+          for (int hh = 0; hh < kJC / 2; ++hh) {
+            acc[2 * hh] += w[u][hh].x * zz;
+            acc[2 * hh + 1] += w[u][hh].y * zz;
+          }
         }
       }
+      for (int Jp0 = xlo; Jp0 <= xhi; Jp0 += 2) {
+        if (Jp0 + 1 < Xlo || Jp0 > Xhi) continue;
+        bool v[2];
+        double z[2];
+        double2 w[2][kJC / 2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int Jp = min(Jp0 + u, xhi);
+          v[u] = Jp0 + u <= xhi && Jp >= Xlo && Jp <= Xhi;
+          const int cc = 6 * Jp + 2 * (tx + 1) + px;
+          const int nkc = s_ccn[cc];
+          const int ik = Ti[Jp * norb + k];
+          const int lj = Tj[Jp * norb + l];
+          z[u] = Zb[v[u] ? s_zoff[cc] + (long)((l > j ? nkc : 0) + ik) * nkc + lj : 0];
+          const double *s6 = sixq + s6x + (long)Jp * s6J + (long)(c0 >> 1) * s6P;
+#pragma unroll
+          for (int hh = 0; hh < kJC / 2; ++hh) w[u][hh] = *(const double2 *)(s6 + hh * s6P);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const double zz = v[u] ? fx * z[u] : 0.0;
+#pragma unroll
+          for (int hh = 0; hh < kJC / 2; ++hh) {
+            acc[2 * hh] += w[u][hh].x * zz;
+            acc[2 * hh + 1] -= w[u][hh].y * zz;
+          }
+        }
+      }
+      // the ket's local indices in the chunk's channels, all loaded and waited for
+      // before the first store: loads and stores share one counter, so a load
+      // still in flight behind a store would cost a wait for that store
+      int ikv[kJC];
+#pragma unroll
+      for (int t = 0; t < kJC; ++t) ikv[t] = ((vm >> t) & 1) ? kmapT[(long)(c0 + t) * nk + kk] : -1;
+      __builtin_amdgcn_s_waitcnt(0);
 #pragma unroll
       for (int t = 0; t < kJC; ++t) {
         if (!((vm >> t) & 1)) continue;
@@ -1596,7 +1627,7 @@ __global__ __launch_bounds__(256, 5) void k_invpandya(PairArg pa, const int4 *it
         const double f = (k == l) ? 1.0 - s : (i == j) ? 1.0 - s * pij : 1.0;
         const double z = acc[t] * f / nrm;
         const int n = s_chn[J];
-        const int ib = s_ib[J], ik = kmapT[(long)J * nk + kk];
+        const int ib = s_ib[J], ik = ikv[t];
         if (ik == ib)  // diagonal: kept apart (the pp/hh GEMM may still be writing Z's diagonal)
           P.diag[s_chks[J] + max(ib, 0)] = z;
         else
