@@ -755,7 +755,8 @@ static int run_commutator(imsrg_ctx *ctx, const PairArg &pa, int npairs, const i
   if (fork_late)
     if (int rc = fork_side()) return rc;
   HIPCHK(ctx, mark(6));
-  if (mask & MPH) HIPCHK(ctx, launch_invpandya(pa, items[4], cnt[4], maxjmax, maxnorb, maxnch, maxncls, s));
+  if (mask & MPH)
+    HIPCHK(ctx, launch_invpandya(pa, items[4], cnt[4], maxjmax, maxnorb, maxnch, maxncls, prec_ph == 1, s));
   if (overlap) HIPCHK(ctx, hipStreamWaitEvent(s, ctx->join, 0));
   HIPCHK(ctx, mark(7));
   HIPCHK(ctx, launch_finish(pa, items[6], cnt[6], (mask & MPH) != 0, (mask & M122) != 0, s));

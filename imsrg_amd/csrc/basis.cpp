@@ -462,6 +462,9 @@ std::string DeviceBasis::upload(const HostBasis &hb, int emax6j) {
   cc_xoff[nch] = xo;
   cc_zoff[nch] = zo;
   B.maxcc = maxcc;
+  B.ccjmax_ph = -1;
+  for (int c = 0; c < nch; ++c)
+    if (cc_nph[c] > 0) B.ccjmax_ph = std::max(B.ccjmax_ph, c / 6);
   B.xsize = xo;
   B.zsize = zo;
   B.cc_n = dupload(*this, cc_n, err); B.cc_nph = dupload(*this, cc_nph, err);

@@ -58,6 +58,7 @@ struct BasisDev {
   long two_off;  // offset of the two-body arena inside an operator buffer
   long opsize;   // doubles per operator buffer (including 8 doubles of zero slack at the end)
   int maxn, maxcc, maxblk;
+  int ccjmax_ph;  // largest J of a CC channel with ph kets (-1: none); Zbar is zero above it
   long n122;           // comm122 partner entries (including the padding to multiples of kC122Blk per ket)
   const IMSRG_G int *j2, *l, *tz2, *cvq;
   const IMSRG_G double *occ;

@@ -300,6 +300,9 @@ template <int MODE, bool MIXED>
 __global__ __launch_bounds__(256) void k_syr2k(PairArg pa, const int4 *items, int accumulate) {
   constexpr int LD = kTile + 2;
   using ST = typename std::conditional<MIXED, float, double>::type;
+  // the FP32-product ph GEMM keeps Zbar in FP32 (its inverse Pandya gather reads
+  // 4-byte elements); the FP64 kernels keep it in FP64
+  using ZT = typename std::conditional<MIXED && MODE != 0, float, double>::type;
   __shared__ ST As[2][kTileK][LD];
   __shared__ ST Bs[2][kTileK][LD];
   const int4 it = items[blockIdx.x];
@@ -312,7 +315,8 @@ __global__ __launch_bounds__(256) void k_syr2k(PairArg pa, const int4 *items, in
   const int *phl = nullptr;
   const signed char *phi = nullptr;
   double alpha, h;
-  double *D;
+  double *D = nullptr;
+  ZT *Dz = nullptr;
   if (MODE == 0) {
     const int ch = it.y;
     n = B.ch_n[ch];
@@ -334,7 +338,7 @@ __global__ __launch_bounds__(256) void k_syr2k(PairArg pa, const int4 *items, in
     const long xo = B.cc_xoff[cc];
     Am = P.Xt + xo;
     Bm = P.Yt + xo;
-    D = P.Zbar + B.cc_zoff[cc] + (MODE == 2 ? (long)n * n : 0);
+    Dz = reinterpret_cast<ZT *>(P.Zbar) + B.cc_zoff[cc] + (MODE == 2 ? (long)n * n : 0);
     alpha = 1.0;
     h = (MODE == 1) ? (P.hZ == 1 ? 1.0 : -1.0) : 1.0;
     phi = B.cc_phi + B.cc_kstart[cc];
@@ -494,12 +498,12 @@ __global__ __launch_bounds__(256) void k_syr2k(PairArg pa, const int4 *items, in
     for (int e = threadIdx.x; e < kTile * kTile; e += blockDim.x) {
       const int rr = e & (kTile - 1), cc = e / kTile;
       const int row = R0 + rr, col = S0 + cc;
-      if (row < n && col < n && row <= col) D[(long)col * n + row] = Tt[rr][cc];
+      if (row < n && col < n && row <= col) Dz[(long)col * n + row] = (ZT)Tt[rr][cc];
       // mirrored element (S0+rr, R0+cc) from the upper element (R0+cc, S0+rr)
       const int row2 = S0 + rr, col2 = R0 + cc;
       if (row2 < n && col2 < n && row2 > col2) {
         const double sym = (MODE == 1) ? h : (double)(phi[row2] * phi[col2]);
-        D[(long)col2 * n + row2] = sym * Tt[cc][rr];
+        Dz[(long)col2 * n + row2] = (ZT)(sym * Tt[cc][rr]);
       }
     }
   }
@@ -540,6 +544,13 @@ __device__ __forceinline__ void st_nt2(double *p, double a, double b) {
   const d2v v = {a, b};
   __builtin_nontemporal_store(v, reinterpret_cast<d2v *>(p));
 }
+// FP32 Zbar (the FP32-product ph GEMM): the values rounded once, when stored
+__device__ __forceinline__ void st_nt(float *p, double v) { __builtin_nontemporal_store((float)v, p); }
+__device__ __forceinline__ void st_nt2(float *p, double a, double b) {
+  typedef float f2s __attribute__((ext_vector_type(2)));
+  const f2s v = {(float)a, (float)b};
+  __builtin_nontemporal_store(v, reinterpret_cast<f2s *>(p));
+}
 
 // A wave of a 128x128 tile owns the 64 x kGW sub-tile at rows (w & 1) * 64,
 // columns (w >> 1) * kGW; it has work when that sub-tile holds an upper-
@@ -556,8 +567,8 @@ __device__ __forceinline__ bool tile_wave_live(int R0, int S0, int n, int w) {
 // lower triangle as (head) h or (tail) phi_r phi_s times the upper element, as
 // the reference's folds make it (src/Commutator.cc:1792-1805); s_ph holds the
 // tile's row (0) and column (1) phases.
-template <int MODE, bool F32L>
-__device__ __forceinline__ void gemm_epilogue(const dbl4 (&acc)[4][kGN], double *D, int n, int R0, int S0,
+template <int MODE, bool F32L, typename TD = double>
+__device__ __forceinline__ void gemm_epilogue(const dbl4 (&acc)[4][kGN], TD *D, int n, int R0, int S0,
                                               double alpha, double h, const signed char (*s_ph)[kTileB],
                                               double *tb0, double *tb1, double *tb2, double *tb3) {
   const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -611,7 +622,7 @@ __device__ __forceinline__ void gemm_epilogue(const dbl4 (&acc)[4][kGN], double 
           const int rr = 2 * (e & 31), cc = e >> 5;
           const int row = R1 + rr, col = S1 + cc;
           if (col >= n || row >= n || row > col) continue;
-          double *dp = D + (long)col * n + row;
+          TD *dp = D + (long)col * n + row;
           if (row + 1 < n && row + 1 <= col)
             st_nt2(dp, Tt[rr][cc], Tt[rr + 1][cc]);
           else
@@ -626,7 +637,7 @@ __device__ __forceinline__ void gemm_epilogue(const dbl4 (&acc)[4][kGN], double 
           const bool lo = row2 > col2, hi = row2 + 1 < n && row2 + 1 > col2;
           // phases from LDS: row2 - S0 and col2 - R0 lie in [0, 128)
           const double s0 = (MODE == 1) ? h : (double)(s_ph[1][row2 - S0] * s_ph[0][col2 - R0]);
-          double *dp = D + (long)col2 * n + row2;
+          TD *dp = D + (long)col2 * n + row2;
           if (lo && hi) {
             const double s1 = (MODE == 1) ? h : (double)(s_ph[1][row2 + 1 - S0] * s_ph[0][col2 - R0]);
             st_nt2(dp, s0 * Tt[rr][cc], s1 * Tt[rr][cc + 1]);
@@ -666,7 +677,9 @@ __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
   const double *kw = nullptr;
   const signed char *phi = nullptr, *phph = nullptr;
   double alpha, h;
-  double *D;
+  // Zbar in FP32 from the FP32-product ph GEMM, in FP64 otherwise
+  using ZT = typename std::conditional<MODE != 0 && MIXED && !SPLIT, float, double>::type;
+  ZT *D;
   if constexpr (MODE == 0) {
     const int ch = it.y;
     n = B.ch_n[ch];
@@ -688,7 +701,7 @@ __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
     const long xo = B.cc_xoff[cc];
     Am = P.Xt + xo;
     Bm = P.Yt + xo;
-    D = P.Zbar + B.cc_zoff[cc] + (MODE == 2 ? (long)n * n : 0);
+    D = reinterpret_cast<ZT *>(P.Zbar) + B.cc_zoff[cc] + (MODE == 2 ? (long)n * n : 0);
     alpha = 1.0;
     h = (MODE == 1) ? (P.hZ == 1 ? 1.0 : -1.0) : 1.0;
     phi = B.cc_phi + B.cc_kstart[cc];
@@ -869,7 +882,7 @@ __device__ __forceinline__ void gemm_tile(PairArg pa, const int4 *items) {
   }
   if constexpr (MIXED) flush_t();  // a last stage without its pair
   static_assert(64 * (kGW + 1) <= kTileK * LD, "transpose tile exceeds the operand buffer");
-  gemm_epilogue<MODE, MIXED>(acc, D, n, R0, S0, alpha, h, s_ph, &A0[0][0], &A1[0][0], &B0[0][0], &B1[0][0]);
+  gemm_epilogue<MODE, MIXED, ZT>(acc, D, n, R0, S0, alpha, h, s_ph, &A0[0][0], &A1[0][0], &B0[0][0], &B1[0][0]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1361,11 +1374,6 @@ __global__ __launch_bounds__(256) void k_pandya(PairArg pa, const int4 *items) {
   P.Yt[o] = ybar * normf;
 }
 
-// Zbar element (row, col) of CC channel cc (full storage, head | tail).
-__device__ __forceinline__ double zbar_at(const PairPtrs &P, const BasisDev &B, int cc, int row, int col) {
-  const int nk = B.cc_n[cc];
-  return P.Zbar[B.cc_zoff[cc] + (long)col * nk + row];
-}
 
 // wave-wide min / max of an int, returned as a wave-uniform (scalar) value
 __device__ __forceinline__ int wave_min(int v) {
@@ -1403,6 +1411,7 @@ __host__ __device__ inline size_t invp_lds_bytes(int jmax, int norb, int nch, in
   return 8 * (size_t)nch + 8 * nJ + 4 * (size_t)nch + 4 * nJ * (2 * (size_t)norb + 3) + 4 * (2 * (size_t)ncls + 1);
 }
 
+template <typename TZ>
 __global__ __launch_bounds__(256, 5) void k_invpandya(PairArg pa, const int4 *items) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const int4 it = items[blockIdx.x];
@@ -1425,8 +1434,10 @@ __global__ __launch_bounds__(256, 5) void k_invpandya(PairArg pa, const int4 *it
   const int kb = B.grp_kets[gs + b];
   const int emin = (P.hZ == 1) ? b : b + 1;
   const int i = B.kp[kb], j = B.kq[kb];
+  // a CC channel without ph kets has Zbar = 0, which the ph GEMM does not write:
+  // offset -1 marks it, and its terms are skipped
   for (int c = threadIdx.x; c < nch; c += blockDim.x) {
-    s_zoff[c] = B.cc_zoff[c];
+    s_zoff[c] = B.cc_nph[c] > 0 ? B.cc_zoff[c] : -1;
     s_ccn[c] = B.cc_n[c];
   }
   for (int J = threadIdx.x; J < nJ; J += blockDim.x) {
@@ -1476,7 +1487,7 @@ __global__ __launch_bounds__(256, 5) void k_invpandya(PairArg pa, const int4 *it
   const int li = B.l[i], tzi = B.tz2[i];
   const int d2q = B.d2q;
   const double hf = (P.hZ == 1) ? 1.0 : -1.0;
-  const double *Zb = P.Zbar;
+  const TZ *Zb = reinterpret_cast<const TZ *>(P.Zbar);  // FP32 after the FP32-product ph GEMM
   double *Ztwo = P.Z + B.two_off;
   const IMSRG_G double *sixq = B.sixjQ;  // held in registers: the stores below could alias B
   const IMSRG_G int *kmapT = B.kmapT;
@@ -1508,7 +1519,8 @@ __global__ __launch_bounds__(256, 5) void k_invpandya(PairArg pa, const int4 *it
     const int wlo = wave_min(Jlo), whi = wave_max(Jhi);
     if (wlo > whi) continue;  // wave-uniform: no lane of this wave has a channel
     // direct term: CC channels of (i,l) for J' in tri(ji,jl) ∩ tri(jk,jj)
-    int Dlo = max(abs(jji - jjl), abs(jjk - jjj)) / 2, Dhi = min(jji + jjl, jjk + jjj) / 2;
+    const int jpz = B.ccjmax_ph;  // Zbar is zero for J' above it
+    int Dlo = max(abs(jji - jjl), abs(jjk - jjj)) / 2, Dhi = min(min(jji + jjl, jjk + jjj) / 2, jpz);
     if (!live) {
       Dlo = 1 << 20;
       Dhi = -1;
@@ -1516,7 +1528,7 @@ __global__ __launch_bounds__(256, 5) void k_invpandya(PairArg pa, const int4 *it
     const int dlo = wave_min(Dlo), dhi = wave_max(Dhi);
     // exchange term: CC channels of (i,k) for J' in tri(jj,jl) ∩ tri(jk,ji)
     const bool exch = live && k != l && i != j;
-    int Xlo = max(abs(jjj - jjl), abs(jjk - jji)) / 2, Xhi = min(jjj + jjl, jjk + jji) / 2;
+    int Xlo = max(abs(jjj - jjl), abs(jjk - jji)) / 2, Xhi = min(min(jjj + jjl, jjk + jji) / 2, jpz);
     if (!exch) {
       Xlo = 1 << 20;
       Xhi = -1;
@@ -1561,12 +1573,13 @@ __global__ __launch_bounds__(256, 5) void k_invpandya(PairArg pa, const int4 *it
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
           const int Jp = min(Jp0 + u, dhi);
-          v[u] = Jp0 + u <= dhi && Jp >= Dlo && Jp <= Dhi;
           const int cc = 6 * Jp + 2 * (td + 1) + pd;
           const int nkc = s_ccn[cc];
+          const long zo = s_zoff[cc];
+          v[u] = Jp0 + u <= dhi && Jp >= Dlo && Jp <= Dhi && zo >= 0;
           const int il = Ti[Jp * norb + l];
           const int kj = Tj[Jp * norb + k] + (k > j ? nkc : 0);
-          z[u] = Zb[v[u] ? s_zoff[cc] + (long)kj * nkc + il : 0];
+          z[u] = Zb[v[u] ? zo + (long)kj * nkc + il : 0];
           const double *s6 = sixq + s6d + (long)Jp * s6J + (long)(c0 >> 1) * s6P;
 #pragma unroll
           for (int hh = 0; hh < kJC / 2; ++hh) w[u][hh] = *(const double2 *)(s6 + hh * s6P);
@@ -1590,12 +1603,13 @@ __global__ __launch_bounds__(256, 5) void k_invpandya(PairArg pa, const int4 *it
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
           const int Jp = min(Jp0 + u, xhi);
-          v[u] = Jp0 + u <= xhi && Jp >= Xlo && Jp <= Xhi;
           const int cc = 6 * Jp + 2 * (tx + 1) + px;
           const int nkc = s_ccn[cc];
+          const long zo = s_zoff[cc];
+          v[u] = Jp0 + u <= xhi && Jp >= Xlo && Jp <= Xhi && zo >= 0;
           const int ik = Ti[Jp * norb + k];
           const int lj = Tj[Jp * norb + l];
-          z[u] = Zb[v[u] ? s_zoff[cc] + (long)((l > j ? nkc : 0) + ik) * nkc + lj : 0];
+          z[u] = Zb[v[u] ? zo + (long)((l > j ? nkc : 0) + ik) * nkc + lj : 0];
           const double *s6 = sixq + s6x + (long)Jp * s6J + (long)(c0 >> 1) * s6P;
 #pragma unroll
           for (int hh = 0; hh < kJC / 2; ++hh) w[u][hh] = *(const double2 *)(s6 + hh * s6P);
@@ -1967,11 +1981,14 @@ hipError_t launch_pandya(const PairArg &pa, const int4 *items, int nitems, int m
   return hipGetLastError();
 }
 hipError_t launch_invpandya(const PairArg &pa, const int4 *items, int nitems, int jmax, int norb, int nch,
-                            int ncls, hipStream_t s) {
+                            int ncls, bool zbar_f32, hipStream_t s) {
   if (nitems <= 0) return hipSuccess;
   const size_t lds = invp_lds_bytes(jmax, norb, nch, ncls);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_invpandya, dim3(nitems), dim3(kChunk), lds, s, pa, items);
+  if (zbar_f32)
+    hipLaunchKernelGGL(k_invpandya<float>, dim3(nitems), dim3(kChunk), lds, s, pa, items);
+  else
+    hipLaunchKernelGGL(k_invpandya<double>, dim3(nitems), dim3(kChunk), lds, s, pa, items);
   return hipGetLastError();
 }
 hipError_t launch_finish(const PairArg &pa, const int4 *items, int nitems, bool fold, bool c122, hipStream_t s) {

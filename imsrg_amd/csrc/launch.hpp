@@ -26,11 +26,13 @@ hipError_t launch_pphh(const PairArg &pa, const int4 *items, int nitems, const i
 hipError_t launch_c122_coef(const PairArg &pa, int npairs, long maxn122, hipStream_t s);
 // maxnch: the largest channel count of the pairs (sizes the kernel's LDS channel table)
 hipError_t launch_pandya(const PairArg &pa, const int4 *items, int nitems, int maxnch, hipStream_t s);
-// ph GEMM (head and tail of Zbar) on 128x128 tiles (big) or 64x64 tiles
+// ph GEMM (head and tail of Zbar) on 128x128 tiles (big) or 64x64 tiles; precision 1
+// (FP32 products) stores Zbar in FP32, the others in FP64
 hipError_t launch_ph_gemm(const PairArg &pa, const int4 *items, int nitems, int precision, bool big, hipStream_t s);
-// jmax, norb, nch, ncls: the largest of the pairs (they size the kernel's LDS tables)
+// jmax, norb, nch, ncls: the largest of the pairs (they size the kernel's LDS tables);
+// zbar_f32: Zbar holds FP32 elements (written by the precision-1 ph GEMM)
 hipError_t launch_invpandya(const PairArg &pa, const int4 *items, int nitems, int jmax, int norb, int nch,
-                            int ncls, hipStream_t s);
+                            int ncls, bool zbar_f32, hipStream_t s);
 // last phase: with fold, upper += the inverse-Pandya contributions stored at the
 // transposed slots; with c122, Z2 -= W + hZ W^T (comm122ss); then lower triangle
 // <- hZ * upper
