@@ -547,9 +547,14 @@ __device__ __forceinline__ void st_nt2(double *p, double a, double b) {
 // FP32 Zbar (the FP32-product ph GEMM): the values rounded once, when stored
 __device__ __forceinline__ void st_nt(float *p, double v) { __builtin_nontemporal_store((float)v, p); }
 __device__ __forceinline__ void st_nt2(float *p, double a, double b) {
-  typedef float f2s __attribute__((ext_vector_type(2)));
+  typedef float f2s __attribute__((ext_vector_type(2), aligned(4)));
   const f2s v = {(float)a, (float)b};
   __builtin_nontemporal_store(v, reinterpret_cast<f2s *>(p));
+}
+__device__ __forceinline__ void st_nt4(float *p, double a, double b, double c, double d) {
+  typedef float f4s __attribute__((ext_vector_type(4), aligned(4)));  // 4-byte aligned rows
+  const f4s v = {(float)a, (float)b, (float)c, (float)d};
+  __builtin_nontemporal_store(v, reinterpret_cast<f4s *>(p));
 }
 
 // A wave of a 128x128 tile owns the 64 x kGW sub-tile at rows (w & 1) * 64,
@@ -615,37 +620,77 @@ __device__ __forceinline__ void gemm_epilogue(const dbl4 (&acc)[4][kGN], TD *D, 
         if (!tile_wave_live(R0, S0, n, w)) continue;  // block-uniform
         double(*Tt)[kGW + 1] = Tb(j);
         const int R1 = R0 + (w & 1) * 64, S1 = S0 + (w >> 1) * kGW;
-        // Two rows per thread and one 16-byte store where both are written (store
-        // instructions, not bytes, bound this epilogue); R1 and S1 are even.
-        // upper elements (R1 + rr, S1 + cc), consecutive threads on consecutive row pairs
-        for (int e = threadIdx.x; e < 32 * kGW; e += blockDim.x) {
-          const int rr = 2 * (e & 31), cc = e >> 5;
-          const int row = R1 + rr, col = S1 + cc;
-          if (col >= n || row >= n || row > col) continue;
-          TD *dp = D + (long)col * n + row;
-          if (row + 1 < n && row + 1 <= col)
-            st_nt2(dp, Tt[rr][cc], Tt[rr + 1][cc]);
-          else
-            st_nt(dp, Tt[rr][cc]);
-        }
-        // mirrored elements (S1 + cc, R1 + rr) = sym * upper (R1 + rr, S1 + cc),
-        // consecutive threads on consecutive row pairs S1 + cc
-        for (int e = threadIdx.x; e < 32 * kGW; e += blockDim.x) {
-          const int cc = 2 * (e & (kGW / 2 - 1)), rr = e / (kGW / 2);
-          const int row2 = S1 + cc, col2 = R1 + rr;
-          if (col2 >= n || row2 >= n) continue;
-          const bool lo = row2 > col2, hi = row2 + 1 < n && row2 + 1 > col2;
-          // phases from LDS: row2 - S0 and col2 - R0 lie in [0, 128)
-          const double s0 = (MODE == 1) ? h : (double)(s_ph[1][row2 - S0] * s_ph[0][col2 - R0]);
-          TD *dp = D + (long)col2 * n + row2;
-          if (lo && hi) {
-            const double s1 = (MODE == 1) ? h : (double)(s_ph[1][row2 + 1 - S0] * s_ph[0][col2 - R0]);
-            st_nt2(dp, s0 * Tt[rr][cc], s1 * Tt[rr][cc + 1]);
-          } else if (lo) {
-            st_nt(dp, s0 * Tt[rr][cc]);
-          } else if (hi) {
-            const double s1 = (MODE == 1) ? h : (double)(s_ph[1][row2 + 1 - S0] * s_ph[0][col2 - R0]);
-            st_nt(dp + 1, s1 * Tt[rr][cc + 1]);
+        if constexpr (std::is_same<TD, float>::value) {
+          // FP32 Zbar: four rows per thread and one 16-byte store where all four
+          // are written; R1 and S1 are multiples of 4
+          for (int e = threadIdx.x; e < 16 * kGW; e += blockDim.x) {
+            const int rr = 4 * (e & 15), cc = e >> 4;
+            const int row = R1 + rr, col = S1 + cc;
+            if (col >= n || row >= n || row > col) continue;
+            float *dp = D + (long)col * n + row;
+            if (row + 3 < n && row + 3 <= col) {
+              st_nt4(dp, Tt[rr][cc], Tt[rr + 1][cc], Tt[rr + 2][cc], Tt[rr + 3][cc]);
+            } else {
+#pragma unroll
+              for (int q = 0; q < 4; ++q)
+                if (row + q < n && row + q <= col) st_nt(dp + q, Tt[rr + q][cc]);
+            }
+          }
+          for (int e = threadIdx.x; e < 16 * kGW; e += blockDim.x) {
+            const int cc = 4 * (e & (kGW / 4 - 1)), rr = e / (kGW / 4);
+            const int row2 = S1 + cc, col2 = R1 + rr;
+            if (col2 >= n || row2 >= n) continue;
+            double v[4];
+            bool ok[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              ok[q] = row2 + q < n && row2 + q > col2;
+              // phases from LDS: row2 + q - S0 and col2 - R0 lie in [0, 128)
+              const double sq = (MODE == 1) ? h : (double)(s_ph[1][min(row2 + q - S0, kTileB - 1)] * s_ph[0][col2 - R0]);
+              v[q] = sq * Tt[rr][cc + q];
+            }
+            float *dp = D + (long)col2 * n + row2;
+            if (ok[0] && ok[1] && ok[2] && ok[3]) {
+              st_nt4(dp, v[0], v[1], v[2], v[3]);
+            } else {
+#pragma unroll
+              for (int q = 0; q < 4; ++q)
+                if (ok[q]) st_nt(dp + q, v[q]);
+            }
+          }
+        } else {
+          // Two rows per thread and one 16-byte store where both are written (store
+          // instructions, not bytes, bound this epilogue); R1 and S1 are even.
+          // upper elements (R1 + rr, S1 + cc), consecutive threads on consecutive row pairs
+          for (int e = threadIdx.x; e < 32 * kGW; e += blockDim.x) {
+            const int rr = 2 * (e & 31), cc = e >> 5;
+            const int row = R1 + rr, col = S1 + cc;
+            if (col >= n || row >= n || row > col) continue;
+            TD *dp = D + (long)col * n + row;
+            if (row + 1 < n && row + 1 <= col)
+              st_nt2(dp, Tt[rr][cc], Tt[rr + 1][cc]);
+            else
+              st_nt(dp, Tt[rr][cc]);
+          }
+          // mirrored elements (S1 + cc, R1 + rr) = sym * upper (R1 + rr, S1 + cc),
+          // consecutive threads on consecutive row pairs S1 + cc
+          for (int e = threadIdx.x; e < 32 * kGW; e += blockDim.x) {
+            const int cc = 2 * (e & (kGW / 2 - 1)), rr = e / (kGW / 2);
+            const int row2 = S1 + cc, col2 = R1 + rr;
+            if (col2 >= n || row2 >= n) continue;
+            const bool lo = row2 > col2, hi = row2 + 1 < n && row2 + 1 > col2;
+            // phases from LDS: row2 - S0 and col2 - R0 lie in [0, 128)
+            const double s0 = (MODE == 1) ? h : (double)(s_ph[1][row2 - S0] * s_ph[0][col2 - R0]);
+            TD *dp = D + (long)col2 * n + row2;
+            if (lo && hi) {
+              const double s1 = (MODE == 1) ? h : (double)(s_ph[1][row2 + 1 - S0] * s_ph[0][col2 - R0]);
+              st_nt2(dp, s0 * Tt[rr][cc], s1 * Tt[rr][cc + 1]);
+            } else if (lo) {
+              st_nt(dp, s0 * Tt[rr][cc]);
+            } else if (hi) {
+              const double s1 = (MODE == 1) ? h : (double)(s_ph[1][row2 + 1 - S0] * s_ph[0][col2 - R0]);
+              st_nt(dp + 1, s1 * Tt[rr][cc + 1]);
+            }
           }
         }
       }
