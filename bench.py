@@ -37,9 +37,12 @@ def algorithmic_counts(ms):
     basis tables (src/ModelSpace.cc:1120-1201, src/TwoBodyChannel.cc:142-188):
       F_A = sum_ch 2 n^2 (n_pp + 2 n_hh + n_ph)   pp/hh GEMMs (src/Commutator.cc:1176-1183)
       F_C = sum_cc 8 c^2 n_ph,cc                   ph GEMM (src/Commutator.cc:1789)
-      B_I = 8 (sum_cc 2 c^2 + sum_ch n(n+1)/2)     inverse Pandya: one read of Zbar,
-                                                   one store per upper Z element
-    Returns (F_A, F_A executed, F_C, B_I)."""
+      B_I = z sum_cc 2 c^2 + 8 sum_ch n(n+1)/2     inverse Pandya: one read of Zbar over the
+                                                   CC channels with ph kets (Zbar is zero in
+                                                   the others), one store per upper Z element;
+                                                   z = 8 for FP64 Zbar, 4 for the FP32 Zbar of
+                                                   the FP32-product ph GEMM
+    Returns (F_A, F_A executed, F_C, B_I with FP64 Zbar, B_I with FP32 Zbar)."""
     orb = ms.orbits
     occ = [o["occ"] for o in orb]
     FA = 0
@@ -73,8 +76,8 @@ def algorithmic_counts(ms):
                 e[0] += 1
                 e[1] += ph
     FC = sum(8 * c * c * nph for c, nph in cc.values() if nph > 0)
-    BI = 8 * (sum(2 * c * c for c, _ in cc.values()) + upper)
-    return FA, executed_A, FC, BI
+    zbar = sum(2 * c * c for c, nph in cc.values() if nph > 0)
+    return FA, executed_A, FC, 8 * zbar + 8 * upper, 4 * zbar + 8 * upper
 
 
 def profiled_traffic(kernel):
@@ -373,7 +376,10 @@ def main():
     # (phase durations are HIP events recorded around each launch during the
     # timed steps, on the stream the kernel runs on; with the pp/hh GEMM
     # overlapping other phases, durations include the sharing of the GPU)
-    FA, execA, FC, BI = algorithmic_counts(ms)
+    FA, execA, FC, BI64, BI32 = algorithmic_counts(ms)
+    # the FP32-product ph GEMM (mixed_ph_fp32 = 1) leaves Zbar in FP32
+    zbar32 = use_mixed and tun["mixed_ph_fp32"] == 1
+    BI = BI32 if zbar32 else BI64
     dom = int(np.argmax(phase_avg))
     t_dom = phase_avg[dom] * 1e-6
     if PHASES[dom] == "pphh_gemm":
@@ -395,8 +401,9 @@ def main():
                 "executed_flops": execA}
     elif PHASES[dom] == "inverse_pandya":
         ach = BI / t_dom
-        roof = {"kernel": "k_invpandya", "bound": "hbm", "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9,
-                "unit": "GB/s", "frac": ach / PEAK_HBM, "traffic": profiled_traffic("k_invpandya"),
+        kname = "k_invpandya<float>" if zbar32 else "k_invpandya<double>"
+        roof = {"kernel": kname, "bound": "hbm", "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9,
+                "unit": "GB/s", "frac": ach / PEAK_HBM, "traffic": profiled_traffic(kname),
                 "algorithmic_bytes": BI}
     else:
         roof = {"kernel": PHASES[dom], "bound": "hbm", "achieved": None, "peak": PEAK_HBM / 1e9, "unit": "GB/s",

@@ -57,7 +57,9 @@ extern "C" {
                               flushed into FP64 accumulators; channels of up to
                               pphh_small_tile_max kets on FP64 64x64 tiles.  Setting 1:
                               FP32-rounded operands on FP32 MFMA, FP32 partial sums over
-                              32 k (16 k on 64x64 tiles) flushed into FP64 */
+                              32 k (16 k on 64x64 tiles) flushed into FP64; the ph GEMM on
+                              this setting (its default) stores its output Zbar in FP32,
+                              which the inverse Pandya gather sums in FP64 */
 #define IMSRG_PREC_SPLIT 2 /* pp/hh and ph GEMMs on 128x128 tiles: each FP64 operand split into
                               two FP32 values (hi + lo), three FP32 MFMA products per
                               product (hi*hi + hi*lo + lo*hi), FP32 partial sums flushed
@@ -270,8 +272,9 @@ typedef struct {
                                operands (default; channels on 64x64 tiles stay FP64), 1 = FP32 MFMA
                                on FP32-rounded operands, 0 = FP64 MFMA (IMSRG_PREC_SPLIT ignores both
                                flags) */
-  int mixed_ph_fp32;        /* IMSRG_PREC_MIXED: ph GEMM, same choices (default 1: its time is set by
-                               the FP64 Zbar it writes, and the bf16 kernel is slower there) */
+  int mixed_ph_fp32;        /* IMSRG_PREC_MIXED: ph GEMM, same choices (default 1, which also keeps
+                               Zbar in FP32: its time is set by the Zbar it writes and its per-tile
+                               costs, and the bf16 kernel, which writes FP64 Zbar, is slower) */
   int ph_tile_128;          /* 1: ph GEMM on 128x128 tiles, 0: on 64x64 tiles */
   int finish_order;         /* tile pairs of the last kernel (fold, comm122, mirror): 0 in channel and
                                row-block order, 1 cut into 8 runs of consecutive tiles, one per XCD */
