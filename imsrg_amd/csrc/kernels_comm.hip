@@ -1452,10 +1452,13 @@ __device__ __forceinline__ int wave_max(int v) {
 //   commij(J) = -sum_J' (2J'+1) {ji jj J; jk jl J'} Zbar^{J'}[il, kj]
 //   commji(J) = -sum_J' (2J'+1) {jj ji J; jk jl J'} Zbar^{J'}[jl, ki]
 //   Z^J_{ij,kl} += -(commij - (-1)^{(jk+jl)/2 - J} commji) / norm
-// with the reference's |6j| < 1e-8 skip, which the 6j copy sixjQ holds as zeros.
-// Every J of a chunk is accumulated unmasked: a J outside the lane's triangles
-// has a zero 6j, and the accumulators of J values the lane does not own (odd J
-// of a (p,p) ket, J past the range) are never stored.  The cross-coupled local indices of
+// The 6j copy sixjQ holds the weights (2J'+1) {...}, with zeros where the
+// reference skips |6j| < 1e-8.  Every J of a chunk is accumulated unmasked: a J
+// outside the lane's triangles has a zero 6j, and the accumulators of J values
+// the lane does not own (odd J of a (p,p) ket, J past the range) are never
+// stored.  CC channels without ph kets (Zbar = 0, not written by the ph GEMM)
+// are skipped.  TZ is the element type of Zbar: float after the FP32-product ph
+// GEMM, double otherwise; the sums are FP64.  The cross-coupled local indices of
 // the kets (i,x) and (j,x), the CC channel sizes/offsets and the bra's
 // standard-channel data are staged in LDS once per work-group.  Every upper
 // element (ij <= kl, or ij < kl for anti-Hermitian Z) of every channel is
