@@ -247,6 +247,19 @@ class DeviceOperator:
     def TwoBodyNorm(self):
         return self.norms()[1]
 
+    def DoNormalOrdering(self, out=None):
+        """Operator::DoNormalOrdering (src/Operator.cc:392-400, 461-579): a new operator
+        (or `out`, which may be self) normal-ordered over the model space's reference."""
+        out = out if out is not None else DeviceOperator(self.ms, self.herm)
+        check(lib().imsrg_hip_op_normal_order(out.h, self.h, 1), self.ms.h)
+        return out
+
+    def UndoNormalOrdering(self, out=None):
+        """Operator::UndoNormalOrdering (src/Operator.cc:402-415)."""
+        out = out if out is not None else DeviceOperator(self.ms, self.herm)
+        check(lib().imsrg_hip_op_normal_order(out.h, self.h, -1), self.ms.h)
+        return out
+
     def WriteBinary(self, path):
         """Operator::WriteBinary (src/Operator.cc:341-362): the reference's byte layout."""
         check(lib().imsrg_hip_op_write_binary(self.h, str(path).encode()), self.ms.h)

@@ -631,6 +631,25 @@ extern "C" int imsrg_hip_op_norm(imsrg_dop *op, double *n1, double *n2, double *
   return op_norm(op, n1, n2, n);
 }
 
+// Operator::DoNormalOrdering (sign +1) / UndoNormalOrdering (sign -1) of a scalar
+// operator over the context's reference (src/Operator.cc:392-415, 461-579).
+extern "C" int imsrg_hip_op_normal_order(imsrg_dop *out, const imsrg_dop *op, int sign) {
+  if (!out || !op) return IMSRG_E_INVALID;
+  if (!same_ctx(out, op)) return IMSRG_E_INVALID;
+  imsrg_ctx *ctx = out->ctx;
+  if (sign != 1 && sign != -1) return fail(ctx, IMSRG_E_INVALID, "normal-ordering sign must be +1 or -1");
+  if (op->herm != 1 && op->herm != -1)
+    return fail(ctx, IMSRG_E_UNSUPPORTED, "normal ordering needs a Hermitian or anti-Hermitian operator");
+  // the one-body sum runs only when the two-body norm exceeds 1e-7 (src/Operator.cc:493)
+  double n2 = 0.0;
+  if (int rc = op_norm(const_cast<imsrg_dop *>(op), nullptr, &n2, nullptr)) return rc;
+  if (out != op) HIPCHK(ctx, hipMemcpyAsync(out->buf, op->buf, sizeof(double) * ctx->db.h.opsize,
+                                            hipMemcpyDeviceToDevice, ctx->stream));
+  out->herm = op->herm;
+  HIPCHK(ctx, launch_normal_order(ctx->db.d, ctx->db.h, out->buf, sign, op->herm, n2 > 1e-7, ctx->stream));
+  return IMSRG_OK;
+}
+
 // ---------------------------------------------------------------------------
 static int comm_opts_ok(imsrg_ctx *ctx, const imsrg_comm_opts *o, unsigned *mask, int *prec) {
   *mask = IMSRG_ALL_TERMS;

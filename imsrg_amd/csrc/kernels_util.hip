@@ -218,6 +218,85 @@ __global__ void k_norm_final(const double *part, int nblk2, double *out) {
   }
 }
 
+// Operator::DoNormalOrdering2 of a scalar operator (src/Operator.cc:461-579) in
+// place, sign +1 (normal ordering) or -1 (undoing it), over the reference's holes.
+// Zero-body part (:467-490): one thread, in the reference's order (holes k, holes
+// l >= k, J), E0 += sign (2j_k+1) n_k h_kk + (2J+1) n_k n_l V^J_{kl,kl} (normalized);
+// it reads the one-body diagonal before k_no_one changes it.
+__global__ void k_no_zero(const BasisDev *Bp, double *op, int sign) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const BasisDev &B = *Bp;
+  const int norb = B.norb;
+  const double *one = op + B.one_off, *two = op + B.two_off;
+  double e0 = op[0];
+  for (int ik = 0; ik < B.nholes; ++ik) {
+    const int k = B.holes[ik];
+    e0 += (B.j2[k] + 1) * sign * B.occ[k] * one[(long)k * norb + k];
+    for (int il = 0; il < B.nholes; ++il) {
+      const int l = B.holes[il];
+      if (l < k) continue;
+      const int par = (B.l[k] + B.l[l]) & 1, tz = (B.tz2[k] + B.tz2[l]) / 2;
+      const int kl = ket_index(norb, k, l);
+      for (int J = abs(B.j2[k] - B.j2[l]) / 2; J <= (B.j2[k] + B.j2[l]) / 2; ++J) {
+        const int i = B.kmapT[(long)J * B.nkets + kl];  // -1: (k,k) at odd J
+        if (i < 0) continue;
+        const int ch = 6 * J + 2 * (tz + 1) + par, n = B.ch_n[ch];
+        e0 += (2 * J + 1) * B.occ[k] * B.occ[l] * two[B.ch_off[ch] + (long)i * n + i];
+      }
+    }
+  }
+  op[0] = e0;
+}
+
+// One-body part (:493-560) and the (anti-)Hermitian completion (Symmetrize /
+// AntiSymmetrize, :1797-1830): one thread per pair a <= b of a one-body block,
+//   f_ab += sign (2J+1)/(2j_a+1) n_h V~^J_{ah,bh}
+// over the channels in index order (J, then Tz and parity) and the holes h, with
+// the tilde element (sqrt 2 per ket with equal orbits, (-1)^{(j_p+j_q)/2+1+J} per
+// swapped ket); add = 0 when the two-body norm is below the reference's 1e-7.
+__global__ __launch_bounds__(256) void k_no_one(const BasisDev *Bp, double *op, int sign, int herm, int add) {
+  const BasisDev &B = *Bp;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= B.n1pairs) return;
+  const int2 pr = B.pairs1[p];
+  const int a = pr.x, b = pr.y, norb = B.norb;
+  double *one = op + B.one_off;
+  const double *two = op + B.two_off;
+  double f = one[(long)b * norb + a];
+  if (add) {
+    const double ja = B.j2[a] / 2.0;
+    for (int J = 0; J <= B.jmax; ++J)
+      for (int c6 = 0; c6 < 6; ++c6)
+        for (int ih = 0; ih < B.nholes; ++ih) {
+          const int h = B.holes[ih];
+          const int par = (B.l[a] + B.l[h]) & 1, tz = (B.tz2[a] + B.tz2[h]) / 2;
+          if (2 * (tz + 1) + par != c6) continue;
+          const int i = B.kmapT[(long)J * B.nkets + ket_index(norb, min(a, h), max(a, h))];
+          const int k = B.kmapT[(long)J * B.nkets + ket_index(norb, min(b, h), max(b, h))];
+          if (i < 0 || k < 0) continue;
+          const int ch = 6 * J + c6, n = B.ch_n[ch];
+          double ph = 1.0;
+          if (a > h) ph *= ((B.j2[a] + B.j2[h]) / 2 + 1 + J) & 1 ? -1.0 : 1.0;
+          if (b > h) ph *= ((B.j2[b] + B.j2[h]) / 2 + 1 + J) & 1 ? -1.0 : 1.0;
+          const double nrm = (a == h ? 1.4142135623730951 : 1.0) * (b == h ? 1.4142135623730951 : 1.0);
+          const double v = nrm * (ph * two[B.ch_off[ch] + (long)k * n + i]);
+          const double w = (2 * J + 1.0) / (2 * ja + 1.0);
+          f += w * sign * B.occ[h] * v;
+        }
+  }
+  if (herm == -1 && a == b) f = 0.0;
+  one[(long)b * norb + a] = f;
+  if (a != b) one[(long)a * norb + b] = herm * f;
+}
+
+hipError_t launch_normal_order(const BasisDev *bd, const BasisDev &bh, double *op, int sign, int herm, bool add,
+                               hipStream_t s) {
+  hipLaunchKernelGGL(k_no_zero, dim3(1), dim3(64), 0, s, bd, op, sign);
+  if (bh.n1pairs > 0)
+    hipLaunchKernelGGL(k_no_one, dim3((bh.n1pairs + 255) / 256), dim3(256), 0, s, bd, op, sign, herm, add ? 1 : 0);
+  return hipGetLastError();
+}
+
 hipError_t launch_norm(const BasisDev *bd, const BasisDev &bh, const double *op, double *partials, int npart,
                        double *out, hipStream_t s) {
   (void)bh;

@@ -156,6 +156,14 @@ int imsrg_hip_op_axpy(imsrg_dop *y, double a, const imsrg_dop *x); /* y += a*x *
 int imsrg_hip_op_zero(imsrg_dop *op);
 /* Operator::Norm / OneBodyNorm / TwoBodyNorm (src/Operator.cc:1610-1646, src/TwoBodyME.cc:703-724) */
 int imsrg_hip_op_norm(imsrg_dop *op, double *norm1, double *norm2, double *norm);
+/* Operator::DoNormalOrdering (sign = +1) and Operator::UndoNormalOrdering (sign = -1) of a
+   scalar two-body operator, DoNormalOrdering2 over the reference's holes
+   (src/Operator.cc:392-415, 461-579): E0 += sign sum_k (2j_k+1) n_k h_kk + sum_{k<=l} sum_J
+   (2J+1) n_k n_l V^J_klkl, f_ab += sign sum_J (2J+1)/(2j_a+1) sum_h n_h V~^J_ahbh (when the
+   two-body norm exceeds 1e-7), the one-body part completed from its upper triangle; the
+   two-body part is unchanged.  out may be op (in place); both on the same context.
+   Asynchronous on the context's stream after one host read of the two-body norm. */
+int imsrg_hip_op_normal_order(imsrg_dop *out, const imsrg_dop *op, int sign);
 
 /* device-resident versions of the drop-ins (no PCIe traffic) */
 int imsrg_hip_commutator_ss_dev(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y, imsrg_dop *Z,

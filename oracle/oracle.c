@@ -63,7 +63,8 @@ static inline int phase_of(int x) { return (x % 2 == 0) ? 1 : -1; }
  * in the Pandya transform; 5: two-body norm weighted by (2J+1) instead of
  * (2J+1)^2; 6: one-body norm without the (2j+1) weight; 7: Omega not cleared at
  * a split; 8: BCH_Transform of H(s) in place of H0 / H_saved; 9: BCH_Product
- * without its 1/12 [[Y,X],X] term. */
+ * without its 1/12 [[Y,X],X] term; 10: normal-ordering one-body sum without the
+ * 1/(2j_a+1) factor. */
 static int g_fault = 0;
 void orc_set_fault(int f) { g_fault = f; }
 
@@ -1145,6 +1146,70 @@ double orc_twobody_norm(const oms_t *ms, const orc_op *op) {
     nrm += n2 * n2;
   }
   return sqrt(nrm);
+}
+
+/* Operator::DoNormalOrdering2 for a scalar operator (src/Operator.cc:461-579),
+ * with sign +1 (DoNormalOrdering, :392-400) or -1 (UndoNormalOrdering, :402-415)
+ * over the reference's holes with their occupations:
+ *   E0 += sign sum_k (2j_k+1) n_k h_kk + sum_{k<=l} sum_J (2J+1) n_k n_l V^J_{kl,kl} (normalized)
+ *   f_ab += sign (2J+1)/(2j_a+1) n_h V~^J_{ah,bh} over the channels (ascending), then the holes h,
+ *           for b >= a in a's one-body block (only when the two-body norm exceeds 1e-7);
+ * then the one-body part is made (anti-)Hermitian from its upper triangle (Symmetrize /
+ * AntiSymmetrize, :1797-1830).  The two-body part is unchanged.  out may alias in. */
+void orc_normal_order(const oms_t *ms, const orc_op *in, orc_op *out, int sign) {
+  const int norb = ms->norb;
+  double e0 = *in->zero;
+  for (int ik = 0; ik < ms->nholes; ++ik) {  /* :469-490 */
+    const int k = ms->holes[ik];
+    e0 += (ms->j2[k] + 1) * sign * ms->occ[k] * ONE(in, k, k);
+    for (int il = 0; il < ms->nholes; ++il) {
+      const int l = ms->holes[il];
+      if (l < k) continue;
+      const int Jmin = abs(ms->j2[k] - ms->j2[l]) / 2, Jmax = (ms->j2[k] + ms->j2[l]) / 2;
+      for (int J = Jmin; J <= Jmax; ++J)
+        e0 += (2 * J + 1) * ms->occ[k] * ms->occ[l] * tbme_norm_J(ms, in->two, J, k, l, k, l);
+    }
+  }
+  double *f = (double *)malloc(sizeof(double) * (size_t)norb * norb);
+  memcpy(f, in->one, sizeof(double) * (size_t)norb * norb);
+  if (orc_twobody_norm(ms, in) > 1e-7) {  /* :493 */
+    for (int c = 0; c < ms->nch; ++c) {   /* :495-560, scalar operator: ch_bra = ch_ket */
+      const chan_t *ch = &ms->ch[c];
+      const int J = ch->J;
+      for (int a = 0; a < norb; ++a) {
+        const double ja = ms->j2[a] / 2.0;
+        for (int ib = 0; ib < ms->obn[a]; ++ib) {
+          const int b = ms->obl[a][ib];
+          if (b < a) continue;
+          for (int ih = 0; ih < ms->nholes; ++ih) {
+            const int h = ms->holes[ih];
+            /* GetTBME(ch, ch, a, h, b, h): zero unless both kets lie in channel c */
+            const int par = (ms->l[a] + ms->l[h]) % 2, tz = (ms->tz2[a] + ms->tz2[h]) / 2;
+            if (par != ch->par || tz != ch->tz) continue;
+            double v = orc_tbme_J(ms, in->two, J, a, h, b, h);
+            double w = (2 * J + 1.0) / (2 * ja + 1.0);
+            if (g_fault == 10) w = (2 * J + 1.0);
+            f[(long)b * norb + a] += w * sign * ms->occ[h] * v;
+          }
+        }
+      }
+    }
+  }
+  for (int a = 0; a < norb; ++a)  /* upper triangle -> lower */
+    for (int b = a; b < norb; ++b) {
+      const double u = f[(long)b * norb + a];
+      if (in->herm == -1) {
+        f[(long)b * norb + a] = (a == b) ? 0.0 : u;
+        f[(long)a * norb + b] = (a == b) ? 0.0 : -u;
+      } else {
+        f[(long)a * norb + b] = u;
+      }
+    }
+  memcpy(out->one, f, sizeof(double) * (size_t)norb * norb);
+  free(f);
+  if (out->two != in->two) memcpy(out->two, in->two, sizeof(double) * (size_t)ms->tbsize);
+  *out->zero = e0;
+  out->herm = in->herm;
 }
 
 /* Operator::Norm, src/Operator.cc:1610-1628 */

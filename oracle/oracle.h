@@ -76,6 +76,11 @@ double orc_norm(const oms_t *ms, const orc_op *op);
 double orc_onebody_norm(const oms_t *ms, const orc_op *op);
 double orc_twobody_norm(const oms_t *ms, const orc_op *op);
 
+/* Operator::DoNormalOrdering (sign +1) / UndoNormalOrdering (sign -1) of a scalar
+ * operator over the reference's holes (reference src/Operator.cc:392-415, 461-579);
+ * out may alias in */
+void orc_normal_order(const oms_t *ms, const orc_op *in, orc_op *out, int sign);
+
 /* BCH (reference src/BCH.cc:87-201, 267-350).
  * mode 0: reference epsilon rule with threshold thr (default 1e-9);
  * mode 1: stop when ||nested|| < thr (absolute).

@@ -78,6 +78,7 @@ def lib():
         L.orc_bch_product.restype = C.c_int
         L.orc_bch_product.argtypes = [C.c_void_p, C.POINTER(_Op), C.POINTER(_Op), C.POINTER(_Op), C.c_double]
         L.orc_white_generator.argtypes = [C.c_void_p, C.POINTER(_Op), C.POINTER(_Op)]
+        L.orc_normal_order.argtypes = [C.c_void_p, C.POINTER(_Op), C.POINTER(_Op), C.c_int]
         L.orc_magnus_flow.restype = C.c_int
         L.orc_magnus_flow.argtypes = [C.c_void_p, C.POINTER(_Op), C.POINTER(_Op), C.c_double, C.c_double,
                                       C.c_double, C.c_double, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp,
@@ -257,6 +258,12 @@ class OracleModelSpace:
         a, b, o = _Holder(X), _Holder(Y), _Holder(self.zero_op(X["herm"]))
         n = lib().orc_bch_product(self.h, C.byref(a.s), C.byref(b.s), C.byref(o.s), thr)
         return o.op(), n
+
+    def normal_order(self, op, sign=1):
+        """DoNormalOrdering (sign +1) / UndoNormalOrdering (sign -1) of a scalar operator."""
+        a, o = _Holder(op), _Holder(self.zero_op(op["herm"]))
+        lib().orc_normal_order(self.h, C.byref(a.s), C.byref(o.s), int(sign))
+        return o.op()
 
     def white_generator(self, H):
         a, o = _Holder(H), _Holder(self.zero_op(-1))
