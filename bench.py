@@ -267,6 +267,52 @@ def workload_flow(args, ia, np):
             "emax4_flow_eager_vs_graphs": res["emax4_flow"]}
 
 
+def workload_normal_order(args, ia, np):
+    """Operator::DoNormalOrdering of a scalar Hermitian operator at emax=10 (seed 6001,
+    E0 = 0.5) into another operator: device time per call (the copy of the operator,
+    its two-body norm, which the reference tests against 1e-7, and the two kernels),
+    the oracle's time for the same call on one host core, and the difference of the
+    two results."""
+    emax = 10
+    ms = ia.ModelSpace(emax, "O16", device=0)
+    H = ia.DeviceOperator(ms, 1).fill_random(1, 6001)
+    Hh = H.to_host()
+    Hh.ZeroBody = 0.5
+    H.upload(Hh)
+    out = ia.DeviceOperator(ms, 1)
+    for _ in range(3):
+        H.DoNormalOrdering(out=out)
+    ms.synchronize()
+    reps = max(args.steps, 20)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        H.DoNormalOrdering(out=out)
+    ms.synchronize()
+    us = (time.perf_counter() - t0) / reps * 1e6
+    got = out.to_host()
+    line = {"metric": f"normal ordering (Operator::DoNormalOrdering) of a scalar operator at emax={emax}",
+            "value": us, "unit": "us/call", "n_gpus": 1, "steps": reps, "higher_is_better": False, "dtype": "f64",
+            "data": "synthetic (seeded, library generator)",
+            "config": {"workload": f"DoNormalOrdering, HO emax={emax}, O16 reference, {len(ms.orbits)} orbits"},
+            "E0_after": float(got.ZeroBody)}
+    if not args.no_cpu_baseline:
+        from oracle.pyoracle import OracleModelSpace
+        orc = OracleModelSpace(emax, 16, 8)
+        op = {"zero": float(Hh.ZeroBody), "one": np.asarray(Hh.OneBody), "two": np.asarray(Hh.TwoBody), "herm": 1}
+        samples = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            ref = orc.normal_order(op, 1)
+            samples.append(time.perf_counter() - t0)
+        scale = max(np.max(np.abs(ref["one"])), 1.0)
+        line["cpu_baseline"] = {"value": float(np.median(samples)) * 1e6, "unit": "us/call", "cores": 1,
+                                "kind": "port", "sample": "median of 5 calls of the oracle's C restatement "
+                                                          "of DoNormalOrdering2 on the same operator"}
+        line["gpu_vs_cpu_rel_maxabs"] = float(np.max(np.abs(np.asarray(got.OneBody) - ref["one"])) / scale)
+        line["gpu_vs_cpu_dE0"] = float(abs(got.ZeroBody - ref["zero"]))
+    return line
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -276,9 +322,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--autotune-cache", default=None,
                     help="text cache of the autotuner: settings found there are applied without timing trials")
-    ap.add_argument("--workload", default="commutator", choices=["commutator", "batched", "emax12", "emax14", "flow"],
+    ap.add_argument("--workload", default="commutator",
+                    choices=["commutator", "batched", "emax12", "emax14", "flow", "normal_order"],
                     help="commutator: the headline metric line; batched: 48 independent pairs at emax 4-7; "
-                         "emax12 / emax14: precision report at emax=12 / 14; flow: Magnus flow at emax=10")
+                         "emax12 / emax14: precision report at emax=12 / 14; flow: Magnus flow at emax=10; "
+                         "normal_order: DoNormalOrdering at emax=10")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X; runs on more than one GPU are not part of this build "
@@ -291,7 +339,8 @@ def main():
 
     if args.workload != "commutator":
         line = {"batched": workload_batched, "emax12": workload_emax12,
-                "emax14": lambda a, i, n: workload_emax12(a, i, n, emax=14), "flow": workload_flow}[args.workload](
+                "emax14": lambda a, i, n: workload_emax12(a, i, n, emax=14), "flow": workload_flow,
+                "normal_order": workload_normal_order}[args.workload](
             args, ia, np)
         print(json.dumps(line), flush=True)
         return

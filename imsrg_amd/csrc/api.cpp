@@ -640,13 +640,13 @@ extern "C" int imsrg_hip_op_normal_order(imsrg_dop *out, const imsrg_dop *op, in
   if (sign != 1 && sign != -1) return fail(ctx, IMSRG_E_INVALID, "normal-ordering sign must be +1 or -1");
   if (op->herm != 1 && op->herm != -1)
     return fail(ctx, IMSRG_E_UNSUPPORTED, "normal ordering needs a Hermitian or anti-Hermitian operator");
-  // the one-body sum runs only when the two-body norm exceeds 1e-7 (src/Operator.cc:493)
-  double n2 = 0.0;
-  if (int rc = op_norm(const_cast<imsrg_dop *>(op), nullptr, &n2, nullptr)) return rc;
+  // the one-body sum runs only when the two-body norm exceeds 1e-7 (src/Operator.cc:493):
+  // the norm stays on the device, where the kernel tests it
+  HIPCHK(ctx, launch_norm(ctx->db.d, ctx->db.h, op->buf, ctx->partials, ctx->npart, ctx->norm_dev, ctx->stream));
   if (out != op) HIPCHK(ctx, hipMemcpyAsync(out->buf, op->buf, sizeof(double) * ctx->db.h.opsize,
                                             hipMemcpyDeviceToDevice, ctx->stream));
   out->herm = op->herm;
-  HIPCHK(ctx, launch_normal_order(ctx->db.d, ctx->db.h, out->buf, sign, op->herm, n2 > 1e-7, ctx->stream));
+  HIPCHK(ctx, launch_normal_order(ctx->db.d, ctx->db.h, out->buf, sign, op->herm, ctx->norm_dev, ctx->stream));
   return IMSRG_OK;
 }
 

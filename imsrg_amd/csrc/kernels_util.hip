@@ -220,80 +220,83 @@ __global__ void k_norm_final(const double *part, int nblk2, double *out) {
 
 // Operator::DoNormalOrdering2 of a scalar operator (src/Operator.cc:461-579) in
 // place, sign +1 (normal ordering) or -1 (undoing it), over the reference's holes.
-// Zero-body part (:467-490): one thread, in the reference's order (holes k, holes
-// l >= k, J), E0 += sign (2j_k+1) n_k h_kk + (2J+1) n_k n_l V^J_{kl,kl} (normalized);
-// it reads the one-body diagonal before k_no_one changes it.
-__global__ void k_no_zero(const BasisDev *Bp, double *op, int sign) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// Zero-body part (:467-490): E0 += sign sum_k (2j_k+1) n_k h_kk + sum_{k<=l} sum_J
+// (2J+1) n_k n_l V^J_{kl,kl} (normalized), one thread per hole pair and a
+// fixed-order block sum; it reads the one-body diagonal before k_no_one changes it.
+__global__ __launch_bounds__(256) void k_no_zero(const BasisDev *Bp, double *op, int sign) {
+  __shared__ double red[4];
   const BasisDev &B = *Bp;
-  const int norb = B.norb;
+  const int norb = B.norb, nh = B.nholes;
   const double *one = op + B.one_off, *two = op + B.two_off;
-  double e0 = op[0];
-  for (int ik = 0; ik < B.nholes; ++ik) {
-    const int k = B.holes[ik];
-    e0 += (B.j2[k] + 1) * sign * B.occ[k] * one[(long)k * norb + k];
-    for (int il = 0; il < B.nholes; ++il) {
-      const int l = B.holes[il];
-      if (l < k) continue;
-      const int par = (B.l[k] + B.l[l]) & 1, tz = (B.tz2[k] + B.tz2[l]) / 2;
-      const int kl = ket_index(norb, k, l);
-      for (int J = abs(B.j2[k] - B.j2[l]) / 2; J <= (B.j2[k] + B.j2[l]) / 2; ++J) {
-        const int i = B.kmapT[(long)J * B.nkets + kl];  // -1: (k,k) at odd J
-        if (i < 0) continue;
-        const int ch = 6 * J + 2 * (tz + 1) + par, n = B.ch_n[ch];
-        e0 += (2 * J + 1) * B.occ[k] * B.occ[l] * two[B.ch_off[ch] + (long)i * n + i];
-      }
+  double acc = 0.0;
+  for (int t = threadIdx.x; t < nh * nh; t += blockDim.x) {
+    const int ik = t / nh, il = t - ik * nh;
+    const int k = B.holes[ik], l = B.holes[il];
+    if (ik == il) acc += (B.j2[k] + 1) * sign * B.occ[k] * one[(long)k * norb + k];
+    if (l < k) continue;
+    const int par = (B.l[k] + B.l[l]) & 1, tz = (B.tz2[k] + B.tz2[l]) / 2;
+    const int kl = ket_index(norb, k, l);
+    for (int J = abs(B.j2[k] - B.j2[l]) / 2; J <= (B.j2[k] + B.j2[l]) / 2; ++J) {
+      const int i = B.kmapT[(long)J * B.nkets + kl];  // -1: (k,k) at odd J
+      if (i < 0) continue;
+      const int ch = 6 * J + 2 * (tz + 1) + par, n = B.ch_n[ch];
+      acc += (2 * J + 1) * B.occ[k] * B.occ[l] * two[B.ch_off[ch] + (long)i * n + i];
     }
   }
-  op[0] = e0;
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) op[0] += (red[0] + red[1]) + (red[2] + red[3]);
 }
 
 // One-body part (:493-560) and the (anti-)Hermitian completion (Symmetrize /
-// AntiSymmetrize, :1797-1830): one thread per pair a <= b of a one-body block,
-//   f_ab += sign (2J+1)/(2j_a+1) n_h V~^J_{ah,bh}
-// over the channels in index order (J, then Tz and parity) and the holes h, with
-// the tilde element (sqrt 2 per ket with equal orbits, (-1)^{(j_p+j_q)/2+1+J} per
-// swapped ket); add = 0 when the two-body norm is below the reference's 1e-7.
-__global__ __launch_bounds__(256) void k_no_one(const BasisDev *Bp, double *op, int sign, int herm, int add) {
+// AntiSymmetrize, :1797-1830): one wave per pair a <= b of a one-body block, one
+// lane per hole h,
+//   f_ab += sign sum_J (2J+1)/(2j_a+1) n_h V~^J_{ah,bh}
+// with the tilde element (sqrt 2 per ket with equal orbits, (-1)^{(j_p+j_q)/2+1+J}
+// per swapped ket), summed over the lanes in a fixed order.  The sum is skipped
+// when the two-body norm (n2sq = its square, on the device) is at most 1e-7.
+__global__ __launch_bounds__(256) void k_no_one(const BasisDev *Bp, double *op, int sign, int herm,
+                                                const double *n2sq) {
   const BasisDev &B = *Bp;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= B.n1pairs) return;
+  const int p = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (p >= B.n1pairs) return;  // wave-uniform
   const int2 pr = B.pairs1[p];
   const int a = pr.x, b = pr.y, norb = B.norb;
   double *one = op + B.one_off;
   const double *two = op + B.two_off;
-  double f = one[(long)b * norb + a];
-  if (add) {
-    const double ja = B.j2[a] / 2.0;
-    for (int J = 0; J <= B.jmax; ++J)
-      for (int c6 = 0; c6 < 6; ++c6)
-        for (int ih = 0; ih < B.nholes; ++ih) {
-          const int h = B.holes[ih];
-          const int par = (B.l[a] + B.l[h]) & 1, tz = (B.tz2[a] + B.tz2[h]) / 2;
-          if (2 * (tz + 1) + par != c6) continue;
-          const int i = B.kmapT[(long)J * B.nkets + ket_index(norb, min(a, h), max(a, h))];
-          const int k = B.kmapT[(long)J * B.nkets + ket_index(norb, min(b, h), max(b, h))];
-          if (i < 0 || k < 0) continue;
-          const int ch = 6 * J + c6, n = B.ch_n[ch];
-          double ph = 1.0;
-          if (a > h) ph *= ((B.j2[a] + B.j2[h]) / 2 + 1 + J) & 1 ? -1.0 : 1.0;
-          if (b > h) ph *= ((B.j2[b] + B.j2[h]) / 2 + 1 + J) & 1 ? -1.0 : 1.0;
-          const double nrm = (a == h ? 1.4142135623730951 : 1.0) * (b == h ? 1.4142135623730951 : 1.0);
-          const double v = nrm * (ph * two[B.ch_off[ch] + (long)k * n + i]);
-          const double w = (2 * J + 1.0) / (2 * ja + 1.0);
-          f += w * sign * B.occ[h] * v;
-        }
+  double acc = 0.0;
+  if (*n2sq > 1e-14) {
+    const double w0 = 1.0 / (B.j2[a] + 1.0);  // 1 / (2 j_a + 1)
+    for (int ih = lane; ih < B.nholes; ih += 64) {
+      const int h = B.holes[ih];
+      const int par = (B.l[a] + B.l[h]) & 1, tz = (B.tz2[a] + B.tz2[h]) / 2;
+      const int kah = ket_index(norb, min(a, h), max(a, h)), kbh = ket_index(norb, min(b, h), max(b, h));
+      const double nrm = (a == h ? 1.4142135623730951 : 1.0) * (b == h ? 1.4142135623730951 : 1.0);
+      for (int J = abs(B.j2[a] - B.j2[h]) / 2; J <= (B.j2[a] + B.j2[h]) / 2; ++J) {
+        const int i = B.kmapT[(long)J * B.nkets + kah], k = B.kmapT[(long)J * B.nkets + kbh];
+        if (i < 0 || k < 0) continue;
+        const int ch = 6 * J + 2 * (tz + 1) + par, n = B.ch_n[ch];
+        double ph = 1.0;
+        if (a > h) ph *= ((B.j2[a] + B.j2[h]) / 2 + 1 + J) & 1 ? -1.0 : 1.0;
+        if (b > h) ph *= ((B.j2[b] + B.j2[h]) / 2 + 1 + J) & 1 ? -1.0 : 1.0;
+        acc += (2 * J + 1) * w0 * sign * B.occ[h] * (nrm * ph * two[B.ch_off[ch] + (long)k * n + i]);
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
   }
+  if (lane != 0) return;
+  double f = one[(long)b * norb + a] + acc;
   if (herm == -1 && a == b) f = 0.0;
   one[(long)b * norb + a] = f;
   if (a != b) one[(long)a * norb + b] = herm * f;
 }
 
-hipError_t launch_normal_order(const BasisDev *bd, const BasisDev &bh, double *op, int sign, int herm, bool add,
-                               hipStream_t s) {
-  hipLaunchKernelGGL(k_no_zero, dim3(1), dim3(64), 0, s, bd, op, sign);
+hipError_t launch_normal_order(const BasisDev *bd, const BasisDev &bh, double *op, int sign, int herm,
+                               const double *n2sq, hipStream_t s) {
+  hipLaunchKernelGGL(k_no_zero, dim3(1), dim3(256), 0, s, bd, op, sign);
   if (bh.n1pairs > 0)
-    hipLaunchKernelGGL(k_no_one, dim3((bh.n1pairs + 255) / 256), dim3(256), 0, s, bd, op, sign, herm, add ? 1 : 0);
+    hipLaunchKernelGGL(k_no_one, dim3((bh.n1pairs + 3) / 4), dim3(256), 0, s, bd, op, sign, herm, n2sq);
   return hipGetLastError();
 }
 

@@ -47,10 +47,11 @@ hipError_t launch_axpy(double *y, double a, const double *x, long n, hipStream_t
 hipError_t launch_scale(double *y, double a, long n, hipStream_t s);
 hipError_t launch_fill(double *y, double v, long n, hipStream_t s);  // y[i] = v
 // Operator::DoNormalOrdering2 / UndoNormalOrdering of a scalar operator in place
-// (sign +1 / -1); add: the one-body sum over the holes (off when the two-body norm
-// is below 1e-7), the (anti-)Hermitian completion of the one-body part always
-hipError_t launch_normal_order(const BasisDev *bd, const BasisDev &bh, double *op, int sign, int herm, bool add,
-                               hipStream_t s);
+// (sign +1 / -1); n2sq: the squared two-body norm on the device (launch_norm's second
+// value): the one-body sum over the holes runs when the norm exceeds 1e-7, the
+// (anti-)Hermitian completion of the one-body part always
+hipError_t launch_normal_order(const BasisDev *bd, const BasisDev &bh, double *op, int sign, int herm,
+                               const double *n2sq, hipStream_t s);
 // partial sums of squares; out[0] = one-body, out[1] = two-body (weighted)
 hipError_t launch_norm(const BasisDev *bd, const BasisDev &bh, const double *op, double *partials, int npart,
                        double *out, hipStream_t s);

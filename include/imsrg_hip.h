@@ -162,7 +162,7 @@ int imsrg_hip_op_norm(imsrg_dop *op, double *norm1, double *norm2, double *norm)
    (2J+1) n_k n_l V^J_klkl, f_ab += sign sum_J (2J+1)/(2j_a+1) sum_h n_h V~^J_ahbh (when the
    two-body norm exceeds 1e-7), the one-body part completed from its upper triangle; the
    two-body part is unchanged.  out may be op (in place); both on the same context.
-   Asynchronous on the context's stream after one host read of the two-body norm. */
+   Asynchronous on the context's stream. */
 int imsrg_hip_op_normal_order(imsrg_dop *out, const imsrg_dop *op, int sign);
 
 /* device-resident versions of the drop-ins (no PCIe traffic) */
