@@ -226,7 +226,7 @@ static int ctx_init(imsrg_ctx *ctx, const imsrg_basis_desc *b, int device) {
   HIPCHK(ctx, hipMalloc(&ctx->obpart, sizeof(double) * std::max(B.nobpart, 1)));
   HIPCHK(ctx, hipMalloc(&ctx->diag, sizeof(double) * std::max(B.ndiag, 1)));
   HIPCHK(ctx, hipMalloc(&ctx->c122w, sizeof(double) * (B.tbsize + 8)));
-  ctx->npart = 257;
+  ctx->npart = 2049;  // 2048 two-body chunks and the one-body block
   HIPCHK(ctx, hipMalloc(&ctx->partials, sizeof(double) * ctx->npart));
   HIPCHK(ctx, hipMalloc(&ctx->norm_dev, sizeof(double) * 2));
   HIPCHK(ctx, hipHostMalloc(&ctx->norm_host, sizeof(double) * 2, hipHostMallocDefault));

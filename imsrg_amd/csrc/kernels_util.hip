@@ -183,16 +183,22 @@ __global__ __launch_bounds__(256) void k_norm_partial(const BasisDev *bd, const 
     const double *one = op + B.one_off;
     for (long e = threadIdx.x; e < (long)B.norb * B.norb; e += blockDim.x) acc += one[e] * one[e] * B.w1norm[e];
   } else {
+    // block b takes the b-th contiguous chunk of the two-body arena and the
+    // channels it overlaps, each with its weight (2J+1)^2
     const double *two = op + B.two_off;
-    // channels are visited in order; block b takes channel-aligned strided elements
-    for (int ch = 0; ch < B.nch; ++ch) {
-      const long nn = (long)B.ch_n[ch] * B.ch_n[ch];
-      if (nn == 0) continue;
+    const long tb = B.tbsize, chunk = (tb + nblk2 - 1) / nblk2;
+    const long c0 = (long)blockIdx.x * chunk, c1 = min(c0 + chunk, tb);
+    int lo = 0, hi = B.nch - 1;  // last channel starting at or before c0
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (B.ch_off[mid] <= c0) lo = mid; else hi = mid - 1;
+    }
+    for (int ch = lo; ch < B.nch && B.ch_off[ch] < c1; ++ch) {
+      const long off = B.ch_off[ch], nn = (long)B.ch_n[ch] * B.ch_n[ch];
+      const long e0 = max(c0, off), e1 = min(c1, off + nn);
       const double w = (double)((2 * (ch / 6) + 1) * (2 * (ch / 6) + 1));
-      const double *m = two + B.ch_off[ch];
       double s = 0.0;
-      for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < nn; e += (long)nblk2 * blockDim.x)
-        s += m[e] * m[e];
+      for (long e = e0 + threadIdx.x; e < e1; e += blockDim.x) s += two[e] * two[e];
       acc += w * s;
     }
   }
