@@ -613,9 +613,15 @@ extern "C" int imsrg_hip_op_zero(imsrg_dop *op) {
   return IMSRG_OK;
 }
 
-static int op_norm(imsrg_dop *op, double *n1, double *n2, double *n) {
+// norm of op; with axpy_to, axpy_to += axpy_a * op in the same pass over op
+static int op_norm(imsrg_dop *op, double *n1, double *n2, double *n, imsrg_dop *axpy_to = nullptr,
+                   double axpy_a = 0.0) {
   imsrg_ctx *ctx = op->ctx;
-  HIPCHK(ctx, launch_norm(ctx->db.d, ctx->db.h, op->buf, ctx->partials, ctx->npart, ctx->norm_dev, ctx->stream));
+  if (axpy_to)
+    HIPCHK(ctx, launch_axpy_norm(ctx->db.d, axpy_to->buf, axpy_a, op->buf, ctx->partials, ctx->npart,
+                                 ctx->norm_dev, ctx->stream));
+  else
+    HIPCHK(ctx, launch_norm(ctx->db.d, ctx->db.h, op->buf, ctx->partials, ctx->npart, ctx->norm_dev, ctx->stream));
   HIPCHK(ctx, hipMemcpyAsync(ctx->norm_host, ctx->norm_dev, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   double a = std::sqrt(ctx->norm_host[0]), b = std::sqrt(ctx->norm_host[1]);
@@ -1024,10 +1030,9 @@ static int bch_transform(imsrg_ctx *ctx, const imsrg_dop *Op, const imsrg_dop *O
       nc++;
       std::swap(nested, next);
       fact /= i;
-      if ((rc = imsrg_hip_op_axpy(Out, fact, nested))) return rc;
       eps *= i + 1;
-      double nn;
-      if ((rc = op_norm(nested, nullptr, nullptr, &nn))) return rc;
+      double nn;  // Out += fact * nested, and the norm of nested, in one pass
+      if ((rc = op_norm(nested, nullptr, nullptr, &nn, Out, fact))) return rc;
       if ((o.mode == 0 && nn < eps) || (o.mode == 1 && nn < o.transform_thr)) {
         converged = true;
         break;

@@ -208,6 +208,61 @@ __global__ __launch_bounds__(256) void k_norm_partial(const BasisDev *bd, const 
   if (threadIdx.x == 0) part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
+// y += a x over the whole operator buffer, and the norm partial sums of x with the
+// blocks of k_norm_partial (the same sums in the same order): one pass over x in
+// place of an axpy and a norm (the BCH_Transform step, src/BCH.cc:176-191)
+__global__ __launch_bounds__(256) void k_axpy_norm(const BasisDev *bd, double *y, double a, const double *x,
+                                                   double *part, int nblk2) {
+  const BasisDev &B = *bd;
+  __shared__ double red[4];
+  double acc = 0.0;
+  if ((int)blockIdx.x == nblk2) {
+    const long n1 = (long)B.norb * B.norb;
+    // the zero-body and padding words around the one-body block and after the two-body arena
+    const long rg[3][2] = {{0, B.one_off}, {B.one_off + n1, B.two_off}, {B.two_off + B.tbsize, B.opsize}};
+    for (int g = 0; g < 3; ++g)
+      for (long e = rg[g][0] + threadIdx.x; e < rg[g][1]; e += blockDim.x) {
+        double t = a * x[e];
+        y[e] += t;
+      }
+    const double *one = x + B.one_off;
+    double *yo = y + B.one_off;
+    for (long e = threadIdx.x; e < n1; e += blockDim.x) {
+      const double v = one[e];
+      acc += v * v * B.w1norm[e];
+      double t = a * v;
+      yo[e] += t;
+    }
+  } else {
+    const double *two = x + B.two_off;
+    double *y2 = y + B.two_off;
+    const long tb = B.tbsize, chunk = (tb + nblk2 - 1) / nblk2;
+    const long c0 = (long)blockIdx.x * chunk, c1 = min(c0 + chunk, tb);
+    int lo = 0, hi = B.nch - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (B.ch_off[mid] <= c0) lo = mid; else hi = mid - 1;
+    }
+    for (int ch = lo; ch < B.nch && B.ch_off[ch] < c1; ++ch) {
+      const long off = B.ch_off[ch], nn = (long)B.ch_n[ch] * B.ch_n[ch];
+      const long e0 = max(c0, off), e1 = min(c1, off + nn);
+      const double w = (double)((2 * (ch / 6) + 1) * (2 * (ch / 6) + 1));
+      double s = 0.0;
+      for (long e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+        const double v = two[e];
+        s += v * v;
+        double t = a * v;
+        y2[e] += t;
+      }
+      acc += w * s;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
 __global__ void k_norm_final(const double *part, int nblk2, double *out) {
   __shared__ double red[256];
   double acc = 0.0;
@@ -306,6 +361,13 @@ hipError_t launch_normal_order(const BasisDev *bd, const BasisDev &bh, double *o
   return hipGetLastError();
 }
 
+hipError_t launch_axpy_norm(const BasisDev *bd, double *y, double a, const double *x, double *partials, int npart,
+                            double *out, hipStream_t s) {
+  const int nblk2 = npart - 1;
+  hipLaunchKernelGGL(k_axpy_norm, dim3(npart), dim3(256), 0, s, bd, y, a, x, partials, nblk2);
+  hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(256), 0, s, partials, nblk2, out);
+  return hipGetLastError();
+}
 hipError_t launch_norm(const BasisDev *bd, const BasisDev &bh, const double *op, double *partials, int npart,
                        double *out, hipStream_t s) {
   (void)bh;

@@ -53,6 +53,9 @@ hipError_t launch_fill(double *y, double v, long n, hipStream_t s);  // y[i] = v
 hipError_t launch_normal_order(const BasisDev *bd, const BasisDev &bh, double *op, int sign, int herm,
                                const double *n2sq, hipStream_t s);
 // partial sums of squares; out[0] = one-body, out[1] = two-body (weighted)
+// y += a x over an operator buffer, with the norm sums of x into out[0..1] (as launch_norm)
+hipError_t launch_axpy_norm(const BasisDev *bd, double *y, double a, const double *x, double *partials, int npart,
+                            double *out, hipStream_t s);
 hipError_t launch_norm(const BasisDev *bd, const BasisDev &bh, const double *op, double *partials, int npart,
                        double *out, hipStream_t s);
 hipError_t launch_fill_random(const BasisDev *bd, const BasisDev &bh, double *op, int herm,
