@@ -313,6 +313,63 @@ def workload_normal_order(args, ia, np):
     return line
 
 
+def workload_transform(args, ia, np):
+    """HartreeFock::TransformToHFBasis of a scalar Hermitian operator at emax=10 (seed
+    6101) with a basis change that is orthogonal and block diagonal in (l, j, tz)
+    (seed 6102, as a Hartree-Fock C): device time per call (C's upload, the channel
+    matrices D, and C^T O C plus D^T V D per channel as rocBLAS DGEMMs), its FP64
+    rate against the MFMA peak, and the oracle's time for the same call (its channel
+    products through the host BLAS, one thread)."""
+    emax = 10
+    ms = ia.ModelSpace(emax, "O16", device=0)
+    H = ia.DeviceOperator(ms, 1).fill_random(1, 6101)
+    rng = np.random.default_rng(6102)
+    Cm = np.eye(ms.norb)
+    blocks = {}
+    for i, o in enumerate(ms.orbits):
+        blocks.setdefault((o["l"], o["j2"], o["tz2"]), []).append(i)
+    for blk in blocks.values():
+        Q, R = np.linalg.qr(rng.normal(size=(len(blk), len(blk))))
+        Cm[np.ix_(blk, blk)] = Q * np.sign(np.diag(R))
+    Cm = np.asfortranarray(Cm)
+    out = ia.DeviceOperator(ms, 1)
+    for _ in range(3):
+        H.TransformToHFBasis(Cm, out=out)
+    reps = max(args.steps, 10)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        H.TransformToHFBasis(Cm, out=out)
+    sec = (time.perf_counter() - t0) / reps
+    flops = 4.0 * sum(c["n"] ** 3 for c in ms.channels) + 4.0 * ms.norb ** 3
+    got = out.to_host()
+    line = {"metric": f"basis transformation (HartreeFock::TransformToHFBasis) of a scalar operator at emax={emax}",
+            "value": sec * 1e6, "unit": "us/call", "n_gpus": 1, "steps": reps, "higher_is_better": False,
+            "dtype": "f64", "data": "synthetic (seeded operator and orthogonal block-diagonal C)",
+            "config": {"workload": f"TransformToHFBasis, HO emax={emax}, O16 reference, {len(ms.orbits)} orbits"},
+            "roofline": {"bound": "mfma", "achieved": flops / sec / 1e12, "peak": 78.6, "unit": "TFLOP/s",
+                         "frac": flops / sec / 1e12 / 78.6, "traffic": None,
+                         "note": "whole call (upload, D, 2 DGEMMs per channel) against the FP64 MFMA peak; "
+                                 f"{flops / 1e9:.1f} GFLOP of products per call"}}
+    if not args.no_cpu_baseline:
+        from oracle import pyoracle
+        blas = pyoracle.use_blas(True)
+        orc = pyoracle.OracleModelSpace(emax, 16, 8)
+        Hh = H.to_host()
+        op = {"zero": float(Hh.ZeroBody), "one": np.asarray(Hh.OneBody), "two": np.asarray(Hh.TwoBody), "herm": 1}
+        t0 = time.perf_counter()
+        ref = orc.transform_basis(op, Cm)
+        cpu = time.perf_counter() - t0
+        pyoracle.use_blas(False)
+        line["cpu_baseline"] = {"value": cpu * 1e6, "unit": "us/call", "cores": 1, "kind": "port",
+                                "sample": f"one call of the oracle's restatement, channel products through {blas} "
+                                          "on one thread"}
+        two = np.asarray(got.TwoBody)
+        line["gpu_vs_cpu_rel_maxabs"] = float(max(
+            np.max(np.abs(two - ref["two"])) / np.max(np.abs(ref["two"])),
+            np.max(np.abs(np.asarray(got.OneBody) - ref["one"])) / np.max(np.abs(ref["one"]))))
+    return line
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -323,10 +380,10 @@ def main():
     ap.add_argument("--autotune-cache", default=None,
                     help="text cache of the autotuner: settings found there are applied without timing trials")
     ap.add_argument("--workload", default="commutator",
-                    choices=["commutator", "batched", "emax12", "emax14", "flow", "normal_order"],
+                    choices=["commutator", "batched", "emax12", "emax14", "flow", "normal_order", "transform"],
                     help="commutator: the headline metric line; batched: 48 independent pairs at emax 4-7; "
                          "emax12 / emax14: precision report at emax=12 / 14; flow: Magnus flow at emax=10; "
-                         "normal_order: DoNormalOrdering at emax=10")
+                         "normal_order: DoNormalOrdering at emax=10; transform: TransformToHFBasis at emax=10")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X; runs on more than one GPU are not part of this build "
@@ -340,7 +397,7 @@ def main():
     if args.workload != "commutator":
         line = {"batched": workload_batched, "emax12": workload_emax12,
                 "emax14": lambda a, i, n: workload_emax12(a, i, n, emax=14), "flow": workload_flow,
-                "normal_order": workload_normal_order}[args.workload](
+                "normal_order": workload_normal_order, "transform": workload_transform}[args.workload](
             args, ia, np)
         print(json.dumps(line), flush=True)
         return

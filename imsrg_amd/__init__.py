@@ -260,6 +260,18 @@ class DeviceOperator:
         check(lib().imsrg_hip_op_normal_order(out.h, self.h, -1), self.ms.h)
         return out
 
+    def TransformToHFBasis(self, C, out=None):
+        """HartreeFock::TransformToHFBasis (src/HartreeFock.cc:905-1000) with the basis
+        change C (norb x norb, C[i, a] = <i|a>): a new operator (or `out`, which may be
+        self) with OneBody' = C^T OneBody C and V'^J = D^T V^J D per channel."""
+        norb = self.ms.norb
+        Cf = np.asfortranarray(np.asarray(C, dtype=np.float64))
+        if Cf.shape != (norb, norb):
+            raise ValueError(f"C must be {norb} x {norb}, got {Cf.shape}")
+        out = out if out is not None else DeviceOperator(self.ms, self.herm)
+        check(lib().imsrg_hip_op_transform_basis(out.h, self.h, Cf.ctypes.data), self.ms.h)
+        return out
+
     def WriteBinary(self, path):
         """Operator::WriteBinary (src/Operator.cc:341-362): the reference's byte layout."""
         check(lib().imsrg_hip_op_write_binary(self.h, str(path).encode()), self.ms.h)

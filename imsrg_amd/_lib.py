@@ -100,6 +100,7 @@ SIGNATURES = {
     "imsrg_hip_op_zero": (C.c_int, [_vp]),
     "imsrg_hip_op_norm": (C.c_int, [_vp, _dp, _dp, _dp]),
     "imsrg_hip_op_normal_order": (C.c_int, [_vp, _vp, C.c_int]),
+    "imsrg_hip_op_transform_basis": (C.c_int, [_vp, _vp, _vp]),
     "imsrg_hip_commutator_ss_dev": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(CommOpts)]),
     "imsrg_hip_commutator_ss_dev_async": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(CommOpts), _vp]),
     "imsrg_hip_commutator_ss_dev_noalloc": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(CommOpts)]),

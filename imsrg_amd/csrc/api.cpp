@@ -3,6 +3,7 @@
 #include "../../include/imsrg_hip.h"
 
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <array>
@@ -19,6 +20,7 @@
 #include <string>
 #include <vector>
 
+#include <dlfcn.h>
 #include <unistd.h>
 
 #include "basis.hpp"
@@ -147,6 +149,11 @@ struct imsrg_ctx {
   double *panel = nullptr;   // bf16x3 operand panels of the mixed pp/hh GEMM (allocated on first use)
   double *c122w = nullptr;   // comm122 W in the two-body layout (tbsize + 8)
   double *partials = nullptr;
+  // basis transformation: channel matrices D, one channel's product, C and a
+  // one-body product (allocated on first use), and the rocBLAS handle
+  double *xform = nullptr;
+  long xform_n = 0;
+  rocblas_handle blas = nullptr;
   double *norm_dev = nullptr;
   double *norm_host = nullptr;  // pinned
   int npart = 0;
@@ -189,6 +196,38 @@ static std::string g_err;
 extern "C" const char *imsrg_hip_last_error(const imsrg_ctx *ctx) {
   return ctx ? ctx->err.c_str() : g_err.c_str();
 }
+
+// ---------------------------------------------------------------------------
+// rocBLAS, opened on the first basis transformation (nothing else needs it): the
+// channel products of imsrg_hip_op_transform_basis are plain DGEMMs.
+namespace {
+struct BlasLib {
+  decltype(&rocblas_create_handle) create = nullptr;
+  decltype(&rocblas_destroy_handle) destroy = nullptr;
+  decltype(&rocblas_set_stream) set_stream = nullptr;
+  decltype(&rocblas_dgemm) dgemm = nullptr;
+  std::string err;
+};
+const BlasLib &blas_lib() {
+  static const BlasLib lib = [] {
+    BlasLib b;
+    void *h = dlopen("librocblas.so.5", RTLD_NOW | RTLD_LOCAL);
+    if (!h) h = dlopen("librocblas.so", RTLD_NOW | RTLD_LOCAL);
+    if (!h) {
+      const char *m = dlerror();
+      b.err = std::string("cannot load rocBLAS: ") + (m ? m : "dlopen failed");
+      return b;
+    }
+    b.create = reinterpret_cast<decltype(b.create)>(dlsym(h, "rocblas_create_handle"));
+    b.destroy = reinterpret_cast<decltype(b.destroy)>(dlsym(h, "rocblas_destroy_handle"));
+    b.set_stream = reinterpret_cast<decltype(b.set_stream)>(dlsym(h, "rocblas_set_stream"));
+    b.dgemm = reinterpret_cast<decltype(b.dgemm)>(dlsym(h, "rocblas_dgemm"));
+    if (!b.create || !b.destroy || !b.set_stream || !b.dgemm) b.err = "rocBLAS lacks a needed symbol";
+    return b;
+  }();
+  return lib;
+}
+}  // namespace
 
 // ---------------------------------------------------------------------------
 static int ctx_init(imsrg_ctx *ctx, const imsrg_basis_desc *b, int device) {
@@ -286,6 +325,8 @@ extern "C" int imsrg_hip_ctx_destroy(imsrg_ctx *ctx) {
   if (ctx->panel) (void)hipFree(ctx->panel);
   if (ctx->c122w) (void)hipFree(ctx->c122w);
   if (ctx->partials) (void)hipFree(ctx->partials);
+  if (ctx->xform) (void)hipFree(ctx->xform);
+  if (ctx->blas) blas_lib().destroy(ctx->blas);
   if (ctx->norm_dev) (void)hipFree(ctx->norm_dev);
   if (ctx->norm_host) (void)hipHostFree(ctx->norm_host);
   for (int i = 0; i < 2; ++i) {
@@ -653,6 +694,67 @@ extern "C" int imsrg_hip_op_normal_order(imsrg_dop *out, const imsrg_dop *op, in
                                             hipMemcpyDeviceToDevice, ctx->stream));
   out->herm = op->herm;
   HIPCHK(ctx, launch_normal_order(ctx->db.d, ctx->db.h, out->buf, sign, op->herm, ctx->norm_dev, ctx->stream));
+  return IMSRG_OK;
+}
+
+// HartreeFock::TransformToHFBasis of a scalar operator (src/HartreeFock.cc:905-1000):
+// OneBody' = C^T OneBody C and, per channel, V' = D^T V D with the matrices D of
+// k_hf_D; the zero-body part and Hermiticity carry over.  C: host, norb x norb,
+// column-major.  Synchronous: returns when out holds the result.
+extern "C" int imsrg_hip_op_transform_basis(imsrg_dop *out, const imsrg_dop *op, const double *C) {
+  if (!out || !op || !C) return IMSRG_E_INVALID;
+  if (!same_ctx(out, op)) return IMSRG_E_INVALID;
+  imsrg_ctx *ctx = out->ctx;
+  const BlasLib &bl = blas_lib();
+  if (!bl.err.empty()) return fail(ctx, IMSRG_E_HIP, bl.err);
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const BasisDev &B = ctx->db.h;
+  const long n1 = (long)B.norb * B.norb;
+  const long need = B.tbsize + (long)B.maxn * B.maxn + 2 * n1;
+  if (ctx->xform_n < need) {
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->xform) HIPCHK(ctx, hipFree(ctx->xform));
+    ctx->xform = nullptr;
+    ctx->xform_n = 0;
+    HIPCHK(ctx, hipMalloc(&ctx->xform, sizeof(double) * need));
+    ctx->xform_n = need;
+  }
+  if (!ctx->blas) {
+    rocblas_handle h = nullptr;
+    if (bl.create(&h) != rocblas_status_success) return fail(ctx, IMSRG_E_HIP, "rocblas_create_handle failed");
+    ctx->blas = h;
+  }
+  if (bl.set_stream(ctx->blas, ctx->stream) != rocblas_status_success)
+    return fail(ctx, IMSRG_E_HIP, "rocblas_set_stream failed");
+  double *D = ctx->xform, *T = D + B.tbsize, *Cd = T + (long)B.maxn * B.maxn, *T1 = Cd + n1;
+  HIPCHK(ctx, hipMemcpyAsync(Cd, C, sizeof(double) * n1, hipMemcpyHostToDevice, ctx->stream));
+  if (out != op) HIPCHK(ctx, hipMemcpyAsync(out->buf, op->buf, sizeof(double) * B.opsize,
+                                            hipMemcpyDeviceToDevice, ctx->stream));
+  out->herm = op->herm;
+  HIPCHK(ctx, launch_hf_D(ctx->db.d, B, Cd, D, ctx->stream));
+  const double one = 1.0, zero = 0.0;
+  const auto N = rocblas_operation_none, Tr = rocblas_operation_transpose;
+  // one-body: T1 = O C, then O' = C^T T1
+  const int no = B.norb;
+  if (no > 0) {
+    if (bl.dgemm(ctx->blas, N, N, no, no, no, &one, op->buf + B.one_off, no, Cd, no, &zero, T1, no) !=
+            rocblas_status_success ||
+        bl.dgemm(ctx->blas, Tr, N, no, no, no, &one, Cd, no, T1, no, &zero, out->buf + B.one_off, no) !=
+            rocblas_status_success)
+      return fail(ctx, IMSRG_E_HIP, "rocblas_dgemm failed (one-body)");
+  }
+  // two-body, channel by channel: T = V D, then V' = D^T T
+  for (int c = 0; c < ctx->hb.nch; ++c) {
+    const int n = (int)ctx->hb.ch[c].kets.size();
+    if (n == 0) continue;
+    const long off = ctx->hb.ch_off[c];
+    if (bl.dgemm(ctx->blas, N, N, n, n, n, &one, op->buf + B.two_off + off, n, D + off, n, &zero, T, n) !=
+            rocblas_status_success ||
+        bl.dgemm(ctx->blas, Tr, N, n, n, n, &one, D + off, n, T, n, &zero, out->buf + B.two_off + off, n) !=
+            rocblas_status_success)
+      return fail(ctx, IMSRG_E_HIP, "rocblas_dgemm failed (channel " + std::to_string(c) + ")");
+  }
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return IMSRG_OK;
 }
 

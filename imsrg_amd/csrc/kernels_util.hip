@@ -378,6 +378,49 @@ hipError_t launch_norm(const BasisDev *bd, const BasisDev &bh, const double *op,
 }
 
 // ---------------------------------------------------------------------------
+// The two-body basis-change matrix of HartreeFock::TransformToHFBasis
+// (src/HartreeFock.cc:942-960), for every channel at once, in the two-body layout:
+// element (i, a) of channel ch's block is
+//   D(i, a) = C(p, p') C(q, q') + [p != q] Phase_pq(J) C(q, p') C(p, q'),
+// times sqrt2 if p == q, divided by sqrt2 if p' == q', for the channel's kets
+// i = (p, q) (old basis, rows) and a = (p', q') (new basis, columns);
+// Phase_pq(J) = (-1)^{(j_p + j_q)/2 + 1 + J} (src/ModelSpace.hh:72-78).
+// C is norb x norb column-major, C(p, p') = <p|p'>.  One thread per element.
+__global__ __launch_bounds__(256) void k_hf_D(const BasisDev *bd, const double *C, double *D) {
+  const BasisDev &B = *bd;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= B.tbsize) return;
+  int lo = 0, hi = B.nch - 1;  // the last channel starting at or before e holds it
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (B.ch_off[mid] <= e) lo = mid; else hi = mid - 1;
+  }
+  const int ch = lo, n = B.ch_n[ch], J = ch / 6;
+  const long r = e - B.ch_off[ch];
+  const int i = (int)(r % n), a = (int)(r / n);
+  const int ki = B.ch_kets[B.ch_kstart[ch] + i], ka = B.ch_kets[B.ch_kstart[ch] + a];
+  const int p = B.kp[ki], q = B.kq[ki], pp = B.kp[ka], qq = B.kq[ka];
+  const long N = B.norb;
+  double d = C[p + pp * N] * C[q + qq * N];
+  if (p != q) {
+    const double ph = (((B.j2[p] + B.j2[q]) / 2 + 1 + J) & 1) ? -1.0 : 1.0;
+    double x = C[q + pp * N] * C[p + qq * N];
+    d += ph * x;
+  }
+  if (p == q) d *= 1.4142135623730951;
+  if (pp == qq) d /= 1.4142135623730951;
+  D[e] = d;
+}
+
+hipError_t launch_hf_D(const BasisDev *bd, const BasisDev &bh, const double *C, double *D, hipStream_t s) {
+  if (bh.tbsize > 0) {
+    const long nb = (bh.tbsize + 255) / 256;
+    hipLaunchKernelGGL(k_hf_D, dim3((unsigned)nb), dim3(256), 0, s, bd, C, D);
+  }
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Counter-based N(0,1): SplitMix64 of (seed, index) -> two uniforms -> Box-Muller.
 __device__ __forceinline__ unsigned long long splitmix64(unsigned long long x) {
   x += 0x9E3779B97F4A7C15ull;

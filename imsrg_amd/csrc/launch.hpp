@@ -52,12 +52,15 @@ hipError_t launch_fill(double *y, double v, long n, hipStream_t s);  // y[i] = v
 // (anti-)Hermitian completion of the one-body part always
 hipError_t launch_normal_order(const BasisDev *bd, const BasisDev &bh, double *op, int sign, int herm,
                                const double *n2sq, hipStream_t s);
-// partial sums of squares; out[0] = one-body, out[1] = two-body (weighted)
 // y += a x over an operator buffer, with the norm sums of x into out[0..1] (as launch_norm)
 hipError_t launch_axpy_norm(const BasisDev *bd, double *y, double a, const double *x, double *partials, int npart,
                             double *out, hipStream_t s);
+// partial sums of squares; out[0] = one-body, out[1] = two-body (weighted)
 hipError_t launch_norm(const BasisDev *bd, const BasisDev &bh, const double *op, double *partials, int npart,
                        double *out, hipStream_t s);
+// the basis-change matrices D of every channel (two-body layout, tbsize doubles) from
+// the one-body basis change C (norb x norb, column-major)
+hipError_t launch_hf_D(const BasisDev *bd, const BasisDev &bh, const double *C, double *D, hipStream_t s);
 hipError_t launch_fill_random(const BasisDev *bd, const BasisDev &bh, double *op, int herm,
                               unsigned long long seed, hipStream_t s);
 hipError_t launch_white(const BasisDev *bd, const BasisDev &bh, const double *H, double *Eta, const int4 *gen_items,

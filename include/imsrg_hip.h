@@ -164,6 +164,16 @@ int imsrg_hip_op_norm(imsrg_dop *op, double *norm1, double *norm2, double *norm)
    two-body part is unchanged.  out may be op (in place); both on the same context.
    Asynchronous on the context's stream. */
 int imsrg_hip_op_normal_order(imsrg_dop *out, const imsrg_dop *op, int sign);
+/* HartreeFock::TransformToHFBasis of a scalar two-body operator (src/HartreeFock.cc:905-1000):
+   OneBody' = C^T OneBody C, and per (J, parity, Tz) channel V' = D^T V D with
+   D(pq, p'q') = C(p,p') C(q,q') + [p != q] (-1)^{(j_p+j_q)/2+1+J} C(q,p') C(p,q'),
+   times sqrt2 if p == q, divided by sqrt2 if p' == q' (kets p <= q of the old basis,
+   p' <= q' of the new); the zero-body part and the Hermiticity are unchanged.
+   C: host memory, norb x norb, column-major, C(i,a) = <i|a> (HartreeFock::C, block
+   diagonal in (l, j, tz) for an HF basis).  The channel products run as DGEMMs through
+   rocBLAS, loaded on the first call.  out may be op; both on the same context.
+   Synchronous: returns when out holds the result. */
+int imsrg_hip_op_transform_basis(imsrg_dop *out, const imsrg_dop *op, const double *C);
 
 /* device-resident versions of the drop-ins (no PCIe traffic) */
 int imsrg_hip_commutator_ss_dev(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y, imsrg_dop *Z,

@@ -64,7 +64,8 @@ static inline int phase_of(int x) { return (x % 2 == 0) ? 1 : -1; }
  * (2J+1)^2; 6: one-body norm without the (2j+1) weight; 7: Omega not cleared at
  * a split; 8: BCH_Transform of H(s) in place of H0 / H_saved; 9: BCH_Product
  * without its 1/12 [[Y,X],X] term; 10: normal-ordering one-body sum without the
- * 1/(2j_a+1) factor. */
+ * 1/(2j_a+1) factor; 11: basis transformation without the 1/sqrt2 for new-basis
+ * kets with p' == q'. */
 static int g_fault = 0;
 void orc_set_fault(int f) { g_fault = f; }
 
@@ -1209,6 +1210,78 @@ void orc_normal_order(const oms_t *ms, const orc_op *in, orc_op *out, int sign) 
   free(f);
   if (out->two != in->two) memcpy(out->two, in->two, sizeof(double) * (size_t)ms->tbsize);
   *out->zero = e0;
+  out->herm = in->herm;
+}
+
+/* HartreeFock::TransformToHFBasis of a scalar two-body operator,
+ * src/HartreeFock.cc:905-1000: OneBody' = C^T OneBody C (:920-923) and, for each
+ * channel, V' = Dbra V Dket with Dbra = Dket^T (:933-989), where for the kets
+ * i = (p, q) of the old basis and a = (p', q') of the new (both p <= q)
+ *   Dket(i, a) = C(p,p') C(q,q') + [p != q] Phase_pq(J) C(q,p') C(p,q')  (:948-953)
+ * times sqrt2 if p == q and divided by sqrt2 if p' == q' (:954-955).  C is norb x
+ * norb column-major, C(i, a) = <i|a>.  out may alias in. */
+void orc_transform_basis(const oms_t *ms, const orc_op *in, const double *C, orc_op *out) {
+  const int norb = ms->norb;
+  const long n1 = (long)norb * norb;
+  double *t1 = (double *)calloc((size_t)n1, sizeof(double));
+  double *o1 = (double *)calloc((size_t)n1, sizeof(double));
+  for (int b = 0; b < norb; ++b)  /* t1 = OneBody C */
+    for (int k = 0; k < norb; ++k) {
+      const double c = C[(long)b * norb + k];
+      for (int a = 0; a < norb; ++a) t1[(long)b * norb + a] += in->one[(long)k * norb + a] * c;
+    }
+  for (int b = 0; b < norb; ++b)  /* o1 = C^T t1 */
+    for (int a = 0; a < norb; ++a) {
+      double acc = 0.0;
+      for (int k = 0; k < norb; ++k) acc += C[(long)a * norb + k] * t1[(long)b * norb + k];
+      o1[(long)b * norb + a] = acc;
+    }
+  memcpy(out->one, o1, sizeof(double) * (size_t)n1);
+  free(t1);
+  free(o1);
+  for (int c = 0; c < ms->nch; ++c) {
+    const chan_t *ch = &ms->ch[c];
+    const int n = ch->n, J = ch->J;
+    if (n == 0) continue;
+    const long nn = (long)n * n;
+    double *D = (double *)malloc(sizeof(double) * (size_t)nn);
+    double *T = (double *)calloc((size_t)nn, sizeof(double));
+    for (int a = 0; a < n; ++a) {
+      const int pp = ms->kp[ch->kets[a]], qq = ms->kq[ch->kets[a]];
+      for (int i = 0; i < n; ++i) {
+        const int p = ms->kp[ch->kets[i]], q = ms->kq[ch->kets[i]];
+        double d = C[(long)pp * norb + p] * C[(long)qq * norb + q];
+        if (p != q) d += C[(long)pp * norb + q] * C[(long)qq * norb + p] * ket_phase(ms, p, q, J);
+        if (p == q) d *= SQRT2;
+        if (pp == qq && g_fault != 11) d /= SQRT2;
+        D[(long)a * n + i] = d;
+      }
+    }
+    const double *V = in->two + ch->off;
+    double *W = out->two + ch->off;
+    if (g_dgemm) {  /* T = V D; W = D^T T */
+      const double one = 1.0, zero = 0.0;
+      g_dgemm("N", "N", &n, &n, &n, &one, V, &n, D, &n, &zero, T, &n);
+      g_dgemm("T", "N", &n, &n, &n, &one, D, &n, T, &n, &zero, W, &n);
+    } else {
+#pragma omp parallel for schedule(static)
+      for (int b = 0; b < n; ++b)
+        for (int k = 0; k < n; ++k) {
+          const double dk = D[(long)b * n + k];
+          for (int a = 0; a < n; ++a) T[(long)b * n + a] += V[(long)k * n + a] * dk;
+        }
+#pragma omp parallel for schedule(static)
+      for (int b = 0; b < n; ++b)
+        for (int a = 0; a < n; ++a) {
+          double acc = 0.0;
+          for (int k = 0; k < n; ++k) acc += D[(long)a * n + k] * T[(long)b * n + k];
+          W[(long)b * n + a] = acc;
+        }
+    }
+    free(D);
+    free(T);
+  }
+  if (out != in) *out->zero = *in->zero;
   out->herm = in->herm;
 }
 

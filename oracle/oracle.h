@@ -81,6 +81,11 @@ double orc_twobody_norm(const oms_t *ms, const orc_op *op);
  * out may alias in */
 void orc_normal_order(const oms_t *ms, const orc_op *in, orc_op *out, int sign);
 
+/* HartreeFock::TransformToHFBasis of a scalar operator (reference
+ * src/HartreeFock.cc:905-1000) with the basis change C (norb x norb, column-major,
+ * C(i,a) = <i|a>); out may alias in */
+void orc_transform_basis(const oms_t *ms, const orc_op *in, const double *C, orc_op *out);
+
 /* BCH (reference src/BCH.cc:87-201, 267-350).
  * mode 0: reference epsilon rule with threshold thr (default 1e-9);
  * mode 1: stop when ||nested|| < thr (absolute).

@@ -79,6 +79,7 @@ def lib():
         L.orc_bch_product.argtypes = [C.c_void_p, C.POINTER(_Op), C.POINTER(_Op), C.POINTER(_Op), C.c_double]
         L.orc_white_generator.argtypes = [C.c_void_p, C.POINTER(_Op), C.POINTER(_Op)]
         L.orc_normal_order.argtypes = [C.c_void_p, C.POINTER(_Op), C.POINTER(_Op), C.c_int]
+        L.orc_transform_basis.argtypes = [C.c_void_p, C.POINTER(_Op), _dp, C.POINTER(_Op)]
         L.orc_magnus_flow.restype = C.c_int
         L.orc_magnus_flow.argtypes = [C.c_void_p, C.POINTER(_Op), C.POINTER(_Op), C.c_double, C.c_double,
                                       C.c_double, C.c_double, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp,
@@ -263,6 +264,13 @@ class OracleModelSpace:
         """DoNormalOrdering (sign +1) / UndoNormalOrdering (sign -1) of a scalar operator."""
         a, o = _Holder(op), _Holder(self.zero_op(op["herm"]))
         lib().orc_normal_order(self.h, C.byref(a.s), C.byref(o.s), int(sign))
+        return o.op()
+
+    def transform_basis(self, op, Cmat):
+        """HartreeFock::TransformToHFBasis of a scalar operator; Cmat[i, a] = <i|a>."""
+        a, o = _Holder(op), _Holder(self.zero_op(op["herm"]))
+        Cf = np.asfortranarray(np.asarray(Cmat, dtype=np.float64))
+        lib().orc_transform_basis(self.h, C.byref(a.s), Cf.ctypes.data_as(_dp), C.byref(o.s))
         return o.op()
 
     def white_generator(self, H):

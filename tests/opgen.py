@@ -36,6 +36,30 @@ def random_op(ms, herm, seed, scale1=1.0, scale2=1.0):
     return {"zero": 0.0, "one": one, "two": two, "herm": herm}
 
 
+def basis_change(orbits, seed, kind="orthogonal", only_unoccupied=False):
+    """A norb x norb basis change C (C[i, a] = <i|a>), block diagonal in (l, j2, tz2)
+    like a Hartree-Fock basis.  kind "orthogonal": Q of the QR decomposition of a
+    normal matrix per block; "general": 1 + 0.3 N(0,1) per block (not orthogonal).
+    only_unoccupied: rotate only the orbits with occupation 0 of each block (the
+    reference state is then unchanged)."""
+    rng = np.random.default_rng(seed)
+    norb = len(orbits)
+    Cm = np.eye(norb)
+    for blk in one_body_blocks(orbits):
+        if only_unoccupied:
+            blk = [i for i in blk if orbits[i]["occ"] == 0.0]
+        m = len(blk)
+        if m == 0:
+            continue
+        if kind == "orthogonal":
+            Q, R = np.linalg.qr(rng.normal(size=(m, m)))
+            M = Q * np.sign(np.diag(R))
+        else:
+            M = np.eye(m) + 0.3 * rng.normal(size=(m, m))
+        Cm[np.ix_(blk, blk)] = M
+    return Cm
+
+
 def synthetic_hamiltonian(ms, seed, offdiag_sigma=0.5):
     """Flow Hamiltonian: one-body diagonal 10*(2n+l+1.5) plus symmetric
     N(0, sigma) off-diagonals inside each (l, j2, tz2) block; two-body
