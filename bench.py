@@ -370,6 +370,69 @@ def workload_transform(args, ia, np):
     return line
 
 
+def workload_hf(args, ia, np):
+    """HartreeFock::Solve + GetNormalOrderedH at emax=10 for the synthetic flow
+    Hamiltonian (one-body 10(2n+l+3/2) plus N(0, 0.5) inside each (l, j, tz) block,
+    two-body (R+R^T)/sqrt(2n), seed 7300): device monopoles, host iteration with the
+    reference's density mixing, then the normal-ordered Hamiltonian in the HF basis.
+    Beside it the oracle's restatement of Solve (numpy, its monopoles through the
+    oracle's C GetTBMEmonopole) on one host core."""
+    emax = 10
+    ms = ia.ModelSpace(emax, "O16", device=0)
+    rng = np.random.default_rng(7300)
+    norb = ms.norb
+    one = np.zeros((norb, norb), order="F")
+    blocks = {}
+    for i, o in enumerate(ms.orbits):
+        blocks.setdefault((o["l"], o["j2"], o["tz2"]), []).append(i)
+    for blk in blocks.values():
+        for a in blk:
+            o = ms.orbits[a]
+            one[a, a] = 10.0 * (2 * o["n"] + o["l"] + 1.5)
+        for x in range(len(blk)):
+            for y in range(x + 1, len(blk)):
+                one[blk[x], blk[y]] = one[blk[y], blk[x]] = rng.normal(0.0, 0.5)
+    two = np.zeros(ms.tbsize)
+    for c in ms.channels:
+        n = c["n"]
+        if n:
+            R = rng.normal(size=(n, n)) / np.sqrt(n)
+            two[c["off"]:c["off"] + n * n] = ((R + R.T) / np.sqrt(2.0)).reshape(-1, order="F")
+    H = ia.Operator(ms, 1, 0.0, one, two).to_device()
+    out = ia.DeviceOperator(ms, 1)
+    hf = ia.HartreeFock(H).Solve()
+    hf.GetNormalOrderedH(out=out)
+    reps = max(min(args.steps, 10), 3)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        hf = ia.HartreeFock(H).Solve()
+    t_solve = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        hf.GetNormalOrderedH(out=out)
+    t_no = (time.perf_counter() - t0) / reps
+    line = {"metric": f"Hartree-Fock (HartreeFock::Solve + GetNormalOrderedH) at emax={emax}",
+            "value": (t_solve + t_no) * 1e3, "unit": "ms/call", "n_gpus": 1, "steps": reps,
+            "higher_is_better": False, "dtype": "f64", "data": "synthetic Hamiltonian (seed 7300)",
+            "config": {"workload": f"HF with frozen occupations, HO emax={emax}, O16 reference, {norb} orbits"},
+            "solve_ms": t_solve * 1e3, "normal_ordered_h_ms": t_no * 1e3, "iterations": hf.iterations,
+            "converged": hf.converged, "EHF": hf.EHF}
+    if not args.no_cpu_baseline:
+        from oracle.pyoracle import OracleModelSpace
+        orc = OracleModelSpace(emax, 16, 8)
+        op = {"zero": 0.0, "one": one, "two": two, "herm": 1}
+        t0 = time.perf_counter()
+        ref = orc.hf_solve(op)
+        cpu = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": cpu * 1e3, "unit": "ms/call (Solve only)", "cores": 1, "kind": "port",
+                                "sample": "one call of the oracle's Solve (numpy and Python loops, monopoles "
+                                          "through its C GetTBMEmonopole); not a tuned CPU code"}
+        line["gpu_vs_cpu_dEHF"] = float(abs(hf.EHF - ref["EHF"]))
+        line["gpu_vs_cpu_C_maxabs"] = float(np.max(np.abs(hf.C - ref["C"])))
+        line["same_iterations"] = hf.iterations == ref["iterations"]
+    return line
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -380,10 +443,11 @@ def main():
     ap.add_argument("--autotune-cache", default=None,
                     help="text cache of the autotuner: settings found there are applied without timing trials")
     ap.add_argument("--workload", default="commutator",
-                    choices=["commutator", "batched", "emax12", "emax14", "flow", "normal_order", "transform"],
+                    choices=["commutator", "batched", "emax12", "emax14", "flow", "normal_order", "transform", "hf"],
                     help="commutator: the headline metric line; batched: 48 independent pairs at emax 4-7; "
                          "emax12 / emax14: precision report at emax=12 / 14; flow: Magnus flow at emax=10; "
-                         "normal_order: DoNormalOrdering at emax=10; transform: TransformToHFBasis at emax=10")
+                         "normal_order: DoNormalOrdering at emax=10; transform: TransformToHFBasis at emax=10; "
+                         "hf: HartreeFock Solve + GetNormalOrderedH at emax=10")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X; runs on more than one GPU are not part of this build "
@@ -397,7 +461,8 @@ def main():
     if args.workload != "commutator":
         line = {"batched": workload_batched, "emax12": workload_emax12,
                 "emax14": lambda a, i, n: workload_emax12(a, i, n, emax=14), "flow": workload_flow,
-                "normal_order": workload_normal_order, "transform": workload_transform}[args.workload](
+                "normal_order": workload_normal_order, "transform": workload_transform,
+                "hf": workload_hf}[args.workload](
             args, ia, np)
         print(json.dumps(line), flush=True)
         return

@@ -22,7 +22,8 @@ import numpy as np
 from . import _lib
 from ._lib import ALL_TERMS, PREC_FP64, PREC_MIXED, PREC_SPLIT, TERMS, HipError, check, lib
 
-__all__ = ["ModelSpace", "Operator", "DeviceOperator", "Commutator", "BCH", "IMSRGSolver", "TERMS", "ALL_TERMS",
+__all__ = ["ModelSpace", "Operator", "DeviceOperator", "Commutator", "BCH", "IMSRGSolver", "HartreeFock", "TERMS",
+           "ALL_TERMS",
            "PREC_FP64", "PREC_MIXED", "PREC_SPLIT", "HipError"]
 
 _ELEM = {"H": 1, "He": 2, "Li": 3, "Be": 4, "B": 5, "C": 6, "N": 7, "O": 8, "F": 9, "Ne": 10, "Na": 11, "Mg": 12,
@@ -471,6 +472,55 @@ class _Solver:
             self.close()
         except Exception:
             pass
+
+
+class HartreeFock:
+    """HartreeFock (src/HartreeFock.hh, src/HartreeFock.cc) for a two-body Hermitian
+    Hamiltonian (a DeviceOperator) on its model space's reference, occupations frozen:
+    Solve() (monopoles on the device, the iteration of src/HartreeFock.cc:55-167 on
+    the host, up to the reference's switch to DIIS at 100 iterations), then C
+    (C[i, a] = <i|a>), energies, F, EHF, e1hf, e2hf, iterations;
+    TransformToHFBasis(op) and GetNormalOrderedH()."""
+
+    def __init__(self, Hbare):
+        self.Hbare = Hbare
+        self.ms = Hbare.ms
+        n = self.ms.norb
+        self.C = np.eye(n, order="F")
+        self.energies = np.zeros(n)
+        self.F = np.zeros((n, n), order="F")
+        self.EHF = self.e1hf = self.e2hf = 0.0
+        self.iterations = 0
+        self.tolerance = 1e-8
+
+    def Solve(self, max_iter=100):
+        n = self.ms.norb
+        C_ = np.zeros((n, n), order="F")
+        F_ = np.zeros((n, n), order="F")
+        e = np.zeros(n)
+        ehf = np.zeros(3)
+        it = C.c_int()
+        o = _lib.HFOpts(float(self.tolerance), int(max_iter))
+        rc = lib().imsrg_hip_hf_solve(self.Hbare.h, C.byref(o), C_.ctypes.data, e.ctypes.data, F_.ctypes.data,
+                                      ehf.ctypes.data, C.byref(it))
+        if rc not in (0, _lib.E_NOT_CONVERGED):
+            check(rc, self.ms.h)
+        self.C, self.F, self.energies = C_, F_, e
+        self.EHF, self.e1hf, self.e2hf = (float(x) for x in ehf)
+        self.iterations = it.value
+        self.converged = rc == 0
+        return self
+
+    def TransformToHFBasis(self, OpHO, out=None):
+        return OpHO.TransformToHFBasis(self.C, out)
+
+    def GetNormalOrderedH(self, out=None):
+        out = out if out is not None else DeviceOperator(self.ms, 1)
+        Cf = np.asfortranarray(self.C)
+        Ff = np.asfortranarray(self.F)
+        check(lib().imsrg_hip_hf_normal_ordered_h(out.h, self.Hbare.h, Cf.ctypes.data, Ff.ctypes.data,
+                                                  float(self.EHF)), self.ms.h)
+        return out
 
 
 IMSRGSolver = types.SimpleNamespace(WhiteGenerator=_white, Solve_magnus_euler=_magnus, Solver=_Solver)

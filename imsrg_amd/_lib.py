@@ -10,6 +10,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, "libimsrg_hip.so")
 
 IMSRG_OK = 0
+E_NOT_CONVERGED = -6
 ERRORS = {-1: "IMSRG_E_INVALID", -2: "IMSRG_E_UNSUPPORTED", -3: "IMSRG_E_HIP", -4: "IMSRG_E_NOMEM",
           -5: "IMSRG_E_NONFINITE", -6: "IMSRG_E_NOT_CONVERGED"}
 
@@ -51,6 +52,10 @@ class CommOpts(C.Structure):
 class BCHOpts(C.Structure):
     _fields_ = [("mode", C.c_int), ("transform_thr", C.c_double), ("product_thr", C.c_double),
                 ("max_terms", C.c_int), ("precision", C.c_int)]
+
+
+class HFOpts(C.Structure):
+    _fields_ = [("tolerance", C.c_double), ("max_iter", C.c_int)]
 
 
 class Tuning(C.Structure):
@@ -101,6 +106,8 @@ SIGNATURES = {
     "imsrg_hip_op_norm": (C.c_int, [_vp, _dp, _dp, _dp]),
     "imsrg_hip_op_normal_order": (C.c_int, [_vp, _vp, C.c_int]),
     "imsrg_hip_op_transform_basis": (C.c_int, [_vp, _vp, _vp]),
+    "imsrg_hip_hf_solve": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "imsrg_hip_hf_normal_ordered_h": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double]),
     "imsrg_hip_commutator_ss_dev": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(CommOpts)]),
     "imsrg_hip_commutator_ss_dev_async": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(CommOpts), _vp]),
     "imsrg_hip_commutator_ss_dev_noalloc": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(CommOpts)]),

@@ -759,6 +759,271 @@ extern "C" int imsrg_hip_op_transform_basis(imsrg_dop *out, const imsrg_dop *op,
 }
 
 // ---------------------------------------------------------------------------
+// Hartree-Fock (src/HartreeFock.cc) for a two-body Hamiltonian with the reference's
+// frozen occupations.  The monopoles come from the device (k_hf_mono); the
+// iteration works on norb x norb matrices (column-major) on the host.
+
+// Eigenvalues (ascending) and eigenvectors (columns) of a symmetric m x m matrix A
+// (column-major), by cyclic Jacobi rotations (Golub and Van Loan, Matrix
+// Computations, algorithms 8.4.1-8.4.3): the Fock blocks have a few rows each.
+static void sym_eig(int m, std::vector<double> A, std::vector<double> &w, std::vector<double> &V) {
+  auto at = [m](std::vector<double> &M, int r, int c) -> double & { return M[(size_t)c * m + r]; };
+  std::vector<double> U((size_t)m * m, 0.0);
+  for (int i = 0; i < m; ++i) at(U, i, i) = 1.0;
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    double off = 0.0, all = 0.0;
+    for (int c = 0; c < m; ++c)
+      for (int r = 0; r < m; ++r) {
+        all += at(A, r, c) * at(A, r, c);
+        if (r != c) off += at(A, r, c) * at(A, r, c);
+      }
+    if (off <= 1e-34 * all || off == 0.0) break;
+    for (int p = 0; p < m - 1; ++p)
+      for (int q = p + 1; q < m; ++q) {
+        const double apq = at(A, p, q);
+        if (apq == 0.0) continue;
+        const double tau = (at(A, q, q) - at(A, p, p)) / (2.0 * apq);
+        const double t = tau >= 0.0 ? 1.0 / (tau + std::sqrt(1.0 + tau * tau)) : -1.0 / (-tau + std::sqrt(1.0 + tau * tau));
+        const double c = 1.0 / std::sqrt(1.0 + t * t), sn = t * c;
+        for (int k = 0; k < m; ++k) {  // A J
+          const double x = at(A, k, p), y = at(A, k, q);
+          at(A, k, p) = c * x - sn * y;
+          at(A, k, q) = sn * x + c * y;
+        }
+        for (int k = 0; k < m; ++k) {  // J^T (A J)
+          const double x = at(A, p, k), y = at(A, q, k);
+          at(A, p, k) = c * x - sn * y;
+          at(A, q, k) = sn * x + c * y;
+        }
+        for (int k = 0; k < m; ++k) {  // U J
+          const double x = at(U, k, p), y = at(U, k, q);
+          at(U, k, p) = c * x - sn * y;
+          at(U, k, q) = sn * x + c * y;
+        }
+      }
+  }
+  std::vector<int> ord(m);
+  for (int i = 0; i < m; ++i) ord[i] = i;
+  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return at(A, x, x) < at(A, y, y); });
+  w.assign(m, 0.0);
+  V.assign((size_t)m * m, 0.0);
+  for (int k = 0; k < m; ++k) {
+    w[k] = at(A, ord[k], ord[k]);
+    for (int r = 0; r < m; ++r) V[(size_t)k * m + r] = at(U, r, ord[k]);
+  }
+}
+
+namespace {
+struct HFState {
+  int norb = 0;
+  std::vector<std::vector<int>> blocks;  // orbits of each (l, j2, tz2) block, ascending
+  std::vector<int> blk;                  // block of each orbit
+  std::vector<long> wbase;               // quad offset of block pair (bi, ba): wbase[bi * nb + ba]
+  std::vector<double> W;                 // monopoles of the quads (i, a, j, b)
+  std::vector<double> KE, C, rho, Vij, F, energies;
+  std::vector<int> j2;
+  std::vector<int> holes;
+  std::vector<double> hole_occ;
+  double e1 = 0, e2 = 0;
+  double &m(std::vector<double> &M, int r, int c) { return M[(size_t)c * norb + r]; }
+  double Wq(int i, int a, int j, int b) const {
+    const int bi = blk[i], ba = blk[a], nb = (int)blocks.size();
+    const int mi = (int)blocks[bi].size(), ma = (int)blocks[ba].size();
+    const int ii = (int)(std::lower_bound(blocks[bi].begin(), blocks[bi].end(), i) - blocks[bi].begin());
+    const int jj = (int)(std::lower_bound(blocks[bi].begin(), blocks[bi].end(), j) - blocks[bi].begin());
+    const int aa = (int)(std::lower_bound(blocks[ba].begin(), blocks[ba].end(), a) - blocks[ba].begin());
+    const int bb = (int)(std::lower_bound(blocks[ba].begin(), blocks[ba].end(), b) - blocks[ba].begin());
+    return W[wbase[(size_t)bi * nb + ba] + (((long)ii * mi + jj) * ma + aa) * ma + bb];
+  }
+  // UpdateDensityMatrix, src/HartreeFock.cc:613-617: rho = C_h diag(occ) C_h^T
+  void density() {
+    std::fill(rho.begin(), rho.end(), 0.0);
+    for (int c = 0; c < norb; ++c)
+      for (int r = 0; r < norb; ++r) {
+        double acc = 0.0;
+        for (size_t h = 0; h < holes.size(); ++h) acc += m(C, r, holes[h]) * hole_occ[h] * m(C, c, holes[h]);
+        m(rho, r, c) = acc;
+      }
+  }
+  // UpdateF, src/HartreeFock.cc:750-822 (two-body part): Vij(i, j) = sum_{a, b in
+  // block(a)} rho(a, b) W(ia, jb) / (2j_i+1) for i <= j of one block, mirrored
+  void fock() {
+    std::fill(Vij.begin(), Vij.end(), 0.0);
+    for (const auto &B : blocks)
+      for (int i : B)
+        for (int j : B) {
+          if (j < i) continue;
+          double acc = 0.0;
+          for (int a = 0; a < norb; ++a)
+            for (int b : blocks[blk[a]]) acc += m(rho, a, b) * Wq(i, a, j, b);
+          m(Vij, i, j) = acc / (j2[i] + 1);
+        }
+    for (int c = 0; c < norb; ++c)
+      for (int r = c + 1; r < norb; ++r) m(Vij, r, c) = m(Vij, c, r);
+    for (size_t e = 0; e < F.size(); ++e) F[e] = KE[e] + Vij[e];
+  }
+  // Diagonalize + ReorderCoefficients, src/HartreeFock.cc:211-245, 866-903: each
+  // block's eigenvectors in ascending energy, then columns with C(i, i) < 0 negated
+  void diagonalize() {
+    for (const auto &B : blocks) {
+      const int mb = (int)B.size();
+      std::vector<double> Fb((size_t)mb * mb), w, V;
+      for (int c = 0; c < mb; ++c)
+        for (int r = 0; r < mb; ++r) Fb[(size_t)c * mb + r] = m(F, B[r], B[c]);
+      sym_eig(mb, Fb, w, V);
+      for (int c = 0; c < mb; ++c) {
+        energies[B[c]] = w[c];
+        for (int r = 0; r < mb; ++r) m(C, B[r], B[c]) = V[(size_t)c * mb + r];
+      }
+    }
+    for (int c = 0; c < norb; ++c)
+      if (m(C, c, c) < 0)
+        for (int r = 0; r < norb; ++r) m(C, r, c) = -m(C, r, c);
+  }
+  // CalcEHF, src/HartreeFock.cc:169-190 (no three-body part)
+  double ehf() {
+    e1 = e2 = 0.0;
+    for (int i = 0; i < norb; ++i) {
+      const int jf = j2[i] + 1;
+      for (int j : blocks[blk[i]]) {
+        e1 += m(rho, i, j) * jf * m(KE, i, j);
+        e2 += m(rho, i, j) * jf * 0.5 * m(Vij, i, j);
+      }
+    }
+    return e1 + e2;
+  }
+};
+}  // namespace
+
+extern "C" int imsrg_hip_hf_solve(const imsrg_dop *H, const imsrg_hf_opts *opts, double *C_out,
+                                  double *energies_out, double *F_out, double *ehf_out, int *iterations) {
+  if (!H || !C_out) return IMSRG_E_INVALID;
+  imsrg_ctx *ctx = H->ctx;
+  const double tol = (opts && opts->tolerance > 0) ? opts->tolerance : 1e-8;
+  const int maxiter = (opts && opts->max_iter > 0) ? opts->max_iter : 100;
+  if (maxiter > 100)
+    return fail(ctx, IMSRG_E_UNSUPPORTED, "Hartree-Fock beyond 100 iterations (the reference's DIIS phase) is not provided");
+  if (H->herm != 1) return fail(ctx, IMSRG_E_UNSUPPORTED, "Hartree-Fock needs a Hermitian Hamiltonian");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const HostBasis &hb = ctx->hb;
+  const BasisDev &B = ctx->db.h;
+  HFState st;
+  const int norb = st.norb = hb.norb;
+  st.blk.assign(norb, -1);
+  for (int o = 0; o < norb; ++o)
+    if (hb.ob[o].front() == o) {
+      for (int x : hb.ob[o]) st.blk[x] = (int)st.blocks.size();
+      st.blocks.push_back(hb.ob[o]);
+    }
+  const int nb = (int)st.blocks.size();
+  std::vector<int4> quads;
+  st.wbase.assign((size_t)nb * nb, 0);
+  for (int bi = 0; bi < nb; ++bi)
+    for (int ba = 0; ba < nb; ++ba) {
+      st.wbase[(size_t)bi * nb + ba] = (long)quads.size();
+      for (int i : st.blocks[bi])
+        for (int j : st.blocks[bi])
+          for (int a : st.blocks[ba])
+            for (int b : st.blocks[ba]) quads.push_back(make_int4(i, a, j, b));
+    }
+  const int nq = (int)quads.size();
+  st.W.assign(nq, 0.0);
+  {  // monopoles on the device
+    int4 *dq = nullptr;
+    double *dw = nullptr;
+    HIPCHK(ctx, hipMalloc(&dq, sizeof(int4) * std::max(nq, 1)));
+    hipError_t e = hipMalloc(&dw, sizeof(double) * std::max(nq, 1));
+    if (e == hipSuccess) e = hipMemcpyAsync(dq, quads.data(), sizeof(int4) * nq, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = launch_hf_mono(ctx->db.d, H->buf, dq, nq, dw, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(st.W.data(), dw, sizeof(double) * nq, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(dq);
+    if (dw) (void)hipFree(dw);
+    if (e != hipSuccess) return fail(ctx, IMSRG_E_HIP, std::string("Hartree-Fock monopoles: ") + hipGetErrorString(e));
+  }
+  const size_t n2 = (size_t)norb * norb;
+  st.KE.assign(n2, 0.0);
+  HIPCHK(ctx, hipMemcpyAsync(st.KE.data(), H->buf + B.one_off, sizeof(double) * n2, hipMemcpyDeviceToHost,
+                             ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  st.j2 = hb.j2;
+  st.holes = hb.holes;
+  for (int h : hb.holes) st.hole_occ.push_back(hb.occ[h]);
+  st.C.assign(n2, 0.0);
+  for (int i = 0; i < norb; ++i) st.m(st.C, i, i) = 1.0;
+  st.rho.assign(n2, 0.0);
+  st.Vij.assign(n2, 0.0);
+  st.F.assign(n2, 0.0);
+  st.energies.assign(norb, 0.0);
+  for (int i = 0; i < norb; ++i) st.energies[i] = st.m(st.KE, i, i);
+  // Solve, src/HartreeFock.cc:55-167, up to the switch to DIIS at 100 iterations
+  st.density();
+  st.fock();
+  const double mix = 0.2;  // density_mixing_factor; field_mixing_factor is 0 in these iterations
+  int it = 0;
+  bool converged = false;
+  for (it = 0; it < maxiter; ++it) {
+    const std::vector<double> prev = st.energies;
+    st.diagonalize();
+    const std::vector<double> last_rho = st.rho;
+    st.density();
+    for (size_t e = 0; e < n2; ++e) st.rho[e] = (1.0 - mix) * st.rho[e] + mix * last_rho[e];
+    st.fock();
+    st.ehf();  // CheckConvergence, :834-844
+    double d2 = 0.0;
+    for (int i = 0; i < norb; ++i) d2 += (st.energies[i] - prev[i]) * (st.energies[i] - prev[i]);
+    if (std::sqrt(d2) / norb < tol) {
+      converged = true;
+      break;
+    }
+  }
+  const double e = st.ehf();
+  std::memcpy(C_out, st.C.data(), sizeof(double) * n2);
+  if (energies_out) std::memcpy(energies_out, st.energies.data(), sizeof(double) * norb);
+  if (F_out) std::memcpy(F_out, st.F.data(), sizeof(double) * n2);
+  if (ehf_out) {
+    ehf_out[0] = e;
+    ehf_out[1] = st.e1;
+    ehf_out[2] = st.e2;
+  }
+  if (iterations) *iterations = it;
+  if (!converged) return fail(ctx, IMSRG_E_NOT_CONVERGED, "Hartree-Fock did not converge within max_iter iterations");
+  return IMSRG_OK;
+}
+
+// HartreeFock::GetNormalOrderedH (src/HartreeFock.cc:1079-1172) for a two-body
+// Hamiltonian: E0 = EHF, OneBody = C^T F C, and the two-body part D^T V D as in
+// TransformToHFBasis; the result is Hermitian.
+extern "C" int imsrg_hip_hf_normal_ordered_h(imsrg_dop *out, const imsrg_dop *H, const double *C, const double *F,
+                                             double ehf) {
+  if (!out || !H || !C || !F) return IMSRG_E_INVALID;
+  if (!same_ctx(out, H)) return IMSRG_E_INVALID;
+  imsrg_ctx *ctx = out->ctx;
+  if (int rc = imsrg_hip_op_transform_basis(out, H, C)) return rc;
+  const int norb = ctx->hb.norb;
+  const size_t n2 = (size_t)norb * norb;
+  std::vector<double> T(n2, 0.0), f(n2, 0.0);
+  for (int b = 0; b < norb; ++b)  // T = F C
+    for (int k = 0; k < norb; ++k) {
+      const double c = C[(size_t)b * norb + k];
+      if (c == 0.0) continue;
+      for (int a = 0; a < norb; ++a) T[(size_t)b * norb + a] += F[(size_t)k * norb + a] * c;
+    }
+  for (int b = 0; b < norb; ++b)  // f = C^T T
+    for (int a = 0; a < norb; ++a) {
+      double acc = 0.0;
+      for (int k = 0; k < norb; ++k) acc += C[(size_t)a * norb + k] * T[(size_t)b * norb + k];
+      f[(size_t)b * norb + a] = acc;
+    }
+  HIPCHK(ctx, hipMemcpyAsync(out->buf + ctx->db.h.one_off, f.data(), sizeof(double) * n2, hipMemcpyHostToDevice,
+                             ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(out->buf, &ehf, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  out->herm = 1;
+  return IMSRG_OK;
+}
+
+// ---------------------------------------------------------------------------
 static int comm_opts_ok(imsrg_ctx *ctx, const imsrg_comm_opts *o, unsigned *mask, int *prec) {
   *mask = IMSRG_ALL_TERMS;
   *prec = IMSRG_PREC_FP64;

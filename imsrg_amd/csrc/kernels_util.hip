@@ -412,6 +412,45 @@ __global__ __launch_bounds__(256) void k_hf_D(const BasisDev *bd, const double *
   D[e] = d;
 }
 
+// The two-body monopoles of the Hartree-Fock field (HartreeFock::BuildMonopoleV and
+// the lookups of UpdateF, src/HartreeFock.cc:250-285, 771-800):
+//   W(i a, j b) = sum_J (2J+1) V~^J_{ia,jb}
+// with the tilde element (sqrt2 per ket with equal orbits, (-1)^{(j_p+j_q)/2+1+J}
+// per swapped ket), J over the couplings of (i, a), summed in ascending J; equal to
+// the reference's Vmon / Vmon_exch entry, GetTBMEmonopole times (2j_i+1)(2j_a+1)
+// (src/TwoBodyME.cc:614-679).  quads[q] = (i, a, j, b); one thread per quad.
+__global__ __launch_bounds__(256) void k_hf_mono(const BasisDev *Bp, const double *op, const int4 *quads, int nq,
+                                                 double *W) {
+  const BasisDev &B = *Bp;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  const int4 t = quads[q];
+  const int i = t.x, a = t.y, j = t.z, b = t.w, norb = B.norb;
+  const double *two = op + B.two_off;
+  const int par = (B.l[i] + B.l[a]) & 1, tz = (B.tz2[i] + B.tz2[a]) / 2;
+  double acc = 0.0;
+  if (par == ((B.l[j] + B.l[b]) & 1) && tz == (B.tz2[j] + B.tz2[b]) / 2) {
+    const int kia = ket_index(norb, min(i, a), max(i, a)), kjb = ket_index(norb, min(j, b), max(j, b));
+    const double nrm = (i == a ? 1.4142135623730951 : 1.0) * (j == b ? 1.4142135623730951 : 1.0);
+    for (int J = abs(B.j2[i] - B.j2[a]) / 2; J <= (B.j2[i] + B.j2[a]) / 2; ++J) {
+      const int r = B.kmapT[(long)J * B.nkets + kia], c = B.kmapT[(long)J * B.nkets + kjb];
+      if (r < 0 || c < 0) continue;
+      const int ch = 6 * J + 2 * (tz + 1) + par, n = B.ch_n[ch];
+      double ph = 1.0;
+      if (i > a) ph *= ((B.j2[i] + B.j2[a]) / 2 + 1 + J) & 1 ? -1.0 : 1.0;
+      if (j > b) ph *= ((B.j2[j] + B.j2[b]) / 2 + 1 + J) & 1 ? -1.0 : 1.0;
+      acc += (2 * J + 1) * (nrm * ph * two[B.ch_off[ch] + (long)c * n + r]);
+    }
+  }
+  W[q] = acc;
+}
+
+hipError_t launch_hf_mono(const BasisDev *bd, const double *op, const int4 *quads, int nq, double *W,
+                          hipStream_t s) {
+  if (nq > 0) hipLaunchKernelGGL(k_hf_mono, dim3((nq + 255) / 256), dim3(256), 0, s, bd, op, quads, nq, W);
+  return hipGetLastError();
+}
+
 hipError_t launch_hf_D(const BasisDev *bd, const BasisDev &bh, const double *C, double *D, hipStream_t s) {
   if (bh.tbsize > 0) {
     const long nb = (bh.tbsize + 255) / 256;

@@ -60,6 +60,9 @@ hipError_t launch_norm(const BasisDev *bd, const BasisDev &bh, const double *op,
                        double *out, hipStream_t s);
 // the basis-change matrices D of every channel (two-body layout, tbsize doubles) from
 // the one-body basis change C (norb x norb, column-major)
+// the Hartree-Fock monopoles W[q] = sum_J (2J+1) V~^J_{ia,jb} of quads[q] = (i, a, j, b)
+hipError_t launch_hf_mono(const BasisDev *bd, const double *op, const int4 *quads, int nq, double *W,
+                          hipStream_t s);
 hipError_t launch_hf_D(const BasisDev *bd, const BasisDev &bh, const double *C, double *D, hipStream_t s);
 hipError_t launch_fill_random(const BasisDev *bd, const BasisDev &bh, double *op, int herm,
                               unsigned long long seed, hipStream_t s);

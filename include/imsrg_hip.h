@@ -175,6 +175,30 @@ int imsrg_hip_op_normal_order(imsrg_dop *out, const imsrg_dop *op, int sign);
    Synchronous: returns when out holds the result. */
 int imsrg_hip_op_transform_basis(imsrg_dop *out, const imsrg_dop *op, const double *C);
 
+/* HartreeFock::Solve options (src/HartreeFock.cc:55-167); zero fields take the defaults. */
+typedef struct {
+  double tolerance; /* convergence of the single-particle energies (default 1e-8, HartreeFock::tolerance) */
+  int max_iter;     /* iterations with density mixing 0.2 (default and maximum 100: the reference
+                       switches to DIIS after 100, which is not provided) */
+} imsrg_hf_opts;
+
+/* HartreeFock::Solve + CalcEHF for a two-body Hermitian Hamiltonian H on the context's
+   reference, occupations frozen (the reference's default): the monopoles
+   sum_J (2J+1) V~^J_{ia,jb} of BuildMonopoleV / UpdateF (src/HartreeFock.cc:250-285, 750-822)
+   come from the device; the iteration (Diagonalize per (l, j, tz) block, ReorderCoefficients'
+   sign rule, UpdateDensityMatrix, density mixing 0.2, UpdateF, CheckConvergence:
+   src/HartreeFock.cc:211-245, 613-617, 834-903) runs on the host.  Outputs (host):
+   C (norb x norb, column-major, C(i,a) = <i|a>), energies (norb, may be NULL), F (the Fock
+   matrix, norb x norb, may be NULL), ehf[3] = {EHF, e1hf, e2hf} (may be NULL), iterations
+   (may be NULL).  IMSRG_E_NOT_CONVERGED (outputs written) when max_iter is reached. */
+int imsrg_hip_hf_solve(const imsrg_dop *H, const imsrg_hf_opts *opts, double *C, double *energies, double *F,
+                       double *ehf, int *iterations);
+/* HartreeFock::GetNormalOrderedH (src/HartreeFock.cc:1079-1172) without three-body forces:
+   out = H in the basis C with ZeroBody = ehf, OneBody = C^T F C, two-body part as
+   imsrg_hip_op_transform_basis; out is Hermitian.  Synchronous. */
+int imsrg_hip_hf_normal_ordered_h(imsrg_dop *out, const imsrg_dop *H, const double *C, const double *F,
+                                  double ehf);
+
 /* device-resident versions of the drop-ins (no PCIe traffic) */
 int imsrg_hip_commutator_ss_dev(imsrg_ctx *ctx, const imsrg_dop *X, const imsrg_dop *Y, imsrg_dop *Z,
                                 const imsrg_comm_opts *opts);

@@ -139,8 +139,14 @@ def use_blas(on=True):
     return ";".join(sorted({f"{i.get('internal_api')} {i.get('version')}" for i in info})) or "scipy BLAS"
 
 
+_FAULT = 0
+
+
 def set_fault(f):
-    """Plant a known mistake in the oracle (checker tests only); 0 turns it off."""
+    """Plant a known mistake in the oracle (checker tests only); 0 turns it off.
+    12 (Python part): Hartree-Fock field without its 1/(2j_i+1)."""
+    global _FAULT
+    _FAULT = int(f)
     lib().orc_set_fault(int(f))
 
 
@@ -272,6 +278,94 @@ class OracleModelSpace:
         Cf = np.asfortranarray(np.asarray(Cmat, dtype=np.float64))
         lib().orc_transform_basis(self.h, C.byref(a.s), Cf.ctypes.data_as(_dp), C.byref(o.s))
         return o.op()
+
+    def hf_solve(self, H, tol=1e-8, max_iter=100):
+        """HartreeFock::Solve + CalcEHF for a two-body Hamiltonian, frozen occupations,
+        the iterations before the reference's switch to DIIS (src/HartreeFock.cc:55-167).
+        Returns dict(C, energies, F, rho, Vij, EHF, e1hf, e2hf, iterations, converged)."""
+        norb, orb = self.norb, self.orbits
+        blocks = {}
+        for i, o in enumerate(orb):
+            blocks.setdefault((o["l"], o["j2"], o["tz2"]), []).append(i)
+        blk = {i: b for b in blocks.values() for i in b}
+        holes = [i for i, o in enumerate(orb) if o["occ"] > 1e-6]  # OCC_CUT, src/ModelSpace.cc:14
+        hole_occ = np.array([orb[h]["occ"] for h in holes])
+
+        def kidx(p, q):  # Index2, src/ModelSpace.cc:1108-1118
+            return p * (2 * norb - 1 - p) // 2 + q
+
+        def mono_norm(p, q, r, t):  # GetTBMEmonopole * (2j_p+1)(2j_q+1)
+            return self.monopole(H, p, q, r, t) * (orb[p]["j2"] + 1) * (orb[q]["j2"] + 1)
+
+        vcache = {}
+
+        def vmon(i, a, j, b):
+            """The Vmon / Vmon_exch entry that UpdateF reads (src/HartreeFock.cc:250-285,
+            771-796): upper triangle computed from the bra, lower by symmatu."""
+            key = (i, a, j, b)
+            if key not in vcache:
+                bra, ket = (min(i, a), max(i, a)), (min(j, b), max(j, b))
+                exch = (a > i) != (b > j)
+                if kidx(*bra) <= kidx(*ket):
+                    v = mono_norm(bra[0], bra[1], ket[1], ket[0]) if exch else mono_norm(*bra, *ket)
+                else:
+                    v = mono_norm(ket[0], ket[1], bra[1], bra[0]) if exch else mono_norm(*ket, *bra)
+                vcache[key] = v
+            return vcache[key]
+
+        KE = np.array(H["one"], dtype=np.float64, order="F")
+        Cm = np.eye(norb)
+        energies = np.diag(KE).copy()
+
+        def density(Cm):  # UpdateDensityMatrix, :613-617
+            t = Cm[:, holes]
+            return (t * hole_occ) @ t.T
+
+        def fock(rho):  # UpdateF, :750-822
+            V = np.zeros((norb, norb))
+            for b_i in blocks.values():
+                for i in b_i:
+                    for j in b_i:
+                        if j < i:
+                            continue
+                        acc = 0.0
+                        for a in range(norb):
+                            for b in blk[a]:
+                                acc += rho[a, b] * vmon(i, a, j, b)
+                        V[i, j] = acc if _FAULT == 12 else acc / (orb[i]["j2"] + 1)
+            V = np.triu(V) + np.triu(V, 1).T
+            return V, KE + V
+
+        def calc_ehf(rho, V):  # CalcEHF, :169-190
+            e1 = e2 = 0.0
+            for i in range(norb):
+                jf = orb[i]["j2"] + 1
+                for j in blk[i]:
+                    e1 += rho[i, j] * jf * KE[i, j]
+                    e2 += rho[i, j] * jf * 0.5 * V[i, j]
+            return e1 + e2, e1, e2
+
+        rho = density(Cm)
+        V, F = fock(rho)
+        it, converged = 0, False
+        for it in range(max_iter):
+            prev = energies.copy()
+            for b in blocks.values():  # Diagonalize, :211-245
+                w, U = np.linalg.eigh(F[np.ix_(b, b)])
+                energies[b] = w
+                Cm[np.ix_(b, b)] = U
+            for c in range(norb):  # ReorderCoefficients' sign rule, :897-901
+                if Cm[c, c] < 0:
+                    Cm[:, c] *= -1
+            last = rho
+            rho = 0.8 * density(Cm) + 0.2 * last
+            V, F = fock(rho)
+            if np.linalg.norm(energies - prev) / norb < tol:  # CheckConvergence, :834-844
+                converged = True
+                break
+        ehf, e1, e2 = calc_ehf(rho, V)
+        return dict(C=Cm, energies=energies, F=F, rho=rho, Vij=V, EHF=ehf, e1hf=e1, e2hf=e2, iterations=it,
+                    converged=converged)
 
     def white_generator(self, H):
         a, o = _Holder(H), _Holder(self.zero_op(-1))

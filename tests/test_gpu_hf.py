@@ -1,0 +1,95 @@
+"""GPU parity of Hartree-Fock (imsrg_hip_hf_solve, imsrg_hip_hf_normal_ordered_h)
+with the oracle's restatement of HartreeFock::Solve / GetNormalOrderedH
+(src/HartreeFock.cc:55-190, 1079-1172), whose fixed point tests/test_oracle_hf.py
+pins; at emax 10 the properties of the fixed point on the device alone."""
+import sys
+
+import numpy as np
+import pytest
+
+from opgen import rel_maxabs, synthetic_hamiltonian
+from oracle.pyoracle import OracleModelSpace
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ia():
+    import imsrg_amd
+    return imsrg_amd
+
+
+_MS = {}
+
+
+def spaces(ia, emax, A=16, Z=8):
+    key = (emax, A, Z)
+    if key not in _MS:
+        ref = {v: k for k, v in ia._ELEM.items()}[Z] + str(A)
+        _MS[key] = (ia.ModelSpace(emax, ref), OracleModelSpace(emax, A, Z))
+    return _MS[key]
+
+
+def to_dev(ia, ms, op):
+    return ia.Operator(ms, op["herm"], op["zero"], op["one"], op["two"]).to_device()
+
+
+def host(d):
+    o = d.to_host()
+    return {"zero": o.ZeroBody, "one": np.asarray(o.OneBody), "two": np.asarray(o.TwoBody), "herm": o.herm}
+
+
+@pytest.mark.parametrize("emax,A,Z", [(4, 16, 8), (3, 18, 8), (3, 14, 7)])
+def test_hf_matches_oracle(ia, emax, A, Z):
+    ms, orc = spaces(ia, emax, A, Z)
+    say(f"[gpu] Hartree-Fock emax={emax} A={A} Z={Z}")
+    H = synthetic_hamiltonian(orc, 7100 + emax + A)
+    hf = ia.HartreeFock(to_dev(ia, ms, H)).Solve()
+    ref = orc.hf_solve(H)
+    assert hf.converged and ref["converged"] and hf.iterations == ref["iterations"]
+    assert abs(hf.EHF - ref["EHF"]) < 1e-12 * abs(ref["EHF"])
+    assert abs(hf.e1hf - ref["e1hf"]) < 1e-12 * abs(ref["e1hf"])
+    assert np.max(np.abs(hf.energies - ref["energies"])) < 1e-11 * np.max(np.abs(ref["energies"]))
+    assert np.max(np.abs(hf.C - ref["C"])) < 1e-10
+    assert rel_maxabs(hf.F, ref["F"]) < 1e-12
+    # GetNormalOrderedH: E0 = EHF, f = C^T F C, two-body D^T V D
+    HNO = host(hf.GetNormalOrderedH())
+    assert HNO["herm"] == 1 and HNO["zero"] == hf.EHF
+    assert rel_maxabs(HNO["one"], ref["C"].T @ ref["F"] @ ref["C"]) < 1e-11
+    assert rel_maxabs(HNO["two"], orc.transform_basis(H, ref["C"])["two"]) < 1e-10
+
+
+def test_hf_iteration_limit(ia):
+    ms, orc = spaces(ia, 4)
+    H = to_dev(ia, ms, synthetic_hamiltonian(orc, 7150))
+    hf = ia.HartreeFock(H).Solve(max_iter=1)
+    assert not hf.converged and hf.iterations == 1 and hf.EHF != 0.0
+    with pytest.raises(ia.HipError) as e:
+        ia.HartreeFock(H).Solve(max_iter=101)  # the reference's DIIS phase is not provided
+    assert e.value.code == -2
+
+
+def test_hf_emax10_fixed_point(ia):
+    ms = ia.ModelSpace(10, "O16")
+    say("[gpu] Hartree-Fock emax=10")
+    H = to_dev(ia, ms, synthetic_hamiltonian(ms, 7200))
+    hf = ia.HartreeFock(H)
+    hf.tolerance = 1e-13
+    hf.Solve()
+    assert hf.converged
+    HNO = hf.GetNormalOrderedH()
+    # the normal ordering of the HF-basis Hamiltonian over the same holes reproduces it
+    N = host(hf.TransformToHFBasis(H).DoNormalOrdering())
+    h = host(HNO)
+    assert abs(N["zero"] - hf.EHF) < 1e-11 * abs(hf.EHF)
+    assert rel_maxabs(N["one"], h["one"]) < 1e-11
+    # the HF basis diagonalises the Fock matrix, and EHF lies below the HO reference energy
+    f = h["one"]
+    assert np.max(np.abs(np.diag(f) - hf.energies)) < 1e-10 * np.max(np.abs(hf.energies))
+    assert np.max(np.abs(f - np.diag(np.diag(f)))) < 1e-10 * np.max(np.abs(hf.energies))
+    assert hf.EHF < host(H.DoNormalOrdering())["zero"]
