@@ -22,7 +22,7 @@ import numpy as np
 from . import _lib
 from ._lib import ALL_TERMS, PREC_FP64, PREC_MIXED, PREC_SPLIT, TERMS, HipError, check, lib
 
-__all__ = ["ModelSpace", "Operator", "DeviceOperator", "Commutator", "BCH", "IMSRGSolver", "HartreeFock", "TERMS",
+__all__ = ["ModelSpace", "Operator", "DeviceOperator", "Commutator", "BCH", "IMSRGSolver", "HartreeFock", "ReadWrite", "TERMS",
            "ALL_TERMS",
            "PREC_FP64", "PREC_MIXED", "PREC_SPLIT", "HipError"]
 
@@ -472,6 +472,19 @@ class _Solver:
             self.close()
         except Exception:
             pass
+
+
+class ReadWrite:
+    """ReadWrite (src/ReadWrite.hh, src/ReadWrite.cc): interaction files into operators."""
+
+    def ReadBareTBME_Darmstadt(self, filename, Hbare, emax=-1, E2max=-1, lmax=-1):
+        """The two-body part of the Hermitian DeviceOperator Hbare from a Darmstadt me2j
+        file (text, .gz or .bin; src/ReadWrite.cc:664-728, 834-957).  Returns the number
+        of (a b c d J) records read."""
+        n = C.c_long()
+        check(lib().imsrg_hip_read_me2j(Hbare.h, str(filename).encode(), int(emax), int(E2max), int(lmax),
+                                        C.byref(n)), Hbare.ms.h)
+        return n.value
 
 
 class HartreeFock:

@@ -106,6 +106,7 @@ SIGNATURES = {
     "imsrg_hip_op_norm": (C.c_int, [_vp, _dp, _dp, _dp]),
     "imsrg_hip_op_normal_order": (C.c_int, [_vp, _vp, C.c_int]),
     "imsrg_hip_op_transform_basis": (C.c_int, [_vp, _vp, _vp]),
+    "imsrg_hip_read_me2j": (C.c_int, [_vp, C.c_char_p, C.c_int, C.c_int, C.c_int, _vp]),
     "imsrg_hip_hf_solve": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "imsrg_hip_hf_normal_ordered_h": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double]),
     "imsrg_hip_commutator_ss_dev": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(CommOpts)]),

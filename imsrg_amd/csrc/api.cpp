@@ -26,6 +26,7 @@
 #include "basis.hpp"
 #include "launch.hpp"
 #include "noalloc.hpp"
+#include "readwrite.hpp"
 
 using namespace imsrg;
 
@@ -154,6 +155,16 @@ struct imsrg_ctx {
   double *xform = nullptr;
   long xform_n = 0;
   rocblas_handle blas = nullptr;
+  // ket groups of the block-sparse transformation (built on first use)
+  struct {
+    bool built = false, usable = false;
+    XformDev d{};
+    std::vector<void *> allocs;
+    int2 *right = nullptr, *left = nullptr;
+    int nright = 0, nleft = 0, R = 64, kc = 4;
+    long dg_size = 0;
+    std::vector<int> blk;  // one-body block of each orbit
+  } xf;
   double *norm_dev = nullptr;
   double *norm_host = nullptr;  // pinned
   int npart = 0;
@@ -326,6 +337,7 @@ extern "C" int imsrg_hip_ctx_destroy(imsrg_ctx *ctx) {
   if (ctx->c122w) (void)hipFree(ctx->c122w);
   if (ctx->partials) (void)hipFree(ctx->partials);
   if (ctx->xform) (void)hipFree(ctx->xform);
+  for (void *p : ctx->xf.allocs) (void)hipFree(p);
   if (ctx->blas) blas_lib().destroy(ctx->blas);
   if (ctx->norm_dev) (void)hipFree(ctx->norm_dev);
   if (ctx->norm_host) (void)hipHostFree(ctx->norm_host);
@@ -697,6 +709,89 @@ extern "C" int imsrg_hip_op_normal_order(imsrg_dop *out, const imsrg_dop *op, in
   return IMSRG_OK;
 }
 
+// Ket groups of the block-sparse transformation: per channel, the kets whose orbits
+// lie in the same unordered pair of one-body blocks; work items of the two passes.
+template <typename T>
+static T *xf_upload(imsrg_ctx *ctx, const std::vector<T> &v, hipError_t &e) {
+  T *p = nullptr;
+  if (e != hipSuccess) return nullptr;
+  e = hipMalloc(&p, sizeof(T) * std::max<size_t>(v.size(), 1));
+  if (e != hipSuccess) return nullptr;
+  ctx->xf.allocs.push_back(p);
+  if (!v.empty()) e = hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice);
+  return p;
+}
+
+static int xform_tables(imsrg_ctx *ctx) {
+  auto &xf = ctx->xf;
+  if (xf.built) return IMSRG_OK;
+  const HostBasis &hb = ctx->hb;
+  const int norb = hb.norb;
+  xf.blk.assign(norb, -1);
+  for (int o = 0; o < norb; ++o) xf.blk[o] = hb.ob[o].front();  // the block's first orbit names it
+  std::vector<int> g_start{0}, g_kets, g_ch, k_start(hb.nch + 1, 0), k_grp, k_pos;
+  std::vector<long> g_doff;
+  std::vector<int2> right, left;
+  long doff = 0;
+  int maxm = 0, maxn = 0;
+  for (int c = 0; c < hb.nch; ++c) {
+    const auto &kets = hb.ch[c].kets;
+    const int n = (int)kets.size();
+    k_start[c] = (int)k_grp.size();
+    maxn = std::max(maxn, n);
+    std::map<std::pair<int, int>, std::vector<int>> groups;
+    for (int i = 0; i < n; ++i) {
+      const int bp = xf.blk[hb.kp[kets[i]]], bq = xf.blk[hb.kq[kets[i]]];
+      groups[{std::min(bp, bq), std::max(bp, bq)}].push_back(i);
+    }
+    k_grp.resize(k_start[c] + n);
+    k_pos.resize(k_start[c] + n);
+    for (const auto &kv : groups) {
+      const int G = (int)g_ch.size(), m = (int)kv.second.size();
+      g_ch.push_back(c);
+      g_doff.push_back(doff);
+      doff += (long)m * m;
+      maxm = std::max(maxm, m);
+      for (int pos = 0; pos < m; ++pos) {
+        g_kets.push_back(kv.second[pos]);
+        k_grp[k_start[c] + kv.second[pos]] = G;
+        k_pos[k_start[c] + kv.second[pos]] = pos;
+      }
+      g_start.push_back((int)g_kets.size());
+    }
+  }
+  k_start[hb.nch] = (int)k_grp.size();
+  xf.built = true;
+  // LDS of one work-group stays within 64 KiB; beyond that the dense path runs
+  xf.R = maxm <= 48 ? 64 : 32;
+  xf.kc = maxn <= 2048 ? 4 : maxn <= 4096 ? 2 : 1;
+  if (sizeof(double) * ((size_t)maxm * xf.R + (size_t)maxm * maxm) > 65536 || (size_t)maxn * 8 * xf.kc > 65536)
+    return IMSRG_OK;
+  for (int G = 0; G < (int)g_ch.size(); ++G)
+    for (int r0 = 0; r0 < (int)hb.ch[g_ch[G]].kets.size(); r0 += xf.R) right.push_back(make_int2(G, r0));
+  for (int c = 0; c < hb.nch; ++c)
+    for (int b0 = 0; b0 < (int)hb.ch[c].kets.size(); b0 += xf.kc) left.push_back(make_int2(c, b0));
+  hipError_t e = hipSuccess;
+  XformDev &d = xf.d;
+  d.g_start = xf_upload(ctx, g_start, e);
+  d.g_kets = xf_upload(ctx, g_kets, e);
+  d.g_ch = xf_upload(ctx, g_ch, e);
+  d.g_doff = xf_upload(ctx, g_doff, e);
+  d.k_start = xf_upload(ctx, k_start, e);
+  d.k_grp = xf_upload(ctx, k_grp, e);
+  d.k_pos = xf_upload(ctx, k_pos, e);
+  xf.right = xf_upload(ctx, right, e);
+  xf.left = xf_upload(ctx, left, e);
+  if (e != hipSuccess) return fail(ctx, IMSRG_E_HIP, std::string("basis transformation tables: ") + hipGetErrorString(e));
+  d.ngroups = (int)g_ch.size();
+  d.maxm = maxm;
+  xf.nright = (int)right.size();
+  xf.nleft = (int)left.size();
+  xf.dg_size = doff;
+  xf.usable = true;
+  return IMSRG_OK;
+}
+
 // HartreeFock::TransformToHFBasis of a scalar operator (src/HartreeFock.cc:905-1000):
 // OneBody' = C^T OneBody C and, per channel, V' = D^T V D with the matrices D of
 // k_hf_D; the zero-body part and Hermiticity carry over.  C: host, norb x norb,
@@ -705,12 +800,19 @@ extern "C" int imsrg_hip_op_transform_basis(imsrg_dop *out, const imsrg_dop *op,
   if (!out || !op || !C) return IMSRG_E_INVALID;
   if (!same_ctx(out, op)) return IMSRG_E_INVALID;
   imsrg_ctx *ctx = out->ctx;
-  const BlasLib &bl = blas_lib();
-  if (!bl.err.empty()) return fail(ctx, IMSRG_E_HIP, bl.err);
   HIPCHK(ctx, hipSetDevice(ctx->device));
   const BasisDev &B = ctx->db.h;
   const long n1 = (long)B.norb * B.norb;
-  const long need = B.tbsize + (long)B.maxn * B.maxn + 2 * n1;
+  if (int rc = xform_tables(ctx)) return rc;
+  // a C that is block diagonal in (l, j2, tz2) takes the block-sparse passes
+  bool blockdiag = ctx->xf.usable;
+  for (int a = 0; a < B.norb && blockdiag; ++a)
+    for (int i = 0; i < B.norb; ++i)
+      if (ctx->xf.blk[i] != ctx->xf.blk[a] && C[(long)a * B.norb + i] != 0.0) {
+        blockdiag = false;
+        break;
+      }
+  const long need = B.tbsize + std::max((long)B.maxn * B.maxn, ctx->xf.dg_size) + 2 * n1;
   if (ctx->xform_n < need) {
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     if (ctx->xform) HIPCHK(ctx, hipFree(ctx->xform));
@@ -719,6 +821,25 @@ extern "C" int imsrg_hip_op_transform_basis(imsrg_dop *out, const imsrg_dop *op,
     HIPCHK(ctx, hipMalloc(&ctx->xform, sizeof(double) * need));
     ctx->xform_n = need;
   }
+  if (blockdiag) {
+    double *T = ctx->xform, *Dg = T + B.tbsize, *Cd = Dg + std::max((long)B.maxn * B.maxn, ctx->xf.dg_size);
+    double *T1 = Cd + n1;
+    HIPCHK(ctx, hipMemcpyAsync(Cd, C, sizeof(double) * n1, hipMemcpyHostToDevice, ctx->stream));
+    if (out != op) {  // the zero-body part and the slack around the one- and two-body parts
+      const long rg[3][2] = {{0, B.one_off}, {B.one_off + n1, B.two_off}, {B.two_off + B.tbsize, B.opsize}};
+      for (const auto &r : rg)
+        if (r[1] > r[0])
+          HIPCHK(ctx, hipMemcpyAsync(out->buf + r[0], op->buf + r[0], sizeof(double) * (r[1] - r[0]),
+                                     hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    out->herm = op->herm;
+    HIPCHK(ctx, launch_xform_blocks(ctx->db.d, B, ctx->xf.d, Cd, op->buf, out->buf, T, Dg, T1, ctx->xf.right,
+                                    ctx->xf.nright, ctx->xf.R, ctx->xf.left, ctx->xf.nleft, ctx->xf.kc, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return IMSRG_OK;
+  }
+  const BlasLib &bl = blas_lib();
+  if (!bl.err.empty()) return fail(ctx, IMSRG_E_HIP, bl.err);
   if (!ctx->blas) {
     rocblas_handle h = nullptr;
     if (bl.create(&h) != rocblas_status_success) return fail(ctx, IMSRG_E_HIP, "rocblas_create_handle failed");
@@ -726,7 +847,8 @@ extern "C" int imsrg_hip_op_transform_basis(imsrg_dop *out, const imsrg_dop *op,
   }
   if (bl.set_stream(ctx->blas, ctx->stream) != rocblas_status_success)
     return fail(ctx, IMSRG_E_HIP, "rocblas_set_stream failed");
-  double *D = ctx->xform, *T = D + B.tbsize, *Cd = T + (long)B.maxn * B.maxn, *T1 = Cd + n1;
+  double *D = ctx->xform, *T = D + B.tbsize, *Cd = T + std::max((long)B.maxn * B.maxn, ctx->xf.dg_size);
+  double *T1 = Cd + n1;
   HIPCHK(ctx, hipMemcpyAsync(Cd, C, sizeof(double) * n1, hipMemcpyHostToDevice, ctx->stream));
   if (out != op) HIPCHK(ctx, hipMemcpyAsync(out->buf, op->buf, sizeof(double) * B.opsize,
                                             hipMemcpyDeviceToDevice, ctx->stream));
@@ -754,6 +876,23 @@ extern "C" int imsrg_hip_op_transform_basis(imsrg_dop *out, const imsrg_dop *op,
             rocblas_status_success)
       return fail(ctx, IMSRG_E_HIP, "rocblas_dgemm failed (channel " + std::to_string(c) + ")");
   }
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return IMSRG_OK;
+}
+
+// ReadWrite::ReadBareTBME_Darmstadt (src/ReadWrite.cc:664-728, 834-957): the two-body
+// part of a Hermitian operator from a Darmstadt me2j file; the rest is unchanged.
+extern "C" int imsrg_hip_read_me2j(imsrg_dop *op, const char *path, int emax, int E2max, int lmax, long *nrecords) {
+  if (!op || !path) return IMSRG_E_INVALID;
+  imsrg_ctx *ctx = op->ctx;
+  if (op->herm != 1) return fail(ctx, IMSRG_E_UNSUPPORTED, "me2j files hold Hermitian interactions");
+  const BasisDev &B = ctx->db.h;
+  std::vector<double> two((size_t)B.tbsize, 0.0);
+  const std::string e = read_me2j(ctx->hb, path, emax, E2max, lmax, two, nrecords);
+  if (!e.empty()) return fail(ctx, IMSRG_E_INVALID, e);
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, hipMemcpyAsync(op->buf + B.two_off, two.data(), sizeof(double) * B.tbsize, hipMemcpyHostToDevice,
+                             ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return IMSRG_OK;
 }

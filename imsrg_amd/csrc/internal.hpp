@@ -143,6 +143,19 @@ struct BasisDev {
   const IMSRG_G int *gcore_start, *gcore_loc, *gnc_start, *gnc_loc;
 };
 
+// Ket groups of the block-sparse basis transformation: the kets of a channel whose
+// orbits come from the same unordered pair of one-body (l, j2, tz2) blocks.
+struct XformDev {
+  const IMSRG_G int *g_start;   // ngroups + 1: group g holds g_kets[g_start[g] .. g_start[g+1])
+  const IMSRG_G int *g_kets;    // channel-local ket indices, ascending within a group
+  const IMSRG_G int *g_ch;      // channel of each group
+  const IMSRG_G long *g_doff;   // offset of each group's m x m block of D (column-major)
+  const IMSRG_G int *k_start;   // per channel: where its kets start in k_grp / k_pos
+  const IMSRG_G int *k_grp;     // group of each channel ket
+  const IMSRG_G int *k_pos;     // its position inside the group
+  int ngroups, maxm;
+};
+
 // Everything one commutator needs: operand buffers, workspaces, flags.
 struct PairPtrs {
   const IMSRG_G BasisDev *basis;  // device pointer

@@ -451,6 +451,140 @@ hipError_t launch_hf_mono(const BasisDev *bd, const double *op, const int4 *quad
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Basis transformation with a basis change C that is block diagonal in (l, j, tz),
+// as every Hartree-Fock C is.  D(i, a) is then nonzero only when the kets i and a of
+// a channel take their orbits from the same pair of one-body blocks, so after
+// grouping each channel's kets by that pair, D is block diagonal with groups of at
+// most (block size)^2 kets, and V' = D^T V D is two passes over the channel:
+//   T  = V D     column a of T mixes the columns of V in a's group   (k_xf_right)
+//   V' = D^T T   row a of V' mixes the rows of T in a's group        (k_xf_left)
+// Every element of V, T and V' is read or written once from HBM.
+
+// D blocks, one work-group per group g: Dg[doff[g] + r + c m] = D(kets r, c of g)
+// (the formula of k_hf_D).
+__global__ __launch_bounds__(256) void k_xf_D(const BasisDev *bd, XformDev X, const double *C, double *Dg) {
+  const BasisDev &B = *bd;
+  const int g = blockIdx.x, s = X.g_start[g], m = X.g_start[g + 1] - s, ch = X.g_ch[g], J = ch / 6;
+  const long N = B.norb;
+  const int *kets = B.ch_kets + B.ch_kstart[ch];
+  for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
+    const int r = e % m, c = e / m;
+    const int ki = kets[X.g_kets[s + r]], ka = kets[X.g_kets[s + c]];
+    const int p = B.kp[ki], q = B.kq[ki], pp = B.kp[ka], qq = B.kq[ka];
+    double d = C[p + pp * N] * C[q + qq * N];
+    if (p != q) {
+      const double ph = (((B.j2[p] + B.j2[q]) / 2 + 1 + J) & 1) ? -1.0 : 1.0;
+      double x = C[q + pp * N] * C[p + qq * N];
+      d += ph * x;
+    }
+    if (p == q) d *= 1.4142135623730951;
+    if (pp == qq) d /= 1.4142135623730951;
+    Dg[X.g_doff[g] + e] = d;
+  }
+}
+
+// T = V D on one item (group g, rows r0 .. r0+R-1 of its channel): V's rows in
+// the group's columns and the group's D block staged in LDS, then each of the
+// R x m outputs is a dot product of length m.
+__global__ __launch_bounds__(256) void k_xf_right(const BasisDev *bd, XformDev X, const double *V, const double *Dg,
+                                                  double *T, const int2 *items, int R) {
+  extern __shared__ double lds[];
+  const BasisDev &B = *bd;
+  const int2 it = items[blockIdx.x];
+  const int g = it.x, r0 = it.y, s = X.g_start[g], m = X.g_start[g + 1] - s, ch = X.g_ch[g];
+  const int n = B.ch_n[ch];
+  const long off = B.ch_off[ch];
+  double *Vs = lds, *Ds = lds + (long)X.maxm * R;
+  for (int e = threadIdx.x; e < m * R; e += blockDim.x) {
+    const int k = e / R, row = e - k * R;
+    Vs[e] = (r0 + row < n) ? V[off + (long)X.g_kets[s + k] * n + r0 + row] : 0.0;
+  }
+  for (int e = threadIdx.x; e < m * m; e += blockDim.x) Ds[e] = Dg[X.g_doff[g] + e];
+  __syncthreads();
+  const int row = threadIdx.x % R, lanes = blockDim.x / R;
+  if (r0 + row >= n) return;
+  for (int c = threadIdx.x / R; c < m; c += lanes) {
+    double acc = 0.0;
+    for (int k = 0; k < m; ++k) acc += Vs[k * R + row] * Ds[c * m + k];
+    T[off + (long)X.g_kets[s + c] * n + r0 + row] = acc;
+  }
+}
+
+// V' = D^T T on one item (channel ch, columns b0 .. b0+kc-1): the columns of T
+// staged in LDS (ket-major, kc values per ket), then row a of each column is the
+// dot product of a's D column with the rows of T in a's group.
+__global__ __launch_bounds__(256) void k_xf_left(const BasisDev *bd, XformDev X, const double *T, const double *Dg,
+                                                 double *Vout, const int2 *items, int kc) {
+  extern __shared__ double lds[];
+  const BasisDev &B = *bd;
+  const int2 it = items[blockIdx.x];
+  const int ch = it.x, b0 = it.y, n = B.ch_n[ch];
+  const int nb = min(kc, n - b0);
+  const long off = B.ch_off[ch];
+  for (int e = threadIdx.x; e < n * nb; e += blockDim.x) {
+    const int bb = e / n, i = e - bb * n;
+    lds[i * kc + bb] = T[off + (long)(b0 + bb) * n + i];
+  }
+  __syncthreads();
+  const int kbase = X.k_start[ch];
+  for (int a = threadIdx.x; a < n; a += blockDim.x) {
+    const int g = X.k_grp[kbase + a], c = X.k_pos[kbase + a];
+    const int s = X.g_start[g], m = X.g_start[g + 1] - s;
+    const double *Dc = Dg + X.g_doff[g] + (long)c * m;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < m; ++k) {
+      const double d = Dc[k];
+      const double *t = lds + X.g_kets[s + k] * kc;
+      for (int bb = 0; bb < 4; ++bb)
+        if (bb < nb) acc[bb] += d * t[bb];
+    }
+    for (int bb = 0; bb < 4; ++bb)
+      if (bb < nb) Vout[off + (long)(b0 + bb) * n + a] = acc[bb];
+  }
+}
+
+// One-body part, C^T O C with C block diagonal: out(a, b) = sum over k in a's block
+// and l in b's block of C(k, a) O(k, l) C(l, b); one thread per element.
+__global__ __launch_bounds__(256) void k_xf_one(const BasisDev *bd, const double *C, const double *O, double *out) {
+  const BasisDev &B = *bd;
+  const int N = B.norb;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= N * N) return;
+  const int a = e % N, b = e / N;
+  double acc = 0.0;
+  for (int ik = B.obstart[a]; ik < B.obstart[a + 1]; ++ik) {
+    const int k = B.oblist[ik];
+    double t = 0.0;
+    for (int il = B.obstart[b]; il < B.obstart[b + 1]; ++il) {
+      const int l = B.oblist[il];
+      t += O[k + (long)l * N] * C[l + (long)b * N];
+    }
+    acc += C[k + (long)a * N] * t;
+  }
+  out[e] = acc;
+}
+
+hipError_t launch_xform_blocks(const BasisDev *bd, const BasisDev &bh, const XformDev &X, const double *C,
+                               const double *op, double *out, double *T, double *Dg, double *one_tmp,
+                               const int2 *right_items, int nright, int R, const int2 *left_items, int nleft, int kc,
+                               hipStream_t s) {
+  if (X.ngroups > 0) {
+    hipLaunchKernelGGL(k_xf_D, dim3(X.ngroups), dim3(256), 0, s, bd, X, C, Dg);
+    const size_t lr = sizeof(double) * ((size_t)X.maxm * R + (size_t)X.maxm * X.maxm);
+    hipLaunchKernelGGL(k_xf_right, dim3(nright), dim3(256), lr, s, bd, X, op + bh.two_off, Dg, T, right_items, R);
+    const size_t ll = sizeof(double) * (size_t)bh.maxn * kc;
+    hipLaunchKernelGGL(k_xf_left, dim3(nleft), dim3(256), ll, s, bd, X, T, Dg, out + bh.two_off, left_items, kc);
+  }
+  const int n1 = bh.norb * bh.norb;
+  if (n1 > 0) {
+    hipLaunchKernelGGL(k_xf_one, dim3((n1 + 255) / 256), dim3(256), 0, s, bd, C, op + bh.one_off, one_tmp);
+    hipError_t e = hipMemcpyAsync(out + bh.one_off, one_tmp, sizeof(double) * n1, hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return e;
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_hf_D(const BasisDev *bd, const BasisDev &bh, const double *C, double *D, hipStream_t s) {
   if (bh.tbsize > 0) {
     const long nb = (bh.tbsize + 255) / 256;

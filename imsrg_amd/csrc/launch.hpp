@@ -63,6 +63,13 @@ hipError_t launch_norm(const BasisDev *bd, const BasisDev &bh, const double *op,
 // the Hartree-Fock monopoles W[q] = sum_J (2J+1) V~^J_{ia,jb} of quads[q] = (i, a, j, b)
 hipError_t launch_hf_mono(const BasisDev *bd, const double *op, const int4 *quads, int nq, double *W,
                           hipStream_t s);
+// block-sparse basis transformation (C block diagonal in (l, j2, tz2)): the D blocks
+// into Dg, T = V D (items (group, first row), R rows each), out two-body = D^T T (items
+// (channel, first column), kc columns each), out one-body = C^T O C (through one_tmp)
+hipError_t launch_xform_blocks(const BasisDev *bd, const BasisDev &bh, const XformDev &X, const double *C,
+                               const double *op, double *out, double *T, double *Dg, double *one_tmp,
+                               const int2 *right_items, int nright, int R, const int2 *left_items, int nleft, int kc,
+                               hipStream_t s);
 hipError_t launch_hf_D(const BasisDev *bd, const BasisDev &bh, const double *C, double *D, hipStream_t s);
 hipError_t launch_fill_random(const BasisDev *bd, const BasisDev &bh, double *op, int herm,
                               unsigned long long seed, hipStream_t s);

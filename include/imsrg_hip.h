@@ -175,6 +175,17 @@ int imsrg_hip_op_normal_order(imsrg_dop *out, const imsrg_dop *op, int sign);
    Synchronous: returns when out holds the result. */
 int imsrg_hip_op_transform_basis(imsrg_dop *out, const imsrg_dop *op, const double *C);
 
+/* ReadWrite::ReadBareTBME_Darmstadt (src/ReadWrite.cc:664-728, 834-957): the two-body part
+   of op (Hermitian) from a Darmstadt me2j file -- plain text (first line a header), gzip (.gz)
+   or binary (.bin: 255-byte header, then floats) -- with the file's emax, E2max and lmax
+   (emax < 0, E2max <= 0, lmax <= 0: the basis's emax, 2 emax, emax).  Records (a b c d J) in
+   the file's order carry (T,Tz) = (0,0), (1,1), (1,0), (1,-1) un-normalized elements in float;
+   pp and nn are set, proton-neutron kets get the isospin combination (Set_pn_TBME_from_iso).
+   The rest of op is unchanged.  A file with fewer records than the loops need is
+   IMSRG_E_INVALID (the reference reads zeros).  *nrecords (may be NULL): records read.
+   Synchronous. */
+int imsrg_hip_read_me2j(imsrg_dop *op, const char *path, int emax, int E2max, int lmax, long *nrecords);
+
 /* HartreeFock::Solve options (src/HartreeFock.cc:55-167); zero fields take the defaults. */
 typedef struct {
   double tolerance; /* convergence of the single-particle energies (default 1e-8, HartreeFock::tolerance) */

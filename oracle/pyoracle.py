@@ -30,6 +30,7 @@ TERMS = {
 ALL_TERMS = 0xFF
 
 _dp = C.POINTER(C.c_double)
+SQRT2 = float(np.sqrt(2.0))
 _ip = C.POINTER(C.c_int)
 
 
@@ -144,7 +145,8 @@ _FAULT = 0
 
 def set_fault(f):
     """Plant a known mistake in the oracle (checker tests only); 0 turns it off.
-    12 (Python part): Hartree-Fock field without its 1/(2j_i+1)."""
+    12 (Python part): Hartree-Fock field without its 1/(2j_i+1); 13: me2j isospin
+    phase of the exchanged proton-neutron ket reversed."""
     global _FAULT
     _FAULT = int(f)
     lib().orc_set_fault(int(f))
@@ -278,6 +280,114 @@ class OracleModelSpace:
         Cf = np.asfortranarray(np.asarray(Cmat, dtype=np.float64))
         lib().orc_transform_basis(self.h, C.byref(a.s), Cf.ctypes.data_as(_dp), C.byref(o.s))
         return o.op()
+
+    def read_me2j(self, path, emax=-1, E2max=0, lmax=0):
+        """ReadWrite::ReadBareTBME_Darmstadt (src/ReadWrite.cc:664-728, 834-957) into a
+        Hermitian operator: text (header line skipped), .gz or .bin (255-byte header)
+        numbers as float32, records (a b c d J) in the file's loop order with the
+        (T,Tz) = (0,0), (1,1), (1,0), (1,-1) elements; SetTBME for pp and nn,
+        Set_pn_TBME_from_iso (src/TwoBodyME.cc) for the proton-neutron kets.
+        Returns (operator, records read)."""
+        import gzip
+        path = str(path)
+        if path.endswith(".bin"):
+            raw = open(path, "rb").read()[255:]
+            vals = np.frombuffer(raw[:len(raw) // 4 * 4], dtype=np.float32)
+        else:
+            text = gzip.open(path, "rt").read() if path.endswith(".gz") else open(path).read()
+            body = text.split("\n", 1)[1] if "\n" in text else ""
+            vals = np.array(body.split(), dtype=np.float64).astype(np.float32)
+        orb, norb = self.orbits, self.norb
+        if emax < 0:
+            emax = self.emax
+        lmax = lmax if lmax > 0 else emax
+        E2max = E2max if E2max > 0 else 2 * emax
+        index = {(o["n"], o["l"], o["j2"], o["tz2"]): i for i, o in enumerate(orb)}
+        nlj = []
+        for e in range(min(emax, self.emax) + 1):
+            for l in range(e % 2, min(e, lmax) + 1, 2):
+                for tj in range(abs(2 * l - 1), 2 * l + 2, 2):
+                    nlj.append((index[((e - l) // 2, l, tj, -1)], e, l, tj))
+        local = []
+        for c in self.channels:
+            local.append({pq: i for i, pq in enumerate(c["kets"])})
+        op = self.zero_op(1)
+        two = op["two"]
+
+        def element(J, par, tz, a, b, c, d):
+            ch = 6 * J + 2 * (tz + 1) + par
+            cinfo = self.channels[ch]
+            ib = local[ch][(min(a, b), max(a, b))]
+            ik = local[ch][(min(c, d), max(c, d))]
+            ph = 1.0
+            if a > b:
+                ph *= 1 if ((orb[a]["j2"] + orb[b]["j2"]) // 2 + 1 + J) % 2 == 0 else -1
+            if c > d:
+                ph *= 1 if ((orb[c]["j2"] + orb[d]["j2"]) // 2 + 1 + J) % 2 == 0 else -1
+            n, off = cinfo["n"], cinfo["off"]
+            return off + ik * n + ib, off + ib * n + ik, ph
+
+        def set_tbme(J, par, tz, a, b, c, d, v):  # TwoBodyME::SetTBME, Hermitian
+            e1, e2, ph = element(J, par, tz, a, b, c, d)
+            two[e1] = ph * v
+            two[e2] = ph * v
+
+        def add_tbme(J, par, tz, a, b, c, d, v):  # TwoBodyME::AddToTBME, Hermitian
+            e1, e2, ph = element(J, par, tz, a, b, c, d)
+            two[e1] += ph * v
+            if e1 != e2:
+                two[e2] += ph * v
+
+        def pn_from_iso(J, T, a, b, c, d, v):  # TwoBodyME::Set_pn_TBME_from_iso
+            par = (orb[a]["l"] + orb[b]["l"]) % 2
+            eta = 2 * T - 1 if _FAULT != 13 else 1 - 2 * T
+            v *= 0.5
+            if a == b:
+                v *= SQRT2
+            if c == d:
+                v *= SQRT2
+            add_tbme(J, par, 0, a, b + 1, c, d + 1, v)
+            if c != d:
+                add_tbme(J, par, 0, a, b + 1, c + 1, d, v * eta)
+            if a != b and c != d:
+                add_tbme(J, par, 0, a + 1, b, c + 1, d, v)
+            if a != b and (a != c or b != d):
+                add_tbme(J, par, 0, a + 1, b, c, d + 1, v * eta)
+
+        pos = 0
+        for i1, (a, e1, l1, j1) in enumerate(nlj):
+            if e1 > self.emax:
+                break
+            for i2 in range(i1 + 1):
+                b, e2, l2, j2 = nlj[i2]
+                if e1 + e2 > E2max:
+                    break
+                par = (orb[a]["l"] + orb[b]["l"]) % 2
+                for i3 in range(i1 + 1):
+                    c, e3, l3, j3 = nlj[i3]
+                    for i4 in range((i2 if i3 == i1 else i3) + 1):
+                        d, e4, l4, j4 = nlj[i4]
+                        if e3 + e4 > E2max:
+                            break
+                        if (l1 + l2 + l3 + l4) % 2:
+                            continue
+                        for J in range(max(abs(j1 - j2), abs(j3 - j4)) // 2, min(j1 + j2, j3 + j4) // 2 + 1):
+                            if pos + 4 > len(vals):
+                                raise ValueError("me2j file ends early")
+                            v00, vnn, v10, vpp = (float(x) for x in vals[pos:pos + 4])
+                            pos += 4
+                            nf = 1.0
+                            if a == b:
+                                nf /= SQRT2
+                            if c == d:
+                                nf /= SQRT2
+                            if nf > 0.9 or J % 2 == 0:
+                                set_tbme(J, par, -1, a, b, c, d, vpp * nf)
+                                set_tbme(J, par, 1, a + 1, b + 1, c + 1, d + 1, vnn * nf)
+                                pn_from_iso(J, 1, a, b, c, d, v10 * nf)
+                            if nf > 0.9 or J % 2 != 0:
+                                pn_from_iso(J, 0, a, b, c, d, v00 * nf)
+        return op, pos // 4
 
     def hf_solve(self, H, tol=1e-8, max_iter=100):
         """HartreeFock::Solve + CalcEHF for a two-body Hamiltonian, frozen occupations,

@@ -65,6 +65,19 @@ def test_transform_matches_oracle(ia, emax, A, Z, herm, kind):
     assert np.array_equal(host(d)["two"], op["two"])  # the input is left as it was
 
 
+def test_transform_dense_path_matches_oracle(ia):
+    # an entry of C between two one-body blocks: every channel product runs as a DGEMM
+    ms, orc = spaces(ia, 4)
+    say("[gpu] transform basis emax=4, C with an entry outside the blocks")
+    op = random_op(orc, 1, 5150)
+    Cm = basis_change(orc.orbits, 5151, "general")
+    Cm[0, 1] = 0.125  # orbits 0 and 1 lie in different (l, j, tz) blocks
+    assert orc.orbits[0]["tz2"] != orc.orbits[1]["tz2"] or orc.orbits[0]["l"] != orc.orbits[1]["l"]
+    got = host(to_dev(ia, ms, op).TransformToHFBasis(Cm))
+    ref = orc.transform_basis(op, Cm)
+    assert rel_maxabs(got["one"], ref["one"]) < 1e-13 and rel_maxabs(got["two"], ref["two"]) < 1e-13
+
+
 def test_transform_in_place(ia):
     ms, orc = spaces(ia, 4)
     say("[gpu] transform basis in place emax=4")
@@ -86,7 +99,7 @@ def test_transform_rejects_wrong_shape(ia):
 
 def test_transform_emax10_properties(ia):
     ms = ia.ModelSpace(10, "O16")
-    say("[gpu] transform basis emax=10: identity, norm, round trip")
+    say("[gpu] transform basis emax=10: identity, norm, round trip, both paths")
     H = ia.DeviceOperator(ms, 1).fill_random(1, 5300)
     ref = host(H)
     same = host(H.TransformToHFBasis(np.eye(ms.norb)))
@@ -96,6 +109,12 @@ def test_transform_emax10_properties(ia):
     assert abs(HF.Norm() - H.Norm()) < 1e-12 * H.Norm()
     back = host(HF.TransformToHFBasis(Cm.T))
     assert rel_maxabs(back["one"], ref["one"]) < 1e-12 and rel_maxabs(back["two"], ref["two"]) < 1e-12
+    # the dense DGEMM path (an entry of 1e-300 outside the blocks) gives the same operator
+    Cd = Cm.copy()
+    Cd[0, 1] = 1e-300
+    dense = host(H.TransformToHFBasis(Cd))
+    sparse = host(HF)
+    assert rel_maxabs(dense["one"], sparse["one"]) < 1e-13 and rel_maxabs(dense["two"], sparse["two"]) < 1e-13
 
 
 def test_unoccupied_rotation_commutes_with_commutator_emax8(ia):
