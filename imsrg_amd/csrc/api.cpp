@@ -957,6 +957,7 @@ struct HFState {
   int norb = 0;
   std::vector<std::vector<int>> blocks;  // orbits of each (l, j2, tz2) block, ascending
   std::vector<int> blk;                  // block of each orbit
+  std::vector<int> pos;                  // position of each orbit in its block
   std::vector<long> wbase;               // quad offset of block pair (bi, ba): wbase[bi * nb + ba]
   std::vector<double> W;                 // monopoles of the quads (i, a, j, b)
   std::vector<double> KE, C, rho, Vij, F, energies;
@@ -968,11 +969,7 @@ struct HFState {
   double Wq(int i, int a, int j, int b) const {
     const int bi = blk[i], ba = blk[a], nb = (int)blocks.size();
     const int mi = (int)blocks[bi].size(), ma = (int)blocks[ba].size();
-    const int ii = (int)(std::lower_bound(blocks[bi].begin(), blocks[bi].end(), i) - blocks[bi].begin());
-    const int jj = (int)(std::lower_bound(blocks[bi].begin(), blocks[bi].end(), j) - blocks[bi].begin());
-    const int aa = (int)(std::lower_bound(blocks[ba].begin(), blocks[ba].end(), a) - blocks[ba].begin());
-    const int bb = (int)(std::lower_bound(blocks[ba].begin(), blocks[ba].end(), b) - blocks[ba].begin());
-    return W[wbase[(size_t)bi * nb + ba] + (((long)ii * mi + jj) * ma + aa) * ma + bb];
+    return W[wbase[(size_t)bi * nb + ba] + (((long)pos[i] * mi + pos[j]) * ma + pos[a]) * ma + pos[b]];
   }
   // UpdateDensityMatrix, src/HartreeFock.cc:613-617: rho = C_h diag(occ) C_h^T
   void density() {
@@ -1049,9 +1046,13 @@ extern "C" int imsrg_hip_hf_solve(const imsrg_dop *H, const imsrg_hf_opts *opts,
   HFState st;
   const int norb = st.norb = hb.norb;
   st.blk.assign(norb, -1);
+  st.pos.assign(norb, 0);
   for (int o = 0; o < norb; ++o)
     if (hb.ob[o].front() == o) {
-      for (int x : hb.ob[o]) st.blk[x] = (int)st.blocks.size();
+      for (size_t k = 0; k < hb.ob[o].size(); ++k) {
+        st.blk[hb.ob[o][k]] = (int)st.blocks.size();
+        st.pos[hb.ob[o][k]] = (int)k;
+      }
       st.blocks.push_back(hb.ob[o]);
     }
   const int nb = (int)st.blocks.size();

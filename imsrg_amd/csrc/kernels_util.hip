@@ -1,5 +1,6 @@
-// kernels_util.hip -- 6j table, operator arithmetic and norms, synthetic
-// operators, White generator, MFMA layout probe.
+// kernels_util.hip -- 6j table, operator arithmetic and norms, normal ordering,
+// basis transformation, Hartree-Fock monopoles, synthetic operators, White
+// generator, MFMA layout probe.
 #include <hip/hip_runtime.h>
 
 #include "internal.hpp"
