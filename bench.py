@@ -447,6 +447,71 @@ def workload_hf(args, ia, np):
     return line
 
 
+def me2j_record_count(emax, np):
+    """Records (a b c d J) of a Darmstadt me2j file to emax with E2max = 2 emax: pairs
+    (nlj1 >= nlj2) in file order, every pair (nlj3, nlj4) up to it in that order with
+    the same parity, one record per common J (src/ReadWrite.cc:877-935)."""
+    orb = []
+    for e in range(emax + 1):
+        for l in range(e % 2, e + 1, 2):
+            for tj in range(abs(2 * l - 1), 2 * l + 2, 2):
+                orb.append((l, tj))
+    pairs = [(i1, i2) for i1 in range(len(orb)) for i2 in range(i1 + 1)]
+    par = np.array([(orb[a][0] + orb[b][0]) % 2 for a, b in pairs])
+    jmin = np.array([abs(orb[a][1] - orb[b][1]) // 2 for a, b in pairs])
+    jmax = np.array([(orb[a][1] + orb[b][1]) // 2 for a, b in pairs])
+    nJ = np.minimum(jmax[:, None], jmax[None, :]) - np.maximum(jmin[:, None], jmin[None, :]) + 1
+    ok = (par[:, None] == par[None, :]) & (nJ > 0)
+    return int(np.sum(np.tril(np.where(ok, nJ, 0))))
+
+
+def workload_pipeline(args, ia, np):
+    """From an interaction file to the IMSRG(2) ground-state energy at emax=10: a
+    synthetic charge-independent me2j file (binary, N(0, 0.1^2) elements, seed 7400)
+    with the one-body part 10(2n+l+3/2) on the diagonal, read
+    (ReadBareTBME_Darmstadt), Hartree-Fock (Solve), GetNormalOrderedH, then the
+    Magnus-Euler flow (White generator, ds = 0.5, BCH nested-norm cut 1e-8, mixed
+    precision) to ||eta|| < 1e-6.  Wall time of each stage; the file is written
+    before the timing."""
+    import tempfile
+    emax = 10
+    ms = ia.ModelSpace(emax, "O16", device=0)
+    nrec = me2j_record_count(emax, np)
+    rng = np.random.default_rng(7400)
+    vals = (0.1 * rng.normal(size=(nrec, 4))).astype(np.float32)
+    vals[:, 2] = vals[:, 1]
+    vals[:, 3] = vals[:, 1]
+    one = np.asfortranarray(np.diag([10.0 * (2 * o["n"] + o["l"] + 1.5) for o in ms.orbits]))
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "H.me2j.bin")
+        with open(path, "wb") as f:
+            f.write(b"synthetic me2j".ljust(255, b" "))
+            f.write(vals.astype("<f4").tobytes())
+        H = ia.DeviceOperator(ms, 1)
+        H.upload(ia.Operator(ms, 1, 0.0, one, np.zeros(ms.tbsize)))
+        ms.synchronize()
+        t0 = time.perf_counter()
+        n = ia.ReadWrite().ReadBareTBME_Darmstadt(path, H, emax)
+        t1 = time.perf_counter()
+    hf = ia.HartreeFock(H).Solve()
+    t2 = time.perf_counter()
+    HNO = hf.GetNormalOrderedH()
+    t3 = time.perf_counter()
+    Hs, e0, en, nc = ia.IMSRGSolver.Solve_magnus_euler(HNO, ds=0.5, smax=50.0, eta_criterion=1e-6, max_steps=60,
+                                                       mode=1, transform_thr=1e-8, precision=ia.PREC_MIXED)
+    ms.synchronize()
+    t4 = time.perf_counter()
+    return {"metric": f"interaction file to IMSRG(2) ground-state energy at emax={emax}", "value": t4 - t0,
+            "unit": "s", "n_gpus": 1, "higher_is_better": False,
+            "dtype": "mixed: bf16x3 MFMA pp/hh + f32 MFMA ph + f64 (flow); f64 (reader, HF, normal ordering)",
+            "data": f"synthetic charge-independent me2j file, {n} records (seed 7400)",
+            "config": {"workload": f"ReadBareTBME_Darmstadt -> HartreeFock -> GetNormalOrderedH -> Magnus-Euler "
+                                   f"flow, HO emax={emax}, O16 reference"},
+            "read_s": t1 - t0, "hf_solve_s": t2 - t1, "hf_iterations": hf.iterations, "EHF": hf.EHF,
+            "normal_ordered_h_s": t3 - t2, "flow_s": t4 - t3, "flow_steps": int(len(e0)),
+            "flow_commutators": int(np.sum(nc)), "eta_norm_final": float(en[len(e0)]), "E0_final": float(e0[-1])}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -457,11 +522,13 @@ def main():
     ap.add_argument("--autotune-cache", default=None,
                     help="text cache of the autotuner: settings found there are applied without timing trials")
     ap.add_argument("--workload", default="commutator",
-                    choices=["commutator", "batched", "emax12", "emax14", "flow", "normal_order", "transform", "hf"],
+                    choices=["commutator", "batched", "emax12", "emax14", "flow", "normal_order", "transform", "hf",
+                             "pipeline"],
                     help="commutator: the headline metric line; batched: 48 independent pairs at emax 4-7; "
                          "emax12 / emax14: precision report at emax=12 / 14; flow: Magnus flow at emax=10; "
                          "normal_order: DoNormalOrdering at emax=10; transform: TransformToHFBasis at emax=10; "
-                         "hf: HartreeFock Solve + GetNormalOrderedH at emax=10")
+                         "hf: HartreeFock Solve + GetNormalOrderedH at emax=10; pipeline: me2j file -> HF -> "
+                         "normal ordering -> Magnus flow at emax=10")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X; runs on more than one GPU are not part of this build "
@@ -476,7 +543,7 @@ def main():
         line = {"batched": workload_batched, "emax12": workload_emax12,
                 "emax14": lambda a, i, n: workload_emax12(a, i, n, emax=14), "flow": workload_flow,
                 "normal_order": workload_normal_order, "transform": workload_transform,
-                "hf": workload_hf}[args.workload](
+                "hf": workload_hf, "pipeline": workload_pipeline}[args.workload](
             args, ia, np)
         print(json.dumps(line), flush=True)
         return

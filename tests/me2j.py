@@ -44,13 +44,13 @@ def records(emax, E2max=None, lmax=None):
     return orb, out
 
 
-def write_me2j(path, emax, seed, E2max=None, lmax=None, charge_independent=False):
-    """Random float32 elements; charge_independent: the (1,1), (1,0), (1,-1) elements
-    equal.  The file kind follows the name (.gz, .bin or text).  Returns (nlj orbits,
-    records, values[nrec, 4])."""
+def write_me2j(path, emax, seed, E2max=None, lmax=None, charge_independent=False, scale=1.0):
+    """Random float32 elements (N(0, scale^2)); charge_independent: the (1,1), (1,0),
+    (1,-1) elements equal.  The file kind follows the name (.gz, .bin or text).
+    Returns (nlj orbits, records, values[nrec, 4])."""
     orb, rec = records(emax, E2max, lmax)
     rng = np.random.default_rng(seed)
-    vals = rng.normal(size=(len(rec), 4)).astype(np.float32)
+    vals = (scale * rng.normal(size=(len(rec), 4))).astype(np.float32)
     if charge_independent:
         vals[:, 2] = vals[:, 1]
         vals[:, 3] = vals[:, 1]
