@@ -9,8 +9,11 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <algorithm>
+#include <atomic>
 #include <map>
 #include <sstream>
+#include <thread>
 #include <tuple>
 
 namespace imsrg {
@@ -28,6 +31,19 @@ struct NumberSource {
   std::vector<float> bin;
   size_t pos = 0;
   bool binary = false;
+  // up to n numbers from the current position into out
+  void take_all(std::vector<float> &out, long n) {
+    if (binary) {
+      const size_t k = std::min(bin.size() - std::min(pos, bin.size()), (size_t)std::max(n, 0L));
+      out.assign(bin.begin() + pos, bin.begin() + pos + k);
+      pos += k;
+      return;
+    }
+    out.clear();
+    out.reserve((size_t)std::max(n, 0L));
+    float x;
+    while ((long)out.size() < n && next(x)) out.push_back(x);
+  }
   bool next(float &x) {
     if (binary) {
       if (pos >= bin.size()) return false;
@@ -168,46 +184,68 @@ std::string read_me2j(const HostBasis &hb, const std::string &path, int emax, in
         jv.push_back(tj);
       }
     }
-  TwoBodyWriter w{hb, two};
   const int nlj = (int)remap.size();
-  long nread = 0;
-  for (int i1 = 0; i1 < nlj; ++i1) {
-    const int a = remap[i1];
-    if (ev[i1] > hb.emax) break;
+  // The records of one nlj1 (the file's outermost orbit) write only elements whose bra
+  // holds that orbit (or, mirrored, whose ket does with nlj3 = nlj1), so the nlj1 blocks
+  // write disjoint elements: they run in parallel, each in file order, and the result
+  // equals the sequential read bit for bit.  First the records of each block.
+  auto for_records = [&](int i1, auto &&fn) {
     for (int i2 = 0; i2 <= i1; ++i2) {
-      const int b = remap[i2];
       if (ev[i1] + ev[i2] > E2max) break;
-      const int par = (hb.l[a] + hb.l[b]) % 2;
-      for (int i3 = 0; i3 <= i1; ++i3) {
-        const int c = remap[i3];
+      for (int i3 = 0; i3 <= i1; ++i3)
         for (int i4 = 0; i4 <= (i3 == i1 ? i2 : i3); ++i4) {
-          const int d = remap[i4];
           if (ev[i3] + ev[i4] > E2max) break;
           if ((lv[i1] + lv[i2] + lv[i3] + lv[i4]) % 2 != 0) continue;
           const int Jmin = std::max(std::abs(jv[i1] - jv[i2]), std::abs(jv[i3] - jv[i4])) / 2;
           const int Jmax = std::min(jv[i1] + jv[i2], jv[i3] + jv[i4]) / 2;
-          for (int J = Jmin; J <= Jmax; ++J) {
-            // (T, Tz) = (0,0), (1,1), (1,0), (1,-1): T=0, nn, T=1 pn, pp
-            float v00, vnn, v10, vpp;
-            if (!src.next(v00) || !src.next(vnn) || !src.next(v10) || !src.next(vpp))
-              return "file ends after " + std::to_string(nread) + " records of 4 elements: " + path;
-            ++nread;
-            double nf = 1.0;  // the file holds un-normalized elements
-            if (a == b) nf /= kSqrt2;
-            if (c == d) nf /= kSqrt2;
-            bool ok = true;
-            if (nf > 0.9 || J % 2 == 0) {
-              ok = ok && w.set(J, par, -1, a, b, c, d, vpp * nf);
-              ok = ok && w.set(J, par, 1, a + 1, b + 1, c + 1, d + 1, vnn * nf);
-              ok = ok && w.pn_from_iso(J, 1, a, b, c, d, v10 * nf);
-            }
-            if (nf > 0.9 || J % 2 != 0) ok = ok && w.pn_from_iso(J, 0, a, b, c, d, v00 * nf);
-            if (!ok) return "a ket of the file is missing from its channel (J=" + std::to_string(J) + ")";
-          }
+          for (int J = Jmin; J <= Jmax; ++J) fn(i2, i3, i4, J);
         }
-      }
     }
+  };
+  std::vector<long> first(nlj + 1, 0);
+  for (int i1 = 0; i1 < nlj; ++i1) {
+    long cnt = 0;
+    if (ev[i1] <= hb.emax) for_records(i1, [&](int, int, int, int) { ++cnt; });
+    first[i1 + 1] = first[i1] + cnt;
   }
+  std::vector<float> vals;
+  src.take_all(vals, 4 * first[nlj]);
+  if ((long)vals.size() < 4 * first[nlj])
+    return "file ends after " + std::to_string(vals.size() / 4) + " records of 4 elements: " + path;
+  std::atomic<int> next{0};
+  std::atomic<bool> bad{false};
+  auto work = [&]() {
+    TwoBodyWriter w{hb, two};
+    for (int i1; (i1 = next.fetch_add(1)) < nlj;) {
+      const int a = remap[i1];
+      long rec = first[i1];
+      for_records(i1, [&](int i2, int i3, int i4, int J) {
+        const int b = remap[i2], c = remap[i3], d = remap[i4];
+        const int par = (hb.l[a] + hb.l[b]) % 2;
+        // (T, Tz) = (0,0), (1,1), (1,0), (1,-1): T=0, nn, T=1 pn, pp
+        const float *v = vals.data() + 4 * rec++;
+        const float v00 = v[0], vnn = v[1], v10 = v[2], vpp = v[3];
+        double nf = 1.0;  // the file holds un-normalized elements
+        if (a == b) nf /= kSqrt2;
+        if (c == d) nf /= kSqrt2;
+        bool ok = true;
+        if (nf > 0.9 || J % 2 == 0) {
+          ok = ok && w.set(J, par, -1, a, b, c, d, vpp * nf);
+          ok = ok && w.set(J, par, 1, a + 1, b + 1, c + 1, d + 1, vnn * nf);
+          ok = ok && w.pn_from_iso(J, 1, a, b, c, d, v10 * nf);
+        }
+        if (nf > 0.9 || J % 2 != 0) ok = ok && w.pn_from_iso(J, 0, a, b, c, d, v00 * nf);
+        if (!ok) bad = true;
+      });
+    }
+  };
+  const int nthr = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nthr; ++t) pool.emplace_back(work);
+  work();
+  for (auto &t : pool) t.join();
+  if (bad) return "a ket of the file is missing from its channel";
+  const long nread = first[nlj];
   if (nrecords) *nrecords = nread;
   return "";
 }
