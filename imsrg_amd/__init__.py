@@ -491,7 +491,7 @@ class HartreeFock:
     """HartreeFock (src/HartreeFock.hh, src/HartreeFock.cc) for a two-body Hermitian
     Hamiltonian (a DeviceOperator) on its model space's reference, occupations frozen:
     Solve() (monopoles on the device, the iteration of src/HartreeFock.cc:55-167 on
-    the host, up to the reference's switch to DIIS at 100 iterations), then C
+    the host, with its DIIS phase, up to 600 iterations), then C
     (C[i, a] = <i|a>), energies, F, EHF, e1hf, e2hf, iterations;
     TransformToHFBasis(op) and GetNormalOrderedH()."""
 
@@ -506,7 +506,7 @@ class HartreeFock:
         self.iterations = 0
         self.tolerance = 1e-8
 
-    def Solve(self, max_iter=100):
+    def Solve(self, max_iter=600):
         n = self.ms.norb
         C_ = np.zeros((n, n), order="F")
         F_ = np.zeros((n, n), order="F")

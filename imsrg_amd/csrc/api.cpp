@@ -9,6 +9,7 @@
 #include <array>
 #include <atomic>
 #include <condition_variable>
+#include <deque>
 #include <functional>
 #include <mutex>
 #include <thread>
@@ -1016,6 +1017,77 @@ struct HFState {
       if (m(C, c, c) < 0)
         for (int r = 0; r < norb; ++r) m(C, r, c) = -m(C, r, c);
   }
+  // UpdateDensityMatrix_DIIS, src/HartreeFock.cc:623-683: the error F rho - rho F and
+  // the new density join the last ones kept (up to 6); with 5 kept before this one, rho
+  // is the combination of the kept densities whose coefficients solve the DIIS system
+  // B c = (0, .., 0, 1), B(i, j) = ||E_i E_j^T||_F over the first nsave of them, taken
+  // in absolute value and normalised to sum 1
+  void diis(std::deque<std::vector<double>> &derr, std::deque<std::vector<double>> &drho) {
+    const size_t n2 = (size_t)norb * norb;
+    std::vector<double> err(n2, 0.0);
+    for (int c = 0; c < norb; ++c)
+      for (int r = 0; r < norb; ++r) {
+        double x = 0.0, y = 0.0;
+        for (int k = 0; k < norb; ++k) {
+          x += m(F, r, k) * m(rho, k, c);
+          y += m(rho, r, k) * m(F, k, c);
+        }
+        err[(size_t)c * norb + r] = x - y;
+      }
+    density();
+    const size_t nsave = derr.size();
+    if (nsave > 5) derr.pop_front();
+    derr.push_back(err);
+    if (drho.size() > 5) drho.pop_front();
+    drho.push_back(rho);
+    if (nsave < 5) return;
+    const int nb = (int)nsave + 1;
+    std::vector<double> Bm((size_t)nb * nb, 1.0), rhs(nb, 0.0);
+    Bm[(size_t)nsave * nb + nsave] = 0.0;
+    for (size_t i = 0; i < nsave; ++i)
+      for (size_t j = 0; j < nsave; ++j) {
+        double acc = 0.0;
+        for (int c = 0; c < norb; ++c)
+          for (int r = 0; r < norb; ++r) {
+            double x = 0.0;
+            for (int k = 0; k < norb; ++k) x += derr[i][(size_t)k * norb + r] * derr[j][(size_t)k * norb + c];
+            acc += x * x;
+          }
+        Bm[i * nb + j] = std::sqrt(acc);  // row-major here
+      }
+    rhs[nsave] = 1.0;
+    // Gaussian elimination with partial pivoting
+    for (int col = 0; col < nb; ++col) {
+      int piv = col;
+      for (int r = col + 1; r < nb; ++r)
+        if (std::fabs(Bm[(size_t)r * nb + col]) > std::fabs(Bm[(size_t)piv * nb + col])) piv = r;
+      if (piv != col) {
+        for (int k = 0; k < nb; ++k) std::swap(Bm[(size_t)col * nb + k], Bm[(size_t)piv * nb + k]);
+        std::swap(rhs[col], rhs[piv]);
+      }
+      const double d = Bm[(size_t)col * nb + col];
+      if (d == 0.0) continue;
+      for (int r = col + 1; r < nb; ++r) {
+        const double f = Bm[(size_t)r * nb + col] / d;
+        for (int k = col; k < nb; ++k) Bm[(size_t)r * nb + k] -= f * Bm[(size_t)col * nb + k];
+        rhs[r] -= f * rhs[col];
+      }
+    }
+    std::vector<double> cv(nb, 0.0);
+    for (int r = nb - 1; r >= 0; --r) {
+      double x = rhs[r];
+      for (int k = r + 1; k < nb; ++k) x -= Bm[(size_t)r * nb + k] * cv[k];
+      const double d = Bm[(size_t)r * nb + r];
+      cv[r] = d != 0.0 ? x / d : 0.0;
+    }
+    double sum = 0.0;
+    for (size_t i = 0; i < nsave; ++i) sum += std::fabs(cv[i]);
+    std::fill(rho.begin(), rho.end(), 0.0);
+    for (size_t i = 0; i < nsave; ++i) {
+      const double ci = std::fabs(cv[i]) / sum;
+      for (size_t e = 0; e < n2; ++e) rho[e] += ci * drho[i][e];
+    }
+  }
   // CalcEHF, src/HartreeFock.cc:169-190 (no three-body part)
   double ehf() {
     e1 = e2 = 0.0;
@@ -1031,14 +1103,21 @@ struct HFState {
 };
 }  // namespace
 
+static double frob(const std::vector<double> &M) {
+  double acc = 0.0;
+  for (double x : M) acc += x * x;
+  return std::sqrt(acc);
+}
+
 extern "C" int imsrg_hip_hf_solve(const imsrg_dop *H, const imsrg_hf_opts *opts, double *C_out,
                                   double *energies_out, double *F_out, double *ehf_out, int *iterations) {
   if (!H || !C_out) return IMSRG_E_INVALID;
   imsrg_ctx *ctx = H->ctx;
   const double tol = (opts && opts->tolerance > 0) ? opts->tolerance : 1e-8;
-  const int maxiter = (opts && opts->max_iter > 0) ? opts->max_iter : 100;
-  if (maxiter > 100)
-    return fail(ctx, IMSRG_E_UNSUPPORTED, "Hartree-Fock beyond 100 iterations (the reference's DIIS phase) is not provided");
+  const int maxiter = (opts && opts->max_iter > 0) ? opts->max_iter : 600;
+  if (maxiter > 600)
+    return fail(ctx, IMSRG_E_UNSUPPORTED,
+                "Hartree-Fock beyond 600 iterations (the reference's randomised mixing) is not provided");
   if (H->herm != 1) return fail(ctx, IMSRG_E_UNSUPPORTED, "Hartree-Fock needs a Hermitian Hamiltonian");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   const HostBasis &hb = ctx->hb;
@@ -1096,19 +1175,31 @@ extern "C" int imsrg_hip_hf_solve(const imsrg_dop *H, const imsrg_hf_opts *opts,
   st.F.assign(n2, 0.0);
   st.energies.assign(norb, 0.0);
   for (int i = 0; i < norb; ++i) st.energies[i] = st.m(st.KE, i, i);
-  // Solve, src/HartreeFock.cc:55-167, up to the switch to DIIS at 100 iterations
+  // Solve, src/HartreeFock.cc:55-167: density mixing, the DIIS update from iteration 101
+  // while its last error matrix exceeds 0.01, the mixing factors 0.7 / 0.5 from 500
   st.density();
   st.fock();
-  const double mix = 0.2;  // density_mixing_factor; field_mixing_factor is 0 in these iterations
+  double dmix = 0.2, fmix = 0.0;  // density_mixing_factor, field_mixing_factor
+  std::deque<std::vector<double>> derr, drho;
   int it = 0;
   bool converged = false;
   for (it = 0; it < maxiter; ++it) {
     const std::vector<double> prev = st.energies;
     st.diagonalize();
-    const std::vector<double> last_rho = st.rho;
-    st.density();
-    for (size_t e = 0; e < n2; ++e) st.rho[e] = (1.0 - mix) * st.rho[e] + mix * last_rho[e];
+    if (it == 500) {
+      dmix = 0.7;
+      fmix = 0.5;
+    }
+    if (it > 100 && (derr.empty() || frob(derr.back()) > 0.01)) {
+      st.diis(derr, drho);
+    } else {
+      const std::vector<double> last_rho = st.rho;
+      st.density();
+      for (size_t e = 0; e < n2; ++e) st.rho[e] = (1.0 - dmix) * st.rho[e] + dmix * last_rho[e];
+    }
+    const std::vector<double> last_F = st.F;
     st.fock();
+    for (size_t e = 0; e < n2; ++e) st.F[e] = (1.0 - fmix) * st.F[e] + fmix * last_F[e];
     st.ehf();  // CheckConvergence, :834-844
     double d2 = 0.0;
     for (int i = 0; i < norb; ++i) d2 += (st.energies[i] - prev[i]) * (st.energies[i] - prev[i]);

@@ -189,16 +189,18 @@ int imsrg_hip_read_me2j(imsrg_dop *op, const char *path, int emax, int E2max, in
 /* HartreeFock::Solve options (src/HartreeFock.cc:55-167); zero fields take the defaults. */
 typedef struct {
   double tolerance; /* convergence of the single-particle energies (default 1e-8, HartreeFock::tolerance) */
-  int max_iter;     /* iterations with density mixing 0.2 (default and maximum 100: the reference
-                       switches to DIIS after 100, which is not provided) */
+  int max_iter;     /* iterations (default and maximum 600: beyond 600 the reference mixes with
+                       random factors, which is not provided) */
 } imsrg_hf_opts;
 
 /* HartreeFock::Solve + CalcEHF for a two-body Hermitian Hamiltonian H on the context's
    reference, occupations frozen (the reference's default): the monopoles
    sum_J (2J+1) V~^J_{ia,jb} of BuildMonopoleV / UpdateF (src/HartreeFock.cc:250-285, 750-822)
    come from the device; the iteration (Diagonalize per (l, j, tz) block, ReorderCoefficients'
-   sign rule, UpdateDensityMatrix, density mixing 0.2, UpdateF, CheckConvergence:
-   src/HartreeFock.cc:211-245, 613-617, 834-903) runs on the host.  Outputs (host):
+   sign rule, UpdateDensityMatrix with density mixing 0.2, from iteration 101 the DIIS update
+   UpdateDensityMatrix_DIIS while its last error matrix exceeds 0.01, from iteration 500 the
+   mixing factors 0.7 / 0.5, UpdateF, CheckConvergence: src/HartreeFock.cc:55-167, 211-245,
+   613-683, 834-903) runs on the host.  Outputs (host):
    C (norb x norb, column-major, C(i,a) = <i|a>), energies (norb, may be NULL), F (the Fock
    matrix, norb x norb, may be NULL), ehf[3] = {EHF, e1hf, e2hf} (may be NULL), iterations
    (may be NULL).  IMSRG_E_NOT_CONVERGED (outputs written) when max_iter is reached. */

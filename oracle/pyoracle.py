@@ -389,9 +389,11 @@ class OracleModelSpace:
                                 pn_from_iso(J, 0, a, b, c, d, v00 * nf)
         return op, pos // 4
 
-    def hf_solve(self, H, tol=1e-8, max_iter=100):
-        """HartreeFock::Solve + CalcEHF for a two-body Hamiltonian, frozen occupations,
-        the iterations before the reference's switch to DIIS (src/HartreeFock.cc:55-167).
+    def hf_solve(self, H, tol=1e-8, max_iter=600):
+        """HartreeFock::Solve + CalcEHF for a two-body Hamiltonian, frozen occupations
+        (src/HartreeFock.cc:55-167): density mixing, from iteration 101 the DIIS update
+        while its last error matrix exceeds 0.01, from iteration 500 the mixing factors
+        0.7 / 0.5; at most 600 iterations (the reference's randomised mixing beyond).
         Returns dict(C, energies, F, rho, Vij, EHF, e1hf, e2hf, iterations, converged)."""
         norb, orb = self.norb, self.orbits
         blocks = {}
@@ -457,8 +459,11 @@ class OracleModelSpace:
 
         rho = density(Cm)
         V, F = fock(rho)
-        it, converged = 0, False
-        for it in range(max_iter):
+        converged = False
+        dmix, fmix = 0.2, 0.0  # density_mixing_factor, field_mixing_factor
+        diis_err, diis_rho = [], []
+        it = 0
+        while it < max_iter:
             prev = energies.copy()
             for b in blocks.values():  # Diagonalize, :211-245
                 w, U = np.linalg.eigh(F[np.ix_(b, b)])
@@ -467,12 +472,42 @@ class OracleModelSpace:
             for c in range(norb):  # ReorderCoefficients' sign rule, :897-901
                 if Cm[c, c] < 0:
                     Cm[:, c] *= -1
-            last = rho
-            rho = 0.8 * density(Cm) + 0.2 * last
+            if it == 500:  # :84-91
+                dmix, fmix = 0.7, 0.5
+            if it > 100 and (not diis_err or np.linalg.norm(diis_err[-1]) > 0.01):
+                # UpdateDensityMatrix_DIIS, :623-683
+                err = F @ rho - rho @ F
+                rho = density(Cm)
+                nsave = len(diis_err)
+                if nsave > 5:
+                    diis_err.pop(0)
+                diis_err.append(err)
+                if len(diis_rho) > 5:
+                    diis_rho.pop(0)
+                diis_rho.append(rho)
+                if nsave >= 5:
+                    B = np.ones((nsave + 1, nsave + 1))
+                    B[nsave, nsave] = 0.0
+                    for i in range(nsave):
+                        for j in range(nsave):
+                            B[i, j] = np.linalg.norm(diis_err[i] @ diis_err[j].T)
+                    rhs = np.zeros(nsave + 1)
+                    rhs[nsave] = 1.0
+                    cv = np.abs(np.linalg.solve(B, rhs))
+                    cv /= np.sum(cv[:nsave])
+                    rho = np.zeros_like(rho)
+                    for i in range(nsave):
+                        rho = rho + cv[i] * diis_rho[i]
+            else:
+                last = rho
+                rho = (1.0 - dmix) * density(Cm) + dmix * last
+            last_F = F
             V, F = fock(rho)
+            F = (1.0 - fmix) * F + fmix * last_F
             if np.linalg.norm(energies - prev) / norb < tol:  # CheckConvergence, :834-844
                 converged = True
                 break
+            it += 1
         ehf, e1, e2 = calc_ehf(rho, V)
         return dict(C=Cm, energies=energies, F=F, rho=rho, Vij=V, EHF=ehf, e1hf=e1, e2hf=e2, iterations=it,
                     converged=converged)

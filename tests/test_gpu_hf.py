@@ -70,8 +70,26 @@ def test_hf_iteration_limit(ia):
     hf = ia.HartreeFock(H).Solve(max_iter=1)
     assert not hf.converged and hf.iterations == 1 and hf.EHF != 0.0
     with pytest.raises(ia.HipError) as e:
-        ia.HartreeFock(H).Solve(max_iter=101)  # the reference's DIIS phase is not provided
+        ia.HartreeFock(H).Solve(max_iter=601)  # the reference's randomised mixing is not provided
     assert e.value.code == -2
+
+
+@pytest.mark.parametrize("seed,scale,iters", [(9004, 12.0, 105), (9004, 13.0, 214), (9005, 7.0, 515)])
+def test_hf_diis_phase_matches_oracle(ia, seed, scale, iters):
+    # two-body parts strong enough that plain mixing does not converge in 100 iterations:
+    # the DIIS phase (and for the last case the mixing switch at 500) finishes the job
+    ms, orc = spaces(ia, 3)
+    say(f"[gpu] Hartree-Fock DIIS phase emax=3 seed={seed} scale={scale}")
+    H = synthetic_hamiltonian(orc, seed)
+    H["two"] = H["two"] * scale
+    hf = ia.HartreeFock(to_dev(ia, ms, H)).Solve()
+    ref = orc.hf_solve(H)
+    assert ref["converged"] and ref["iterations"] == iters
+    # after the 500-iteration switch the last step before the 1e-8 energy test can land
+    # on either side of it (the device and the oracle differ in rounding): one iteration apart
+    assert hf.converged and abs(hf.iterations - iters) <= (1 if iters > 500 else 0)
+    assert abs(hf.EHF - ref["EHF"]) < 1e-9 * abs(ref["EHF"])
+    assert np.max(np.abs(hf.C - ref["C"])) < 1e-6
 
 
 def test_hf_emax10_fixed_point(ia):

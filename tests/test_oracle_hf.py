@@ -64,3 +64,17 @@ def test_planted_hf_mistake_is_caught(oracle_ms_cache):
         set_fault(0)
     dE, df, dc = _check(ms, H, r)
     assert dE > 1e-3
+
+
+@pytest.mark.parametrize("seed,scale", [(9004, 13.0), (9005, 7.0)])
+def test_hf_diis_phase_reaches_the_fixed_point(oracle_ms_cache, seed, scale):
+    # plain mixing does not converge in 100 iterations here; the DIIS phase must still end
+    # at the self-consistent point (to the default tolerance's accuracy)
+    ms = oracle_ms_cache(3, 16, 8)
+    H = synthetic_hamiltonian(ms, seed)
+    H["two"] = H["two"] * scale
+    assert not ms.hf_solve(H, max_iter=100)["converged"]
+    r = ms.hf_solve(H)
+    assert r["converged"] and r["iterations"] > 100
+    dE, df, dc = _check(ms, H, r)
+    assert dE < 1e-6 * abs(r["EHF"]) and dc < 1e-4 * np.max(np.abs(r["F"]))
